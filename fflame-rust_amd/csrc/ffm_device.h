@@ -1,0 +1,301 @@
+// ffm_device.h -- gfx950 device code of the chaos-game step.
+//
+// Semantics follow the reference kernel shaders/core/main_template.wgsl
+// (assembled 2D reading: tests/shader_dumps/in-app-2d.wgsl:1097-1400).
+// Every FP32 expression is written in the reference's operation order and
+// the files that include this header are built with -ffp-contract=off, so
+// the +,-,*,/ chains round exactly as the reference's source states them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ffm_internal.h"
+
+namespace ffm {
+
+// ---------------------------------------------------------------- PCG32
+
+// pcg_hash (shaders/core/rng.wgsl:22-27).
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t input) {
+  uint32_t state = input * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+struct Rng {
+  uint32_t state, inc;
+};
+
+// rng_init: per-walker stream (shaders/core/rng.wgsl:29-37).
+__device__ __forceinline__ Rng rng_init(uint32_t walker, uint32_t seed) {
+  Rng r;
+  r.state = pcg_hash(walker ^ seed);
+  r.inc = (pcg_hash(pcg_hash(walker) + seed) << 1u) | 1u;
+  return r;
+}
+
+// rng_next: LCG step then RXS-M-XS output of the old state (rng.wgsl:39-45).
+__device__ __forceinline__ uint32_t rng_next(Rng& r) {
+  uint32_t old = r.state;
+  r.state = old * 747796405u + r.inc;
+  uint32_t xs = ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
+  return (xs >> 22u) ^ xs;
+}
+
+// Top 24 bits times 2^-24 (rng.wgsl:53-55): exact in f32.
+__device__ __forceinline__ float rng_nextf(Rng& r) {
+  return (float)(rng_next(r) >> 8u) * (1.0f / 16777216.0f);
+}
+
+// ----------------------------------------------------------- variations
+
+constexpr float PI_F = 3.14159265359f;        // the reference's literal
+constexpr float TWO_PI_F = 6.28318530718f;
+constexpr float PI_ATAN = 3.14159265358979f;  // ff_atan2's literal
+
+// ff_atan2 (shaders/core/utilities.wgsl:604-611).
+__device__ __forceinline__ float ff_atan2(float y, float x) {
+  if (y == 0.0f && x == 0.0f) {
+    float mag = (__float_as_uint(x) & 0x80000000u) ? PI_ATAN : 0.0f;
+    return (__float_as_uint(y) & 0x80000000u) ? -mag : mag;
+  }
+  return atan2f(y, x);
+}
+
+// One variation body V(x, y). `row` is the transform's table row (waves
+// reads its own b, e, d, f), `p` the variation's params. Bodies:
+// src/variations/defs/basic.rs:37-205, advanced.rs:32-1700.
+__device__ __forceinline__ void eval_variation(int id, const float* __restrict__ row,
+                                               const float* __restrict__ p, float x, float y,
+                                               Rng& rng, float& ox, float& oy) {
+  switch (id) {
+    case FFM_VAR_LINEAR:
+      ox = x;
+      oy = y;
+      break;
+    case FFM_VAR_SINUSOIDAL:
+      ox = sinf(x);
+      oy = sinf(y);
+      break;
+    case FFM_VAR_SPHERICAL: {
+      float r2 = x * x + y * y + 1e-6f;
+      ox = x / r2;
+      oy = y / r2;
+      break;
+    }
+    case FFM_VAR_SWIRL: {
+      float r2 = x * x + y * y;
+      float s, c;
+      sincosf(r2, &s, &c);
+      ox = x * s - y * c;
+      oy = x * c + y * s;
+      break;
+    }
+    case FFM_VAR_HORSESHOE: {
+      float r = sqrtf(x * x + y * y) + 1e-6f;
+      float r_inv = 1.0f / r;
+      ox = (x - y) * (x + y) * r_inv;
+      oy = 2.0f * x * y * r_inv;
+      break;
+    }
+    case FFM_VAR_POLAR: {
+      float theta = atan2f(x, y);
+      float r = sqrtf(x * x + y * y);
+      ox = theta / PI_F;
+      oy = r - 1.0f;
+      break;
+    }
+    case FFM_VAR_DISC: {
+      float theta = atan2f(x, y);
+      float r = sqrtf(x * x + y * y);
+      float theta_pi = theta / PI_F;
+      float pi_r = PI_F * r;
+      float s, c;
+      sincosf(pi_r, &s, &c);
+      ox = theta_pi * s;
+      oy = theta_pi * c;
+      break;
+    }
+    case FFM_VAR_HYPERBOLIC: {
+      float r2 = x * x + y * y + 1e-6f;
+      ox = x / r2;
+      oy = y;
+      break;
+    }
+    case FFM_VAR_JULIA: {
+      float r = sqrtf(x * x + y * y);
+      float theta = atan2f(y, x);
+      float sqrt_r = sqrtf(r);
+      float omega = (rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
+      float half_theta = theta / 2.0f + omega;
+      float s, c;
+      sincosf(half_theta, &s, &c);
+      ox = sqrt_r * c;
+      oy = sqrt_r * s;
+      break;
+    }
+    case FFM_VAR_WAVES: {
+      float b = row[XF_B], e = row[XF_E], d = row[XF_D], f = row[XF_F];
+      ox = x + b * sinf(y / (e * e + 1e-6f));
+      oy = y + d * sinf(x / (f * f + 1e-6f));
+      break;
+    }
+    case FFM_VAR_JULIAN: {
+      float power = p[0];
+      float cpower = p[2];
+      float r2 = x * x + y * y;
+      float r = powf(r2, cpower);
+      float theta = ff_atan2(y, x);
+      float trunc_val = floorf(fabsf(power) * rng_nextf(rng));
+      float t = (theta + TWO_PI_F * trunc_val) / power;
+      float s, c;
+      sincosf(t, &s, &c);
+      ox = r * c;
+      oy = r * s;
+      break;
+    }
+    case FFM_VAR_BUBBLE: {
+      float r2 = x * x + y * y;
+      float r = r2 / 4.0f + 1.0f;
+      float scale = 1.0f / r;
+      ox = scale * x;
+      oy = scale * y;
+      break;
+    }
+    case FFM_VAR_CURL: {
+      float c1 = p[0], c2 = p[1];
+      float re = 1.0f + c1 * x + c2 * (x * x - y * y);
+      float im = c1 * y + 2.0f * c2 * x * y;
+      float r = 1.0f / (re * re + im * im);
+      ox = (x * re + y * im) * r;
+      oy = (y * re - x * im) * r;
+      break;
+    }
+    default:
+      ox = 0.0f;
+      oy = 0.0f;
+      break;
+  }
+}
+
+// Affine -> weighted variation sum in the flame's local-index order
+// (src/shader_builder_v2.rs:2525-2806) -> post-affine when enabled
+// (main_template.wgsl:355-384). `ids` are the flame's variation ids
+// (uniform across the wave), `row` the transform's row in LDS.
+__device__ __forceinline__ void apply_xform(const float* __restrict__ row, const int* ids, int nv,
+                                            float& x, float& y, Rng& rng) {
+  float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
+  float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
+  float rx = 0.0f, ry = 0.0f;
+  for (int k = 0; k < nv; k++) {
+    int id = __builtin_amdgcn_readfirstlane(ids[k]);
+    float w = row[XF_W0 + k];
+    if (w != 0.0f) {
+      float vx, vy;
+      eval_variation(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
+      rx += w * vx;
+      ry += w * vy;
+    }
+  }
+  if (row[XF_POST] != 0.0f) {
+    float qx = row[XF_PA] * rx + row[XF_PB] * ry + row[XF_PE];
+    float qy = row[XF_PC] * rx + row[XF_PD] * ry + row[XF_PF];
+    rx = qx;
+    ry = qy;
+  }
+  x = rx;
+  y = ry;
+}
+
+// Saturating f32 -> i32 as the reference's i32(floor(.)) behaves on GPUs
+// (NaN -> 0).
+__device__ __forceinline__ int32_t sat_i32(float f) {
+  if (f != f) return 0;
+  if (f >= 2147483520.0f) return 2147483647;
+  if (f <= -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)f;
+}
+
+// world_to_pixel (shaders/core/utilities.wgsl:417-445).
+__device__ __forceinline__ void world_to_pixel(const ViewConst& v, float x, float y, int32_t& px,
+                                               int32_t& py) {
+  float tx = x - v.pan_x, ty = y - v.pan_y;
+  float rx = tx * v.cos_r - ty * v.sin_r;
+  float ry = tx * v.sin_r + ty * v.cos_r;
+  rx = rx * v.zoom;
+  ry = ry * v.zoom;
+  px = sat_i32(floorf(v.cx + rx * v.scale));
+  py = sat_i32(floorf(v.cy + ry * v.scale));
+}
+
+// Nearest texel of the clamp-to-edge palette at u = colour index.
+__device__ __forceinline__ int palette_texel(float ci, int size) {
+  float u = ci * (float)size;
+  if (!(u >= 0.0f)) return 0;
+  if (u >= (float)size) return size - 1;
+  int i = (int)floorf(u);
+  return i > size - 1 ? size - 1 : i;
+}
+
+enum StepFlags { ST_PLOT = 1, ST_RESPAWN = 2, ST_BURN = 4 };
+
+// The flame as one lane sees it: LDS rows + uniform header values.
+struct FlameView {
+  const float* rows;  // LDS, (n_xforms + n_finals) * XF_STRIDE
+  const int* ids;     // LDS, n_vars
+  int n_xforms, n_finals, n_vars;
+};
+
+// One iteration of main_template.wgsl's loop body (:193-1383), 2D,
+// palette colour mode, direct histogram. Walker state is (x, y, ci, fuse).
+// Returns ST_* flags; on ST_PLOT, (px, py) is the in-frame pixel and
+// `texel` the palette texel of the sample.
+__device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& v, int pal_size,
+                                          uint32_t burn_in, Rng& rng, float& x, float& y,
+                                          float& ci, uint32_t& fuse, int& xi_out, int32_t& px,
+                                          int32_t& py, int& texel) {
+  // select_transform_const (utilities.wgsl:35-55): first i with
+  // u * total <= cum_i, else the last; cum is the sequential f32 sum.
+  float u = rng_nextf(rng);
+  const int n = fl.n_xforms;
+  float target = u * fl.rows[(n - 1) * XF_STRIDE + XF_CUM];
+  int xi = n - 1;
+  for (int i = n - 2; i >= 0; i--)
+    if (target <= fl.rows[i * XF_STRIDE + XF_CUM]) xi = i;
+  xi_out = xi;
+  const float* row = fl.rows + xi * XF_STRIDE;
+  // Opacity draw: always consumed (main_template.wgsl:249).
+  bool should_plot = rng_nextf(rng) < row[XF_OPACITY];
+  float nx = x, ny = y;
+  apply_xform(row, fl.ids, fl.n_vars, nx, ny, rng);
+  // Bad-value respawn + re-fuse (main_template.wgsl:397-479).
+  if (!(fabsf(nx) <= 1e32f) || !(fabsf(ny) <= 1e32f)) {
+    x = rng_nextf(rng) * 2.0f - 1.0f;
+    y = rng_nextf(rng) * 2.0f - 1.0f;
+    fuse = burn_in;
+    px = py = -1;
+    return ST_RESPAWN;
+  }
+  x = nx;
+  y = ny;
+  // Colour-index blend, symmetry semantics (main_template.wgsl:496-503).
+  ci = ci * row[XF_C1] + row[XF_C2];
+  if (fuse > 0u) {
+    fuse -= 1u;
+    px = py = -1;
+    return ST_BURN;
+  }
+  // Final chain: reshapes the plot only (main_template.wgsl:663-717).
+  float fx = nx, fy = ny;
+  for (int i = 0; i < fl.n_finals; i++)
+    apply_xform(fl.rows + (n + i) * XF_STRIDE, fl.ids, fl.n_vars, fx, fy, rng);
+  world_to_pixel(v, fx, fy, px, py);
+  if (px >= 0 && px < v.width && py >= 0 && py < v.height && should_plot) {
+    texel = palette_texel(ci, pal_size);
+    return ST_PLOT;
+  }
+  return 0;
+}
+
+}  // namespace ffm

@@ -1,0 +1,99 @@
+// ffm_internal.h -- layouts shared by the host runtime and the gfx950 kernels.
+//
+// The flame table is what FlameRenderer::load_config uploads
+// (src/renderer/compute_kernel.rs:2748; GpuTransform src/gpu/buffers.rs:94-160)
+// re-laid for LDS: one row of XF_STRIDE floats per transform (odd stride, so
+// lanes reading different transforms hit different LDS banks), followed by
+// the per-flame uniform header.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ffm.h"
+
+namespace ffm {
+
+// Row layout of one transform in the table (floats).
+enum XfField : int {
+  XF_A = 0, XF_B, XF_C, XF_D, XF_E, XF_F,
+  XF_PA, XF_PB, XF_PC, XF_PD, XF_PE, XF_PF,
+  XF_CUM,      // sequential f32 cumulative selection weight (normal xforms)
+  XF_C1,       // (1 + s) / 2          (main_template.wgsl:496-503)
+  XF_C2,       // colour * (1 - s) / 2
+  XF_OPACITY,
+  XF_POST,     // 1.0 when the post-affine applies
+  XF_W0,       // FFM_MAX_VARS variation weights
+  XF_P0 = XF_W0 + FFM_MAX_VARS,  // FFM_MAX_VARS x FFM_MAX_VAR_PARAMS params
+  XF_STRIDE = XF_P0 + FFM_MAX_VARS * FFM_MAX_VAR_PARAMS  // 97: odd
+};
+
+// Per-flame header that precedes the rows (ints stored as int32).
+struct FlameHeader {
+  int32_t n_xforms;
+  int32_t n_finals;
+  int32_t n_vars;
+  int32_t var_mask;  // bit per ffm variation slot family, for fast paths
+  int32_t var_id[FFM_MAX_VARS];
+};
+
+// A flame table in device memory: header then (n_xforms + n_finals) rows.
+inline size_t flame_table_floats(int n_rows) {
+  return sizeof(FlameHeader) / 4 + (size_t)n_rows * XF_STRIDE;
+}
+
+// View constants with the uniform hoists of world_to_pixel done on the host
+// (cos/sin of the rotation; shaders/core/utilities.wgsl:417-445).
+struct ViewConst {
+  float pan_x, pan_y, cos_r, sin_r, zoom, scale, cx, cy;
+  int32_t width, height;
+};
+
+// Arguments of one iterate dispatch.
+struct IterArgs {
+  const float* flame;       // FlameHeader + rows
+  int32_t n_rows;           // n_xforms + n_finals (sizes the LDS copy)
+  const uint32_t* pal_u32;  // packed splat colour per texel: r | g<<8 | b<<16
+  const float* pal_f32;     // linear colour per texel x3 (f32 splat mode)
+  int32_t pal_size;
+  ViewConst view;
+  uint64_t walkers;
+  uint32_t walker0;         // first walker id (batched launches)
+  uint32_t ipt, burn_in, seed;
+  float4* orbits;           // x, y, colour index, fuse bits; may be null
+  int32_t fresh;
+  void* hist;               // u32 or f32 words
+  uint32_t hist_copy_words; // XCD mode: words per private copy
+  unsigned long long* plotted;
+  unsigned long long* inframe;
+};
+
+// Kernel launchers (defined in the .hip files).
+hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block,
+                          hipStream_t s);
+hipError_t launch_trace(const IterArgs& a, uint32_t walker, uint32_t steps, uint32_t* out,
+                        hipStream_t s);
+hipError_t launch_fold_xcd(uint32_t* hist, const uint32_t* copies, size_t words, int ncopies,
+                           hipStream_t s);
+hipError_t launch_clear_u32(void* p, size_t words, hipStream_t s);
+hipError_t launch_accumulate(const void* hist, int hist_is_f32, const float* prev, float* out,
+                             uint32_t* count, int W, int H, int mode, float blend,
+                             hipStream_t s);
+hipError_t launch_blur(uint32_t* hist, uint32_t* scratch, int W, int H, float sigma,
+                       float density_sigma, hipStream_t s);
+hipError_t launch_density_estimate(const float* in, float* out, int W, int H,
+                                   const float* filt_h2, const float* filt_inv_norm, int n_filt,
+                                   int max_R, float curve, hipStream_t s);
+
+struct TonemapConst {
+  int32_t mode, highlight, levels_enabled;
+  float k1, k2, gamma, funcval, gamma_threshold, vib, notvib, white, exposure;
+  float density_scale, saturation, hue_shift, alpha_low, alpha_high;
+  float levels_low, levels_high, levels_gamma, sample_density;
+  float bg_lin[3];
+  float inv22;
+};
+hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H,
+                          const TonemapConst& tc, hipStream_t s);
+
+}  // namespace ffm

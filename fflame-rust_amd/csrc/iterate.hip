@@ -1,0 +1,226 @@
+// iterate.hip -- the chaos-game walker kernel and its splat, for gfx950.
+//
+// Replaces the reference's compute dispatch `main`
+// (shaders/core/main_template.wgsl:20-1428, @workgroup_size(64), 128
+// workgroups, src/renderer/render.rs:350,647-666). Here the launch is a
+// persistent grid: each lane walks walkers w = gid, gid + nthreads, ...
+// with that walker's own PCG32 stream rng_init(walker, seed), so the
+// result does not depend on the launch shape. The flame table and the
+// palette live in LDS for the whole launch.
+//
+// Splat modes (the reference: 4 x u32 atomicAdd per in-frame sample,
+// main_template.wgsl:1253-1320):
+//   U32_GLOBAL  device-scope u32 atomics into one histogram.
+//   U32_XCD     one private histogram per XCD, u32 atomics at workgroup
+//               scope (resolved in that XCD's L2), folded by fold_xcd.
+//   F32_GLOBAL  f32 atomics ("float4 accumulation").
+#include "ffm_device.h"
+
+namespace ffm {
+
+// XCC id of the XCD this wave runs on (gfx940+ hardware register).
+__device__ __forceinline__ uint32_t xcc_id() {
+  uint32_t v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return v & 0xFu;
+}
+
+struct LdsFlame {
+  FlameView fv;
+  const uint32_t* pal_u32;
+  const float* pal_f32;
+};
+
+// Copy the flame table and palette into LDS; returns the LDS views.
+template <bool F32PAL>
+__device__ __forceinline__ LdsFlame load_lds(const IterArgs& a, uint32_t* lds) {
+  const FlameHeader* hdr = reinterpret_cast<const FlameHeader*>(a.flame);
+  const int nx = hdr->n_xforms, nf = hdr->n_finals, nv = hdr->n_vars;
+  const int rows_f = (nx + nf) * XF_STRIDE;
+  const int hdr_w = (int)(sizeof(FlameHeader) / 4);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.flame);
+  for (int i = threadIdx.x; i < hdr_w + rows_f; i += blockDim.x) lds[i] = src[i];
+  uint32_t* pal = lds + hdr_w + rows_f;
+  for (int i = threadIdx.x; i < a.pal_size; i += blockDim.x) pal[i] = a.pal_u32[i];
+  float* palf = reinterpret_cast<float*>(pal + a.pal_size);
+  if (F32PAL)
+    for (int i = threadIdx.x; i < 3 * a.pal_size; i += blockDim.x) palf[i] = a.pal_f32[i];
+  __syncthreads();
+  LdsFlame l;
+  l.fv.rows = reinterpret_cast<const float*>(lds + hdr_w);
+  l.fv.ids = reinterpret_cast<const int*>(lds + 4);
+  l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
+  l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
+  l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);
+  l.pal_u32 = pal;
+  l.pal_f32 = palf;
+  return l;
+}
+
+// Wave-level sum then one atomic per wave.
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long v) {
+  if (!dst) return;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v)
+    __hip_atomic_fetch_add(dst, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void iterate_kernel(IterArgs a) {
+  extern __shared__ uint32_t lds[];
+  LdsFlame L = load_lds<MODE == FFM_SPLAT_F32_GLOBAL>(a, lds);
+  const ViewConst v = a.view;
+  const int pal_size = a.pal_size;
+  uint32_t* hist_u = reinterpret_cast<uint32_t*>(a.hist);
+  float* hist_f = reinterpret_cast<float*>(a.hist);
+  if (MODE == FFM_SPLAT_U32_XCD) hist_u += (size_t)xcc_id() * a.hist_copy_words;
+
+  unsigned long long n_plotted = 0, n_in = 0;
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t wi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; wi < a.walkers;
+       wi += nthreads) {
+    const uint32_t walker = a.walker0 + (uint32_t)wi;
+    Rng rng = rng_init(walker, a.seed);
+    // Start point: two draws, always (main_template.wgsl:25-56).
+    float x = rng_nextf(rng) * 2.0f - 1.0f;
+    float y = rng_nextf(rng) * 2.0f - 1.0f;
+    float ci = 0.0f;
+    uint32_t fuse = a.burn_in;
+    if (a.orbits && !a.fresh) {
+      float4 o = a.orbits[wi];
+      x = o.x;
+      y = o.y;
+      ci = o.z;
+      fuse = __float_as_uint(o.w);
+    }
+    for (uint32_t it = 0; it < a.ipt; it++) {
+      int xi, texel = 0;
+      int32_t px, py;
+      int st = chaos_step(L.fv, v, pal_size, a.burn_in, rng, x, y, ci, fuse, xi, px, py, texel);
+      n_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
+      if (st & ST_PLOT) {
+        n_in++;
+        const size_t base = 4 * ((size_t)py * (size_t)v.width + (size_t)px);
+        if (MODE == FFM_SPLAT_F32_GLOBAL) {
+          const float* c = L.pal_f32 + 3 * texel;
+          __hip_atomic_fetch_add(hist_f + base + 0, c[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hist_f + base + 1, c[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hist_f + base + 2, c[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hist_f + base + 3, 100.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          const uint32_t pk = L.pal_u32[texel];
+          constexpr int SCOPE =
+              MODE == FFM_SPLAT_U32_XCD ? __HIP_MEMORY_SCOPE_WORKGROUP : __HIP_MEMORY_SCOPE_AGENT;
+          __hip_atomic_fetch_add(hist_u + base + 0, pk & 0xFFu, __ATOMIC_RELAXED, SCOPE);
+          __hip_atomic_fetch_add(hist_u + base + 1, (pk >> 8) & 0xFFu, __ATOMIC_RELAXED, SCOPE);
+          __hip_atomic_fetch_add(hist_u + base + 2, (pk >> 16) & 0xFFu, __ATOMIC_RELAXED, SCOPE);
+          __hip_atomic_fetch_add(hist_u + base + 3, 100u, __ATOMIC_RELAXED, SCOPE);
+        }
+      }
+    }
+    if (a.orbits) a.orbits[wi] = make_float4(x, y, ci, __uint_as_float(fuse));
+  }
+  wave_add_u64(a.plotted, n_plotted);
+  wave_add_u64(a.inframe, n_in);
+}
+
+// Per-step records of one walker (the parity trace): same step function.
+__global__ void trace_kernel(IterArgs a, uint32_t walker, uint32_t steps, uint32_t* out) {
+  extern __shared__ uint32_t lds[];
+  LdsFlame L = load_lds<false>(a, lds);
+  if (threadIdx.x != 0) return;
+  Rng rng = rng_init(walker, a.seed);
+  float x = rng_nextf(rng) * 2.0f - 1.0f;
+  float y = rng_nextf(rng) * 2.0f - 1.0f;
+  float ci = 0.0f;
+  uint32_t fuse = a.burn_in;
+  for (uint32_t s = 0; s < steps; s++) {
+    int xi, texel = 0;
+    int32_t px, py;
+    int st = chaos_step(L.fv, a.view, a.pal_size, a.burn_in, rng, x, y, ci, fuse, xi, px, py, texel);
+    uint32_t* o = out + 8 * (size_t)s;
+    o[0] = rng.state;
+    o[1] = (uint32_t)xi;
+    o[2] = __float_as_uint(x);
+    o[3] = __float_as_uint(y);
+    o[4] = __float_as_uint(ci);
+    o[5] = (uint32_t)px;
+    o[6] = (uint32_t)py;
+    o[7] = (uint32_t)st;
+  }
+}
+
+// Sum the per-XCD private copies into the histogram (coalesced uint4).
+__global__ void fold_xcd_kernel(uint32_t* __restrict__ hist, const uint32_t* __restrict__ copies,
+                                size_t words, int ncopies) {
+  const size_t n4 = words / 4;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (size_t)gridDim.x * blockDim.x) {
+    uint4 acc = reinterpret_cast<const uint4*>(copies)[i];
+    for (int c = 1; c < ncopies; c++) {
+      uint4 v = reinterpret_cast<const uint4*>(copies + (size_t)c * words)[i];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+    reinterpret_cast<uint4*>(hist)[i] = acc;
+  }
+}
+
+__global__ void clear_u32_kernel(uint4* __restrict__ p, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0, 0, 0, 0);
+}
+
+static size_t lds_bytes(const IterArgs& a, int nrows, bool f32pal) {
+  size_t w = flame_table_floats(nrows) + (size_t)a.pal_size + (f32pal ? 3 * (size_t)a.pal_size : 0);
+  return w * 4;
+}
+
+hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block, hipStream_t s) {
+  size_t shm = lds_bytes(a, a.n_rows, splat_mode == FFM_SPLAT_F32_GLOBAL);
+  switch (splat_mode) {
+    case FFM_SPLAT_U32_GLOBAL:
+      hipLaunchKernelGGL(iterate_kernel<FFM_SPLAT_U32_GLOBAL>, dim3(grid), dim3(block), shm, s, a);
+      break;
+    case FFM_SPLAT_U32_XCD:
+      hipLaunchKernelGGL(iterate_kernel<FFM_SPLAT_U32_XCD>, dim3(grid), dim3(block), shm, s, a);
+      break;
+    case FFM_SPLAT_F32_GLOBAL:
+      hipLaunchKernelGGL(iterate_kernel<FFM_SPLAT_F32_GLOBAL>, dim3(grid), dim3(block), shm, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_trace(const IterArgs& a, uint32_t walker, uint32_t steps, uint32_t* out,
+                        hipStream_t s) {
+  size_t shm = lds_bytes(a, a.n_rows, false);
+  hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(64), shm, s, a, walker, steps, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_xcd(uint32_t* hist, const uint32_t* copies, size_t words, int ncopies,
+                           hipStream_t s) {
+  size_t n4 = words / 4;
+  int grid = (int)((n4 + 255) / 256);
+  if (grid > 8192) grid = 8192;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(fold_xcd_kernel, dim3(grid), dim3(256), 0, s, hist, copies, words, ncopies);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_u32(void* p, size_t words, hipStream_t s) {
+  size_t n4 = words / 4;
+  int grid = (int)((n4 + 255) / 256);
+  if (grid > 8192) grid = 8192;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(clear_u32_kernel, dim3(grid), dim3(256), 0, s, reinterpret_cast<uint4*>(p), n4);
+  return hipGetLastError();
+}
+
+}  // namespace ffm

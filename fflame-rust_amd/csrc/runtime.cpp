@@ -1,0 +1,732 @@
+// runtime.cpp -- host side of libffm.so: the C ABI of include/ffm.h.
+//
+// Stands in for the reference's FlameRenderer (src/renderer/compute_kernel.rs)
+// and render loop (src/renderer/render.rs:371-800): it packs the flame and
+// palette into device tables, owns the per-context stream and buffers, runs
+// the dispatch / accumulate / tonemap cadence, and reports errors as status
+// codes plus a message, as RenderError and pyfae's fae_call do.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ffm_internal.h"
+
+using namespace ffm;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+}  // namespace
+
+struct ffm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::atomic<int> cancel{0};
+  int num_cu = 0;
+  // flame
+  bool has_flame = false;
+  ffm_flame_desc flame;
+  int n_rows = 0;
+  DevBuf table;
+  // palette
+  bool has_palette = false;
+  int pal_size = 0;
+  DevBuf pal_u32, pal_f32;
+  // render buffers
+  DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int fail(ffm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHECK(c, expr)                                                               \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail((c), e_ == hipErrorOutOfMemory ? FFM_ERR_OUT_OF_MEMORY : FFM_ERR_DEVICE, \
+                  "%s: %s", #expr, hipGetErrorString(e_));                              \
+  } while (0)
+
+int ensure(ffm_ctx* c, DevBuf& b, size_t bytes, const char* what) {
+  if (b.bytes >= bytes && b.p) return FFM_OK;
+  b.release();
+  if (bytes == 0) return FFM_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, FFM_ERR_OUT_OF_MEMORY,
+                "the GPU ran out of memory (%s, %zu bytes); try a smaller size", what, bytes);
+  }
+  b.bytes = bytes;
+  return FFM_OK;
+}
+
+hipStream_t pick(ffm_ctx* c, void* stream) {
+  return stream ? reinterpret_cast<hipStream_t>(stream) : c->stream;
+}
+
+bool supported_var(int id) {
+  switch (id) {
+    case FFM_VAR_LINEAR: case FFM_VAR_SINUSOIDAL: case FFM_VAR_SPHERICAL: case FFM_VAR_SWIRL:
+    case FFM_VAR_HORSESHOE: case FFM_VAR_POLAR: case FFM_VAR_DISC: case FFM_VAR_HYPERBOLIC:
+    case FFM_VAR_JULIA: case FFM_VAR_WAVES: case FFM_VAR_JULIAN: case FFM_VAR_BUBBLE:
+    case FFM_VAR_CURL:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Validation and derived values shared by every flame entry point.
+int check_flame(ffm_ctx* c, const ffm_flame_desc* f) {
+  if (!f) return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame is NULL");
+  if (f->n_xforms < 1) return fail(c, FFM_ERR_INVALID_ARGUMENT, "a flame needs at least one transform");
+  if (f->n_xforms > FFM_MAX_XFORMS)
+    return fail(c, FFM_ERR_TOO_LARGE, "%d transforms: at most %d fit", f->n_xforms, FFM_MAX_XFORMS);
+  if (f->n_finals < 0 || f->n_finals > FFM_MAX_FINALS)
+    return fail(c, FFM_ERR_TOO_LARGE, "%d final transforms: at most %d fit", f->n_finals, FFM_MAX_FINALS);
+  if (f->n_vars < 0 || f->n_vars > FFM_MAX_VARS)
+    return fail(c, FFM_ERR_TOO_LARGE, "%d active variations: at most %d fit", f->n_vars, FFM_MAX_VARS);
+  for (int k = 0; k < f->n_vars; k++)
+    if (!supported_var(f->var_id[k]))
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "variation id %d is not supported", f->var_id[k]);
+  return FFM_OK;
+}
+
+// init_julian (src/variations/defs/advanced.rs:467-474), in place.
+void prepare_flame(ffm_flame_desc* f) {
+  for (int k = 0; k < f->n_vars; k++) {
+    if (f->var_id[k] != FFM_VAR_JULIAN) continue;
+    for (int i = 0; i < f->n_xforms + f->n_finals; i++) {
+      ffm_xform* xf = i < f->n_xforms ? &f->xforms[i] : &f->finals[i - f->n_xforms];
+      float power = xf->var_param[k][0], dist = xf->var_param[k][1];
+      xf->var_param[k][2] = dist / std::max(std::fabs(power), 1e-30f) / 2.0f;
+    }
+  }
+}
+
+// Pack the flame into the device table layout of ffm_internal.h.
+std::vector<float> pack_flame(const ffm_flame_desc* f) {
+  const int rows = f->n_xforms + f->n_finals;
+  std::vector<float> t(flame_table_floats(rows), 0.0f);
+  FlameHeader h{};
+  h.n_xforms = f->n_xforms;
+  h.n_finals = f->n_finals;
+  h.n_vars = f->n_vars;
+  for (int k = 0; k < f->n_vars; k++) {
+    h.var_id[k] = f->var_id[k];
+    h.var_mask |= 1 << (f->var_id[k] & 31);
+  }
+  std::memcpy(t.data(), &h, sizeof h);
+  float* r0 = t.data() + sizeof(FlameHeader) / 4;
+  float cum = 0.0f;
+  for (int i = 0; i < rows; i++) {
+    const ffm_xform& x = i < f->n_xforms ? f->xforms[i] : f->finals[i - f->n_xforms];
+    float* r = r0 + (size_t)i * XF_STRIDE;
+    r[XF_A] = x.a; r[XF_B] = x.b; r[XF_C] = x.c; r[XF_D] = x.d; r[XF_E] = x.e; r[XF_F] = x.f;
+    r[XF_PA] = x.post_a; r[XF_PB] = x.post_b; r[XF_PC] = x.post_c;
+    r[XF_PD] = x.post_d; r[XF_PE] = x.post_e; r[XF_PF] = x.post_f;
+    if (i < f->n_xforms) {
+      cum += x.weight;  // the reference's sequential f32 sum
+      r[XF_CUM] = cum;
+    }
+    // (1 + s) / 2 and colour * (1 - s) / 2, as main_template.wgsl:496-503
+    r[XF_C1] = (1.0f + x.color_speed) / 2.0f;
+    r[XF_C2] = x.color * (1.0f - x.color_speed) / 2.0f;
+    r[XF_OPACITY] = x.opacity;
+    r[XF_POST] = x.post_enabled ? 1.0f : 0.0f;
+    for (int k = 0; k < f->n_vars; k++) {
+      r[XF_W0 + k] = x.var_weight[k];
+      for (int q = 0; q < FFM_MAX_VAR_PARAMS; q++)
+        r[XF_P0 + k * FFM_MAX_VAR_PARAMS + q] = x.var_param[k][q];
+    }
+  }
+  return t;
+}
+
+// Palette table (src/scene/palette.rs:265-360 sample/generate; f16 storage
+// src/gpu/buffers.rs:3215-3221; srgb_to_linear utilities.wgsl:143-146).
+void palette_sample(const float* s, int n, float t, float out[3]) {
+  if (n <= 0) {
+    out[0] = out[1] = out[2] = 1.0f;
+    return;
+  }
+  t = std::min(std::max(t, 0.0f), 1.0f);
+  if (t < s[0]) {
+    for (int c = 0; c < 3; c++) out[c] = s[1 + c];
+    return;
+  }
+  int idx = 0;
+  for (int i = 0; i < n; i++) {
+    if (s[4 * i] > t) break;
+    idx = i;
+  }
+  if (idx == n - 1) {
+    for (int c = 0; c < 3; c++) out[c] = s[4 * idx + 1 + c];
+    return;
+  }
+  const float* a = s + 4 * idx;
+  const float* b = a + 4;
+  float range = b[0] - a[0];
+  if (range < 1e-6f) {
+    for (int c = 0; c < 3; c++) out[c] = a[1 + c];
+    return;
+  }
+  float lt = (t - a[0]) / range;
+  for (int c = 0; c < 3; c++) out[c] = a[1 + c] + (b[1 + c] - a[1 + c]) * lt;
+}
+
+void palette_tables(const float* stops, int n, int size, std::vector<uint32_t>& pk,
+                    std::vector<float>& lin) {
+  pk.assign(size, 0);
+  lin.assign(3 * (size_t)size, 0.0f);
+  for (int i = 0; i < size; i++) {
+    float t = (float)i / (float)(size - 1);
+    float c[3];
+    palette_sample(stops, n, t, c);
+    uint32_t packed = 0;
+    for (int k = 0; k < 3; k++) {
+      float v = std::min(std::max(c[k], 0.0f), 1.0f);
+      float h = (float)(_Float16)v;  // round-to-nearest-even binary16
+      float l = std::pow(std::max(h, 0.0f), 2.2f);
+      lin[3 * i + k] = std::min(std::max(l, 0.0f), 1.0f) * 100.0f;
+      packed |= ((uint32_t)(std::min(std::max(l, 0.0f), 1.0f) * 100.0f)) << (8 * k);
+    }
+    pk[i] = packed;
+  }
+}
+
+ViewConst make_view(const ffm_view& v) {
+  ViewConst c;
+  c.pan_x = v.pan_x;
+  c.pan_y = v.pan_y;
+  c.cos_r = std::cos(v.rotation);
+  c.sin_r = std::sin(v.rotation);
+  c.zoom = v.zoom;
+  c.scale = (float)std::min(v.width, v.height) * 0.25f;
+  c.cx = (float)v.width * 0.5f;
+  c.cy = (float)v.height * 0.5f;
+  c.width = v.width;
+  c.height = v.height;
+  return c;
+}
+
+int check_view(ffm_ctx* c, const ffm_view* v) {
+  if (!v) return fail(c, FFM_ERR_INVALID_ARGUMENT, "view is NULL");
+  if (v->width < 1 || v->height < 1)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "image size %dx%d is empty", v->width, v->height);
+  if (v->width > 32768 || v->height > 32768)
+    return fail(c, FFM_ERR_TOO_LARGE, "image side over 32768 (%dx%d)", v->width, v->height);
+  return FFM_OK;
+}
+
+IterArgs base_args(ffm_ctx* c, const ffm_view& v) {
+  IterArgs a{};
+  a.flame = reinterpret_cast<const float*>(c->table.p);
+  a.n_rows = c->n_rows;
+  a.pal_u32 = reinterpret_cast<const uint32_t*>(c->pal_u32.p);
+  a.pal_f32 = reinterpret_cast<const float*>(c->pal_f32.p);
+  a.pal_size = c->pal_size;
+  a.view = make_view(v);
+  return a;
+}
+
+int iterate_grid(ffm_ctx* c, uint64_t walkers) {
+  uint64_t want = (walkers + 255) / 256;
+  uint64_t cap = (uint64_t)std::max(c->num_cu, 1) * 8;  // 8 x 256 lanes resident per CU
+  return (int)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+constexpr int XCD_COPIES = 8;
+
+// Host twin of OrbitFuses (src/renderer/compute_kernel.rs:73-102).
+struct OrbitFuses {
+  std::vector<uint32_t> left;
+  uint64_t dispatch(uint32_t threads, uint32_t ipt, uint32_t burn_in, uint32_t period,
+                    uint32_t seed) {
+    if (left.size() < threads) left.resize(threads, burn_in);
+    uint64_t plotted = 0;
+    for (uint32_t tid = 0; tid < threads; tid++) {
+      if (period != 0 && (tid + seed) % period == 0) left[tid] = burn_in;
+      uint32_t burnt = std::min(left[tid], ipt);
+      plotted += ipt - burnt;
+      left[tid] -= burnt;
+    }
+    return plotted;
+  }
+};
+
+// flam3 filter table (see ffm_density_estimate in include/ffm.h).
+void de_table(float max_r, float min_r, float curve, std::vector<float>& h2,
+              std::vector<float>& inv, int& R) {
+  double comp_max = (double)max_r + 1.0, comp_min = (double)min_r + 1.0;
+  h2.clear();
+  inv.clear();
+  const int keep = 100;
+  R = (int)std::ceil(comp_max);
+  for (int i = 0; i < 4096; i++) {
+    double adj = i < keep ? (double)i : std::pow((double)(i - keep), 1.0 / curve) + keep;
+    double h = comp_max / std::pow(adj + 1.0, curve);
+    bool last = false;
+    if (h <= comp_min) {
+      h = comp_min;
+      last = true;
+    }
+    int r = (int)std::ceil(h);
+    double sum = 0.0;
+    for (int dy = -r; dy <= r; dy++)
+      for (int dx = -r; dx <= r; dx++) {
+        double d = std::sqrt((double)(dx * dx + dy * dy)) / h;
+        if (d <= 1.0) sum += std::exp(-2.0 * (1.5 * d) * (1.5 * d));
+      }
+    h2.push_back((float)(h * h));
+    inv.push_back((float)(1.0 / sum));
+    if (last) break;
+  }
+}
+
+TonemapConst make_tonemap(const ffm_tonemap_params& tp, int W, int H, float zoom,
+                          double samples_in_buffer) {
+  // update_tonemap (src/renderer/compute_kernel.rs:3813-3918) and
+  // brightness_scale's uniform terms (shaders/tonemap.wgsl:94-113).
+  TonemapConst t{};
+  float apo_zoom = std::log2(zoom);
+  float ppu = (float)std::min(W, H) * 0.25f * std::pow(2.0f, apo_zoom);
+  float area = (float)((uint32_t)W * (uint32_t)H) / (ppu * ppu);
+  float total_pixels = std::max((float)W * (float)H, 1.0f);
+  float sd = std::max((float)samples_in_buffer / total_pixels, 1e-6f);
+  t.mode = tp.tonemap_mode;
+  t.highlight = tp.highlight_mode;
+  t.levels_enabled = tp.levels_enabled;
+  t.white = tp.white_level;
+  t.k1 = 1.0f * 2.3f * tp.brightness * 268.0f * 67108864.0f / 256.0f;
+  t.k2 = 1.0f / (1.0f * area * tp.white_level * sd);
+  t.gamma = tp.gamma == 0.0f ? tp.gamma : 1.0f / tp.gamma;
+  t.gamma_threshold = tp.gamma_threshold;
+  t.funcval = tp.gamma_threshold != 0.0f ? std::pow(tp.gamma_threshold, t.gamma - 1.0f) : 0.0f;
+  t.vib = std::nearbyint(tp.vibrancy * 256.0f);
+  t.notvib = 256.0f - t.vib;
+  t.exposure = tp.exposure;
+  t.density_scale = tp.density_scale;
+  t.saturation = tp.saturation;
+  t.hue_shift = tp.hue_shift;
+  t.alpha_low = tp.alpha_blend_low;
+  t.alpha_high = tp.alpha_blend_high;
+  t.levels_low = tp.levels_low;
+  t.levels_high = tp.levels_high;
+  t.levels_gamma = tp.levels_gamma;
+  t.sample_density = sd;
+  for (int k = 0; k < 3; k++) t.bg_lin[k] = std::pow(std::max(tp.background[k], 0.0f), 2.2f);
+  t.inv22 = (float)(1.0 / 2.2);
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffm_abi_version(void) { return FFM_ABI_VERSION; }
+
+int ffm_ctx_create(int device, ffm_ctx** out) {
+  if (!out) return FFM_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return FFM_ERR_DEVICE;
+  if (device < 0 || device >= n) return FFM_ERR_INVALID_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return FFM_ERR_DEVICE;
+  ffm_ctx* c = new ffm_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return FFM_ERR_DEVICE;
+  }
+  c->num_cu = prop.multiProcessorCount;
+  *out = c;
+  return FFM_OK;
+}
+
+int ffm_ctx_destroy(ffm_ctx* c) {
+  if (!c) return FFM_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->table, &c->pal_u32, &c->pal_f32, &c->hist, &c->copies, &c->scratch,
+                    &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
+                    &c->de_inv})
+    b->release();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FFM_OK;
+}
+
+const char* ffm_last_error(const ffm_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int ffm_cancel(ffm_ctx* c) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  c->cancel.store(1);
+  return FFM_OK;
+}
+
+int ffm_flame_set(ffm_ctx* c, const ffm_flame_desc* f) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  int st = check_flame(c, f);
+  if (st) return st;
+  (void)hipSetDevice(c->device);
+  c->flame = *f;
+  prepare_flame(&c->flame);
+  std::vector<float> t = pack_flame(&c->flame);
+  if ((st = ensure(c, c->table, t.size() * 4, "flame table"))) return st;
+  HIPCHECK(c, hipMemcpyAsync(c->table.p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->n_rows = f->n_xforms + f->n_finals;
+  c->has_flame = true;
+  return FFM_OK;
+}
+
+int ffm_palette_set(ffm_ctx* c, const float* stops, int n_stops, int table_size) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!stops || n_stops < 1) return fail(c, FFM_ERR_NO_PALETTE, "No palette found");
+  if (table_size < FFM_MIN_PALETTE || table_size > FFM_MAX_PALETTE)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "palette size %d outside %d..%d", table_size,
+                FFM_MIN_PALETTE, FFM_MAX_PALETTE);
+  (void)hipSetDevice(c->device);
+  std::vector<uint32_t> pk;
+  std::vector<float> lin;
+  palette_tables(stops, n_stops, table_size, pk, lin);
+  int st;
+  if ((st = ensure(c, c->pal_u32, pk.size() * 4, "palette"))) return st;
+  if ((st = ensure(c, c->pal_f32, lin.size() * 4, "palette"))) return st;
+  HIPCHECK(c, hipMemcpyAsync(c->pal_u32.p, pk.data(), pk.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->pal_f32.p, lin.data(), lin.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->pal_size = table_size;
+  c->has_palette = true;
+  return FFM_OK;
+}
+
+int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt, uint32_t burn_in,
+                uint32_t seed, void* orbits, int fresh, void* hist, int splat_mode,
+                uint64_t* plotted, uint64_t* inframe, void* stream) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!c->has_flame) return fail(c, FFM_ERR_NO_FLAME, "no flame set");
+  if (!c->has_palette) return fail(c, FFM_ERR_NO_PALETTE, "No palette found");
+  int st = check_view(c, view);
+  if (st) return st;
+  if (!hist) return fail(c, FFM_ERR_INVALID_ARGUMENT, "histogram is NULL");
+  if (walkers == 0 || walkers > 0xFFFFFFFFull)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers %llu outside 1..2^32-1",
+                (unsigned long long)walkers);
+  if (splat_mode != FFM_SPLAT_U32_GLOBAL && splat_mode != FFM_SPLAT_U32_XCD &&
+      splat_mode != FFM_SPLAT_F32_GLOBAL)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "splat mode %d not available here", splat_mode);
+  (void)hipSetDevice(c->device);
+  hipStream_t s = pick(c, stream);
+  IterArgs a = base_args(c, *view);
+  a.walkers = walkers;
+  a.ipt = ipt;
+  a.burn_in = burn_in;
+  a.seed = seed;
+  a.orbits = reinterpret_cast<float4*>(orbits);
+  a.fresh = fresh;
+  a.plotted = reinterpret_cast<unsigned long long*>(plotted);
+  a.inframe = reinterpret_cast<unsigned long long*>(inframe);
+  const size_t words = (size_t)view->width * view->height * 4;
+  if (splat_mode == FFM_SPLAT_U32_XCD) {
+    if ((st = ensure(c, c->copies, words * 4 * XCD_COPIES, "per-XCD histograms"))) return st;
+    HIPCHECK(c, launch_clear_u32(c->copies.p, words * XCD_COPIES, s));
+    a.hist = c->copies.p;
+    a.hist_copy_words = (uint32_t)words;
+  } else {
+    a.hist = hist;
+  }
+  HIPCHECK(c, launch_iterate(a, splat_mode, iterate_grid(c, walkers), 256, s));
+  if (splat_mode == FFM_SPLAT_U32_XCD)
+    HIPCHECK(c, launch_fold_xcd(reinterpret_cast<uint32_t*>(hist),
+                                reinterpret_cast<const uint32_t*>(c->copies.p), words, XCD_COPIES, s));
+  return FFM_OK;
+}
+
+int ffm_fold_xcd(ffm_ctx* c, uint32_t* hist, int width, int height, void* stream) {
+  if (!c || !hist) return FFM_ERR_INVALID_ARGUMENT;
+  size_t words = (size_t)width * height * 4;
+  if (c->copies.bytes < words * 4 * XCD_COPIES)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "no per-XCD histograms of that size");
+  HIPCHECK(c, launch_fold_xcd(hist, reinterpret_cast<const uint32_t*>(c->copies.p), words,
+                              XCD_COPIES, pick(c, stream)));
+  return FFM_OK;
+}
+
+int ffm_trace(ffm_ctx* c, const ffm_view* view, uint32_t walker, uint32_t seed, uint32_t burn_in,
+              uint32_t steps, uint32_t* out, void* stream) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!c->has_flame) return fail(c, FFM_ERR_NO_FLAME, "no flame set");
+  if (!c->has_palette) return fail(c, FFM_ERR_NO_PALETTE, "No palette found");
+  int st = check_view(c, view);
+  if (st) return st;
+  if (!out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "trace buffer is NULL");
+  (void)hipSetDevice(c->device);
+  IterArgs a = base_args(c, *view);
+  a.seed = seed;
+  a.burn_in = burn_in;
+  HIPCHECK(c, launch_trace(a, walker, steps, out, pick(c, stream)));
+  return FFM_OK;
+}
+
+int ffm_blur(ffm_ctx* c, uint32_t* hist, uint32_t* scratch, int width, int height, float sigma_px,
+             float density_sigma, void* stream) {
+  if (!c || !hist || !scratch || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  HIPCHECK(c, launch_blur(hist, scratch, width, height, sigma_px, density_sigma, pick(c, stream)));
+  return FFM_OK;
+}
+
+int ffm_density_estimate(ffm_ctx* c, const float* in, float* out, int width, int height,
+                         float max_radius, float min_radius, float curve, void* stream) {
+  if (!c || !in || !out || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
+  if (!(max_radius > 0.0f) || max_radius > 15.0f || min_radius < 0.0f || !(curve > 0.0f))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                "density estimation needs 0 < max_radius <= 15, min_radius >= 0, curve > 0");
+  (void)hipSetDevice(c->device);
+  std::vector<float> h2, inv;
+  int R;
+  de_table(max_radius, min_radius, curve, h2, inv, R);
+  int st;
+  if ((st = ensure(c, c->de_h2, h2.size() * 4, "estimator table"))) return st;
+  if ((st = ensure(c, c->de_inv, inv.size() * 4, "estimator table"))) return st;
+  hipStream_t s = pick(c, stream);
+  HIPCHECK(c, hipMemcpyAsync(c->de_h2.p, h2.data(), h2.size() * 4, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(c->de_inv.p, inv.data(), inv.size() * 4, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, launch_density_estimate(in, out, width, height,
+                                      reinterpret_cast<const float*>(c->de_h2.p),
+                                      reinterpret_cast<const float*>(c->de_inv.p), (int)h2.size(),
+                                      R, curve, s));
+  HIPCHECK(c, hipStreamSynchronize(s));  // the host tables are released on return
+  return FFM_OK;
+}
+
+int ffm_accumulate(ffm_ctx* c, const void* hist, int hist_is_f32, const float* prev, float* out,
+                   uint32_t* count, int width, int height, int mode, float blend_factor,
+                   void* stream) {
+  if (!c || !hist || !out || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
+  if (mode < 0 || mode > 2) return fail(c, FFM_ERR_INVALID_ARGUMENT, "accumulate mode %d", mode);
+  if (!prev && mode != 2) return fail(c, FFM_ERR_INVALID_ARGUMENT, "prev is NULL");
+  (void)hipSetDevice(c->device);
+  HIPCHECK(c, launch_accumulate(hist, hist_is_f32, prev ? prev : out, out, count, width, height,
+                                mode, blend_factor, pick(c, stream)));
+  return FFM_OK;
+}
+
+int ffm_tonemap(ffm_ctx* c, const float* accum, uint8_t* rgba, int width, int height, float zoom,
+                double samples_in_buffer, const ffm_tonemap_params* tp, void* stream) {
+  if (!c || !accum || !rgba || !tp || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
+  if (tp->tonemap_mode < 0 || tp->tonemap_mode > 2 || tp->highlight_mode < 0 ||
+      tp->highlight_mode > 3)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode %d / highlight mode %d not supported",
+                tp->tonemap_mode, tp->highlight_mode);
+  (void)hipSetDevice(c->device);
+  TonemapConst t = make_tonemap(*tp, width, height, zoom, samples_in_buffer);
+  HIPCHECK(c, launch_tonemap(accum, rgba, width, height, t, pick(c, stream)));
+  return FFM_OK;
+}
+
+// render_with (src/renderer/render.rs:483-800): dispatch until the target,
+// accumulate every batch_size dispatches (blur first when asked) and once
+// more for a partial batch, stop early after burn_in/ipt + 4 empty
+// dispatches; then the density estimator and the tonemap.
+static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float* accum_out,
+                       ffm_render_stats* stats) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!job) return fail(c, FFM_ERR_INVALID_ARGUMENT, "job is NULL");
+  if (!c->has_flame) return fail(c, FFM_ERR_NO_FLAME, "no flame set");
+  if (!c->has_palette) return fail(c, FFM_ERR_NO_PALETTE, "No palette found");
+  int st = check_view(c, &job->view);
+  if (st) return st;
+  if (job->walkers == 0 || job->walkers > 0xFFFFFFFFull || job->ipt == 0)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers and iterations per thread must be positive");
+  const int mode = job->splat_mode;
+  if (mode != FFM_SPLAT_U32_GLOBAL && mode != FFM_SPLAT_U32_XCD && mode != FFM_SPLAT_F32_GLOBAL)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "splat mode %d not available here", mode);
+  if (mode == FFM_SPLAT_F32_GLOBAL && job->filter_radius > 0.0f)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "the histogram blur reads a u32 histogram");
+  (void)hipSetDevice(c->device);
+  c->cancel.store(0);
+  auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = c->stream;
+  const int W = job->view.width, H = job->view.height;
+  const size_t npix = (size_t)W * H, words = npix * 4;
+  if ((st = ensure(c, c->hist, words * 4, "histogram"))) return st;
+  if ((st = ensure(c, c->accum, npix * 16, "accumulator"))) return st;
+  if (job->filter_radius > 0.0f && (st = ensure(c, c->scratch, words * 8, "blur scratch"))) return st;
+  if (mode == FFM_SPLAT_U32_XCD &&
+      (st = ensure(c, c->copies, words * 4 * XCD_COPIES, "per-XCD histograms")))
+    return st;
+  if (job->persistent_orbits && (st = ensure(c, c->orbits, job->walkers * 16, "orbits"))) return st;
+  if ((st = ensure(c, c->counters, 16, "counters"))) return st;
+  HIPCHECK(c, hipMemsetAsync(c->accum.p, 0, npix * 16, s));
+
+  IterArgs a = base_args(c, job->view);
+  a.walkers = job->walkers;
+  a.ipt = job->ipt;
+  a.burn_in = job->burn_in;
+  a.orbits = job->persistent_orbits ? reinterpret_cast<float4*>(c->orbits.p) : nullptr;
+  if (mode == FFM_SPLAT_U32_XCD) {
+    a.hist = c->copies.p;
+    a.hist_copy_words = (uint32_t)words;
+  } else {
+    a.hist = c->hist.p;
+  }
+  const int grid = iterate_grid(c, job->walkers);
+  OrbitFuses fuses;
+  uint64_t total = 0, batch_samples = 0, samples_in_buffer = 0, k = 0;
+  uint32_t batch_count = 0, empty = 0;
+  const uint32_t stall_limit = job->burn_in / std::max(job->ipt, 1u) + 4;
+  const uint32_t bs = job->batch_size ? job->batch_size : 4;
+  float iterate_ms = 0.0f;
+  for (;;) {
+    if (c->cancel.load()) return fail(c, FFM_ERR_CANCELLED, "Render cancelled");
+    if (batch_count == 0) {
+      if (mode == FFM_SPLAT_U32_XCD)
+        HIPCHECK(c, launch_clear_u32(c->copies.p, words * XCD_COPIES, s));
+      else
+        HIPCHECK(c, launch_clear_u32(c->hist.p, words, s));
+    }
+    a.seed = job->seed_base + (uint32_t)k * job->seed_stride;
+    a.fresh = k == 0;
+    HIPCHECK(c, hipEventRecord(c->ev0, s));
+    HIPCHECK(c, launch_iterate(a, mode, grid, 256, s));
+    HIPCHECK(c, hipEventRecord(c->ev1, s));
+    uint64_t plotted;
+    if (job->persistent_orbits) {
+      plotted = fuses.dispatch((uint32_t)job->walkers, job->ipt, job->burn_in, 0, a.seed);
+    } else {
+      plotted = job->walkers * (uint64_t)(job->ipt > job->burn_in ? job->ipt - job->burn_in : 0);
+    }
+    k++;
+    total += plotted;
+    batch_samples += plotted;
+    batch_count++;
+    empty = plotted == 0 ? empty + 1 : 0;
+    const bool done = total >= job->target_samples || empty >= stall_limit;
+    if (batch_count >= bs || done) {
+      if (mode == FFM_SPLAT_U32_XCD)
+        HIPCHECK(c, launch_fold_xcd(reinterpret_cast<uint32_t*>(c->hist.p),
+                                    reinterpret_cast<const uint32_t*>(c->copies.p), words,
+                                    XCD_COPIES, s));
+      if (job->filter_radius > 0.0f) {
+        float pixel_count = (float)W * (float)H;
+        float dsig = (((float)batch_samples / pixel_count) * 100.0f) *
+                     std::pow(100.0f, 2.0f * job->filter_blur_edges - 1.0f);
+        HIPCHECK(c, launch_blur(reinterpret_cast<uint32_t*>(c->hist.p),
+                                reinterpret_cast<uint32_t*>(c->scratch.p), W, H, job->filter_radius,
+                                dsig, s));
+      }
+      HIPCHECK(c, launch_accumulate(c->hist.p, mode == FFM_SPLAT_F32_GLOBAL,
+                                    reinterpret_cast<float*>(c->accum.p),
+                                    reinterpret_cast<float*>(c->accum.p), nullptr, W, H, 0, 0.0f, s));
+      samples_in_buffer += batch_samples;
+      batch_count = 0;
+      batch_samples = 0;
+    }
+    HIPCHECK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    iterate_ms += ms;
+    if (done) break;
+  }
+  float* final_acc = reinterpret_cast<float*>(c->accum.p);
+  if (job->de_max_radius > 0.0f) {
+    if ((st = ensure(c, c->accum2, npix * 16, "accumulator"))) return st;
+    st = ffm_density_estimate(c, final_acc, reinterpret_cast<float*>(c->accum2.p), W, H,
+                              job->de_max_radius, job->de_min_radius, job->de_curve, s);
+    if (st) return st;
+    final_acc = reinterpret_cast<float*>(c->accum2.p);
+  }
+  if (accum_out) {
+    HIPCHECK(c, hipMemcpyAsync(accum_out, final_acc, npix * 16, hipMemcpyDeviceToHost, s));
+  }
+  if (rgba_out) {
+    if ((st = ensure(c, c->rgba, npix * 4, "pixels"))) return st;
+    TonemapConst t = make_tonemap(job->tonemap, W, H, job->view.zoom, (double)samples_in_buffer);
+    if (job->tonemap.tonemap_mode < 0 || job->tonemap.tonemap_mode > 2 ||
+        job->tonemap.highlight_mode < 0 || job->tonemap.highlight_mode > 3)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode not supported");
+    HIPCHECK(c, launch_tonemap(final_acc, reinterpret_cast<uint8_t*>(c->rgba.p), W, H, t, s));
+    hipError_t e = hipMemcpyAsync(rgba_out, c->rgba.p, npix * 4, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess)
+      return fail(c, FFM_ERR_PIXEL_READ_FAILED, "pixel readback: %s", hipGetErrorString(e));
+  }
+  HIPCHECK(c, hipStreamSynchronize(s));
+  if (stats) {
+    stats->total_samples = total;
+    stats->dispatches = k;
+    stats->iterate_ms = iterate_ms;
+    stats->render_time_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return FFM_OK;
+}
+
+int ffm_render(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, ffm_render_stats* stats) {
+  if (!rgba_out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel buffer is NULL");
+  return render_impl(c, job, rgba_out, nullptr, stats);
+}
+
+int ffm_render_accum(ffm_ctx* c, const ffm_job* job, float* accum_out, ffm_render_stats* stats) {
+  if (!accum_out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "accumulator buffer is NULL");
+  return render_impl(c, job, nullptr, accum_out, stats);
+}
+
+int ffm_batch_render(ffm_ctx* c, int, const ffm_flame_desc*, const float* const*, const int*,
+                     const ffm_job*, uint8_t* const*, ffm_render_stats*) {
+  return fail(c, FFM_ERR_INVALID_ARGUMENT, "batched rendering is not built yet");
+}
+
+int ffm_batch_render_serial(ffm_ctx* c, int, const ffm_flame_desc*, const float* const*,
+                            const int*, const ffm_job*, uint8_t* const*, ffm_render_stats*) {
+  return fail(c, FFM_ERR_INVALID_ARGUMENT, "batched rendering is not built yet");
+}
+
+int ffm_softsplat_forward(ffm_ctx* c, const ffm_view*, uint64_t, uint32_t, uint32_t, uint32_t,
+                          float*, void*) {
+  return fail(c, FFM_ERR_INVALID_ARGUMENT, "soft splat is not built yet");
+}
+
+int ffm_softsplat_adjoint(ffm_ctx* c, const ffm_view*, uint64_t, uint32_t, uint32_t, uint32_t,
+                          uint32_t, const float*, float*, void*) {
+  return fail(c, FFM_ERR_INVALID_ARGUMENT, "soft splat is not built yet");
+}
+
+}  // extern "C"

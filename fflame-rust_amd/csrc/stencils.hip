@@ -1,0 +1,429 @@
+// stencils.hip -- the per-pixel passes after the splat, for gfx950:
+// accumulate, bilateral histogram blur, flam3-style density estimation and
+// the log-density tonemap. All are HBM-bound streaming stencils (no MFMA:
+// nothing here is a contraction); each thread owns whole pixels and moves
+// them as 16-byte vectors.
+#include <hip/hip_runtime.h>
+
+#include "ffm_internal.h"
+
+namespace ffm {
+
+static int grid_for(size_t n, int block, int cap = 16384) {
+  size_t g = (n + (size_t)block - 1) / (size_t)block;
+  if (g > (size_t)cap) g = (size_t)cap;
+  return g < 1 ? 1 : (int)g;
+}
+
+// ------------------------------------------------------------ accumulate
+
+// shaders/accumulate.wgsl:115-230 (no depth buffer). One pixel per thread:
+// reads the batch histogram (u32x4 or f32x4) and the previous accumulator,
+// writes the new accumulator (may alias prev) and, when `count` is set, the
+// exact 64-bit density count.
+template <bool F32H>
+__global__ __launch_bounds__(256) void accumulate_kernel(const void* __restrict__ hist_v,
+                                                         const float4* prev_buf, float4* out_buf,
+                                                         uint2* count, size_t npix, int mode,
+                                                         float blend) {
+  const float SCALE = 100.0f;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (size_t)gridDim.x * blockDim.x) {
+    float4 prev = mode == 2 ? make_float4(0.f, 0.f, 0.f, 0.f) : prev_buf[p];
+    float h0, h1, h2, h3;
+    uint32_t hd;
+    if (F32H) {
+      float4 h = reinterpret_cast<const float4*>(hist_v)[p];
+      h0 = h.x; h1 = h.y; h2 = h.z; h3 = h.w;
+      hd = (uint32_t)h.w;
+    } else {
+      uint4 h = reinterpret_cast<const uint4*>(hist_v)[p];
+      h0 = (float)h.x; h1 = (float)h.y; h2 = (float)h.z; h3 = (float)h.w;
+      hd = h.w;
+    }
+    float new_density = h3 / SCALE;
+    float total = prev.w + new_density;
+    if (count) {
+      uint2 c = count[p];
+      if (prev.w == 0.0f) {
+        c = make_uint2(0u, 0u);
+      } else if (c.x == 0u && c.y == 0u) {
+        float seed = prev.w * SCALE;
+        float hi = floorf(seed / 4294967296.0f);
+        c = make_uint2((uint32_t)(seed - hi * 4294967296.0f), (uint32_t)hi);
+      }
+      uint32_t lo = c.x + hd;
+      c = make_uint2(lo, c.y + (lo < c.x ? 1u : 0u));
+      count[p] = c;
+      total = ((float)c.y * 4294967296.0f + (float)c.x) / SCALE;
+      prev.w = total;
+    }
+    if (new_density <= 0.0f) {
+      out_buf[p] = prev;
+      continue;
+    }
+    float dn = fmaxf(h3, 1.0f);
+    float n0 = fminf(fmaxf(h0 / dn, 0.0f), 1.0f);
+    float n1 = fminf(fmaxf(h1 / dn, 0.0f), 1.0f);
+    float n2 = fminf(fmaxf(h2 / dn, 0.0f), 1.0f);
+    float cum_alpha = new_density / total;
+    float alpha = mode == 1 ? blend : cum_alpha;
+    float4 o;
+    o.x = prev.x + (n0 - prev.x) * alpha;
+    o.y = prev.y + (n1 - prev.y) * alpha;
+    o.z = prev.z + (n2 - prev.z) * alpha;
+    o.w = total;
+    out_buf[p] = o;
+  }
+}
+
+hipError_t launch_accumulate(const void* hist, int hist_is_f32, const float* prev, float* out,
+                             uint32_t* count, int W, int H, int mode, float blend, hipStream_t s) {
+  size_t npix = (size_t)W * H;
+  int grid = grid_for(npix, 256);
+  if (hist_is_f32)
+    hipLaunchKernelGGL(accumulate_kernel<true>, dim3(grid), dim3(256), 0, s, hist,
+                       reinterpret_cast<const float4*>(prev), reinterpret_cast<float4*>(out),
+                       reinterpret_cast<uint2*>(count), npix, mode, blend);
+  else
+    hipLaunchKernelGGL(accumulate_kernel<false>, dim3(grid), dim3(256), 0, s, hist,
+                       reinterpret_cast<const float4*>(prev), reinterpret_cast<float4*>(out),
+                       reinterpret_cast<uint2*>(count), npix, mode, blend);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ blur
+
+// shaders/histogram_blur.wgsl:39-119, one pass. `center` is the original
+// primary buffer, the joint-bilateral reference in both passes.
+__global__ __launch_bounds__(256) void blur_kernel(const uint4* __restrict__ in,
+                                                   uint4* __restrict__ out,
+                                                   const uint4* __restrict__ center, int W, int H,
+                                                   float radius_pixels, float density_sigma,
+                                                   int direction) {
+  const int KMAX = 8;
+  const size_t npix = (size_t)W * H;
+  float sigma = fmaxf(radius_pixels, 0.0001f);
+  int kh = (int)ceilf(3.0f * sigma);
+  if (kh > KMAX) kh = KMAX;
+  float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
+  float sigma_d = fmaxf(density_sigma, 1.0f);
+  float inv_2sd2 = 1.0f / (2.0f * sigma_d * sigma_d);
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (size_t)gridDim.x * blockDim.x) {
+    int x = (int)(p % (size_t)W), y = (int)(p / (size_t)W);
+    float cd = (float)center[p].w;
+    float sr = 0.f, sg = 0.f, sb = 0.f, sd = 0.f, wt = 0.f;
+    for (int k = -kh; k <= kh; k++) {
+      int nx = x, ny = y;
+      if (direction == 0)
+        nx = min(max(x + k, 0), W - 1);
+      else
+        ny = min(max(y + k, 0), H - 1);
+      uint4 t = in[(size_t)ny * W + nx];
+      float tr = (float)t.x, tg = (float)t.y, tb = (float)t.z, td = (float)t.w;
+      float fk = (float)k;
+      float w_sp = expf(-fk * fk * inv_2s2);
+      float delta = td - cd;
+      float w_d = expf(-delta * delta * inv_2sd2);
+      float w = w_sp * w_d;
+      sr = sr + tr * w;
+      sg = sg + tg * w;
+      sb = sb + tb * w;
+      sd = sd + td * w;
+      wt = wt + w;
+    }
+    float inv_w = 1.0f / fmaxf(wt, 1e-6f);
+    out[p] = make_uint4((uint32_t)(sr * inv_w), (uint32_t)(sg * inv_w), (uint32_t)(sb * inv_w),
+                        (uint32_t)(sd * inv_w));
+  }
+}
+
+__global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// H: hist -> scratch (centre from hist). V: scratch -> tmp (centre from hist),
+// where tmp is the second half of `scratch` (the caller gives 2x room), then
+// tmp -> hist.
+hipError_t launch_blur(uint32_t* hist, uint32_t* scratch, int W, int H, float sigma,
+                       float density_sigma, hipStream_t s) {
+  size_t npix = (size_t)W * H;
+  int grid = grid_for(npix, 256);
+  uint4* h = reinterpret_cast<uint4*>(hist);
+  uint4* sc = reinterpret_cast<uint4*>(scratch);
+  uint4* tmp = sc + npix;
+  hipLaunchKernelGGL(blur_kernel, dim3(grid), dim3(256), 0, s, h, sc, h, W, H, sigma,
+                     density_sigma, 0);
+  hipLaunchKernelGGL(blur_kernel, dim3(grid), dim3(256), 0, s, sc, tmp, h, W, H, sigma,
+                     density_sigma, 1);
+  hipLaunchKernelGGL(copy_u4_kernel, dim3(grid), dim3(256), 0, s, h, tmp, npix);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------- density estimation
+
+// Gather form of the flam3 estimator the host tabulates: pixel q of count
+// c uses filter idx(c) with squared half-width h2[idx] and normaliser
+// inv_norm[idx]; q reaches p when |p-q|^2 <= h2, with weight
+// exp(-4.5 |p-q|^2 / h2) * inv_norm. Each block gathers a 16x16 output
+// tile from an LDS tile holding the (16+2R)^2 sources.
+constexpr int DE_T = 16;
+constexpr int DE_RMAX = 16;
+
+__device__ __forceinline__ int de_index(float c, float curve, int n_filt) {
+  int idx;
+  if (c <= 100.0f)
+    idx = (int)ceilf(c) - 1;
+  else
+    idx = 100 + (int)floorf(powf(c - 100.0f, curve));
+  return min(max(idx, 0), n_filt - 1);
+}
+
+__global__ __launch_bounds__(256) void de_kernel(const float4* __restrict__ in,
+                                                 float4* __restrict__ out, int W, int H,
+                                                 const float* __restrict__ filt_h2,
+                                                 const float* __restrict__ filt_inv, int n_filt,
+                                                 int R, float curve) {
+  extern __shared__ float4 de_lds[];
+  const int S = DE_T + 2 * R;
+  float4* val = de_lds;                                   // S*S: (r c, g c, b c, c)
+  float2* flt = reinterpret_cast<float2*>(de_lds + S * S);  // S*S: (4.5/h2, inv_norm) ; h2
+  float* fh2 = reinterpret_cast<float*>(flt + S * S);
+  const int x0 = blockIdx.x * DE_T - R, y0 = blockIdx.y * DE_T - R;
+  for (int i = threadIdx.x; i < S * S; i += blockDim.x) {
+    int sx = x0 + i % S, sy = y0 + i / S;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 f = make_float2(0.f, 0.f);
+    float h2 = -1.0f;
+    if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+      float4 a = in[(size_t)sy * W + sx];
+      float c = a.w;
+      if (c > 0.0f) {
+        v = make_float4(a.x * c, a.y * c, a.z * c, c);
+        int idx = de_index(c, curve, n_filt);
+        h2 = filt_h2[idx];
+        f = make_float2(4.5f / h2, filt_inv[idx]);
+      }
+    }
+    val[i] = v;
+    flt[i] = f;
+    fh2[i] = h2;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % DE_T, ty = threadIdx.x / DE_T;
+  const int ox = blockIdx.x * DE_T + tx, oy = blockIdx.y * DE_T + ty;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int dy = -R; dy <= R; dy++) {
+    const int row = (ty + R + dy) * S + tx + R;
+    for (int dx = -R; dx <= R; dx++) {
+      const int i = row + dx;
+      const float d2 = (float)(dx * dx + dy * dy);
+      if (d2 <= fh2[i]) {
+        float w = expf(-d2 * flt[i].x) * flt[i].y;
+        float4 v = val[i];
+        acc.x += v.x * w;
+        acc.y += v.y * w;
+        acc.z += v.z * w;
+        acc.w += v.w * w;
+      }
+    }
+  }
+  if (ox < W && oy < H) {
+    float4 o;
+    if (acc.w > 0.0f) {
+      o = make_float4(acc.x / acc.w, acc.y / acc.w, acc.z / acc.w, acc.w);
+    } else {
+      o = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    out[(size_t)oy * W + ox] = o;
+  }
+}
+
+hipError_t launch_density_estimate(const float* in, float* out, int W, int H, const float* filt_h2,
+                                   const float* filt_inv_norm, int n_filt, int max_R, float curve,
+                                   hipStream_t s) {
+  if (max_R > DE_RMAX) return hipErrorInvalidValue;
+  const int S = DE_T + 2 * max_R;
+  size_t shm = (size_t)S * S * (sizeof(float4) + sizeof(float2) + sizeof(float));
+  dim3 grid((W + DE_T - 1) / DE_T, (H + DE_T - 1) / DE_T);
+  hipLaunchKernelGGL(de_kernel, grid, dim3(DE_T * DE_T), shm, s,
+                     reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out), W, H,
+                     filt_h2, filt_inv_norm, n_filt, max_R, curve);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------- tonemap
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) {
+  return fminf(fmaxf(v, lo), hi);
+}
+
+__device__ __forceinline__ uint8_t unorm8(float v) {
+  v = (v != v) ? 0.0f : clampf(v, 0.0f, 1.0f);
+  return (uint8_t)floorf(v * 255.0f + 0.5f);
+}
+
+// shaders/tonemap.wgsl:153-211
+__device__ void hsv_adjust(float c[3], float hue_shift, float saturation) {
+  float r = c[0], g = c[1], b = c[2];
+  float mx = fmaxf(fmaxf(r, g), b), mn = fminf(fminf(r, g), b);
+  float delta = mx - mn;
+  float h = 0.0f, s = 0.0f, v = mx;
+  if (delta > 0.00001f) {
+    s = delta / mx;
+    if (r >= mx)
+      h = (g - b) / delta;
+    else if (g >= mx)
+      h = 2.0f + (b - r) / delta;
+    else
+      h = 4.0f + (r - g) / delta;
+    h = h * 60.0f;
+    if (h < 0.0f) h = h + 360.0f;
+  }
+  if (hue_shift != 0.0f) {
+    h = h + hue_shift;
+    if (h < 0.0f)
+      h = h + 360.0f;
+    else if (h >= 360.0f)
+      h = h - 360.0f;
+  }
+  if (saturation != 1.0f) s = clampf(s * saturation, 0.0f, 1.0f);
+  if (s <= 0.0f) {
+    c[0] = c[1] = c[2] = v;
+    return;
+  }
+  float hh = h;
+  if (hh >= 360.0f) hh = 0.0f;
+  hh = hh / 60.0f;
+  uint32_t i = (uint32_t)hh;
+  float ff = hh - (float)i;
+  float p = v * (1.0f - s);
+  float q = v * (1.0f - (s * ff));
+  float t = v * (1.0f - (s * (1.0f - ff)));
+  switch (i) {
+    case 0: c[0] = v; c[1] = t; c[2] = p; break;
+    case 1: c[0] = q; c[1] = v; c[2] = p; break;
+    case 2: c[0] = p; c[1] = v; c[2] = t; break;
+    case 3: c[0] = p; c[1] = q; c[2] = v; break;
+    case 4: c[0] = t; c[1] = p; c[2] = v; break;
+    default: c[0] = v; c[1] = p; c[2] = q; break;
+  }
+}
+
+// shaders/tonemap.wgsl:224-590 fs_main as a compute pass over the
+// accumulator (uniform terms precomputed on the host).
+__global__ __launch_bounds__(256) void tonemap_kernel(const float4* __restrict__ accum,
+                                                      uchar4* __restrict__ rgba, size_t npix,
+                                                      TonemapConst t) {
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const float4 a = accum[p];
+    const float bucket_count = a.w * 100.0f;
+    float color[3];
+    float alpha = 0.0f;
+    if (t.mode == 0) {
+      color[0] = powf(a.x * t.exposure, t.gamma);
+      color[1] = powf(a.y * t.exposure, t.gamma);
+      color[2] = powf(a.z * t.exposure, t.gamma);
+      alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
+    } else if (t.mode == 2) {
+      float nd = clampf(bucket_count * 0.01f * t.exposure, 0.0f, 1.0f);
+      float gd = powf(nd, t.gamma);
+      color[0] = color[1] = color[2] = gd;
+      alpha = nd;
+    } else {
+      float bs = 0.0f;
+      if (!(bucket_count < 0.001f)) {
+        float l10 = logf(1.0f + t.white * bucket_count * t.k2) / logf(10.0f);
+        bs = (t.k1 * l10) / (t.white * bucket_count);
+      }
+      float ls = bs / 67108864.0f;
+      float fp0 = ls * (a.x * bucket_count);
+      float fp1 = ls * (a.y * bucket_count);
+      float fp2 = ls * (a.z * bucket_count);
+      float fp3 = ls * bucket_count * t.white;
+      if (fp3 > 0.0f) {
+        if (fp3 <= t.gamma_threshold) {
+          float frac = fp3 / t.gamma_threshold;
+          alpha = (1.0f - frac) * fp3 * t.funcval + frac * powf(fp3, t.gamma);
+        } else {
+          alpha = powf(fp3, t.gamma);
+        }
+      }
+      ls = fp3 > 0.0f ? t.vib * alpha / fp3 : 0.0f;
+      if (t.notvib > 0.0f) {
+        color[0] = ls * fp0 + t.notvib * powf(fp0, t.gamma);
+        color[1] = ls * fp1 + t.notvib * powf(fp1, t.gamma);
+        color[2] = ls * fp2 + t.notvib * powf(fp2, t.gamma);
+      } else {
+        color[0] = ls * fp0;
+        color[1] = ls * fp1;
+        color[2] = ls * fp2;
+      }
+    }
+    if (t.saturation != 1.0f || t.hue_shift != 0.0f) hsv_adjust(color, t.hue_shift, t.saturation);
+    if (t.mode == 1) {
+      color[0] = color[0] * t.exposure;
+      color[1] = color[1] * t.exposure;
+      color[2] = color[2] * t.exposure;
+    }
+    for (int k = 0; k < 3; k++) color[k] = fmaxf(color[k], 0.0f);
+    if (t.highlight == 1) {
+      float m = fmaxf(color[0], fmaxf(color[1], color[2]));
+      if (m > 1.0f)
+        for (int k = 0; k < 3; k++) color[k] = color[k] / m;
+    } else if (t.highlight == 2) {
+      float lum = color[0] * 0.2126f + color[1] * 0.7152f + color[2] * 0.0722f;
+      if (lum > 1e-6f) {
+        float lm = lum / (1.0f + lum);
+        for (int k = 0; k < 3; k++) color[k] = color[k] * (lm / lum);
+      }
+      for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
+    } else if (t.highlight == 3) {
+      for (int k = 0; k < 3; k++) {
+        float c = color[k];
+        c = (c * (2.51f * c + 0.03f)) / (c * (2.43f * c + 0.59f) + 0.14f);
+        color[k] = clampf(c, 0.0f, 1.0f);
+      }
+    } else {
+      for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
+    }
+    float leveled = 1.0f;
+    if (t.sample_density > 0.0f) {
+      float nd = a.w / t.sample_density;
+      if (t.levels_high <= t.levels_low) {
+        leveled = nd > t.levels_low ? 1.0f : 0.0f;
+      } else {
+        float nrm = (nd - t.levels_low) / (t.levels_high - t.levels_low);
+        float cl = clampf(nrm, 0.0f, 1.0f);
+        leveled = (t.levels_gamma != 1.0f && t.levels_gamma > 0.0f) ? powf(cl, t.levels_gamma) : cl;
+      }
+    }
+    float linear_alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
+    float gamma_alpha = clampf(alpha, 0.0f, 1.0f);
+    float st = clampf((gamma_alpha - t.alpha_low) / (t.alpha_high - t.alpha_low), 0.0f, 1.0f);
+    float blend_t = st * st * (3.0f - 2.0f * st);
+    float base_alpha = gamma_alpha * (1.0f - blend_t) + linear_alpha * blend_t;
+    float fa = t.levels_enabled ? fminf(base_alpha, leveled) : base_alpha;
+    uchar4 o;
+    o.x = unorm8(powf(t.bg_lin[0] * (1.0f - fa) + color[0] * fa, t.inv22));
+    o.y = unorm8(powf(t.bg_lin[1] * (1.0f - fa) + color[1] * fa, t.inv22));
+    o.z = unorm8(powf(t.bg_lin[2] * (1.0f - fa) + color[2] * fa, t.inv22));
+    o.w = 255;
+    rgba[p] = o;
+  }
+}
+
+hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H, const TonemapConst& tc,
+                          hipStream_t s) {
+  size_t npix = (size_t)W * H;
+  hipLaunchKernelGGL(tonemap_kernel, dim3(grid_for(npix, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(accum), reinterpret_cast<uchar4*>(rgba), npix,
+                     tc);
+  return hipGetLastError();
+}
+
+}  // namespace ffm

@@ -1,0 +1,299 @@
+/*
+ * ffm.h -- C ABI of the MI355X fractal-flame chaos-game engine (libffm.so).
+ *
+ * The path this library replaces is the reference's flame render:
+ * iterate -> splat -> (blur | density estimate) -> accumulate -> tonemap.
+ * Every entry point below names the reference interface it stands in for.
+ * Reference files are cited relative to the calibas/fflame-rust tree.
+ *
+ * Conventions (mirroring the reference's render API and pyfae's C ABI):
+ *  - Every call returns FFM_OK (0) or a negative ffm_status; the message of
+ *    the last failure on a context is ffm_last_error(ctx)
+ *    (src/renderer/render.rs:312-330 RenderError; wasm/python/src/lib.rs:80-101
+ *    returns status + error text the same way).
+ *  - Descriptors are caller-owned and copied on set.
+ *  - Device buffers passed to the granular calls are plain device pointers
+ *    (hipMalloc'd or a torch tensor's data_ptr()); `stream` is a hipStream_t
+ *    passed as void*, NULL meaning the context's own stream.
+ *  - One context per host thread; calls on one context are serialised.
+ *  - Plain C: no torch or HIP types appear in any signature.
+ */
+#ifndef FFM_H
+#define FFM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFM_ABI_VERSION 1
+
+/* Limits. The reference caps a flame at 128 transforms
+ * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
+ * texels (src/config/defaults.rs:30, src/gpu/buffers.rs:1511). */
+#define FFM_MAX_XFORMS 128
+#define FFM_MAX_FINALS 4
+#define FFM_MAX_VARS 16       /* distinct variations active in one flame */
+#define FFM_MAX_VAR_PARAMS 4  /* user params + derived slots per variation */
+#define FFM_MIN_PALETTE 256
+#define FFM_MAX_PALETTE 4096
+
+/* Status codes: RenderError::{OutOfMemory, TooLarge, Cancelled,
+ * PixelReadFailed, NoPaletteFound} (src/renderer/render.rs:312-330) plus
+ * argument and device errors. */
+typedef enum {
+  FFM_OK = 0,
+  FFM_ERR_INVALID_ARGUMENT = -1,
+  FFM_ERR_OUT_OF_MEMORY = -2,
+  FFM_ERR_TOO_LARGE = -3,
+  FFM_ERR_CANCELLED = -4,
+  FFM_ERR_PIXEL_READ_FAILED = -5,
+  FFM_ERR_NO_PALETTE = -6,
+  FFM_ERR_DEVICE = -7,
+  FFM_ERR_NO_FLAME = -8
+} ffm_status;
+
+/* Variation ids are the reference registry indices, which are append-only
+ * "for preset compatibility" (src/variations/defs/mod.rs:442-501). */
+typedef enum {
+  FFM_VAR_LINEAR = 0,
+  FFM_VAR_SINUSOIDAL = 1,
+  FFM_VAR_SPHERICAL = 2,
+  FFM_VAR_SWIRL = 3,
+  FFM_VAR_HORSESHOE = 4,
+  FFM_VAR_POLAR = 5,
+  FFM_VAR_DISC = 8,
+  FFM_VAR_HYPERBOLIC = 10,
+  FFM_VAR_JULIA = 13,
+  FFM_VAR_WAVES = 15,
+  FFM_VAR_JULIAN = 24, /* params: [0]=power, [1]=dist; [2]=cpower derived */
+  FFM_VAR_BUBBLE = 27,
+  FFM_VAR_CURL = 47    /* params: [0]=c1, [1]=c2 */
+} ffm_variation_id;
+
+/* One transform. Field meaning follows the reference Transform / GPU
+ * Transform (src/scene/transforms.rs:337-482; shaders/core/header.wgsl:5-65):
+ * x' = a*x + b*y + e, y' = c*x + d*y + f (shaders/core/affine.wgsl:4-19).
+ * `var_weight[k]` is the weight of the flame's k-th active variation
+ * (ffm_flame_desc.var_id[k]); 0 leaves it out, as the reference's
+ * `if (xform.variations[k] != 0.0)` gate does
+ * (src/shader_builder_v2.rs:2742-2758). */
+typedef struct {
+  float a, b, c, d, e, f;
+  float post_a, post_b, post_c, post_d, post_e, post_f;
+  int32_t post_enabled; /* reference: post_enabled > 0.5 */
+  float weight;         /* selection weight (ignored for finals) */
+  float color;          /* palette position of this transform */
+  float color_speed;    /* Apophysis "symmetry" s (shaders/core/main_template.wgsl:496-503) */
+  float opacity;
+  float var_weight[FFM_MAX_VARS];
+  float var_param[FFM_MAX_VARS][FFM_MAX_VAR_PARAMS];
+} ffm_xform;
+
+/* A flame: normal transforms, the final chain, and the flame's active
+ * variation list in emission (local index) order
+ * (src/scene/transforms.rs:30-85 compute_local_index_map). */
+typedef struct {
+  int32_t n_xforms;
+  int32_t n_finals;
+  int32_t n_vars;
+  int32_t var_id[FFM_MAX_VARS];
+  ffm_xform xforms[FFM_MAX_XFORMS];
+  ffm_xform finals[FFM_MAX_FINALS];
+} ffm_flame_desc;
+
+/* The view: FractalConfig zoom/pan/rotation as world_to_pixel reads them
+ * (shaders/core/utilities.wgsl:417-445). */
+typedef struct {
+  int32_t width, height;
+  float zoom, pan_x, pan_y, rotation;
+} ffm_view;
+
+/* Histogram / splat modes. The reference splats 4 x u32 per in-frame
+ * sample with global atomicAdd (shaders/core/main_template.wgsl:1253-1320). */
+typedef enum {
+  FFM_SPLAT_U32_GLOBAL = 0, /* 4 x u32 device-scope atomics, as the reference */
+  FFM_SPLAT_U32_XCD = 1,    /* u32, one private copy per XCD, folded after */
+  FFM_SPLAT_U32_LDS = 2,    /* u32, LDS-privatised screen tiles flushed with integer atomics */
+  FFM_SPLAT_F32_GLOBAL = 3  /* 4 x f32 atomics ("float4 accumulation") */
+} ffm_splat_mode;
+
+/* Tonemap parameters: TonemapParams (shaders/tonemap.wgsl:9-63) as the
+ * host fills it (src/renderer/compute_kernel.rs:3813-3918), log branch and
+ * the modes around it. */
+typedef struct {
+  int32_t tonemap_mode;   /* 0 linear, 1 logarithmic (default), 2 density */
+  int32_t highlight_mode; /* 0 clip, 1 maxnorm, 2 reinhard, 3 aces */
+  float exposure, gamma, gamma_threshold, brightness, vibrancy, white_level;
+  float density_scale, saturation, hue_shift;
+  float alpha_blend_low, alpha_blend_high;
+  int32_t levels_enabled;
+  float levels_low, levels_high, levels_gamma;
+  float background[3]; /* sRGB */
+} ffm_tonemap_params;
+
+/* A render job: RenderJob (src/renderer/render.rs:111-138) plus the knobs
+ * the loop reads. The loop dispatches `walkers` walkers of `ipt` iterations
+ * until `target_samples` samples have been plotted (0: one dispatch),
+ * accumulating every `batch_size` dispatches (render.rs:350-351,624-710). */
+typedef struct {
+  ffm_view view;
+  uint64_t walkers;
+  uint32_t ipt, burn_in;
+  uint32_t seed_base, seed_stride; /* seed_k = base + k*stride (compute_kernel.rs:3541-3554) */
+  uint64_t target_samples;
+  uint32_t batch_size;
+  int32_t splat_mode;      /* ffm_splat_mode */
+  int32_t persistent_orbits; /* 1: walkers resume across dispatches */
+  float filter_radius;     /* histogram_blur sigma in px (0: off) */
+  float filter_blur_edges; /* bilateral edge slider, 0..1 (compute_kernel.rs:1668-1700) */
+  float de_max_radius, de_min_radius, de_curve; /* flam3 density estimation (0 max: off) */
+  ffm_tonemap_params tonemap;
+} ffm_job;
+
+typedef struct {
+  uint64_t total_samples;   /* plotted samples (RenderOutput::total_iterations) */
+  uint64_t dispatches;
+  double render_time_ms;
+  double iterate_ms;        /* GPU time in the iterate kernels */
+} ffm_render_stats;
+
+typedef struct ffm_ctx ffm_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* FlameRenderer::for_job (src/renderer/compute_kernel.rs:784): one renderer
+ * per device, owning its stream and device buffers. */
+int ffm_ctx_create(int device, ffm_ctx** out);
+int ffm_ctx_destroy(ffm_ctx* ctx);
+/* pyfae's error text (wasm/python/src/lib.rs:80-101 status 1 + message). */
+const char* ffm_last_error(const ffm_ctx* ctx);
+int ffm_abi_version(void);
+
+/* FlameRenderer::load_config (src/renderer/compute_kernel.rs:2748): packs
+ * the transforms (GpuTransform, src/gpu/buffers.rs:94-160) and derives
+ * variation init values (init_julian, src/variations/defs/advanced.rs:467). */
+int ffm_flame_set(ffm_ctx* ctx, const ffm_flame_desc* flame);
+
+/* Palette upload: stops (position, r, g, b) as sRGB floats, sampled into a
+ * `table_size`-texel table with linear interpolation
+ * (src/scene/palette.rs:265-360), stored as f16 (src/gpu/buffers.rs:3215-3221)
+ * and read nearest (src/gpu/buffers.rs:1866-1875). */
+int ffm_palette_set(ffm_ctx* ctx, const float* stops_prgb, int n_stops, int table_size);
+
+/* ---- whole render ------------------------------------------------------ */
+
+/* render / render_with (src/renderer/render.rs:371,483-800) ending in the
+ * tonemap and pixel readback (src/renderer/tail.rs:112-114): RGBA8 sRGB
+ * pixels into `rgba_out` (width*height*4 bytes, host memory). */
+int ffm_render(ffm_ctx* ctx, const ffm_job* job, uint8_t* rgba_out, ffm_render_stats* stats);
+
+/* The same render, leaving the accumulator (RGBA32F, row-major) in
+ * `accum_out` (host, width*height*4 floats) instead of tonemapping. */
+int ffm_render_accum(ffm_ctx* ctx, const ffm_job* job, float* accum_out, ffm_render_stats* stats);
+
+/* RenderProgress::is_cancelled (src/renderer/render.rs:293-302): polled
+ * between dispatches; the render returns FFM_ERR_CANCELLED. */
+int ffm_cancel(ffm_ctx* ctx);
+
+/* ---- granular kernels on device buffers --------------------------------- */
+
+/* One compute dispatch (compute_pass, src/renderer/compute_kernel.rs:1243;
+ * kernel shaders/core/main_template.wgsl:20-1428): `walkers` walkers run
+ * `ipt` iterations from rng_init(walker, seed). `orbits` (16 B per walker:
+ * x, y, colour index, fuse) holds resumable walker state
+ * (main_template.wgsl:125-181,1384-1416); NULL starts every walker afresh.
+ * `fresh` != 0 (re)starts walkers whose state is in `orbits`.
+ * `hist` is width*height*4 words (u32, or f32 for FFM_SPLAT_F32_GLOBAL).
+ * `plotted` (device u64, may be NULL) gets the number of samples that
+ * passed the fuse (in frame or not); `inframe` (device u64, may be NULL)
+ * the number splatted. */
+int ffm_iterate(ffm_ctx* ctx, const ffm_view* view, uint64_t walkers, uint32_t ipt,
+                uint32_t burn_in, uint32_t seed, void* orbits, int fresh, void* hist,
+                int splat_mode, uint64_t* plotted, uint64_t* inframe, void* stream);
+
+/* Per-step trace of one walker for parity tests: for each of `steps`
+ * iterations writes 8 words {rng state after the step, xform index,
+ * bits(x), bits(y), bits(colour index), pixel x, pixel y, flags}
+ * (flags: bit0 plotted in frame, bit1 respawned, bit2 burning in).
+ * `out` is device memory of steps*8 u32. */
+int ffm_trace(ffm_ctx* ctx, const ffm_view* view, uint32_t walker, uint32_t seed,
+              uint32_t burn_in, uint32_t steps, uint32_t* out, void* stream);
+
+/* Separable bilateral Gaussian on the batch histogram
+ * (shaders/histogram_blur.wgsl:39-119, host compute_kernel.rs:1668-1727):
+ * H pass hist -> scratch, V pass scratch -> hist. */
+int ffm_blur(ffm_ctx* ctx, uint32_t* hist, uint32_t* scratch, int width, int height,
+             float sigma_px, float density_sigma, void* stream);
+
+/* flam3-style density estimation on the accumulator (no reference
+ * counterpart: src/flame_xml.rs:1488-1490 keeps the estimator attributes
+ * inert). Each pixel's colour sums and count spread over a Gaussian disc
+ * whose radius shrinks with the pixel's count as max_radius/(count)^curve,
+ * floored at min_radius (the Draves-Reckase estimator, flam3's filter
+ * table). In/out are RGBA32F accumulators (mean colour, count). */
+int ffm_density_estimate(ffm_ctx* ctx, const float* accum_in, float* accum_out, int width,
+                         int height, float max_radius, float min_radius, float curve,
+                         void* stream);
+
+/* accumulate.wgsl (shaders/accumulate.wgsl:115-230): batch histogram into
+ * the RGBA32F running accumulator. mode 0 cumulative mean, 1 fixed EMA at
+ * blend_factor, 2 overwrite. `count` (u32x2 per pixel) non-NULL enables the
+ * exact 64-bit density count. `hist_is_f32` reads an f32 histogram. */
+int ffm_accumulate(ffm_ctx* ctx, const void* hist, int hist_is_f32, const float* prev,
+                   float* out, uint32_t* count, int width, int height, int mode,
+                   float blend_factor, void* stream);
+
+/* tonemap.wgsl fs_main (shaders/tonemap.wgsl:224-590) as a compute
+ * stencil: accumulator -> RGBA8 sRGB. `samples_in_buffer` feeds
+ * sample_density (compute_kernel.rs:1796-1841); area per update_tonemap
+ * (compute_kernel.rs:3813-3918). */
+int ffm_tonemap(ffm_ctx* ctx, const float* accum, uint8_t* rgba, int width, int height,
+                float zoom, double samples_in_buffer, const ffm_tonemap_params* tp,
+                void* stream);
+
+/* Fold the per-XCD private histograms of FFM_SPLAT_U32_XCD into `hist`. */
+int ffm_fold_xcd(ffm_ctx* ctx, uint32_t* hist, int width, int height, void* stream);
+
+/* ---- batched gallery ------------------------------------------------------ */
+
+/* Many independent flames in one persistent launch (no reference
+ * counterpart: the gallery renders in series, wasm/render/src/lib.rs:45-53).
+ * Flame i renders jobs[i] (its own view, walkers, ipt, seed) with palette
+ * i (stops_prgb[i], n_stops[i]); its RGBA8 pixels go to rgba_out[i]. */
+int ffm_batch_render(ffm_ctx* ctx, int n, const ffm_flame_desc* flames,
+                     const float* const* stops_prgb, const int* n_stops,
+                     const ffm_job* jobs, uint8_t* const* rgba_out, ffm_render_stats* stats);
+
+/* The same flames launched one after another (the reference's behaviour),
+ * for the batched/serial comparison. */
+int ffm_batch_render_serial(ffm_ctx* ctx, int n, const ffm_flame_desc* flames,
+                            const float* const* stops_prgb, const int* n_stops,
+                            const ffm_job* jobs, uint8_t* const* rgba_out,
+                            ffm_render_stats* stats);
+
+/* ---- differentiable soft splat ---------------------------------------------- */
+
+/* Bilinear soft splat of the chaos game into an f32 density image (no
+ * reference counterpart: splat_size is hard-wired to 1.0 and read by
+ * nothing, src/renderer/compute_kernel.rs:1354). The differentiable
+ * parameters are packed as theta = [n_xforms x 6 affine (a,b,c,d,e,f)]
+ * then [n_xforms x n_vars variation weights]. */
+int ffm_softsplat_forward(ffm_ctx* ctx, const ffm_view* view, uint64_t walkers, uint32_t ipt,
+                          uint32_t burn_in, uint32_t seed, float* density, void* stream);
+
+/* Truncated adjoint: d loss / d theta where dL/d density is `grad_density`
+ * (f32, width*height), through the last `K` iterations of each walker,
+ * recomputing trajectories by RNG replay. `grad_theta` (device f32,
+ * n_xforms*6 + n_xforms*n_vars) is overwritten. */
+int ffm_softsplat_adjoint(ffm_ctx* ctx, const ffm_view* view, uint64_t walkers, uint32_t ipt,
+                          uint32_t burn_in, uint32_t seed, uint32_t K,
+                          const float* grad_density, float* grad_theta, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFM_H */

@@ -1,0 +1,1212 @@
+/*
+ * ffm_oracle.c -- CPU restatement of the reference's flame path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg. The product
+ * library (libffm.so) never links, loads or calls it.
+ *
+ * Build: -O2 -ffp-contract=off (no FMA contraction, no fast-math), so each
+ * FP32 operation rounds once, in the order the reference's WGSL writes it.
+ * Transcendentals are the host libm's (sinf, cosf, atan2f, powf, expf,
+ * logf); the reference's are whatever the GPU driver compiles, which no
+ * reference test pins.
+ *
+ * Pinning (what checks each function against something other than itself):
+ *  - RNG: published PCG-RXS-M-XS constants plus an independent numpy
+ *    restatement and its period/bijectivity properties (tests).
+ *  - select / affine / colour / world_to_pixel / splat / accumulate /
+ *    OrbitFuses: the reference's known-answer tests
+ *    (src/renderer/compute_kernel.rs:6056-6144,6404-6508,6997-7120) and
+ *    brute-force/closed-form checks.
+ *  - variations: closed forms evaluated independently in FP64 (tests).
+ *  - tonemap: the reference's own rendered baseline
+ *    tests/visual/baseline/2d-simple-linear.png (golden fixture) through
+ *    the full render.
+ *  - density estimation: no reference implementation exists
+ *    (src/flame_xml.rs:1488-1490); pinned by mass conservation, constant
+ *    fields and brute force only.
+ */
+#include "ffm_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define F32(x) ((float)(x))
+
+/* ------------------------------------------------------------------ RNG */
+
+/* shaders/core/rng.wgsl:22-27 */
+uint32_t orc_pcg_hash(uint32_t input) {
+  uint32_t state = input * 747796405u + 2891336453u;
+  uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+/* shaders/core/rng.wgsl:29-37 */
+void orc_rng_init(orc_rng* r, uint32_t thread_id, uint32_t seed) {
+  r->state = orc_pcg_hash(thread_id ^ seed);
+  r->inc = (orc_pcg_hash(orc_pcg_hash(thread_id) + seed) << 1u) | 1u;
+}
+
+/* shaders/core/rng.wgsl:39-45 */
+uint32_t orc_rng_next(orc_rng* r) {
+  uint32_t old = r->state;
+  r->state = old * 747796405u + r->inc;
+  uint32_t xs = ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
+  return (xs >> 22u) ^ xs;
+}
+
+/* shaders/core/rng.wgsl:53-55: top 24 bits times 2^-24 */
+float orc_rng_nextf(orc_rng* r) { return (float)(orc_rng_next(r) >> 8u) * (1.0f / 16777216.0f); }
+
+void orc_rng_stream(uint32_t tid, uint32_t seed, uint32_t n, uint32_t* out) {
+  orc_rng r;
+  orc_rng_init(&r, tid, seed);
+  for (uint32_t i = 0; i < n; i++) out[i] = orc_rng_next(&r);
+}
+
+/* ------------------------------------------------------------ selection */
+
+/* shaders/core/utilities.wgsl:35-55 select_transform_const: sequential f32
+ * sums, first i with target <= cumulative, else the last. */
+int orc_select(const ffm_flame_desc* f, float rand_val) {
+  float total = 0.0f;
+  for (int i = 0; i < f->n_xforms; i++) total += f->xforms[i].weight;
+  float target = rand_val * total;
+  float cum = 0.0f;
+  for (int i = 0; i < f->n_xforms; i++) {
+    cum += f->xforms[i].weight;
+    if (target <= cum) return i;
+  }
+  return f->n_xforms - 1;
+}
+
+/* ----------------------------------------------------------- variations */
+
+static const float PI_F = F32(3.14159265359);  /* the WGSL literal */
+
+/* shaders/core/utilities.wgsl:604-611 ff_atan2: signed-zero origin handled
+ * by sign bits, else atan2. */
+static float ff_atan2(float y, float x) {
+  if (y == 0.0f && x == 0.0f) {
+    const float pi = F32(3.14159265358979);
+    uint32_t bx, by;
+    memcpy(&bx, &x, 4);
+    memcpy(&by, &y, 4);
+    float mag = (bx & 0x80000000u) ? pi : 0.0f;
+    return (by & 0x80000000u) ? -mag : mag;
+  }
+  return atan2f(y, x);
+}
+
+static float len2(float x, float y) { return x * x + y * y; }
+
+/* The 13 bodies. Sources: src/variations/defs/basic.rs (linear :37,
+ * sinusoidal :100, spherical :130, swirl :162, horseshoe :198) and
+ * src/variations/defs/advanced.rs (polar :32, disc :134, hyperbolic :213,
+ * julia :329, waves :414, julian :478, bubble :639, curl :1686). */
+void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, float y,
+                   orc_rng* rng, float* ox, float* oy) {
+  switch (var_id) {
+    case FFM_VAR_LINEAR:
+      *ox = x;
+      *oy = y;
+      return;
+    case FFM_VAR_SINUSOIDAL:
+      *ox = sinf(x);
+      *oy = sinf(y);
+      return;
+    case FFM_VAR_SPHERICAL: {
+      float r2 = len2(x, y) + 1e-6f;
+      *ox = x / r2;
+      *oy = y / r2;
+      return;
+    }
+    case FFM_VAR_SWIRL: {
+      float r2 = len2(x, y);
+      float s = sinf(r2), c = cosf(r2);
+      *ox = x * s - y * c;
+      *oy = x * c + y * s;
+      return;
+    }
+    case FFM_VAR_HORSESHOE: {
+      float r = sqrtf(len2(x, y)) + 1e-6f;
+      float r_inv = 1.0f / r;
+      *ox = (x - y) * (x + y) * r_inv;
+      *oy = 2.0f * x * y * r_inv;
+      return;
+    }
+    case FFM_VAR_POLAR: {
+      float theta = atan2f(x, y);
+      float r = sqrtf(len2(x, y));
+      *ox = theta / PI_F;
+      *oy = r - 1.0f;
+      return;
+    }
+    case FFM_VAR_DISC: {
+      float theta = atan2f(x, y);
+      float r = sqrtf(len2(x, y));
+      float theta_pi = theta / PI_F;
+      float pi_r = PI_F * r;
+      *ox = theta_pi * sinf(pi_r);
+      *oy = theta_pi * cosf(pi_r);
+      return;
+    }
+    case FFM_VAR_HYPERBOLIC: {
+      float r2 = len2(x, y) + 1e-6f;
+      *ox = x / r2;
+      *oy = y;
+      return;
+    }
+    case FFM_VAR_JULIA: {
+      float r = sqrtf(len2(x, y));
+      float theta = atan2f(y, x);
+      float sqrt_r = sqrtf(r);
+      float omega = (orc_rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
+      float half_theta = theta / 2.0f + omega;
+      *ox = sqrt_r * cosf(half_theta);
+      *oy = sqrt_r * sinf(half_theta);
+      return;
+    }
+    case FFM_VAR_WAVES:
+      *ox = x + xf->b * sinf(y / (xf->e * xf->e + 1e-6f));
+      *oy = y + xf->d * sinf(x / (xf->f * xf->f + 1e-6f));
+      return;
+    case FFM_VAR_JULIAN: {
+      float power = p[0];
+      float cpower = p[2];
+      float r2 = len2(x, y);
+      float r = powf(r2, cpower);
+      float theta = ff_atan2(y, x);
+      float trunc_val = floorf(fabsf(power) * orc_rng_nextf(rng));
+      float t = (theta + F32(6.28318530718) * trunc_val) / power;
+      *ox = r * cosf(t);
+      *oy = r * sinf(t);
+      return;
+    }
+    case FFM_VAR_BUBBLE: {
+      float r2 = len2(x, y);
+      float r = r2 / 4.0f + 1.0f;
+      float scale = 1.0f / r;
+      *ox = scale * x;
+      *oy = scale * y;
+      return;
+    }
+    case FFM_VAR_CURL: {
+      float c1 = p[0], c2 = p[1];
+      float re = 1.0f + c1 * x + c2 * (x * x - y * y);
+      float im = c1 * y + 2.0f * c2 * x * y;
+      float r = 1.0f / (re * re + im * im);
+      *ox = (x * re + y * im) * r;
+      *oy = (y * re - x * im) * r;
+      return;
+    }
+    default:
+      *ox = 0.0f;
+      *oy = 0.0f;
+      return;
+  }
+}
+
+/* init_julian (src/variations/defs/advanced.rs:467-474):
+ * cpower = dist / max(|power|, 1e-30) / 2. */
+void orc_flame_prepare(ffm_flame_desc* f) {
+  for (int k = 0; k < f->n_vars; k++) {
+    if (f->var_id[k] != FFM_VAR_JULIAN) continue;
+    for (int i = 0; i < f->n_xforms + f->n_finals; i++) {
+      ffm_xform* xf = i < f->n_xforms ? &f->xforms[i] : &f->finals[i - f->n_xforms];
+      float power = xf->var_param[k][0], dist = xf->var_param[k][1];
+      xf->var_param[k][2] = dist / fmaxf(fabsf(power), 1e-30f) / 2.0f;
+    }
+  }
+}
+
+/* affine (shaders/core/affine.wgsl:4-19) -> variations in local-index
+ * order, weight-gated (src/shader_builder_v2.rs:2525-2806, normal phase
+ * `result += w * V(temp)`) -> post-affine if enabled
+ * (shaders/core/main_template.wgsl:355-384). */
+void orc_apply_xform(const ffm_flame_desc* f, const ffm_xform* xf, float* px, float* py,
+                     orc_rng* rng) {
+  float x = *px, y = *py;
+  float ax = xf->a * x + xf->b * y + xf->e;
+  float ay = xf->c * x + xf->d * y + xf->f;
+  float rx = 0.0f, ry = 0.0f;
+  for (int k = 0; k < f->n_vars; k++) {
+    float w = xf->var_weight[k];
+    if (w != 0.0f) {
+      float vx, vy;
+      orc_variation(f->var_id[k], xf->var_param[k], xf, ax, ay, rng, &vx, &vy);
+      rx += w * vx;
+      ry += w * vy;
+    }
+  }
+  if (xf->post_enabled) {
+    float qx = xf->post_a * rx + xf->post_b * ry + xf->post_e;
+    float qy = xf->post_c * rx + xf->post_d * ry + xf->post_f;
+    rx = qx;
+    ry = qy;
+  }
+  *px = rx;
+  *py = ry;
+}
+
+/* -------------------------------------------------------------- palette */
+
+/* src/scene/palette.rs:310-360 sample_color. */
+void orc_palette_sample(const float* s, int n, float t, float out[3]) {
+  if (n <= 0) {
+    out[0] = out[1] = out[2] = 1.0f;
+    return;
+  }
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  if (t < s[0]) {
+    out[0] = s[1];
+    out[1] = s[2];
+    out[2] = s[3];
+    return;
+  }
+  int idx = 0;
+  for (int i = 0; i < n; i++) {
+    if (s[4 * i] > t) break;
+    idx = i;
+  }
+  if (idx == n - 1) {
+    out[0] = s[4 * idx + 1];
+    out[1] = s[4 * idx + 2];
+    out[2] = s[4 * idx + 3];
+    return;
+  }
+  const float* a = s + 4 * idx;
+  const float* b = s + 4 * (idx + 1);
+  float range = b[0] - a[0];
+  if (range < 1e-6f) {
+    out[0] = a[1];
+    out[1] = a[2];
+    out[2] = a[3];
+    return;
+  }
+  float lt = (t - a[0]) / range;
+  for (int c = 0; c < 3; c++) out[c] = a[1 + c] + (b[1 + c] - a[1 + c]) * lt;
+}
+
+/* IEEE binary16, round to nearest even (half::f16::from_f32). */
+uint16_t orc_f32_to_f16(float v) {
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  uint32_t sign = (u >> 16) & 0x8000u;
+  uint32_t exp = (u >> 23) & 0xFFu;
+  uint32_t man = u & 0x7FFFFFu;
+  if (exp == 0xFFu) return (uint16_t)(sign | 0x7C00u | (man ? 0x200u : 0u));
+  int e = (int)exp - 127 + 15;
+  if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)sign;
+    man |= 0x800000u;
+    int shift = 14 - e;
+    uint32_t half = man >> shift;
+    uint32_t rem = man & ((1u << shift) - 1u);
+    uint32_t mid = 1u << (shift - 1);
+    if (rem > mid || (rem == mid && (half & 1u))) half++;
+    return (uint16_t)(sign | half);
+  }
+  uint32_t half = ((uint32_t)e << 10) | (man >> 13);
+  uint32_t rem = man & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (half & 1u))) half++;
+  return (uint16_t)(sign | half);
+}
+
+float orc_f16_to_f32(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  uint32_t exp = (h >> 10) & 0x1Fu;
+  uint32_t man = h & 0x3FFu;
+  uint32_t u;
+  if (exp == 0) {
+    if (man == 0) {
+      u = sign;
+    } else {
+      int e = -1;
+      do {
+        e++;
+        man <<= 1;
+      } while (!(man & 0x400u));
+      u = sign | ((uint32_t)(127 - 15 - e) << 23) | ((man & 0x3FFu) << 13);
+    }
+  } else if (exp == 31) {
+    u = sign | 0x7F800000u | (man << 13);
+  } else {
+    u = sign | ((exp + 127 - 15) << 23) | (man << 13);
+  }
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+/* Texel table: t = i/(size-1) (src/scene/palette.rs:265-278), clamp to
+ * [0,1] and f16 (src/gpu/buffers.rs:3215-3221), then srgb_to_linear
+ * pow(max(c,0), 2.2) (shaders/core/utilities.wgsl:143-146). */
+void orc_palette_linear(const float* stops, int n, int size, float* lin) {
+  for (int i = 0; i < size; i++) {
+    float t = (float)i / (float)(size - 1);
+    float c[3];
+    orc_palette_sample(stops, n, t, c);
+    for (int k = 0; k < 3; k++) {
+      float v = c[k] < 0.0f ? 0.0f : (c[k] > 1.0f ? 1.0f : c[k]);
+      float h = orc_f16_to_f32(orc_f32_to_f16(v));
+      lin[3 * i + k] = powf(fmaxf(h, 0.0f), F32(2.2));
+    }
+  }
+}
+
+/* Nearest texel of a clamp-to-edge sampler at u = colour index. */
+int orc_palette_texel(float ci, int size) {
+  float u = ci * (float)size;
+  if (!(u >= 0.0f)) return 0;
+  if (u >= (float)size) return size - 1;
+  int i = (int)floorf(u);
+  return i > size - 1 ? size - 1 : i;
+}
+
+/* ----------------------------------------------------------------- view */
+
+/* shaders/core/utilities.wgsl:417-445 (2D, not cylinder-relative):
+ * pan, rotate, zoom, scale by min(W,H)/4 about the centre, floor. */
+void orc_world_to_pixel(const ffm_view* v, float x, float y, int32_t* px, int32_t* py) {
+  float tx = x - v->pan_x, ty = y - v->pan_y;
+  float cr = cosf(v->rotation), sr = sinf(v->rotation);
+  float rx = tx * cr - ty * sr;
+  float ry = tx * sr + ty * cr;
+  rx = rx * v->zoom;
+  ry = ry * v->zoom;
+  float scale = (float)(v->width < v->height ? v->width : v->height) * 0.25f;
+  float cx = (float)v->width * 0.5f, cy = (float)v->height * 0.5f;
+  float pxf = cx + rx * scale;
+  float pyf = cy + ry * scale;
+  float fx = floorf(pxf), fy = floorf(pyf);
+  /* i32(f32) saturates on GPUs (WGSL); far points are off-frame either way */
+  *px = fx >= 2147483520.0f ? INT32_MAX : (fx <= -2147483648.0f ? INT32_MIN : (fx != fx ? 0 : (int32_t)fx));
+  *py = fy >= 2147483520.0f ? INT32_MAX : (fy <= -2147483648.0f ? INT32_MIN : (fy != fy ? 0 : (int32_t)fy));
+}
+
+/* ----------------------------------------------------------- chaos game */
+
+typedef struct {
+  const ffm_flame_desc* f;
+  const float* pal;
+  int pal_size;
+  const ffm_view* v;
+  uint32_t burn_in;
+} step_ctx;
+
+enum { ST_PLOT = 1, ST_RESPAWN = 2, ST_BURN = 4 };
+
+/* One iteration of the loop body, shaders/core/main_template.wgsl
+ * :193-1383 in its 2D / palette colour / direct-histogram form
+ * (tests/shader_dumps/in-app-2d.wgsl:1163-1382). Returns ST_* flags;
+ * *pix gets the pixel index when plotted in frame and *rgb the splat. */
+static int step(const step_ctx* c, orc_walker* w, orc_rng* rng, int32_t* ppx, int32_t* ppy,
+                int* xf_out, uint32_t rgb[3]) {
+  const ffm_flame_desc* f = c->f;
+  float rand_val = orc_rng_nextf(rng);
+  int xi = orc_select(f, rand_val);
+  *xf_out = xi;
+  const ffm_xform* xf = &f->xforms[xi];
+  int should_plot = orc_rng_nextf(rng) < xf->opacity;
+  float x = w->x, y = w->y;
+  orc_apply_xform(f, xf, &x, &y, rng);
+  /* bad-value respawn (main_template.wgsl:397-479) */
+  if (!(fabsf(x) <= 1e32f) || !(fabsf(y) <= 1e32f)) {
+    w->x = orc_rng_nextf(rng) * 2.0f - 1.0f;
+    w->y = orc_rng_nextf(rng) * 2.0f - 1.0f;
+    w->fuse = c->burn_in;
+    *ppx = *ppy = -1;
+    return ST_RESPAWN;
+  }
+  w->x = x;
+  w->y = y;
+  /* colour-index blend (main_template.wgsl:496-503) */
+  float sym = xf->color_speed;
+  float c1 = (1.0f + sym) / 2.0f;
+  float c2 = xf->color * (1.0f - sym) / 2.0f;
+  w->color_index = w->color_index * c1 + c2;
+  if (w->fuse > 0u) {
+    w->fuse -= 1u;
+    *ppx = *ppy = -1;
+    return ST_BURN;
+  }
+  /* final chain (main_template.wgsl:663-717): reshapes the plot only */
+  float fx = x, fy = y;
+  for (int i = 0; i < f->n_finals; i++) orc_apply_xform(f, &f->finals[i], &fx, &fy, rng);
+  int32_t px, py;
+  orc_world_to_pixel(c->v, fx, fy, &px, &py);
+  *ppx = px;
+  *ppy = py;
+  if (px >= 0 && px < c->v->width && py >= 0 && py < c->v->height && should_plot) {
+    int t = orc_palette_texel(w->color_index, c->pal_size);
+    for (int k = 0; k < 3; k++) {
+      float v = c->pal[3 * t + k];
+      v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+      rgb[k] = (uint32_t)(v * 100.0f); /* main_template.wgsl:1273-1320 */
+    }
+    return ST_PLOT;
+  }
+  return 0;
+}
+
+static void walker_start(orc_walker* w, orc_rng* rng, uint32_t burn_in) {
+  /* main_template.wgsl:25-56: two draws for the start point, always */
+  float x = orc_rng_nextf(rng) * 2.0f - 1.0f;
+  float y = orc_rng_nextf(rng) * 2.0f - 1.0f;
+  w->x = x;
+  w->y = y;
+  w->color_index = 0.0f;
+  w->fuse = burn_in;
+}
+
+uint64_t orc_iterate(const ffm_flame_desc* f, const float* pal, int pal_size, const ffm_view* v,
+                     uint32_t walker0, uint32_t n, uint32_t ipt, uint32_t burn_in,
+                     uint32_t seed, orc_walker* orbits, int fresh, uint32_t* hist,
+                     uint64_t* inframe) {
+  step_ctx c = {f, pal, pal_size, v, burn_in};
+  uint64_t plotted = 0, in = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t tid = walker0 + i;
+    orc_rng rng;
+    orc_rng_init(&rng, tid, seed);
+    orc_walker w;
+    walker_start(&w, &rng, burn_in);
+    if (orbits && !fresh) w = orbits[i];
+    for (uint32_t it = 0; it < ipt; it++) {
+      int32_t px, py;
+      int xi;
+      uint32_t rgb[3];
+      int st = step(&c, &w, &rng, &px, &py, &xi, rgb);
+      if (!(st & (ST_RESPAWN | ST_BURN))) plotted++;
+      if (st & ST_PLOT) {
+        in++;
+        if (hist) {
+          uint32_t* h = hist + 4 * ((size_t)py * (size_t)v->width + (size_t)px);
+          h[0] += rgb[0];
+          h[1] += rgb[1];
+          h[2] += rgb[2];
+          h[3] += 100u;
+        }
+      }
+    }
+    if (orbits) orbits[i] = w;
+  }
+  if (inframe) *inframe = in;
+  return plotted;
+}
+
+typedef struct {
+  const ffm_flame_desc* f;
+  const float* pal;
+  int pal_size;
+  const ffm_view* v;
+  uint32_t w0, n, ipt, burn_in, seed;
+  uint32_t* hist;
+  uint64_t plotted;
+} mt_job;
+
+static void* mt_run(void* p) {
+  mt_job* j = (mt_job*)p;
+  j->plotted = orc_iterate(j->f, j->pal, j->pal_size, j->v, j->w0, j->n, j->ipt, j->burn_in,
+                           j->seed, NULL, 1, j->hist, NULL);
+  return NULL;
+}
+
+uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
+                        const ffm_view* v, uint32_t n, uint32_t ipt, uint32_t burn_in,
+                        uint32_t seed, uint32_t* hist, int threads) {
+  if (threads < 1) threads = 1;
+  size_t words = (size_t)v->width * (size_t)v->height * 4;
+  mt_job* jobs = (mt_job*)calloc((size_t)threads, sizeof(mt_job));
+  pthread_t* th = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  uint32_t per = n / (uint32_t)threads, rem = n % (uint32_t)threads, w0 = 0;
+  for (int t = 0; t < threads; t++) {
+    uint32_t cnt = per + ((uint32_t)t < rem ? 1u : 0u);
+    jobs[t] = (mt_job){f, pal, pal_size, v, w0, cnt, ipt, burn_in, seed, NULL, 0};
+    jobs[t].hist = t == 0 ? hist : (uint32_t*)calloc(words, 4);
+    w0 += cnt;
+    pthread_create(&th[t], NULL, mt_run, &jobs[t]);
+  }
+  uint64_t plotted = 0;
+  for (int t = 0; t < threads; t++) {
+    pthread_join(th[t], NULL);
+    plotted += jobs[t].plotted;
+  }
+  for (int t = 1; t < threads; t++) {
+    for (size_t i = 0; i < words; i++) hist[i] += jobs[t].hist[i];
+    free(jobs[t].hist);
+  }
+  free(jobs);
+  free(th);
+  return plotted;
+}
+
+void orc_trace(const ffm_flame_desc* f, const float* pal, int pal_size, const ffm_view* v,
+               uint32_t walker, uint32_t seed, uint32_t burn_in, uint32_t steps, uint32_t* out) {
+  step_ctx c = {f, pal, pal_size, v, burn_in};
+  orc_rng rng;
+  orc_rng_init(&rng, walker, seed);
+  orc_walker w;
+  walker_start(&w, &rng, burn_in);
+  for (uint32_t s = 0; s < steps; s++) {
+    int32_t px, py;
+    int xi;
+    uint32_t rgb[3];
+    int st = step(&c, &w, &rng, &px, &py, &xi, rgb);
+    uint32_t* o = out + 8 * (size_t)s;
+    o[0] = rng.state;
+    o[1] = (uint32_t)xi;
+    memcpy(&o[2], &w.x, 4);
+    memcpy(&o[3], &w.y, 4);
+    memcpy(&o[4], &w.color_index, 4);
+    o[5] = (uint32_t)px;
+    o[6] = (uint32_t)py;
+    o[7] = (uint32_t)st;
+  }
+}
+
+/* ---------------------------------------------------------- OrbitFuses */
+
+/* src/renderer/compute_kernel.rs:73-102 */
+uint64_t orc_orbit_fuses_dispatch(orc_orbit_fuses* c, uint32_t threads, uint32_t ipt,
+                                  uint32_t burn_in, uint32_t period, uint32_t seed) {
+  if (c->len < threads) {
+    c->left = (uint32_t*)realloc(c->left, (size_t)threads * 4);
+    for (uint32_t i = c->len; i < threads; i++) c->left[i] = burn_in;
+    c->len = threads;
+  }
+  uint64_t plotted = 0;
+  for (uint32_t tid = 0; tid < threads; tid++) {
+    if (period != 0 && (tid + seed) % period == 0) c->left[tid] = burn_in;
+    uint32_t burnt = c->left[tid] < ipt ? c->left[tid] : ipt;
+    plotted += (uint64_t)(ipt - burnt);
+    c->left[tid] -= burnt;
+  }
+  return plotted;
+}
+
+void orc_orbit_fuses_free(orc_orbit_fuses* c) {
+  free(c->left);
+  c->left = NULL;
+  c->len = 0;
+}
+
+/* ------------------------------------------------------------- stencils */
+
+/* shaders/histogram_blur.wgsl:39-119, one pass. `center_src` is the buffer
+ * the centre density comes from (the original primary in both passes). */
+void orc_blur_pass(const uint32_t* in, uint32_t* out, const uint32_t* center_src, int W, int H,
+                   float radius_pixels, float density_sigma, int direction) {
+  const int KMAX = 8;
+  float sigma = fmaxf(radius_pixels, 0.0001f);
+  int kh = (int)ceilf(3.0f * sigma);
+  if (kh > KMAX) kh = KMAX;
+  float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
+  float sigma_d = fmaxf(density_sigma, 1.0f);
+  float inv_2sd2 = 1.0f / (2.0f * sigma_d * sigma_d);
+  for (int y = 0; y < H; y++) {
+    for (int x = 0; x < W; x++) {
+      size_t ci = ((size_t)y * W + x) * 4;
+      float cd = (float)center_src[ci + 3];
+      float sr = 0.0f, sg = 0.0f, sb = 0.0f, sd = 0.0f, wt = 0.0f;
+      for (int k = -KMAX; k <= KMAX; k++) {
+        if (abs(k) > kh) continue;
+        int nx = x, ny = y;
+        if (direction == 0) {
+          nx = x + k < 0 ? 0 : (x + k > W - 1 ? W - 1 : x + k);
+        } else {
+          ny = y + k < 0 ? 0 : (y + k > H - 1 ? H - 1 : y + k);
+        }
+        size_t ni = ((size_t)ny * W + nx) * 4;
+        float tr = (float)in[ni], tg = (float)in[ni + 1], tb = (float)in[ni + 2],
+              td = (float)in[ni + 3];
+        float fk = (float)k;
+        float w_sp = expf(-fk * fk * inv_2s2);
+        float delta = td - cd;
+        float w_d = expf(-delta * delta * inv_2sd2);
+        float w = w_sp * w_d;
+        sr = sr + tr * w;
+        sg = sg + tg * w;
+        sb = sb + tb * w;
+        sd = sd + td * w;
+        wt = wt + w;
+      }
+      float inv_w = 1.0f / fmaxf(wt, 1e-6f);
+      out[ci] = (uint32_t)(sr * inv_w);
+      out[ci + 1] = (uint32_t)(sg * inv_w);
+      out[ci + 2] = (uint32_t)(sb * inv_w);
+      out[ci + 3] = (uint32_t)(sd * inv_w);
+    }
+  }
+}
+
+/* H pass primary -> scratch, V pass scratch -> primary with the centre
+ * density read from the primary before it is written
+ * (histogram_blur.wgsl:57-70): the V pass writes a copy, then swaps. */
+void orc_blur(uint32_t* hist, uint32_t* scratch, int W, int H, float sigma, float density_sigma) {
+  size_t words = (size_t)W * H * 4;
+  orc_blur_pass(hist, scratch, hist, W, H, sigma, density_sigma, 0);
+  uint32_t* tmp = (uint32_t*)malloc(words * 4);
+  orc_blur_pass(scratch, tmp, hist, W, H, sigma, density_sigma, 1);
+  memcpy(hist, tmp, words * 4);
+  free(tmp);
+}
+
+/* src/renderer/compute_kernel.rs:1688-1691:
+ * sigma_d = (samples/pixels)*100 * 100^(2*edges-1). */
+float orc_blur_density_sigma(double samples, int W, int H, float blur_edges) {
+  float pixel_count = (float)W * (float)H;
+  float mean_scaled = ((float)samples / pixel_count) * 100.0f;
+  return mean_scaled * powf(100.0f, 2.0f * blur_edges - 1.0f);
+}
+
+/* shaders/accumulate.wgsl:115-230 (no depth). mode 0 cumulative, 1 EMA,
+ * 2 overwrite; count != NULL is the exact 64-bit count. */
+void orc_accumulate(const void* hist_v, int hist_is_f32, const float* prev_buf, float* out,
+                    uint32_t* count, int W, int H, int mode, float blend) {
+  const float SCALE = 100.0f;
+  for (size_t p = 0; p < (size_t)W * H; p++) {
+    float prev[4];
+    for (int k = 0; k < 4; k++) prev[k] = mode == 2 ? 0.0f : prev_buf[4 * p + k];
+    float h[4];
+    uint32_t hd = 0;
+    if (hist_is_f32) {
+      const float* hf = (const float*)hist_v;
+      for (int k = 0; k < 4; k++) h[k] = hf[4 * p + k];
+      hd = (uint32_t)h[3];
+    } else {
+      const uint32_t* hu = (const uint32_t*)hist_v;
+      for (int k = 0; k < 4; k++) h[k] = (float)hu[4 * p + k];
+      hd = hu[4 * p + 3];
+    }
+    float new_density = h[3] / SCALE;
+    float total = prev[3] + new_density;
+    if (count) {
+      uint32_t lo = count[2 * p], hi = count[2 * p + 1];
+      if (prev[3] == 0.0f) {
+        lo = hi = 0;
+      } else if (lo == 0 && hi == 0) {
+        float seed = prev[3] * SCALE;
+        float fhi = floorf(seed / 4294967296.0f);
+        lo = (uint32_t)(seed - fhi * 4294967296.0f);
+        hi = (uint32_t)fhi;
+      }
+      uint32_t nlo = lo + hd;
+      hi = hi + (nlo < lo ? 1u : 0u);
+      lo = nlo;
+      count[2 * p] = lo;
+      count[2 * p + 1] = hi;
+      total = ((float)hi * 4294967296.0f + (float)lo) / SCALE;
+      prev[3] = total;
+    }
+    if (new_density <= 0.0f) {
+      for (int k = 0; k < 4; k++) out[4 * p + k] = prev[k];
+      continue;
+    }
+    float dn = fmaxf(h[3], 1.0f);
+    float nc[3];
+    for (int k = 0; k < 3; k++) {
+      float v = h[k] / dn;
+      nc[k] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    }
+    float cum_alpha = new_density / total;
+    float alpha = mode == 1 ? blend : cum_alpha;
+    for (int k = 0; k < 3; k++) out[4 * p + k] = prev[k] + (nc[k] - prev[k]) * alpha;
+    out[4 * p + 3] = total;
+  }
+}
+
+/* ------------------------------------------------- density estimation */
+
+/* flam3's estimator as Draves & Reckase describe it ("The Fractal Flame
+ * Algorithm", 2003/2008, section on density estimation) and flam3 builds
+ * it: a table of Gaussian discs, filter i of half-width
+ * h_i = (max_r+1) / (i+1)^curve (indices past 100 spaced by
+ * (i-100)^(1/curve)), floored at min_r+1; a pixel of count c uses
+ * filter ceil(c)-1 up to 100, then 100 + floor((c-100)^curve). Tap (dx,dy)
+ * of filter h weighs exp(-2 (1.5 d/h)^2) for d = |(dx,dy)| <= h, normalised
+ * over the disc; each pixel scatters its colour sums and count through its
+ * filter. The reference has no estimator, so nothing here is pinned by it. */
+typedef struct {
+  int n;
+  double* h;
+  double* inv_norm;
+} de_table;
+
+static void de_build(de_table* t, float max_r, float min_r, float curve) {
+  double comp_max = (double)max_r + 1.0, comp_min = (double)min_r + 1.0;
+  int cap = 4096;
+  t->h = (double*)malloc(sizeof(double) * cap);
+  t->inv_norm = (double*)malloc(sizeof(double) * cap);
+  t->n = 0;
+  const int keep = 100;
+  for (int i = 0; i < cap; i++) {
+    double adj = i < keep ? (double)i : pow((double)(i - keep), 1.0 / curve) + keep;
+    double h = comp_max / pow(adj + 1.0, curve);
+    int last = 0;
+    if (h <= comp_min) {
+      h = comp_min;
+      last = 1;
+    }
+    int R = (int)ceil(h);
+    double sum = 0.0;
+    for (int dy = -R; dy <= R; dy++)
+      for (int dx = -R; dx <= R; dx++) {
+        double d = sqrt((double)(dx * dx + dy * dy)) / h;
+        if (d <= 1.0) sum += exp(-2.0 * (1.5 * d) * (1.5 * d));
+      }
+    t->h[t->n] = h;
+    t->inv_norm[t->n] = 1.0 / sum;
+    t->n++;
+    if (last) break;
+  }
+}
+
+static int de_index(const de_table* t, double c, float curve) {
+  int idx;
+  if (c <= 100.0)
+    idx = (int)ceil(c) - 1;
+  else
+    idx = 100 + (int)floor(pow(c - 100.0, curve));
+  if (idx < 0) idx = 0;
+  if (idx > t->n - 1) idx = t->n - 1;
+  return idx;
+}
+
+void orc_density_estimate(const float* in, float* out, int W, int H, float max_r, float min_r,
+                          float curve) {
+  de_table t;
+  de_build(&t, max_r, min_r, curve);
+  size_t np = (size_t)W * H;
+  double* acc = (double*)calloc(np * 4, sizeof(double));
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      size_t p = (size_t)y * W + x;
+      double c = in[4 * p + 3];
+      if (!(c > 0.0)) continue;
+      int idx = de_index(&t, c, curve);
+      double h = t.h[idx], inv = t.inv_norm[idx];
+      int R = (int)ceil(h);
+      double v[4] = {in[4 * p] * c, in[4 * p + 1] * c, in[4 * p + 2] * c, c};
+      for (int dy = -R; dy <= R; dy++) {
+        int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;
+        for (int dx = -R; dx <= R; dx++) {
+          int xx = x + dx;
+          if (xx < 0 || xx >= W) continue;
+          double d = sqrt((double)(dx * dx + dy * dy)) / h;
+          if (d > 1.0) continue;
+          double wgt = exp(-2.0 * (1.5 * d) * (1.5 * d)) * inv;
+          size_t q = (size_t)yy * W + xx;
+          for (int k = 0; k < 4; k++) acc[4 * q + k] += v[k] * wgt;
+        }
+      }
+    }
+  for (size_t p = 0; p < np; p++) {
+    double c = acc[4 * p + 3];
+    if (c > 0.0) {
+      for (int k = 0; k < 3; k++) out[4 * p + k] = (float)(acc[4 * p + k] / c);
+    } else {
+      for (int k = 0; k < 3; k++) out[4 * p + k] = 0.0f;
+    }
+    out[4 * p + 3] = (float)c;
+  }
+  free(acc);
+  free(t.h);
+  free(t.inv_norm);
+}
+
+/* -------------------------------------------------------------- tonemap */
+
+void orc_tonemap_defaults(ffm_tonemap_params* tp) {
+  /* src/config/defaults.rs:19-47,100-101 */
+  memset(tp, 0, sizeof(*tp));
+  tp->tonemap_mode = 1;
+  tp->highlight_mode = 0;
+  tp->exposure = 1.0f;
+  tp->gamma = 4.0f;
+  tp->gamma_threshold = 5.0f;
+  tp->brightness = 4.0f;
+  tp->vibrancy = 1.0f;
+  tp->white_level = 200.0f;
+  tp->density_scale = 1.0f;
+  tp->saturation = 1.0f;
+  tp->hue_shift = 0.0f;
+  tp->alpha_blend_low = 0.3f;
+  tp->alpha_blend_high = 0.8f;
+  tp->levels_enabled = 1;
+  tp->levels_low = 0.0f;
+  tp->levels_high = 10.0f;
+  tp->levels_gamma = 0.5f;
+}
+
+static float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* shaders/tonemap.wgsl:153-211 */
+static void rgb_to_hsv(const float c[3], float o[3]) {
+  float r = c[0], g = c[1], b = c[2];
+  float mx = fmaxf(fmaxf(r, g), b), mn = fminf(fminf(r, g), b);
+  float delta = mx - mn;
+  float h = 0.0f, s = 0.0f;
+  if (delta > 0.00001f) {
+    s = delta / mx;
+    if (r >= mx)
+      h = (g - b) / delta;
+    else if (g >= mx)
+      h = 2.0f + (b - r) / delta;
+    else
+      h = 4.0f + (r - g) / delta;
+    h = h * 60.0f;
+    if (h < 0.0f) h = h + 360.0f;
+  }
+  o[0] = h;
+  o[1] = s;
+  o[2] = mx;
+}
+
+static void hsv_to_rgb(const float hsv[3], float o[3]) {
+  float h = hsv[0], s = hsv[1], v = hsv[2];
+  if (s <= 0.0f) {
+    o[0] = o[1] = o[2] = v;
+    return;
+  }
+  float hh = h;
+  if (hh >= 360.0f) hh = 0.0f;
+  hh = hh / 60.0f;
+  uint32_t i = (uint32_t)hh;
+  float ff = hh - (float)i;
+  float p = v * (1.0f - s);
+  float q = v * (1.0f - (s * ff));
+  float t = v * (1.0f - (s * (1.0f - ff)));
+  switch (i) {
+    case 0: o[0] = v; o[1] = t; o[2] = p; break;
+    case 1: o[0] = q; o[1] = v; o[2] = p; break;
+    case 2: o[0] = p; o[1] = v; o[2] = t; break;
+    case 3: o[0] = p; o[1] = q; o[2] = v; break;
+    case 4: o[0] = t; o[1] = p; o[2] = v; break;
+    default: o[0] = v; o[1] = p; o[2] = q; break;
+  }
+}
+
+static uint8_t unorm8(float v) {
+  v = v != v ? 0.0f : clampf(v, 0.0f, 1.0f);
+  return (uint8_t)floorf(v * 255.0f + 0.5f);
+}
+
+/* shaders/tonemap.wgsl:224-590 fs_main, with TonemapParams as
+ * update_tonemap fills them (src/renderer/compute_kernel.rs:3813-3918). */
+void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
+                 double samples_in_buffer, const ffm_tonemap_params* tp) {
+  const float BRIGHT_ADJUST = 2.3f, PREFILTER_WHITE = 67108864.0f;
+  float apo_zoom = log2f(zoom);
+  float ppu = (float)(W < H ? W : H) * 0.25f * powf(2.0f, apo_zoom);
+  float area = (float)((uint32_t)W * (uint32_t)H) / (ppu * ppu);
+  float total_pixels = fmaxf((float)W * (float)H, 1.0f);
+  float sample_density = fmaxf((float)samples_in_buffer / total_pixels, 1e-6f);
+  float white = tp->white_level;
+  float k1 = 1.0f * BRIGHT_ADJUST * tp->brightness * 268.0f * PREFILTER_WHITE / 256.0f;
+  float k2 = 1.0f / (1.0f * area * white * sample_density);
+  float gamma = tp->gamma == 0.0f ? tp->gamma : 1.0f / tp->gamma;
+  float funcval = tp->gamma_threshold != 0.0f ? powf(tp->gamma_threshold, gamma - 1.0f) : 0.0f;
+  float vib = rintf(tp->vibrancy * 256.0f);
+  float notvib = 256.0f - vib;
+  float bg_lin[3];
+  for (int k = 0; k < 3; k++) bg_lin[k] = powf(fmaxf(tp->background[k], 0.0f), F32(2.2));
+  const float inv22 = F32(1.0 / 2.2);
+  for (size_t p = 0; p < (size_t)W * H; p++) {
+    const float* a = accum + 4 * p;
+    float bucket_count = a[3] * 100.0f;
+    float color[3];
+    float alpha = 0.0f;
+    if (tp->tonemap_mode == 0) {
+      float g = tp->gamma == 0.0f ? tp->gamma : 1.0f / tp->gamma;
+      for (int k = 0; k < 3; k++) color[k] = powf(a[k] * tp->exposure, g);
+      alpha = clampf(bucket_count * 0.01f * tp->density_scale, 0.0f, 1.0f);
+    } else if (tp->tonemap_mode == 2) {
+      float nd = clampf(bucket_count * 0.01f * tp->exposure, 0.0f, 1.0f);
+      float g = tp->gamma == 0.0f ? tp->gamma : 1.0f / tp->gamma;
+      float gd = powf(nd, g);
+      color[0] = color[1] = color[2] = gd;
+      alpha = nd;
+    } else {
+      float bs;
+      if (bucket_count < 0.001f) {
+        bs = 0.0f;
+      } else {
+        float l10 = logf(1.0f + white * bucket_count * k2) / logf(10.0f);
+        bs = (k1 * l10) / (white * bucket_count);
+      }
+      float ls = bs / PREFILTER_WHITE;
+      float fp0 = ls * (a[0] * bucket_count);
+      float fp1 = ls * (a[1] * bucket_count);
+      float fp2 = ls * (a[2] * bucket_count);
+      float fp3 = ls * bucket_count * white;
+      if (fp3 > 0.0f) {
+        if (fp3 <= tp->gamma_threshold) {
+          float frac = fp3 / tp->gamma_threshold;
+          alpha = (1.0f - frac) * fp3 * funcval + frac * powf(fp3, gamma);
+        } else {
+          alpha = powf(fp3, gamma);
+        }
+      }
+      ls = fp3 > 0.0f ? vib * alpha / fp3 : 0.0f;
+      if (notvib > 0.0f) {
+        color[0] = ls * fp0 + notvib * powf(fp0, gamma);
+        color[1] = ls * fp1 + notvib * powf(fp1, gamma);
+        color[2] = ls * fp2 + notvib * powf(fp2, gamma);
+      } else {
+        color[0] = ls * fp0;
+        color[1] = ls * fp1;
+        color[2] = ls * fp2;
+      }
+    }
+    if (tp->saturation != 1.0f || tp->hue_shift != 0.0f) {
+      float hsv[3];
+      rgb_to_hsv(color, hsv);
+      if (tp->hue_shift != 0.0f) {
+        hsv[0] = hsv[0] + tp->hue_shift;
+        if (hsv[0] < 0.0f)
+          hsv[0] = hsv[0] + 360.0f;
+        else if (hsv[0] >= 360.0f)
+          hsv[0] = hsv[0] - 360.0f;
+      }
+      if (tp->saturation != 1.0f) hsv[1] = clampf(hsv[1] * tp->saturation, 0.0f, 1.0f);
+      hsv_to_rgb(hsv, color);
+    }
+    if (tp->tonemap_mode == 1)
+      for (int k = 0; k < 3; k++) color[k] = color[k] * tp->exposure;
+    for (int k = 0; k < 3; k++) color[k] = fmaxf(color[k], 0.0f);
+    if (tp->highlight_mode == 1) {
+      float m = fmaxf(color[0], fmaxf(color[1], color[2]));
+      if (m > 1.0f)
+        for (int k = 0; k < 3; k++) color[k] = color[k] / m;
+    } else if (tp->highlight_mode == 2) {
+      float lum = color[0] * 0.2126f + color[1] * 0.7152f + color[2] * 0.0722f;
+      if (lum > 1e-6f) {
+        float lm = lum / (1.0f + lum);
+        for (int k = 0; k < 3; k++) color[k] = color[k] * (lm / lum);
+      }
+      for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
+    } else if (tp->highlight_mode == 3) {
+      for (int k = 0; k < 3; k++) {
+        float c = color[k];
+        c = (c * (2.51f * c + 0.03f)) / (c * (2.43f * c + 0.59f) + 0.14f);
+        color[k] = clampf(c, 0.0f, 1.0f);
+      }
+    } else {
+      for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
+    }
+    /* apply_levels (tonemap.wgsl:125-151) on accum.a, mean = sample_density */
+    float leveled = 1.0f;
+    {
+      float low = tp->levels_low, high = tp->levels_high, lg = tp->levels_gamma;
+      float mean = sample_density;
+      if (mean > 0.0f) {
+        float nd = a[3] / mean;
+        if (high <= low) {
+          leveled = nd > low ? 1.0f : 0.0f;
+        } else {
+          float nrm = (nd - low) / (high - low);
+          float cl = clampf(nrm, 0.0f, 1.0f);
+          leveled = (lg != 1.0f && lg > 0.0f) ? powf(cl, lg) : cl;
+        }
+      }
+    }
+    float linear_alpha = clampf(bucket_count * 0.01f * tp->density_scale, 0.0f, 1.0f);
+    float gamma_alpha = clampf(alpha, 0.0f, 1.0f);
+    float t = clampf((gamma_alpha - tp->alpha_blend_low) / (tp->alpha_blend_high - tp->alpha_blend_low),
+                     0.0f, 1.0f);
+    float blend_t = t * t * (3.0f - 2.0f * t);
+    float base_alpha = gamma_alpha * (1.0f - blend_t) + linear_alpha * blend_t;
+    float fa = tp->levels_enabled ? fminf(base_alpha, leveled) : base_alpha;
+    for (int k = 0; k < 3; k++) {
+      float fin = bg_lin[k] * (1.0f - fa) + color[k] * fa;
+      rgba[4 * p + k] = unorm8(powf(fin, inv22));
+    }
+    rgba[4 * p + 3] = 255;
+  }
+}
+
+/* ---------------------------------------------------------- the render */
+
+/* render_with's loop (src/renderer/render.rs:624-710): dispatch, count
+ * with OrbitFuses (compute_kernel.rs:1402-1431), accumulate every
+ * batch_size dispatches and once more for a partial batch, stop at the
+ * target or after burn_in/ipt+4 empty dispatches; then tonemap. */
+int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int pal_size,
+               const ffm_job* job, float* accum_out, uint8_t* rgba_out, uint64_t* total_out,
+               int threads) {
+  ffm_flame_desc* f = (ffm_flame_desc*)malloc(sizeof(ffm_flame_desc));
+  memcpy(f, f_in, sizeof(*f));
+  orc_flame_prepare(f);
+  const ffm_view* v = &job->view;
+  int W = v->width, H = v->height;
+  size_t np = (size_t)W * H;
+  float* pal = (float*)malloc(sizeof(float) * 3 * (size_t)pal_size);
+  orc_palette_linear(stops, n_stops, pal_size, pal);
+  uint32_t* hist = (uint32_t*)calloc(np * 4, 4);
+  uint32_t* scratch = (uint32_t*)calloc(np * 4, 4);
+  float* acc = (float*)calloc(np * 4, 4);
+  float* acc2 = (float*)calloc(np * 4, 4);
+  uint32_t nw = (uint32_t)job->walkers;
+  orc_walker* orbits = job->persistent_orbits ? (orc_walker*)calloc(nw, sizeof(orc_walker)) : NULL;
+  orc_orbit_fuses fuses = {NULL, 0};
+  uint64_t total = 0, batch_samples = 0, samples_in_buffer = 0;
+  uint32_t batch_count = 0, empty = 0, k = 0;
+  uint32_t stall_limit = job->burn_in / (job->ipt ? job->ipt : 1) + 4;
+  uint32_t bs = job->batch_size ? job->batch_size : 4;
+  for (;;) {
+    uint32_t seed = job->seed_base + k * job->seed_stride;
+    if (batch_count == 0) memset(hist, 0, np * 16);
+    if (threads > 1 && !orbits) {
+      uint32_t* h2 = (uint32_t*)calloc(np * 4, 4);
+      orc_iterate_mt(f, pal, pal_size, v, nw, job->ipt, job->burn_in, seed, h2, threads);
+      for (size_t i = 0; i < np * 4; i++) hist[i] += h2[i];
+      free(h2);
+    } else {
+      orc_iterate(f, pal, pal_size, v, 0, nw, job->ipt, job->burn_in, seed, orbits, k == 0,
+                  hist, NULL);
+    }
+    uint64_t plotted;
+    if (orbits) {
+      plotted = orc_orbit_fuses_dispatch(&fuses, nw, job->ipt, job->burn_in, 0, seed);
+    } else {
+      uint32_t per = job->ipt > job->burn_in ? job->ipt - job->burn_in : 0;
+      plotted = (uint64_t)nw * per;
+    }
+    k++;
+    total += plotted;
+    batch_samples += plotted;
+    batch_count++;
+    empty = plotted == 0 ? empty + 1 : 0;
+    int stalled = empty >= stall_limit;
+    int done = total >= job->target_samples || stalled;
+    if (batch_count >= bs || done) {
+      if (job->filter_radius > 0.0f) {
+        float dsig = orc_blur_density_sigma((double)batch_samples, W, H, job->filter_blur_edges);
+        orc_blur(hist, scratch, W, H, job->filter_radius, dsig);
+      }
+      orc_accumulate(hist, 0, acc, acc2, NULL, W, H, 0, 0.0f);
+      float* t = acc;
+      acc = acc2;
+      acc2 = t;
+      samples_in_buffer += batch_samples;
+      batch_count = 0;
+      batch_samples = 0;
+    }
+    if (done) break;
+  }
+  if (job->de_max_radius > 0.0f) {
+    orc_density_estimate(acc, acc2, W, H, job->de_max_radius, job->de_min_radius, job->de_curve);
+    float* t = acc;
+    acc = acc2;
+    acc2 = t;
+  }
+  if (accum_out) memcpy(accum_out, acc, np * 16);
+  if (rgba_out) orc_tonemap(acc, rgba_out, W, H, v->zoom, (double)samples_in_buffer, &job->tonemap);
+  if (total_out) *total_out = total;
+  orc_orbit_fuses_free(&fuses);
+  free(orbits);
+  free(hist);
+  free(scratch);
+  free(acc);
+  free(acc2);
+  free(pal);
+  free(f);
+  return 0;
+}
+
+/* ------------------------------------------------------------ soft splat */
+
+/* FP64 forward of the differentiable mode: the same walk (selection,
+ * opacity draw, transform, respawn, fuse, final chain) in double, with
+ * each plotted in-frame sample spread bilinearly over the four pixels
+ * around its continuous pixel position (x_pix - 0.5, y_pix - 0.5), weight 1
+ * per sample, colour ignored. */
+static void apply_xform_d(const ffm_flame_desc* f, const ffm_xform* xf, double* px, double* py,
+                          orc_rng* rng) {
+  double x = *px, y = *py;
+  double ax = (double)xf->a * x + (double)xf->b * y + (double)xf->e;
+  double ay = (double)xf->c * x + (double)xf->d * y + (double)xf->f;
+  double rx = 0.0, ry = 0.0;
+  for (int k = 0; k < f->n_vars; k++) {
+    double w = xf->var_weight[k];
+    if (w == 0.0) continue;
+    double vx = 0.0, vy = 0.0, r2 = ax * ax + ay * ay;
+    switch (f->var_id[k]) {
+      case FFM_VAR_LINEAR: vx = ax; vy = ay; break;
+      case FFM_VAR_SINUSOIDAL: vx = sin(ax); vy = sin(ay); break;
+      case FFM_VAR_SPHERICAL: vx = ax / (r2 + 1e-6); vy = ay / (r2 + 1e-6); break;
+      case FFM_VAR_SWIRL: vx = ax * sin(r2) - ay * cos(r2); vy = ax * cos(r2) + ay * sin(r2); break;
+      case FFM_VAR_BUBBLE: vx = ax / (r2 / 4.0 + 1.0); vy = ay / (r2 / 4.0 + 1.0); break;
+      case FFM_VAR_HYPERBOLIC: vx = ax / (r2 + 1e-6); vy = ay; break;
+      default: {
+        float fx, fy;
+        orc_variation(f->var_id[k], xf->var_param[k], xf, (float)ax, (float)ay, rng, &fx, &fy);
+        vx = fx;
+        vy = fy;
+      }
+    }
+    rx += w * vx;
+    ry += w * vy;
+  }
+  if (xf->post_enabled) {
+    double qx = (double)xf->post_a * rx + (double)xf->post_b * ry + (double)xf->post_e;
+    double qy = (double)xf->post_c * rx + (double)xf->post_d * ry + (double)xf->post_f;
+    rx = qx;
+    ry = qy;
+  }
+  *px = rx;
+  *py = ry;
+}
+
+void orc_softsplat(const ffm_flame_desc* f, const ffm_view* v, uint32_t walkers, uint32_t ipt,
+                   uint32_t burn_in, uint32_t seed, double* density) {
+  int W = v->width, H = v->height;
+  double scale = (double)(W < H ? W : H) * 0.25;
+  double cr = cos((double)v->rotation), sr = sin((double)v->rotation);
+  for (uint32_t tid = 0; tid < walkers; tid++) {
+    orc_rng rng;
+    orc_rng_init(&rng, tid, seed);
+    double x = orc_rng_nextf(&rng) * 2.0 - 1.0;
+    double y = orc_rng_nextf(&rng) * 2.0 - 1.0;
+    uint32_t fuse = burn_in;
+    for (uint32_t it = 0; it < ipt; it++) {
+      int xi = orc_select(f, orc_rng_nextf(&rng));
+      const ffm_xform* xf = &f->xforms[xi];
+      int plot = orc_rng_nextf(&rng) < xf->opacity;
+      apply_xform_d(f, xf, &x, &y, &rng);
+      if (!(fabs(x) <= 1e32) || !(fabs(y) <= 1e32)) {
+        x = orc_rng_nextf(&rng) * 2.0 - 1.0;
+        y = orc_rng_nextf(&rng) * 2.0 - 1.0;
+        fuse = burn_in;
+        continue;
+      }
+      if (fuse > 0) {
+        fuse--;
+        continue;
+      }
+      double fx = x, fy = y;
+      for (int i = 0; i < f->n_finals; i++) apply_xform_d(f, &f->finals[i], &fx, &fy, &rng);
+      if (!plot) continue;
+      double tx = fx - v->pan_x, ty = fy - v->pan_y;
+      double qx = (tx * cr - ty * sr) * v->zoom, qy = (tx * sr + ty * cr) * v->zoom;
+      double sx = W * 0.5 + qx * scale - 0.5, sy = H * 0.5 + qy * scale - 0.5;
+      double x0 = floor(sx), y0 = floor(sy);
+      double ux = sx - x0, uy = sy - y0;
+      int ix = (int)x0, iy = (int)y0;
+      if (!(sx > -1.0 && sy > -1.0 && sx < W && sy < H)) continue;
+      double wts[4] = {(1 - ux) * (1 - uy), ux * (1 - uy), (1 - ux) * uy, ux * uy};
+      int dx[4] = {0, 1, 0, 1}, dy[4] = {0, 0, 1, 1};
+      for (int c = 0; c < 4; c++) {
+        int X = ix + dx[c], Y = iy + dy[c];
+        if (X >= 0 && X < W && Y >= 0 && Y < H) density[(size_t)Y * W + X] += wts[c];
+      }
+    }
+  }
+}

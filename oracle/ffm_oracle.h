@@ -1,0 +1,114 @@
+/*
+ * ffm_oracle.h -- CPU restatement of the reference's flame path, used ONLY
+ * as the checker: by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg. The product (libffm.so) never links, loads or calls it.
+ *
+ * Every function restates the reference's algorithm from its source text
+ * (file:line cited at each definition); FP32 arithmetic is evaluated in the
+ * order the reference's WGSL writes it, with FMA contraction off
+ * (compiled -ffp-contract=off), so integer paths and +,-,*,/ chains are
+ * exact restatements and transcendental calls use the host libm.
+ */
+#ifndef FFM_ORACLE_H
+#define FFM_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/ffm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  uint32_t state, inc;
+} orc_rng;
+
+typedef struct {
+  float x, y, color_index;
+  uint32_t fuse;
+} orc_walker;
+
+/* --- RNG (shaders/core/rng.wgsl:22-55) --- */
+uint32_t orc_pcg_hash(uint32_t input);
+void orc_rng_init(orc_rng* r, uint32_t thread_id, uint32_t seed);
+uint32_t orc_rng_next(orc_rng* r);
+float orc_rng_nextf(orc_rng* r);
+/* Bulk: n consecutive words of walker `tid`'s stream (tests). */
+void orc_rng_stream(uint32_t tid, uint32_t seed, uint32_t n, uint32_t* out);
+
+/* --- transform selection (shaders/core/utilities.wgsl:35-55) --- */
+int orc_select(const ffm_flame_desc* f, float rand_val);
+
+/* --- one variation / one transform --- */
+void orc_variation(int var_id, const float* params, const ffm_xform* xf, float x, float y,
+                   orc_rng* rng, float* ox, float* oy);
+void orc_apply_xform(const ffm_flame_desc* f, const ffm_xform* xf, float* x, float* y,
+                     orc_rng* rng);
+/* Fill derived variation slots (init_julian) in place. */
+void orc_flame_prepare(ffm_flame_desc* f);
+
+/* --- palette (src/scene/palette.rs:265-360; src/gpu/buffers.rs:3215-3221) --- */
+void orc_palette_sample(const float* stops_prgb, int n_stops, float t, float out_rgb[3]);
+uint16_t orc_f32_to_f16(float v);
+float orc_f16_to_f32(uint16_t h);
+/* Linear-light RGB per texel (f16 storage, then srgb_to_linear). */
+void orc_palette_linear(const float* stops_prgb, int n_stops, int size, float* lin_rgb);
+int orc_palette_texel(float color_index, int size);
+
+/* --- view (shaders/core/utilities.wgsl:417-445) --- */
+void orc_world_to_pixel(const ffm_view* v, float x, float y, int32_t* px, int32_t* py);
+
+/* --- the chaos game (shaders/core/main_template.wgsl) --- */
+/* One dispatch of walkers [walker0, walker0+n). hist: W*H*4 u32 (NULL: no
+ * splat). orbits: per-walker resumable state indexed by walker-walker0
+ * (NULL: start fresh, state dropped); fresh!=0 restarts stored walkers.
+ * Returns samples past the fuse; *inframe (may be NULL) the splatted ones. */
+uint64_t orc_iterate(const ffm_flame_desc* f, const float* pal_lin, int pal_size,
+                     const ffm_view* v, uint32_t walker0, uint32_t n, uint32_t ipt,
+                     uint32_t burn_in, uint32_t seed, orc_walker* orbits, int fresh,
+                     uint32_t* hist, uint64_t* inframe);
+/* Same, with several host threads each splatting into a private histogram
+ * that is summed at the end (the CPU baseline). */
+uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal_lin, int pal_size,
+                        const ffm_view* v, uint32_t n, uint32_t ipt, uint32_t burn_in,
+                        uint32_t seed, uint32_t* hist, int threads);
+/* Per-step records of one walker, 8 words a step (see ffm_trace). */
+void orc_trace(const ffm_flame_desc* f, const float* pal_lin, int pal_size, const ffm_view* v,
+               uint32_t walker, uint32_t seed, uint32_t burn_in, uint32_t steps, uint32_t* out);
+
+/* --- host sample accounting: OrbitFuses (src/renderer/compute_kernel.rs:73-102) --- */
+typedef struct {
+  uint32_t* left;
+  uint32_t len;
+} orc_orbit_fuses;
+uint64_t orc_orbit_fuses_dispatch(orc_orbit_fuses* c, uint32_t threads, uint32_t ipt,
+                                  uint32_t burn_in, uint32_t period, uint32_t seed);
+void orc_orbit_fuses_free(orc_orbit_fuses* c);
+
+/* --- stencils --- */
+void orc_blur_pass(const uint32_t* in, uint32_t* out, const uint32_t* center_src, int W, int H,
+                   float sigma, float density_sigma, int direction);
+void orc_blur(uint32_t* hist, uint32_t* scratch, int W, int H, float sigma, float density_sigma);
+float orc_blur_density_sigma(double samples, int W, int H, float blur_edges);
+void orc_accumulate(const void* hist, int hist_is_f32, const float* prev, float* out,
+                    uint32_t* count, int W, int H, int mode, float blend);
+void orc_density_estimate(const float* in, float* out, int W, int H, float max_r, float min_r,
+                          float curve);
+void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
+                 double samples_in_buffer, const ffm_tonemap_params* tp);
+void orc_tonemap_defaults(ffm_tonemap_params* tp);
+
+/* --- whole render (src/renderer/render.rs:483-800) --- */
+int orc_render(const ffm_flame_desc* f, const float* stops_prgb, int n_stops, int pal_size,
+               const ffm_job* job, float* accum_out, uint8_t* rgba_out, uint64_t* total_out,
+               int threads);
+
+/* --- soft splat (no reference counterpart; defines the forward the adjoint differentiates) --- */
+void orc_softsplat(const ffm_flame_desc* f, const ffm_view* v, uint32_t walkers, uint32_t ipt,
+                   uint32_t burn_in, uint32_t seed, double* density);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
