@@ -1,0 +1,118 @@
+"""Chaos-game throughput on one MI355X: the 4K single-flame workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one whole render of the single4k workload through libffm.so:
+6 transforms + a final (3 variations each), 3840x2160, 2^24 walkers x 256
+iterations (one dispatch), accumulate, density estimation (radius 9 -> 0,
+curve 0.4) and the log tonemap, pixels left in HBM. `value` is chaos-game
+samples (walker iterations) per second over the timed steps.
+
+Prints ONE JSON line. The CPU baseline is the oracle's chaos game on this
+host's cores over a bounded sample of the same flame and frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fflame-rust_amd"))
+
+
+def cpu_baseline(fd, pal, job, seconds=10.0):
+    """The oracle (a scalar C restatement of the reference's step) timed on
+    this host's cores: calibrate on a small sample, then run about
+    `seconds` of work."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    v = job.view
+    n = 1 << 12
+    t0 = time.perf_counter()
+    O.iterate_mt(fd, pal, v, n, job.ipt, job.burn_in, 1, threads)
+    dt = time.perf_counter() - t0
+    n = int(max(1 << 12, min(1 << 22, n * seconds / max(dt, 1e-3))))
+    n = (n // threads) * threads
+    t0 = time.perf_counter()
+    O.iterate_mt(fd, pal, v, n, job.ipt, job.burn_in, 1, threads)
+    dt = time.perf_counter() - t0
+    return {"value": n * job.ipt / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"{n} walkers x {job.ipt} iterations of the same flame at "
+                      f"{v.width}x{v.height} (iterate + splat), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--splat-mode", type=int, default=0)
+    ap.add_argument("--walkers-log2", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the walkers would shard with one histogram reduce per batch; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pyfae
+    from pyfae import configs, roofline
+
+    fd, pal, job = configs.single4k(walkers=1 << args.walkers_log2, splat_mode=args.splat_mode)
+    W, H = job.view.width, job.view.height
+    eng = pyfae.Engine(0)
+    eng.set_flame(fd)
+    eng.set_palette(pal, 256)
+    rgba = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda:0")
+    for _ in range(args.warmup):
+        eng.render_to_device(job, rgba)
+    torch.cuda.synchronize()
+    it_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = eng.render_to_device(job, rgba)
+        it_ms.append(st.iterate_ms)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    samples = job.walkers * job.ipt
+    ms_per_step = dt / args.steps * 1e3
+    value = samples / (ms_per_step / 1e3)
+    iterate_ms = sum(it_ms) / len(it_ms)
+    achieved, frac, slots = roofline.iterate_roofline(fd, samples, iterate_ms / 1e3,
+                                                      job.burn_in, job.ipt)
+    out = {
+        "metric": "chaos-game samples/s at 4K",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "single4k: 6 xforms + final xform, 3 of 10 variations each, "
+                               "3840x2160, 2^%d walkers x 256 iterations, density estimation "
+                               "9->0 curve 0.4, log tonemap" % args.walkers_log2,
+                   "splat_mode": args.splat_mode, "samples_per_step": samples},
+        "roofline": {"bound": "valu", "kernel": "iterate_kernel",
+                     "achieved": achieved / 1e12, "peak": roofline.PEAK_SLOTS / 1e12,
+                     "unit": "TFLOP/s", "frac": frac, "traffic": None,
+                     "note": "VALU issue slots per lane (quarter-rate ops = 4); %.0f slots per "
+                             "iteration; peak = 157.3 TFLOP/s FP32 / 4" % slots},
+        "kernel_ms": {"iterate": iterate_ms, "rest_of_step": ms_per_step - iterate_ms},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(fd, pal, job)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

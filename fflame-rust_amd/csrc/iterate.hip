@@ -22,7 +22,7 @@ namespace ffm {
 __device__ __forceinline__ uint32_t xcc_id() {
   uint32_t v;
   asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-  return v & 0xFu;
+  return v & 7u;  // 8 XCDs; the host allocates 8 private copies
 }
 
 struct LdsFlame {

@@ -567,7 +567,7 @@ int ffm_tonemap(ffm_ctx* c, const float* accum, uint8_t* rgba, int width, int he
 // more for a partial batch, stop early after burn_in/ipt + 4 empty
 // dispatches; then the density estimator and the tonemap.
 static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float* accum_out,
-                       ffm_render_stats* stats) {
+                       ffm_render_stats* stats, uint8_t* rgba_dev = nullptr) {
   if (!c) return FFM_ERR_INVALID_ARGUMENT;
   if (!job) return fail(c, FFM_ERR_INVALID_ARGUMENT, "job is NULL");
   if (!c->has_flame) return fail(c, FFM_ERR_NO_FLAME, "no flame set");
@@ -677,16 +677,19 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   if (accum_out) {
     HIPCHECK(c, hipMemcpyAsync(accum_out, final_acc, npix * 16, hipMemcpyDeviceToHost, s));
   }
-  if (rgba_out) {
-    if ((st = ensure(c, c->rgba, npix * 4, "pixels"))) return st;
+  if (rgba_out || rgba_dev) {
+    if (!rgba_dev && (st = ensure(c, c->rgba, npix * 4, "pixels"))) return st;
+    uint8_t* pix = rgba_dev ? rgba_dev : reinterpret_cast<uint8_t*>(c->rgba.p);
     TonemapConst t = make_tonemap(job->tonemap, W, H, job->view.zoom, (double)samples_in_buffer);
     if (job->tonemap.tonemap_mode < 0 || job->tonemap.tonemap_mode > 2 ||
         job->tonemap.highlight_mode < 0 || job->tonemap.highlight_mode > 3)
       return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode not supported");
-    HIPCHECK(c, launch_tonemap(final_acc, reinterpret_cast<uint8_t*>(c->rgba.p), W, H, t, s));
-    hipError_t e = hipMemcpyAsync(rgba_out, c->rgba.p, npix * 4, hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess)
-      return fail(c, FFM_ERR_PIXEL_READ_FAILED, "pixel readback: %s", hipGetErrorString(e));
+    HIPCHECK(c, launch_tonemap(final_acc, pix, W, H, t, s));
+    if (rgba_out) {
+      hipError_t e = hipMemcpyAsync(rgba_out, pix, npix * 4, hipMemcpyDeviceToHost, s);
+      if (e != hipSuccess)
+        return fail(c, FFM_ERR_PIXEL_READ_FAILED, "pixel readback: %s", hipGetErrorString(e));
+    }
   }
   HIPCHECK(c, hipStreamSynchronize(s));
   if (stats) {
@@ -702,6 +705,11 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
 int ffm_render(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, ffm_render_stats* stats) {
   if (!rgba_out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel buffer is NULL");
   return render_impl(c, job, rgba_out, nullptr, stats);
+}
+
+int ffm_render_to_device(ffm_ctx* c, const ffm_job* job, void* rgba_dev, ffm_render_stats* stats) {
+  if (!rgba_dev) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel buffer is NULL");
+  return render_impl(c, job, nullptr, nullptr, stats, reinterpret_cast<uint8_t*>(rgba_dev));
 }
 
 int ffm_render_accum(ffm_ctx* c, const ffm_job* job, float* accum_out, ffm_render_stats* stats) {

@@ -1,0 +1,254 @@
+"""pyfae (MI355X build): the flame path of the Fractal Art Editor's Python API,
+over libffm.so -- hand-written gfx950 HIP kernels behind a C ABI.
+
+Mirrors the reference's Python surface for this path (python/pyfae/__init__.py:
+`Engine.render(config, width, height, iterations=..., iterations_per_thread=...)`
+returning an image) and exposes the granular kernels for device buffers
+(torch tensors or raw device pointers) plus the differentiable soft-splat
+mode as a torch.autograd.Function (pyfae.diff).
+
+There is no CPU fallback: without libffm.so (or without a GPU) every call
+raises EngineError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FlameDesc, Job, RenderStats, TonemapParams, View, Xform,  # noqa: F401
+                   VARIATIONS, tonemap_defaults)
+
+__version__ = "0.5.0+mi355x"
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libffm.so")
+
+
+class EngineError(RuntimeError):
+    """A failed engine call: `status` is the ffm_status code."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+
+
+_lib = None
+
+
+def library():
+    """The loaded libffm.so (raises EngineError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(_abi.ERR_DEVICE,
+                              f"{LIB_PATH} is not built: run fflame-rust_amd/build.py")
+        _lib = _abi.bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor, an int, or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    raise TypeError(f"expected a tensor or device pointer, got {type(x)}")
+
+
+# ---------------------------------------------------------------- flames
+
+def xform(a=1.0, b=0.0, c=0.0, d=1.0, e=0.0, f=0.0, *, weight=1.0, color=0.0, color_speed=0.5,
+          opacity=1.0, variations=None, params=None, post=None):
+    """A transform as the reference's Transform holds it (src/scene/transforms.rs:337-530):
+    `variations` {name: weight}, `params` {"julian.power": 3.0, ...},
+    `post` (a, b, c, d, e, f) or None."""
+    return {"a": a, "b": b, "c": c, "d": d, "e": e, "f": f, "weight": weight, "color": color,
+            "color_speed": color_speed, "opacity": opacity,
+            "variations": dict(variations or {"linear": 1.0}), "params": dict(params or {}),
+            "post": tuple(post) if post is not None else None}
+
+
+def flame_desc(transforms, finals=()):
+    """Pack transforms (dicts from `xform`) into an ffm_flame_desc. Active
+    variations get local indices in first-appearance order over the
+    transforms, then the finals (src/scene/transforms.rs:30-85)."""
+    if len(transforms) > _abi.MAX_XFORMS:
+        raise EngineError(_abi.ERR_TOO_LARGE,
+                          f"{len(transforms)} transforms: at most {_abi.MAX_XFORMS} fit")
+    order = []
+    for t in list(transforms) + list(finals):
+        for name in t["variations"]:
+            if name not in VARIATIONS:
+                raise EngineError(_abi.ERR_INVALID_ARGUMENT, f"unknown variation {name!r}")
+            if name not in order:
+                order.append(name)
+    if len(order) > _abi.MAX_VARS:
+        raise EngineError(_abi.ERR_TOO_LARGE, f"{len(order)} active variations")
+    fd = FlameDesc()
+    fd.n_xforms = len(transforms)
+    fd.n_finals = len(finals)
+    fd.n_vars = len(order)
+    for k, name in enumerate(order):
+        fd.var_id[k] = VARIATIONS[name]
+
+    def fill(dst, t):
+        for key in ("a", "b", "c", "d", "e", "f", "weight", "color", "color_speed", "opacity"):
+            setattr(dst, key, float(t[key]))
+        post = t.get("post")
+        if post is not None:
+            dst.post_enabled = 1
+            dst.post_a, dst.post_b, dst.post_c, dst.post_d, dst.post_e, dst.post_f = map(float, post)
+        else:
+            dst.post_enabled = 0
+            dst.post_a, dst.post_d = 1.0, 1.0
+        for k, name in enumerate(order):
+            dst.var_weight[k] = float(t["variations"].get(name, 0.0))
+            names = _abi.VARIATION_PARAMS.get(name, ())
+            defaults = _abi.PARAM_DEFAULTS.get(name, ())
+            for q, pname in enumerate(names):
+                dst.var_param[k][q] = float(t["params"].get(f"{name}.{pname}", defaults[q]))
+
+    for i, t in enumerate(transforms):
+        fill(fd.xforms[i], t)
+    for i, t in enumerate(finals):
+        fill(fd.finals[i], t)
+    return fd
+
+
+def palette_stops(stops):
+    """(n, 4) float32 array of (position, r, g, b) stops, sRGB in 0..1."""
+    a = np.ascontiguousarray(np.asarray(stops, dtype=np.float32).reshape(-1, 4))
+    return a
+
+
+def make_job(width, height, *, walkers=8192, ipt=256, burn_in=20, seed_base=12345,
+             seed_stride=2654435761, target_samples=0, batch_size=4, splat_mode=0,
+             persistent_orbits=1, zoom=1.0, pan_x=0.0, pan_y=0.0, rotation=0.0,
+             filter_radius=0.0, filter_blur_edges=0.5, de=None, tonemap=None):
+    """An ffm_job. Defaults follow RenderJob::new and the render loop
+    (src/renderer/render.rs:111-138,350-351) and the deterministic seed
+    schedule 12345 + k*2654435761 (src/renderer/compute_kernel.rs:3541-3554)."""
+    j = Job()
+    j.view = View(width, height, zoom, pan_x, pan_y, rotation)
+    j.walkers = walkers
+    j.ipt = ipt
+    j.burn_in = burn_in
+    j.seed_base = seed_base & 0xFFFFFFFF
+    j.seed_stride = seed_stride & 0xFFFFFFFF
+    j.target_samples = target_samples
+    j.batch_size = batch_size
+    j.splat_mode = splat_mode
+    j.persistent_orbits = persistent_orbits
+    j.filter_radius = filter_radius
+    j.filter_blur_edges = filter_blur_edges
+    if de is not None:
+        j.de_max_radius, j.de_min_radius, j.de_curve = de
+    j.tonemap = tonemap if tonemap is not None else tonemap_defaults()
+    return j
+
+
+# ---------------------------------------------------------------- engine
+
+class Engine:
+    """One render context on one GPU (FlameRenderer, src/renderer/compute_kernel.rs:374)."""
+
+    def __init__(self, device=0):
+        self.lib = library()
+        ctx = C.c_void_p()
+        st = self.lib.ffm_ctx_create(int(device), C.byref(ctx))
+        if st != 0:
+            raise EngineError(st, f"no usable GPU {device}")
+        self.ctx = ctx
+        self.device = device
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.ffm_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            msg = self.lib.ffm_last_error(self.ctx)
+            raise EngineError(st, msg.decode() if msg else "error")
+
+    # -- state
+    def set_flame(self, flame):
+        fd = flame if isinstance(flame, FlameDesc) else flame_desc(*flame)
+        self._check(self.lib.ffm_flame_set(self.ctx, C.byref(fd)))
+        return fd
+
+    def set_palette(self, stops, size=256):
+        s = palette_stops(stops)
+        self._check(self.lib.ffm_palette_set(self.ctx, s.ctypes.data_as(C.POINTER(C.c_float)),
+                                             int(s.shape[0]), int(size)))
+
+    def cancel(self):
+        self._check(self.lib.ffm_cancel(self.ctx))
+
+    # -- whole renders
+    def render(self, job):
+        """RGBA8 (H, W, 4) uint8 pixels of the current flame; returns (pixels, stats)."""
+        w, h = job.view.width, job.view.height
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        stats = RenderStats()
+        self._check(self.lib.ffm_render(self.ctx, C.byref(job),
+                                        out.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(stats)))
+        return out, stats
+
+    def render_to_device(self, job, rgba):
+        """Render into a device buffer (a (H, W, 4) uint8 tensor); returns stats."""
+        stats = RenderStats()
+        self._check(self.lib.ffm_render_to_device(self.ctx, C.byref(job), _ptr(rgba),
+                                                  C.byref(stats)))
+        return stats
+
+    def render_accum(self, job):
+        """The RGBA32F accumulator (mean colour, count) after the render."""
+        w, h = job.view.width, job.view.height
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        stats = RenderStats()
+        self._check(self.lib.ffm_render_accum(self.ctx, C.byref(job),
+                                              out.ctypes.data_as(C.POINTER(C.c_float)),
+                                              C.byref(stats)))
+        return out, stats
+
+    # -- granular kernels on device buffers
+    def iterate(self, view, walkers, ipt, burn_in, seed, hist, *, orbits=None, fresh=True,
+                splat_mode=0, plotted=None, inframe=None, stream=None):
+        self._check(self.lib.ffm_iterate(self.ctx, C.byref(view), int(walkers), int(ipt),
+                                         int(burn_in), int(seed) & 0xFFFFFFFF, _ptr(orbits),
+                                         int(bool(fresh)), _ptr(hist), int(splat_mode),
+                                         _ptr(plotted), _ptr(inframe), _ptr(stream)))
+
+    def trace(self, view, walker, seed, burn_in, steps, out, stream=None):
+        self._check(self.lib.ffm_trace(self.ctx, C.byref(view), int(walker), int(seed) & 0xFFFFFFFF,
+                                       int(burn_in), int(steps), _ptr(out), _ptr(stream)))
+
+    def blur(self, hist, scratch, width, height, sigma, density_sigma, stream=None):
+        self._check(self.lib.ffm_blur(self.ctx, _ptr(hist), _ptr(scratch), width, height,
+                                      sigma, density_sigma, _ptr(stream)))
+
+    def density_estimate(self, acc_in, acc_out, width, height, max_r, min_r, curve, stream=None):
+        self._check(self.lib.ffm_density_estimate(self.ctx, _ptr(acc_in), _ptr(acc_out), width,
+                                                  height, max_r, min_r, curve, _ptr(stream)))
+
+    def accumulate(self, hist, prev, out, width, height, *, hist_is_f32=False, count=None, mode=0,
+                   blend=0.0, stream=None):
+        self._check(self.lib.ffm_accumulate(self.ctx, _ptr(hist), int(hist_is_f32), _ptr(prev),
+                                            _ptr(out), _ptr(count), width, height, mode, blend,
+                                            _ptr(stream)))
+
+    def tonemap(self, accum, rgba, width, height, zoom, samples_in_buffer, params=None,
+                stream=None):
+        tp = params if params is not None else tonemap_defaults()
+        self._check(self.lib.ffm_tonemap(self.ctx, _ptr(accum), _ptr(rgba), width, height, zoom,
+                                         float(samples_in_buffer), C.byref(tp), _ptr(stream)))

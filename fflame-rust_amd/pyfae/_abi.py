@@ -135,6 +135,7 @@ SIGNATURES = [
     ("ffm_flame_set", C.c_int, [_vp, C.POINTER(FlameDesc)]),
     ("ffm_palette_set", C.c_int, [_vp, _fp, C.c_int, C.c_int]),
     ("ffm_render", C.c_int, [_vp, C.POINTER(Job), _u8p, C.POINTER(RenderStats)]),
+    ("ffm_render_to_device", C.c_int, [_vp, C.POINTER(Job), _vp, C.POINTER(RenderStats)]),
     ("ffm_render_accum", C.c_int, [_vp, C.POINTER(Job), _fp, C.POINTER(RenderStats)]),
     ("ffm_iterate", C.c_int, [_vp, C.POINTER(View), C.c_uint64, C.c_uint32, C.c_uint32,
                               C.c_uint32, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),

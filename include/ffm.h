@@ -194,6 +194,12 @@ int ffm_render(ffm_ctx* ctx, const ffm_job* job, uint8_t* rgba_out, ffm_render_s
  * `accum_out` (host, width*height*4 floats) instead of tonemapping. */
 int ffm_render_accum(ffm_ctx* ctx, const ffm_job* job, float* accum_out, ffm_render_stats* stats);
 
+/* The same render with the RGBA8 pixels left in device memory
+ * (`rgba_dev`, width*height*4 bytes): no host readback, so a caller that
+ * keeps working on the GPU (torch, a video encoder) pays no PCIe copy. */
+int ffm_render_to_device(ffm_ctx* ctx, const ffm_job* job, void* rgba_dev,
+                         ffm_render_stats* stats);
+
 /* RenderProgress::is_cancelled (src/renderer/render.rs:293-302): polled
  * between dispatches; the render returns FFM_ERR_CANCELLED. */
 int ffm_cancel(ffm_ctx* ctx);

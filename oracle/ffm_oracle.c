@@ -507,19 +507,22 @@ typedef struct {
   const ffm_view* v;
   uint32_t w0, n, ipt, burn_in, seed;
   uint32_t* hist;
+  orc_walker* orbits;
+  int fresh;
   uint64_t plotted;
 } mt_job;
 
 static void* mt_run(void* p) {
   mt_job* j = (mt_job*)p;
   j->plotted = orc_iterate(j->f, j->pal, j->pal_size, j->v, j->w0, j->n, j->ipt, j->burn_in,
-                           j->seed, NULL, 1, j->hist, NULL);
+                           j->seed, j->orbits, j->fresh, j->hist, NULL);
   return NULL;
 }
 
-uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
-                        const ffm_view* v, uint32_t n, uint32_t ipt, uint32_t burn_in,
-                        uint32_t seed, uint32_t* hist, int threads) {
+static uint64_t iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
+                           const ffm_view* v, uint32_t n, uint32_t ipt, uint32_t burn_in,
+                           uint32_t seed, orc_walker* orbits, int fresh, uint32_t* hist,
+                           int threads) {
   if (threads < 1) threads = 1;
   size_t words = (size_t)v->width * (size_t)v->height * 4;
   mt_job* jobs = (mt_job*)calloc((size_t)threads, sizeof(mt_job));
@@ -527,7 +530,8 @@ uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
   uint32_t per = n / (uint32_t)threads, rem = n % (uint32_t)threads, w0 = 0;
   for (int t = 0; t < threads; t++) {
     uint32_t cnt = per + ((uint32_t)t < rem ? 1u : 0u);
-    jobs[t] = (mt_job){f, pal, pal_size, v, w0, cnt, ipt, burn_in, seed, NULL, 0};
+    jobs[t] = (mt_job){f, pal, pal_size, v, w0, cnt, ipt, burn_in, seed, NULL,
+                       orbits ? orbits + w0 : NULL, fresh, 0};
     jobs[t].hist = t == 0 ? hist : (uint32_t*)calloc(words, 4);
     w0 += cnt;
     pthread_create(&th[t], NULL, mt_run, &jobs[t]);
@@ -544,6 +548,12 @@ uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
   free(jobs);
   free(th);
   return plotted;
+}
+
+uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
+                        const ffm_view* v, uint32_t n, uint32_t ipt, uint32_t burn_in,
+                        uint32_t seed, uint32_t* hist, int threads) {
+  return iterate_mt(f, pal, pal_size, v, n, ipt, burn_in, seed, NULL, 1, hist, threads);
 }
 
 void orc_trace(const ffm_flame_desc* f, const float* pal, int pal_size, const ffm_view* v,
@@ -568,6 +578,26 @@ void orc_trace(const ffm_flame_desc* f, const float* pal, int pal_size, const ff
     o[6] = (uint32_t)py;
     o[7] = (uint32_t)st;
   }
+}
+
+/* One step from a given RNG and walker state (per-step parity): writes the
+ * same 8-word record as orc_trace and advances *rng and *w. */
+int orc_step_from(const ffm_flame_desc* f, const float* pal, int pal_size, const ffm_view* v,
+                  uint32_t burn_in, orc_rng* rng, orc_walker* w, uint32_t* o) {
+  step_ctx c = {f, pal, pal_size, v, burn_in};
+  int32_t px, py;
+  int xi;
+  uint32_t rgb[3];
+  int st = step(&c, w, rng, &px, &py, &xi, rgb);
+  o[0] = rng->state;
+  o[1] = (uint32_t)xi;
+  memcpy(&o[2], &w->x, 4);
+  memcpy(&o[3], &w->y, 4);
+  memcpy(&o[4], &w->color_index, 4);
+  o[5] = (uint32_t)px;
+  o[6] = (uint32_t)py;
+  o[7] = (uint32_t)st;
+  return st;
 }
 
 /* ---------------------------------------------------------- OrbitFuses */
@@ -1063,9 +1093,10 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
   for (;;) {
     uint32_t seed = job->seed_base + k * job->seed_stride;
     if (batch_count == 0) memset(hist, 0, np * 16);
-    if (threads > 1 && !orbits) {
+    if (threads > 1) {
       uint32_t* h2 = (uint32_t*)calloc(np * 4, 4);
-      orc_iterate_mt(f, pal, pal_size, v, nw, job->ipt, job->burn_in, seed, h2, threads);
+      iterate_mt(f, pal, pal_size, v, nw, job->ipt, job->burn_in, seed, orbits, k == 0, h2,
+                 threads);
       for (size_t i = 0; i < np * 4; i++) hist[i] += h2[i];
       free(h2);
     } else {

@@ -77,6 +77,9 @@ uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal_lin, int pal_s
 void orc_trace(const ffm_flame_desc* f, const float* pal_lin, int pal_size, const ffm_view* v,
                uint32_t walker, uint32_t seed, uint32_t burn_in, uint32_t steps, uint32_t* out);
 
+int orc_step_from(const ffm_flame_desc* f, const float* pal_lin, int pal_size, const ffm_view* v,
+                  uint32_t burn_in, orc_rng* rng, orc_walker* w, uint32_t* out8);
+
 /* --- host sample accounting: OrbitFuses (src/renderer/compute_kernel.rs:73-102) --- */
 typedef struct {
   uint32_t* left;

@@ -1,0 +1,238 @@
+"""ctypes binding of the CPU oracle (libffm_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker, never as the product.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "fflame-rust_amd"))
+from pyfae import _abi  # noqa: E402
+from pyfae._abi import FlameDesc, Job, TonemapParams, View  # noqa: E402
+
+LIB = os.path.join(HERE, "libffm_oracle.so")
+
+
+class Walker(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("color_index", C.c_float),
+                ("fuse", C.c_uint32)]
+
+
+class Rng(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("inc", C.c_uint32)]
+
+
+class OrbitFuses(C.Structure):
+    _fields_ = [("left", C.POINTER(C.c_uint32)), ("len", C.c_uint32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
+
+
+_lib = None
+_vp, _fp, _u32p, _u8p, _u64p = (C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_uint8), C.POINTER(C.c_uint64))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sig = {
+            "orc_pcg_hash": (C.c_uint32, [C.c_uint32]),
+            "orc_rng_init": (None, [C.POINTER(Rng), C.c_uint32, C.c_uint32]),
+            "orc_rng_next": (C.c_uint32, [C.POINTER(Rng)]),
+            "orc_rng_nextf": (C.c_float, [C.POINTER(Rng)]),
+            "orc_rng_stream": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _u32p]),
+            "orc_select": (C.c_int, [C.POINTER(FlameDesc), C.c_float]),
+            "orc_variation": (None, [C.c_int, _fp, C.POINTER(_abi.Xform), C.c_float, C.c_float,
+                                     C.POINTER(Rng), _fp, _fp]),
+            "orc_apply_xform": (None, [C.POINTER(FlameDesc), C.POINTER(_abi.Xform), _fp, _fp,
+                                       C.POINTER(Rng)]),
+            "orc_flame_prepare": (None, [C.POINTER(FlameDesc)]),
+            "orc_palette_sample": (None, [_fp, C.c_int, C.c_float, _fp]),
+            "orc_f32_to_f16": (C.c_uint16, [C.c_float]),
+            "orc_f16_to_f32": (C.c_float, [C.c_uint16]),
+            "orc_palette_linear": (None, [_fp, C.c_int, C.c_int, _fp]),
+            "orc_palette_texel": (C.c_int, [C.c_float, C.c_int]),
+            "orc_world_to_pixel": (None, [C.POINTER(View), C.c_float, C.c_float,
+                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+            "orc_iterate": (C.c_uint64, [C.POINTER(FlameDesc), _fp, C.c_int, C.POINTER(View),
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, C.POINTER(Walker), C.c_int, _u32p, _u64p]),
+            "orc_iterate_mt": (C.c_uint64, [C.POINTER(FlameDesc), _fp, C.c_int, C.POINTER(View),
+                                            C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p,
+                                            C.c_int]),
+            "orc_trace": (None, [C.POINTER(FlameDesc), _fp, C.c_int, C.POINTER(View), C.c_uint32,
+                                 C.c_uint32, C.c_uint32, C.c_uint32, _u32p]),
+            "orc_step_from": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.POINTER(View),
+                                        C.c_uint32, C.POINTER(Rng), C.POINTER(Walker), _u32p]),
+            "orc_orbit_fuses_dispatch": (C.c_uint64, [C.POINTER(OrbitFuses), C.c_uint32,
+                                                      C.c_uint32, C.c_uint32, C.c_uint32,
+                                                      C.c_uint32]),
+            "orc_orbit_fuses_free": (None, [C.POINTER(OrbitFuses)]),
+            "orc_blur_pass": (None, [_u32p, _u32p, _u32p, C.c_int, C.c_int, C.c_float, C.c_float,
+                                     C.c_int]),
+            "orc_blur": (None, [_u32p, _u32p, C.c_int, C.c_int, C.c_float, C.c_float]),
+            "orc_blur_density_sigma": (C.c_float, [C.c_double, C.c_int, C.c_int, C.c_float]),
+            "orc_accumulate": (None, [_vp, C.c_int, _fp, _fp, _u32p, C.c_int, C.c_int, C.c_int,
+                                      C.c_float]),
+            "orc_density_estimate": (None, [_fp, _fp, C.c_int, C.c_int, C.c_float, C.c_float,
+                                            C.c_float]),
+            "orc_tonemap": (None, [_fp, _u8p, C.c_int, C.c_int, C.c_float, C.c_double,
+                                   C.POINTER(TonemapParams)]),
+            "orc_render": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.c_int, C.POINTER(Job),
+                                     _fp, _u8p, _u64p, C.c_int]),
+            "orc_softsplat": (None, [C.POINTER(FlameDesc), C.POINTER(View), C.c_uint32,
+                                     C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.POINTER(C.c_double)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def fp(a):
+    return a.ctypes.data_as(_fp)
+
+
+def u32p(a):
+    return a.ctypes.data_as(_u32p)
+
+
+# ------------------------------------------------------------ conveniences
+
+def prepared(fd):
+    """A copy of the flame with derived variation slots filled."""
+    f2 = FlameDesc()
+    C.memmove(C.byref(f2), C.byref(fd), C.sizeof(FlameDesc))
+    lib().orc_flame_prepare(C.byref(f2))
+    return f2
+
+
+def palette_linear(stops, size=256):
+    s = np.ascontiguousarray(stops, dtype=np.float32)
+    out = np.zeros(3 * size, dtype=np.float32)
+    lib().orc_palette_linear(fp(s), s.shape[0], size, fp(out))
+    return out
+
+
+def iterate(fd, stops, view, walkers, ipt, burn_in, seed, *, walker0=0, pal_size=256,
+            orbits=None, fresh=True):
+    """(hist u32 (H, W, 4), plotted, inframe) of one dispatch."""
+    f2 = prepared(fd)
+    pal = palette_linear(stops, pal_size)
+    hist = np.zeros((view.height, view.width, 4), dtype=np.uint32)
+    infr = C.c_uint64(0)
+    orb = orbits.ctypes.data_as(C.POINTER(Walker)) if orbits is not None else None
+    plotted = lib().orc_iterate(C.byref(f2), fp(pal), pal_size, C.byref(view), walker0, walkers,
+                                ipt, burn_in, seed & 0xFFFFFFFF, orb, int(bool(fresh)),
+                                u32p(hist), C.byref(infr))
+    return hist, plotted, infr.value
+
+
+def iterate_mt(fd, stops, view, walkers, ipt, burn_in, seed, threads, pal_size=256):
+    f2 = prepared(fd)
+    pal = palette_linear(stops, pal_size)
+    hist = np.zeros((view.height, view.width, 4), dtype=np.uint32)
+    plotted = lib().orc_iterate_mt(C.byref(f2), fp(pal), pal_size, C.byref(view), walkers, ipt,
+                                   burn_in, seed & 0xFFFFFFFF, u32p(hist), threads)
+    return hist, plotted
+
+
+def trace(fd, stops, view, walker, seed, burn_in, steps, pal_size=256):
+    f2 = prepared(fd)
+    pal = palette_linear(stops, pal_size)
+    out = np.zeros((steps, 8), dtype=np.uint32)
+    lib().orc_trace(C.byref(f2), fp(pal), pal_size, C.byref(view), walker, seed & 0xFFFFFFFF,
+                    burn_in, steps, u32p(out))
+    return out
+
+
+def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=256):
+    """Apply one oracle step to each state of a (device) trace: row t of the
+    result is the oracle's step from the trace's state after step t-1 (the
+    walker's start state for t = 0). Returns an (n, 8) record array."""
+    f2 = prepared(fd)
+    pal = palette_linear(stops, pal_size)
+    L = lib()
+    r = Rng()
+    L.orc_rng_init(C.byref(r), walker, seed & 0xFFFFFFFF)
+    inc = r.inc
+    w = Walker()
+    two, one = np.float32(2.0), np.float32(1.0)
+    w.x = float(np.float32(L.orc_rng_nextf(C.byref(r))) * two - one)
+    w.y = float(np.float32(L.orc_rng_nextf(C.byref(r))) * two - one)
+    w.color_index = 0.0
+    w.fuse = burn_in
+    fuse = burn_in
+    out = np.zeros((len(trace_rows), 8), dtype=np.uint32)
+    rec = (C.c_uint32 * 8)()
+    for t, row in enumerate(trace_rows):
+        L.orc_step_from(C.byref(f2), fp(pal), pal_size, C.byref(view), burn_in, C.byref(r),
+                        C.byref(w), rec)
+        out[t] = list(rec)
+        # continue from the device's state after this step
+        flags = int(row[7])
+        fuse = burn_in if flags & 2 else (fuse - 1 if flags & 4 else fuse)
+        r.state, r.inc = int(row[0]), inc
+        w.x = np.uint32(row[2]).view(np.float32).item()
+        w.y = np.uint32(row[3]).view(np.float32).item()
+        w.color_index = np.uint32(row[4]).view(np.float32).item()
+        w.fuse = fuse
+    return out
+
+
+def render(fd, stops, job, pal_size=256, threads=1):
+    """(rgba uint8 (H, W, 4), accum float32 (H, W, 4), total samples)."""
+    w, h = job.view.width, job.view.height
+    rgba = np.zeros((h, w, 4), dtype=np.uint8)
+    acc = np.zeros((h, w, 4), dtype=np.float32)
+    tot = C.c_uint64(0)
+    s = np.ascontiguousarray(stops, dtype=np.float32)
+    lib().orc_render(C.byref(fd), fp(s), s.shape[0], pal_size, C.byref(job), fp(acc),
+                     rgba.ctypes.data_as(_u8p), C.byref(tot), threads)
+    return rgba, acc, tot.value
+
+
+def accumulate(hist, prev, count=None, mode=0, blend=0.0):
+    h, w = hist.shape[:2]
+    out = np.zeros((h, w, 4), dtype=np.float32)
+    is_f32 = hist.dtype == np.float32
+    lib().orc_accumulate(hist.ctypes.data_as(_vp), int(is_f32), fp(prev), fp(out),
+                         u32p(count) if count is not None else None, w, h, mode, blend)
+    return out
+
+
+def tonemap(acc, zoom, samples, tp):
+    h, w = acc.shape[:2]
+    out = np.zeros((h, w, 4), dtype=np.uint8)
+    lib().orc_tonemap(fp(acc), out.ctypes.data_as(_u8p), w, h, zoom, float(samples), C.byref(tp))
+    return out
+
+
+def blur(hist, sigma, density_sigma):
+    h, w = hist.shape[:2]
+    hh = np.ascontiguousarray(hist, dtype=np.uint32).copy()
+    scratch = np.zeros_like(hh)
+    lib().orc_blur(u32p(hh), u32p(scratch), w, h, sigma, density_sigma)
+    return hh
+
+
+def density_estimate(acc, max_r, min_r, curve):
+    h, w = acc.shape[:2]
+    out = np.zeros_like(acc)
+    lib().orc_density_estimate(fp(np.ascontiguousarray(acc)), fp(out), w, h, max_r, min_r, curve)
+    return out

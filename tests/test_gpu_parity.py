@@ -1,0 +1,294 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Bars: bit-exact for the RNG stream, transform choices, bins and integer
+histograms; per-step FP32 positions within 4 ulp of the step's scale for
+transcendental flames (see test_step_parity_transcendental); whole images
+within the seed-to-seed Monte-Carlo noise floor the oracle measures.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O  # noqa: E402
+import pyfae  # noqa: E402
+from pyfae import configs  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+DEV = "cuda:0"
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dev_u32(shape):
+    return torch.zeros(shape, dtype=torch.int32, device=DEV)
+
+
+def as_u32(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def set_flame(engine, fd, pal, size=256):
+    engine.set_flame(fd)
+    engine.set_palette(pal, size)
+
+
+def gpu_trace(engine, fd, pal, view, walker, seed, burn_in, steps):
+    set_flame(engine, fd, pal)
+    out = dev_u32((steps, 8))
+    engine.trace(view, walker, seed, burn_in, steps, out)
+    torch.cuda.synchronize()
+    return as_u32(out)
+
+
+def gpu_hist(engine, fd, pal, view, walkers, ipt, burn_in, seed, mode=0):
+    set_flame(engine, fd, pal)
+    dt = torch.float32 if mode == 3 else torch.int32
+    h = torch.zeros((view.height, view.width, 4), dtype=dt, device=DEV)
+    cnt = torch.zeros(2, dtype=torch.int64, device=DEV)
+    engine.iterate(view, walkers, ipt, burn_in, seed, h, splat_mode=mode, plotted=cnt[0:1].data_ptr(),
+                   inframe=cnt[1:2].data_ptr())
+    torch.cuda.synchronize()
+    c = cnt.cpu().numpy()
+    return (h.cpu().numpy() if mode == 3 else as_u32(h)), int(c[0]), int(c[1])
+
+
+# ------------------------------------------------------------ bit-exact
+
+@pytest.mark.parametrize("walker", [0, 1, 777, 65535, 1 << 20])
+def test_trace_linear_is_bit_exact(engine, walker):
+    say("trace plain walker", walker)
+    fd, pal, job = configs.plain()
+    g = gpu_trace(engine, fd, pal, job.view, walker, 1, 20, 1000)
+    o = O.trace(fd, pal, job.view, walker, 1, 20, 1000)
+    assert np.array_equal(g, o)
+
+
+def test_plain_histogram_is_bit_exact(engine):
+    """The full plain workload: 2^16 walkers x 1000 iterations, 512x512."""
+    say("plain histogram")
+    fd, pal, job = configs.plain()
+    g, plotted, inframe = gpu_hist(engine, fd, pal, job.view, 1 << 16, 1000, 20, 1)
+    o, oplotted = O.iterate_mt(fd, pal, job.view, 1 << 16, 1000, 20, 1, 8)
+    assert plotted == oplotted == (1 << 16) * 980
+    assert inframe == int(o[..., 3].sum()) // 100
+    assert np.array_equal(g, o)
+
+
+def test_xcd_private_histograms_equal_global(engine):
+    say("xcd vs global")
+    fd, pal, job = configs.single4k(walkers=1 << 16, ipt=64, width=960, height=540)
+    g0, p0, i0 = gpu_hist(engine, fd, pal, job.view, 1 << 16, 64, 20, 1, mode=0)
+    g1, p1, i1 = gpu_hist(engine, fd, pal, job.view, 1 << 16, 64, 20, 1, mode=1)
+    assert (p0, i0) == (p1, i1)
+    assert np.array_equal(g0, g1)
+
+
+def test_f32_histogram_matches_u32(engine):
+    say("f32 splat")
+    fd, pal, job = configs.plain(walkers=1 << 14, ipt=200)
+    g0, _, _ = gpu_hist(engine, fd, pal, job.view, 1 << 14, 200, 20, 1, mode=0)
+    g3, _, _ = gpu_hist(engine, fd, pal, job.view, 1 << 14, 200, 20, 1, mode=3)
+    # density channel: sums of 100.0, exact below 2^24
+    assert np.array_equal(g3[..., 3], g0[..., 3].astype(np.float32))
+    # colour: f32 keeps the fraction the u32 path truncates (< 1 per hit)
+    hits = g0[..., 3] / 100.0
+    d = g3[..., :3] - g0[..., :3]
+    assert (d >= -1e-3 * g3[..., :3] - 1e-3).all() and (d <= hits[..., None] + 1e-3 * g3[..., :3] + 1e-3).all()
+
+
+# ---------------------------------------------------- per-step tolerance
+
+def ulp(x):
+    x = np.abs(np.asarray(x, dtype=np.float32))
+    return np.spacing(np.maximum(x, np.float32(2.0 ** -20))).astype(np.float64)
+
+
+@pytest.mark.parametrize("cfg", ["single4k", "stress"])
+def test_step_parity_transcendental(engine, cfg):
+    """One iterator step on the GPU vs the oracle's step from the same state:
+    RNG words, transform choices and flags exact; x, y within 4 ulp of
+    max(|x|, |y|) of the step's output (the scale at which the step's
+    transcendental terms are summed) for 99.9% of steps, 64 ulp for all."""
+    say("step parity", cfg)
+    fd, pal, job = getattr(configs, cfg)()
+    errs = []
+    for walker in (3, 1000, 123456):
+        g = gpu_trace(engine, fd, pal, job.view, walker, 1, 20, 2000)
+        o = O.replay_steps(fd, pal, job.view, walker, 1, 20, g)
+        assert np.array_equal(g[:, 0], o[:, 0]), "rng stream"
+        assert np.array_equal(g[:, 1], o[:, 1]), "transform choice"
+        assert np.array_equal(g[:, 7] & 2, o[:, 7] & 2), "respawns"
+        gx = g[:, 2:4].view(np.float32).astype(np.float64)
+        ox = o[:, 2:4].view(np.float32).astype(np.float64)
+        scale = ulp(np.max(np.abs(ox), axis=1))[:, None]
+        ok = np.isfinite(ox).all(1)
+        errs.append((np.abs(gx - ox) / scale)[ok].max(1))
+        # colour index: a pure +,* chain
+        assert np.array_equal(g[:, 4], o[:, 4])
+    e = np.concatenate(errs)
+    say(f"  ulp error: median {np.median(e):.2f}  p99.9 {np.quantile(e, 0.999):.2f}  max {e.max():.2f}")
+    assert np.quantile(e, 0.999) <= 4.0 and e.max() <= 64.0
+
+
+# -------------------------------------------------------------- stencils
+
+def test_accumulate_bit_exact_and_count(engine):
+    say("accumulate")
+    rng = np.random.default_rng(1)
+    H, W = 37, 53
+    hist = rng.integers(0, 5000, (H, W, 4)).astype(np.uint32)
+    hist[..., :3] = np.minimum(hist[..., :3], hist[..., 3:4])
+    hist[::3, ::4] = 0
+    prev = np.zeros((H, W, 4), np.float32)
+    prev[..., 3] = rng.uniform(0, 1e6, (H, W)).astype(np.float32)
+    prev[..., :3] = rng.uniform(0, 1, (H, W, 3))
+    prev[::5, ::2] = 0
+    for mode, blend in ((0, 0.0), (1, 0.25), (2, 0.0)):
+        count = np.zeros((H, W, 2), np.uint32)
+        o = O.accumulate(hist, prev, count=count, mode=mode, blend=blend)
+        th = torch.from_numpy(hist.view(np.int32)).to(DEV)
+        tp = torch.from_numpy(prev).to(DEV)
+        to = torch.zeros_like(tp)
+        tc = torch.zeros((H, W, 2), dtype=torch.int32, device=DEV)
+        engine.accumulate(th, tp, to, W, H, count=tc, mode=mode, blend=blend)
+        torch.cuda.synchronize()
+        assert to.cpu().numpy().tobytes() == o.tobytes(), mode
+        assert np.array_equal(as_u32(tc), count), mode
+
+
+def test_tonemap_within_one_level(engine):
+    say("tonemap")
+    rng = np.random.default_rng(2)
+    H, W = 64, 96
+    acc = np.zeros((H, W, 4), np.float32)
+    acc[..., :3] = rng.uniform(0, 1, (H, W, 3))
+    acc[..., 3] = np.exp(rng.uniform(-3, 12, (H, W))).astype(np.float32)
+    acc[::7, ::3] = 0
+    for tp in (pyfae.tonemap_defaults(), configs.log_tonemap(2.2)):
+        tp.saturation = 1.3
+        tp.background[:] = [0.1, 0.2, 0.05]
+        o = O.tonemap(acc, 4.0 / 3.0, 5e6, tp)
+        ta = torch.from_numpy(acc).to(DEV)
+        tr = torch.zeros((H, W, 4), dtype=torch.uint8, device=DEV)
+        engine.tonemap(ta, tr, W, H, 4.0 / 3.0, 5e6, tp)
+        torch.cuda.synchronize()
+        d = np.abs(tr.cpu().numpy().astype(int) - o.astype(int))
+        assert d.max() <= 1 and (d > 0).mean() < 0.01
+
+
+def test_blur_within_one_count(engine):
+    say("blur")
+    rng = np.random.default_rng(3)
+    H, W = 40, 70
+    hist = rng.integers(0, 20000, (H, W, 4)).astype(np.uint32)
+    o = O.blur(hist, 1.7, 3000.0)
+    th = torch.from_numpy(hist.view(np.int32)).to(DEV)
+    sc = torch.zeros((2, H, W, 4), dtype=torch.int32, device=DEV)
+    engine.blur(th, sc, W, H, 1.7, 3000.0)
+    torch.cuda.synchronize()
+    d = np.abs(as_u32(th).astype(np.int64) - o.astype(np.int64))
+    assert d.max() <= 1
+
+
+def test_density_estimate_matches_oracle(engine):
+    say("density estimate")
+    rng = np.random.default_rng(4)
+    H, W = 50, 70
+    acc = np.zeros((H, W, 4), np.float32)
+    acc[..., 3] = np.where(rng.uniform(size=(H, W)) < 0.5, rng.integers(1, 400, (H, W)), 0)
+    acc[..., :3] = rng.uniform(0, 1, (H, W, 3))
+    o = O.density_estimate(acc, 9.0, 0.0, 0.4)
+    ta = torch.from_numpy(acc).to(DEV)
+    to = torch.zeros_like(ta)
+    engine.density_estimate(ta, to, W, H, 9.0, 0.0, 0.4)
+    torch.cuda.synchronize()
+    g = to.cpu().numpy()
+    np.testing.assert_allclose(g[..., 3], o[..., 3], rtol=2e-4, atol=1e-4)
+    m = o[..., 3] > 1e-3
+    np.testing.assert_allclose(g[..., :3][m], o[..., :3][m], rtol=2e-4, atol=1e-5)
+
+
+# ---------------------------------------------------------- whole images
+
+def test_render_matches_reference_baseline_image(engine):
+    """The reference's own rendered baseline (tests/visual/baseline) through
+    the GPU path, by the reference's visual rule."""
+    say("render simple-linear vs reference baseline")
+    from test_oracle import _simple_linear_job, visual_rule
+    fd, stops, job = _simple_linear_job()
+    set_flame(engine, fd, stops)
+    rgba, stats = engine.render(job)
+    assert stats.total_samples >= job.target_samples
+    mean, frac = visual_rule(rgba, os.path.join(GOLDEN, "ref_2d_simple_linear.png"))
+    say(f"  mean {mean:.3f} frac>40 {frac:.4f}")
+    assert mean <= 2.0 and frac <= 0.01
+
+
+def test_plain_render_equals_oracle(engine):
+    say("plain render")
+    fd, pal, job = configs.plain()
+    set_flame(engine, fd, pal)
+    rgba, stats = engine.render(job)
+    orgba, oacc, tot = O.render(fd, pal, job, threads=8)
+    assert stats.total_samples == tot
+    d = np.abs(rgba.astype(int) - orgba.astype(int))
+    assert d.max() <= 1
+
+
+def block_sums(acc, b=16):
+    h, w = acc.shape[:2]
+    d = acc[: h // b * b, : w // b * b, 3].reshape(h // b, b, w // b, b).sum((1, 3))
+    return d
+
+
+def test_single4k_image_within_noise_floor(engine):
+    """Transcendental flame, reduced walkers: GPU vs oracle block densities
+    differ no more than two oracle seeds differ from each other."""
+    say("single4k noise floor")
+    fd, pal, job = configs.single4k(walkers=1 << 15, ipt=256, width=960, height=540)
+    job.de_max_radius = 0.0
+    set_flame(engine, fd, pal)
+    gacc, _ = engine.render_accum(job)
+    _, o1, _ = O.render(fd, pal, job, threads=8)
+    job2 = configs.single4k(walkers=1 << 15, ipt=256, width=960, height=540)[2]
+    job2.de_max_radius = 0.0
+    job2.seed_base = 2
+    _, o2, _ = O.render(fd, pal, job2, threads=8)
+    bg, b1, b2 = block_sums(gacc), block_sums(o1), block_sums(o2)
+    floor = np.abs(b1 - b2).sum() / b1.sum()
+    diff = np.abs(bg - b1).sum() / b1.sum()
+    say(f"  floor {floor:.4f} gpu-vs-oracle {diff:.4f}")
+    assert diff <= 1.25 * floor + 1e-3
+
+
+# ------------------------------------------------------------ error paths
+
+def test_errors_are_status_codes(engine):
+    say("errors")
+    e2 = pyfae.Engine(0)
+    v = pyfae.View(64, 64, 1.0, 0, 0, 0)
+    h = dev_u32((64, 64, 4))
+    with pytest.raises(pyfae.EngineError) as ei:
+        e2.iterate(v, 64, 10, 0, 1, h)
+    assert ei.value.status == pyfae._abi.ERR_NO_FLAME
+    fd = pyfae.flame_desc([pyfae.xform()])
+    e2.set_flame(fd)
+    with pytest.raises(pyfae.EngineError) as ei:
+        e2.iterate(v, 64, 10, 0, 1, h)
+    assert ei.value.status == pyfae._abi.ERR_NO_PALETTE
+    e2.set_palette([[0, 1, 1, 1]])
+    with pytest.raises(pyfae.EngineError) as ei:
+        e2.iterate(pyfae.View(40000, 10, 1, 0, 0, 0), 64, 10, 0, 1, h)
+    assert ei.value.status == pyfae._abi.ERR_TOO_LARGE
+    with pytest.raises(pyfae.EngineError) as ei:
+        e2.render(pyfae.make_job(64, 64, walkers=0))
+    assert ei.value.status == pyfae._abi.ERR_INVALID_ARGUMENT
+    e2.close()
