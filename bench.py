@@ -29,11 +29,11 @@ def cpu_baseline(fd, pal, job, seconds=10.0):
     from oracle import oracle as O
     threads = max(1, min(os.cpu_count() or 1, 16))
     v = job.view
-    n = 1 << 12
+    n = 1 << 16
     t0 = time.perf_counter()
     O.iterate_mt(fd, pal, v, n, job.ipt, job.burn_in, 1, threads)
     dt = time.perf_counter() - t0
-    n = int(max(1 << 12, min(1 << 22, n * seconds / max(dt, 1e-3))))
+    n = int(max(1 << 16, min(1 << 24, n * seconds / max(dt, 1e-3))))
     n = (n // threads) * threads
     t0 = time.perf_counter()
     O.iterate_mt(fd, pal, v, n, job.ipt, job.burn_in, 1, threads)

@@ -298,4 +298,36 @@ __device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& 
   return 0;
 }
 
+struct LdsFlame {
+  FlameView fv;
+  const uint32_t* pal_u32;
+  const float* pal_f32;
+};
+
+// Copy the flame table and palette into LDS; returns the LDS views.
+template <bool F32PAL>
+__device__ __forceinline__ LdsFlame load_lds(const IterArgs& a, uint32_t* lds) {
+  const FlameHeader* hdr = reinterpret_cast<const FlameHeader*>(a.flame);
+  const int nx = hdr->n_xforms, nf = hdr->n_finals, nv = hdr->n_vars;
+  const int rows_f = (nx + nf) * XF_STRIDE;
+  const int hdr_w = (int)(sizeof(FlameHeader) / 4);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.flame);
+  for (int i = threadIdx.x; i < hdr_w + rows_f; i += blockDim.x) lds[i] = src[i];
+  uint32_t* pal = lds + hdr_w + rows_f;
+  for (int i = threadIdx.x; i < a.pal_size; i += blockDim.x) pal[i] = a.pal_u32[i];
+  float* palf = reinterpret_cast<float*>(pal + a.pal_size);
+  if (F32PAL)
+    for (int i = threadIdx.x; i < 3 * a.pal_size; i += blockDim.x) palf[i] = a.pal_f32[i];
+  __syncthreads();
+  LdsFlame l;
+  l.fv.rows = reinterpret_cast<const float*>(lds + hdr_w);
+  l.fv.ids = reinterpret_cast<const int*>(lds + 4);
+  l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
+  l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
+  l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);
+  l.pal_u32 = pal;
+  l.pal_f32 = palf;
+  return l;
+}
+
 }  // namespace ffm

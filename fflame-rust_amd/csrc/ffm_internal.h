@@ -38,7 +38,7 @@ struct FlameHeader {
 };
 
 // A flame table in device memory: header then (n_xforms + n_finals) rows.
-inline size_t flame_table_floats(int n_rows) {
+__host__ __device__ inline size_t flame_table_floats(int n_rows) {
   return sizeof(FlameHeader) / 4 + (size_t)n_rows * XF_STRIDE;
 }
 
@@ -67,6 +67,29 @@ struct IterArgs {
   unsigned long long* plotted;
   unsigned long long* inframe;
 };
+
+// Screen tiles of the LDS splat: 128x64 pixels = 128 KB of u32x4 in LDS.
+constexpr int TILE_W = 128, TILE_H = 64, TILE_PX = TILE_W * TILE_H;
+constexpr int TILE_MAX = 2048;  // tiles per image the LDS counting sort holds
+
+struct TileArgs {
+  uint32_t* tile_fill;   // n_tiles: records requested per tile
+  uint32_t* region;      // n_tiles: first slot of the tile's region in `records`
+  uint32_t* cap;         // n_tiles: slots in the region
+  uint32_t* item_start;  // n_tiles + 1: work list of tile_accum
+  uint32_t* records;     // tile-local pixel | texel << 13
+  uint32_t* hist;        // the u32x4 histogram
+  int32_t tiles_x, n_tiles, width, height;
+};
+
+size_t bin_lds_bytes(int n_rows, int pal_size);
+size_t accum_lds_bytes(int pal_size);
+hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_only, int grid,
+                              hipStream_t s);
+hipError_t launch_tile_capacity(const TileArgs& t, float scale, uint32_t margin,
+                                uint64_t buffer_records, hipStream_t s);
+hipError_t launch_tile_accum(const TileArgs& t, const uint32_t* pal, int pal_size, int grid,
+                             hipStream_t s);
 
 // Kernel launchers (defined in the .hip files).
 hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block,

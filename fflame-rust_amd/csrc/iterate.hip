@@ -25,38 +25,6 @@ __device__ __forceinline__ uint32_t xcc_id() {
   return v & 7u;  // 8 XCDs; the host allocates 8 private copies
 }
 
-struct LdsFlame {
-  FlameView fv;
-  const uint32_t* pal_u32;
-  const float* pal_f32;
-};
-
-// Copy the flame table and palette into LDS; returns the LDS views.
-template <bool F32PAL>
-__device__ __forceinline__ LdsFlame load_lds(const IterArgs& a, uint32_t* lds) {
-  const FlameHeader* hdr = reinterpret_cast<const FlameHeader*>(a.flame);
-  const int nx = hdr->n_xforms, nf = hdr->n_finals, nv = hdr->n_vars;
-  const int rows_f = (nx + nf) * XF_STRIDE;
-  const int hdr_w = (int)(sizeof(FlameHeader) / 4);
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.flame);
-  for (int i = threadIdx.x; i < hdr_w + rows_f; i += blockDim.x) lds[i] = src[i];
-  uint32_t* pal = lds + hdr_w + rows_f;
-  for (int i = threadIdx.x; i < a.pal_size; i += blockDim.x) pal[i] = a.pal_u32[i];
-  float* palf = reinterpret_cast<float*>(pal + a.pal_size);
-  if (F32PAL)
-    for (int i = threadIdx.x; i < 3 * a.pal_size; i += blockDim.x) palf[i] = a.pal_f32[i];
-  __syncthreads();
-  LdsFlame l;
-  l.fv.rows = reinterpret_cast<const float*>(lds + hdr_w);
-  l.fv.ids = reinterpret_cast<const int*>(lds + 4);
-  l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
-  l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
-  l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);
-  l.pal_u32 = pal;
-  l.pal_f32 = palf;
-  return l;
-}
-
 // Wave-level sum then one atomic per wave.
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long v) {
   if (!dst) return;
@@ -179,7 +147,19 @@ static size_t lds_bytes(const IterArgs& a, int nrows, bool f32pal) {
   return w * 4;
 }
 
+// A 128-transform flame with a 4096-texel palette needs more than the
+// default 64 KB of dynamic LDS.
+static hipError_t allow_lds(const void* fn) {
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
 hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block, hipStream_t s) {
+  static const hipError_t e0 =
+      allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_U32_GLOBAL>));
+  static const hipError_t e1 = allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_U32_XCD>));
+  static const hipError_t e3 =
+      allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_F32_GLOBAL>));
+  if (e0 != hipSuccess || e1 != hipSuccess || e3 != hipSuccess) return hipErrorInvalidValue;
   size_t shm = lds_bytes(a, a.n_rows, splat_mode == FFM_SPLAT_F32_GLOBAL);
   switch (splat_mode) {
     case FFM_SPLAT_U32_GLOBAL:
@@ -199,6 +179,8 @@ hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block
 
 hipError_t launch_trace(const IterArgs& a, uint32_t walker, uint32_t steps, uint32_t* out,
                         hipStream_t s) {
+  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&trace_kernel));
+  if (e0 != hipSuccess) return e0;
   size_t shm = lds_bytes(a, a.n_rows, false);
   hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(64), shm, s, a, walker, steps, out);
   return hipGetLastError();

@@ -52,6 +52,13 @@ struct ffm_ctx {
   DevBuf pal_u32, pal_f32;
   // render buffers
   DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv;
+  // LDS tile splat: per-tile fill/region/capacity, work list, record buffer
+  DevBuf t_fill, t_region, t_cap, t_items, t_records;
+  int t_W = 0, t_H = 0, t_nrows = 0, t_pal = 0;
+  uint64_t t_walkers = 0;
+  uint32_t t_ipt = 0;
+  bool t_planned = false;
+  ViewConst t_view{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -267,6 +274,77 @@ int iterate_grid(ffm_ctx* c, uint64_t walkers) {
 }
 
 constexpr int XCD_COPIES = 8;
+constexpr uint64_t RECORD_BUDGET = 1ull << 31;  // staged samples held at once (8 GB)
+constexpr size_t LDS_PER_CU = 160 * 1024;
+
+// The LDS tile splat of one dispatch (see tilesplat.hip): walkers in chunks
+// whose records fit the buffer; region capacities from a count-only
+// pre-pass over a subset of the first chunk (kept while the flame, size and
+// dispatch shape stay the same); per chunk: clear fills, binned iterate,
+// tile accumulate into `hist`.
+int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s) {
+  const int W = a.view.width, H = a.view.height;
+  const int tiles_x = (W + TILE_W - 1) / TILE_W, tiles_y = (H + TILE_H - 1) / TILE_H;
+  const int n_tiles = tiles_x * tiles_y;
+  if (n_tiles > TILE_MAX)
+    return fail(c, FFM_ERR_TOO_LARGE, "the LDS tile splat holds %d tiles of %dx%d; %dx%d needs %d",
+                TILE_MAX, TILE_W, TILE_H, W, H, n_tiles);
+  if (bin_lds_bytes(a.n_rows, a.pal_size) > LDS_PER_CU)
+    return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
+  const uint64_t per_walker = (uint64_t)a.ipt + 1;
+  uint64_t chunk = std::max<uint64_t>(1, (RECORD_BUDGET * 3 / 4) / per_walker);
+  chunk = std::min<uint64_t>(chunk, a.walkers);
+  const uint64_t rec_cap = std::min<uint64_t>(RECORD_BUDGET, chunk * per_walker * 5 / 4 +
+                                                             (uint64_t)n_tiles * 4096);
+  int st;
+  if ((st = ensure(c, c->t_fill, (size_t)TILE_MAX * 4, "tile fills"))) return st;
+  if ((st = ensure(c, c->t_region, (size_t)TILE_MAX * 4, "tile regions"))) return st;
+  if ((st = ensure(c, c->t_cap, (size_t)TILE_MAX * 4, "tile capacities"))) return st;
+  if ((st = ensure(c, c->t_items, (size_t)(TILE_MAX + 4) * 4, "tile work list"))) return st;
+  if ((st = ensure(c, c->t_records, rec_cap * 4, "tile records"))) return st;
+  TileArgs t{};
+  t.tile_fill = reinterpret_cast<uint32_t*>(c->t_fill.p);
+  t.region = reinterpret_cast<uint32_t*>(c->t_region.p);
+  t.cap = reinterpret_cast<uint32_t*>(c->t_cap.p);
+  t.item_start = reinterpret_cast<uint32_t*>(c->t_items.p);
+  t.records = reinterpret_cast<uint32_t*>(c->t_records.p);
+  t.hist = hist;
+  t.tiles_x = tiles_x;
+  t.n_tiles = n_tiles;
+  t.width = W;
+  t.height = H;
+  const int grid = std::max(c->num_cu, 1);
+  const bool same = c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
+                    c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
+                    std::memcmp(&c->t_view, &a.view, sizeof(ViewConst)) == 0;
+  if (!same) {
+    IterArgs pre = a;
+    pre.walkers = std::min<uint64_t>(chunk, std::max<uint64_t>(chunk / 32, 65536));
+    pre.plotted = pre.inframe = nullptr;
+    HIPCHECK(c, launch_clear_u32(t.tile_fill, TILE_MAX, s));
+    HIPCHECK(c, launch_bin_iterate(pre, t, true, grid, s));
+    HIPCHECK(c, launch_tile_capacity(t, (float)((double)chunk / (double)pre.walkers), 1024,
+                                     rec_cap, s));
+    c->t_planned = true;
+    c->t_W = W;
+    c->t_H = H;
+    c->t_nrows = a.n_rows;
+    c->t_pal = a.pal_size;
+    c->t_walkers = chunk;
+    c->t_ipt = a.ipt;
+    c->t_view = a.view;
+  }
+  for (uint64_t w0 = 0; w0 < a.walkers; w0 += chunk) {
+    IterArgs ch = a;
+    ch.walker0 = a.walker0 + (uint32_t)w0;
+    ch.walkers = std::min<uint64_t>(chunk, a.walkers - w0);
+    if (a.orbits) ch.orbits = a.orbits + w0;
+    HIPCHECK(c, launch_clear_u32(t.tile_fill, TILE_MAX, s));
+    HIPCHECK(c, launch_bin_iterate(ch, t, false, grid, s));
+    HIPCHECK(c, launch_tile_accum(t, a.pal_u32, a.pal_size, grid, s));
+  }
+  return FFM_OK;
+}
 
 // Host twin of OrbitFuses (src/renderer/compute_kernel.rs:73-102).
 struct OrbitFuses {
@@ -383,7 +461,7 @@ int ffm_ctx_destroy(ffm_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->table, &c->pal_u32, &c->pal_f32, &c->hist, &c->copies, &c->scratch,
                     &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
-                    &c->de_inv})
+                    &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records})
     b->release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -413,6 +491,7 @@ int ffm_flame_set(ffm_ctx* c, const ffm_flame_desc* f) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->n_rows = f->n_xforms + f->n_finals;
   c->has_flame = true;
+  c->t_planned = false;  // the attractor, hence the tile capacities, changed
   return FFM_OK;
 }
 
@@ -449,8 +528,7 @@ int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt
   if (walkers == 0 || walkers > 0xFFFFFFFFull)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers %llu outside 1..2^32-1",
                 (unsigned long long)walkers);
-  if (splat_mode != FFM_SPLAT_U32_GLOBAL && splat_mode != FFM_SPLAT_U32_XCD &&
-      splat_mode != FFM_SPLAT_F32_GLOBAL)
+  if (splat_mode < FFM_SPLAT_U32_GLOBAL || splat_mode > FFM_SPLAT_F32_GLOBAL)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "splat mode %d not available here", splat_mode);
   (void)hipSetDevice(c->device);
   hipStream_t s = pick(c, stream);
@@ -464,6 +542,7 @@ int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt
   a.plotted = reinterpret_cast<unsigned long long*>(plotted);
   a.inframe = reinterpret_cast<unsigned long long*>(inframe);
   const size_t words = (size_t)view->width * view->height * 4;
+  if (splat_mode == FFM_SPLAT_U32_LDS) return lds_splat(c, a, reinterpret_cast<uint32_t*>(hist), s);
   if (splat_mode == FFM_SPLAT_U32_XCD) {
     if ((st = ensure(c, c->copies, words * 4 * XCD_COPIES, "per-XCD histograms"))) return st;
     HIPCHECK(c, launch_clear_u32(c->copies.p, words * XCD_COPIES, s));
@@ -577,7 +656,7 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   if (job->walkers == 0 || job->walkers > 0xFFFFFFFFull || job->ipt == 0)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers and iterations per thread must be positive");
   const int mode = job->splat_mode;
-  if (mode != FFM_SPLAT_U32_GLOBAL && mode != FFM_SPLAT_U32_XCD && mode != FFM_SPLAT_F32_GLOBAL)
+  if (mode < FFM_SPLAT_U32_GLOBAL || mode > FFM_SPLAT_F32_GLOBAL)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "splat mode %d not available here", mode);
   if (mode == FFM_SPLAT_F32_GLOBAL && job->filter_radius > 0.0f)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "the histogram blur reads a u32 histogram");
@@ -626,7 +705,11 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     a.seed = job->seed_base + (uint32_t)k * job->seed_stride;
     a.fresh = k == 0;
     HIPCHECK(c, hipEventRecord(c->ev0, s));
-    HIPCHECK(c, launch_iterate(a, mode, grid, 256, s));
+    if (mode == FFM_SPLAT_U32_LDS) {
+      if ((st = lds_splat(c, a, reinterpret_cast<uint32_t*>(c->hist.p), s))) return st;
+    } else {
+      HIPCHECK(c, launch_iterate(a, mode, grid, 256, s));
+    }
     HIPCHECK(c, hipEventRecord(c->ev1, s));
     uint64_t plotted;
     if (job->persistent_orbits) {

@@ -1,0 +1,384 @@
+// tilesplat.hip -- the LDS-privatised screen-tile splat (FFM_SPLAT_U32_LDS).
+//
+// The reference splats each in-frame sample with four u32 atomicAdd on the
+// global histogram (shaders/core/main_template.wgsl:1253-1320). At 4K the
+// histogram (132.7 MB) is far larger than any cache, so every sample costs
+// a random read-modify-write of a 128-byte line in HBM. Here the splat is
+// split in two streaming passes:
+//
+//  1. bin_iterate_kernel: the same walkers as iterate_kernel, but each
+//     in-frame sample becomes a 4-byte record (pixel index inside its
+//     128x64 screen tile | palette texel << 13) staged in LDS. Every few
+//     iterations the workgroup counting-sorts the staged records by tile
+//     (LDS atomics + a block scan), reserves a run in each tile's region of
+//     the record buffer with ONE global atomic per (flush, tile), and
+//     writes the runs tile-contiguously.
+//  2. tile_accum_kernel: one workgroup per (tile, slice of its records)
+//     accumulates the slice into a 128x64 u32x4 tile held in LDS (LDS
+//     atomics) and folds the tile into the histogram: a plain
+//     read-add-write when it owns the whole tile, integer atomics when the
+//     tile's records are split over several workgroups.
+//
+// Integer sums commute, so the histogram is bit-identical to the atomic
+// splat's. Records that do not fit a tile's region (its capacity comes
+// from a count-only pre-pass over a subset of the walkers) fall back to
+// direct global atomics, so correctness never depends on the estimate.
+#include "ffm_device.h"
+
+namespace ffm {
+
+constexpr int BIN_BLOCK = 1024;
+constexpr int STAGE = 10240;  // staged records per workgroup
+constexpr int CHECK_EVERY = 4;  // iterations between fill checks
+constexpr int FLUSH_AT = STAGE - CHECK_EVERY * BIN_BLOCK;
+constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
+
+size_t bin_lds_bytes(int n_rows, int pal_size) {
+  size_t flame = flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4;
+  size_t stage = (size_t)STAGE * (4 + 4 + 2) + (size_t)TILE_MAX * 8 + 16 * 4 + 16;
+  return flame + stage;
+}
+
+size_t accum_lds_bytes(int pal_size) { return (size_t)TILE_PX * 16 + (size_t)pal_size * 4; }
+
+// Exclusive scan of a[0..n) (n <= 2 * blockDim.x) in place; all threads.
+__device__ void block_exclusive_scan(uint32_t* a, int n, uint32_t* wave_sums, uint32_t* total) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  const int i0 = 2 * tid, i1 = i0 + 1;
+  const uint32_t v0 = i0 < n ? a[i0] : 0u, v1 = i1 < n ? a[i1] : 0u;
+  const uint32_t s = v0 + v1;
+  uint32_t x = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wave_sums[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t w = lane < nw ? wave_sums[lane] : 0u;
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t y = __shfl_up(w, off, 64);
+      if (lane >= off) w += y;
+    }
+    if (lane < nw) wave_sums[lane] = w;
+    if (lane == nw - 1 && total) *total = w;
+  }
+  __syncthreads();
+  const uint32_t base = (wave > 0 ? wave_sums[wave - 1] : 0u) + x - s;
+  if (i0 < n) a[i0] = base;
+  if (i1 < n) a[i1] = base + v0;
+  __syncthreads();
+}
+
+struct BinLds {
+  uint32_t* rec;    // STAGE
+  uint32_t* tr;     // STAGE: tile | rank << 16
+  uint16_t* perm;   // STAGE
+  uint32_t* cnt;    // TILE_MAX
+  uint32_t* gbase;  // TILE_MAX
+  uint32_t* wsum;   // 16
+  uint32_t* fill;   // 1
+};
+
+// Splat one record straight into the histogram (region overflow).
+__device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t* pal, int tile,
+                                               uint32_t rec) {
+  const int tx = tile % t.tiles_x, ty = tile / t.tiles_x;
+  const int local = (int)(rec & 0x1FFFu);
+  const int px = tx * TILE_W + (local % TILE_W), py = ty * TILE_H + (local / TILE_W);
+  const uint32_t pk = pal[rec >> 13];
+  uint32_t* h = t.hist + 4 * ((size_t)py * t.width + px);
+  __hip_atomic_fetch_add(h + 0, pk & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(h + 1, (pk >> 8) & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(h + 2, (pk >> 16) & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(h + 3, 100u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sort the staged records by tile and write them to their tile regions.
+// Called by every thread of the workgroup.
+__device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* pal) {
+  const int tid = threadIdx.x;
+  const int n = (int)min(*L.fill, (uint32_t)STAGE);
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
+  __syncthreads();
+  for (int j = tid; j < n; j += blockDim.x) {
+    const uint32_t tile = L.tr[j];
+    const uint32_t r = atomicAdd(&L.cnt[tile], 1u);
+    L.tr[j] = tile | (r << 16);
+  }
+  __syncthreads();
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) {
+    const uint32_t c = L.cnt[i];
+    L.gbase[i] = c ? __hip_atomic_fetch_add(t.tile_fill + i, c, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)
+                   : 0u;
+  }
+  __syncthreads();
+  block_exclusive_scan(L.cnt, t.n_tiles, L.wsum, nullptr);  // cnt -> start
+  for (int j = tid; j < n; j += blockDim.x) {
+    const uint32_t v = L.tr[j];
+    L.perm[L.cnt[v & 0xFFFFu] + (v >> 16)] = (uint16_t)j;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += blockDim.x) {
+    const int j = L.perm[i];
+    const uint32_t v = L.tr[j];
+    const int tile = (int)(v & 0xFFFFu);
+    const uint32_t dst = L.gbase[tile] + (v >> 16);
+    if (dst < t.cap[tile])
+      t.records[(size_t)t.region[tile] + dst] = L.rec[j];
+    else
+      overflow_splat(t, pal, tile, L.rec[j]);
+  }
+  __syncthreads();
+  if (tid == 0) *L.fill = 0u;
+  __syncthreads();
+}
+
+// The walker loop of iterate_kernel with the splat replaced by staging.
+// Rounds are uniform across the workgroup so every barrier is reached by
+// all threads; lanes past the last walker step nothing.
+template <bool COUNT_ONLY>
+__global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, TileArgs t) {
+  extern __shared__ uint32_t lds[];
+  LdsFlame F = load_lds<false>(a, lds);
+  uint32_t* p = lds + flame_table_floats(a.n_rows) + a.pal_size;
+  BinLds L;
+  L.rec = p;
+  L.tr = p + STAGE;
+  L.perm = reinterpret_cast<uint16_t*>(p + 2 * STAGE);
+  L.cnt = p + 2 * STAGE + STAGE / 2;
+  L.gbase = L.cnt + TILE_MAX;
+  L.wsum = L.gbase + TILE_MAX;
+  L.fill = L.wsum + 16;
+  const int tid = threadIdx.x;
+  if (COUNT_ONLY) {
+    for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
+  }
+  if (tid == 0) *L.fill = 0u;
+  __syncthreads();
+
+  const ViewConst v = a.view;
+  unsigned long long n_plotted = 0, n_in = 0;
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t rounds = (a.walkers + nthreads - 1) / nthreads;
+  for (uint64_t rd = 0; rd < rounds; rd++) {
+    const uint64_t wi = rd * nthreads + (uint64_t)blockIdx.x * blockDim.x + tid;
+    const bool active = wi < a.walkers;
+    const uint32_t walker = a.walker0 + (uint32_t)wi;
+    Rng rng = rng_init(walker, a.seed);
+    float x = rng_nextf(rng) * 2.0f - 1.0f;
+    float y = rng_nextf(rng) * 2.0f - 1.0f;
+    float ci = 0.0f;
+    uint32_t fuse = a.burn_in;
+    if (active && a.orbits && !a.fresh) {
+      float4 o = a.orbits[wi];
+      x = o.x;
+      y = o.y;
+      ci = o.z;
+      fuse = __float_as_uint(o.w);
+    }
+    for (uint32_t it = 0; it < a.ipt; it++) {
+      if (active) {
+        int xi, texel = 0;
+        int32_t px, py;
+        int st = chaos_step(F.fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, xi, px, py, texel);
+        n_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
+        if (st & ST_PLOT) {
+          n_in++;
+          const int tile = (py / TILE_H) * t.tiles_x + px / TILE_W;
+          if (COUNT_ONLY) {
+            atomicAdd(&L.cnt[tile], 1u);
+          } else {
+            const uint32_t local = (uint32_t)((py % TILE_H) * TILE_W + px % TILE_W);
+            // wave-compacted append: one LDS atomic per wave
+            const uint64_t m = __ballot(1);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            uint32_t base = 0;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            if ((int)(tid & 63) == leader) base = atomicAdd(L.fill, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            L.rec[base + rank] = local | ((uint32_t)texel << 13);
+            L.tr[base + rank] = (uint32_t)tile;
+          }
+        }
+      }
+      if (!COUNT_ONLY && ((it % CHECK_EVERY) == CHECK_EVERY - 1 || it + 1 == a.ipt)) {
+        __syncthreads();
+        const uint32_t f = *L.fill;
+        __syncthreads();
+        if (f >= (uint32_t)FLUSH_AT) flush_stage(t, L, F.pal_u32);
+      }
+    }
+    if (!COUNT_ONLY && active && a.orbits)
+      a.orbits[wi] = make_float4(x, y, ci, __uint_as_float(fuse));
+  }
+  __syncthreads();
+  if (COUNT_ONLY) {
+    for (int i = tid; i < t.n_tiles; i += blockDim.x)
+      if (L.cnt[i])
+        __hip_atomic_fetch_add(t.tile_fill + i, L.cnt[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    if (*L.fill > 0u) flush_stage(t, L, F.pal_u32);
+    for (int off = 32; off > 0; off >>= 1) {
+      n_plotted += __shfl_xor(n_plotted, off, 64);
+      n_in += __shfl_xor(n_in, off, 64);
+    }
+    if ((tid & 63) == 0) {
+      if (a.plotted && n_plotted)
+        __hip_atomic_fetch_add(a.plotted, n_plotted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.inframe && n_in)
+        __hip_atomic_fetch_add(a.inframe, n_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Capacities from the pre-pass counts: cap = est * scale + margin, regions
+// laid out back to back, scaled down to fit `buffer_records` if needed;
+// then the fill counters are cleared for the real pass. One workgroup.
+__global__ __launch_bounds__(BIN_BLOCK) void tile_capacity_kernel(TileArgs t, float scale,
+                                                                  uint32_t margin,
+                                                                  uint64_t buffer_records) {
+  __shared__ uint32_t a[TILE_MAX];
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t total;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) {
+    float c = (float)t.tile_fill[i] * scale;
+    a[i] = t.tile_fill[i] ? (uint32_t)fminf(c * 1.25f + (float)margin, 2.0e9f) : margin / 4u;
+  }
+  __syncthreads();
+  // total before scaling (64-bit safe: each entry < 2^31, at most 2048)
+  __shared__ unsigned long long sum64;
+  if (tid == 0) sum64 = 0ull;
+  __syncthreads();
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) atomicAdd(&sum64, (unsigned long long)a[i]);
+  __syncthreads();
+  if (sum64 > buffer_records) {
+    const double f = (double)buffer_records / (double)sum64;
+    for (int i = tid; i < t.n_tiles; i += blockDim.x) a[i] = (uint32_t)((double)a[i] * f);
+  }
+  __syncthreads();
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) t.cap[i] = a[i];
+  __syncthreads();
+  block_exclusive_scan(a, t.n_tiles, wsum, &total);
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) {
+    t.region[i] = a[i];
+    t.tile_fill[i] = 0u;
+  }
+}
+
+// Work list for tile_accum_kernel: tile i gets ceil(stored_i / SLICE) items.
+__global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
+  __shared__ uint32_t a[TILE_MAX];
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t total;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) {
+    const uint32_t stored = min(t.tile_fill[i], t.cap[i]);
+    a[i] = (stored + SLICE - 1) / SLICE;
+  }
+  __syncthreads();
+  block_exclusive_scan(a, t.n_tiles, wsum, &total);
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) t.item_start[i] = a[i];
+  if (tid == 0) t.item_start[t.n_tiles] = total;
+}
+
+// Accumulate a slice of one tile's records in LDS and fold it into the
+// histogram. Persistent grid over the work list.
+__global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const uint32_t* pal_g,
+                                                               int pal_size) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* tile = lds;                 // TILE_PX x 4
+  uint32_t* pal = lds + 4 * TILE_PX;    // pal_size
+  const int tid = threadIdx.x;
+  for (int i = tid; i < pal_size; i += blockDim.x) pal[i] = pal_g[i];
+  const uint32_t n_items = t.item_start[t.n_tiles];
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    // the tile whose item range holds `item` (uniform across the block)
+    int lo = 0, hi = t.n_tiles - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (t.item_start[mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const int ti = lo;
+    const uint32_t first = t.item_start[ti], n_slices = t.item_start[ti + 1] - first;
+    const uint32_t stored = min(t.tile_fill[ti], t.cap[ti]);
+    const uint32_t r0 = (item - first) * SLICE, r1 = min(r0 + SLICE, stored);
+    for (int i = tid; i < 4 * TILE_PX; i += blockDim.x) tile[i] = 0u;
+    __syncthreads();
+    const uint32_t* rec = t.records + t.region[ti];
+    for (uint32_t r = r0 + tid; r < r1; r += blockDim.x) {
+      const uint32_t v = rec[r];
+      const uint32_t local = v & 0x1FFFu;
+      const uint32_t pk = pal[v >> 13];
+      atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
+      atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
+      atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
+      atomicAdd(&tile[4 * local + 3], 100u);
+    }
+    __syncthreads();
+    const int tx = ti % t.tiles_x, ty = ti / t.tiles_x;
+    for (int q = tid; q < TILE_PX; q += blockDim.x) {
+      const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
+      if (px >= t.width || py >= t.height) continue;
+      const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
+      if (v.w == 0u) continue;
+      uint4* h = reinterpret_cast<uint4*>(t.hist) + ((size_t)py * t.width + px);
+      if (n_slices == 1) {
+        uint4 o = *h;
+        o.x += v.x;
+        o.y += v.y;
+        o.z += v.z;
+        o.w += v.w;
+        *h = o;
+      } else {
+        uint32_t* hw = reinterpret_cast<uint32_t*>(h);
+        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Kernels here use more than the default 64 KB of dynamic LDS.
+static hipError_t allow_lds(const void* fn) {
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_only, int grid,
+                              hipStream_t s) {
+  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&bin_iterate_kernel<true>));
+  static const hipError_t e1 = allow_lds(reinterpret_cast<const void*>(&bin_iterate_kernel<false>));
+  if (e0 != hipSuccess) return e0;
+  if (e1 != hipSuccess) return e1;
+  const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size);
+  if (count_only)
+    hipLaunchKernelGGL(bin_iterate_kernel<true>, dim3(grid), dim3(BIN_BLOCK), shm, s, a, t);
+  else
+    hipLaunchKernelGGL(bin_iterate_kernel<false>, dim3(grid), dim3(BIN_BLOCK), shm, s, a, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_capacity(const TileArgs& t, float scale, uint32_t margin,
+                                uint64_t buffer_records, hipStream_t s) {
+  hipLaunchKernelGGL(tile_capacity_kernel, dim3(1), dim3(BIN_BLOCK), 0, s, t, scale, margin,
+                     buffer_records);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_accum(const TileArgs& t, const uint32_t* pal, int pal_size, int grid,
+                             hipStream_t s) {
+  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&tile_accum_kernel));
+  if (e0 != hipSuccess) return e0;
+  hipLaunchKernelGGL(tile_plan_kernel, dim3(1), dim3(BIN_BLOCK), 0, s, t);
+  hipLaunchKernelGGL(tile_accum_kernel, dim3(grid), dim3(BIN_BLOCK), accum_lds_bytes(pal_size), s,
+                     t, pal, pal_size);
+  return hipGetLastError();
+}
+
+}  // namespace ffm

@@ -91,6 +91,23 @@ def test_xcd_private_histograms_equal_global(engine):
     assert np.array_equal(g0, g1)
 
 
+def test_lds_tile_splat_equals_oracle_and_global(engine):
+    """The binned LDS-tile splat gives the same integer histogram."""
+    say("lds tile splat: plain vs oracle")
+    fd, pal, job = configs.plain()
+    g, plotted, inframe = gpu_hist(engine, fd, pal, job.view, 1 << 16, 1000, 20, 1, mode=2)
+    o, oplotted = O.iterate_mt(fd, pal, job.view, 1 << 16, 1000, 20, 1, 8)
+    assert plotted == oplotted
+    assert np.array_equal(g, o)
+    for (w, h, walkers, ipt) in ((960, 540, 1 << 16, 64), (3840, 2160, 1 << 18, 64), (300, 200, 5000, 37)):
+        say(f"lds tile splat: single4k {w}x{h} {walkers}x{ipt}")
+        fd, pal, job = configs.single4k(walkers=walkers, ipt=ipt, width=w, height=h)
+        g0, p0, i0 = gpu_hist(engine, fd, pal, job.view, walkers, ipt, 20, 7, mode=0)
+        g2, p2, i2 = gpu_hist(engine, fd, pal, job.view, walkers, ipt, 20, 7, mode=2)
+        assert (p0, i0) == (p2, i2)
+        assert np.array_equal(g0, g2)
+
+
 def test_f32_histogram_matches_u32(engine):
     say("f32 splat")
     fd, pal, job = configs.plain(walkers=1 << 14, ipt=200)
@@ -113,10 +130,13 @@ def ulp(x):
 
 @pytest.mark.parametrize("cfg", ["single4k", "stress"])
 def test_step_parity_transcendental(engine, cfg):
-    """One iterator step on the GPU vs the oracle's step from the same state:
-    RNG words, transform choices and flags exact; x, y within 4 ulp of
-    max(|x|, |y|) of the step's output (the scale at which the step's
-    transcendental terms are summed) for 99.9% of steps, 64 ulp for all."""
+    """One iterator step on the GPU vs the oracle's step from the same state.
+    RNG words, transform choices, respawn flags and the colour index are
+    exact. x and y are within 4 ulp of the step's scale -- the largest of
+    |x|, |y| going in and coming out, the magnitude at which the step's
+    affine, variation and post-affine terms are summed -- for 99.9% of steps
+    and within 16 ulp for all. (Device and host transcendentals each round
+    within ~1 ulp of their own result; sums of such terms cancel.)"""
     say("step parity", cfg)
     fd, pal, job = getattr(configs, cfg)()
     errs = []
@@ -128,14 +148,15 @@ def test_step_parity_transcendental(engine, cfg):
         assert np.array_equal(g[:, 7] & 2, o[:, 7] & 2), "respawns"
         gx = g[:, 2:4].view(np.float32).astype(np.float64)
         ox = o[:, 2:4].view(np.float32).astype(np.float64)
-        scale = ulp(np.max(np.abs(ox), axis=1))[:, None]
-        ok = np.isfinite(ox).all(1)
-        errs.append((np.abs(gx - ox) / scale)[ok].max(1))
+        prev = np.vstack([np.zeros((1, 2)), gx[:-1]])
+        scale = ulp(np.maximum(np.abs(ox).max(1), np.abs(prev).max(1)))
+        ok = np.isfinite(ox).all(1) & np.isfinite(gx).all(1)
+        errs.append((np.abs(gx - ox).max(1) / scale)[ok])
         # colour index: a pure +,* chain
         assert np.array_equal(g[:, 4], o[:, 4])
     e = np.concatenate(errs)
     say(f"  ulp error: median {np.median(e):.2f}  p99.9 {np.quantile(e, 0.999):.2f}  max {e.max():.2f}")
-    assert np.quantile(e, 0.999) <= 4.0 and e.max() <= 64.0
+    assert np.quantile(e, 0.999) <= 4.0 and e.max() <= 16.0
 
 
 # -------------------------------------------------------------- stencils
