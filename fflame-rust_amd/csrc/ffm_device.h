@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ffm_fastmath.h"
 #include "ffm_internal.h"
 
 namespace ffm {
@@ -75,8 +76,8 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       oy = y;
       break;
     case FFM_VAR_SINUSOIDAL:
-      ox = sinf(x);
-      oy = sinf(y);
+      ox = ffm_sinf(x);
+      oy = ffm_sinf(y);
       break;
     case FFM_VAR_SPHERICAL: {
       float r2 = x * x + y * y + 1e-6f;
@@ -87,7 +88,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     case FFM_VAR_SWIRL: {
       float r2 = x * x + y * y;
       float s, c;
-      sincosf(r2, &s, &c);
+      ffm_sincosf(r2, &s, &c);
       ox = x * s - y * c;
       oy = x * c + y * s;
       break;
@@ -112,7 +113,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       float theta_pi = theta / PI_F;
       float pi_r = PI_F * r;
       float s, c;
-      sincosf(pi_r, &s, &c);
+      ffm_sincosf(pi_r, &s, &c);
       ox = theta_pi * s;
       oy = theta_pi * c;
       break;
@@ -130,15 +131,15 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       float omega = (rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
       float half_theta = theta / 2.0f + omega;
       float s, c;
-      sincosf(half_theta, &s, &c);
+      ffm_sincosf(half_theta, &s, &c);
       ox = sqrt_r * c;
       oy = sqrt_r * s;
       break;
     }
     case FFM_VAR_WAVES: {
       float b = row[XF_B], e = row[XF_E], d = row[XF_D], f = row[XF_F];
-      ox = x + b * sinf(y / (e * e + 1e-6f));
-      oy = y + d * sinf(x / (f * f + 1e-6f));
+      ox = x + b * ffm_sinf(y / (e * e + 1e-6f));
+      oy = y + d * ffm_sinf(x / (f * f + 1e-6f));
       break;
     }
     case FFM_VAR_JULIAN: {
@@ -150,7 +151,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       float trunc_val = floorf(fabsf(power) * rng_nextf(rng));
       float t = (theta + TWO_PI_F * trunc_val) / power;
       float s, c;
-      sincosf(t, &s, &c);
+      ffm_sincosf(t, &s, &c);
       ox = r * c;
       oy = r * s;
       break;

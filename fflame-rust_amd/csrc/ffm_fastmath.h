@@ -1,0 +1,69 @@
+// ffm_fastmath.h -- single-precision sin/cos for the walker's hot path.
+//
+// Method: Cody-Waite reduction by pi/2 with a three-part constant
+// (1.5703125 + 4.837512969970703125e-4 + 7.54978995489188216e-8, as in
+// Moshier's Cephes sinf/cosf, applied with fused multiply-adds), then the
+// Cephes minimax polynomials on [-pi/4, pi/4]:
+//   sin r = r + r^3 (-1.6666654611e-1 + r^2 (8.3321608736e-3 - r^2 1.9515295891e-4))
+//   cos r = 1 - r^2/2 + r^4 (4.166664568298827e-2 + r^2 (-1.388731625493765e-3
+//                                                  + r^2 2.443315711809948e-5))
+// and the quadrant k mod 4 selects sign and function. C1 has 9 significant
+// bits, so k*C1 is exact for |k| < 2^15: the fast path serves |x| <= 32768;
+// larger arguments (rare on the chaos game's attractors) take the library's
+// full-range sincosf in a separate non-inlined function, keeping its
+// Payne-Hanek reduction out of the hot loop's registers.
+//
+// The same text compiles as C (for a host accuracy check) and as HIP.
+#pragma once
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define FFM_FM_INLINE __device__ __forceinline__
+#define FFM_FM_NOINLINE __device__ __noinline__
+#else
+#include <math.h>
+#define FFM_FM_INLINE static inline
+#define FFM_FM_NOINLINE static
+#endif
+
+FFM_FM_NOINLINE void ffm_sincos_wide(float x, float* s, float* c) {
+#ifdef __HIPCC__
+  sincosf(x, s, c);
+#else
+  *s = sinf(x);
+  *c = cosf(x);
+#endif
+}
+
+FFM_FM_INLINE void ffm_sincos_reduced(float x, float* s, float* c) {
+  const float k = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-k, 1.5703125f, x);
+  r = fmaf(-k, 4.837512969970703125e-4f, r);
+  r = fmaf(-k, 7.54978995489188216e-8f, r);
+  const float r2 = r * r;
+  const float sp = fmaf(fmaf(fmaf(-1.9515295891e-4f, r2, 8.3321608736e-3f), r2,
+                             -1.6666654611e-1f),
+                        r2 * r, r);
+  const float cp = fmaf(fmaf(fmaf(2.443315711809948e-5f, r2, -1.388731625493765e-3f), r2,
+                             4.166664568298827e-2f),
+                        r2 * r2, fmaf(-0.5f, r2, 1.0f));
+  const int q = (int)k & 3;
+  const float ss = (q & 1) ? cp : sp;
+  const float cc = (q & 1) ? sp : cp;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+FFM_FM_INLINE void ffm_sincosf(float x, float* s, float* c) {
+  if (fabsf(x) <= 32768.0f) {
+    ffm_sincos_reduced(x, s, c);
+  } else {
+    ffm_sincos_wide(x, s, c);
+  }
+}
+
+FFM_FM_INLINE float ffm_sinf(float x) {
+  float s, c;
+  ffm_sincosf(x, &s, &c);
+  return s;
+}
