@@ -71,19 +71,24 @@ def main():
     for _ in range(args.warmup):
         eng.render_to_device(job, rgba)
     torch.cuda.synchronize()
-    it_ms = []
+    stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st = eng.render_to_device(job, rgba)
-        it_ms.append(st.iterate_ms)
+        stats.append(eng.render_to_device(job, rgba))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     samples = job.walkers * job.ipt
     ms_per_step = dt / args.steps * 1e3
     value = samples / (ms_per_step / 1e3)
-    iterate_ms = sum(it_ms) / len(it_ms)
-    achieved, frac, slots = roofline.iterate_roofline(fd, samples, iterate_ms / 1e3,
+    # the dominant kernel: the walker kernel, timed per launch with HIP events
+    # recorded around each launch on the stream it runs on
+    launches = sum(s.walker_launches for s in stats)
+    launch_ms = sum(s.walker_kernel_ms for s in stats) / launches
+    samples_per_launch = samples * args.steps / launches
+    achieved, frac, slots = roofline.iterate_roofline(fd, samples_per_launch, launch_ms / 1e3,
                                                       job.burn_in, job.ipt)
+    iterate_ms = sum(s.iterate_ms for s in stats) / args.steps
+    accum_ms = sum(s.tile_accum_ms for s in stats) / args.steps
     out = {
         "metric": "chaos-game samples/s at 4K",
         "value": value,
@@ -101,12 +106,15 @@ def main():
                                "3840x2160, 2^%d walkers x 256 iterations, density estimation "
                                "9->0 curve 0.4, log tonemap" % args.walkers_log2,
                    "splat_mode": args.splat_mode, "samples_per_step": samples},
-        "roofline": {"bound": "valu", "kernel": "iterate_kernel",
+        "roofline": {"bound": "valu",
+                     "kernel": "bin_iterate_kernel" if args.splat_mode == 2 else "iterate_kernel",
+                     "launch_ms": launch_ms, "samples_per_launch": samples_per_launch,
                      "achieved": achieved / 1e12, "peak": roofline.PEAK_SLOTS / 1e12,
                      "unit": "TFLOP/s", "frac": frac, "traffic": None,
                      "note": "VALU issue slots per lane (quarter-rate ops = 4); %.0f slots per "
                              "iteration; peak = 157.3 TFLOP/s FP32 / 4" % slots},
-        "kernel_ms": {"iterate": iterate_ms, "rest_of_step": ms_per_step - iterate_ms},
+        "kernel_ms": {"walkers": launch_ms * launches / args.steps, "tile_accum": accum_ms,
+                      "dispatch_phase": iterate_ms, "rest_of_step": ms_per_step - iterate_ms},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(fd, pal, job)

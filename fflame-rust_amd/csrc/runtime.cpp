@@ -60,6 +60,10 @@ struct ffm_ctx {
   bool t_planned = false;
   ViewConst t_view{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // event pairs around individual kernel launches of one dispatch
+  std::vector<hipEvent_t> kev;
+  std::vector<int> kev_kind;  // 0 walker kernel, 1 tile accumulate
+  size_t kev_used = 0;
 };
 
 namespace {
@@ -282,6 +286,37 @@ constexpr size_t LDS_PER_CU = 160 * 1024;
 // pre-pass over a subset of the first chunk (kept while the flame, size and
 // dispatch shape stay the same); per chunk: clear fills, binned iterate,
 // tile accumulate into `hist`.
+// Bracket a kernel launch with a pair of timing events (kind 0: walker
+// kernel, 1: tile accumulate); read back by collect_timings.
+hipError_t mark(ffm_ctx* c, hipStream_t s, int kind, bool begin) {
+  if (begin) {
+    while (c->kev.size() < c->kev_used + 2) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      c->kev.push_back(e);
+      c->kev_kind.push_back(0);
+    }
+    c->kev_kind[c->kev_used] = kind;
+    return hipEventRecord(c->kev[c->kev_used++], s);
+  }
+  return hipEventRecord(c->kev[c->kev_used++], s);
+}
+
+void collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, double& accum_ms) {
+  for (size_t i = 0; i + 1 < c->kev_used; i += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->kev[i], c->kev[i + 1]) != hipSuccess) continue;
+    if (c->kev_kind[i] == 0) {
+      walker_ms += ms;
+      launches++;
+    } else {
+      accum_ms += ms;
+    }
+  }
+  c->kev_used = 0;
+}
+
 int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s) {
   const int W = a.view.width, H = a.view.height;
   const int tiles_x = (W + TILE_W - 1) / TILE_W, tiles_y = (H + TILE_H - 1) / TILE_H;
@@ -340,8 +375,12 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s) {
     ch.walkers = std::min<uint64_t>(chunk, a.walkers - w0);
     if (a.orbits) ch.orbits = a.orbits + w0;
     HIPCHECK(c, launch_clear_u32(t.tile_fill, TILE_MAX, s));
+    HIPCHECK(c, mark(c, s, 0, true));
     HIPCHECK(c, launch_bin_iterate(ch, t, false, grid, s));
+    HIPCHECK(c, mark(c, s, 0, false));
+    HIPCHECK(c, mark(c, s, 1, true));
     HIPCHECK(c, launch_tile_accum(t, a.pal_u32, a.pal_size, grid, s));
+    HIPCHECK(c, mark(c, s, 1, false));
   }
   return FFM_OK;
 }
@@ -463,6 +502,7 @@ int ffm_ctx_destroy(ffm_ctx* c) {
                     &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
                     &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records})
     b->release();
+  for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -532,6 +572,7 @@ int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "splat mode %d not available here", splat_mode);
   (void)hipSetDevice(c->device);
   hipStream_t s = pick(c, stream);
+  c->kev_used = 0;
   IterArgs a = base_args(c, *view);
   a.walkers = walkers;
   a.ipt = ipt;
@@ -694,6 +735,9 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   const uint32_t stall_limit = job->burn_in / std::max(job->ipt, 1u) + 4;
   const uint32_t bs = job->batch_size ? job->batch_size : 4;
   float iterate_ms = 0.0f;
+  double walker_ms = 0.0, accum_ms = 0.0;
+  uint64_t launches = 0;
+  c->kev_used = 0;
   for (;;) {
     if (c->cancel.load()) return fail(c, FFM_ERR_CANCELLED, "Render cancelled");
     if (batch_count == 0) {
@@ -708,7 +752,9 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     if (mode == FFM_SPLAT_U32_LDS) {
       if ((st = lds_splat(c, a, reinterpret_cast<uint32_t*>(c->hist.p), s))) return st;
     } else {
+      HIPCHECK(c, mark(c, s, 0, true));
       HIPCHECK(c, launch_iterate(a, mode, grid, 256, s));
+      HIPCHECK(c, mark(c, s, 0, false));
     }
     HIPCHECK(c, hipEventRecord(c->ev1, s));
     uint64_t plotted;
@@ -747,6 +793,7 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     float ms = 0.0f;
     HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     iterate_ms += ms;
+    collect_timings(c, walker_ms, launches, accum_ms);
     if (done) break;
   }
   float* final_acc = reinterpret_cast<float*>(c->accum.p);
@@ -779,6 +826,9 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     stats->total_samples = total;
     stats->dispatches = k;
     stats->iterate_ms = iterate_ms;
+    stats->walker_kernel_ms = walker_ms;
+    stats->walker_launches = launches;
+    stats->tile_accum_ms = accum_ms;
     stats->render_time_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }

@@ -94,7 +94,9 @@ class Job(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("total_samples", C.c_uint64), ("dispatches", C.c_uint64),
-                ("render_time_ms", C.c_double), ("iterate_ms", C.c_double)]
+                ("render_time_ms", C.c_double), ("iterate_ms", C.c_double),
+                ("walker_kernel_ms", C.c_double), ("walker_launches", C.c_uint64),
+                ("tile_accum_ms", C.c_double)]
 
 
 def tonemap_defaults():

@@ -157,7 +157,10 @@ typedef struct {
   uint64_t total_samples;   /* plotted samples (RenderOutput::total_iterations) */
   uint64_t dispatches;
   double render_time_ms;
-  double iterate_ms;        /* GPU time in the iterate kernels */
+  double iterate_ms;        /* GPU time of the dispatch phase (walkers + splat) */
+  double walker_kernel_ms;  /* summed durations of the walker-kernel launches */
+  uint64_t walker_launches; /* number of walker-kernel launches (pre-pass excluded) */
+  double tile_accum_ms;     /* summed durations of the tile-accumulate launches */
 } ffm_render_stats;
 
 typedef struct ffm_ctx ffm_ctx;
