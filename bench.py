@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--splat-mode", type=int, default=2)
     ap.add_argument("--walkers-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 1 GiB device-to-device first (a known-size kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
@@ -68,6 +70,12 @@ def main():
     eng.set_flame(fd)
     eng.set_palette(pal, 256)
     rgba = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         eng.render_to_device(job, rgba)
     torch.cuda.synchronize()

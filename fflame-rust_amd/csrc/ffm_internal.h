@@ -78,12 +78,14 @@ struct TileArgs {
   uint32_t* cap;         // n_tiles: slots in the region
   uint32_t* item_start;  // n_tiles + 1: work list of tile_accum
   uint32_t* records;     // tile-local pixel | texel << 13
-  uint32_t* hist;        // the u32x4 histogram
+  uint32_t* hist;        // the u32x4 (or f32x4) histogram
+  const float* pal_f32;  // f32 splat: linear colour x 100 per texel, x3
   int32_t tiles_x, n_tiles, width, height;
+  int32_t f32;           // 1: f32 histogram (float4 accumulation)
 };
 
 size_t bin_lds_bytes(int n_rows, int pal_size);
-size_t accum_lds_bytes(int pal_size);
+size_t accum_lds_bytes(int pal_size, bool f32);
 hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_only, int grid,
                               hipStream_t s);
 hipError_t launch_tile_capacity(const TileArgs& t, float scale, uint32_t margin,

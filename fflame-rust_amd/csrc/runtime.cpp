@@ -317,14 +317,15 @@ void collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, double& 
   c->kev_used = 0;
 }
 
-int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s) {
+int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = false) {
   const int W = a.view.width, H = a.view.height;
   const int tiles_x = (W + TILE_W - 1) / TILE_W, tiles_y = (H + TILE_H - 1) / TILE_H;
   const int n_tiles = tiles_x * tiles_y;
   if (n_tiles > TILE_MAX)
     return fail(c, FFM_ERR_TOO_LARGE, "the LDS tile splat holds %d tiles of %dx%d; %dx%d needs %d",
                 TILE_MAX, TILE_W, TILE_H, W, H, n_tiles);
-  if (bin_lds_bytes(a.n_rows, a.pal_size) > LDS_PER_CU)
+  if (bin_lds_bytes(a.n_rows, a.pal_size) > LDS_PER_CU ||
+      accum_lds_bytes(a.pal_size, f32) > LDS_PER_CU)
     return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
   const uint64_t per_walker = (uint64_t)a.ipt + 1;
   uint64_t chunk = std::max<uint64_t>(1, (RECORD_BUDGET * 3 / 4) / per_walker);
@@ -344,6 +345,8 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s) {
   t.item_start = reinterpret_cast<uint32_t*>(c->t_items.p);
   t.records = reinterpret_cast<uint32_t*>(c->t_records.p);
   t.hist = hist;
+  t.pal_f32 = a.pal_f32;
+  t.f32 = f32 ? 1 : 0;
   t.tiles_x = tiles_x;
   t.n_tiles = n_tiles;
   t.width = W;
@@ -568,7 +571,7 @@ int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt
   if (walkers == 0 || walkers > 0xFFFFFFFFull)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers %llu outside 1..2^32-1",
                 (unsigned long long)walkers);
-  if (splat_mode < FFM_SPLAT_U32_GLOBAL || splat_mode > FFM_SPLAT_F32_GLOBAL)
+  if (splat_mode < FFM_SPLAT_U32_GLOBAL || splat_mode > FFM_SPLAT_F32_LDS)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "splat mode %d not available here", splat_mode);
   (void)hipSetDevice(c->device);
   hipStream_t s = pick(c, stream);
@@ -583,7 +586,8 @@ int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt
   a.plotted = reinterpret_cast<unsigned long long*>(plotted);
   a.inframe = reinterpret_cast<unsigned long long*>(inframe);
   const size_t words = (size_t)view->width * view->height * 4;
-  if (splat_mode == FFM_SPLAT_U32_LDS) return lds_splat(c, a, reinterpret_cast<uint32_t*>(hist), s);
+  if (splat_mode == FFM_SPLAT_U32_LDS || splat_mode == FFM_SPLAT_F32_LDS)
+    return lds_splat(c, a, reinterpret_cast<uint32_t*>(hist), s, splat_mode == FFM_SPLAT_F32_LDS);
   if (splat_mode == FFM_SPLAT_U32_XCD) {
     if ((st = ensure(c, c->copies, words * 4 * XCD_COPIES, "per-XCD histograms"))) return st;
     HIPCHECK(c, launch_clear_u32(c->copies.p, words * XCD_COPIES, s));
@@ -697,9 +701,10 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   if (job->walkers == 0 || job->walkers > 0xFFFFFFFFull || job->ipt == 0)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers and iterations per thread must be positive");
   const int mode = job->splat_mode;
-  if (mode < FFM_SPLAT_U32_GLOBAL || mode > FFM_SPLAT_F32_GLOBAL)
+  if (mode < FFM_SPLAT_U32_GLOBAL || mode > FFM_SPLAT_F32_LDS)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "splat mode %d not available here", mode);
-  if (mode == FFM_SPLAT_F32_GLOBAL && job->filter_radius > 0.0f)
+  const bool f32h = mode == FFM_SPLAT_F32_GLOBAL || mode == FFM_SPLAT_F32_LDS;
+  if (f32h && job->filter_radius > 0.0f)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "the histogram blur reads a u32 histogram");
   (void)hipSetDevice(c->device);
   c->cancel.store(0);
@@ -749,8 +754,8 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     a.seed = job->seed_base + (uint32_t)k * job->seed_stride;
     a.fresh = k == 0;
     HIPCHECK(c, hipEventRecord(c->ev0, s));
-    if (mode == FFM_SPLAT_U32_LDS) {
-      if ((st = lds_splat(c, a, reinterpret_cast<uint32_t*>(c->hist.p), s))) return st;
+    if (mode == FFM_SPLAT_U32_LDS || mode == FFM_SPLAT_F32_LDS) {
+      if ((st = lds_splat(c, a, reinterpret_cast<uint32_t*>(c->hist.p), s, f32h))) return st;
     } else {
       HIPCHECK(c, mark(c, s, 0, true));
       HIPCHECK(c, launch_iterate(a, mode, grid, 256, s));
@@ -782,7 +787,7 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
                                 reinterpret_cast<uint32_t*>(c->scratch.p), W, H, job->filter_radius,
                                 dsig, s));
       }
-      HIPCHECK(c, launch_accumulate(c->hist.p, mode == FFM_SPLAT_F32_GLOBAL,
+      HIPCHECK(c, launch_accumulate(c->hist.p, f32h,
                                     reinterpret_cast<float*>(c->accum.p),
                                     reinterpret_cast<float*>(c->accum.p), nullptr, W, H, 0, 0.0f, s));
       samples_in_buffer += batch_samples;

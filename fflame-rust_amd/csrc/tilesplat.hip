@@ -39,7 +39,9 @@ size_t bin_lds_bytes(int n_rows, int pal_size) {
   return flame + stage;
 }
 
-size_t accum_lds_bytes(int pal_size) { return (size_t)TILE_PX * 16 + (size_t)pal_size * 4; }
+size_t accum_lds_bytes(int pal_size, bool f32) {
+  return (size_t)TILE_PX * 16 + (size_t)pal_size * (f32 ? 12 : 4);
+}
 
 // Exclusive scan of a[0..n) (n <= 2 * blockDim.x) in place; all threads.
 __device__ void block_exclusive_scan(uint32_t* a, int n, uint32_t* wave_sums, uint32_t* total) {
@@ -86,8 +88,18 @@ __device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t
   const int tx = tile % t.tiles_x, ty = tile / t.tiles_x;
   const int local = (int)(rec & 0x1FFFu);
   const int px = tx * TILE_W + (local % TILE_W), py = ty * TILE_H + (local / TILE_W);
+  const size_t base = 4 * ((size_t)py * t.width + px);
+  if (t.f32) {
+    const float* c = t.pal_f32 + 3 * (rec >> 13);
+    float* hf = reinterpret_cast<float*>(t.hist) + base;
+    __hip_atomic_fetch_add(hf + 0, c[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(hf + 1, c[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(hf + 2, c[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(hf + 3, 100.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
   const uint32_t pk = pal[rec >> 13];
-  uint32_t* h = t.hist + 4 * ((size_t)py * t.width + px);
+  uint32_t* h = t.hist + base;
   __hip_atomic_fetch_add(h + 0, pk & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_fetch_add(h + 1, (pk >> 8) & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_fetch_add(h + 2, (pk >> 16) & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -286,14 +298,21 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
 }
 
 // Accumulate a slice of one tile's records in LDS and fold it into the
-// histogram. Persistent grid over the work list.
+// histogram. Persistent grid over the work list. F32: float4 accumulation
+// (LDS float atomics, f32 histogram).
+template <bool F32>
 __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const uint32_t* pal_g,
                                                                int pal_size) {
   extern __shared__ uint32_t lds[];
-  uint32_t* tile = lds;                 // TILE_PX x 4
-  uint32_t* pal = lds + 4 * TILE_PX;    // pal_size
+  uint32_t* tile = lds;               // TILE_PX x 4 (u32 or f32 bits)
+  float* tilef = reinterpret_cast<float*>(lds);
+  uint32_t* pal = lds + 4 * TILE_PX;  // pal_size u32, or 3 x pal_size f32
+  float* palf = reinterpret_cast<float*>(pal);
   const int tid = threadIdx.x;
-  for (int i = tid; i < pal_size; i += blockDim.x) pal[i] = pal_g[i];
+  if (F32)
+    for (int i = tid; i < 3 * pal_size; i += blockDim.x) palf[i] = t.pal_f32[i];
+  else
+    for (int i = tid; i < pal_size; i += blockDim.x) pal[i] = pal_g[i];
   const uint32_t n_items = t.item_start[t.n_tiles];
   for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     // the tile whose item range holds `item` (uniform across the block)
@@ -312,33 +331,61 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const
     for (uint32_t r = r0 + tid; r < r1; r += blockDim.x) {
       const uint32_t v = rec[r];
       const uint32_t local = v & 0x1FFFu;
-      const uint32_t pk = pal[v >> 13];
-      atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
-      atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
-      atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
-      atomicAdd(&tile[4 * local + 3], 100u);
+      if (F32) {
+        const float* c = palf + 3 * (v >> 13);
+        atomicAdd(&tilef[4 * local + 0], c[0]);
+        atomicAdd(&tilef[4 * local + 1], c[1]);
+        atomicAdd(&tilef[4 * local + 2], c[2]);
+        atomicAdd(&tilef[4 * local + 3], 100.0f);
+      } else {
+        const uint32_t pk = pal[v >> 13];
+        atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
+        atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
+        atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
+        atomicAdd(&tile[4 * local + 3], 100u);
+      }
     }
     __syncthreads();
     const int tx = ti % t.tiles_x, ty = ti / t.tiles_x;
     for (int q = tid; q < TILE_PX; q += blockDim.x) {
       const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
       if (px >= t.width || py >= t.height) continue;
-      const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
-      if (v.w == 0u) continue;
-      uint4* h = reinterpret_cast<uint4*>(t.hist) + ((size_t)py * t.width + px);
-      if (n_slices == 1) {
-        uint4 o = *h;
-        o.x += v.x;
-        o.y += v.y;
-        o.z += v.z;
-        o.w += v.w;
-        *h = o;
+      if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
+      const size_t p = (size_t)py * t.width + px;
+      if (F32) {
+        const float4 v = make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
+        float4* h = reinterpret_cast<float4*>(t.hist) + p;
+        if (n_slices == 1) {
+          float4 o = *h;
+          o.x += v.x;
+          o.y += v.y;
+          o.z += v.z;
+          o.w += v.w;
+          *h = o;
+        } else {
+          float* hw = reinterpret_cast<float*>(h);
+          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       } else {
-        uint32_t* hw = reinterpret_cast<uint32_t*>(h);
-        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
+        uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
+        if (n_slices == 1) {
+          uint4 o = *h;
+          o.x += v.x;
+          o.y += v.y;
+          o.z += v.z;
+          o.w += v.w;
+          *h = o;
+        } else {
+          uint32_t* hw = reinterpret_cast<uint32_t*>(h);
+          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
     __syncthreads();
@@ -373,11 +420,17 @@ hipError_t launch_tile_capacity(const TileArgs& t, float scale, uint32_t margin,
 
 hipError_t launch_tile_accum(const TileArgs& t, const uint32_t* pal, int pal_size, int grid,
                              hipStream_t s) {
-  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&tile_accum_kernel));
+  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&tile_accum_kernel<false>));
+  static const hipError_t e1 = allow_lds(reinterpret_cast<const void*>(&tile_accum_kernel<true>));
   if (e0 != hipSuccess) return e0;
+  if (e1 != hipSuccess) return e1;
   hipLaunchKernelGGL(tile_plan_kernel, dim3(1), dim3(BIN_BLOCK), 0, s, t);
-  hipLaunchKernelGGL(tile_accum_kernel, dim3(grid), dim3(BIN_BLOCK), accum_lds_bytes(pal_size), s,
-                     t, pal, pal_size);
+  if (t.f32)
+    hipLaunchKernelGGL(tile_accum_kernel<true>, dim3(grid), dim3(BIN_BLOCK),
+                       accum_lds_bytes(pal_size, true), s, t, pal, pal_size);
+  else
+    hipLaunchKernelGGL(tile_accum_kernel<false>, dim3(grid), dim3(BIN_BLOCK),
+                       accum_lds_bytes(pal_size, false), s, t, pal, pal_size);
   return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@ SPLAT_U32_GLOBAL = 0
 SPLAT_U32_XCD = 1
 SPLAT_U32_LDS = 2
 SPLAT_F32_GLOBAL = 3
+SPLAT_F32_LDS = 4
 
 # Reference registry indices (src/variations/defs/mod.rs:442-501).
 VARIATIONS = {

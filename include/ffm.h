@@ -117,7 +117,8 @@ typedef enum {
   FFM_SPLAT_U32_GLOBAL = 0, /* 4 x u32 device-scope atomics, as the reference */
   FFM_SPLAT_U32_XCD = 1,    /* u32, one private copy per XCD, folded after */
   FFM_SPLAT_U32_LDS = 2,    /* u32, LDS-privatised screen tiles flushed with integer atomics */
-  FFM_SPLAT_F32_GLOBAL = 3  /* 4 x f32 atomics ("float4 accumulation") */
+  FFM_SPLAT_F32_GLOBAL = 3, /* 4 x f32 atomics ("float4 accumulation") */
+  FFM_SPLAT_F32_LDS = 4     /* f32 through the LDS screen tiles (palette <= 2048 texels) */
 } ffm_splat_mode;
 
 /* Tonemap parameters: TonemapParams (shaders/tonemap.wgsl:9-63) as the

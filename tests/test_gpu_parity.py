@@ -50,14 +50,14 @@ def gpu_trace(engine, fd, pal, view, walker, seed, burn_in, steps):
 
 def gpu_hist(engine, fd, pal, view, walkers, ipt, burn_in, seed, mode=0):
     set_flame(engine, fd, pal)
-    dt = torch.float32 if mode == 3 else torch.int32
+    dt = torch.float32 if mode in (3, 4) else torch.int32
     h = torch.zeros((view.height, view.width, 4), dtype=dt, device=DEV)
     cnt = torch.zeros(2, dtype=torch.int64, device=DEV)
     engine.iterate(view, walkers, ipt, burn_in, seed, h, splat_mode=mode, plotted=cnt[0:1].data_ptr(),
                    inframe=cnt[1:2].data_ptr())
     torch.cuda.synchronize()
     c = cnt.cpu().numpy()
-    return (h.cpu().numpy() if mode == 3 else as_u32(h)), int(c[0]), int(c[1])
+    return (h.cpu().numpy() if mode in (3, 4) else as_u32(h)), int(c[0]), int(c[1])
 
 
 # ------------------------------------------------------------ bit-exact
@@ -106,6 +106,16 @@ def test_lds_tile_splat_equals_oracle_and_global(engine):
         g2, p2, i2 = gpu_hist(engine, fd, pal, job.view, walkers, ipt, 20, 7, mode=2)
         assert (p0, i0) == (p2, i2)
         assert np.array_equal(g0, g2)
+
+
+def test_f32_lds_tile_splat_matches_f32_atomics(engine):
+    say("f32 lds tile splat")
+    fd, pal, job = configs.single4k(walkers=1 << 16, ipt=64, width=960, height=540)
+    g3, p3, i3 = gpu_hist(engine, fd, pal, job.view, 1 << 16, 64, 20, 3, mode=3)
+    g4, p4, i4 = gpu_hist(engine, fd, pal, job.view, 1 << 16, 64, 20, 3, mode=4)
+    assert (p3, i3) == (p4, i4)
+    assert np.array_equal(g3[..., 3], g4[..., 3])  # sums of 100.0: exact
+    np.testing.assert_allclose(g4[..., :3], g3[..., :3], rtol=2e-6, atol=1e-3)
 
 
 def test_f32_histogram_matches_u32(engine):
