@@ -43,6 +43,22 @@ def cpu_baseline(fd, pal, job, seconds=10.0):
                       f"{v.width}x{v.height} (iterate + splat), {dt:.1f} s"}
 
 
+def recorded_traffic(kernel, splat_mode, walkers):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), when it
+    was measured on this exact configuration; else None."""
+    path = os.path.join(ROOT, "profiles", "bin_iterate_single4k_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("TRAFFIC_JSON "):
+                d = json.loads(line[len("TRAFFIC_JSON "):])
+                if (d["kernel"], d["splat_mode"], d["walkers"]) == (kernel, splat_mode, walkers):
+                    return d["bytes_per_launch"]
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +112,8 @@ def main():
     achieved, frac, slots = roofline.iterate_roofline(fd, samples_per_launch, launch_ms / 1e3,
                                                       job.burn_in, job.ipt)
     iterate_ms = sum(s.iterate_ms for s in stats) / args.steps
+    kname = "bin_iterate_kernel" if args.splat_mode in (2, 4) else "iterate_kernel"
+    traffic = recorded_traffic(kname, args.splat_mode, job.walkers)
     accum_ms = sum(s.tile_accum_ms for s in stats) / args.steps
     out = {
         "metric": "chaos-game samples/s at 4K",
@@ -115,10 +133,10 @@ def main():
                                "9->0 curve 0.4, log tonemap" % args.walkers_log2,
                    "splat_mode": args.splat_mode, "samples_per_step": samples},
         "roofline": {"bound": "valu",
-                     "kernel": "bin_iterate_kernel" if args.splat_mode == 2 else "iterate_kernel",
+                     "kernel": kname,
                      "launch_ms": launch_ms, "samples_per_launch": samples_per_launch,
                      "achieved": achieved / 1e12, "peak": roofline.PEAK_SLOTS / 1e12,
-                     "unit": "TFLOP/s", "frac": frac, "traffic": None,
+                     "unit": "TFLOP/s", "frac": frac, "traffic": traffic,
                      "note": "VALU issue slots per lane (quarter-rate ops = 4); %.0f slots per "
                              "iteration; peak = 157.3 TFLOP/s FP32 / 4" % slots},
         "kernel_ms": {"walkers": launch_ms * launches / args.steps, "tile_accum": accum_ms,
