@@ -62,6 +62,9 @@ def recorded_traffic(kernel, splat_mode, walkers):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="single4k", choices=["single4k", "gallery", "plain"],
+                    help="single4k (the headline), gallery (batched vs one by one), plain")
+    ap.add_argument("--gallery-size", type=int, default=512)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--splat-mode", type=int, default=2)
@@ -80,7 +83,59 @@ def main():
     import pyfae
     from pyfae import configs, roofline
 
+    if args.workload == "gallery":
+        return bench_gallery(args)
+    if args.workload == "plain":
+        fd, pal, job = configs.plain()
+        job.splat_mode = args.splat_mode
+        args.walkers_log2 = 16
+        return bench_flame(args, fd, pal, job, "plain: 3 linear xforms, 512x512, 2^16 walkers x "
+                           "1000 iterations, log tonemap", "chaos-game samples/s (plain 512x512)")
     fd, pal, job = configs.single4k(walkers=1 << args.walkers_log2, splat_mode=args.splat_mode)
+    return bench_flame(args, fd, pal, job,
+                       "single4k: 6 xforms + final xform, 3 of 10 variations each, 3840x2160, "
+                       "2^%d walkers x 256 iterations, density estimation 9->0 curve 0.4, log "
+                       "tonemap" % args.walkers_log2, "chaos-game samples/s at 4K")
+
+
+def bench_gallery(args):
+    """Batched gallery: flames/s of one batched render (persistent launches in a
+    hipGraph) against the same flames rendered one by one."""
+    import pyfae
+    from pyfae import configs
+    items = configs.gallery(n=args.gallery_size)
+    eng = pyfae.Engine(0)
+    for _ in range(max(args.warmup, 1)):  # the first call autotunes and builds the graph
+        eng.batch_render(items)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, st = eng.batch_render(items)
+    dt = (time.perf_counter() - t0) / args.steps
+    t1 = time.perf_counter()
+    _, st_s = eng.batch_render(items, serial=True)
+    dt_s = time.perf_counter() - t1
+    n = len(items)
+    samples = st.total_samples
+    px = sum(j.view.width * j.view.height for _, _, j in items)
+    eng.close()
+    out = {
+        "metric": "batched flames/s (gallery of %d)" % n,
+        "value": n / dt, "unit": "flames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "gallery: %d flames, 2..12 xforms, sides log-uniform 128..1024, "
+                               "200 samples/pixel, ipt 256" % n,
+                   "pixels": px, "samples_per_step": samples},
+        "serial_flames_per_s": n / dt_s, "serial_ms": dt_s * 1e3,
+        "speedup_vs_serial": dt_s / dt, "samples_per_s": samples / dt,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_flame(args, fd, pal, job, workload, metric):
+    import torch
+    import pyfae
+    from pyfae import roofline
     W, H = job.view.width, job.view.height
     eng = pyfae.Engine(0)
     eng.set_flame(fd)
@@ -116,7 +171,7 @@ def main():
     traffic = recorded_traffic(kname, args.splat_mode, job.walkers)
     accum_ms = sum(s.tile_accum_ms for s in stats) / args.steps
     out = {
-        "metric": "chaos-game samples/s at 4K",
+        "metric": metric,
         "value": value,
         "unit": "samples/s",
         "n_gpus": 1,
@@ -128,10 +183,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "single4k: 6 xforms + final xform, 3 of 10 variations each, "
-                               "3840x2160, 2^%d walkers x 256 iterations, density estimation "
-                               "9->0 curve 0.4, log tonemap" % args.walkers_log2,
-                   "splat_mode": args.splat_mode, "samples_per_step": samples},
+        "config": {"workload": workload, "splat_mode": args.splat_mode,
+                   "samples_per_step": samples},
         "roofline": {"bound": "valu",
                      "kernel": kname,
                      "launch_ms": launch_ms, "samples_per_launch": samples_per_launch,

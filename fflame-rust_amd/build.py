@@ -56,7 +56,7 @@ def build(verbose=False):
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(OBJ_DIR, src + ".o")
-        _run([hipcc, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj])
+        _run([hipcc, *COMMON, "--offload-arch=gfx950", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])

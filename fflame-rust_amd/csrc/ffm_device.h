@@ -307,18 +307,19 @@ struct LdsFlame {
 
 // Copy the flame table and palette into LDS; returns the LDS views.
 template <bool F32PAL>
-__device__ __forceinline__ LdsFlame load_lds(const IterArgs& a, uint32_t* lds) {
-  const FlameHeader* hdr = reinterpret_cast<const FlameHeader*>(a.flame);
+__device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_t* pal_u32,
+                                               const float* pal_f32, int pal_size, uint32_t* lds) {
+  const FlameHeader* hdr = reinterpret_cast<const FlameHeader*>(table);
   const int nx = hdr->n_xforms, nf = hdr->n_finals, nv = hdr->n_vars;
   const int rows_f = (nx + nf) * XF_STRIDE;
   const int hdr_w = (int)(sizeof(FlameHeader) / 4);
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(a.flame);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(table);
   for (int i = threadIdx.x; i < hdr_w + rows_f; i += blockDim.x) lds[i] = src[i];
   uint32_t* pal = lds + hdr_w + rows_f;
-  for (int i = threadIdx.x; i < a.pal_size; i += blockDim.x) pal[i] = a.pal_u32[i];
-  float* palf = reinterpret_cast<float*>(pal + a.pal_size);
+  for (int i = threadIdx.x; i < pal_size; i += blockDim.x) pal[i] = pal_u32[i];
+  float* palf = reinterpret_cast<float*>(pal + pal_size);
   if (F32PAL)
-    for (int i = threadIdx.x; i < 3 * a.pal_size; i += blockDim.x) palf[i] = a.pal_f32[i];
+    for (int i = threadIdx.x; i < 3 * pal_size; i += blockDim.x) palf[i] = pal_f32[i];
   __syncthreads();
   LdsFlame l;
   l.fv.rows = reinterpret_cast<const float*>(lds + hdr_w);
@@ -329,6 +330,11 @@ __device__ __forceinline__ LdsFlame load_lds(const IterArgs& a, uint32_t* lds) {
   l.pal_u32 = pal;
   l.pal_f32 = palf;
   return l;
+}
+
+template <bool F32PAL>
+__device__ __forceinline__ LdsFlame load_lds(const IterArgs& a, uint32_t* lds) {
+  return load_table<F32PAL>(a.flame, a.pal_u32, a.pal_f32, a.pal_size, lds);
 }
 
 }  // namespace ffm

@@ -82,6 +82,7 @@ struct TileArgs {
   const float* pal_f32;  // f32 splat: linear colour x 100 per texel, x3
   int32_t tiles_x, n_tiles, width, height;
   int32_t f32;           // 1: f32 histogram (float4 accumulation)
+  int32_t flush_every;   // iterations between staging-fill checks (1..8)
 };
 
 size_t bin_lds_bytes(int n_rows, int pal_size);
@@ -120,5 +121,47 @@ struct TonemapConst {
 };
 hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H,
                           const TonemapConst& tc, hipStream_t s);
+
+// ---- batched gallery: many flames in one persistent launch
+
+struct BatchFlame {
+  uint32_t table_off;  // floats into the concatenated flame tables
+  uint32_t pal_off;    // texels into the concatenated palettes
+  int32_t n_rows, pal_size;
+  ViewConst view;
+  uint32_t ipt, burn_in, seed;
+  uint32_t tile_base;  // first of this flame's tiles in the global tile arrays
+  int32_t tiles_x, n_tiles;
+  uint64_t hist_off;   // pixels into the concatenated histograms
+};
+
+struct BatchItem {
+  uint32_t flame, w0, w1, pad;  // walkers [w0, w1) of one flame
+};
+
+struct BatchArgs {
+  const BatchFlame* flames;
+  int32_t n_flames;
+  const BatchItem* items;
+  int32_t n_items;
+  const float* tables;
+  const uint32_t* pal_u32;
+  int32_t max_rows, max_pal;
+  uint32_t count_div;  // pre-pass walks 1/count_div of every item
+  TileArgs g;          // global tile arrays; g.hist = base of the histograms
+};
+
+struct BatchPix {
+  uint64_t pix_off;  // pixels into the concatenated histograms / images
+  uint64_t npix;
+  TonemapConst tc;
+};
+
+hipError_t launch_bin_batch(const BatchArgs& b, bool count_only, int grid, hipStream_t s);
+hipError_t launch_tile_capacity_n(const TileArgs& t, float scale, uint32_t margin,
+                                  uint64_t buffer_records, hipStream_t s);
+hipError_t launch_tile_accum_batch(const BatchArgs& b, int grid, hipStream_t s);
+hipError_t launch_batch_finish(const uint32_t* hist, uint8_t* rgba, const BatchPix* fl, int n,
+                               hipStream_t s);
 
 }  // namespace ffm

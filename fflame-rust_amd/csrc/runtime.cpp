@@ -64,6 +64,7 @@ struct ffm_ctx {
   std::vector<hipEvent_t> kev;
   std::vector<int> kev_kind;  // 0 walker kernel, 1 tile accumulate
   size_t kev_used = 0;
+  struct ffm_batch_state* batch = nullptr;
 };
 
 namespace {
@@ -351,6 +352,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   t.n_tiles = n_tiles;
   t.width = W;
   t.height = H;
+  t.flush_every = 4;
   const int grid = std::max(c->num_cu, 1);
   const bool same = c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
                     c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
@@ -472,6 +474,8 @@ TonemapConst make_tonemap(const ffm_tonemap_params& tp, int W, int H, float zoom
 
 }  // namespace
 
+void free_batch(ffm_batch_state* b);
+
 extern "C" {
 
 int ffm_abi_version(void) { return FFM_ABI_VERSION; }
@@ -505,6 +509,7 @@ int ffm_ctx_destroy(ffm_ctx* c) {
                     &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
                     &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records})
     b->release();
+  free_batch(c->batch);
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -855,15 +860,390 @@ int ffm_render_accum(ffm_ctx* c, const ffm_job* job, float* accum_out, ffm_rende
   return render_impl(c, job, nullptr, accum_out, stats);
 }
 
-int ffm_batch_render(ffm_ctx* c, int, const ffm_flame_desc*, const float* const*, const int*,
-                     const ffm_job*, uint8_t* const*, ffm_render_stats*) {
-  return fail(c, FFM_ERR_INVALID_ARGUMENT, "batched rendering is not built yet");
+// ---- batched gallery ---------------------------------------------------
+
+namespace {
+
+constexpr uint64_t BATCH_RECORD_BUDGET = 1ull << 31;  // records staged per wave (8 GB)
+constexpr int BATCH_PALETTE = 256;
+constexpr uint32_t BATCH_COUNT_DIV = 8;  // the pre-pass walks 1/8 of wave 0
+
+struct BatchPlan {
+  // what the graph was built for
+  std::vector<uint64_t> sig;
+  int n = 0, n_waves = 0, max_rows = 0, n_tiles = 0;
+  uint64_t total_pix = 0;
+  uint32_t item_walkers = 0;
+  int flush_every = 4;
+  std::vector<uint32_t> wave_start;  // items of wave w: [wave_start[w], wave_start[w+1])
+  std::vector<BatchPix> pix;
+  std::vector<uint64_t> samples;     // plotted samples per flame
+  DevBuf flames, items, tables, pals, pixd, hist, rgba, fill, region, cap, item_start, records;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  void release() {
+    drop_graph();
+    for (DevBuf* b : {&flames, &items, &tables, &pals, &pixd, &hist, &rgba, &fill, &region, &cap,
+                      &item_start, &records})
+      b->release();
+  }
+};
+
+struct Tuning {
+  std::vector<uint64_t> sig;
+  uint32_t item_walkers;
+  int flush_every;
+};
+
+}  // namespace
+
+struct ffm_batch_state {
+  BatchPlan plan;
+  std::vector<Tuning> tuned;
+};
+
+namespace {
+
+ffm_batch_state* batch_state(ffm_ctx* c);
+
+int check_batch(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* const* stops,
+                const int* n_stops, const ffm_job* jobs, uint8_t* const* rgba_out) {
+  if (n < 1) return fail(c, FFM_ERR_INVALID_ARGUMENT, "an empty batch");
+  if (!flames || !stops || !n_stops || !jobs || !rgba_out)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "batch arrays must not be NULL");
+  for (int i = 0; i < n; i++) {
+    int st = check_flame(c, &flames[i]);
+    if (st) return st;
+    if ((st = check_view(c, &jobs[i].view))) return st;
+    if (!stops[i] || n_stops[i] < 1) return fail(c, FFM_ERR_NO_PALETTE, "flame %d: No palette found", i);
+    if (!rgba_out[i]) return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame %d: pixel buffer is NULL", i);
+    const ffm_job& j = jobs[i];
+    if (j.walkers == 0 || j.walkers > 0xFFFFFFFFull || j.ipt == 0)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame %d: walkers and ipt must be positive", i);
+    if (j.filter_radius > 0.0f || j.de_max_radius > 0.0f)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                  "flame %d: blur and density estimation are not part of the batched path", i);
+    const int tiles = ((j.view.width + TILE_W - 1) / TILE_W) * ((j.view.height + TILE_H - 1) / TILE_H);
+    if (tiles > TILE_MAX)
+      return fail(c, FFM_ERR_TOO_LARGE, "flame %d: %dx%d is too large for the batched path", i,
+                  j.view.width, j.view.height);
+  }
+  return FFM_OK;
 }
 
-int ffm_batch_render_serial(ffm_ctx* c, int, const ffm_flame_desc*, const float* const*,
-                            const int*, const ffm_job*, uint8_t* const*, ffm_render_stats*) {
-  return fail(c, FFM_ERR_INVALID_ARGUMENT, "batched rendering is not built yet");
+std::vector<uint64_t> batch_signature(int n, const ffm_flame_desc* flames, const ffm_job* jobs) {
+  std::vector<uint64_t> sig;
+  sig.push_back((uint64_t)n);
+  for (int i = 0; i < n; i++) {
+    sig.push_back(((uint64_t)jobs[i].view.width << 32) | (uint32_t)jobs[i].view.height);
+    sig.push_back(jobs[i].walkers);
+    sig.push_back(((uint64_t)jobs[i].ipt << 32) | jobs[i].burn_in);
+    sig.push_back((uint64_t)(flames[i].n_xforms + flames[i].n_finals));
+  }
+  return sig;
 }
+
+// Lay out the batch (buffers, waves, items) and capture its launches in a
+// graph. Flame tables and palettes are uploaded separately, every call.
+int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, const ffm_job* jobs,
+                uint32_t item_walkers, int flush_every) {
+  P.drop_graph();
+  P.n = n;
+  P.item_walkers = item_walkers;
+  P.flush_every = flush_every;
+  std::vector<BatchFlame> bf(n);
+  uint64_t tile_base = 0, pix_off = 0, table_off = 0, total_samples = 0;
+  int max_rows = 0;
+  P.pix.assign(n, BatchPix{});
+  P.samples.assign(n, 0);
+  for (int i = 0; i < n; i++) {
+    const ffm_job& j = jobs[i];
+    const int rows = flames[i].n_xforms + flames[i].n_finals;
+    max_rows = std::max(max_rows, rows);
+    BatchFlame& f = bf[i];
+    f.table_off = (uint32_t)table_off;
+    f.pal_off = (uint32_t)i * BATCH_PALETTE;
+    f.n_rows = rows;
+    f.pal_size = BATCH_PALETTE;
+    f.view = make_view(j.view);
+    f.ipt = j.ipt;
+    f.burn_in = j.burn_in;
+    f.seed = j.seed_base;  // one dispatch per flame: seed_0
+    f.tiles_x = (j.view.width + TILE_W - 1) / TILE_W;
+    f.n_tiles = f.tiles_x * ((j.view.height + TILE_H - 1) / TILE_H);
+    f.tile_base = (uint32_t)tile_base;
+    f.hist_off = pix_off;
+    const uint64_t npix = (uint64_t)j.view.width * j.view.height;
+    P.pix[i].pix_off = pix_off;
+    P.pix[i].npix = npix;
+    P.samples[i] = j.walkers * (uint64_t)(j.ipt > j.burn_in ? j.ipt - j.burn_in : 0);
+    P.pix[i].tc = make_tonemap(j.tonemap, j.view.width, j.view.height, j.view.zoom,
+                               (double)P.samples[i]);
+    tile_base += (uint64_t)f.n_tiles;
+    pix_off += npix;
+    table_off += flame_table_floats(rows);
+    total_samples += j.walkers * (uint64_t)(j.ipt + 1);
+  }
+  P.max_rows = max_rows;
+  P.n_tiles = (int)tile_base;
+  P.total_pix = pix_off;
+  // waves: each a proportional slice of every flame, sized to the record budget
+  P.n_waves = (int)std::max<uint64_t>(1, (total_samples * 5 / 4 + BATCH_RECORD_BUDGET - 1) /
+                                             BATCH_RECORD_BUDGET);
+  std::vector<BatchItem> items;
+  P.wave_start.assign(1, 0);
+  for (int w = 0; w < P.n_waves; w++) {
+    for (int i = 0; i < n; i++) {
+      const uint64_t W = jobs[i].walkers;
+      const uint64_t a = W * w / P.n_waves, b = W * (w + 1) / P.n_waves;
+      for (uint64_t x = a; x < b; x += item_walkers) {
+        BatchItem it{(uint32_t)i, (uint32_t)x, (uint32_t)std::min<uint64_t>(b, x + item_walkers), 0u};
+        items.push_back(it);
+      }
+    }
+    P.wave_start.push_back((uint32_t)items.size());
+  }
+  int st;
+  if ((st = ensure(c, P.flames, bf.size() * sizeof(BatchFlame), "batch flames"))) return st;
+  if ((st = ensure(c, P.items, items.size() * sizeof(BatchItem), "batch items"))) return st;
+  if ((st = ensure(c, P.tables, table_off * 4, "batch tables"))) return st;
+  if ((st = ensure(c, P.pals, (size_t)n * BATCH_PALETTE * 4, "batch palettes"))) return st;
+  if ((st = ensure(c, P.pixd, P.pix.size() * sizeof(BatchPix), "batch images"))) return st;
+  if ((st = ensure(c, P.hist, P.total_pix * 16, "batch histograms"))) return st;
+  if ((st = ensure(c, P.rgba, P.total_pix * 4, "batch pixels"))) return st;
+  const size_t tiles_pad = ((size_t)P.n_tiles + 8) & ~(size_t)3;
+  if ((st = ensure(c, P.fill, tiles_pad * 4, "batch tiles"))) return st;
+  if ((st = ensure(c, P.region, tiles_pad * 4, "batch tiles"))) return st;
+  if ((st = ensure(c, P.cap, tiles_pad * 4, "batch tiles"))) return st;
+  if ((st = ensure(c, P.item_start, tiles_pad * 4, "batch tiles"))) return st;
+  if ((st = ensure(c, P.records, BATCH_RECORD_BUDGET * 4, "batch records"))) return st;
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipMemcpyAsync(P.flames.p, bf.data(), bf.size() * sizeof(BatchFlame),
+                             hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(P.items.p, items.data(), items.size() * sizeof(BatchItem),
+                             hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(P.pixd.p, P.pix.data(), P.pix.size() * sizeof(BatchPix),
+                             hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipStreamSynchronize(s));
+
+  // the launch sequence, captured once
+  BatchArgs b{};
+  b.flames = reinterpret_cast<const BatchFlame*>(P.flames.p);
+  b.n_flames = n;
+  b.tables = reinterpret_cast<const float*>(P.tables.p);
+  b.pal_u32 = reinterpret_cast<const uint32_t*>(P.pals.p);
+  b.max_rows = max_rows;
+  b.max_pal = BATCH_PALETTE;
+  b.count_div = BATCH_COUNT_DIV;
+  b.g.tile_fill = reinterpret_cast<uint32_t*>(P.fill.p);
+  b.g.region = reinterpret_cast<uint32_t*>(P.region.p);
+  b.g.cap = reinterpret_cast<uint32_t*>(P.cap.p);
+  b.g.item_start = reinterpret_cast<uint32_t*>(P.item_start.p);
+  b.g.records = reinterpret_cast<uint32_t*>(P.records.p);
+  b.g.hist = reinterpret_cast<uint32_t*>(P.hist.p);
+  b.g.n_tiles = P.n_tiles;
+  b.g.flush_every = flush_every;
+  const int grid = std::max(c->num_cu, 1);
+  const BatchItem* items_d = reinterpret_cast<const BatchItem*>(P.items.p);
+  HIPCHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  auto body = [&]() -> hipError_t {
+    hipError_t e;
+    if ((e = launch_clear_u32(P.hist.p, P.total_pix * 4, s)) != hipSuccess) return e;
+    if ((e = launch_clear_u32(P.fill.p, tiles_pad, s)) != hipSuccess) return e;
+    BatchArgs w0 = b;
+    w0.items = items_d;
+    w0.n_items = (int32_t)P.wave_start[1];
+    if ((e = launch_bin_batch(w0, true, grid, s)) != hipSuccess) return e;
+    if ((e = launch_tile_capacity_n(b.g, (float)BATCH_COUNT_DIV, 1024, BATCH_RECORD_BUDGET, s)) !=
+        hipSuccess)
+      return e;
+    for (int w = 0; w < P.n_waves; w++) {
+      BatchArgs bw = b;
+      bw.items = items_d + P.wave_start[w];
+      bw.n_items = (int32_t)(P.wave_start[w + 1] - P.wave_start[w]);
+      if (w > 0 && (e = launch_clear_u32(P.fill.p, tiles_pad, s)) != hipSuccess) return e;
+      if ((e = launch_bin_batch(bw, false, grid, s)) != hipSuccess) return e;
+      if ((e = launch_tile_accum_batch(bw, grid, s)) != hipSuccess) return e;
+    }
+    return launch_batch_finish(reinterpret_cast<const uint32_t*>(P.hist.p),
+                               reinterpret_cast<uint8_t*>(P.rgba.p),
+                               reinterpret_cast<const BatchPix*>(P.pixd.p), n, s);
+  };
+  hipError_t eb = body();
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(s, &g);
+  if (eb != hipSuccess || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(c, FFM_ERR_DEVICE, "capturing the batch: %s",
+                hipGetErrorString(eb != hipSuccess ? eb : ee));
+  }
+  P.graph = g;
+  HIPCHECK(c, hipGraphInstantiate(&P.exec, g, nullptr, nullptr, 0));
+  P.sig = batch_signature(n, flames, jobs);
+  return FFM_OK;
+}
+
+// Per-call uploads: flame tables and palettes (their content may change
+// between calls with the same layout).
+int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const float* const* stops,
+                 const int* n_stops) {
+  std::vector<float> tables;
+  std::vector<uint32_t> pals((size_t)P.n * BATCH_PALETTE);
+  std::vector<float> lin;
+  std::vector<uint32_t> pk;
+  for (int i = 0; i < P.n; i++) {
+    ffm_flame_desc* f = new ffm_flame_desc(flames[i]);
+    prepare_flame(f);
+    std::vector<float> t = pack_flame(f);
+    delete f;
+    tables.insert(tables.end(), t.begin(), t.end());
+    palette_tables(stops[i], n_stops[i], BATCH_PALETTE, pk, lin);
+    std::copy(pk.begin(), pk.end(), pals.begin() + (size_t)i * BATCH_PALETTE);
+  }
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipMemcpyAsync(P.tables.p, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(P.pals.p, pals.data(), pals.size() * 4, hipMemcpyHostToDevice, s));
+  return FFM_OK;
+}
+
+int run_batch_graph(ffm_ctx* c, BatchPlan& P, float* gpu_ms) {
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipEventRecord(c->ev0, s));
+  HIPCHECK(c, hipGraphLaunch(P.exec, s));
+  HIPCHECK(c, hipEventRecord(c->ev1, s));
+  HIPCHECK(c, hipEventSynchronize(c->ev1));
+  HIPCHECK(c, hipEventElapsedTime(gpu_ms, c->ev0, c->ev1));
+  return FFM_OK;
+}
+
+}  // namespace
+
+// Many flames in one persistent launch per wave, the whole sequence a
+// hipGraph. The first call for a batch layout autotunes walkers per work
+// item (walkers per lane = item / 1024) and the staging flush cadence on
+// the batch itself, keeping the fastest; the graph is reused while the
+// layout (sizes, walker counts, transform counts) stays the same.
+int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* const* stops,
+                     const int* n_stops, const ffm_job* jobs, uint8_t* const* rgba_out,
+                     ffm_render_stats* stats) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  int st = check_batch(c, n, flames, stops, n_stops, jobs, rgba_out);
+  if (st) return st;
+  (void)hipSetDevice(c->device);
+  auto t0 = std::chrono::steady_clock::now();
+  ffm_batch_state* B = batch_state(c);
+  BatchPlan& P = B->plan;
+  const std::vector<uint64_t> sig = batch_signature(n, flames, jobs);
+  const Tuning* tune = nullptr;
+  for (const Tuning& t : B->tuned)
+    if (t.sig == sig) tune = &t;
+  float gpu_ms = 0.0f;
+  if (!tune) {
+    static const uint32_t kItems[] = {16384, 65536};
+    static const int kFlush[] = {2, 4, 8};
+    float best = 1e30f;
+    Tuning tb{sig, 65536, 4};
+    for (uint32_t iw : kItems)
+      for (int fe : kFlush) {
+        if ((st = build_batch(c, P, n, flames, jobs, iw, fe))) return st;
+        if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
+        float ms = 0.0f;
+        if ((st = run_batch_graph(c, P, &ms))) return st;
+        if (ms < best) {
+          best = ms;
+          tb.item_walkers = iw;
+          tb.flush_every = fe;
+        }
+      }
+    B->tuned.push_back(tb);
+    tune = &B->tuned.back();
+    P.sig.clear();  // rebuild with the winner below
+  }
+  if (P.sig != sig || P.item_walkers != tune->item_walkers || P.flush_every != tune->flush_every || !P.exec) {
+    if ((st = build_batch(c, P, n, flames, jobs, tune->item_walkers, tune->flush_every))) return st;
+  }
+  if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
+  if ((st = run_batch_graph(c, P, &gpu_ms))) return st;
+  uint64_t total = 0;
+  for (int i = 0; i < n; i++) {
+    hipError_t e = hipMemcpyAsync(rgba_out[i], reinterpret_cast<uint8_t*>(P.rgba.p) + P.pix[i].pix_off * 4,
+                                  P.pix[i].npix * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess)
+      return fail(c, FFM_ERR_PIXEL_READ_FAILED, "pixel readback: %s", hipGetErrorString(e));
+    total += P.samples[i];
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (stats) {
+    stats->total_samples = total;
+    stats->dispatches = (uint64_t)P.n_waves;
+    stats->iterate_ms = gpu_ms;
+    stats->walker_kernel_ms = 0.0;
+    stats->walker_launches = (uint64_t)P.n_waves;
+    stats->tile_accum_ms = 0.0;
+    stats->render_time_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return FFM_OK;
+}
+
+// The reference's gallery behaviour: the same flames rendered one after
+// another (wasm/render/src/lib.rs:45-53), each a full ffm_render.
+int ffm_batch_render_serial(ffm_ctx* c, int n, const ffm_flame_desc* flames,
+                            const float* const* stops, const int* n_stops, const ffm_job* jobs,
+                            uint8_t* const* rgba_out, ffm_render_stats* stats) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  int st = check_batch(c, n, flames, stops, n_stops, jobs, rgba_out);
+  if (st) return st;
+  auto t0 = std::chrono::steady_clock::now();
+  uint64_t total = 0;
+  double it_ms = 0.0;
+  for (int i = 0; i < n; i++) {
+    if ((st = ffm_flame_set(c, &flames[i]))) return st;
+    if ((st = ffm_palette_set(c, stops[i], n_stops[i], BATCH_PALETTE))) return st;
+    ffm_job j = jobs[i];
+    j.target_samples = 0;  // one dispatch, as the batch renders it
+    j.persistent_orbits = 0;
+    j.splat_mode = FFM_SPLAT_U32_LDS;
+    ffm_render_stats one{};
+    if ((st = ffm_render(c, &j, rgba_out[i], &one))) return st;
+    total += one.total_samples;
+    it_ms += one.iterate_ms;
+  }
+  if (stats) {
+    stats->total_samples = total;
+    stats->dispatches = (uint64_t)n;
+    stats->iterate_ms = it_ms;
+    stats->walker_kernel_ms = 0.0;
+    stats->walker_launches = 0;
+    stats->tile_accum_ms = 0.0;
+    stats->render_time_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return FFM_OK;
+}
+
+}  // extern "C"
+
+void free_batch(ffm_batch_state* b) {
+  if (!b) return;
+  b->plan.release();
+  delete b;
+}
+
+namespace {
+ffm_batch_state* batch_state(ffm_ctx* c) {
+  if (!c->batch) c->batch = new ffm_batch_state();
+  return c->batch;
+}
+}  // namespace
+
+extern "C" {
 
 int ffm_softsplat_forward(ffm_ctx* c, const ffm_view*, uint64_t, uint32_t, uint32_t, uint32_t,
                           float*, void*) {

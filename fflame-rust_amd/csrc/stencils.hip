@@ -17,6 +17,44 @@ static int grid_for(size_t n, int block, int cap = 16384) {
 
 // ------------------------------------------------------------ accumulate
 
+// One pixel of accumulate.wgsl: h = this batch's (r, g, b, density) sums
+// as f32, hd the density word as an integer (for the exact count).
+__device__ __forceinline__ float4 accum_one(float4 prev, float h0, float h1, float h2, float h3,
+                                            uint32_t hd, uint2* count, int mode, float blend) {
+  const float SCALE = 100.0f;
+  if (mode == 2) prev = make_float4(0.f, 0.f, 0.f, 0.f);
+  float new_density = h3 / SCALE;
+  float total = prev.w + new_density;
+  if (count) {
+    uint2 c = *count;
+    if (prev.w == 0.0f) {
+      c = make_uint2(0u, 0u);
+    } else if (c.x == 0u && c.y == 0u) {
+      float seed = prev.w * SCALE;
+      float hi = floorf(seed / 4294967296.0f);
+      c = make_uint2((uint32_t)(seed - hi * 4294967296.0f), (uint32_t)hi);
+    }
+    uint32_t lo = c.x + hd;
+    c = make_uint2(lo, c.y + (lo < c.x ? 1u : 0u));
+    *count = c;
+    total = ((float)c.y * 4294967296.0f + (float)c.x) / SCALE;
+    prev.w = total;
+  }
+  if (new_density <= 0.0f) return prev;
+  float dn = fmaxf(h3, 1.0f);
+  float n0 = fminf(fmaxf(h0 / dn, 0.0f), 1.0f);
+  float n1 = fminf(fmaxf(h1 / dn, 0.0f), 1.0f);
+  float n2 = fminf(fmaxf(h2 / dn, 0.0f), 1.0f);
+  float cum_alpha = new_density / total;
+  float alpha = mode == 1 ? blend : cum_alpha;
+  float4 o;
+  o.x = prev.x + (n0 - prev.x) * alpha;
+  o.y = prev.y + (n1 - prev.y) * alpha;
+  o.z = prev.z + (n2 - prev.z) * alpha;
+  o.w = total;
+  return o;
+}
+
 // shaders/accumulate.wgsl:115-230 (no depth buffer). One pixel per thread:
 // reads the batch histogram (u32x4 or f32x4) and the previous accumulator,
 // writes the new accumulator (may alias prev) and, when `count` is set, the
@@ -26,53 +64,18 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const void* __restrict_
                                                          const float4* prev_buf, float4* out_buf,
                                                          uint2* count, size_t npix, int mode,
                                                          float blend) {
-  const float SCALE = 100.0f;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
        p += (size_t)gridDim.x * blockDim.x) {
-    float4 prev = mode == 2 ? make_float4(0.f, 0.f, 0.f, 0.f) : prev_buf[p];
-    float h0, h1, h2, h3;
-    uint32_t hd;
-    if (F32H) {
-      float4 h = reinterpret_cast<const float4*>(hist_v)[p];
-      h0 = h.x; h1 = h.y; h2 = h.z; h3 = h.w;
-      hd = (uint32_t)h.w;
-    } else {
-      uint4 h = reinterpret_cast<const uint4*>(hist_v)[p];
-      h0 = (float)h.x; h1 = (float)h.y; h2 = (float)h.z; h3 = (float)h.w;
-      hd = h.w;
-    }
-    float new_density = h3 / SCALE;
-    float total = prev.w + new_density;
-    if (count) {
-      uint2 c = count[p];
-      if (prev.w == 0.0f) {
-        c = make_uint2(0u, 0u);
-      } else if (c.x == 0u && c.y == 0u) {
-        float seed = prev.w * SCALE;
-        float hi = floorf(seed / 4294967296.0f);
-        c = make_uint2((uint32_t)(seed - hi * 4294967296.0f), (uint32_t)hi);
-      }
-      uint32_t lo = c.x + hd;
-      c = make_uint2(lo, c.y + (lo < c.x ? 1u : 0u));
-      count[p] = c;
-      total = ((float)c.y * 4294967296.0f + (float)c.x) / SCALE;
-      prev.w = total;
-    }
-    if (new_density <= 0.0f) {
-      out_buf[p] = prev;
-      continue;
-    }
-    float dn = fmaxf(h3, 1.0f);
-    float n0 = fminf(fmaxf(h0 / dn, 0.0f), 1.0f);
-    float n1 = fminf(fmaxf(h1 / dn, 0.0f), 1.0f);
-    float n2 = fminf(fmaxf(h2 / dn, 0.0f), 1.0f);
-    float cum_alpha = new_density / total;
-    float alpha = mode == 1 ? blend : cum_alpha;
+    const float4 prev = mode == 2 ? make_float4(0.f, 0.f, 0.f, 0.f) : prev_buf[p];
     float4 o;
-    o.x = prev.x + (n0 - prev.x) * alpha;
-    o.y = prev.y + (n1 - prev.y) * alpha;
-    o.z = prev.z + (n2 - prev.z) * alpha;
-    o.w = total;
+    if (F32H) {
+      const float4 h = reinterpret_cast<const float4*>(hist_v)[p];
+      o = accum_one(prev, h.x, h.y, h.z, h.w, (uint32_t)h.w, count ? count + p : nullptr, mode, blend);
+    } else {
+      const uint4 h = reinterpret_cast<const uint4*>(hist_v)[p];
+      o = accum_one(prev, (float)h.x, (float)h.y, (float)h.z, (float)h.w, h.w,
+                    count ? count + p : nullptr, mode, blend);
+    }
     out_buf[p] = o;
   }
 }
@@ -313,108 +316,111 @@ __device__ void hsv_adjust(float c[3], float hue_shift, float saturation) {
   }
 }
 
-// shaders/tonemap.wgsl:224-590 fs_main as a compute pass over the
-// accumulator (uniform terms precomputed on the host).
+// One pixel of tonemap.wgsl fs_main (shaders/tonemap.wgsl:224-590), with the
+// uniform terms precomputed on the host.
+__device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst& t) {
+  const float bucket_count = a.w * 100.0f;
+  float color[3];
+  float alpha = 0.0f;
+  if (t.mode == 0) {
+    color[0] = powf(a.x * t.exposure, t.gamma);
+    color[1] = powf(a.y * t.exposure, t.gamma);
+    color[2] = powf(a.z * t.exposure, t.gamma);
+    alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
+  } else if (t.mode == 2) {
+    float nd = clampf(bucket_count * 0.01f * t.exposure, 0.0f, 1.0f);
+    float gd = powf(nd, t.gamma);
+    color[0] = color[1] = color[2] = gd;
+    alpha = nd;
+  } else {
+    float bs = 0.0f;
+    if (!(bucket_count < 0.001f)) {
+      float l10 = logf(1.0f + t.white * bucket_count * t.k2) / logf(10.0f);
+      bs = (t.k1 * l10) / (t.white * bucket_count);
+    }
+    float ls = bs / 67108864.0f;
+    float fp0 = ls * (a.x * bucket_count);
+    float fp1 = ls * (a.y * bucket_count);
+    float fp2 = ls * (a.z * bucket_count);
+    float fp3 = ls * bucket_count * t.white;
+    if (fp3 > 0.0f) {
+      if (fp3 <= t.gamma_threshold) {
+        float frac = fp3 / t.gamma_threshold;
+        alpha = (1.0f - frac) * fp3 * t.funcval + frac * powf(fp3, t.gamma);
+      } else {
+        alpha = powf(fp3, t.gamma);
+      }
+    }
+    ls = fp3 > 0.0f ? t.vib * alpha / fp3 : 0.0f;
+    if (t.notvib > 0.0f) {
+      color[0] = ls * fp0 + t.notvib * powf(fp0, t.gamma);
+      color[1] = ls * fp1 + t.notvib * powf(fp1, t.gamma);
+      color[2] = ls * fp2 + t.notvib * powf(fp2, t.gamma);
+    } else {
+      color[0] = ls * fp0;
+      color[1] = ls * fp1;
+      color[2] = ls * fp2;
+    }
+  }
+  if (t.saturation != 1.0f || t.hue_shift != 0.0f) hsv_adjust(color, t.hue_shift, t.saturation);
+  if (t.mode == 1) {
+    color[0] = color[0] * t.exposure;
+    color[1] = color[1] * t.exposure;
+    color[2] = color[2] * t.exposure;
+  }
+  for (int k = 0; k < 3; k++) color[k] = fmaxf(color[k], 0.0f);
+  if (t.highlight == 1) {
+    float m = fmaxf(color[0], fmaxf(color[1], color[2]));
+    if (m > 1.0f)
+      for (int k = 0; k < 3; k++) color[k] = color[k] / m;
+  } else if (t.highlight == 2) {
+    float lum = color[0] * 0.2126f + color[1] * 0.7152f + color[2] * 0.0722f;
+    if (lum > 1e-6f) {
+      float lm = lum / (1.0f + lum);
+      for (int k = 0; k < 3; k++) color[k] = color[k] * (lm / lum);
+    }
+    for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
+  } else if (t.highlight == 3) {
+    for (int k = 0; k < 3; k++) {
+      float c = color[k];
+      c = (c * (2.51f * c + 0.03f)) / (c * (2.43f * c + 0.59f) + 0.14f);
+      color[k] = clampf(c, 0.0f, 1.0f);
+    }
+  } else {
+    for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
+  }
+  float leveled = 1.0f;
+  if (t.sample_density > 0.0f) {
+    float nd = a.w / t.sample_density;
+    if (t.levels_high <= t.levels_low) {
+      leveled = nd > t.levels_low ? 1.0f : 0.0f;
+    } else {
+      float nrm = (nd - t.levels_low) / (t.levels_high - t.levels_low);
+      float cl = clampf(nrm, 0.0f, 1.0f);
+      leveled = (t.levels_gamma != 1.0f && t.levels_gamma > 0.0f) ? powf(cl, t.levels_gamma) : cl;
+    }
+  }
+  float linear_alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
+  float gamma_alpha = clampf(alpha, 0.0f, 1.0f);
+  float st = clampf((gamma_alpha - t.alpha_low) / (t.alpha_high - t.alpha_low), 0.0f, 1.0f);
+  float blend_t = st * st * (3.0f - 2.0f * st);
+  float base_alpha = gamma_alpha * (1.0f - blend_t) + linear_alpha * blend_t;
+  float fa = t.levels_enabled ? fminf(base_alpha, leveled) : base_alpha;
+  uchar4 o;
+  o.x = unorm8(powf(t.bg_lin[0] * (1.0f - fa) + color[0] * fa, t.inv22));
+  o.y = unorm8(powf(t.bg_lin[1] * (1.0f - fa) + color[1] * fa, t.inv22));
+  o.z = unorm8(powf(t.bg_lin[2] * (1.0f - fa) + color[2] * fa, t.inv22));
+  o.w = 255;
+  return o;
+}
+
+// fs_main as a compute pass over the accumulator.
 __global__ __launch_bounds__(256) void tonemap_kernel(const float4* __restrict__ accum,
                                                       uchar4* __restrict__ rgba, size_t npix,
                                                       TonemapConst t) {
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix;
-       p += (size_t)gridDim.x * blockDim.x) {
-    const float4 a = accum[p];
-    const float bucket_count = a.w * 100.0f;
-    float color[3];
-    float alpha = 0.0f;
-    if (t.mode == 0) {
-      color[0] = powf(a.x * t.exposure, t.gamma);
-      color[1] = powf(a.y * t.exposure, t.gamma);
-      color[2] = powf(a.z * t.exposure, t.gamma);
-      alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
-    } else if (t.mode == 2) {
-      float nd = clampf(bucket_count * 0.01f * t.exposure, 0.0f, 1.0f);
-      float gd = powf(nd, t.gamma);
-      color[0] = color[1] = color[2] = gd;
-      alpha = nd;
-    } else {
-      float bs = 0.0f;
-      if (!(bucket_count < 0.001f)) {
-        float l10 = logf(1.0f + t.white * bucket_count * t.k2) / logf(10.0f);
-        bs = (t.k1 * l10) / (t.white * bucket_count);
-      }
-      float ls = bs / 67108864.0f;
-      float fp0 = ls * (a.x * bucket_count);
-      float fp1 = ls * (a.y * bucket_count);
-      float fp2 = ls * (a.z * bucket_count);
-      float fp3 = ls * bucket_count * t.white;
-      if (fp3 > 0.0f) {
-        if (fp3 <= t.gamma_threshold) {
-          float frac = fp3 / t.gamma_threshold;
-          alpha = (1.0f - frac) * fp3 * t.funcval + frac * powf(fp3, t.gamma);
-        } else {
-          alpha = powf(fp3, t.gamma);
-        }
-      }
-      ls = fp3 > 0.0f ? t.vib * alpha / fp3 : 0.0f;
-      if (t.notvib > 0.0f) {
-        color[0] = ls * fp0 + t.notvib * powf(fp0, t.gamma);
-        color[1] = ls * fp1 + t.notvib * powf(fp1, t.gamma);
-        color[2] = ls * fp2 + t.notvib * powf(fp2, t.gamma);
-      } else {
-        color[0] = ls * fp0;
-        color[1] = ls * fp1;
-        color[2] = ls * fp2;
-      }
-    }
-    if (t.saturation != 1.0f || t.hue_shift != 0.0f) hsv_adjust(color, t.hue_shift, t.saturation);
-    if (t.mode == 1) {
-      color[0] = color[0] * t.exposure;
-      color[1] = color[1] * t.exposure;
-      color[2] = color[2] * t.exposure;
-    }
-    for (int k = 0; k < 3; k++) color[k] = fmaxf(color[k], 0.0f);
-    if (t.highlight == 1) {
-      float m = fmaxf(color[0], fmaxf(color[1], color[2]));
-      if (m > 1.0f)
-        for (int k = 0; k < 3; k++) color[k] = color[k] / m;
-    } else if (t.highlight == 2) {
-      float lum = color[0] * 0.2126f + color[1] * 0.7152f + color[2] * 0.0722f;
-      if (lum > 1e-6f) {
-        float lm = lum / (1.0f + lum);
-        for (int k = 0; k < 3; k++) color[k] = color[k] * (lm / lum);
-      }
-      for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
-    } else if (t.highlight == 3) {
-      for (int k = 0; k < 3; k++) {
-        float c = color[k];
-        c = (c * (2.51f * c + 0.03f)) / (c * (2.43f * c + 0.59f) + 0.14f);
-        color[k] = clampf(c, 0.0f, 1.0f);
-      }
-    } else {
-      for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
-    }
-    float leveled = 1.0f;
-    if (t.sample_density > 0.0f) {
-      float nd = a.w / t.sample_density;
-      if (t.levels_high <= t.levels_low) {
-        leveled = nd > t.levels_low ? 1.0f : 0.0f;
-      } else {
-        float nrm = (nd - t.levels_low) / (t.levels_high - t.levels_low);
-        float cl = clampf(nrm, 0.0f, 1.0f);
-        leveled = (t.levels_gamma != 1.0f && t.levels_gamma > 0.0f) ? powf(cl, t.levels_gamma) : cl;
-      }
-    }
-    float linear_alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
-    float gamma_alpha = clampf(alpha, 0.0f, 1.0f);
-    float st = clampf((gamma_alpha - t.alpha_low) / (t.alpha_high - t.alpha_low), 0.0f, 1.0f);
-    float blend_t = st * st * (3.0f - 2.0f * st);
-    float base_alpha = gamma_alpha * (1.0f - blend_t) + linear_alpha * blend_t;
-    float fa = t.levels_enabled ? fminf(base_alpha, leveled) : base_alpha;
-    uchar4 o;
-    o.x = unorm8(powf(t.bg_lin[0] * (1.0f - fa) + color[0] * fa, t.inv22));
-    o.y = unorm8(powf(t.bg_lin[1] * (1.0f - fa) + color[1] * fa, t.inv22));
-    o.z = unorm8(powf(t.bg_lin[2] * (1.0f - fa) + color[2] * fa, t.inv22));
-    o.w = 255;
-    rgba[p] = o;
-  }
+       p += (size_t)gridDim.x * blockDim.x)
+    rgba[p] = tonemap_one(accum[p], t);
 }
 
 hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H, const TonemapConst& tc,
@@ -423,6 +429,32 @@ hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H, const
   hipLaunchKernelGGL(tonemap_kernel, dim3(grid_for(npix, 256)), dim3(256), 0, s,
                      reinterpret_cast<const float4*>(accum), reinterpret_cast<uchar4*>(rgba), npix,
                      tc);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- batches
+
+// Accumulate (fresh accumulator, cumulative mode) then tonemap, for many
+// flames in one launch: blockIdx.y is the flame, blocks stride its pixels.
+__global__ __launch_bounds__(256) void batch_finish_kernel(const uint4* __restrict__ hist,
+                                                           uchar4* __restrict__ rgba,
+                                                           const BatchPix* __restrict__ fl) {
+  const BatchPix f = fl[blockIdx.y];
+  const uint4* h = hist + f.pix_off;
+  uchar4* o = rgba + f.pix_off;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < f.npix;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const uint4 v = h[p];
+    const float4 a = accum_one(make_float4(0.f, 0.f, 0.f, 0.f), (float)v.x, (float)v.y, (float)v.z,
+                               (float)v.w, v.w, nullptr, 0, 0.0f);
+    o[p] = tonemap_one(a, f.tc);
+  }
+}
+
+hipError_t launch_batch_finish(const uint32_t* hist, uint8_t* rgba, const BatchPix* fl, int n,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(batch_finish_kernel, dim3(64, n), dim3(256), 0, s,
+                     reinterpret_cast<const uint4*>(hist), reinterpret_cast<uchar4*>(rgba), fl);
   return hipGetLastError();
 }
 

@@ -29,8 +29,7 @@ namespace ffm {
 
 constexpr int BIN_BLOCK = 1024;
 constexpr int STAGE = 10240;  // staged records per workgroup
-constexpr int CHECK_EVERY = 4;  // iterations between fill checks
-constexpr int FLUSH_AT = STAGE - CHECK_EVERY * BIN_BLOCK;
+constexpr int MAX_FLUSH_EVERY = 8;  // iterations between fill checks, at most
 constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
 
 size_t bin_lds_bytes(int n_rows, int pal_size) {
@@ -147,36 +146,24 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
   __syncthreads();
 }
 
-// The walker loop of iterate_kernel with the splat replaced by staging.
-// Rounds are uniform across the workgroup so every barrier is reached by
-// all threads; lanes past the last walker step nothing.
+// Walk walkers base + rd * stride + tid (< limit) for a.ipt iterations,
+// staging every in-frame sample (or, COUNT_ONLY, counting it per tile).
+// Rounds are uniform across the workgroup so every barrier is reached by all
+// threads; lanes past `limit` step nothing.
 template <bool COUNT_ONLY>
-__global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, TileArgs t) {
-  extern __shared__ uint32_t lds[];
-  LdsFlame F = load_lds<false>(a, lds);
-  uint32_t* p = lds + flame_table_floats(a.n_rows) + a.pal_size;
-  BinLds L;
-  L.rec = p;
-  L.tr = p + STAGE;
-  L.perm = reinterpret_cast<uint16_t*>(p + 2 * STAGE);
-  L.cnt = p + 2 * STAGE + STAGE / 2;
-  L.gbase = L.cnt + TILE_MAX;
-  L.wsum = L.gbase + TILE_MAX;
-  L.fill = L.wsum + 16;
+__device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame& F,
+                            const BinLds& L, uint64_t base, uint64_t stride, uint64_t limit,
+                            unsigned long long& n_plotted, unsigned long long& n_in) {
   const int tid = threadIdx.x;
-  if (COUNT_ONLY) {
-    for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
-  }
-  if (tid == 0) *L.fill = 0u;
-  __syncthreads();
-
   const ViewConst v = a.view;
-  unsigned long long n_plotted = 0, n_in = 0;
-  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t rounds = (a.walkers + nthreads - 1) / nthreads;
+  const uint64_t rounds = limit > base ? (limit - base + stride - 1) / stride : 0;
+  // a fill check every `every` iterations; at most `every` x blockDim
+  // records arrive between checks, so flushing at STAGE - that never overflows
+  const uint32_t every = (uint32_t)min(max(t.flush_every, 1), MAX_FLUSH_EVERY);
+  const uint32_t flush_at = (uint32_t)STAGE - every * blockDim.x;
   for (uint64_t rd = 0; rd < rounds; rd++) {
-    const uint64_t wi = rd * nthreads + (uint64_t)blockIdx.x * blockDim.x + tid;
-    const bool active = wi < a.walkers;
+    const uint64_t wi = base + rd * stride + tid;
+    const bool active = wi < limit;
     const uint32_t walker = a.walker0 + (uint32_t)wi;
     Rng rng = rng_init(walker, a.seed);
     float x = rng_nextf(rng) * 2.0f - 1.0f;
@@ -207,32 +194,72 @@ __global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, Tile
             const uint64_t m = __ballot(1);
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            uint32_t base = 0;
+            uint32_t b = 0;
             const int leader = __ffsll((unsigned long long)m) - 1;
-            if ((int)(tid & 63) == leader) base = atomicAdd(L.fill, (uint32_t)__popcll(m));
-            base = __shfl(base, leader, 64);
-            L.rec[base + rank] = local | ((uint32_t)texel << 13);
-            L.tr[base + rank] = (uint32_t)tile;
+            if ((int)(tid & 63) == leader) b = atomicAdd(L.fill, (uint32_t)__popcll(m));
+            b = __shfl(b, leader, 64);
+            L.rec[b + rank] = local | ((uint32_t)texel << 13);
+            L.tr[b + rank] = (uint32_t)tile;
           }
         }
       }
-      if (!COUNT_ONLY && ((it % CHECK_EVERY) == CHECK_EVERY - 1 || it + 1 == a.ipt)) {
+      if (!COUNT_ONLY && ((it % every) == every - 1 || it + 1 == a.ipt)) {
         __syncthreads();
         const uint32_t f = *L.fill;
         __syncthreads();
-        if (f >= (uint32_t)FLUSH_AT) flush_stage(t, L, F.pal_u32);
+        if (f >= flush_at) flush_stage(t, L, F.pal_u32);
       }
     }
     if (!COUNT_ONLY && active && a.orbits)
       a.orbits[wi] = make_float4(x, y, ci, __uint_as_float(fuse));
   }
+}
+
+__device__ __forceinline__ BinLds bin_lds_views(uint32_t* p) {
+  BinLds L;
+  L.rec = p;
+  L.tr = p + STAGE;
+  L.perm = reinterpret_cast<uint16_t*>(p + 2 * STAGE);
+  L.cnt = p + 2 * STAGE + STAGE / 2;
+  L.gbase = L.cnt + TILE_MAX;
+  L.wsum = L.gbase + TILE_MAX;
+  L.fill = L.wsum + 16;
+  return L;
+}
+
+// Finish a workgroup's walking: publish per-tile counts (COUNT_ONLY) or
+// flush what is still staged. All threads.
+template <bool COUNT_ONLY>
+__device__ void finish_walkers(const TileArgs& t, const BinLds& L, const uint32_t* pal) {
+  const int tid = threadIdx.x;
   __syncthreads();
   if (COUNT_ONLY) {
-    for (int i = tid; i < t.n_tiles; i += blockDim.x)
+    for (int i = tid; i < t.n_tiles; i += blockDim.x) {
       if (L.cnt[i])
         __hip_atomic_fetch_add(t.tile_fill + i, L.cnt[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      L.cnt[i] = 0u;
+    }
+    __syncthreads();
   } else {
-    if (*L.fill > 0u) flush_stage(t, L, F.pal_u32);
+    if (*L.fill > 0u) flush_stage(t, L, pal);
+  }
+}
+
+// One dispatch of one flame: walkers grid-strided over the whole grid.
+template <bool COUNT_ONLY>
+__global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, TileArgs t) {
+  extern __shared__ uint32_t lds[];
+  LdsFlame F = load_lds<false>(a, lds);
+  const BinLds L = bin_lds_views(lds + flame_table_floats(a.n_rows) + a.pal_size);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
+  if (tid == 0) *L.fill = 0u;
+  __syncthreads();
+  unsigned long long n_plotted = 0, n_in = 0;
+  run_walkers<COUNT_ONLY>(a, t, F, L, (uint64_t)blockIdx.x * blockDim.x,
+                          (uint64_t)gridDim.x * blockDim.x, a.walkers, n_plotted, n_in);
+  finish_walkers<COUNT_ONLY>(t, L, F.pal_u32);
+  if (!COUNT_ONLY) {
     for (int off = 32; off > 0; off >>= 1) {
       n_plotted += __shfl_xor(n_plotted, off, 64);
       n_in += __shfl_xor(n_in, off, 64);
@@ -246,66 +273,161 @@ __global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, Tile
   }
 }
 
-// Capacities from the pre-pass counts: cap = est * scale + margin, regions
-// laid out back to back, scaled down to fit `buffer_records` if needed;
-// then the fill counters are cleared for the real pass. One workgroup.
+// Exclusive scan of value(i), i < n, into out[] by one workgroup, 2048 at a
+// time with a running carry; returns the total. All threads.
+template <typename Fn>
+__device__ uint32_t block_scan_n(int n, Fn value, uint32_t* out, uint32_t* buf, uint32_t* wsum,
+                                 uint32_t* total) {
+  uint32_t carry = 0;
+  for (int c0 = 0; c0 < n; c0 += 2 * BIN_BLOCK) {
+    const int m = min(2 * BIN_BLOCK, n - c0);
+    for (int i = threadIdx.x; i < m; i += blockDim.x) buf[i] = value(c0 + i);
+    __syncthreads();
+    block_exclusive_scan(buf, m, wsum, total);
+    for (int i = threadIdx.x; i < m; i += blockDim.x) out[c0 + i] = buf[i] + carry;
+    carry += *total;
+    __syncthreads();
+  }
+  return carry;
+}
+
+// Capacities from the pre-pass counts: cap = est * scale * 1.25 + margin,
+// regions laid out back to back, scaled down to fit `buffer_records` if
+// needed; then the fill counters are cleared for the real pass. One
+// workgroup; any number of tiles.
 __global__ __launch_bounds__(BIN_BLOCK) void tile_capacity_kernel(TileArgs t, float scale,
                                                                   uint32_t margin,
                                                                   uint64_t buffer_records) {
-  __shared__ uint32_t a[TILE_MAX];
+  __shared__ uint32_t buf[2 * BIN_BLOCK];
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t total;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) {
-    float c = (float)t.tile_fill[i] * scale;
-    a[i] = t.tile_fill[i] ? (uint32_t)fminf(c * 1.25f + (float)margin, 2.0e9f) : margin / 4u;
-  }
-  __syncthreads();
-  // total before scaling (64-bit safe: each entry < 2^31, at most 2048)
   __shared__ unsigned long long sum64;
+  const int tid = threadIdx.x;
+  auto cap_of = [&](int i) -> uint32_t {
+    const uint32_t f = t.tile_fill[i];
+    return f ? (uint32_t)fminf((float)f * scale * 1.25f + (float)margin, 2.0e9f) : margin / 4u;
+  };
   if (tid == 0) sum64 = 0ull;
   __syncthreads();
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) atomicAdd(&sum64, (unsigned long long)a[i]);
+  unsigned long long mine = 0;
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) mine += cap_of(i);
+  atomicAdd(&sum64, mine);
   __syncthreads();
-  if (sum64 > buffer_records) {
-    const double f = (double)buffer_records / (double)sum64;
-    for (int i = tid; i < t.n_tiles; i += blockDim.x) a[i] = (uint32_t)((double)a[i] * f);
-  }
+  const double f = sum64 > buffer_records ? (double)buffer_records / (double)sum64 : 1.0;
+  auto scaled = [&](int i) -> uint32_t {
+    const uint32_t c = cap_of(i);
+    return f < 1.0 ? (uint32_t)((double)c * f) : c;
+  };
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) t.cap[i] = scaled(i);
   __syncthreads();
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) t.cap[i] = a[i];
-  __syncthreads();
-  block_exclusive_scan(a, t.n_tiles, wsum, &total);
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) {
-    t.region[i] = a[i];
-    t.tile_fill[i] = 0u;
-  }
+  block_scan_n(t.n_tiles, [&](int i) { return t.cap[i]; }, t.region, buf, wsum, &total);
+  for (int i = tid; i < t.n_tiles; i += blockDim.x) t.tile_fill[i] = 0u;
 }
 
-// Work list for tile_accum_kernel: tile i gets ceil(stored_i / SLICE) items.
+// Work list for the tile accumulation: tile i gets ceil(stored_i / SLICE)
+// items; item_start[n_tiles] is the total.
 __global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
-  __shared__ uint32_t a[TILE_MAX];
+  __shared__ uint32_t buf[2 * BIN_BLOCK];
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t total;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) {
-    const uint32_t stored = min(t.tile_fill[i], t.cap[i]);
-    a[i] = (stored + SLICE - 1) / SLICE;
-  }
-  __syncthreads();
-  block_exclusive_scan(a, t.n_tiles, wsum, &total);
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) t.item_start[i] = a[i];
-  if (tid == 0) t.item_start[t.n_tiles] = total;
+  const uint32_t all = block_scan_n(
+      t.n_tiles,
+      [&](int i) { return (min(t.tile_fill[i], t.cap[i]) + SLICE - 1) / SLICE; },
+      t.item_start, buf, wsum, &total);
+  if (threadIdx.x == 0) t.item_start[t.n_tiles] = all;
 }
 
-// Accumulate a slice of one tile's records in LDS and fold it into the
-// histogram. Persistent grid over the work list. F32: float4 accumulation
-// (LDS float atomics, f32 histogram).
+// Accumulate records [r0, r1) of tile `ti` (of the flame `t` describes)
+// into the LDS tile and fold it into the histogram: a read-add-write when
+// the slice is the tile's only one, integer / float atomics otherwise.
+// All threads; F32: float4 accumulation.
+template <bool F32>
+__device__ void accum_slice(const TileArgs& t, int ti, uint32_t r0, uint32_t r1, uint32_t n_slices,
+                            uint32_t* tile, const uint32_t* pal, const float* palf) {
+  const int tid = threadIdx.x;
+  float* tilef = reinterpret_cast<float*>(tile);
+  for (int i = tid; i < 4 * TILE_PX; i += blockDim.x) tile[i] = 0u;
+  __syncthreads();
+  const uint32_t* rec = t.records + t.region[ti];
+  for (uint32_t r = r0 + tid; r < r1; r += blockDim.x) {
+    const uint32_t v = rec[r];
+    const uint32_t local = v & 0x1FFFu;
+    if (F32) {
+      const float* c = palf + 3 * (v >> 13);
+      atomicAdd(&tilef[4 * local + 0], c[0]);
+      atomicAdd(&tilef[4 * local + 1], c[1]);
+      atomicAdd(&tilef[4 * local + 2], c[2]);
+      atomicAdd(&tilef[4 * local + 3], 100.0f);
+    } else {
+      const uint32_t pk = pal[v >> 13];
+      atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
+      atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
+      atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
+      atomicAdd(&tile[4 * local + 3], 100u);
+    }
+  }
+  __syncthreads();
+  const int tx = ti % t.tiles_x, ty = ti / t.tiles_x;
+  for (int q = tid; q < TILE_PX; q += blockDim.x) {
+    const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
+    if (px >= t.width || py >= t.height) continue;
+    if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
+    const size_t p = (size_t)py * t.width + px;
+    if (F32) {
+      const float4 v = make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
+      float4* h = reinterpret_cast<float4*>(t.hist) + p;
+      if (n_slices == 1) {
+        float4 o = *h;
+        o.x += v.x;
+        o.y += v.y;
+        o.z += v.z;
+        o.w += v.w;
+        *h = o;
+      } else {
+        float* hw = reinterpret_cast<float*>(h);
+        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else {
+      const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
+      uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
+      if (n_slices == 1) {
+        uint4 o = *h;
+        o.x += v.x;
+        o.y += v.y;
+        o.z += v.z;
+        o.w += v.w;
+        *h = o;
+      } else {
+        uint32_t* hw = reinterpret_cast<uint32_t*>(h);
+        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// The largest i < n with start[i] <= x (start non-decreasing). Uniform.
+__device__ __forceinline__ int find_range(const uint32_t* start, int n, uint32_t x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One flame's tiles: persistent grid over the work list.
 template <bool F32>
 __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const uint32_t* pal_g,
                                                                int pal_size) {
   extern __shared__ uint32_t lds[];
   uint32_t* tile = lds;               // TILE_PX x 4 (u32 or f32 bits)
-  float* tilef = reinterpret_cast<float*>(lds);
   uint32_t* pal = lds + 4 * TILE_PX;  // pal_size u32, or 3 x pal_size f32
   float* palf = reinterpret_cast<float*>(pal);
   const int tid = threadIdx.x;
@@ -315,80 +437,93 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const
     for (int i = tid; i < pal_size; i += blockDim.x) pal[i] = pal_g[i];
   const uint32_t n_items = t.item_start[t.n_tiles];
   for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-    // the tile whose item range holds `item` (uniform across the block)
-    int lo = 0, hi = t.n_tiles - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (t.item_start[mid] <= item) lo = mid; else hi = mid - 1;
-    }
-    const int ti = lo;
+    const int ti = find_range(t.item_start, t.n_tiles, item);
     const uint32_t first = t.item_start[ti], n_slices = t.item_start[ti + 1] - first;
     const uint32_t stored = min(t.tile_fill[ti], t.cap[ti]);
     const uint32_t r0 = (item - first) * SLICE, r1 = min(r0 + SLICE, stored);
-    for (int i = tid; i < 4 * TILE_PX; i += blockDim.x) tile[i] = 0u;
-    __syncthreads();
-    const uint32_t* rec = t.records + t.region[ti];
-    for (uint32_t r = r0 + tid; r < r1; r += blockDim.x) {
-      const uint32_t v = rec[r];
-      const uint32_t local = v & 0x1FFFu;
-      if (F32) {
-        const float* c = palf + 3 * (v >> 13);
-        atomicAdd(&tilef[4 * local + 0], c[0]);
-        atomicAdd(&tilef[4 * local + 1], c[1]);
-        atomicAdd(&tilef[4 * local + 2], c[2]);
-        atomicAdd(&tilef[4 * local + 3], 100.0f);
-      } else {
-        const uint32_t pk = pal[v >> 13];
-        atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
-        atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
-        atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
-        atomicAdd(&tile[4 * local + 3], 100u);
-      }
+    accum_slice<F32>(t, ti, r0, r1, n_slices, tile, pal, palf);
+  }
+}
+
+// ------------------------------------------------------------ batches
+
+// A flame's view of the global tile arrays.
+__device__ __forceinline__ TileArgs flame_tiles(const BatchArgs& b, const BatchFlame& f) {
+  TileArgs t = b.g;
+  t.tile_fill += f.tile_base;
+  t.region += f.tile_base;
+  t.cap += f.tile_base;
+  t.hist = b.g.hist + 4 * f.hist_off;
+  t.tiles_x = f.tiles_x;
+  t.n_tiles = f.n_tiles;
+  t.width = f.view.width;
+  t.height = f.view.height;
+  return t;
+}
+
+// Many flames in one persistent launch: each workgroup takes work items
+// (flame, walker range), loads that flame's table and palette into LDS and
+// walks the range with the same staging and flushes as one flame.
+template <bool COUNT_ONLY>
+__global__ __launch_bounds__(BIN_BLOCK) void bin_batch_kernel(BatchArgs b) {
+  extern __shared__ uint32_t lds[];
+  const BinLds L = bin_lds_views(lds + flame_table_floats(b.max_rows) + b.max_pal);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < TILE_MAX; i += blockDim.x) L.cnt[i] = 0u;
+  if (tid == 0) *L.fill = 0u;
+  __syncthreads();
+  unsigned long long n_plotted = 0, n_in = 0;
+  for (int item = blockIdx.x; item < b.n_items; item += gridDim.x) {
+    const BatchItem it = b.items[item];
+    const BatchFlame f = b.flames[it.flame];
+    const LdsFlame F = load_table<false>(b.tables + f.table_off, b.pal_u32 + f.pal_off, nullptr,
+                                         f.pal_size, lds);
+    IterArgs a{};
+    a.view = f.view;
+    a.pal_size = f.pal_size;
+    a.ipt = f.ipt;
+    a.burn_in = f.burn_in;
+    a.seed = f.seed;
+    const TileArgs t = flame_tiles(b, f);
+    uint64_t limit = it.w1;
+    if (COUNT_ONLY) limit = it.w0 + max((it.w1 - it.w0) / b.count_div, 1u);
+    run_walkers<COUNT_ONLY>(a, t, F, L, it.w0, blockDim.x, limit, n_plotted, n_in);
+    finish_walkers<COUNT_ONLY>(t, L, F.pal_u32);
+    __syncthreads();  // the next item reloads the flame table
+  }
+}
+
+// Tile accumulation over every flame's tiles: global tile -> its flame by
+// the flames' tile bases; the palette is reloaded when the flame changes.
+__global__ __launch_bounds__(BIN_BLOCK) void tile_accum_batch_kernel(BatchArgs b) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* tile = lds;
+  uint32_t* pal = lds + 4 * TILE_PX;
+  __shared__ uint32_t bases[1];
+  (void)bases;
+  const int tid = threadIdx.x;
+  const TileArgs& g = b.g;
+  const uint32_t n_items = g.item_start[g.n_tiles];
+  int cur = -1;
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int gt = find_range(g.item_start, g.n_tiles, item);
+    int lo = 0, hi = b.n_flames - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b.flames[mid].tile_base <= (uint32_t)gt) lo = mid; else hi = mid - 1;
     }
-    __syncthreads();
-    const int tx = ti % t.tiles_x, ty = ti / t.tiles_x;
-    for (int q = tid; q < TILE_PX; q += blockDim.x) {
-      const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
-      if (px >= t.width || py >= t.height) continue;
-      if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
-      const size_t p = (size_t)py * t.width + px;
-      if (F32) {
-        const float4 v = make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
-        float4* h = reinterpret_cast<float4*>(t.hist) + p;
-        if (n_slices == 1) {
-          float4 o = *h;
-          o.x += v.x;
-          o.y += v.y;
-          o.z += v.z;
-          o.w += v.w;
-          *h = o;
-        } else {
-          float* hw = reinterpret_cast<float*>(h);
-          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      } else {
-        const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
-        uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
-        if (n_slices == 1) {
-          uint4 o = *h;
-          o.x += v.x;
-          o.y += v.y;
-          o.z += v.z;
-          o.w += v.w;
-          *h = o;
-        } else {
-          uint32_t* hw = reinterpret_cast<uint32_t*>(h);
-          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
+    const BatchFlame f = b.flames[lo];
+    if (lo != cur) {
+      for (int i = tid; i < f.pal_size; i += blockDim.x) pal[i] = b.pal_u32[f.pal_off + i];
+      __syncthreads();
+      cur = lo;
     }
-    __syncthreads();
+    const TileArgs t = flame_tiles(b, f);
+    const int ti = gt - (int)f.tile_base;
+    const uint32_t first = g.item_start[gt], n_slices = g.item_start[gt + 1] - first;
+    const uint32_t stored = min(g.tile_fill[gt], g.cap[gt]);
+    const uint32_t r0 = (item - first) * SLICE, r1 = min(r0 + SLICE, stored);
+    accum_slice<false>(t, ti, r0, r1, n_slices, tile, pal, nullptr);
   }
 }
 
@@ -408,6 +543,35 @@ hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_o
     hipLaunchKernelGGL(bin_iterate_kernel<true>, dim3(grid), dim3(BIN_BLOCK), shm, s, a, t);
   else
     hipLaunchKernelGGL(bin_iterate_kernel<false>, dim3(grid), dim3(BIN_BLOCK), shm, s, a, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_batch(const BatchArgs& b, bool count_only, int grid, hipStream_t s) {
+  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&bin_batch_kernel<true>));
+  static const hipError_t e1 = allow_lds(reinterpret_cast<const void*>(&bin_batch_kernel<false>));
+  if (e0 != hipSuccess) return e0;
+  if (e1 != hipSuccess) return e1;
+  const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal);
+  if (count_only)
+    hipLaunchKernelGGL(bin_batch_kernel<true>, dim3(grid), dim3(BIN_BLOCK), shm, s, b);
+  else
+    hipLaunchKernelGGL(bin_batch_kernel<false>, dim3(grid), dim3(BIN_BLOCK), shm, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_capacity_n(const TileArgs& t, float scale, uint32_t margin,
+                                  uint64_t buffer_records, hipStream_t s) {
+  hipLaunchKernelGGL(tile_capacity_kernel, dim3(1), dim3(BIN_BLOCK), 0, s, t, scale, margin,
+                     buffer_records);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_accum_batch(const BatchArgs& b, int grid, hipStream_t s) {
+  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&tile_accum_batch_kernel));
+  if (e0 != hipSuccess) return e0;
+  hipLaunchKernelGGL(tile_plan_kernel, dim3(1), dim3(BIN_BLOCK), 0, s, b.g);
+  hipLaunchKernelGGL(tile_accum_batch_kernel, dim3(grid), dim3(BIN_BLOCK),
+                     accum_lds_bytes(b.max_pal, false), s, b);
   return hipGetLastError();
 }
 

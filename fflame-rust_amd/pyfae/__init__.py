@@ -221,6 +221,33 @@ class Engine:
                                               C.byref(stats)))
         return out, stats
 
+    # -- batched gallery
+    def _batch_args(self, items):
+        n = len(items)
+        flames = (FlameDesc * n)()
+        jobs = (Job * n)()
+        stops = []
+        outs = []
+        for i, (fd, pal, job) in enumerate(items):
+            C.memmove(C.byref(flames[i]), C.byref(fd), C.sizeof(FlameDesc))
+            C.memmove(C.byref(jobs[i]), C.byref(job), C.sizeof(Job))
+            stops.append(palette_stops(pal))
+            outs.append(np.zeros((job.view.height, job.view.width, 4), dtype=np.uint8))
+        sp = (C.POINTER(C.c_float) * n)(*[a.ctypes.data_as(C.POINTER(C.c_float)) for a in stops])
+        ns = (C.c_int * n)(*[a.shape[0] for a in stops])
+        op = (C.POINTER(C.c_uint8) * n)(*[o.ctypes.data_as(C.POINTER(C.c_uint8)) for o in outs])
+        return n, flames, sp, ns, jobs, op, outs, stops
+
+    def batch_render(self, items, serial=False):
+        """Render [(flame_desc, palette_stops, job), ...]: one persistent launch
+        per wave in a hipGraph (serial=False) or one ffm_render per flame
+        (serial=True). Returns ([rgba arrays], stats)."""
+        n, flames, sp, ns, jobs, op, outs, keep = self._batch_args(items)
+        stats = RenderStats()
+        fn = self.lib.ffm_batch_render_serial if serial else self.lib.ffm_batch_render
+        self._check(fn(self.ctx, n, flames, sp, ns, jobs, op, C.byref(stats)))
+        return outs, stats
+
     # -- granular kernels on device buffers
     def iterate(self, view, walkers, ipt, burn_in, seed, hist, *, orbits=None, fresh=True,
                 splat_mode=0, plotted=None, inframe=None, stream=None):

@@ -301,6 +301,38 @@ def test_single4k_image_within_noise_floor(engine):
     assert diff <= 1.25 * floor + 1e-3
 
 
+# -------------------------------------------------------- batched gallery
+
+def test_batch_equals_serial_and_oracle(engine):
+    """A small gallery (mixed sizes and transform counts) in one batched
+    launch gives exactly the images of the same flames rendered one by one,
+    and those match the oracle's within one level."""
+    say("batch vs serial")
+    items = configs.gallery(n=6, spp=20.0, ipt=64)
+    outs_b, st_b = engine.batch_render(items)
+    outs_b2, _ = engine.batch_render(items)  # graph replay
+    outs_s, st_s = engine.batch_render(items, serial=True)
+    assert st_b.total_samples == st_s.total_samples
+    for i, (a, a2, b) in enumerate(zip(outs_b, outs_b2, outs_s)):
+        assert np.array_equal(a, b), i
+        assert np.array_equal(a, a2), i
+    for i in (0, 3):
+        fd, pal, job = items[i]
+        job.target_samples = 0
+        job.persistent_orbits = 0
+        orgba, _, _ = O.render(fd, pal, job, threads=8)
+        d = np.abs(outs_b[i].astype(int) - orgba.astype(int))
+        say(f"  flame {i}: {job.view.width}x{job.view.height} max diff {d.max()} frac>1 {(d > 1).mean():.5f}")
+        # transcendental flame: trajectories diverge after a few ulp, so the
+        # images agree statistically (block means), not pixel for pixel
+        ga, oa = outs_b[i][..., :3].astype(float), orgba[..., :3].astype(float)
+        b = 16
+        h, w = ga.shape[0] // b * b, ga.shape[1] // b * b
+        gm = ga[:h, :w].reshape(h // b, b, w // b, b, 3).mean((1, 3))
+        om = oa[:h, :w].reshape(h // b, b, w // b, b, 3).mean((1, 3))
+        assert np.abs(gm - om).mean() < 2.0
+
+
 # ------------------------------------------------------------ error paths
 
 def test_errors_are_status_codes(engine):
