@@ -1109,6 +1109,7 @@ int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const f
   hipStream_t s = c->stream;
   HIPCHECK(c, hipMemcpyAsync(P.tables.p, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, s));
   HIPCHECK(c, hipMemcpyAsync(P.pals.p, pals.data(), pals.size() * 4, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipStreamSynchronize(s));  // the host vectors die on return
   return FFM_OK;
 }
 
