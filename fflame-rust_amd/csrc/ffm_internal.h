@@ -148,6 +148,7 @@ struct BatchArgs {
   const uint32_t* pal_u32;
   int32_t max_rows, max_pal;
   uint32_t count_div;  // pre-pass walks 1/count_div of every item
+  uint32_t* next_item; // work-stealing counter of this launch (zeroed before it)
   TileArgs g;          // global tile arrays; g.hist = base of the histograms
 };
 

@@ -864,7 +864,7 @@ int ffm_render_accum(ffm_ctx* c, const ffm_job* job, float* accum_out, ffm_rende
 
 namespace {
 
-constexpr uint64_t BATCH_RECORD_BUDGET = 1ull << 31;  // records staged per wave (8 GB)
+constexpr uint64_t BATCH_RECORD_BUDGET = 3ull << 30;  // records staged per wave (12 GB)
 constexpr int BATCH_PALETTE = 256;
 constexpr uint32_t BATCH_COUNT_DIV = 8;  // the pre-pass walks 1/8 of wave 0
 
@@ -878,7 +878,8 @@ struct BatchPlan {
   std::vector<uint32_t> wave_start;  // items of wave w: [wave_start[w], wave_start[w+1])
   std::vector<BatchPix> pix;
   std::vector<uint64_t> samples;     // plotted samples per flame
-  DevBuf flames, items, tables, pals, pixd, hist, rgba, fill, region, cap, item_start, records;
+  DevBuf flames, items, tables, pals, pixd, hist, rgba, fill, region, cap, item_start, records,
+      counters;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void drop_graph() {
@@ -890,7 +891,7 @@ struct BatchPlan {
   void release() {
     drop_graph();
     for (DevBuf* b : {&flames, &items, &tables, &pals, &pixd, &hist, &rgba, &fill, &region, &cap,
-                      &item_start, &records})
+                      &item_start, &records, &counters})
       b->release();
   }
 };
@@ -1023,6 +1024,9 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   if ((st = ensure(c, P.cap, tiles_pad * 4, "batch tiles"))) return st;
   if ((st = ensure(c, P.item_start, tiles_pad * 4, "batch tiles"))) return st;
   if ((st = ensure(c, P.records, BATCH_RECORD_BUDGET * 4, "batch records"))) return st;
+  const size_t n_counters = ((size_t)P.n_waves + 1 + 3) & ~(size_t)3;
+  if ((st = ensure(c, P.counters, n_counters * 4, "batch counters"))) return st;
+  uint32_t* counters = reinterpret_cast<uint32_t*>(P.counters.p);
   hipStream_t s = c->stream;
   HIPCHECK(c, hipMemcpyAsync(P.flames.p, bf.data(), bf.size() * sizeof(BatchFlame),
                              hipMemcpyHostToDevice, s));
@@ -1056,9 +1060,11 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     hipError_t e;
     if ((e = launch_clear_u32(P.hist.p, P.total_pix * 4, s)) != hipSuccess) return e;
     if ((e = launch_clear_u32(P.fill.p, tiles_pad, s)) != hipSuccess) return e;
+    if ((e = launch_clear_u32(counters, n_counters, s)) != hipSuccess) return e;
     BatchArgs w0 = b;
     w0.items = items_d;
     w0.n_items = (int32_t)P.wave_start[1];
+    w0.next_item = counters + P.n_waves;
     if ((e = launch_bin_batch(w0, true, grid, s)) != hipSuccess) return e;
     if ((e = launch_tile_capacity_n(b.g, (float)BATCH_COUNT_DIV, 1024, BATCH_RECORD_BUDGET, s)) !=
         hipSuccess)
@@ -1067,6 +1073,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
       BatchArgs bw = b;
       bw.items = items_d + P.wave_start[w];
       bw.n_items = (int32_t)(P.wave_start[w + 1] - P.wave_start[w]);
+      bw.next_item = counters + w;
       if (w > 0 && (e = launch_clear_u32(P.fill.p, tiles_pad, s)) != hipSuccess) return e;
       if ((e = launch_bin_batch(bw, false, grid, s)) != hipSuccess) return e;
       if ((e = launch_tile_accum_batch(bw, grid, s)) != hipSuccess) return e;
@@ -1146,7 +1153,7 @@ int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const floa
     if (t.sig == sig) tune = &t;
   float gpu_ms = 0.0f;
   if (!tune) {
-    static const uint32_t kItems[] = {16384, 65536};
+    static const uint32_t kItems[] = {2048, 8192, 32768};
     static const int kFlush[] = {2, 4, 8};
     float best = 1e30f;
     Tuning tb{sig, 65536, 4};

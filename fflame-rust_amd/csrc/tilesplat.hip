@@ -473,7 +473,14 @@ __global__ __launch_bounds__(BIN_BLOCK) void bin_batch_kernel(BatchArgs b) {
   if (tid == 0) *L.fill = 0u;
   __syncthreads();
   unsigned long long n_plotted = 0, n_in = 0;
-  for (int item = blockIdx.x; item < b.n_items; item += gridDim.x) {
+  __shared__ int next;
+  for (;;) {
+    // dynamic work stealing: items differ in cost (transform counts, sizes)
+    if (tid == 0) next = (int)atomicAdd(b.next_item, 1u);
+    __syncthreads();
+    const int item = next;
+    __syncthreads();
+    if (item >= b.n_items) break;
     const BatchItem it = b.items[item];
     const BatchFlame f = b.flames[it.flame];
     const LdsFlame F = load_table<false>(b.tables + f.table_off, b.pal_u32 + f.pal_off, nullptr,
