@@ -68,6 +68,10 @@ struct IterArgs {
   unsigned long long* inframe;
 };
 
+// Dynamic LDS a kernel may request: gfx950 has 160 KB per workgroup; 8 KB
+// stay for the kernels' static __shared__ variables.
+constexpr int LDS_DYNAMIC_MAX = 152 * 1024;
+
 // Screen tiles of the LDS splat: 128x64 pixels = 128 KB of u32x4 in LDS.
 constexpr int TILE_W = 128, TILE_H = 64, TILE_PX = TILE_W * TILE_H;
 constexpr int TILE_MAX = 2048;  // tiles per image the LDS counting sort holds

@@ -150,7 +150,7 @@ static size_t lds_bytes(const IterArgs& a, int nrows, bool f32pal) {
 // A 128-transform flame with a 4096-texel palette needs more than the
 // default 64 KB of dynamic LDS.
 static hipError_t allow_lds(const void* fn) {
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_DYNAMIC_MAX);
 }
 
 hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block, hipStream_t s) {

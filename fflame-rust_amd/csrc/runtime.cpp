@@ -280,7 +280,7 @@ int iterate_grid(ffm_ctx* c, uint64_t walkers) {
 
 constexpr int XCD_COPIES = 8;
 constexpr uint64_t RECORD_BUDGET = 1ull << 31;  // staged samples held at once (8 GB)
-constexpr size_t LDS_PER_CU = 160 * 1024;
+constexpr size_t LDS_PER_CU = LDS_DYNAMIC_MAX;
 
 // The LDS tile splat of one dispatch (see tilesplat.hip): walkers in chunks
 // whose records fit the buffer; region capacities from a count-only
@@ -1056,6 +1056,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   const int grid = std::max(c->num_cu, 1);
   const BatchItem* items_d = reinterpret_cast<const BatchItem*>(P.items.p);
   HIPCHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const char* step = "start";
   auto body = [&]() -> hipError_t {
     hipError_t e;
     if ((e = launch_clear_u32(P.hist.p, P.total_pix * 4, s)) != hipSuccess) return e;
@@ -1065,7 +1066,9 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     w0.items = items_d;
     w0.n_items = (int32_t)P.wave_start[1];
     w0.next_item = counters + P.n_waves;
+    step = "count pre-pass";
     if ((e = launch_bin_batch(w0, true, grid, s)) != hipSuccess) return e;
+    step = "capacities";
     if ((e = launch_tile_capacity_n(b.g, (float)BATCH_COUNT_DIV, 1024, BATCH_RECORD_BUDGET, s)) !=
         hipSuccess)
       return e;
@@ -1075,9 +1078,12 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
       bw.n_items = (int32_t)(P.wave_start[w + 1] - P.wave_start[w]);
       bw.next_item = counters + w;
       if (w > 0 && (e = launch_clear_u32(P.fill.p, tiles_pad, s)) != hipSuccess) return e;
+      step = "walkers";
       if ((e = launch_bin_batch(bw, false, grid, s)) != hipSuccess) return e;
+      step = "tile accumulate";
       if ((e = launch_tile_accum_batch(bw, grid, s)) != hipSuccess) return e;
     }
+    step = "accumulate + tonemap";
     return launch_batch_finish(reinterpret_cast<const uint32_t*>(P.hist.p),
                                reinterpret_cast<uint8_t*>(P.rgba.p),
                                reinterpret_cast<const BatchPix*>(P.pixd.p), n, s);
@@ -1087,7 +1093,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   hipError_t ee = hipStreamEndCapture(s, &g);
   if (eb != hipSuccess || ee != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
-    return fail(c, FFM_ERR_DEVICE, "capturing the batch: %s",
+    return fail(c, FFM_ERR_DEVICE, "capturing the batch (%s): %s", step,
                 hipGetErrorString(eb != hipSuccess ? eb : ee));
   }
   P.graph = g;

@@ -506,8 +506,6 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_batch_kernel(BatchArgs b
   extern __shared__ uint32_t lds[];
   uint32_t* tile = lds;
   uint32_t* pal = lds + 4 * TILE_PX;
-  __shared__ uint32_t bases[1];
-  (void)bases;
   const int tid = threadIdx.x;
   const TileArgs& g = b.g;
   const uint32_t n_items = g.item_start[g.n_tiles];
@@ -536,7 +534,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_batch_kernel(BatchArgs b
 
 // Kernels here use more than the default 64 KB of dynamic LDS.
 static hipError_t allow_lds(const void* fn) {
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_DYNAMIC_MAX);
 }
 
 hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_only, int grid,
