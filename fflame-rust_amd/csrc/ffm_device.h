@@ -61,7 +61,7 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
     float mag = (__float_as_uint(x) & 0x80000000u) ? PI_ATAN : 0.0f;
     return (__float_as_uint(y) & 0x80000000u) ? -mag : mag;
   }
-  return atan2f(y, x);
+  return ffm_atan2f(y, x);
 }
 
 // One variation body V(x, y). `row` is the transform's table row (waves
@@ -101,14 +101,14 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_POLAR: {
-      float theta = atan2f(x, y);
+      float theta = ffm_atan2f(x, y);
       float r = sqrtf(x * x + y * y);
       ox = theta / PI_F;
       oy = r - 1.0f;
       break;
     }
     case FFM_VAR_DISC: {
-      float theta = atan2f(x, y);
+      float theta = ffm_atan2f(x, y);
       float r = sqrtf(x * x + y * y);
       float theta_pi = theta / PI_F;
       float pi_r = PI_F * r;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     }
     case FFM_VAR_JULIA: {
       float r = sqrtf(x * x + y * y);
-      float theta = atan2f(y, x);
+      float theta = ffm_atan2f(y, x);
       float sqrt_r = sqrtf(r);
       float omega = (rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
       float half_theta = theta / 2.0f + omega;

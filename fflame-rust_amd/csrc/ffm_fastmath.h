@@ -62,6 +62,39 @@ FFM_FM_INLINE void ffm_sincosf(float x, float* s, float* c) {
   }
 }
 
+// atan2 by reduction to a = min(|x|,|y|) / max(|x|,|y|) in [0, 1], then
+// a -> (a - 1) / (a + 1) above tan(pi/8) (adding pi/4), and Cephes atanf's
+// minimax polynomial on [-tan(pi/8), tan(pi/8)]:
+//   atan z = z + z^3 (-3.33329491539e-1 + z^2 (1.99777106478e-1
+//                     + z^2 (-1.38776856032e-1 + z^2 8.05374449538e-2)))
+// Octant fix-ups use the operands' sign bits, so the signed-zero cases
+// follow IEEE atan2: atan2(+-0, +0) = +-0, atan2(+-0, -0) = +-pi.
+FFM_FM_INLINE float ffm_atan2f(float y, float x) {
+  // pi/4, pi/2, pi as float + remainder (two-part constants)
+  const float PIO4_HI = 0.785398185253143310546875f, PIO4_LO = -2.18556941e-8f;
+  const float PIO2_HI = 1.57079637050628662109375f, PIO2_LO = -4.37113883e-8f;
+  const float PI_HI = 3.1415927410125732421875f, PI_LO = -8.74227766e-8f;
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  float a = mx > 0.0f ? mn / mx : 0.0f;
+  const int reduced = a > 0.41421356237309503f;
+  if (reduced) a = (a - 1.0f) / (a + 1.0f);
+  const float z = a * a;
+  const float p = fmaf(fmaf(fmaf(8.05374449538e-2f, z, -1.38776856032e-1f), z, 1.99777106478e-1f),
+                       z, -3.33329491539e-1f);
+  const float q = p * z * a;  // atan(a) = a + q
+  float t = reduced ? PIO4_HI + (a + (q + PIO4_LO)) : a + q;
+  if (ay > ax) t = (PIO2_HI - t) + PIO2_LO;
+#ifdef __HIPCC__
+  const bool xneg = (__float_as_uint(x) & 0x80000000u) != 0u;
+  const bool yneg = (__float_as_uint(y) & 0x80000000u) != 0u;
+#else
+  const int xneg = signbit(x) != 0, yneg = signbit(y) != 0;
+#endif
+  if (xneg) t = (PI_HI - t) + PI_LO;
+  return yneg ? -t : t;
+}
+
 FFM_FM_INLINE float ffm_sinf(float x) {
   float s, c;
   ffm_sincosf(x, &s, &c);
