@@ -64,6 +64,15 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
   return ffm_atan2f(y, x);
 }
 
+// The variation id is uniform across the wave, so `switch (id)` is a scalar
+// branch. Every body depends only on the affine point, which does not
+// change over the loop of active variations, so without a barrier the
+// compiler hoists ALL bodies' arithmetic (divisions, sqrt, sincos, atan2)
+// out of that loop and pays for every one on every transform evaluation.
+// Passing the point through an empty volatile asm at the top of each case
+// makes it opaque there, which keeps each body behind its branch.
+#define FFM_NO_SPECULATE() asm volatile("" : "+v"(x), "+v"(y))
+
 // One variation body V(x, y). `row` is the transform's table row (waves
 // reads its own b, e, d, f), `p` the variation's params. Bodies:
 // src/variations/defs/basic.rs:37-205, advanced.rs:32-1700.
@@ -76,16 +85,19 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       oy = y;
       break;
     case FFM_VAR_SINUSOIDAL:
+      FFM_NO_SPECULATE();
       ox = ffm_sinf(x);
       oy = ffm_sinf(y);
       break;
     case FFM_VAR_SPHERICAL: {
+      FFM_NO_SPECULATE();
       float r2 = x * x + y * y + 1e-6f;
       ox = x / r2;
       oy = y / r2;
       break;
     }
     case FFM_VAR_SWIRL: {
+      FFM_NO_SPECULATE();
       float r2 = x * x + y * y;
       float s, c;
       ffm_sincosf(r2, &s, &c);
@@ -94,6 +106,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_HORSESHOE: {
+      FFM_NO_SPECULATE();
       float r = sqrtf(x * x + y * y) + 1e-6f;
       float r_inv = 1.0f / r;
       ox = (x - y) * (x + y) * r_inv;
@@ -101,6 +114,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_POLAR: {
+      FFM_NO_SPECULATE();
       float theta = ffm_atan2f(x, y);
       float r = sqrtf(x * x + y * y);
       ox = theta / PI_F;
@@ -108,6 +122,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_DISC: {
+      FFM_NO_SPECULATE();
       float theta = ffm_atan2f(x, y);
       float r = sqrtf(x * x + y * y);
       float theta_pi = theta / PI_F;
@@ -119,12 +134,14 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_HYPERBOLIC: {
+      FFM_NO_SPECULATE();
       float r2 = x * x + y * y + 1e-6f;
       ox = x / r2;
       oy = y;
       break;
     }
     case FFM_VAR_JULIA: {
+      FFM_NO_SPECULATE();
       float r = sqrtf(x * x + y * y);
       float theta = ffm_atan2f(y, x);
       float sqrt_r = sqrtf(r);
@@ -137,12 +154,14 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_WAVES: {
+      FFM_NO_SPECULATE();
       float b = row[XF_B], e = row[XF_E], d = row[XF_D], f = row[XF_F];
       ox = x + b * ffm_sinf(y / (e * e + 1e-6f));
       oy = y + d * ffm_sinf(x / (f * f + 1e-6f));
       break;
     }
     case FFM_VAR_JULIAN: {
+      FFM_NO_SPECULATE();
       float power = p[0];
       float cpower = p[2];
       float r2 = x * x + y * y;
@@ -157,6 +176,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_BUBBLE: {
+      FFM_NO_SPECULATE();
       float r2 = x * x + y * y;
       float r = r2 / 4.0f + 1.0f;
       float scale = 1.0f / r;
@@ -165,6 +185,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_CURL: {
+      FFM_NO_SPECULATE();
       float c1 = p[0], c2 = p[1];
       float re = 1.0f + c1 * x + c2 * (x * x - y * y);
       float im = c1 * y + 2.0f * c2 * x * y;
