@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "ffm_internal.h"
+#include "ffm_soft.h"
 
 using namespace ffm;
 
@@ -65,6 +66,13 @@ struct ffm_ctx {
   std::vector<int> kev_kind;  // 0 walker kernel, 1 tile accumulate
   size_t kev_used = 0;
   struct ffm_batch_state* batch = nullptr;
+  // differentiable soft splat: the fits set last
+  std::vector<SoftFit> s_fits;
+  DevBuf s_fits_d, s_tables, s_ckpt, s_part;
+  uint64_t s_theta = 0, s_pixels = 0, s_ckpt_bytes = 0, s_part_n = 0;
+  uint32_t s_blocks = 0;
+  size_t s_fwd_lds = 0, s_adj_lds32 = 0, s_adj_lds64 = 0;
+  bool s_any32 = false, s_any64 = false;
 };
 
 namespace {
@@ -507,7 +515,8 @@ int ffm_ctx_destroy(ffm_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->table, &c->pal_u32, &c->pal_f32, &c->hist, &c->copies, &c->scratch,
                     &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
-                    &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records})
+                    &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records,
+                    &c->s_fits_d, &c->s_tables, &c->s_ckpt, &c->s_part})
     b->release();
   free_batch(c->batch);
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
@@ -1257,16 +1266,196 @@ ffm_batch_state* batch_state(ffm_ctx* c) {
 }
 }  // namespace
 
-extern "C" {
+// ---- differentiable soft splat
 
-int ffm_softsplat_forward(ffm_ctx* c, const ffm_view*, uint64_t, uint32_t, uint32_t, uint32_t,
-                          float*, void*) {
-  return fail(c, FFM_ERR_INVALID_ARGUMENT, "soft splat is not built yet");
+namespace {
+
+// Parameters of a fittable flame, or a negative status with the reason.
+int soft_params(ffm_ctx* c, const ffm_flame_desc* f) {
+  if (!f) return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame is NULL");
+  if (f->n_finals != 0)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "soft-splat flames have no final transforms");
+  if (f->n_xforms < 1 || f->n_xforms > FFM_SOFT_MAX_XFORMS)
+    return fail(c, FFM_ERR_TOO_LARGE, "%d transforms: a soft-splat flame has 1..%d", f->n_xforms,
+                FFM_SOFT_MAX_XFORMS);
+  if (f->n_vars < 1 || f->n_vars > FFM_SOFT_MAX_VARS)
+    return fail(c, FFM_ERR_TOO_LARGE, "%d variations: a soft-splat flame has 1..%d", f->n_vars,
+                FFM_SOFT_MAX_VARS);
+  for (int k = 0; k < f->n_vars; k++) {
+    if (!supported_var(f->var_id[k]))
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "variation id %d is not supported", f->var_id[k]);
+    if (f->var_id[k] == FFM_VAR_WAVES)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                  "waves reads the affine coefficients itself and has no soft-splat adjoint");
+  }
+  return f->n_xforms * (6 + f->n_vars);
 }
 
-int ffm_softsplat_adjoint(ffm_ctx* c, const ffm_view*, uint64_t, uint32_t, uint32_t, uint32_t,
-                          uint32_t, const float*, float*, void*) {
-  return fail(c, FFM_ERR_INVALID_ARGUMENT, "soft splat is not built yet");
+int check_soft_job(ffm_ctx* c, const ffm_soft_job* j) {
+  int st = check_view(c, &j->view);
+  if (st) return st;
+  if (j->walkers < 1 || j->walkers > (1ull << 32))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers must be 1..2^32 (got %llu)",
+                (unsigned long long)j->walkers);
+  if (j->ipt < 1) return fail(c, FFM_ERR_INVALID_ARGUMENT, "ipt must be at least 1");
+  if (j->window < 1 || j->window > FFM_SOFT_MAX_WINDOW)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "window must be 1..%d (got %u)", FFM_SOFT_MAX_WINDOW,
+                j->window);
+  if (!std::isfinite(j->density_scale))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "density_scale is not finite");
+  return FFM_OK;
+}
+
+SoftArgs soft_args(ffm_ctx* c) {
+  SoftArgs a{};
+  a.fits = reinterpret_cast<const SoftFit*>(c->s_fits_d.p);
+  a.n_fits = (int)c->s_fits.size();
+  a.tables = reinterpret_cast<const float*>(c->s_tables.p);
+  a.ckpt = reinterpret_cast<unsigned char*>(c->s_ckpt.p);
+  a.partials = reinterpret_cast<double*>(c->s_part.p);
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffm_softsplat_param_count(const ffm_flame_desc* f) { return soft_params(nullptr, f); }
+
+int ffm_softsplat_pack(const ffm_flame_desc* f, float* theta) {
+  int P = soft_params(nullptr, f);
+  if (P < 0) return P;
+  if (!theta) return FFM_ERR_INVALID_ARGUMENT;
+  const int n = f->n_xforms, v = f->n_vars;
+  for (int i = 0; i < n; i++) {
+    const ffm_xform& x = f->xforms[i];
+    const float aff[6] = {x.a, x.b, x.c, x.d, x.e, x.f};
+    for (int q = 0; q < 6; q++) theta[6 * i + q] = aff[q];
+    for (int k = 0; k < v; k++) theta[6 * n + v * i + k] = x.var_weight[k];
+  }
+  return FFM_OK;
+}
+
+int ffm_softsplat_set(ffm_ctx* c, int n, const ffm_flame_desc* flames, const ffm_soft_job* jobs) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (n < 1 || !flames || !jobs) return fail(c, FFM_ERR_INVALID_ARGUMENT, "no fits given");
+  if (n > 65535) return fail(c, FFM_ERR_TOO_LARGE, "%d fits: at most 65535 in one set", n);
+  (void)hipSetDevice(c->device);
+  const int B = soft_block();
+  std::vector<SoftFit> fits((size_t)n);
+  std::vector<float> tables;
+  uint64_t theta = 0, pix = 0, ck = 0, part = 0, blocks = 0;
+  size_t fwd_lds = 0, adj32 = 0, adj64 = 0;
+  for (int i = 0; i < n; i++) {
+    int P = soft_params(c, &flames[i]);
+    if (P < 0) return fail(c, P, "fit %d: %s", i, c->err.c_str());
+    int st = check_soft_job(c, &jobs[i]);
+    if (st) return fail(c, st, "fit %d: %s", i, c->err.c_str());
+    ffm_flame_desc f = flames[i];
+    prepare_flame(&f);
+    std::vector<float> t = pack_flame(&f);
+    const ffm_soft_job& j = jobs[i];
+    SoftFit& F = fits[(size_t)i];
+    F.table_off = (uint32_t)tables.size();
+    tables.insert(tables.end(), t.begin(), t.end());
+    tables.resize((tables.size() + 3) & ~(size_t)3, 0.0f);
+    F.n_xforms = f.n_xforms;
+    F.n_vars = f.n_vars;
+    F.n_params = P;
+    F.view = make_view(j.view);
+    F.walkers = j.walkers;
+    F.ipt = j.ipt;
+    F.burn_in = j.burn_in;
+    F.seed = j.seed;
+    F.window = j.window;
+    F.truncate = j.truncate ? 1 : 0;
+    F.f64 = j.walk_f64 ? 1 : 0;
+    F.density_scale = j.density_scale;
+    F.theta_off = theta;
+    F.pix_off = pix;
+    F.ckpt_off = ck;
+    F.part_off = part;
+    F.block0 = (uint32_t)blocks;
+    F.n_blocks = (uint32_t)((j.walkers + B - 1) / B);
+    theta += (uint64_t)P;
+    pix += (uint64_t)j.view.width * (uint64_t)j.view.height;
+    if (!F.truncate) ck += (soft_ckpt_bytes(j.ipt, j.window, j.walkers, F.f64) + 255) & ~255ull;
+    part += (uint64_t)P * F.n_blocks;
+    blocks += F.n_blocks;
+    if (blocks > 0x7fffffffull)
+      return fail(c, FFM_ERR_TOO_LARGE, "the fits need %llu blocks (at most 2^31 - 1)",
+                  (unsigned long long)blocks);
+    fwd_lds = std::max(fwd_lds, soft_forward_lds_bytes(f.n_xforms));
+    size_t& adj = F.f64 ? adj64 : adj32;
+    adj = std::max(adj, soft_adjoint_lds_bytes(f.n_xforms, P, (int)j.window, F.f64));
+  }
+  if (std::max(adj32, adj64) > (size_t)LDS_DYNAMIC_MAX)
+    return fail(c, FFM_ERR_TOO_LARGE, "adjoint needs %zu bytes of LDS (at most %d)",
+                std::max(adj32, adj64), LDS_DYNAMIC_MAX);
+  int st;
+  if ((st = ensure(c, c->s_fits_d, fits.size() * sizeof(SoftFit), "soft-splat fits")) ||
+      (st = ensure(c, c->s_tables, tables.size() * 4, "soft-splat tables")))
+    return st;
+  HIPCHECK(c, hipMemcpyAsync(c->s_fits_d.p, fits.data(), fits.size() * sizeof(SoftFit),
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->s_tables.p, tables.data(), tables.size() * 4,
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->s_fits = std::move(fits);
+  c->s_theta = theta;
+  c->s_pixels = pix;
+  c->s_ckpt_bytes = ck;
+  c->s_part_n = part;
+  c->s_blocks = (uint32_t)blocks;
+  c->s_fwd_lds = fwd_lds;
+  c->s_adj_lds32 = adj32;
+  c->s_adj_lds64 = adj64;
+  c->s_any32 = c->s_any64 = false;
+  for (const SoftFit& F : c->s_fits) (F.f64 ? c->s_any64 : c->s_any32) = true;
+  return FFM_OK;
+}
+
+int ffm_softsplat_sizes(const ffm_ctx* c, uint64_t* theta_floats, uint64_t* density_floats) {
+  if (!c || !theta_floats || !density_floats) return FFM_ERR_INVALID_ARGUMENT;
+  *theta_floats = c->s_theta;
+  *density_floats = c->s_pixels;
+  return c->s_fits.empty() ? FFM_ERR_NO_FLAME : FFM_OK;
+}
+
+int ffm_softsplat_forward(ffm_ctx* c, const float* theta, float* density, void* stream) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (c->s_fits.empty()) return fail(c, FFM_ERR_NO_FLAME, "no soft-splat fits set");
+  if (!theta || !density) return fail(c, FFM_ERR_INVALID_ARGUMENT, "theta or density is NULL");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = pick(c, stream);
+  SoftArgs a = soft_args(c);
+  a.theta = theta;
+  a.density = density;
+  HIPCHECK(c, launch_clear_u32(density, c->s_pixels, s));
+  HIPCHECK(c, launch_soft_forward(a, c->s_blocks, c->s_fwd_lds, c->s_any32, c->s_any64, s));
+  return FFM_OK;
+}
+
+int ffm_softsplat_adjoint(ffm_ctx* c, const float* theta, const float* grad_density,
+                          float* grad_theta, void* stream) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (c->s_fits.empty()) return fail(c, FFM_ERR_NO_FLAME, "no soft-splat fits set");
+  if (!theta || !grad_density || !grad_theta)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "theta, grad_density or grad_theta is NULL");
+  (void)hipSetDevice(c->device);
+  hipStream_t s = pick(c, stream);
+  int st;
+  if ((st = ensure(c, c->s_ckpt, std::max<uint64_t>(c->s_ckpt_bytes, 256),
+                   "adjoint checkpoints")) ||
+      (st = ensure(c, c->s_part, c->s_part_n * 8, "adjoint partial sums")))
+    return st;
+  SoftArgs a = soft_args(c);
+  a.theta = theta;
+  a.grad_density = grad_density;
+  a.grad_theta = grad_theta;
+  HIPCHECK(c, launch_soft_adjoint(a, c->s_blocks, c->s_any32 ? c->s_adj_lds32 : 0,
+                                  c->s_any64 ? c->s_adj_lds64 : 0, s));
+  return FFM_OK;
 }
 
 }  // extern "C"

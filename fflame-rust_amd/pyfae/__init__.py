@@ -16,8 +16,8 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (FlameDesc, Job, RenderStats, TonemapParams, View, Xform,  # noqa: F401
-                   VARIATIONS, tonemap_defaults)
+from ._abi import (FlameDesc, Job, RenderStats, SoftJob, TonemapParams, View,  # noqa: F401
+                   Xform, VARIATIONS, tonemap_defaults)
 
 __version__ = "0.5.0+mi355x"
 
@@ -150,6 +150,39 @@ def make_job(width, height, *, walkers=8192, ipt=256, burn_in=20, seed_base=1234
     return j
 
 
+def soft_job(width, height, *, walkers=1 << 20, ipt=64, burn_in=20, seed=1, window=8,
+             truncate=False, density_scale=None, walk_f64=True, zoom=1.0, pan_x=0.0, pan_y=0.0,
+             rotation=0.0):
+    """An ffm_soft_job. density_scale defaults to W*H / (walkers * (ipt - burn_in)),
+    so an attractor that fills the frame has mean density 1. walk_f64 walks
+    (and differentiates) in f64; False uses the chaos game's f32 arithmetic."""
+    j = SoftJob()
+    j.view = View(width, height, zoom, pan_x, pan_y, rotation)
+    j.walkers = walkers
+    j.ipt = ipt
+    j.burn_in = burn_in
+    j.seed = seed & 0xFFFFFFFF
+    j.window = window
+    j.truncate = int(bool(truncate))
+    if density_scale is None:
+        density_scale = width * height / float(walkers * max(ipt - burn_in, 1))
+    j.density_scale = density_scale
+    j.walk_f64 = int(bool(walk_f64))
+    return j
+
+
+def soft_theta(fd):
+    """The soft-splat parameter vector of a flame (float64): per transform
+    (a, b, c, d, e, f), then per transform its n_vars variation weights."""
+    n, v = fd.n_xforms, fd.n_vars
+    th = np.zeros(n * (6 + v), dtype=np.float64)
+    for i in range(n):
+        x = fd.xforms[i]
+        th[6 * i:6 * i + 6] = (x.a, x.b, x.c, x.d, x.e, x.f)
+        th[6 * n + v * i:6 * n + v * (i + 1)] = [x.var_weight[k] for k in range(v)]
+    return th
+
+
 # ---------------------------------------------------------------- engine
 
 class Engine:
@@ -247,6 +280,28 @@ class Engine:
         fn = self.lib.ffm_batch_render_serial if serial else self.lib.ffm_batch_render
         self._check(fn(self.ctx, n, flames, sp, ns, jobs, op, C.byref(stats)))
         return outs, stats
+
+    # -- differentiable soft splat (device tensors; see pyfae.diff for autograd)
+    def softsplat_set(self, fits):
+        """Set [(flame_desc, soft_job), ...]; returns (theta floats, density floats)."""
+        n = len(fits)
+        flames = (FlameDesc * n)()
+        jobs = (SoftJob * n)()
+        for i, (fd, job) in enumerate(fits):
+            C.memmove(C.byref(flames[i]), C.byref(fd), C.sizeof(FlameDesc))
+            C.memmove(C.byref(jobs[i]), C.byref(job), C.sizeof(SoftJob))
+        self._check(self.lib.ffm_softsplat_set(self.ctx, n, flames, jobs))
+        t, d = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.ffm_softsplat_sizes(self.ctx, C.byref(t), C.byref(d)))
+        return t.value, d.value
+
+    def softsplat_forward(self, theta, density, stream=None):
+        self._check(self.lib.ffm_softsplat_forward(self.ctx, _ptr(theta), _ptr(density),
+                                                   _ptr(stream)))
+
+    def softsplat_adjoint(self, theta, grad_density, grad_theta, stream=None):
+        self._check(self.lib.ffm_softsplat_adjoint(self.ctx, _ptr(theta), _ptr(grad_density),
+                                                   _ptr(grad_theta), _ptr(stream)))
 
     # -- granular kernels on device buffers
     def iterate(self, view, walkers, ipt, burn_in, seed, hist, *, orbits=None, fresh=True,

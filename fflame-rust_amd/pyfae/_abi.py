@@ -77,6 +77,14 @@ class TonemapParams(C.Structure):
     ]
 
 
+class SoftJob(C.Structure):
+    """ffm_soft_job: one differentiable soft-splat fit."""
+    _fields_ = [("view", View), ("walkers", C.c_uint64), ("ipt", C.c_uint32),
+                ("burn_in", C.c_uint32), ("seed", C.c_uint32), ("window", C.c_uint32),
+                ("truncate", C.c_uint32), ("density_scale", C.c_float),
+                ("walk_f64", C.c_uint32)]
+
+
 class Job(C.Structure):
     _fields_ = [
         ("view", View),
@@ -158,10 +166,12 @@ SIGNATURES = [
     ("ffm_batch_render_serial", C.c_int, [_vp, C.c_int, C.POINTER(FlameDesc), C.POINTER(_fp),
                                           C.POINTER(C.c_int), C.POINTER(Job), C.POINTER(_u8p),
                                           C.POINTER(RenderStats)]),
-    ("ffm_softsplat_forward", C.c_int, [_vp, C.POINTER(View), C.c_uint64, C.c_uint32, C.c_uint32,
-                                        C.c_uint32, _vp, _vp]),
-    ("ffm_softsplat_adjoint", C.c_int, [_vp, C.POINTER(View), C.c_uint64, C.c_uint32,
-                                        C.c_uint32, C.c_uint32, C.c_uint32, _vp, _vp, _vp]),
+    ("ffm_softsplat_param_count", C.c_int, [C.POINTER(FlameDesc)]),
+    ("ffm_softsplat_pack", C.c_int, [C.POINTER(FlameDesc), _fp]),
+    ("ffm_softsplat_set", C.c_int, [_vp, C.c_int, C.POINTER(FlameDesc), C.POINTER(SoftJob)]),
+    ("ffm_softsplat_sizes", C.c_int, [_vp, _u64p, _u64p]),
+    ("ffm_softsplat_forward", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("ffm_softsplat_adjoint", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
 ]
 
 

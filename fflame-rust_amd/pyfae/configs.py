@@ -10,18 +10,20 @@ palette stops) so the same descriptor drives the GPU and the oracle.
   single4k   6 xforms + final, 3 variations each, 3840x2160, 2^24 x 256
   gallery    512 flames, 2..12 xforms, 128^2..1024^2, 200 samples/pixel
   fit        4 xforms {linear, spherical, sinusoidal}, 256x256 soft splat
+  fit_batch  64 independent 3-xform soft-splat fits at 128x128
   stress     12 xforms incl. julian/curl/waves, 12288^2 f32 histogram
 """
 import math
 
 import numpy as np
 
-from . import flame_desc, make_job, tonemap_defaults, xform
+from . import flame_desc, make_job, soft_job, tonemap_defaults, xform
 
 SEED_STRIDE = 2654435761
 C1_VARIATIONS = ["linear", "sinusoidal", "spherical", "swirl", "horseshoe", "polar", "julia",
                  "bubble", "hyperbolic", "disc"]
 C4_VARIATIONS = C1_VARIATIONS + ["julian", "curl", "waves"]
+FIT_VARIATIONS = ["linear", "spherical", "sinusoidal"]
 
 
 class SplitMix64:
@@ -161,21 +163,43 @@ def gallery(n=512, spp=200.0, ipt=256):
     return out
 
 
-def fit(seed=1003, perturb=0.0):
-    """4 xforms over {linear, spherical, sinusoidal} with weights U[0,1];
-    `perturb` adds N(0, perturb) to the affines (the target uses 0.05)."""
-    rng = SplitMix64(seed)
-    xs = []
-    for i in range(4):
+def _fit_flame(rng, n_xforms, perturb_rng, perturb):
+    xs, ts = [], []
+    for i in range(n_xforms):
         aff = list(_affine(rng, 0.8))
-        var = {nm: rng.uniform() for nm in ("linear", "spherical", "sinusoidal")}
-        xs.append(xform(*aff, weight=0.25, color=i / 3.0, color_speed=0.5, variations=var))
-    if perturb:
-        prng = SplitMix64(seed + 7777)
-        for t in xs:
-            for key in "abcdef":
-                t[key] += prng.normal(perturb)
-    return xs
+        var = {nm: rng.uniform() for nm in FIT_VARIATIONS}
+        xs.append(xform(*aff, weight=1.0 / n_xforms, color=i / max(n_xforms - 1, 1),
+                        color_speed=0.5, variations=var))
+        taff = [v + perturb_rng.normal(perturb) for v in aff]
+        ts.append(xform(*taff, weight=1.0 / n_xforms, color=i / max(n_xforms - 1, 1),
+                        color_speed=0.5, variations=dict(var)))
+    return flame_desc(xs), flame_desc(ts)
+
+
+def fit(walkers=1 << 20, ipt=64, side=256, window=8, truncate=False, seed=1003):
+    """Differentiable fit: 4 xforms over {linear, spherical, sinusoidal} with
+    weights U[0,1], affines at spectral norm 0.8, selection weights 1/4;
+    side^2 soft-splat density of walkers x ipt (burn-in 20), adjoint window K.
+    The target is the same flame with affines + N(0, 0.05), rendered at
+    walker seed 2 (the fit walks seed 1).
+    Returns (flame, target_flame, job, target_job)."""
+    fd, tfd = _fit_flame(SplitMix64(seed), 4, SplitMix64(seed + 7777), 0.05)
+    kw = dict(walkers=walkers, ipt=ipt, burn_in=20, window=window, truncate=truncate,
+              zoom=4.0 / 3.0)
+    return fd, tfd, soft_job(side, side, seed=1, **kw), soft_job(side, side, seed=2, **kw)
+
+
+def fit_batch(n=64, side=128, walkers=1 << 16, ipt=64, window=8):
+    """n independent 3-xform fits at side^2 (fit k from splitmix64(1004 * 100003 + k)),
+    each as `fit` builds them. Returns [(flame, target_flame, job, target_job)]."""
+    out = []
+    for k in range(n):
+        fd, tfd = _fit_flame(SplitMix64(1004 * 100003 + k), 3,
+                             SplitMix64(1004 * 100003 + k + 7777), 0.05)
+        kw = dict(walkers=walkers, ipt=ipt, burn_in=20, window=window, zoom=4.0 / 3.0)
+        out.append((fd, tfd, soft_job(side, side, seed=1 + k, **kw),
+                    soft_job(side, side, seed=100001 + k, **kw)))
+    return out
 
 
 def stress(walkers=1 << 26, ipt=256, side=12288):

@@ -286,21 +286,69 @@ int ffm_batch_render_serial(ffm_ctx* ctx, int n, const ffm_flame_desc* flames,
 
 /* ---- differentiable soft splat ---------------------------------------------- */
 
-/* Bilinear soft splat of the chaos game into an f32 density image (no
- * reference counterpart: splat_size is hard-wired to 1.0 and read by
- * nothing, src/renderer/compute_kernel.rs:1354). The differentiable
- * parameters are packed as theta = [n_xforms x 6 affine (a,b,c,d,e,f)]
- * then [n_xforms x n_vars variation weights]. */
-int ffm_softsplat_forward(ffm_ctx* ctx, const ffm_view* view, uint64_t walkers, uint32_t ipt,
-                          uint32_t burn_in, uint32_t seed, float* density, void* stream);
+/* Bilinear soft splat of the chaos game into an f32 density image, and its
+ * adjoint (no reference counterpart: splat_size is hard-wired to 1.0 and
+ * read by nothing, src/renderer/compute_kernel.rs:1354). The walk is the
+ * chaos game's (selection, opacity draw, transform, respawn, fuse); each
+ * plotted sample adds density_scale x its bilinear weights to the four
+ * pixels around its continuous position (world_to_pixel without the floor,
+ * minus half a pixel). Colour does not enter the density.
+ *
+ * The differentiable parameters of a flame with n transforms and v active
+ * variations are theta = [n x (a, b, c, d, e, f)] then [n x v weights]
+ * (the transforms' pre-affine coefficients and variation weights);
+ * everything else (selection weights, opacity, post-affine, variation
+ * params) comes from the flame descriptor. Several independent fits run in
+ * one launch: theta, density and grad arrays hold the fits back to back in
+ * the order given to ffm_softsplat_set. Soft-splat flames have no final
+ * transforms, at most FFM_SOFT_MAX_XFORMS transforms and
+ * FFM_SOFT_MAX_VARS variations, and no waves variation (its body reads the
+ * affine coefficients themselves). */
+#define FFM_SOFT_MAX_XFORMS 8
+#define FFM_SOFT_MAX_VARS 4
+#define FFM_SOFT_MAX_WINDOW 16
 
-/* Truncated adjoint: d loss / d theta where dL/d density is `grad_density`
- * (f32, width*height), through the last `K` iterations of each walker,
- * recomputing trajectories by RNG replay. `grad_theta` (device f32,
- * n_xforms*6 + n_xforms*n_vars) is overwritten. */
-int ffm_softsplat_adjoint(ffm_ctx* ctx, const ffm_view* view, uint64_t walkers, uint32_t ipt,
-                          uint32_t burn_in, uint32_t seed, uint32_t K,
-                          const float* grad_density, float* grad_theta, void* stream);
+typedef struct ffm_soft_job {
+  ffm_view view;        /* density image size and camera */
+  uint64_t walkers;     /* 1 .. 2^32 */
+  uint32_t ipt;         /* iterations per walker */
+  uint32_t burn_in;     /* fuse iterations before a walker plots */
+  uint32_t seed;        /* rng_init seed of every walker */
+  uint32_t window;      /* adjoint replay window K, 1 .. FFM_SOFT_MAX_WINDOW */
+  uint32_t truncate;    /* 0: exact gradient (dL/dpoint carried across windows);
+                           1: truncated, dL/dpoint restarts at zero each window */
+  float density_scale;  /* density = density_scale x sum of bilinear weights */
+  uint32_t walk_f64;    /* 0: the walk in f32, the chaos game's own arithmetic;
+                           1: in f64 (same transform picks and draws), for
+                           gradients that f32 rounding of long walks would blur */
+} ffm_soft_job;
+
+/* Number of differentiable parameters of `flame` (n * (6 + v)), or a
+ * negative status when the flame cannot be fitted. */
+int ffm_softsplat_param_count(const ffm_flame_desc* flame);
+
+/* theta of `flame` as the descriptor holds it (host floats). */
+int ffm_softsplat_pack(const ffm_flame_desc* flame, float* theta_out);
+
+/* Structure of n fits (copied): flames[i] is fitted on jobs[i]. */
+int ffm_softsplat_set(ffm_ctx* ctx, int n, const ffm_flame_desc* flames,
+                      const ffm_soft_job* jobs);
+
+/* Total floats of theta (and grad_theta) and of density (and grad_density)
+ * for the fits set last. */
+int ffm_softsplat_sizes(const ffm_ctx* ctx, uint64_t* theta_floats, uint64_t* density_floats);
+
+/* density (device f32) := the soft splat of every fit at theta (device f32). */
+int ffm_softsplat_forward(ffm_ctx* ctx, const float* theta, float* density, void* stream);
+
+/* grad_theta (device f32, overwritten) := sum over plotted samples of
+ * grad_density (device f32, dL/d density) pulled back through the splat
+ * and the walk. Each walker's trajectory is recomputed by RNG replay in
+ * windows of K iterations from 16-byte checkpoints; nothing per iteration
+ * is stored between calls. Sums are reduced in f64 in a fixed order, so a
+ * call is deterministic. */
+int ffm_softsplat_adjoint(ffm_ctx* ctx, const float* theta, const float* grad_density,
+                          float* grad_theta, void* stream);
 
 #ifdef __cplusplus
 }

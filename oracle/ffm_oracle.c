@@ -1151,93 +1151,3 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
   free(f);
   return 0;
 }
-
-/* ------------------------------------------------------------ soft splat */
-
-/* FP64 forward of the differentiable mode: the same walk (selection,
- * opacity draw, transform, respawn, fuse, final chain) in double, with
- * each plotted in-frame sample spread bilinearly over the four pixels
- * around its continuous pixel position (x_pix - 0.5, y_pix - 0.5), weight 1
- * per sample, colour ignored. */
-static void apply_xform_d(const ffm_flame_desc* f, const ffm_xform* xf, double* px, double* py,
-                          orc_rng* rng) {
-  double x = *px, y = *py;
-  double ax = (double)xf->a * x + (double)xf->b * y + (double)xf->e;
-  double ay = (double)xf->c * x + (double)xf->d * y + (double)xf->f;
-  double rx = 0.0, ry = 0.0;
-  for (int k = 0; k < f->n_vars; k++) {
-    double w = xf->var_weight[k];
-    if (w == 0.0) continue;
-    double vx = 0.0, vy = 0.0, r2 = ax * ax + ay * ay;
-    switch (f->var_id[k]) {
-      case FFM_VAR_LINEAR: vx = ax; vy = ay; break;
-      case FFM_VAR_SINUSOIDAL: vx = sin(ax); vy = sin(ay); break;
-      case FFM_VAR_SPHERICAL: vx = ax / (r2 + 1e-6); vy = ay / (r2 + 1e-6); break;
-      case FFM_VAR_SWIRL: vx = ax * sin(r2) - ay * cos(r2); vy = ax * cos(r2) + ay * sin(r2); break;
-      case FFM_VAR_BUBBLE: vx = ax / (r2 / 4.0 + 1.0); vy = ay / (r2 / 4.0 + 1.0); break;
-      case FFM_VAR_HYPERBOLIC: vx = ax / (r2 + 1e-6); vy = ay; break;
-      default: {
-        float fx, fy;
-        orc_variation(f->var_id[k], xf->var_param[k], xf, (float)ax, (float)ay, rng, &fx, &fy);
-        vx = fx;
-        vy = fy;
-      }
-    }
-    rx += w * vx;
-    ry += w * vy;
-  }
-  if (xf->post_enabled) {
-    double qx = (double)xf->post_a * rx + (double)xf->post_b * ry + (double)xf->post_e;
-    double qy = (double)xf->post_c * rx + (double)xf->post_d * ry + (double)xf->post_f;
-    rx = qx;
-    ry = qy;
-  }
-  *px = rx;
-  *py = ry;
-}
-
-void orc_softsplat(const ffm_flame_desc* f, const ffm_view* v, uint32_t walkers, uint32_t ipt,
-                   uint32_t burn_in, uint32_t seed, double* density) {
-  int W = v->width, H = v->height;
-  double scale = (double)(W < H ? W : H) * 0.25;
-  double cr = cos((double)v->rotation), sr = sin((double)v->rotation);
-  for (uint32_t tid = 0; tid < walkers; tid++) {
-    orc_rng rng;
-    orc_rng_init(&rng, tid, seed);
-    double x = orc_rng_nextf(&rng) * 2.0 - 1.0;
-    double y = orc_rng_nextf(&rng) * 2.0 - 1.0;
-    uint32_t fuse = burn_in;
-    for (uint32_t it = 0; it < ipt; it++) {
-      int xi = orc_select(f, orc_rng_nextf(&rng));
-      const ffm_xform* xf = &f->xforms[xi];
-      int plot = orc_rng_nextf(&rng) < xf->opacity;
-      apply_xform_d(f, xf, &x, &y, &rng);
-      if (!(fabs(x) <= 1e32) || !(fabs(y) <= 1e32)) {
-        x = orc_rng_nextf(&rng) * 2.0 - 1.0;
-        y = orc_rng_nextf(&rng) * 2.0 - 1.0;
-        fuse = burn_in;
-        continue;
-      }
-      if (fuse > 0) {
-        fuse--;
-        continue;
-      }
-      double fx = x, fy = y;
-      for (int i = 0; i < f->n_finals; i++) apply_xform_d(f, &f->finals[i], &fx, &fy, &rng);
-      if (!plot) continue;
-      double tx = fx - v->pan_x, ty = fy - v->pan_y;
-      double qx = (tx * cr - ty * sr) * v->zoom, qy = (tx * sr + ty * cr) * v->zoom;
-      double sx = W * 0.5 + qx * scale - 0.5, sy = H * 0.5 + qy * scale - 0.5;
-      double x0 = floor(sx), y0 = floor(sy);
-      double ux = sx - x0, uy = sy - y0;
-      int ix = (int)x0, iy = (int)y0;
-      if (!(sx > -1.0 && sy > -1.0 && sx < W && sy < H)) continue;
-      double wts[4] = {(1 - ux) * (1 - uy), ux * (1 - uy), (1 - ux) * uy, ux * uy};
-      int dx[4] = {0, 1, 0, 1}, dy[4] = {0, 0, 1, 1};
-      for (int c = 0; c < 4; c++) {
-        int X = ix + dx[c], Y = iy + dy[c];
-        if (X >= 0 && X < W && Y >= 0 && Y < H) density[(size_t)Y * W + X] += wts[c];
-      }
-    }
-  }
-}

@@ -107,9 +107,18 @@ int orc_render(const ffm_flame_desc* f, const float* stops_prgb, int n_stops, in
                const ffm_job* job, float* accum_out, uint8_t* rgba_out, uint64_t* total_out,
                int threads);
 
-/* --- soft splat (no reference counterpart; defines the forward the adjoint differentiates) --- */
-void orc_softsplat(const ffm_flame_desc* f, const ffm_view* v, uint32_t walkers, uint32_t ipt,
-                   uint32_t burn_in, uint32_t seed, double* density);
+/* --- soft splat, FP64 (ffm_soft.c; no reference counterpart) --- */
+/* Parameters of a fittable flame, n * (6 + v), or -1. */
+int orc_soft_param_count(const ffm_flame_desc* f);
+/* density (W*H doubles) := the soft splat at theta (double, the ABI's layout). */
+int orc_soft_forward(const ffm_flame_desc* f, const double* theta, const ffm_soft_job* job,
+                     double* density, int threads);
+/* grad_theta := dL/dtheta for dL/d density = grad_density; exact when
+ * job->truncate == 0, else dL/dpoint restarts at zero at every window start. */
+int orc_soft_adjoint(const ffm_flame_desc* f, const double* theta, const ffm_soft_job* job,
+                     const double* grad_density, double* grad_theta, int threads);
+/* One variation's value V[2] and Jacobian J[4] = [dVx/dx, dVx/dy, dVy/dx, dVy/dy]. */
+int orc_soft_variation(int id, const float* params, double x, double y, double* V, double* J);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "fflame-rust_amd"))
 from pyfae import _abi  # noqa: E402
-from pyfae._abi import FlameDesc, Job, TonemapParams, View  # noqa: E402
+from pyfae._abi import FlameDesc, Job, SoftJob, TonemapParams, View  # noqa: E402
 
 LIB = os.path.join(HERE, "libffm_oracle.so")
 
@@ -39,6 +39,7 @@ def build():
 _lib = None
 _vp, _fp, _u32p, _u8p, _u64p = (C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint8), C.POINTER(C.c_uint64))
+_dp = C.POINTER(C.c_double)
 
 
 def lib():
@@ -92,9 +93,12 @@ def lib():
                                    C.POINTER(TonemapParams)]),
             "orc_render": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.c_int, C.POINTER(Job),
                                      _fp, _u8p, _u64p, C.c_int]),
-            "orc_softsplat": (None, [C.POINTER(FlameDesc), C.POINTER(View), C.c_uint32,
-                                     C.c_uint32, C.c_uint32, C.c_uint32,
-                                     C.POINTER(C.c_double)]),
+            "orc_soft_param_count": (C.c_int, [C.POINTER(FlameDesc)]),
+            "orc_soft_forward": (C.c_int, [C.POINTER(FlameDesc), _dp, C.POINTER(SoftJob), _dp,
+                                           C.c_int]),
+            "orc_soft_adjoint": (C.c_int, [C.POINTER(FlameDesc), _dp, C.POINTER(SoftJob), _dp,
+                                           _dp, C.c_int]),
+            "orc_soft_variation": (C.c_int, [C.c_int, _fp, C.c_double, C.c_double, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -236,3 +240,44 @@ def density_estimate(acc, max_r, min_r, curve):
     out = np.zeros_like(acc)
     lib().orc_density_estimate(fp(np.ascontiguousarray(acc)), fp(out), w, h, max_r, min_r, curve)
     return out
+
+
+# ---------------------------------------------------------------- soft splat
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def soft_forward(fd, theta, job, threads=8):
+    """FP64 soft-splat density (H, W) at theta (float64, the ABI's layout)."""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    out = np.zeros((job.view.height, job.view.width), np.float64)
+    st = lib().orc_soft_forward(C.byref(fd), _dptr(theta), C.byref(job), _dptr(out), threads)
+    if st:
+        raise ValueError("orc_soft_forward: %d" % st)
+    return out
+
+
+def soft_adjoint(fd, theta, job, grad_density, threads=8):
+    """FP64 dL/dtheta for dL/d density = grad_density."""
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    g = np.ascontiguousarray(grad_density, dtype=np.float64)
+    P = lib().orc_soft_param_count(C.byref(fd))
+    if P < 0:
+        raise ValueError("flame cannot be fitted")
+    out = np.zeros(P, np.float64)
+    st = lib().orc_soft_adjoint(C.byref(fd), _dptr(theta), C.byref(job), _dptr(g), _dptr(out),
+                                threads)
+    if st:
+        raise ValueError("orc_soft_adjoint: %d" % st)
+    return out
+
+
+def soft_variation(var_id, x, y, params=(0.0, 0.0, 0.0, 0.0)):
+    """(V[2], J[2x2]) of one variation in FP64."""
+    p = (C.c_float * 4)(*params)
+    V = np.zeros(2)
+    J = np.zeros(4)
+    lib().orc_soft_variation(var_id, p, x, y, _dptr(V), _dptr(J))
+    return V, J.reshape(2, 2)
+
