@@ -62,8 +62,11 @@ def recorded_traffic(kernel, splat_mode, walkers):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="single4k", choices=["single4k", "gallery", "plain"],
-                    help="single4k (the headline), gallery (batched vs one by one), plain")
+    ap.add_argument("--workload", default="single4k",
+                    choices=["single4k", "gallery", "plain", "fit", "fit_batch"],
+                    help="single4k (the headline), gallery (batched vs one by one), plain, "
+                         "fit (differentiable soft-splat fit step), fit_batch (64 fits)")
+    ap.add_argument("--no-fd", action="store_true", help="fit: skip the finite-difference check")
     ap.add_argument("--gallery-size", type=int, default=512)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -85,6 +88,8 @@ def main():
 
     if args.workload == "gallery":
         return bench_gallery(args)
+    if args.workload in ("fit", "fit_batch"):
+        return bench_fit(args)
     if args.workload == "plain":
         fd, pal, job = configs.plain()
         job.splat_mode = args.splat_mode
@@ -130,6 +135,109 @@ def bench_gallery(args):
         "speedup_vs_serial": dt_s / dt, "samples_per_s": samples / dt,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_fit(args):
+    """Differentiable fit: one step = soft-splat forward, the log-density L2
+    loss in torch, and the adjoint, through pyfae.diff's autograd Function.
+    Reports the step time, the forward and adjoint kernel times and, for the
+    single fit, the gradient against FP64 central finite differences of the
+    oracle's loss (outside the timed region)."""
+    import numpy as np
+    import torch
+    import pyfae
+    from pyfae import configs
+    from pyfae.diff import SoftSplat, log_density_l2
+    eng = pyfae.Engine(0)
+    if args.workload == "fit":
+        items = [configs.fit()]
+        workload = ("fit: 4 xforms {linear, spherical, sinusoidal}, 256x256 soft splat, 2^20 "
+                    "walkers x 64 iterations, exact adjoint in windows of K=8, f64 walk")
+    else:
+        items = configs.fit_batch()
+        workload = ("fit_batch: 64 independent 3-xform fits at 128x128, 2^16 walkers x 64 "
+                    "iterations each, exact adjoint in windows of K=8, f64 walk, one launch")
+    target = SoftSplat(eng, [(tfd, tjob) for _, tfd, _, tjob in items])
+    with torch.no_grad():
+        tau = target(target.theta0()).clone()
+    splat = SoftSplat(eng, [(fd, job) for fd, _, job, _ in items])
+    theta0 = splat.theta0()
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        def step():
+            theta = theta0.clone().requires_grad_()
+            loss = log_density_l2(splat(theta), tau)
+            loss.backward()
+            return theta.grad
+        for _ in range(max(args.warmup, 1)):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            grad = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        # forward and adjoint kernels alone, HIP events on the launch stream
+        dens = torch.empty(splat.density_floats, device="cuda")
+        g = torch.randn(splat.density_floats, device="cuda")
+        gt = torch.empty_like(theta0)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        n = 10
+        ev[0].record(stream)
+        for _ in range(n):
+            eng.softsplat_forward(theta0, dens, stream=stream.cuda_stream)
+        ev[1].record(stream)
+        for _ in range(n):
+            eng.softsplat_adjoint(theta0, g, gt, stream=stream.cuda_stream)
+        ev[2].record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = ev[0].elapsed_time(ev[1]) / n
+    adj_ms = ev[1].elapsed_time(ev[2]) / n
+    samples = sum(job.walkers * job.ipt for _, _, job, _ in items)
+    out = {
+        "metric": "adjoint fit step ms (%s)" % args.workload,
+        "value": dt * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "fits": len(items), "params": int(theta0.numel()),
+                   "samples_per_step": samples},
+        "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+        "step_over_forward": (fwd_ms + adj_ms) / fwd_ms,
+        "walk_samples_per_s": {"forward": samples / (fwd_ms / 1e3),
+                               "adjoint": samples / (adj_ms / 1e3)},
+    }
+    if args.workload == "fit" and not args.no_fd:
+        out["fd_check"] = fd_check(items[0], tau.cpu().numpy().astype(np.float64),
+                                   grad.cpu().numpy().astype(np.float64))
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+def fd_check(item, tau, grad):
+    """Relative 2-norm error of the GPU gradient against FP64 central
+    differences of the oracle's loss, at h = 1e-10 (converged) and at 1e-4
+    (where the pathwise loss is rough at that scale)."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    from pyfae import soft_theta
+    fd, _, job, _ = item
+    th = soft_theta(fd).astype(np.float32).astype(np.float64)
+    tau = tau.reshape(job.view.height, job.view.width)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+
+    def L(t):
+        d = np.log1p(O.soft_forward(fd, t, job, threads=threads)) - np.log1p(tau)
+        return float((d * d).sum())
+
+    out = {}
+    t0 = time.perf_counter()
+    for name, h in (("h1e-10", 1e-10), ("h1e-4", 1e-4)):
+        num = np.array([(L(th + h * e) - L(th - h * e)) / (2 * h) for e in np.eye(th.size)])
+        out["rel_err_" + name] = float(np.linalg.norm(grad - num) / np.linalg.norm(num))
+    out["seconds"] = time.perf_counter() - t0
+    out["threads"] = threads
+    return out
 
 
 def bench_flame(args, fd, pal, job, workload, metric):

@@ -15,6 +15,14 @@
 
 namespace ffm {
 
+// XCC id of the XCD this wave runs on (gfx940+ hardware register), 0..7:
+// kernels that keep one private copy of an image per XCD index it with this.
+__device__ __forceinline__ uint32_t xcc_id() {
+  uint32_t v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return v & 7u;
+}
+
 // ---------------------------------------------------------------- PCG32
 
 // pcg_hash (shaders/core/rng.wgsl:22-27).

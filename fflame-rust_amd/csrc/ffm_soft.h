@@ -24,12 +24,17 @@ struct SoftFit {
   uint32_t block0, n_blocks;
 };
 
+// The forward splats into one private density copy per XCD, folded after.
+constexpr int SOFT_XCD_COPIES = 8;
+
 struct SoftArgs {
   const SoftFit* fits;
   int32_t n_fits;
   const float* tables;
   const float* theta;
   float* density;             // forward output
+  float* dens_xcd;            // forward scratch: SOFT_XCD_COPIES x total_pixels
+  uint64_t total_pixels;      // density floats of all fits
   const float* grad_density;  // adjoint input
   unsigned char* ckpt;        // adjoint scratch: window-start states
   double* partials;           // adjoint scratch: per-block sums

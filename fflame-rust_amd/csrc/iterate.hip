@@ -18,13 +18,6 @@
 
 namespace ffm {
 
-// XCC id of the XCD this wave runs on (gfx940+ hardware register).
-__device__ __forceinline__ uint32_t xcc_id() {
-  uint32_t v;
-  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-  return v & 7u;  // 8 XCDs; the host allocates 8 private copies
-}
-
 // Wave-level sum then one atomic per wave.
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long v) {
   if (!dst) return;

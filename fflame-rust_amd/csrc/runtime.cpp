@@ -68,7 +68,7 @@ struct ffm_ctx {
   struct ffm_batch_state* batch = nullptr;
   // differentiable soft splat: the fits set last
   std::vector<SoftFit> s_fits;
-  DevBuf s_fits_d, s_tables, s_ckpt, s_part;
+  DevBuf s_fits_d, s_tables, s_ckpt, s_part, s_xcd;
   uint64_t s_theta = 0, s_pixels = 0, s_ckpt_bytes = 0, s_part_n = 0;
   uint32_t s_blocks = 0;
   size_t s_fwd_lds = 0, s_adj_lds32 = 0, s_adj_lds64 = 0;
@@ -516,7 +516,7 @@ int ffm_ctx_destroy(ffm_ctx* c) {
   for (DevBuf* b : {&c->table, &c->pal_u32, &c->pal_f32, &c->hist, &c->copies, &c->scratch,
                     &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
                     &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records,
-                    &c->s_fits_d, &c->s_tables, &c->s_ckpt, &c->s_part})
+                    &c->s_fits_d, &c->s_tables, &c->s_ckpt, &c->s_part, &c->s_xcd})
     b->release();
   free_batch(c->batch);
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
@@ -1313,6 +1313,7 @@ SoftArgs soft_args(ffm_ctx* c) {
   a.tables = reinterpret_cast<const float*>(c->s_tables.p);
   a.ckpt = reinterpret_cast<unsigned char*>(c->s_ckpt.p);
   a.partials = reinterpret_cast<double*>(c->s_part.p);
+  a.total_pixels = c->s_pixels;
   return a;
 }
 
@@ -1428,10 +1429,12 @@ int ffm_softsplat_forward(ffm_ctx* c, const float* theta, float* density, void* 
   if (!theta || !density) return fail(c, FFM_ERR_INVALID_ARGUMENT, "theta or density is NULL");
   (void)hipSetDevice(c->device);
   hipStream_t s = pick(c, stream);
+  int st = ensure(c, c->s_xcd, (size_t)SOFT_XCD_COPIES * c->s_pixels * 4, "XCD density copies");
+  if (st) return st;
   SoftArgs a = soft_args(c);
   a.theta = theta;
   a.density = density;
-  HIPCHECK(c, launch_clear_u32(density, c->s_pixels, s));
+  a.dens_xcd = reinterpret_cast<float*>(c->s_xcd.p);
   HIPCHECK(c, launch_soft_forward(a, c->s_blocks, c->s_fwd_lds, c->s_any32, c->s_any64, s));
   return FFM_OK;
 }

@@ -363,7 +363,8 @@ __global__ __launch_bounds__(SOFT_BLOCK) void soft_forward_kernel(SoftArgs a) {
   T x, y;
   uint32_t fuse;
   walker_init((uint32_t)tid, F.seed, F.burn_in, rng, x, y, fuse);
-  float* dens = a.density + F.pix_off;
+  // this XCD's private copy: its atomics resolve in the XCD's own L2
+  float* dens = a.dens_xcd + (size_t)xcc_id() * a.total_pixels + F.pix_off;
   const int W = F.view.width, H = F.view.height;
   for (uint32_t it = 0; it < F.ipt; it++) {
     int xi;
@@ -379,8 +380,22 @@ __global__ __launch_bounds__(SOFT_BLOCK) void soft_forward_kernel(SoftArgs a) {
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       int X = ix + (c & 1), Y = iy + (c >> 1);
-      if (X >= 0 && X < W && Y >= 0 && Y < H) atomicAdd(dens + (size_t)Y * W + X, w[c]);
+      if (X >= 0 && X < W && Y >= 0 && Y < H)
+        __hip_atomic_fetch_add(dens + (size_t)Y * W + X, w[c], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+  }
+}
+
+// density = the sum of the XCD copies, in copy order.
+__global__ __launch_bounds__(256) void soft_fold_kernel(SoftArgs a) {
+  const uint64_t n = a.total_pixels;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < SOFT_XCD_COPIES; c++) s += a.dens_xcd[(size_t)c * n + i];
+    a.density[i] = s;
   }
 }
 
@@ -634,10 +649,14 @@ int soft_block() { return SOFT_BLOCK; }
 
 hipError_t launch_soft_forward(const SoftArgs& a, uint32_t blocks, size_t lds, bool any_f32,
                                bool any_f64, hipStream_t s) {
+  hipError_t e = launch_clear_u32(a.dens_xcd, (size_t)SOFT_XCD_COPIES * a.total_pixels, s);
+  if (e != hipSuccess) return e;
   if (any_f32)
     hipLaunchKernelGGL(soft_forward_kernel<float>, dim3(blocks), dim3(SOFT_BLOCK), lds, s, a);
   if (any_f64)
     hipLaunchKernelGGL(soft_forward_kernel<double>, dim3(blocks), dim3(SOFT_BLOCK), lds, s, a);
+  const uint64_t fold_blocks = std::min<uint64_t>((a.total_pixels + 255) / 256, 4096);
+  hipLaunchKernelGGL(soft_fold_kernel, dim3((uint32_t)fold_blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
