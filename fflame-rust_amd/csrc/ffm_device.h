@@ -83,7 +83,10 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
 
 // One variation body V(x, y). `row` is the transform's table row (waves
 // reads its own b, e, d, f), `p` the variation's params. Bodies:
-// src/variations/defs/basic.rs:37-205, advanced.rs:32-1700.
+// src/variations/defs/basic.rs:37-205, advanced.rs:32-1700. Quotients use
+// the refined hardware reciprocal (ffm_fdiv, ffm_rcp) and square roots the
+// 1-ulp hardware form (ffm_sqrt); the reference's WGSL allows 2.5 ulp for
+// both, and the per-step bar against the IEEE oracle is 4 ulp.
 __device__ __forceinline__ void eval_variation(int id, const float* __restrict__ row,
                                                const float* __restrict__ p, float x, float y,
                                                Rng& rng, float& ox, float& oy) {
@@ -100,8 +103,8 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     case FFM_VAR_SPHERICAL: {
       FFM_NO_SPECULATE();
       float r2 = x * x + y * y + 1e-6f;
-      ox = x / r2;
-      oy = y / r2;
+      ox = ffm_fdiv(x, r2);
+      oy = ffm_fdiv(y, r2);
       break;
     }
     case FFM_VAR_SWIRL: {
@@ -115,8 +118,8 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     }
     case FFM_VAR_HORSESHOE: {
       FFM_NO_SPECULATE();
-      float r = sqrtf(x * x + y * y) + 1e-6f;
-      float r_inv = 1.0f / r;
+      float r = ffm_sqrt(x * x + y * y) + 1e-6f;
+      float r_inv = ffm_rcp(r);
       ox = (x - y) * (x + y) * r_inv;
       oy = 2.0f * x * y * r_inv;
       break;
@@ -124,16 +127,16 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     case FFM_VAR_POLAR: {
       FFM_NO_SPECULATE();
       float theta = ffm_atan2f(x, y);
-      float r = sqrtf(x * x + y * y);
-      ox = theta / PI_F;
+      float r = ffm_sqrt(x * x + y * y);
+      ox = ffm_fdiv(theta, PI_F);
       oy = r - 1.0f;
       break;
     }
     case FFM_VAR_DISC: {
       FFM_NO_SPECULATE();
       float theta = ffm_atan2f(x, y);
-      float r = sqrtf(x * x + y * y);
-      float theta_pi = theta / PI_F;
+      float r = ffm_sqrt(x * x + y * y);
+      float theta_pi = ffm_fdiv(theta, PI_F);
       float pi_r = PI_F * r;
       float s, c;
       ffm_sincosf(pi_r, &s, &c);
@@ -143,16 +146,15 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     }
     case FFM_VAR_HYPERBOLIC: {
       FFM_NO_SPECULATE();
-      float r2 = x * x + y * y + 1e-6f;
-      ox = x / r2;
+      ox = ffm_fdiv(x, x * x + y * y + 1e-6f);
       oy = y;
       break;
     }
     case FFM_VAR_JULIA: {
       FFM_NO_SPECULATE();
-      float r = sqrtf(x * x + y * y);
+      float r = ffm_sqrt(x * x + y * y);
       float theta = ffm_atan2f(y, x);
-      float sqrt_r = sqrtf(r);
+      float sqrt_r = ffm_sqrt(r);
       float omega = (rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
       float half_theta = theta / 2.0f + omega;
       float s, c;
@@ -164,8 +166,8 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     case FFM_VAR_WAVES: {
       FFM_NO_SPECULATE();
       float b = row[XF_B], e = row[XF_E], d = row[XF_D], f = row[XF_F];
-      ox = x + b * ffm_sinf(y / (e * e + 1e-6f));
-      oy = y + d * ffm_sinf(x / (f * f + 1e-6f));
+      ox = x + b * ffm_sinf(ffm_fdiv(y, e * e + 1e-6f));
+      oy = y + d * ffm_sinf(ffm_fdiv(x, f * f + 1e-6f));
       break;
     }
     case FFM_VAR_JULIAN: {
@@ -176,7 +178,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       float r = powf(r2, cpower);
       float theta = ff_atan2(y, x);
       float trunc_val = floorf(fabsf(power) * rng_nextf(rng));
-      float t = (theta + TWO_PI_F * trunc_val) / power;
+      float t = ffm_fdiv(theta + TWO_PI_F * trunc_val, power);
       float s, c;
       ffm_sincosf(t, &s, &c);
       ox = r * c;
@@ -187,7 +189,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       FFM_NO_SPECULATE();
       float r2 = x * x + y * y;
       float r = r2 / 4.0f + 1.0f;
-      float scale = 1.0f / r;
+      float scale = ffm_rcp(r);
       ox = scale * x;
       oy = scale * y;
       break;
@@ -197,7 +199,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       float c1 = p[0], c2 = p[1];
       float re = 1.0f + c1 * x + c2 * (x * x - y * y);
       float im = c1 * y + 2.0f * c2 * x * y;
-      float r = 1.0f / (re * re + im * im);
+      float r = ffm_rcp(re * re + im * im);
       ox = (x * re + y * im) * r;
       oy = (y * re - x * im) * r;
       break;

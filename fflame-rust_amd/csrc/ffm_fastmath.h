@@ -26,6 +26,41 @@
 #define FFM_FM_NOINLINE static
 #endif
 
+// Reciprocal and quotient from the hardware's 1-ulp v_rcp_f32 refined by
+// Newton-Raphson with fused multiply-adds: r' = r + r (1 - b r), then the
+// quotient's residual correction q' = q + r' (a - b q). Four FMAs after the
+// rcp instead of the division expansion's scaling/fixup sequence; for
+// operands away from the exponent extremes (|b| within 2^+-126, where the
+// walk lives) the quotient is within 0.5 ulp plus the residual's rounding.
+// The square root is the hardware's 1-ulp v_sqrt_f32 (WGSL's own sqrt bound
+// is 2.5 ulp). The host build (accuracy checks) uses the IEEE operations.
+FFM_FM_INLINE float ffm_rcp(float b) {
+#ifdef __HIPCC__
+  const float r = __builtin_amdgcn_rcpf(b);
+  return fmaf(r, fmaf(-b, r, 1.0f), r);
+#else
+  return 1.0f / b;
+#endif
+}
+
+FFM_FM_INLINE float ffm_fdiv(float a, float b) {
+#ifdef __HIPCC__
+  const float r = ffm_rcp(b);
+  const float q = a * r;
+  return fmaf(r, fmaf(-b, q, a), q);
+#else
+  return a / b;
+#endif
+}
+
+FFM_FM_INLINE float ffm_sqrt(float x) {
+#ifdef __HIPCC__
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 FFM_FM_NOINLINE void ffm_sincos_wide(float x, float* s, float* c) {
 #ifdef __HIPCC__
   sincosf(x, s, c);
@@ -62,7 +97,8 @@ FFM_FM_INLINE void ffm_sincosf(float x, float* s, float* c) {
   }
 }
 
-// atan2 by reduction to a = min(|x|,|y|) / max(|x|,|y|) in [0, 1], then
+// atan2 by reduction to a = min(|x|,|y|) / max(|x|,|y|) in [0, 1] (1.5-ulp
+// quotients), then
 // a -> (a - 1) / (a + 1) above tan(pi/8) (adding pi/4), and Cephes atanf's
 // minimax polynomial on [-tan(pi/8), tan(pi/8)]:
 //   atan z = z + z^3 (-3.33329491539e-1 + z^2 (1.99777106478e-1
@@ -76,9 +112,9 @@ FFM_FM_INLINE float ffm_atan2f(float y, float x) {
   const float PI_HI = 3.1415927410125732421875f, PI_LO = -8.74227766e-8f;
   const float ax = fabsf(x), ay = fabsf(y);
   const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-  float a = mx > 0.0f ? mn / mx : 0.0f;
+  float a = mx > 0.0f ? ffm_fdiv(mn, mx) : 0.0f;
   const int reduced = a > 0.41421356237309503f;
-  if (reduced) a = (a - 1.0f) / (a + 1.0f);
+  if (reduced) a = ffm_fdiv(a - 1.0f, a + 1.0f);
   const float z = a * a;
   const float p = fmaf(fmaf(fmaf(8.05374449538e-2f, z, -1.38776856032e-1f), z, 1.99777106478e-1f),
                        z, -3.33329491539e-1f);
