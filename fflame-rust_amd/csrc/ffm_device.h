@@ -166,6 +166,8 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     case FFM_VAR_WAVES: {
       FFM_NO_SPECULATE();
       float b = row[XF_B], e = row[XF_E], d = row[XF_D], f = row[XF_F];
+      // the row is fixed over the variation loop too: keep e, f opaque here
+      asm volatile("" : "+v"(e), "+v"(f));
       ox = x + b * ffm_sinf(ffm_fdiv(y, e * e + 1e-6f));
       oy = y + d * ffm_sinf(ffm_fdiv(x, f * f + 1e-6f));
       break;
@@ -240,13 +242,14 @@ __device__ __forceinline__ void apply_xform(const float* __restrict__ row, const
   y = ry;
 }
 
-// Saturating f32 -> i32 as the reference's i32(floor(.)) behaves on GPUs
-// (NaN -> 0).
+// Saturating f32 -> i32 as the reference's i32(floor(.)) behaves on GPUs:
+// out-of-range values clamp to INT_MIN / INT_MAX and NaN gives 0, which is
+// exactly v_cvt_i32_f32's definition (one instruction; a C cast would be
+// undefined out of range).
 __device__ __forceinline__ int32_t sat_i32(float f) {
-  if (f != f) return 0;
-  if (f >= 2147483520.0f) return 2147483647;
-  if (f <= -2147483648.0f) return (int32_t)0x80000000u;
-  return (int32_t)f;
+  int32_t r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+  return r;
 }
 
 // world_to_pixel (shaders/core/utilities.wgsl:417-445).
@@ -262,12 +265,11 @@ __device__ __forceinline__ void world_to_pixel(const ViewConst& v, float x, floa
 }
 
 // Nearest texel of the clamp-to-edge palette at u = colour index.
+// u < 0 and NaN give texel 0 (fmaxf returns the non-NaN operand), u at or
+// past size - 1 gives size - 1; in between, floor(u).
 __device__ __forceinline__ int palette_texel(float ci, int size) {
-  float u = ci * (float)size;
-  if (!(u >= 0.0f)) return 0;
-  if (u >= (float)size) return size - 1;
-  int i = (int)floorf(u);
-  return i > size - 1 ? size - 1 : i;
+  float u = fminf(fmaxf(ci * (float)size, 0.0f), (float)(size - 1));
+  return (int)floorf(u);
 }
 
 enum StepFlags { ST_PLOT = 1, ST_RESPAWN = 2, ST_BURN = 4 };

@@ -177,29 +177,32 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
       ci = o.z;
       fuse = __float_as_uint(o.w);
     }
+    uint32_t r_plotted = 0u, r_in = 0u;  // this round's counts (at most ipt each)
     for (uint32_t it = 0; it < a.ipt; it++) {
       if (active) {
         int xi, texel = 0;
         int32_t px, py;
         int st = chaos_step(F.fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, xi, px, py, texel);
-        n_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
+        r_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
         if (st & ST_PLOT) {
-          n_in++;
-          const int tile = (py / TILE_H) * t.tiles_x + px / TILE_W;
+          r_in++;
+          // in-frame: px, py >= 0, so tile arithmetic is unsigned shifts and masks
+          const uint32_t ux = (uint32_t)px, uy = (uint32_t)py;
+          const uint32_t tile = __umul24(uy / TILE_H, (uint32_t)t.tiles_x) + ux / TILE_W;
           if (COUNT_ONLY) {
             atomicAdd(&L.cnt[tile], 1u);
           } else {
-            const uint32_t local = (uint32_t)((py % TILE_H) * TILE_W + px % TILE_W);
-            // wave-compacted append: one LDS atomic per wave
+            const uint32_t local = (uy % TILE_H) * TILE_W + ux % TILE_W;
+            // wave-compacted append: one LDS atomic per wave, by the first
+            // active lane (rank 0), whose result readfirstlane broadcasts
             const uint64_t m = __ballot(1);
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
             uint32_t b = 0;
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            if ((int)(tid & 63) == leader) b = atomicAdd(L.fill, (uint32_t)__popcll(m));
-            b = __shfl(b, leader, 64);
+            if (rank == 0u) b = atomicAdd(L.fill, (uint32_t)__popcll(m));
+            b = __builtin_amdgcn_readfirstlane(b);
             L.rec[b + rank] = local | ((uint32_t)texel << 13);
-            L.tr[b + rank] = (uint32_t)tile;
+            L.tr[b + rank] = tile;
           }
         }
       }
@@ -210,6 +213,8 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
         if (f >= flush_at) flush_stage(t, L, F.pal_u32);
       }
     }
+    n_plotted += r_plotted;
+    n_in += r_in;
     if (!COUNT_ONLY && active && a.orbits)
       a.orbits[wi] = make_float4(x, y, ci, __uint_as_float(fuse));
   }
