@@ -52,6 +52,10 @@ __device__ __forceinline__ uint32_t rng_next(Rng& r) {
   return (xs >> 22u) ^ xs;
 }
 
+// The LCG step alone: a draw whose value is not needed (its output
+// permutation skipped) still advances the stream.
+__device__ __forceinline__ void rng_skip(Rng& r) { r.state = r.state * 747796405u + r.inc; }
+
 // Top 24 bits times 2^-24 (rng.wgsl:53-55): exact in f32.
 __device__ __forceinline__ float rng_nextf(Rng& r) {
   return (float)(rng_next(r) >> 8u) * (1.0f / 16777216.0f);
@@ -299,8 +303,12 @@ __device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& 
     if (target <= fl.rows[i * XF_STRIDE + XF_CUM]) xi = i;
   xi_out = xi;
   const float* row = fl.rows + xi * XF_STRIDE;
-  // Opacity draw: always consumed (main_template.wgsl:249).
-  bool should_plot = rng_nextf(rng) < row[XF_OPACITY];
+  // Opacity draw: always consumed (main_template.wgsl:249). A draw is at
+  // most 1 - 2^-24, so at opacity >= 1 it always passes and only advances.
+  const float opacity = row[XF_OPACITY];
+  bool should_plot = true;
+  if (opacity >= 1.0f) rng_skip(rng);
+  else should_plot = rng_nextf(rng) < opacity;
   float nx = x, ny = y;
   apply_xform(row, fl.ids, fl.n_vars, nx, ny, rng);
   // Bad-value respawn + re-fuse (main_template.wgsl:397-479).
