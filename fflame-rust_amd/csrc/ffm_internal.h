@@ -74,17 +74,29 @@ constexpr int LDS_DYNAMIC_MAX = 152 * 1024;
 
 // Screen tiles of the LDS splat: 128x64 pixels = 128 KB of u32x4 in LDS.
 constexpr int TILE_W = 128, TILE_H = 64, TILE_PX = TILE_W * TILE_H;
-constexpr int TILE_MAX = 2048;  // tiles per image the LDS counting sort holds
+constexpr int TILE_MAX = 2048;  // bins per image the LDS counting sort holds
+
+// Records are sorted into bins: a bin is 2^group_shift consecutive tiles of
+// one tile row (one tile per bin while the image has at most TILE_MAX
+// tiles; larger images group tiles so the bins stay within TILE_MAX).
+// A record is tile-local pixel | texel << 13 | tile-in-bin << 25 (texels
+// below 4096, at most 128 tiles per bin).
+constexpr int REC_TEXEL_SHIFT = 13, REC_SUB_SHIFT = 25;
+constexpr uint32_t REC_LOCAL_MASK = 0x1FFFu, REC_TEXEL_MASK = 0xFFFu;
 
 struct TileArgs {
-  uint32_t* tile_fill;   // n_tiles: records requested per tile
-  uint32_t* region;      // n_tiles: first slot of the tile's region in `records`
+  uint32_t* tile_fill;   // n_tiles: records requested per bin
+  uint32_t* region;      // n_tiles: first slot of the bin's region in `records`
   uint32_t* cap;         // n_tiles: slots in the region
   uint32_t* item_start;  // n_tiles + 1: work list of tile_accum
-  uint32_t* records;     // tile-local pixel | texel << 13
+  uint32_t* records;     // local | texel << 13 | tile-in-bin << 25
   uint32_t* hist;        // the u32x4 (or f32x4) histogram
   const float* pal_f32;  // f32 splat: linear colour x 100 per texel, x3
-  int32_t tiles_x, n_tiles, width, height;
+  int32_t tiles_x;       // tiles per tile row
+  int32_t n_tiles;       // bins (the length of the per-bin arrays)
+  int32_t bins_x;        // bins per tile row: ceil(tiles_x / 2^group_shift)
+  int32_t group_shift;   // log2 of the tiles per bin
+  int32_t width, height;
   int32_t f32;           // 1: f32 histogram (float4 accumulation)
   int32_t flush_every;   // iterations between staging-fill checks (1..8)
 };

@@ -329,10 +329,14 @@ void collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, double& 
 int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = false) {
   const int W = a.view.width, H = a.view.height;
   const int tiles_x = (W + TILE_W - 1) / TILE_W, tiles_y = (H + TILE_H - 1) / TILE_H;
-  const int n_tiles = tiles_x * tiles_y;
+  // bins of 2^g tiles along a row, as few tiles per bin as keep bins <= TILE_MAX
+  int g = 0;
+  while (g < 7 && (int64_t)tiles_y * ((tiles_x + (1 << g) - 1) >> g) > TILE_MAX) g++;
+  const int bins_x = (tiles_x + (1 << g) - 1) >> g;
+  const int n_tiles = bins_x * tiles_y;  // bins
   if (n_tiles > TILE_MAX)
-    return fail(c, FFM_ERR_TOO_LARGE, "the LDS tile splat holds %d tiles of %dx%d; %dx%d needs %d",
-                TILE_MAX, TILE_W, TILE_H, W, H, n_tiles);
+    return fail(c, FFM_ERR_TOO_LARGE, "the LDS tile splat holds %d bins of up to 128 %dx%d tiles; "
+                "%dx%d needs more", TILE_MAX, TILE_W, TILE_H, W, H);
   if (bin_lds_bytes(a.n_rows, a.pal_size) > LDS_PER_CU ||
       accum_lds_bytes(a.pal_size, f32) > LDS_PER_CU)
     return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
@@ -358,6 +362,8 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   t.f32 = f32 ? 1 : 0;
   t.tiles_x = tiles_x;
   t.n_tiles = n_tiles;
+  t.bins_x = bins_x;
+  t.group_shift = g;
   t.width = W;
   t.height = H;
   t.flush_every = 4;
@@ -499,7 +505,7 @@ int ffm_ctx_create(int device, ffm_ctx** out) {
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
     return FFM_ERR_DEVICE;

@@ -8,16 +8,17 @@
 //
 //  1. bin_iterate_kernel: the same walkers as iterate_kernel, but each
 //     in-frame sample becomes a 4-byte record (pixel index inside its
-//     128x64 screen tile | palette texel << 13) staged in LDS. Every few
-//     iterations the workgroup counting-sorts the staged records by tile
-//     (LDS atomics + a block scan), reserves a run in each tile's region of
-//     the record buffer with ONE global atomic per (flush, tile), and
-//     writes the runs tile-contiguously.
-//  2. tile_accum_kernel: one workgroup per (tile, slice of its records)
-//     accumulates the slice into a 128x64 u32x4 tile held in LDS (LDS
-//     atomics) and folds the tile into the histogram: a plain
-//     read-add-write when it owns the whole tile, integer atomics when the
-//     tile's records are split over several workgroups.
+//     128x64 screen tile | palette texel << 13 | tile-in-bin << 25) staged
+//     in LDS. Every few iterations the workgroup counting-sorts the staged
+//     records by bin (LDS atomics + a block scan), reserves a run in each
+//     bin's region of the record buffer with ONE global atomic per
+//     (flush, bin), and writes the runs bin-contiguously. A bin is one tile
+//     up to TILE_MAX tiles per image, else a strip of 2^k tiles of a row.
+//  2. tile_accum_kernel: one workgroup per (bin, slice of its records)
+//     accumulates the slice, tile by tile, into a 128x64 u32x4 tile held in
+//     LDS (LDS atomics) and folds the tile into the histogram: a plain
+//     read-add-write when it owns the whole bin, integer atomics when the
+//     bin's records are split over several workgroups.
 //
 // Integer sums commute, so the histogram is bit-identical to the atomic
 // splat's. Records that do not fit a tile's region (its capacity comes
@@ -81,15 +82,24 @@ struct BinLds {
   uint32_t* fill;   // 1
 };
 
+// Tile column of a bin's sub-tile.
+__device__ __forceinline__ int bin_tile_x(const TileArgs& t, int bin, int sub) {
+  return ((bin % t.bins_x) << t.group_shift) + sub;
+}
+
+__device__ __forceinline__ uint32_t rec_texel(uint32_t rec) {
+  return (rec >> REC_TEXEL_SHIFT) & REC_TEXEL_MASK;
+}
+
 // Splat one record straight into the histogram (region overflow).
-__device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t* pal, int tile,
+__device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t* pal, int bin,
                                                uint32_t rec) {
-  const int tx = tile % t.tiles_x, ty = tile / t.tiles_x;
-  const int local = (int)(rec & 0x1FFFu);
+  const int tx = bin_tile_x(t, bin, (int)(rec >> REC_SUB_SHIFT)), ty = bin / t.bins_x;
+  const int local = (int)(rec & REC_LOCAL_MASK);
   const int px = tx * TILE_W + (local % TILE_W), py = ty * TILE_H + (local / TILE_W);
   const size_t base = 4 * ((size_t)py * t.width + px);
   if (t.f32) {
-    const float* c = t.pal_f32 + 3 * (rec >> 13);
+    const float* c = t.pal_f32 + 3 * rec_texel(rec);
     float* hf = reinterpret_cast<float*>(t.hist) + base;
     __hip_atomic_fetch_add(hf + 0, c[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_fetch_add(hf + 1, c[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -97,7 +107,7 @@ __device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t
     __hip_atomic_fetch_add(hf + 3, 100.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
-  const uint32_t pk = pal[rec >> 13];
+  const uint32_t pk = pal[rec_texel(rec)];
   uint32_t* h = t.hist + base;
   __hip_atomic_fetch_add(h + 0, pk & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_fetch_add(h + 1, (pk >> 8) & 0xFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -187,12 +197,13 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
         if (st & ST_PLOT) {
           r_in++;
           // in-frame: px, py >= 0, so tile arithmetic is unsigned shifts and masks
-          const uint32_t ux = (uint32_t)px, uy = (uint32_t)py;
-          const uint32_t tile = __umul24(uy / TILE_H, (uint32_t)t.tiles_x) + ux / TILE_W;
+          const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, tx = ux / TILE_W;
+          const uint32_t tile = __umul24(uy / TILE_H, (uint32_t)t.bins_x) + (tx >> t.group_shift);
           if (COUNT_ONLY) {
             atomicAdd(&L.cnt[tile], 1u);
           } else {
-            const uint32_t local = (uy % TILE_H) * TILE_W + ux % TILE_W;
+            const uint32_t local = (uy % TILE_H) * TILE_W + ux % TILE_W |
+                                   (tx & ((1u << t.group_shift) - 1u)) << REC_SUB_SHIFT;
             // wave-compacted append: one LDS atomic per wave, by the first
             // active lane (rank 0), whose result readfirstlane broadcasts
             const uint64_t m = __ballot(1);
@@ -201,7 +212,7 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
             uint32_t b = 0;
             if (rank == 0u) b = atomicAdd(L.fill, (uint32_t)__popcll(m));
             b = __builtin_amdgcn_readfirstlane(b);
-            L.rec[b + rank] = local | ((uint32_t)texel << 13);
+            L.rec[b + rank] = local | ((uint32_t)texel << REC_TEXEL_SHIFT);
             L.tr[b + rank] = tile;
           }
         }
@@ -342,79 +353,88 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
   if (threadIdx.x == 0) t.item_start[t.n_tiles] = all;
 }
 
-// Accumulate records [r0, r1) of tile `ti` (of the flame `t` describes)
-// into the LDS tile and fold it into the histogram: a read-add-write when
-// the slice is the tile's only one, integer / float atomics otherwise.
+// Accumulate records [r0, r1) of bin `ti` (of the flame `t` describes):
+// for each tile of the bin, the records of that tile go into the LDS tile,
+// which is then folded into the histogram: a read-add-write when the slice
+// is the bin's only one, integer / float atomics otherwise. A bin of several
+// tiles reads its slice once per tile (from L2 after the first pass).
 // All threads; F32: float4 accumulation.
 template <bool F32>
 __device__ void accum_slice(const TileArgs& t, int ti, uint32_t r0, uint32_t r1, uint32_t n_slices,
                             uint32_t* tile, const uint32_t* pal, const float* palf) {
   const int tid = threadIdx.x;
   float* tilef = reinterpret_cast<float*>(tile);
-  for (int i = tid; i < 4 * TILE_PX; i += blockDim.x) tile[i] = 0u;
-  __syncthreads();
   const uint32_t* rec = t.records + t.region[ti];
-  for (uint32_t r = r0 + tid; r < r1; r += blockDim.x) {
-    const uint32_t v = rec[r];
-    const uint32_t local = v & 0x1FFFu;
-    if (F32) {
-      const float* c = palf + 3 * (v >> 13);
-      atomicAdd(&tilef[4 * local + 0], c[0]);
-      atomicAdd(&tilef[4 * local + 1], c[1]);
-      atomicAdd(&tilef[4 * local + 2], c[2]);
-      atomicAdd(&tilef[4 * local + 3], 100.0f);
-    } else {
-      const uint32_t pk = pal[v >> 13];
-      atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
-      atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
-      atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
-      atomicAdd(&tile[4 * local + 3], 100u);
-    }
-  }
-  __syncthreads();
-  const int tx = ti % t.tiles_x, ty = ti / t.tiles_x;
-  for (int q = tid; q < TILE_PX; q += blockDim.x) {
-    const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
-    if (px >= t.width || py >= t.height) continue;
-    if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
-    const size_t p = (size_t)py * t.width + px;
-    if (F32) {
-      const float4 v = make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
-      float4* h = reinterpret_cast<float4*>(t.hist) + p;
-      if (n_slices == 1) {
-        float4 o = *h;
-        o.x += v.x;
-        o.y += v.y;
-        o.z += v.z;
-        o.w += v.w;
-        *h = o;
+  const int ty = ti / t.bins_x;
+  const int group = 1 << t.group_shift;
+  for (int sub = 0; sub < group; sub++) {
+    const int tx = bin_tile_x(t, ti, sub);
+    if (tx >= t.tiles_x) break;
+    for (int i = tid; i < 4 * TILE_PX; i += blockDim.x) tile[i] = 0u;
+    __syncthreads();
+    for (uint32_t r = r0 + tid; r < r1; r += blockDim.x) {
+      const uint32_t v = rec[r];
+      if ((int)(v >> REC_SUB_SHIFT) != sub) continue;
+      const uint32_t local = v & REC_LOCAL_MASK;
+      if (F32) {
+        const float* c = palf + 3 * rec_texel(v);
+        atomicAdd(&tilef[4 * local + 0], c[0]);
+        atomicAdd(&tilef[4 * local + 1], c[1]);
+        atomicAdd(&tilef[4 * local + 2], c[2]);
+        atomicAdd(&tilef[4 * local + 3], 100.0f);
       } else {
-        float* hw = reinterpret_cast<float*>(h);
-        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    } else {
-      const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
-      uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
-      if (n_slices == 1) {
-        uint4 o = *h;
-        o.x += v.x;
-        o.y += v.y;
-        o.z += v.z;
-        o.w += v.w;
-        *h = o;
-      } else {
-        uint32_t* hw = reinterpret_cast<uint32_t*>(h);
-        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t pk = pal[rec_texel(v)];
+        atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
+        atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
+        atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
+        atomicAdd(&tile[4 * local + 3], 100u);
       }
     }
+    __syncthreads();
+    for (int q = tid; q < TILE_PX; q += blockDim.x) {
+      const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
+      if (px >= t.width || py >= t.height) continue;
+      if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
+      const size_t p = (size_t)py * t.width + px;
+      if (F32) {
+        const float4 v =
+            make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
+        float4* h = reinterpret_cast<float4*>(t.hist) + p;
+        if (n_slices == 1) {
+          float4 o = *h;
+          o.x += v.x;
+          o.y += v.y;
+          o.z += v.z;
+          o.w += v.w;
+          *h = o;
+        } else {
+          float* hw = reinterpret_cast<float*>(h);
+          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      } else {
+        const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
+        uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
+        if (n_slices == 1) {
+          uint4 o = *h;
+          o.x += v.x;
+          o.y += v.y;
+          o.z += v.z;
+          o.w += v.w;
+          *h = o;
+        } else {
+          uint32_t* hw = reinterpret_cast<uint32_t*>(h);
+          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 // The largest i < n with start[i] <= x (start non-decreasing). Uniform.
@@ -461,6 +481,8 @@ __device__ __forceinline__ TileArgs flame_tiles(const BatchArgs& b, const BatchF
   t.hist = b.g.hist + 4 * f.hist_off;
   t.tiles_x = f.tiles_x;
   t.n_tiles = f.n_tiles;
+  t.bins_x = f.tiles_x;  // batched flames have at most TILE_MAX tiles: one tile per bin
+  t.group_shift = 0;
   t.width = f.view.width;
   t.height = f.view.height;
   return t;

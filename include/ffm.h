@@ -14,7 +14,10 @@
  *  - Descriptors are caller-owned and copied on set.
  *  - Device buffers passed to the granular calls are plain device pointers
  *    (hipMalloc'd or a torch tensor's data_ptr()); `stream` is a hipStream_t
- *    passed as void*, NULL meaning the context's own stream.
+ *    passed as void*, NULL meaning the context's own stream. That stream is
+ *    a blocking stream: its work is ordered after and before work on the
+ *    legacy default stream (torch's default stream), so buffers filled there
+ *    are ready when a NULL-stream call runs.
  *  - One context per host thread; calls on one context are serialised.
  *  - Plain C: no torch or HIP types appear in any signature.
  */

@@ -108,6 +108,40 @@ def test_lds_tile_splat_equals_oracle_and_global(engine):
         assert np.array_equal(g0, g2)
 
 
+def gpu_hist_dev(engine, fd, pal, view, walkers, ipt, burn_in, seed, mode):
+    """As gpu_hist, but the histogram stays on the device."""
+    set_flame(engine, fd, pal)
+    dt = torch.float32 if mode in (3, 4) else torch.int32
+    h = torch.zeros((view.height, view.width, 4), dtype=dt, device=DEV)
+    cnt = torch.zeros(2, dtype=torch.int64, device=DEV)
+    engine.iterate(view, walkers, ipt, burn_in, seed, h, splat_mode=mode, plotted=cnt[0:1].data_ptr(),
+                   inframe=cnt[1:2].data_ptr())
+    torch.cuda.synchronize()
+    return h, tuple(int(v) for v in cnt.cpu())
+
+
+@pytest.mark.parametrize("side", [(6144, 3072), (12288, 12288)])
+def test_lds_splat_with_tile_bins(engine, side):
+    """Images past 2048 tiles sort records into bins of 2^k tiles (6144x3072:
+    2 tiles per bin; 12288^2, the stress size: 16). The integer histogram
+    still equals the atomic splat's bit for bit; the float one to rounding."""
+    w, h = side
+    say(f"lds splat with tile bins {w}x{h}")
+    fd, pal, job = configs.stress(walkers=1 << 18, ipt=32, side=w)
+    job.view.height = h
+    g0, c0 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 18, 32, 20, 5, mode=0)
+    g2, c2 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 18, 32, 20, 5, mode=2)
+    assert c0 == c2
+    assert c0[1] > 1 << 20  # most samples land in frame
+    assert torch.equal(g0, g2)
+    del g0, g2
+    g3, c3 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 18, 32, 20, 5, mode=3)
+    g4, c4 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 18, 32, 20, 5, mode=4)
+    assert c3 == c4
+    assert torch.equal(g3[..., 3], g4[..., 3])  # sums of 100.0: exact
+    assert torch.allclose(g4[..., :3], g3[..., :3], rtol=1e-4, atol=1e-3)
+
+
 def test_f32_lds_tile_splat_matches_f32_atomics(engine):
     say("f32 lds tile splat")
     fd, pal, job = configs.single4k(walkers=1 << 16, ipt=64, width=960, height=540)
