@@ -63,9 +63,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="single4k",
-                    choices=["single4k", "gallery", "plain", "fit", "fit_batch"],
+                    choices=["single4k", "gallery", "plain", "fit", "fit_batch", "stress"],
                     help="single4k (the headline), gallery (batched vs one by one), plain, "
-                         "fit (differentiable soft-splat fit step), fit_batch (64 fits)")
+                         "fit (differentiable soft-splat fit step), fit_batch (64 fits), "
+                         "stress (12288^2 f32, 2^34 samples, LDS bins vs global atomics)")
     ap.add_argument("--no-fd", action="store_true", help="fit: skip the finite-difference check")
     ap.add_argument("--gallery-size", type=int, default=512)
     ap.add_argument("--steps", type=int, default=5)
@@ -90,6 +91,16 @@ def main():
         return bench_gallery(args)
     if args.workload in ("fit", "fit_batch"):
         return bench_fit(args)
+    if args.workload == "stress":
+        fd, pal, job = configs.stress(walkers=1 << 26)
+        job.splat_mode = 4
+        args.splat_mode = 4
+        args.walkers_log2 = 26
+        return bench_flame(args, fd, pal, job,
+                           "stress: 12 xforms + final over 13 variations (incl. julian, curl, "
+                           "waves), 12288x12288 float4 histogram, 2^26 walkers x 256 iterations "
+                           "(2^34 samples), LDS tile bins", "chaos-game samples/s at 12288^2 (stress)",
+                           compare_mode=3)
     if args.workload == "plain":
         fd, pal, job = configs.plain()
         job.splat_mode = args.splat_mode
@@ -240,7 +251,7 @@ def fd_check(item, tau, grad):
     return out
 
 
-def bench_flame(args, fd, pal, job, workload, metric):
+def bench_flame(args, fd, pal, job, workload, metric, compare_mode=None):
     import torch
     import pyfae
     from pyfae import roofline
@@ -303,6 +314,19 @@ def bench_flame(args, fd, pal, job, workload, metric):
         "kernel_ms": {"walkers": launch_ms * launches / args.steps, "tile_accum": accum_ms,
                       "dispatch_phase": iterate_ms, "rest_of_step": ms_per_step - iterate_ms},
     }
+    if compare_mode is not None:
+        # the same render with another splat (outside the timed steps)
+        mode = job.splat_mode
+        job.splat_mode = compare_mode
+        eng.render_to_device(job, rgba)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.render_to_device(job, rgba)
+        torch.cuda.synchronize()
+        cmp_ms = (time.perf_counter() - t0) * 1e3
+        job.splat_mode = mode
+        out["compare"] = {"splat_mode": compare_mode, "ms_per_step": cmp_ms,
+                          "speedup": cmp_ms / ms_per_step}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(fd, pal, job)
     eng.close()

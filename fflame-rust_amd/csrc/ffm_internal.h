@@ -90,6 +90,7 @@ struct TileArgs {
   uint32_t* cap;         // n_tiles: slots in the region
   uint32_t* item_start;  // n_tiles + 1: work list of tile_accum
   uint32_t* records;     // local | texel << 13 | tile-in-bin << 25
+  uint32_t* sorted;      // bins of several tiles: records re-ordered by tile, same offsets
   uint32_t* hist;        // the u32x4 (or f32x4) histogram
   const float* pal_f32;  // f32 splat: linear colour x 100 per texel, x3
   int32_t tiles_x;       // tiles per tile row

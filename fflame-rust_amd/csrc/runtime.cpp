@@ -54,7 +54,7 @@ struct ffm_ctx {
   // render buffers
   DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv;
   // LDS tile splat: per-tile fill/region/capacity, work list, record buffer
-  DevBuf t_fill, t_region, t_cap, t_items, t_records;
+  DevBuf t_fill, t_region, t_cap, t_items, t_records, t_sorted;
   int t_W = 0, t_H = 0, t_nrows = 0, t_pal = 0;
   uint64_t t_walkers = 0;
   uint32_t t_ipt = 0;
@@ -351,7 +351,10 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   if ((st = ensure(c, c->t_cap, (size_t)TILE_MAX * 4, "tile capacities"))) return st;
   if ((st = ensure(c, c->t_items, (size_t)(TILE_MAX + 4) * 4, "tile work list"))) return st;
   if ((st = ensure(c, c->t_records, rec_cap * 4, "tile records"))) return st;
+  if (g > 0 && (st = ensure(c, c->t_sorted, rec_cap * 4, "tile records sorted by tile")))
+    return st;
   TileArgs t{};
+  t.sorted = g > 0 ? reinterpret_cast<uint32_t*>(c->t_sorted.p) : nullptr;
   t.tile_fill = reinterpret_cast<uint32_t*>(c->t_fill.p);
   t.region = reinterpret_cast<uint32_t*>(c->t_region.p);
   t.cap = reinterpret_cast<uint32_t*>(c->t_cap.p);
@@ -521,7 +524,7 @@ int ffm_ctx_destroy(ffm_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->table, &c->pal_u32, &c->pal_f32, &c->hist, &c->copies, &c->scratch,
                     &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
-                    &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records,
+                    &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records, &c->t_sorted,
                     &c->s_fits_d, &c->s_tables, &c->s_ckpt, &c->s_part, &c->s_xcd})
     b->release();
   free_batch(c->batch);

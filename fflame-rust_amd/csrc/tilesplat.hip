@@ -40,7 +40,7 @@ size_t bin_lds_bytes(int n_rows, int pal_size) {
 }
 
 size_t accum_lds_bytes(int pal_size, bool f32) {
-  return (size_t)TILE_PX * 16 + (size_t)pal_size * (f32 ? 12 : 4);
+  return (size_t)TILE_PX * 16 + (size_t)pal_size * (f32 ? 12 : 4) + 2 * 129 * 4;
 }
 
 // Exclusive scan of a[0..n) (n <= 2 * blockDim.x) in place; all threads.
@@ -353,85 +353,133 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
   if (threadIdx.x == 0) t.item_start[t.n_tiles] = all;
 }
 
-// Accumulate records [r0, r1) of bin `ti` (of the flame `t` describes):
-// for each tile of the bin, the records of that tile go into the LDS tile,
-// which is then folded into the histogram: a read-add-write when the slice
-// is the bin's only one, integer / float atomics otherwise. A bin of several
-// tiles reads its slice once per tile (from L2 after the first pass).
-// All threads; F32: float4 accumulation.
+// Add records run[0..n) into the LDS tile (LDS atomics). All threads.
+template <bool F32>
+__device__ __forceinline__ void accum_run(const uint32_t* run, uint32_t n, uint32_t* tile,
+                                          const uint32_t* pal, const float* palf) {
+  float* tilef = reinterpret_cast<float*>(tile);
+  for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
+    const uint32_t v = run[r];
+    const uint32_t local = v & REC_LOCAL_MASK;
+    if (F32) {
+      const float* c = palf + 3 * rec_texel(v);
+      atomicAdd(&tilef[4 * local + 0], c[0]);
+      atomicAdd(&tilef[4 * local + 1], c[1]);
+      atomicAdd(&tilef[4 * local + 2], c[2]);
+      atomicAdd(&tilef[4 * local + 3], 100.0f);
+    } else {
+      const uint32_t pk = pal[rec_texel(v)];
+      atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
+      atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
+      atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
+      atomicAdd(&tile[4 * local + 3], 100u);
+    }
+  }
+}
+
+// Fold the LDS tile (tile column tx, row ty) into the histogram: a
+// read-add-write when the slice is its bin's only one, atomics otherwise.
+template <bool F32>
+__device__ __forceinline__ void fold_tile(const TileArgs& t, int tx, int ty, uint32_t n_slices,
+                                          const uint32_t* tile) {
+  const float* tilef = reinterpret_cast<const float*>(tile);
+  for (int q = threadIdx.x; q < TILE_PX; q += blockDim.x) {
+    const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
+    if (px >= t.width || py >= t.height) continue;
+    if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
+    const size_t p = (size_t)py * t.width + px;
+    if (F32) {
+      const float4 v =
+          make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
+      float4* h = reinterpret_cast<float4*>(t.hist) + p;
+      if (n_slices == 1) {
+        float4 o = *h;
+        o.x += v.x;
+        o.y += v.y;
+        o.z += v.z;
+        o.w += v.w;
+        *h = o;
+      } else {
+        float* hw = reinterpret_cast<float*>(h);
+        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    } else {
+      const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
+      uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
+      if (n_slices == 1) {
+        uint4 o = *h;
+        o.x += v.x;
+        o.y += v.y;
+        o.z += v.z;
+        o.w += v.w;
+        *h = o;
+      } else {
+        uint32_t* hw = reinterpret_cast<uint32_t*>(h);
+        __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
+// Accumulate records [r0, r1) of bin `ti` (of the flame `t` describes) and
+// fold them into the histogram, tile by tile. A bin of several tiles first
+// counting-sorts the slice by tile into t.sorted (same offsets as the
+// records), so each tile's records are one contiguous run read once.
+// `sort` holds 2 x 129 u32 of LDS. All threads; F32: float4 accumulation.
 template <bool F32>
 __device__ void accum_slice(const TileArgs& t, int ti, uint32_t r0, uint32_t r1, uint32_t n_slices,
-                            uint32_t* tile, const uint32_t* pal, const float* palf) {
+                            uint32_t* tile, const uint32_t* pal, const float* palf,
+                            uint32_t* sort) {
   const int tid = threadIdx.x;
-  float* tilef = reinterpret_cast<float*>(tile);
-  const uint32_t* rec = t.records + t.region[ti];
+  const uint32_t* rec = t.records + t.region[ti] + r0;
+  const uint32_t n = r1 - r0;
   const int ty = ti / t.bins_x;
   const int group = 1 << t.group_shift;
+  uint32_t* fill = sort;         // group: counts, then next free slot per tile
+  uint32_t* start = sort + 129;  // group + 1: first slot per tile
+  const uint32_t* runs = rec;
+  if (group > 1) {
+    for (int i = tid; i < group; i += blockDim.x) fill[i] = 0u;
+    __syncthreads();
+    for (uint32_t r = tid; r < n; r += blockDim.x) atomicAdd(&fill[rec[r] >> REC_SUB_SHIFT], 1u);
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t acc = 0;
+      for (int g = 0; g < group; g++) {
+        start[g] = acc;
+        acc += fill[g];
+        fill[g] = start[g];
+      }
+      start[group] = acc;
+    }
+    __syncthreads();
+    uint32_t* out = t.sorted + t.region[ti] + r0;
+    for (uint32_t r = tid; r < n; r += blockDim.x) {
+      const uint32_t v = rec[r];
+      out[atomicAdd(&fill[v >> REC_SUB_SHIFT], 1u)] = v;
+    }
+    __syncthreads();  // the workgroup's global stores are visible to it after the barrier
+    runs = out;
+  } else if (tid == 0) {
+    start[0] = 0u;
+    start[1] = n;
+  }
   for (int sub = 0; sub < group; sub++) {
     const int tx = bin_tile_x(t, ti, sub);
     if (tx >= t.tiles_x) break;
     for (int i = tid; i < 4 * TILE_PX; i += blockDim.x) tile[i] = 0u;
     __syncthreads();
-    for (uint32_t r = r0 + tid; r < r1; r += blockDim.x) {
-      const uint32_t v = rec[r];
-      if ((int)(v >> REC_SUB_SHIFT) != sub) continue;
-      const uint32_t local = v & REC_LOCAL_MASK;
-      if (F32) {
-        const float* c = palf + 3 * rec_texel(v);
-        atomicAdd(&tilef[4 * local + 0], c[0]);
-        atomicAdd(&tilef[4 * local + 1], c[1]);
-        atomicAdd(&tilef[4 * local + 2], c[2]);
-        atomicAdd(&tilef[4 * local + 3], 100.0f);
-      } else {
-        const uint32_t pk = pal[rec_texel(v)];
-        atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
-        atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
-        atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
-        atomicAdd(&tile[4 * local + 3], 100u);
-      }
-    }
-    __syncthreads();
-    for (int q = tid; q < TILE_PX; q += blockDim.x) {
-      const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
-      if (px >= t.width || py >= t.height) continue;
-      if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
-      const size_t p = (size_t)py * t.width + px;
-      if (F32) {
-        const float4 v =
-            make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
-        float4* h = reinterpret_cast<float4*>(t.hist) + p;
-        if (n_slices == 1) {
-          float4 o = *h;
-          o.x += v.x;
-          o.y += v.y;
-          o.z += v.z;
-          o.w += v.w;
-          *h = o;
-        } else {
-          float* hw = reinterpret_cast<float*>(h);
-          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      } else {
-        const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
-        uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
-        if (n_slices == 1) {
-          uint4 o = *h;
-          o.x += v.x;
-          o.y += v.y;
-          o.z += v.z;
-          o.w += v.w;
-          *h = o;
-        } else {
-          uint32_t* hw = reinterpret_cast<uint32_t*>(h);
-          __hip_atomic_fetch_add(hw + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
+    const uint32_t s0 = start[sub], s1 = start[sub + 1];
+    if (s1 > s0) {  // uniform
+      accum_run<F32>(runs + s0, s1 - s0, tile, pal, palf);
+      __syncthreads();
+      fold_tile<F32>(t, tx, ty, n_slices, tile);
     }
     __syncthreads();
   }
@@ -455,6 +503,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const
   uint32_t* tile = lds;               // TILE_PX x 4 (u32 or f32 bits)
   uint32_t* pal = lds + 4 * TILE_PX;  // pal_size u32, or 3 x pal_size f32
   float* palf = reinterpret_cast<float*>(pal);
+  uint32_t* sort = pal + (F32 ? 3 : 1) * pal_size;  // 2 x 129
   const int tid = threadIdx.x;
   if (F32)
     for (int i = tid; i < 3 * pal_size; i += blockDim.x) palf[i] = t.pal_f32[i];
@@ -466,7 +515,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const
     const uint32_t first = t.item_start[ti], n_slices = t.item_start[ti + 1] - first;
     const uint32_t stored = min(t.tile_fill[ti], t.cap[ti]);
     const uint32_t r0 = (item - first) * SLICE, r1 = min(r0 + SLICE, stored);
-    accum_slice<F32>(t, ti, r0, r1, n_slices, tile, pal, palf);
+    accum_slice<F32>(t, ti, r0, r1, n_slices, tile, pal, palf, sort);
   }
 }
 
@@ -555,7 +604,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_batch_kernel(BatchArgs b
     const uint32_t first = g.item_start[gt], n_slices = g.item_start[gt + 1] - first;
     const uint32_t stored = min(g.tile_fill[gt], g.cap[gt]);
     const uint32_t r0 = (item - first) * SLICE, r1 = min(r0 + SLICE, stored);
-    accum_slice<false>(t, ti, r0, r1, n_slices, tile, pal, nullptr);
+    accum_slice<false>(t, ti, r0, r1, n_slices, tile, pal, nullptr, pal + b.max_pal);
   }
 }
 
