@@ -285,30 +285,36 @@ struct FlameView {
   int n_xforms, n_finals, n_vars;
 };
 
-// One iteration of main_template.wgsl's loop body (:193-1383), 2D,
-// palette colour mode, direct histogram. Walker state is (x, y, ci, fuse).
-// Returns ST_* flags; on ST_PLOT, (px, py) is the in-frame pixel and
-// `texel` the palette texel of the sample.
-__device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& v, int pal_size,
-                                          uint32_t burn_in, Rng& rng, float& x, float& y,
-                                          float& ci, uint32_t& fuse, int& xi_out, int32_t& px,
-                                          int32_t& py, int& texel) {
-  // select_transform_const (utilities.wgsl:35-55): first i with
-  // u * total <= cum_i, else the last; cum is the sequential f32 sum.
+// The first half of an iteration (main_template.wgsl:235-249): the
+// transform pick and the opacity draw. select_transform_const
+// (utilities.wgsl:35-55): first i with u * total <= cum_i, else the last;
+// cum is the sequential f32 sum. The opacity draw is always consumed; a draw
+// is at most 1 - 2^-24, so at opacity >= 1 it always passes and only
+// advances the stream.
+__device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool& should_plot) {
   float u = rng_nextf(rng);
   const int n = fl.n_xforms;
   float target = u * fl.rows[(n - 1) * XF_STRIDE + XF_CUM];
   int xi = n - 1;
   for (int i = n - 2; i >= 0; i--)
     if (target <= fl.rows[i * XF_STRIDE + XF_CUM]) xi = i;
-  xi_out = xi;
-  const float* row = fl.rows + xi * XF_STRIDE;
-  // Opacity draw: always consumed (main_template.wgsl:249). A draw is at
-  // most 1 - 2^-24, so at opacity >= 1 it always passes and only advances.
-  const float opacity = row[XF_OPACITY];
-  bool should_plot = true;
+  const float opacity = fl.rows[xi * XF_STRIDE + XF_OPACITY];
+  should_plot = true;
   if (opacity >= 1.0f) rng_skip(rng);
   else should_plot = rng_nextf(rng) < opacity;
+  return xi;
+}
+
+// The rest of the iteration (main_template.wgsl:355-1383), 2D, palette
+// colour mode, direct histogram, for transform xi. Walker state is
+// (x, y, ci, fuse). Returns ST_* flags; on ST_PLOT, (px, py) is the
+// in-frame pixel and `texel` the palette texel of the sample.
+__device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& v, int pal_size,
+                                          uint32_t burn_in, Rng& rng, float& x, float& y,
+                                          float& ci, uint32_t& fuse, int xi, bool should_plot,
+                                          int32_t& px, int32_t& py, int& texel) {
+  const int n = fl.n_xforms;
+  const float* row = fl.rows + xi * XF_STRIDE;
   float nx = x, ny = y;
   apply_xform(row, fl.ids, fl.n_vars, nx, ny, rng);
   // Bad-value respawn + re-fuse (main_template.wgsl:397-479).
@@ -338,6 +344,18 @@ __device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& 
     return ST_PLOT;
   }
   return 0;
+}
+
+// One whole iteration of main_template.wgsl's loop body (:193-1383).
+__device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& v, int pal_size,
+                                          uint32_t burn_in, Rng& rng, float& x, float& y,
+                                          float& ci, uint32_t& fuse, int& xi_out, int32_t& px,
+                                          int32_t& py, int& texel) {
+  bool should_plot;
+  const int xi = choose_xform(fl, rng, should_plot);
+  xi_out = xi;
+  return step_xform(fl, v, pal_size, burn_in, rng, x, y, ci, fuse, xi, should_plot, px, py,
+                    texel);
 }
 
 struct LdsFlame {
