@@ -246,6 +246,48 @@ __device__ __forceinline__ void apply_xform(const float* __restrict__ row, const
   y = ry;
 }
 
+// The flame as one lane sees it: LDS rows + uniform header values.
+struct FlameView {
+  const float* rows;         // LDS, (n_xforms + n_finals) * XF_STRIDE
+  const int* ids;            // LDS, n_vars
+  const FlameHeader* hdr;    // LDS copy of the header (final-chain slot lists), or null
+  int n_xforms, n_finals, n_vars;
+};
+
+// Final transform i of the flame: apply_xform's arithmetic with the loop
+// over the final's non-zero variation slots only (a zero weight adds
+// nothing and draws nothing, so the sum is the same). Every lane applies the
+// same final, so each read is one uniform LDS address.
+__device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x, float& y,
+                                            Rng& rng) {
+  const float* row = fl.rows + (fl.n_xforms + i) * XF_STRIDE;
+  if (!fl.hdr) {
+    apply_xform(row, fl.ids, fl.n_vars, x, y, rng);
+    return;
+  }
+  float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
+  float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
+  float rx = 0.0f, ry = 0.0f;
+  const int nact = __builtin_amdgcn_readfirstlane(fl.hdr->fin_nact[i]);
+  for (int j = 0; j < nact; j++) {
+    const int k = __builtin_amdgcn_readfirstlane(fl.hdr->fin_act[i][j]);
+    const int id = __builtin_amdgcn_readfirstlane(fl.ids[k]);
+    const float w = row[XF_W0 + k];
+    float vx, vy;
+    eval_variation(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
+    rx += w * vx;
+    ry += w * vy;
+  }
+  if (row[XF_POST] != 0.0f) {
+    float qx = row[XF_PA] * rx + row[XF_PB] * ry + row[XF_PE];
+    float qy = row[XF_PC] * rx + row[XF_PD] * ry + row[XF_PF];
+    rx = qx;
+    ry = qy;
+  }
+  x = rx;
+  y = ry;
+}
+
 // Saturating f32 -> i32 as the reference's i32(floor(.)) behaves on GPUs:
 // out-of-range values clamp to INT_MIN / INT_MAX and NaN gives 0, which is
 // exactly v_cvt_i32_f32's definition (one instruction; a C cast would be
@@ -278,12 +320,6 @@ __device__ __forceinline__ int palette_texel(float ci, int size) {
 
 enum StepFlags { ST_PLOT = 1, ST_RESPAWN = 2, ST_BURN = 4 };
 
-// The flame as one lane sees it: LDS rows + uniform header values.
-struct FlameView {
-  const float* rows;  // LDS, (n_xforms + n_finals) * XF_STRIDE
-  const int* ids;     // LDS, n_vars
-  int n_xforms, n_finals, n_vars;
-};
 
 // The first half of an iteration (main_template.wgsl:235-249): the
 // transform pick and the opacity draw. select_transform_const
@@ -336,8 +372,7 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
   }
   // Final chain: reshapes the plot only (main_template.wgsl:663-717).
   float fx = nx, fy = ny;
-  for (int i = 0; i < fl.n_finals; i++)
-    apply_xform(fl.rows + (n + i) * XF_STRIDE, fl.ids, fl.n_vars, fx, fy, rng);
+  for (int i = 0; i < fl.n_finals; i++) apply_final(fl, i, fx, fy, rng);
   world_to_pixel(v, fx, fy, px, py);
   if (px >= 0 && px < v.width && py >= 0 && py < v.height && should_plot) {
     texel = palette_texel(ci, pal_size);
@@ -383,6 +418,7 @@ __device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_
   LdsFlame l;
   l.fv.rows = reinterpret_cast<const float*>(lds + hdr_w);
   l.fv.ids = reinterpret_cast<const int*>(lds + 4);
+  l.fv.hdr = reinterpret_cast<const FlameHeader*>(lds);
   l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
   l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
   l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);

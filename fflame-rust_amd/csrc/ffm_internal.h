@@ -35,6 +35,11 @@ struct FlameHeader {
   int32_t n_vars;
   int32_t var_mask;  // bit per ffm variation slot family, for fast paths
   int32_t var_id[FFM_MAX_VARS];
+  // Each final transform's variation slots with a non-zero weight, in
+  // local-index order: every lane applies the same final, so its loop runs
+  // over these slots only.
+  int32_t fin_nact[FFM_MAX_FINALS];
+  int32_t fin_act[FFM_MAX_FINALS][FFM_MAX_VARS];
 };
 
 // A flame table in device memory: header then (n_xforms + n_finals) rows.

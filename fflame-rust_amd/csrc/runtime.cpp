@@ -166,6 +166,9 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
     h.var_id[k] = f->var_id[k];
     h.var_mask |= 1 << (f->var_id[k] & 31);
   }
+  for (int i = 0; i < f->n_finals; i++)
+    for (int k = 0; k < f->n_vars; k++)
+      if (f->finals[i].var_weight[k] != 0.0f) h.fin_act[i][h.fin_nact[i]++] = k;
   std::memcpy(t.data(), &h, sizeof h);
   float* r0 = t.data() + sizeof(FlameHeader) / 4;
   float cum = 0.0f;
