@@ -370,9 +370,21 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
     px = py = -1;
     return ST_BURN;
   }
-  // Final chain: reshapes the plot only (main_template.wgsl:663-717).
+  // Final chain: reshapes the plot only (main_template.wgsl:663-717), the
+  // finals this transform attaches, in order.
   float fx = nx, fy = ny;
-  for (int i = 0; i < fl.n_finals; i++) apply_final(fl, i, fx, fy, rng);
+  if (fl.n_finals > 0) {
+    if (fl.hdr && fl.hdr->fin_uniform) {
+      const int nc = __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain_n);
+      for (int j = 0; j < nc; j++)
+        apply_final(fl, __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain[j]), fx, fy, rng);
+    } else {
+      const int nc = (int)row[XF_NFIN];
+      for (int j = 0; j < nc; j++)
+        apply_xform(fl.rows + (n + (int)row[XF_FIN0 + j]) * XF_STRIDE, fl.ids, fl.n_vars, fx, fy,
+                    rng);
+    }
+  }
   world_to_pixel(v, fx, fy, px, py);
   if (px >= 0 && px < v.width && py >= 0 && py < v.height && should_plot) {
     texel = palette_texel(ci, pal_size);

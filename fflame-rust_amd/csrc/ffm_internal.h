@@ -25,8 +25,11 @@ enum XfField : int {
   XF_POST,     // 1.0 when the post-affine applies
   XF_W0,       // FFM_MAX_VARS variation weights
   XF_P0 = XF_W0 + FFM_MAX_VARS,  // FFM_MAX_VARS x FFM_MAX_VAR_PARAMS params
-  XF_STRIDE = XF_P0 + FFM_MAX_VARS * FFM_MAX_VAR_PARAMS  // 97: odd
+  XF_NFIN = XF_P0 + FFM_MAX_VARS * FFM_MAX_VAR_PARAMS,  // finals this transform applies
+  XF_FIN0,                                              // their indices (FFM_MAX_FINALS)
+  XF_STRIDE = XF_FIN0 + FFM_MAX_FINALS + 1  // 103: odd (one pad word)
 };
+static_assert(XF_STRIDE % 2 == 1, "odd row stride");
 
 // Per-flame header that precedes the rows (ints stored as int32).
 struct FlameHeader {
@@ -40,6 +43,12 @@ struct FlameHeader {
   // over these slots only.
   int32_t fin_nact[FFM_MAX_FINALS];
   int32_t fin_act[FFM_MAX_FINALS][FFM_MAX_VARS];
+  // 1 when every normal transform applies the same final chain (then
+  // fin_chain lists it and the chain is uniform across a wave); 0 when the
+  // chain depends on the transform (its row's XF_NFIN / XF_FIN0 list).
+  int32_t fin_uniform;
+  int32_t fin_chain_n;
+  int32_t fin_chain[FFM_MAX_FINALS];
 };
 
 // A flame table in device memory: header then (n_xforms + n_finals) rows.

@@ -139,6 +139,18 @@ int check_flame(ffm_ctx* c, const ffm_flame_desc* f) {
   for (int k = 0; k < f->n_vars; k++)
     if (!supported_var(f->var_id[k]))
       return fail(c, FFM_ERR_INVALID_ARGUMENT, "variation id %d is not supported", f->var_id[k]);
+  if (f->per_transform_finals)
+    for (int i = 0; i < f->n_xforms; i++) {
+      const ffm_xform& x = f->xforms[i];
+      if (x.n_final_attach < 0 || x.n_final_attach > FFM_MAX_FINALS)
+        return fail(c, FFM_ERR_INVALID_ARGUMENT, "transform %d attaches %d finals (0..%d)", i,
+                    x.n_final_attach, FFM_MAX_FINALS);
+      for (int j = 0; j < x.n_final_attach; j++)
+        if (x.final_attach[j] < 0 || x.final_attach[j] >= f->n_finals)
+          return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                      "transform %d attaches final %d; the flame has %d", i, x.final_attach[j],
+                      f->n_finals);
+    }
   return FFM_OK;
 }
 
@@ -169,6 +181,23 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
   for (int i = 0; i < f->n_finals; i++)
     for (int k = 0; k < f->n_vars; k++)
       if (f->finals[i].var_weight[k] != 0.0f) h.fin_act[i][h.fin_nact[i]++] = k;
+  // each normal transform's final chain (check_flame validated the lists)
+  auto chain = [&](int i, int* out) {
+    if (!f->per_transform_finals) {
+      for (int j = 0; j < f->n_finals; j++) out[j] = j;
+      return f->n_finals;
+    }
+    const ffm_xform& x = f->xforms[i];
+    for (int j = 0; j < x.n_final_attach; j++) out[j] = x.final_attach[j];
+    return (int)x.n_final_attach;
+  };
+  h.fin_uniform = 1;
+  h.fin_chain_n = chain(0, h.fin_chain);
+  for (int i = 1; i < f->n_xforms; i++) {
+    int c[FFM_MAX_FINALS];
+    const int n = chain(i, c);
+    if (n != h.fin_chain_n || !std::equal(c, c + n, h.fin_chain)) h.fin_uniform = 0;
+  }
   std::memcpy(t.data(), &h, sizeof h);
   float* r0 = t.data() + sizeof(FlameHeader) / 4;
   float cum = 0.0f;
@@ -187,6 +216,12 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
     r[XF_C2] = x.color * (1.0f - x.color_speed) / 2.0f;
     r[XF_OPACITY] = x.opacity;
     r[XF_POST] = x.post_enabled ? 1.0f : 0.0f;
+    if (i < f->n_xforms) {
+      int c[FFM_MAX_FINALS];
+      const int n = chain(i, c);
+      r[XF_NFIN] = (float)n;
+      for (int j = 0; j < n; j++) r[XF_FIN0 + j] = (float)c[j];
+    }
     for (int k = 0; k < f->n_vars; k++) {
       r[XF_W0 + k] = x.var_weight[k];
       for (int q = 0; q < FFM_MAX_VAR_PARAMS; q++)

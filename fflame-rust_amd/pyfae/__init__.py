@@ -61,14 +61,17 @@ def _ptr(x):
 # ---------------------------------------------------------------- flames
 
 def xform(a=1.0, b=0.0, c=0.0, d=1.0, e=0.0, f=0.0, *, weight=1.0, color=0.0, color_speed=0.5,
-          opacity=1.0, variations=None, params=None, post=None):
+          opacity=1.0, variations=None, params=None, post=None, final_attachments=None):
     """A transform as the reference's Transform holds it (src/scene/transforms.rs:337-530):
     `variations` {name: weight}, `params` {"julian.power": 3.0, ...},
-    `post` (a, b, c, d, e, f) or None."""
+    `post` (a, b, c, d, e, f) or None, `final_attachments` the indices of the
+    finals it applies (None: every final, in order)."""
     return {"a": a, "b": b, "c": c, "d": d, "e": e, "f": f, "weight": weight, "color": color,
             "color_speed": color_speed, "opacity": opacity,
             "variations": dict(variations or {"linear": 1.0}), "params": dict(params or {}),
-            "post": tuple(post) if post is not None else None}
+            "post": tuple(post) if post is not None else None,
+            "final_attachments": list(final_attachments) if final_attachments is not None
+            else None}
 
 
 def flame_desc(transforms, finals=()):
@@ -115,6 +118,17 @@ def flame_desc(transforms, finals=()):
         fill(fd.xforms[i], t)
     for i, t in enumerate(finals):
         fill(fd.finals[i], t)
+    # per-transform final chains when any transform names its own
+    # (a transform that names none then applies none, as the reference's)
+    if any(t.get("final_attachments") is not None for t in transforms):
+        fd.per_transform_finals = 1
+        for i, t in enumerate(transforms):
+            att = t.get("final_attachments") or []
+            if len(att) > _abi.MAX_FINALS:
+                raise EngineError(_abi.ERR_TOO_LARGE, f"transform {i} attaches {len(att)} finals")
+            fd.xforms[i].n_final_attach = len(att)
+            for j, k in enumerate(att):
+                fd.xforms[i].final_attach[j] = int(k)
     return fd
 
 

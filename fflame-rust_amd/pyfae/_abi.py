@@ -47,6 +47,8 @@ class Xform(C.Structure):
         ("opacity", C.c_float),
         ("var_weight", C.c_float * MAX_VARS),
         ("var_param", (C.c_float * MAX_VAR_PARAMS) * MAX_VARS),
+        ("n_final_attach", C.c_int32),
+        ("final_attach", C.c_int32 * MAX_FINALS),
     ]
 
 
@@ -56,6 +58,7 @@ class FlameDesc(C.Structure):
         ("var_id", C.c_int32 * MAX_VARS),
         ("xforms", Xform * MAX_XFORMS),
         ("finals", Xform * MAX_FINALS),
+        ("per_transform_finals", C.c_int32),
     ]
 
 

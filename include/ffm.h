@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 1
+#define FFM_ABI_VERSION 2
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -93,6 +93,13 @@ typedef struct {
   float opacity;
   float var_weight[FFM_MAX_VARS];
   float var_param[FFM_MAX_VARS][FFM_MAX_VAR_PARAMS];
+  /* The final transforms this normal transform applies, in order, as
+   * indices into ffm_flame_desc.finals, read when the flame's
+   * per_transform_finals is set (Transform::final_attachments; the
+   * reference's AttachmentList, shaders/core/header.wgsl:173-178 and
+   * src/gpu/buffers.rs:596-650). */
+  int32_t n_final_attach;
+  int32_t final_attach[FFM_MAX_FINALS];
 } ffm_xform;
 
 /* A flame: normal transforms, the final chain, and the flame's active
@@ -105,6 +112,10 @@ typedef struct {
   int32_t var_id[FFM_MAX_VARS];
   ffm_xform xforms[FFM_MAX_XFORMS];
   ffm_xform finals[FFM_MAX_FINALS];
+  /* 0: every normal transform applies every final, in order -- a single
+   * legacy final (migrate_legacy_final, src/scene/transforms.rs:2257-2266)
+   * and .flame imports; 1: each applies its own final_attach list. */
+  int32_t per_transform_finals;
 } ffm_flame_desc;
 
 /* The view: FractalConfig zoom/pan/rotation as world_to_pixel reads them

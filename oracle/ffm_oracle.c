@@ -435,9 +435,15 @@ static int step(const step_ctx* c, orc_walker* w, orc_rng* rng, int32_t* ppx, in
     *ppx = *ppy = -1;
     return ST_BURN;
   }
-  /* final chain (main_template.wgsl:663-717): reshapes the plot only */
+  /* final chain (main_template.wgsl:663-717): reshapes the plot only; the
+   * finals this transform attaches (its AttachmentList,
+   * shaders/core/header.wgsl:173-178), or every final in order */
   float fx = x, fy = y;
-  for (int i = 0; i < f->n_finals; i++) orc_apply_xform(f, &f->finals[i], &fx, &fy, rng);
+  int nc = f->per_transform_finals ? xf->n_final_attach : f->n_finals;
+  for (int j = 0; j < nc; j++) {
+    int i = f->per_transform_finals ? xf->final_attach[j] : j;
+    orc_apply_xform(f, &f->finals[i], &fx, &fy, rng);
+  }
   int32_t px, py;
   orc_world_to_pixel(c->v, fx, fy, &px, &py);
   *ppx = px;

@@ -390,3 +390,32 @@ def test_errors_are_status_codes(engine):
         e2.render(pyfae.make_job(64, 64, walkers=0))
     assert ei.value.status == pyfae._abi.ERR_INVALID_ARGUMENT
     e2.close()
+
+
+# ------------------------------------------------------ final attachments
+
+@pytest.mark.parametrize("attach", [None, [[], [1, 0], [0]], [[0, 1], [0, 1], [0, 1]]])
+def test_final_attachments_bit_exact(engine, attach):
+    """Per-transform final chains (the reference's AttachmentList) on a
+    linear flame: the device trace and the LDS-splat histogram equal the
+    oracle's bit for bit, for the shared chain (uniform across a wave) and
+    for chains that differ by transform."""
+    say("final attachments", attach)
+    rng = configs.SplitMix64(77)
+    xs = []
+    for i in range(3):
+        a = configs._affine(rng, 0.7)
+        xs.append(pyfae.xform(*a, weight=1.0, color=i / 2.0, variations={"linear": 1.0},
+                              final_attachments=None if attach is None else attach[i]))
+    fins = [pyfae.xform(0.9, 0.1, -0.1, 0.9, 0.05, 0.0, variations={"linear": 1.0}),
+            pyfae.xform(0.8, 0.0, 0.0, -0.8, 0.0, 0.1, variations={"linear": 1.0})]
+    fd = pyfae.flame_desc(xs, fins)
+    pal = configs.random_palette(configs.SplitMix64(5))
+    view = pyfae.View(320, 240, 1.0, 0.0, 0.0, 0.0)
+    for w in (0, 4097):
+        assert np.array_equal(gpu_trace(engine, fd, pal, view, w, 1, 20, 300),
+                              O.trace(fd, pal, view, w, 1, 20, 300))
+    g, plotted, _ = gpu_hist(engine, fd, pal, view, 1 << 14, 200, 20, 3, mode=2)
+    o, oplotted = O.iterate_mt(fd, pal, view, 1 << 14, 200, 20, 3, 8)
+    assert plotted == oplotted
+    assert np.array_equal(g, o)

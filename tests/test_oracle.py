@@ -494,3 +494,55 @@ def test_density_estimate_dense_pixels_stay_put():
     acc[10, 10, 3] = 1.0
     out = O.density_estimate(acc, 9.0, 0.0, 0.4)
     assert (out[..., 3] > 0).sum() > 250
+
+
+# ------------------------------------------------------ final attachments
+
+def _attach_flame(attach):
+    """3 linear transforms and 2 linear finals; `attach` per transform, or
+    None for the legacy chain (every final, in order)."""
+    rng = configs.SplitMix64(77)
+    xs = []
+    for i in range(3):
+        a = configs._affine(rng, 0.7)
+        xs.append(pyfae.xform(*a, weight=1.0, color=i / 2.0, variations={"linear": 1.0},
+                              final_attachments=None if attach is None else attach[i]))
+    fins = [pyfae.xform(0.9, 0.1, -0.1, 0.9, 0.05, 0.0, variations={"linear": 1.0}),
+            pyfae.xform(0.8, 0.0, 0.0, -0.8, 0.0, 0.1, variations={"linear": 1.0})]
+    return pyfae.flame_desc(xs, fins)
+
+
+def test_final_attachments_all_equal_the_legacy_chain():
+    """Every transform attaching [0, 1] is the single shared chain
+    (migrate_legacy_final attaches the legacy final to every transform,
+    src/scene/transforms.rs:2257-2266)."""
+    fd0 = _attach_flame(None)
+    fd1 = _attach_flame([[0, 1]] * 3)
+    assert fd0.per_transform_finals == 0 and fd1.per_transform_finals == 1
+    pal = configs.random_palette(configs.SplitMix64(5))
+    view = pyfae.View(256, 256, 1.0, 0.0, 0.0, 0.0)
+    for w in (0, 9, 1234):
+        assert np.array_equal(O.trace(fd0, pal, view, w, 1, 5, 300), O.trace(fd1, pal, view, w, 1, 5, 300))
+
+
+def test_final_attachments_per_transform():
+    """A transform attaching no final plots its own output; one attaching
+    [1, 0] applies final 1 then final 0 (AttachmentList order,
+    shaders/core/main_template.wgsl:663-717)."""
+    fd = _attach_flame([[], [1, 0], [0]])
+    pal = configs.random_palette(configs.SplitMix64(5))
+    view = pyfae.View(256, 256, 1.0, 0.0, 0.0, 0.0)
+    tr = O.trace(fd, pal, view, 3, 1, 5, 400)
+    f = O.prepared(fd)
+    seen = set()
+    for row in tr[5:]:  # past the burn-in
+        xi = int(row[1])
+        x, y = np.frombuffer(row[2:4].tobytes(), np.float32)
+        fx, fy = C.c_float(x), C.c_float(y)
+        for k in {0: [], 1: [1, 0], 2: [0]}[xi]:
+            O.lib().orc_apply_xform(C.byref(f), C.byref(f.finals[k]), C.byref(fx), C.byref(fy), None)
+        px, py = C.c_int32(), C.c_int32()
+        O.lib().orc_world_to_pixel(C.byref(view), fx, fy, C.byref(px), C.byref(py))
+        assert (int(np.int32(row[5])), int(np.int32(row[6]))) == (px.value, py.value), xi
+        seen.add(xi)
+    assert seen == {0, 1, 2}
