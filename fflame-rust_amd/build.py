@@ -20,7 +20,7 @@ LIB = os.path.join(OUT_DIR, "libffm.so")
 OBJ_DIR = os.path.join(PKG, "_build")
 
 HIP_SOURCES = ["iterate.hip", "stencils.hip", "tilesplat.hip", "softsplat.hip"]
-HOST_SOURCES = ["runtime.cpp"]
+HOST_SOURCES = ["runtime.cpp", "scene.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]

@@ -14,6 +14,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -263,12 +265,97 @@ void palette_sample(const float* s, int n, float t, float out[3]) {
   for (int c = 0; c < 3; c++) out[c] = a[1 + c] + (b[1 + c] - a[1 + c]) * lt;
 }
 
+// Rust's `f32 as usize`: truncation toward zero, saturating (NaN -> 0).
+size_t sat_index(float v, int size) {
+  if (!(v > 0.0f)) return 0;
+  if (v >= (float)size) return (size_t)size - 1;
+  return std::min((size_t)v, (size_t)size - 1);
+}
+
+// geometric_squeeze_lookup (src/scene/palette.rs:607-640): octave n of the
+// input covers r^n (1 - r) and maps onto the whole palette.
+float geometric_squeeze(float t, float r) {
+  if (r >= 0.999f) return t;
+  if (r <= 0.001f) return std::min(std::max(t / std::max(r, 1e-6f), 0.0f), 1.0f);
+  if (t <= 0.0f) return 0.0f;
+  float start = 0.0f, rn = 1.0f;
+  for (int k = 0; k < 64; k++) {
+    float width = rn * (1.0f - r);
+    float end = start + width;
+    if (t < end) {
+      if (width <= 0.0f) return 0.0f;
+      return std::min(std::max((t - start) / width, 0.0f), 1.0f);
+    }
+    start = end;
+    rn *= r;
+    if (rn < 1e-10f) return 1.0f;
+  }
+  return 1.0f;
+}
+
+// log_redistribute_lookup (src/scene/palette.rs:653-671)
+float log_redistribute(float t, float strength) {
+  if (std::fabs(strength) < 1.1920929e-7f) return t;
+  if (strength > 0.0f) {
+    float den = std::expm1(strength);
+    if (den == 0.0f) return t;
+    return std::min(std::max(std::expm1(strength * t) / den, 0.0f), 1.0f);
+  }
+  float s = -strength;
+  float den = std::expm1(s);
+  if (den == 0.0f) return t;
+  return std::min(std::max(1.0f - std::expm1(s * (1.0f - t)) / den, 0.0f), 1.0f);
+}
+
+// transform_lookup_table (src/scene/palette.rs:703-800) as an index map:
+// entry i of the transformed table is entry src[i] of the sampled one.
+std::vector<int> palette_index_map(int size, const ffm_palette_xf* xf) {
+  std::vector<int> idx(size);
+  for (int i = 0; i < size; i++) idx[i] = i;
+  if (!xf) return idx;
+  auto remap = [&](auto src_t_of) {
+    std::vector<int> out(size);
+    for (int i = 0; i < size; i++) {
+      float t = (float)i / (float)size;
+      out[i] = idx[sat_index(src_t_of(t) * (float)size, size)];
+    }
+    idx.swap(out);
+  };
+  if (xf->squeeze_mode == 0 && xf->squeeze_factor != 1.0f) {
+    float f = xf->squeeze_factor;
+    remap([f](float t) {
+      float x = t * f;
+      return x - std::trunc(x);  // f32::fract
+    });
+  } else if (xf->squeeze_mode == 1) {
+    float r = xf->squeeze_factor;
+    remap([r](float t) { return geometric_squeeze(t, r); });
+  }
+  if (!(std::fabs(xf->log_strength) < 1.1920929e-7f)) {
+    float sgt = xf->log_strength;
+    remap([sgt](float t) { return log_redistribute(t, sgt); });
+  }
+  if (xf->rotation != 0.0f) {
+    int amount = (int)std::round(xf->rotation * (float)size);  // half away from zero
+    std::vector<int> out(size);
+    for (int i = 0; i < size; i++) {
+      int k = (int)(((int64_t)i + amount) % size);
+      if (k < 0) k += size;
+      out[i] = idx[k];
+    }
+    idx.swap(out);
+  }
+  if (xf->reverse) std::reverse(idx.begin(), idx.end());
+  return idx;
+}
+
 void palette_tables(const float* stops, int n, int size, std::vector<uint32_t>& pk,
-                    std::vector<float>& lin) {
+                    std::vector<float>& lin, const ffm_palette_xf* xf = nullptr) {
   pk.assign(size, 0);
   lin.assign(3 * (size_t)size, 0.0f);
+  std::vector<int> src = palette_index_map(size, xf);
   for (int i = 0; i < size; i++) {
-    float t = (float)i / (float)(size - 1);
+    float t = (float)src[i] / (float)(size - 1);
     float c[3];
     palette_sample(stops, n, t, c);
     uint32_t packed = 0;
@@ -600,7 +687,14 @@ int ffm_flame_set(ffm_ctx* c, const ffm_flame_desc* f) {
 }
 
 int ffm_palette_set(ffm_ctx* c, const float* stops, int n_stops, int table_size) {
+  return ffm_palette_set_xf(c, stops, n_stops, table_size, nullptr);
+}
+
+int ffm_palette_set_xf(ffm_ctx* c, const float* stops, int n_stops, int table_size,
+                       const ffm_palette_xf* xf) {
   if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (xf && xf->squeeze_mode != 0 && xf->squeeze_mode != 1)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "palette squeeze mode %d", xf->squeeze_mode);
   if (!stops || n_stops < 1) return fail(c, FFM_ERR_NO_PALETTE, "No palette found");
   if (table_size < FFM_MIN_PALETTE || table_size > FFM_MAX_PALETTE)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "palette size %d outside %d..%d", table_size,
@@ -608,7 +702,7 @@ int ffm_palette_set(ffm_ctx* c, const float* stops, int n_stops, int table_size)
   (void)hipSetDevice(c->device);
   std::vector<uint32_t> pk;
   std::vector<float> lin;
-  palette_tables(stops, n_stops, table_size, pk, lin);
+  palette_tables(stops, n_stops, table_size, pk, lin, xf);
   int st;
   if ((st = ensure(c, c->pal_u32, pk.size() * 4, "palette"))) return st;
   if ((st = ensure(c, c->pal_f32, lin.size() * 4, "palette"))) return st;
@@ -904,6 +998,26 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
 int ffm_render(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, ffm_render_stats* stats) {
   if (!rgba_out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel buffer is NULL");
   return render_impl(c, job, rgba_out, nullptr, stats);
+}
+
+int ffm_render_config(ffm_ctx* c, const char* text, size_t len, int width, int height,
+                      uint64_t iterations, uint32_t ipt, uint32_t flags, uint8_t* rgba_out,
+                      ffm_render_stats* stats) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!rgba_out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel buffer is NULL");
+  std::unique_ptr<ffm_scene> sc(new (std::nothrow) ffm_scene);
+  if (!sc) return fail(c, FFM_ERR_OUT_OF_MEMORY, "out of host memory");
+  char err[512] = {0};
+  int st = ffm_scene_parse(text, len, 0, sc.get(), err, sizeof err);
+  if (st) return fail(c, st, "%s", err);
+  ffm_job job;
+  st = ffm_scene_job(sc.get(), width, height, iterations, ipt, flags, &job, err, sizeof err);
+  if (st) return fail(c, st, "%s", err);
+  if ((st = ffm_flame_set(c, &sc->flame))) return st;
+  if ((st = ffm_palette_set_xf(c, &sc->stops[0][0], sc->n_stops, sc->palette_size,
+                               &sc->palette_xf)))
+    return st;
+  return render_impl(c, &job, rgba_out, nullptr, stats);
 }
 
 int ffm_render_to_device(ffm_ctx* c, const ffm_job* job, void* rgba_dev, ffm_render_stats* stats) {

@@ -11,13 +11,16 @@ There is no CPU fallback: without libffm.so (or without a GPU) every call
 raises EngineError.
 """
 import ctypes as C
+import json
 import os
+import struct
+import zlib
 
 import numpy as np
 
 from . import _abi
-from ._abi import (FlameDesc, Job, RenderStats, SoftJob, TonemapParams, View,  # noqa: F401
-                   Xform, VARIATIONS, tonemap_defaults)
+from ._abi import (FlameDesc, Job, PaletteXf, RenderStats, Scene, SoftJob,  # noqa: F401
+                   TonemapParams, View, Xform, VARIATIONS, tonemap_defaults)
 
 __version__ = "0.5.0+mi355x"
 
@@ -197,6 +200,112 @@ def soft_theta(fd):
     return th
 
 
+# ---------------------------------------------------------------- configs
+
+def _text(x):
+    """Config text from a dict / list (JSON), bytes, JSON or XML text, or a
+    path (python/pyfae/__init__.py:49-60, with XML text besides)."""
+    if isinstance(x, (dict, list)):
+        return json.dumps(x)
+    if isinstance(x, (bytes, bytearray)):
+        return bytes(x).decode("utf-8")
+    if isinstance(x, os.PathLike):
+        x = os.fspath(x)
+    if isinstance(x, str) and not x.lstrip().startswith(("{", "[", "<")) and os.path.exists(x):
+        with open(x, encoding="utf-8") as f:
+            return f.read()
+    if not isinstance(x, str):
+        raise TypeError(f"expected a config (dict, JSON / XML text or a path), got {type(x)}")
+    return x
+
+
+def scene_count(config):
+    """How many configs a text holds: 1 for a JSON object, its length for a
+    JSON array, the number of <flame> elements of a .flame document."""
+    raw = _text(config).encode("utf-8")
+    n, err = C.c_int(), C.create_string_buffer(512)
+    st = library().ffm_scene_count(raw, len(raw), C.byref(n), err, 512)
+    if st != 0:
+        raise EngineError(st, err.value.decode(errors="replace"))
+    return n.value
+
+
+def load(config, index=0):
+    """Config `index` of a .fflame / .fae JSON text or a .flame XML document
+    (text, dict or path) as an ffm_scene: the flame descriptor, palette,
+    view, tone map and budget the reference would render it with. Raises
+    EngineError with status ERR_UNSUPPORTED for a feature this engine does
+    not render (the message names it)."""
+    raw = _text(config).encode("utf-8")
+    sc = Scene()
+    err = C.create_string_buffer(512)
+    st = library().ffm_scene_parse(raw, len(raw), int(index), C.byref(sc), err, 512)
+    if st != 0:
+        raise EngineError(st, err.value.decode(errors="replace"))
+    return sc
+
+
+def scene_stops(sc):
+    """The scene's palette stops as an (n, 4) float32 array."""
+    return np.ctypeslib.as_array(sc.stops)[:sc.n_stops].astype(np.float32).copy()
+
+
+def scene_job(sc, width, height, *, iterations=None, iterations_per_thread=None, as_2d=False):
+    """The ffm_job the reference's export builds for a scene at width x height."""
+    j = Job()
+    err = C.create_string_buffer(512)
+    st = library().ffm_scene_job(C.byref(sc), int(width), int(height), int(iterations or 0),
+                                 int(iterations_per_thread or 0),
+                                 _abi.RENDER_3D_AS_2D if as_2d else 0, C.byref(j), err, 512)
+    if st != 0:
+        raise EngineError(st, err.value.decode(errors="replace"))
+    return j
+
+
+def _png(rgba):
+    """A plain RGBA8 PNG of an (h, w, 4) uint8 array."""
+    h, w = rgba.shape[:2]
+    raw = b"".join(b"\x00" + rgba[y].tobytes() for y in range(h))
+
+    def chunk(tag, data):
+        return (struct.pack(">I", len(data)) + tag + data
+                + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF))
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0))
+            + chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+
+
+class Image:
+    """A rendered frame (python/pyfae/__init__.py:76-113): RGBA8 pixels,
+    `width` x `height`, and its numbers (`info`: iterations, ms,
+    iterations_per_thread, dispatches)."""
+
+    def __init__(self, rgba, info):
+        self.rgba = rgba
+        self.info = info
+        self.width = info["width"]
+        self.height = info["height"]
+
+    def to_numpy(self):
+        """An (height, width, 4) uint8 array."""
+        return self.rgba
+
+    def to_pil(self):
+        from PIL import Image as PILImage
+        return PILImage.fromarray(self.rgba, "RGBA")
+
+    def png(self):
+        """The frame as PNG bytes (pixels only: no config metadata)."""
+        return _png(self.rgba)
+
+    def save(self, path):
+        with open(path, "wb") as f:
+            f.write(self.png())
+
+    def __repr__(self):
+        return (f"<pyfae.Image {self.width}x{self.height}, {self.info['iterations']:,} "
+                f"iterations, {self.info['ms']:.0f} ms>")
+
+
 # ---------------------------------------------------------------- engine
 
 class Engine:
@@ -241,8 +350,44 @@ class Engine:
     def cancel(self):
         self._check(self.lib.ffm_cancel(self.ctx))
 
+    def set_scene(self, sc):
+        """Set a scene's flame and palette (with its lookup-table transform)."""
+        self._check(self.lib.ffm_flame_set(self.ctx, C.byref(sc.flame)))
+        self._check(self.lib.ffm_palette_set_xf(
+            self.ctx, C.cast(C.addressof(sc.stops), C.POINTER(C.c_float)), sc.n_stops,
+            sc.palette_size, C.byref(sc.palette_xf)))
+
     # -- whole renders
-    def render(self, job):
+    def render(self, config, width=None, height=None, *, iterations=None,
+               iterations_per_thread=None, as_2d=False):
+        """Render a config (python/pyfae/__init__.py:201-221): `config` is a
+        .fflame / .fae JSON or .flame XML text, a dict or a path, rendered at
+        `width` x `height` with the config's flame, palette, view and tone map
+        (`iterations`: the sample budget, default the config's own;
+        `iterations_per_thread`: default 1024). Returns an Image. `as_2d`
+        renders a 3D-mode config (every .flame import) whose camera is flat
+        through the 2D chaos game.
+
+        Given an ffm_job instead, renders the current flame and returns
+        (pixels, stats)."""
+        if isinstance(config, Job):
+            return self._render_job(config)
+        if width is None or height is None:
+            raise TypeError("render(config, width, height): width and height are required")
+        raw = _text(config).encode("utf-8")
+        out = np.zeros((int(height), int(width), 4), dtype=np.uint8)
+        stats = RenderStats()
+        ipt = int(iterations_per_thread or 0)
+        self._check(self.lib.ffm_render_config(
+            self.ctx, raw, len(raw), int(width), int(height), int(iterations or 0), ipt,
+            _abi.RENDER_3D_AS_2D if as_2d else 0, out.ctypes.data_as(C.POINTER(C.c_uint8)),
+            C.byref(stats)))
+        return Image(out, {"width": int(width), "height": int(height),
+                           "iterations": int(stats.total_samples), "ms": stats.render_time_ms,
+                           "iterations_per_thread": ipt or 1024,
+                           "dispatches": int(stats.dispatches)})
+
+    def _render_job(self, job):
         """RGBA8 (H, W, 4) uint8 pixels of the current flame; returns (pixels, stats)."""
         w, h = job.view.width, job.view.height
         out = np.zeros((h, w, 4), dtype=np.uint8)

@@ -7,6 +7,9 @@ MAX_VARS = 16
 MAX_VAR_PARAMS = 4
 MIN_PALETTE = 256
 MAX_PALETTE = 4096
+MAX_STOPS = 1024
+SCENE_NAME = 128
+RENDER_3D_AS_2D = 1
 
 OK = 0
 ERR_INVALID_ARGUMENT = -1
@@ -17,6 +20,8 @@ ERR_PIXEL_READ_FAILED = -5
 ERR_NO_PALETTE = -6
 ERR_DEVICE = -7
 ERR_NO_FLAME = -8
+ERR_PARSE = -9
+ERR_UNSUPPORTED = -10
 
 SPLAT_U32_GLOBAL = 0
 SPLAT_U32_XCD = 1
@@ -104,6 +109,33 @@ class Job(C.Structure):
     ]
 
 
+class PaletteXf(C.Structure):
+    """ffm_palette_xf: the palette lookup-table transform."""
+    _fields_ = [("squeeze_mode", C.c_int32), ("squeeze_factor", C.c_float),
+                ("log_strength", C.c_float), ("rotation", C.c_float), ("reverse", C.c_int32)]
+
+
+class Scene(C.Structure):
+    """ffm_scene: one config (.fflame JSON or .flame XML) as this path reads it."""
+    _fields_ = [
+        ("flame", FlameDesc),
+        ("n_stops", C.c_int32),
+        ("stops", (C.c_float * 4) * MAX_STOPS),
+        ("palette_size", C.c_int32),
+        ("palette_xf", PaletteXf),
+        ("zoom", C.c_float), ("rotation", C.c_float),
+        ("pan_x", C.c_double), ("pan_y", C.c_double),
+        ("filter_radius", C.c_float), ("filter_blur_edges", C.c_float),
+        ("max_iterations", C.c_uint64),
+        ("reset_orbits", C.c_int32),
+        ("deterministic_rng", C.c_int32),
+        ("render_3d", C.c_int32),
+        ("image_width", C.c_uint32), ("image_height", C.c_uint32),
+        ("tonemap", TonemapParams),
+        ("name", C.c_char * SCENE_NAME),
+    ]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("total_samples", C.c_uint64), ("dispatches", C.c_uint64),
                 ("render_time_ms", C.c_double), ("iterate_ms", C.c_double),
@@ -175,6 +207,15 @@ SIGNATURES = [
     ("ffm_softsplat_sizes", C.c_int, [_vp, _u64p, _u64p]),
     ("ffm_softsplat_forward", C.c_int, [_vp, _vp, _vp, _vp]),
     ("ffm_softsplat_adjoint", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("ffm_scene_count", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.c_char_p,
+                                  C.c_size_t]),
+    ("ffm_scene_parse", C.c_int, [C.c_char_p, C.c_size_t, C.c_int, C.POINTER(Scene),
+                                  C.c_char_p, C.c_size_t]),
+    ("ffm_palette_set_xf", C.c_int, [_vp, _fp, C.c_int, C.c_int, C.POINTER(PaletteXf)]),
+    ("ffm_scene_job", C.c_int, [C.POINTER(Scene), C.c_int, C.c_int, C.c_uint64, C.c_uint32,
+                                C.c_uint32, C.POINTER(Job), C.c_char_p, C.c_size_t]),
+    ("ffm_render_config", C.c_int, [_vp, C.c_char_p, C.c_size_t, C.c_int, C.c_int, C.c_uint64,
+                                    C.c_uint32, C.c_uint32, _u8p, C.POINTER(RenderStats)]),
 ]
 
 

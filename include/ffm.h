@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 2
+#define FFM_ABI_VERSION 3
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -55,7 +55,9 @@ typedef enum {
   FFM_ERR_PIXEL_READ_FAILED = -5,
   FFM_ERR_NO_PALETTE = -6,
   FFM_ERR_DEVICE = -7,
-  FFM_ERR_NO_FLAME = -8
+  FFM_ERR_NO_FLAME = -8,
+  FFM_ERR_PARSE = -9,       /* a config that is not valid JSON / XML, or misses a required field */
+  FFM_ERR_UNSUPPORTED = -10 /* a config feature this path does not render */
 } ffm_status;
 
 /* Variation ids are the reference registry indices, which are append-only
@@ -363,6 +365,84 @@ int ffm_softsplat_forward(ffm_ctx* ctx, const float* theta, float* density, void
  * call is deterministic. */
 int ffm_softsplat_adjoint(ffm_ctx* ctx, const float* theta, const float* grad_density,
                           float* grad_theta, void* stream);
+
+/* ---- configs: .fflame / .fae JSON and .flame XML -------------------------- */
+
+/* The reference's two on-disk forms, read into what this path renders:
+ * FractalConfig JSON (src/config/fractal_config.rs:178, migrated by version
+ * as FractalConfig::from_json does, :1746-2050; Transform and Flame serde,
+ * src/scene/transforms.rs:1307-1560,2567-2690) and Apophysis / JWildfire
+ * XML (parse_flame_xml, src/flame_xml.rs:41-560,600-1067). A config that
+ * asks for something this path does not render (a variation outside
+ * ffm_variation, xaos, linked transforms, subflames, post symmetry, a
+ * non-identity tone curve, effects, escape-time or simulation modes, a 3D
+ * camera) is refused with FFM_ERR_UNSUPPORTED and a message naming it,
+ * never rendered approximately. */
+#define FFM_MAX_STOPS 1024
+#define FFM_SCENE_NAME 128
+
+/* PaletteTransform (src/scene/palette.rs:511-600): the palette lookup table
+ * is squeezed, log-redistributed, rotated and reversed, in that order
+ * (transform_lookup_table, palette.rs:703-800). All zero but
+ * squeeze_factor = 1 is the identity. */
+typedef struct {
+  int32_t squeeze_mode;  /* 0 linear, 1 geometric */
+  float squeeze_factor;  /* linear: repeat count; geometric: octave ratio r */
+  float log_strength;    /* exponential redistribution; 0 = off */
+  float rotation;        /* cyclic shift, fraction of the table (-1..1) */
+  int32_t reverse;
+} ffm_palette_xf;
+
+/* One config as this path reads it. */
+typedef struct {
+  ffm_flame_desc flame;
+  int32_t n_stops;
+  float stops[FFM_MAX_STOPS][4]; /* position, r, g, b (sRGB 0..1), in file order */
+  int32_t palette_size;          /* lookup-table texels, 256..4096 */
+  ffm_palette_xf palette_xf;
+  float zoom, rotation;
+  double pan_x, pan_y;
+  float filter_radius, filter_blur_edges;
+  uint64_t max_iterations;       /* the config's sample budget */
+  int32_t reset_orbits;          /* restart every orbit each dispatch */
+  int32_t deterministic_rng;     /* fixed seed schedule (else seeded from the clock) */
+  int32_t render_3d;             /* render mode 3d with a flat camera (every .flame import) */
+  uint32_t image_width, image_height; /* the saved size (`image_size`, XML `size`) */
+  ffm_tonemap_params tonemap;
+  char name[FFM_SCENE_NAME];     /* the flame's name, truncated, NUL-terminated */
+} ffm_scene;
+
+/* Number of configs in `text` (a JSON object, a JSON array of configs --
+ * FractalConfig::from_json_multi, fractal_config.rs:1944-1973 -- or an XML
+ * document's <flame> / <jwf-flame> elements). Errors go to err[err_cap]. */
+int ffm_scene_count(const char* text, size_t len, int* count, char* err, size_t err_cap);
+
+/* Parse config `index` of `text` into `out`. */
+int ffm_scene_parse(const char* text, size_t len, int index, ffm_scene* out, char* err,
+                    size_t err_cap);
+
+/* ffm_palette_set with a lookup-table transform applied (NULL: none). */
+int ffm_palette_set_xf(ffm_ctx* ctx, const float* stops_prgb, int n_stops, int table_size,
+                       const ffm_palette_xf* xf);
+
+/* A scene's 3D render mode (a flat camera) renders through the 2D chaos
+ * game only when the caller asks for it with this flag. */
+#define FFM_RENDER_3D_AS_2D 1u
+
+/* The job the reference's export builds for a config (RenderJob::new,
+ * src/renderer/render.rs:111-138, with pyfae's render arguments,
+ * python/pyfae/__init__.py:201-221): `iterations` 0 takes the config's
+ * max_iterations, `ipt` 0 the CLI's 1024. */
+int ffm_scene_job(const ffm_scene* scene, int width, int height, uint64_t iterations,
+                  uint32_t ipt, uint32_t flags, ffm_job* out, char* err, size_t err_cap);
+
+/* pyfae's Engine.render(config, width, height, iterations=,
+ * iterations_per_thread=) (python/pyfae/__init__.py:201-221; the engine's
+ * "render" request): parse config 0 of `text`, set its flame and palette,
+ * render, RGBA8 into rgba_out (host, width*height*4). */
+int ffm_render_config(ffm_ctx* ctx, const char* text, size_t len, int width, int height,
+                      uint64_t iterations, uint32_t ipt, uint32_t flags, uint8_t* rgba_out,
+                      ffm_render_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -359,6 +359,115 @@ void orc_palette_linear(const float* stops, int n, int size, float* lin) {
   }
 }
 
+/* transform_lookup_table (src/scene/palette.rs:703-800), stage by stage on
+ * a table of texel numbers: squeeze (linear repeats or geometric octaves,
+ * palette.rs:607-640), exponential redistribution (palette.rs:653-671),
+ * cyclic rotation, reverse. src[i] is the sampled texel entry i shows. */
+static size_t orc_as_index(float v, int size) { /* Rust `as usize`, clamped */
+  if (!(v > 0.0f)) return 0;
+  size_t k = v >= 4294967295.0f ? (size_t)size : (size_t)v;
+  return k < (size_t)size - 1 ? k : (size_t)size - 1;
+}
+
+static float orc_geometric(float t, float r) {
+  if (r >= 0.999f) return t;
+  if (r <= 0.001f) {
+    float q = t / fmaxf(r, 1e-6f);
+    return q < 0.0f ? 0.0f : (q > 1.0f ? 1.0f : q);
+  }
+  if (t <= 0.0f) return 0.0f;
+  float start = 0.0f, rn = 1.0f;
+  for (int n = 0; n < 64; n++) {
+    float w = rn * (1.0f - r);
+    float end = start + w;
+    if (t < end) {
+      if (w <= 0.0f) return 0.0f;
+      float q = (t - start) / w;
+      return q < 0.0f ? 0.0f : (q > 1.0f ? 1.0f : q);
+    }
+    start = end;
+    rn *= r;
+    if (rn < 1e-10f) return 1.0f;
+  }
+  return 1.0f;
+}
+
+static float orc_redistribute(float t, float s) {
+  if (fabsf(s) < 1.1920929e-7f) return t;
+  float q;
+  if (s > 0.0f) {
+    float d = expm1f(s);
+    if (d == 0.0f) return t;
+    q = expm1f(s * t) / d;
+  } else {
+    float d = expm1f(-s);
+    if (d == 0.0f) return t;
+    q = 1.0f - expm1f(-s * (1.0f - t)) / d;
+  }
+  return q < 0.0f ? 0.0f : (q > 1.0f ? 1.0f : q);
+}
+
+void orc_palette_index_map(int size, const ffm_palette_xf* xf, int32_t* src) {
+  int32_t* a = (int32_t*)malloc(sizeof(int32_t) * (size_t)size);
+  int32_t* b = (int32_t*)malloc(sizeof(int32_t) * (size_t)size);
+  for (int i = 0; i < size; i++) a[i] = i;
+  if (xf) {
+    /* stage 1: squeeze */
+    if (xf->squeeze_mode == 0 && xf->squeeze_factor != 1.0f) {
+      for (int i = 0; i < size; i++) {
+        float t = (float)i / (float)size;
+        float x = t * xf->squeeze_factor;
+        float st = x - truncf(x);
+        b[i] = a[orc_as_index(st * (float)size, size)];
+      }
+      memcpy(a, b, sizeof(int32_t) * (size_t)size);
+    } else if (xf->squeeze_mode == 1) {
+      for (int i = 0; i < size; i++) {
+        float st = orc_geometric((float)i / (float)size, xf->squeeze_factor);
+        b[i] = a[orc_as_index(st * (float)size, size)];
+      }
+      memcpy(a, b, sizeof(int32_t) * (size_t)size);
+    }
+    /* stage 2: logarithmic redistribution */
+    if (!(fabsf(xf->log_strength) < 1.1920929e-7f)) {
+      for (int i = 0; i < size; i++) {
+        float st = orc_redistribute((float)i / (float)size, xf->log_strength);
+        b[i] = a[orc_as_index(st * (float)size, size)];
+      }
+      memcpy(a, b, sizeof(int32_t) * (size_t)size);
+    }
+    /* stage 3: rotation by round(rotation * size) entries (rem_euclid) */
+    if (xf->rotation != 0.0f) {
+      int amount = (int)roundf(xf->rotation * (float)size);
+      for (int i = 0; i < size; i++) {
+        int k = (i + amount) % size;
+        b[i] = a[k < 0 ? k + size : k];
+      }
+      memcpy(a, b, sizeof(int32_t) * (size_t)size);
+    }
+    /* stage 4: reverse */
+    if (xf->reverse) {
+      for (int i = 0; i < size; i++) b[i] = a[size - 1 - i];
+      memcpy(a, b, sizeof(int32_t) * (size_t)size);
+    }
+  }
+  memcpy(src, a, sizeof(int32_t) * (size_t)size);
+  free(a);
+  free(b);
+}
+
+void orc_palette_linear_xf(const float* stops, int n, int size, const ffm_palette_xf* xf,
+                           float* lin) {
+  float* base = (float*)malloc(sizeof(float) * 3 * (size_t)size);
+  int32_t* src = (int32_t*)malloc(sizeof(int32_t) * (size_t)size);
+  orc_palette_linear(stops, n, size, base);
+  orc_palette_index_map(size, xf, src);
+  for (int i = 0; i < size; i++)
+    for (int k = 0; k < 3; k++) lin[3 * i + k] = base[3 * src[i] + k];
+  free(base);
+  free(src);
+}
+
 /* Nearest texel of a clamp-to-edge sampler at u = colour index. */
 int orc_palette_texel(float ci, int size) {
   float u = ci * (float)size;
@@ -1075,8 +1184,8 @@ void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
  * batch_size dispatches and once more for a partial batch, stop at the
  * target or after burn_in/ipt+4 empty dispatches; then tonemap. */
 int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int pal_size,
-               const ffm_job* job, float* accum_out, uint8_t* rgba_out, uint64_t* total_out,
-               int threads) {
+               const ffm_palette_xf* pal_xf, const ffm_job* job, float* accum_out,
+               uint8_t* rgba_out, uint64_t* total_out, int threads) {
   ffm_flame_desc* f = (ffm_flame_desc*)malloc(sizeof(ffm_flame_desc));
   memcpy(f, f_in, sizeof(*f));
   orc_flame_prepare(f);
@@ -1084,7 +1193,7 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
   int W = v->width, H = v->height;
   size_t np = (size_t)W * H;
   float* pal = (float*)malloc(sizeof(float) * 3 * (size_t)pal_size);
-  orc_palette_linear(stops, n_stops, pal_size, pal);
+  orc_palette_linear_xf(stops, n_stops, pal_size, pal_xf, pal);
   uint32_t* hist = (uint32_t*)calloc(np * 4, 4);
   uint32_t* scratch = (uint32_t*)calloc(np * 4, 4);
   float* acc = (float*)calloc(np * 4, 4);

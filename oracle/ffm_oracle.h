@@ -55,6 +55,11 @@ float orc_f16_to_f32(uint16_t h);
 /* Linear-light RGB per texel (f16 storage, then srgb_to_linear). */
 void orc_palette_linear(const float* stops_prgb, int n_stops, int size, float* lin_rgb);
 int orc_palette_texel(float color_index, int size);
+/* The palette lookup-table transform (src/scene/palette.rs:703-800) as the
+ * sampled texel each entry shows; NULL is the identity. */
+void orc_palette_index_map(int size, const ffm_palette_xf* xf, int32_t* src);
+void orc_palette_linear_xf(const float* stops_prgb, int n_stops, int size,
+                           const ffm_palette_xf* xf, float* lin_rgb);
 
 /* --- view (shaders/core/utilities.wgsl:417-445) --- */
 void orc_world_to_pixel(const ffm_view* v, float x, float y, int32_t* px, int32_t* py);
@@ -104,8 +109,8 @@ void orc_tonemap_defaults(ffm_tonemap_params* tp);
 
 /* --- whole render (src/renderer/render.rs:483-800) --- */
 int orc_render(const ffm_flame_desc* f, const float* stops_prgb, int n_stops, int pal_size,
-               const ffm_job* job, float* accum_out, uint8_t* rgba_out, uint64_t* total_out,
-               int threads);
+               const ffm_palette_xf* pal_xf, const ffm_job* job, float* accum_out,
+               uint8_t* rgba_out, uint64_t* total_out, int threads);
 
 /* --- soft splat, FP64 (ffm_soft.c; no reference counterpart) --- */
 /* Parameters of a fittable flame, n * (6 + v), or -1. */

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_binds():
     lib = pyfae.library()
-    assert lib.ffm_abi_version() == 2
+    assert lib.ffm_abi_version() == 3
     bound = {name for name, _, _ in _abi.SIGNATURES}
     assert bound == set(declared())
 
@@ -50,6 +50,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(ffm_xform), sizeof(ffm_flame_desc),
          sizeof(ffm_view), sizeof(ffm_tonemap_params), sizeof(ffm_job), sizeof(ffm_render_stats),
          offsetof(ffm_job, tonemap), offsetof(ffm_flame_desc, finals));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(ffm_palette_xf), sizeof(ffm_scene),
+         offsetof(ffm_scene, pan_x), offsetof(ffm_scene, tonemap), offsetof(ffm_scene, name));
   return 0;
 }
 """
@@ -62,7 +64,9 @@ int main(void) {
         got = list(map(int, subprocess.run([exe], capture_output=True, text=True).stdout.split()))
     want = [C.sizeof(_abi.Xform), C.sizeof(_abi.FlameDesc), C.sizeof(_abi.View),
             C.sizeof(_abi.TonemapParams), C.sizeof(_abi.Job), C.sizeof(_abi.RenderStats),
-            _abi.Job.tonemap.offset, _abi.FlameDesc.finals.offset]
+            _abi.Job.tonemap.offset, _abi.FlameDesc.finals.offset,
+            C.sizeof(_abi.PaletteXf), C.sizeof(_abi.Scene), _abi.Scene.pan_x.offset,
+            _abi.Scene.tonemap.offset, _abi.Scene.name.offset]
     assert got == want
 
 
