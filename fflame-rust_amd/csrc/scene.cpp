@@ -582,6 +582,11 @@ const VarInfo* var_by_name(const std::string& n) {
   return nullptr;
 }
 
+// Variations that leave the plane alone: `flatten` is a post-phase
+// z := 0 (src/variations/defs/depth3d.rs:50-77), whose 2D form returns its
+// input, so with x and y unchanged it draws nothing here.
+bool plane_noop(const std::string& n) { return n == "flatten"; }
+
 const VarInfo* var_by_id(int id) {
   for (const auto& v : kVars)
     if (v.id == id) return &v;
@@ -1162,7 +1167,8 @@ XfIn xml_xform(const XEvent& el, bool final_xf, std::vector<float>* chaos,
             }
         us = k.find('_', us + 1);
       }
-      if (!param && !xform_ignored(k) && t != 0.0f && unknown) unknown->push_back(k);
+      if (!param && !xform_ignored(k) && !plane_noop(k) && t != 0.0f && unknown)
+        unknown->push_back(k);
     }
   }
   for (auto& p : pending) x.set_param(p.first, p.second);
@@ -1441,7 +1447,7 @@ void build_scene(const SceneIn& s, ffm_scene* out) {
   std::vector<int> ids;
   auto absorb = [&](const XfIn& x, const char* what, size_t idx) {
     for (auto& v : x.vars)
-      if (!var_by_name(v.first) && v.second != 0.0f)
+      if (!var_by_name(v.first) && !plane_noop(v.first) && v.second != 0.0f)
         unsupported("variation `" + v.first + "` (" + what + " " + std::to_string(idx) + ")");
     for (auto& p : x.priorities)
       if (p.second != 0 && x.has_var(p.first) && var_by_name(p.first))
