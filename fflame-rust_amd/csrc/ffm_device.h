@@ -251,6 +251,7 @@ struct FlameView {
   const float* rows;         // LDS, (n_xforms + n_finals) * XF_STRIDE
   const int* ids;            // LDS, n_vars
   const FlameHeader* hdr;    // LDS copy of the header (final-chain slot lists), or null
+  const float* xcum;         // LDS xaos selection table (n_xforms^2), or null without xaos
   int n_xforms, n_finals, n_vars;
 };
 
@@ -327,13 +328,33 @@ enum StepFlags { ST_PLOT = 1, ST_RESPAWN = 2, ST_BURN = 4 };
 // cum is the sequential f32 sum. The opacity draw is always consumed; a draw
 // is at most 1 - 2^-24, so at opacity >= 1 it always passes and only
 // advances the stream.
-__device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool& should_plot) {
-  float u = rng_nextf(rng);
+__device__ __forceinline__ int select_const(const FlameView& fl, float u) {
   const int n = fl.n_xforms;
   float target = u * fl.rows[(n - 1) * XF_STRIDE + XF_CUM];
   int xi = n - 1;
   for (int i = n - 2; i >= 0; i--)
     if (target <= fl.rows[i * XF_STRIDE + XF_CUM]) xi = i;
+  return xi;
+}
+
+// With xaos the pick is select_transform_xaos (utilities.wgsl:107-140): the
+// same rule over row `prev` of the table of running sums of
+// weight[j] * xaos[prev][j], and the pick becomes the next `prev`.
+__device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool& should_plot,
+                                            int& prev) {
+  float u = rng_nextf(rng);
+  int xi;
+  if (fl.xcum) {
+    const int n = fl.n_xforms;
+    const float* xr = fl.xcum + prev * n;
+    float target = u * xr[n - 1];
+    xi = n - 1;
+    for (int i = n - 2; i >= 0; i--)
+      if (target <= xr[i]) xi = i;
+    prev = xi;
+  } else {
+    xi = select_const(fl, u);
+  }
   const float opacity = fl.rows[xi * XF_STRIDE + XF_OPACITY];
   should_plot = true;
   if (opacity >= 1.0f) rng_skip(rng);
@@ -396,13 +417,57 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
 // One whole iteration of main_template.wgsl's loop body (:193-1383).
 __device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& v, int pal_size,
                                           uint32_t burn_in, Rng& rng, float& x, float& y,
-                                          float& ci, uint32_t& fuse, int& xi_out, int32_t& px,
-                                          int32_t& py, int& texel) {
+                                          float& ci, uint32_t& fuse, int& prev, int& xi_out,
+                                          int32_t& px, int32_t& py, int& texel) {
   bool should_plot;
-  const int xi = choose_xform(fl, rng, should_plot);
+  const int xi = choose_xform(fl, rng, should_plot, prev);
   xi_out = xi;
   return step_xform(fl, v, pal_size, burn_in, rng, x, y, ci, fuse, xi, should_plot, px, py,
                     texel);
+}
+
+// Walker start or resume (main_template.wgsl:25-70,125-181). A start is
+// the point (two draws), colour index 0 and a full fuse; with xaos one more
+// draw picks the previous transform, unconditioned (select_transform_const),
+// always. A persistent orbit (`orbit` not null) resumes its point, colour,
+// fuse and, with xaos, previous transform (packed as fuse | prev << 24)
+// unless this dispatch starts every orbit (`fresh`) or refreshes this walker
+// ((walker + seed) % refresh_period == 0); an orbit that starts with xaos
+// takes a first pick stratified over the walkers instead: the Kronecker
+// point (walker * 2654435769 + seed) >> 8, over 2^24.
+__device__ __forceinline__ void walker_begin(const FlameView& fl, const IterArgs& a,
+                                             uint32_t walker, const float4* orbit, Rng& rng,
+                                             float& x, float& y, float& ci, uint32_t& fuse,
+                                             int& prev) {
+  rng = rng_init(walker, a.seed);
+  x = rng_nextf(rng) * 2.0f - 1.0f;
+  y = rng_nextf(rng) * 2.0f - 1.0f;
+  ci = 0.0f;
+  fuse = a.burn_in;
+  prev = 0;
+  if (fl.xcum) prev = select_const(fl, rng_nextf(rng));
+  if (!orbit) return;
+  const bool restart =
+      a.fresh || (a.refresh_period != 0u && (walker + a.seed) % a.refresh_period == 0u);
+  if (!restart) {
+    const float4 o = *orbit;
+    x = o.x;
+    y = o.y;
+    ci = o.z;
+    const uint32_t w = __float_as_uint(o.w);
+    fuse = fl.xcum ? (w & 0xFFFFFFu) : w;
+    if (fl.xcum) prev = min((int)(w >> 24), fl.n_xforms - 1);  // a foreign word stays in the table
+  } else if (fl.xcum) {
+    const uint32_t k = walker * 2654435769u + a.seed;
+    prev = select_const(fl, (float)(k >> 8) * (1.0f / 16777216.0f));
+  }
+}
+
+// The orbit word that walker_begin reads back.
+__device__ __forceinline__ float4 walker_orbit(const FlameView& fl, float x, float y, float ci,
+                                               uint32_t fuse, int prev) {
+  const uint32_t w = fl.xcum ? (fuse | (uint32_t)prev << 24) : fuse;
+  return make_float4(x, y, ci, __uint_as_float(w));
 }
 
 struct LdsFlame {
@@ -417,7 +482,7 @@ __device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_
                                                const float* pal_f32, int pal_size, uint32_t* lds) {
   const FlameHeader* hdr = reinterpret_cast<const FlameHeader*>(table);
   const int nx = hdr->n_xforms, nf = hdr->n_finals, nv = hdr->n_vars;
-  const int rows_f = (nx + nf) * XF_STRIDE;
+  const int rows_f = hdr->table_rows * XF_STRIDE;
   const int hdr_w = (int)(sizeof(FlameHeader) / 4);
   const uint32_t* src = reinterpret_cast<const uint32_t*>(table);
   for (int i = threadIdx.x; i < hdr_w + rows_f; i += blockDim.x) lds[i] = src[i];
@@ -431,6 +496,9 @@ __device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_
   l.fv.rows = reinterpret_cast<const float*>(lds + hdr_w);
   l.fv.ids = reinterpret_cast<const int*>(lds + 4);
   l.fv.hdr = reinterpret_cast<const FlameHeader*>(lds);
+  l.fv.xcum = __builtin_amdgcn_readfirstlane(hdr->xaos)
+                  ? l.fv.rows + __builtin_amdgcn_readfirstlane(hdr->xaos_off)
+                  : nullptr;
   l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
   l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
   l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);

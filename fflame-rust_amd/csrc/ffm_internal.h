@@ -49,9 +49,20 @@ struct FlameHeader {
   int32_t fin_uniform;
   int32_t fin_chain_n;
   int32_t fin_chain[FFM_MAX_FINALS];
+  // Rows the table holds after the header: the transforms' and finals'
+  // rows, then, with xaos, ceil(n_xforms^2 / XF_STRIDE) rows holding the
+  // xaos selection table at float xaos_off from the first row: for each
+  // previous transform i, the sequential f32 sums of weight[j] * xaos[i][j]
+  // (select_transform_xaos, shaders/core/utilities.wgsl:107-140).
+  int32_t table_rows;
+  int32_t xaos;
+  int32_t xaos_off;
 };
 
-// A flame table in device memory: header then (n_xforms + n_finals) rows.
+// Rows of the xaos selection table of a flame with n transforms.
+__host__ __device__ inline int xaos_rows(int n) { return (n * n + XF_STRIDE - 1) / XF_STRIDE; }
+
+// A flame table in device memory: header then n_rows rows.
 __host__ __device__ inline size_t flame_table_floats(int n_rows) {
   return sizeof(FlameHeader) / 4 + (size_t)n_rows * XF_STRIDE;
 }
@@ -66,7 +77,7 @@ struct ViewConst {
 // Arguments of one iterate dispatch.
 struct IterArgs {
   const float* flame;       // FlameHeader + rows
-  int32_t n_rows;           // n_xforms + n_finals (sizes the LDS copy)
+  int32_t n_rows;           // table rows after the header (sizes the LDS copy)
   const uint32_t* pal_u32;  // packed splat colour per texel: r | g<<8 | b<<16
   const float* pal_f32;     // linear colour per texel x3 (f32 splat mode)
   int32_t pal_size;
@@ -74,8 +85,10 @@ struct IterArgs {
   uint64_t walkers;
   uint32_t walker0;         // first walker id (batched launches)
   uint32_t ipt, burn_in, seed;
-  float4* orbits;           // x, y, colour index, fuse bits; may be null
+  float4* orbits;           // x, y, colour index, fuse bits (with xaos: fuse | previous
+                            // transform << 24); may be null
   int32_t fresh;
+  uint32_t refresh_period;  // walkers with (walker + seed) % period == 0 restart (0: none)
   void* hist;               // u32 or f32 words
   uint32_t hist_copy_words; // XCD mode: words per private copy
   unsigned long long* plotted;

@@ -41,23 +41,16 @@ __global__ __launch_bounds__(256) void iterate_kernel(IterArgs a) {
   for (uint64_t wi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; wi < a.walkers;
        wi += nthreads) {
     const uint32_t walker = a.walker0 + (uint32_t)wi;
-    Rng rng = rng_init(walker, a.seed);
-    // Start point: two draws, always (main_template.wgsl:25-56).
-    float x = rng_nextf(rng) * 2.0f - 1.0f;
-    float y = rng_nextf(rng) * 2.0f - 1.0f;
-    float ci = 0.0f;
-    uint32_t fuse = a.burn_in;
-    if (a.orbits && !a.fresh) {
-      float4 o = a.orbits[wi];
-      x = o.x;
-      y = o.y;
-      ci = o.z;
-      fuse = __float_as_uint(o.w);
-    }
+    Rng rng;
+    float x, y, ci;
+    uint32_t fuse;
+    int prev;
+    walker_begin(L.fv, a, walker, a.orbits ? a.orbits + wi : nullptr, rng, x, y, ci, fuse, prev);
     for (uint32_t it = 0; it < a.ipt; it++) {
       int xi, texel = 0;
       int32_t px, py;
-      int st = chaos_step(L.fv, v, pal_size, a.burn_in, rng, x, y, ci, fuse, xi, px, py, texel);
+      int st = chaos_step(L.fv, v, pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi, px, py,
+                          texel);
       n_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
       if (st & ST_PLOT) {
         n_in++;
@@ -79,7 +72,7 @@ __global__ __launch_bounds__(256) void iterate_kernel(IterArgs a) {
         }
       }
     }
-    if (a.orbits) a.orbits[wi] = make_float4(x, y, ci, __uint_as_float(fuse));
+    if (a.orbits) a.orbits[wi] = walker_orbit(L.fv, x, y, ci, fuse, prev);
   }
   wave_add_u64(a.plotted, n_plotted);
   wave_add_u64(a.inframe, n_in);
@@ -90,15 +83,16 @@ __global__ void trace_kernel(IterArgs a, uint32_t walker, uint32_t steps, uint32
   extern __shared__ uint32_t lds[];
   LdsFlame L = load_lds<false>(a, lds);
   if (threadIdx.x != 0) return;
-  Rng rng = rng_init(walker, a.seed);
-  float x = rng_nextf(rng) * 2.0f - 1.0f;
-  float y = rng_nextf(rng) * 2.0f - 1.0f;
-  float ci = 0.0f;
-  uint32_t fuse = a.burn_in;
+  Rng rng;
+  float x, y, ci;
+  uint32_t fuse;
+  int prev;
+  walker_begin(L.fv, a, walker, nullptr, rng, x, y, ci, fuse, prev);
   for (uint32_t s = 0; s < steps; s++) {
     int xi, texel = 0;
     int32_t px, py;
-    int st = chaos_step(L.fv, a.view, a.pal_size, a.burn_in, rng, x, y, ci, fuse, xi, px, py, texel);
+    int st = chaos_step(L.fv, a.view, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi, px,
+                        py, texel);
     uint32_t* o = out + 8 * (size_t)s;
     o[0] = rng.state;
     o[1] = (uint32_t)xi;

@@ -48,6 +48,7 @@ struct ffm_ctx {
   bool has_flame = false;
   ffm_flame_desc flame;
   int n_rows = 0;
+  uint32_t refresh_period = 0;  // xaos flames whose walk splits orbits: 4
   DevBuf table;
   // palette
   bool has_palette = false;
@@ -153,7 +154,65 @@ int check_flame(ffm_ctx* c, const ffm_flame_desc* f) {
                       "transform %d attaches final %d; the flame has %d", i, x.final_attach[j],
                       f->n_finals);
     }
+  if (f->has_xaos && f->n_xforms > FFM_MAX_XAOS)
+    return fail(c, FFM_ERR_TOO_LARGE, "xaos on %d transforms: at most %d", f->n_xforms,
+                FFM_MAX_XAOS);
   return FFM_OK;
+}
+
+// Flame::xaos_walk_splits_orbits (src/scene/xaos.rs:64-124): with xaos, the
+// transforms a first pick can reach, the closed groups among them (every
+// transform a member reaches reaches it back), and whether one first pick
+// can end in more than one group. Such a flame restarts one walker in
+// ORBIT_REFRESH_PERIOD = 4 each dispatch (src/renderer/compute_kernel.rs:
+// 46-60); otherwise 0 (no refresh).
+uint32_t xaos_refresh_period(const ffm_flame_desc* f) {
+  const int n = f->n_xforms;
+  if (!f->has_xaos || n == 0) return 0;
+  std::vector<float> w(n);
+  for (int i = 0; i < n; i++) w[i] = std::max(f->xforms[i].weight, 0.0f);
+  std::vector<std::vector<int>> next(n);
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++)
+      if (w[j] * f->xaos[i][j] > 0.0f) next[i].push_back(j);
+    if (next[i].empty()) next[i].push_back(0);
+  }
+  std::vector<std::vector<char>> reach(n, std::vector<char>(n, 0));
+  for (int s = 0; s < n; s++) {
+    std::vector<int> stack{s};
+    reach[s][s] = 1;
+    while (!stack.empty()) {
+      int i = stack.back();
+      stack.pop_back();
+      for (int j : next[i])
+        if (!reach[s][j]) {
+          reach[s][j] = 1;
+          stack.push_back(j);
+        }
+    }
+  }
+  std::vector<char> live(n, 0);
+  for (int i = 0; i < n; i++)
+    if (w[i] > 0.0f)
+      for (int j = 0; j < n; j++) live[j] |= reach[i][j];
+  std::vector<int> closed;
+  for (int i = 0; i < n; i++) {
+    if (!live[i]) continue;
+    bool all_back = true, lowest = true;
+    for (int j = 0; j < n; j++) {
+      if (reach[i][j] && !reach[j][i]) all_back = false;
+      if (j < i && reach[i][j] && reach[j][i]) lowest = false;
+    }
+    if (all_back && lowest) closed.push_back(i);
+  }
+  if (closed.size() < 2) return 0;
+  for (int i = 0; i < n; i++) {
+    if (!(w[i] > 0.0f)) continue;
+    int groups = 0;
+    for (int g : closed) groups += reach[i][g] ? 1 : 0;
+    if (groups > 1) return 4;
+  }
+  return 0;
 }
 
 // init_julian (src/variations/defs/advanced.rs:467-474), in place.
@@ -171,8 +230,12 @@ void prepare_flame(ffm_flame_desc* f) {
 // Pack the flame into the device table layout of ffm_internal.h.
 std::vector<float> pack_flame(const ffm_flame_desc* f) {
   const int rows = f->n_xforms + f->n_finals;
-  std::vector<float> t(flame_table_floats(rows), 0.0f);
+  const int table_rows = rows + (f->has_xaos ? xaos_rows(f->n_xforms) : 0);
+  std::vector<float> t(flame_table_floats(table_rows), 0.0f);
   FlameHeader h{};
+  h.table_rows = table_rows;
+  h.xaos = f->has_xaos ? 1 : 0;
+  h.xaos_off = rows * XF_STRIDE;
   h.n_xforms = f->n_xforms;
   h.n_finals = f->n_finals;
   h.n_vars = f->n_vars;
@@ -228,6 +291,20 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
       r[XF_W0 + k] = x.var_weight[k];
       for (int q = 0; q < FFM_MAX_VAR_PARAMS; q++)
         r[XF_P0 + k * FFM_MAX_VAR_PARAMS + q] = x.var_param[k][q];
+    }
+  }
+  if (f->has_xaos) {
+    // select_transform_xaos's running sums of weight[j] * xaos[i][j], one
+    // row per previous transform i (f32, in order, no contraction)
+    const int n = f->n_xforms;
+    float* xc = r0 + h.xaos_off;
+    for (int i = 0; i < n; i++) {
+      float acc = 0.0f;
+      for (int j = 0; j < n; j++) {
+        float prod = f->xforms[j].weight * f->xaos[i][j];
+        acc += prod;
+        xc[i * n + j] = acc;
+      }
     }
   }
   return t;
@@ -680,7 +757,8 @@ int ffm_flame_set(ffm_ctx* c, const ffm_flame_desc* f) {
   if ((st = ensure(c, c->table, t.size() * 4, "flame table"))) return st;
   HIPCHECK(c, hipMemcpyAsync(c->table.p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  c->n_rows = f->n_xforms + f->n_finals;
+  c->n_rows = reinterpret_cast<const FlameHeader*>(t.data())->table_rows;
+  c->refresh_period = xaos_refresh_period(&c->flame);
   c->has_flame = true;
   c->t_planned = false;  // the attractor, hence the tile capacities, changed
   return FFM_OK;
@@ -738,6 +816,10 @@ int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt
   a.seed = seed;
   a.orbits = reinterpret_cast<float4*>(orbits);
   a.fresh = fresh;
+  a.refresh_period = orbits ? c->refresh_period : 0u;
+  if (orbits && c->flame.has_xaos && burn_in >= (1u << 24))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "xaos orbits keep burn-in below 2^24 (got %u)",
+                burn_in);
   a.plotted = reinterpret_cast<unsigned long long*>(plotted);
   a.inframe = reinterpret_cast<unsigned long long*>(inframe);
   const size_t words = (size_t)view->width * view->height * 4;
@@ -882,6 +964,10 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   a.ipt = job->ipt;
   a.burn_in = job->burn_in;
   a.orbits = job->persistent_orbits ? reinterpret_cast<float4*>(c->orbits.p) : nullptr;
+  a.refresh_period = job->persistent_orbits ? c->refresh_period : 0u;
+  if (job->persistent_orbits && c->flame.has_xaos && job->burn_in >= (1u << 24))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "xaos orbits keep burn-in below 2^24 (got %u)",
+                job->burn_in);
   if (mode == FFM_SPLAT_U32_XCD) {
     a.hist = c->copies.p;
     a.hist_copy_words = (uint32_t)words;
@@ -919,7 +1005,8 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     HIPCHECK(c, hipEventRecord(c->ev1, s));
     uint64_t plotted;
     if (job->persistent_orbits) {
-      plotted = fuses.dispatch((uint32_t)job->walkers, job->ipt, job->burn_in, 0, a.seed);
+      plotted = fuses.dispatch((uint32_t)job->walkers, job->ipt, job->burn_in, a.refresh_period,
+                               a.seed);
     } else {
       plotted = job->walkers * (uint64_t)(job->ipt > job->burn_in ? job->ipt - job->burn_in : 0);
     }
@@ -1091,6 +1178,8 @@ int check_batch(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* co
   for (int i = 0; i < n; i++) {
     int st = check_flame(c, &flames[i]);
     if (st) return st;
+    if (flames[i].has_xaos)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame %d: xaos is not part of the batched path", i);
     if ((st = check_view(c, &jobs[i].view))) return st;
     if (!stops[i] || n_stops[i] < 1) return fail(c, FFM_ERR_NO_PALETTE, "flame %d: No palette found", i);
     if (!rgba_out[i]) return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame %d: pixel buffer is NULL", i);
@@ -1436,6 +1525,8 @@ int soft_params(ffm_ctx* c, const ffm_flame_desc* f) {
   if (!f) return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame is NULL");
   if (f->n_finals != 0)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "soft-splat flames have no final transforms");
+  if (f->has_xaos)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "soft-splat flames have no xaos");
   if (f->n_xforms < 1 || f->n_xforms > FFM_SOFT_MAX_XFORMS)
     return fail(c, FFM_ERR_TOO_LARGE, "%d transforms: a soft-splat flame has 1..%d", f->n_xforms,
                 FFM_SOFT_MAX_XFORMS);

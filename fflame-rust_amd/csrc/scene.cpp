@@ -634,7 +634,7 @@ struct SceneIn {
   std::string name = "Untitled";
   std::vector<XfIn> xforms, finals;
   size_t n_linked = 0;
-  bool xaos_nontrivial = false;
+  std::vector<std::vector<float>> xaos;  // Flame::xaos as written (rows may be short)
   bool solo = false;
   bool subflames = false;
   bool post_symmetry = false;
@@ -979,8 +979,8 @@ SceneIn json_config(const JVal& root) {
   if (const JVal* xa = flame->get("xaos"); xa && xa->k == JVal::ARR) {
     for (auto& row : xa->a) {
       if (row.k != JVal::ARR) raise(FFM_ERR_PARSE, "flame: xaos must be a matrix");
-      for (auto& w : row.a)
-        if (jnum_f32(&w, "xaos") != 1.0f) s.xaos_nontrivial = true;
+      s.xaos.emplace_back();
+      for (auto& w : row.a) s.xaos.back().push_back(jnum_f32(&w, "xaos"));
     }
   }
   if (const JVal* so = flame->get("solo_transform"); so && so->k != JVal::NUL) s.solo = true;
@@ -1347,10 +1347,15 @@ SceneIn xml_flame(XmlReader& rd, const XEvent& start) {
   }
   // xaos from the chaos rows (flame_xml.rs:360-380): 1.0 where unset
   size_t n = s.xforms.size();
-  for (size_t src = 0; src < n; src++)
-    if (chaos_has[src])
-      for (size_t dst = 0; dst < chaos_rows[src].size() && dst < n; dst++)
-        if (chaos_rows[src][dst] != 1.0f) s.xaos_nontrivial = true;
+  bool any_chaos = false;
+  for (size_t src = 0; src < n; src++) any_chaos |= chaos_has[src];
+  if (any_chaos) {
+    s.xaos.assign(n, std::vector<float>(n, 1.0f));
+    for (size_t src = 0; src < n; src++)
+      if (chaos_has[src])
+        for (size_t dst = 0; dst < chaos_rows[src].size() && dst < n; dst++)
+          s.xaos[src][dst] = chaos_rows[src][dst];
+  }
   // every <finalxform>, in file order, attached to every normal transform
   for (size_t k = 0; k < s.finals.size(); k++)
     for (auto& x : s.xforms) x.final_att.push_back((int64_t)k);
@@ -1425,7 +1430,6 @@ void build_scene(const SceneIn& s, ffm_scene* out) {
     if (s.preserve_z) unsupported("preserve_z (z carried across iterations)");
   }
   if (s.color_mode != "palette" && s.color_mode != "Palette") unsupported("color mode `" + s.color_mode + "`");
-  if (s.xaos_nontrivial) unsupported("xaos");
   if (s.solo) unsupported("solo transform");
   if (s.subflames) unsupported("subflames");
   if (s.post_symmetry) unsupported("post symmetry");
@@ -1500,6 +1504,20 @@ void build_scene(const SceneIn& s, ffm_scene* out) {
       }
     }
   };
+  // Flame::has_xaos (src/scene/xaos.rs:29-41): some weight differs from 1
+  // by more than 1e-6; get_xaos reads 1 past a short row
+  bool has_xaos = false;
+  for (auto& row : s.xaos)
+    for (float w : row) has_xaos |= std::fabs(w - 1.0f) > 1e-6f;
+  if (has_xaos) {
+    const size_t n = s.xforms.size();
+    if ((int)n > FFM_MAX_XAOS)
+      raise(FFM_ERR_TOO_LARGE, "xaos on %zu transforms: at most %d", n, FFM_MAX_XAOS);
+    fd.has_xaos = 1;
+    for (size_t i = 0; i < n; i++)
+      for (size_t j = 0; j < n; j++)
+        fd.xaos[i][j] = i < s.xaos.size() && j < s.xaos[i].size() ? s.xaos[i][j] : 1.0f;
+  }
   for (size_t i = 0; i < s.xforms.size(); i++) fill(fd.xforms[i], s.xforms[i]);
   for (size_t i = 0; i < s.finals.size(); i++) fill(fd.finals[i], s.finals[i]);
   if (!s.finals.empty()) {

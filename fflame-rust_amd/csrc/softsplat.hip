@@ -96,6 +96,7 @@ __device__ __forceinline__ FlameView soft_load(const SoftArgs& a, const SoftFit&
   fv.rows = rows;
   fv.ids = reinterpret_cast<const int*>(lds) + 4;
   fv.hdr = nullptr;  // no final transforms in a soft-splat flame
+  fv.xcum = nullptr;  // nor xaos
   fv.n_xforms = nx;
   fv.n_finals = 0;
   fv.n_vars = nv;

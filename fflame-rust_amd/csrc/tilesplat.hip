@@ -175,24 +175,19 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
     const uint64_t wi = base + rd * stride + tid;
     const bool active = wi < limit;
     const uint32_t walker = a.walker0 + (uint32_t)wi;
-    Rng rng = rng_init(walker, a.seed);
-    float x = rng_nextf(rng) * 2.0f - 1.0f;
-    float y = rng_nextf(rng) * 2.0f - 1.0f;
-    float ci = 0.0f;
-    uint32_t fuse = a.burn_in;
-    if (active && a.orbits && !a.fresh) {
-      float4 o = a.orbits[wi];
-      x = o.x;
-      y = o.y;
-      ci = o.z;
-      fuse = __float_as_uint(o.w);
-    }
+    Rng rng;
+    float x, y, ci;
+    uint32_t fuse;
+    int prev;
+    walker_begin(F.fv, a, walker, active && a.orbits ? a.orbits + wi : nullptr, rng, x, y, ci,
+                 fuse, prev);
     uint32_t r_plotted = 0u, r_in = 0u;  // this round's counts (at most ipt each)
     for (uint32_t it = 0; it < a.ipt; it++) {
       if (active) {
         int xi, texel = 0;
         int32_t px, py;
-        int st = chaos_step(F.fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, xi, px, py, texel);
+        int st = chaos_step(F.fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi, px,
+                            py, texel);
         r_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
         if (st & ST_PLOT) {
           r_in++;
@@ -226,8 +221,7 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
     }
     n_plotted += r_plotted;
     n_in += r_in;
-    if (!COUNT_ONLY && active && a.orbits)
-      a.orbits[wi] = make_float4(x, y, ci, __uint_as_float(fuse));
+    if (!COUNT_ONLY && active && a.orbits) a.orbits[wi] = walker_orbit(F.fv, x, y, ci, fuse, prev);
   }
 }
 

@@ -77,10 +77,12 @@ def xform(a=1.0, b=0.0, c=0.0, d=1.0, e=0.0, f=0.0, *, weight=1.0, color=0.0, co
             else None}
 
 
-def flame_desc(transforms, finals=()):
+def flame_desc(transforms, finals=(), xaos=None):
     """Pack transforms (dicts from `xform`) into an ffm_flame_desc. Active
     variations get local indices in first-appearance order over the
-    transforms, then the finals (src/scene/transforms.rs:30-85)."""
+    transforms, then the finals (src/scene/transforms.rs:30-85). `xaos`
+    (n x n, row = the transform applied last) scales the next pick's weights
+    (Flame::xaos, src/scene/transforms.rs:2040-2044)."""
     if len(transforms) > _abi.MAX_XFORMS:
         raise EngineError(_abi.ERR_TOO_LARGE,
                           f"{len(transforms)} transforms: at most {_abi.MAX_XFORMS} fit")
@@ -132,6 +134,17 @@ def flame_desc(transforms, finals=()):
             fd.xforms[i].n_final_attach = len(att)
             for j, k in enumerate(att):
                 fd.xforms[i].final_attach[j] = int(k)
+    if xaos is not None:
+        m = np.asarray(xaos, dtype=np.float32)
+        n = len(transforms)
+        if m.shape != (n, n):
+            raise EngineError(_abi.ERR_INVALID_ARGUMENT, f"xaos must be {n} x {n}")
+        if n > _abi.MAX_XAOS:
+            raise EngineError(_abi.ERR_TOO_LARGE, f"xaos on {n} transforms: at most {_abi.MAX_XAOS}")
+        fd.has_xaos = 1
+        for i in range(n):
+            for j in range(n):
+                fd.xaos[i][j] = float(m[i, j])
     return fd
 
 

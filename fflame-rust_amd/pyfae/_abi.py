@@ -5,6 +5,7 @@ MAX_XFORMS = 128
 MAX_FINALS = 4
 MAX_VARS = 16
 MAX_VAR_PARAMS = 4
+MAX_XAOS = 32
 MIN_PALETTE = 256
 MAX_PALETTE = 4096
 MAX_STOPS = 1024
@@ -64,6 +65,8 @@ class FlameDesc(C.Structure):
         ("xforms", Xform * MAX_XFORMS),
         ("finals", Xform * MAX_FINALS),
         ("per_transform_finals", C.c_int32),
+        ("has_xaos", C.c_int32),
+        ("xaos", (C.c_float * MAX_XAOS) * MAX_XAOS),
     ]
 
 

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 3
+#define FFM_ABI_VERSION 4
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -40,6 +40,7 @@ extern "C" {
 #define FFM_MAX_FINALS 4
 #define FFM_MAX_VARS 16       /* distinct variations active in one flame */
 #define FFM_MAX_VAR_PARAMS 4  /* user params + derived slots per variation */
+#define FFM_MAX_XAOS 32       /* transforms of a flame with xaos weights */
 #define FFM_MIN_PALETTE 256
 #define FFM_MAX_PALETTE 4096
 
@@ -118,6 +119,13 @@ typedef struct {
    * legacy final (migrate_legacy_final, src/scene/transforms.rs:2257-2266)
    * and .flame imports; 1: each applies its own final_attach list. */
   int32_t per_transform_finals;
+  /* Xaos (Flame::xaos, src/scene/transforms.rs:2040-2044): when has_xaos,
+   * the transform after transform i is drawn with weights
+   * weight[j] * xaos[i][j] (select_transform_xaos,
+   * shaders/core/utilities.wgsl:107-140), for flames of at most
+   * FFM_MAX_XAOS transforms. */
+  int32_t has_xaos;
+  float xaos[FFM_MAX_XAOS][FFM_MAX_XAOS];
 } ffm_flame_desc;
 
 /* The view: FractalConfig zoom/pan/rotation as world_to_pixel reads them

@@ -83,6 +83,78 @@ int orc_select(const ffm_flame_desc* f, float rand_val) {
   return f->n_xforms - 1;
 }
 
+/* shaders/core/utilities.wgsl:107-140 select_transform_xaos: the total and
+ * the running sum of weight[i] * xaos[prev][i], each product rounded, in
+ * order; first i with rand * total <= cumulative, else the last. */
+int orc_select_xaos(const ffm_flame_desc* f, float rand_val, int prev) {
+  float total = 0.0f;
+  for (int i = 0; i < f->n_xforms; i++) total += f->xforms[i].weight * f->xaos[prev][i];
+  float threshold = rand_val * total;
+  float cum = 0.0f;
+  for (int i = 0; i < f->n_xforms; i++) {
+    cum += f->xforms[i].weight * f->xaos[prev][i];
+    if (threshold <= cum) return i;
+  }
+  return f->n_xforms - 1;
+}
+
+/* src/scene/xaos.rs:64-124: transitions i -> j where weight_j * xaos[i][j]
+ * > 0 (none: to 0); reachability by depth-first search; the live set
+ * reachable from a first pick (weight > 0); closed groups (members reach
+ * only what reaches them back), each named by its lowest member; a split
+ * when some first pick reaches two groups. */
+uint32_t orc_xaos_refresh_period(const ffm_flame_desc* f) {
+  int n = f->n_xforms;
+  if (!f->has_xaos || n <= 0) return 0;
+  char reach[FFM_MAX_XAOS][FFM_MAX_XAOS];
+  memset(reach, 0, sizeof reach);
+  for (int s0 = 0; s0 < n; s0++) {
+    int stack[FFM_MAX_XAOS * FFM_MAX_XAOS], top = 0;
+    stack[top++] = s0;
+    reach[s0][s0] = 1;
+    while (top) {
+      int i = stack[--top];
+      int any = 0;
+      for (int j = 0; j < n; j++) {
+        float wj = f->xforms[j].weight > 0.0f ? f->xforms[j].weight : 0.0f;
+        if (wj * f->xaos[i][j] > 0.0f) {
+          any = 1;
+          if (!reach[s0][j]) {
+            reach[s0][j] = 1;
+            stack[top++] = j;
+          }
+        }
+      }
+      if (!any && !reach[s0][0]) {
+        reach[s0][0] = 1;
+        stack[top++] = 0;
+      }
+    }
+  }
+  char live[FFM_MAX_XAOS] = {0};
+  for (int i = 0; i < n; i++)
+    if (f->xforms[i].weight > 0.0f)
+      for (int j = 0; j < n; j++) live[j] |= reach[i][j];
+  int closed[FFM_MAX_XAOS], nc = 0;
+  for (int i = 0; i < n; i++) {
+    if (!live[i]) continue;
+    int ok = 1;
+    for (int j = 0; j < n && ok; j++) {
+      if (reach[i][j] && !reach[j][i]) ok = 0;
+      if (j < i && reach[i][j] && reach[j][i]) ok = 0;
+    }
+    if (ok) closed[nc++] = i;
+  }
+  if (nc < 2) return 0;
+  for (int i = 0; i < n; i++) {
+    if (!(f->xforms[i].weight > 0.0f)) continue;
+    int hit = 0;
+    for (int g = 0; g < nc; g++) hit += reach[i][closed[g]];
+    if (hit > 1) return 4;
+  }
+  return 0;
+}
+
 /* ----------------------------------------------------------- variations */
 
 static const float PI_F = F32(3.14159265359);  /* the WGSL literal */
@@ -518,7 +590,13 @@ static int step(const step_ctx* c, orc_walker* w, orc_rng* rng, int32_t* ppx, in
                 int* xf_out, uint32_t rgb[3]) {
   const ffm_flame_desc* f = c->f;
   float rand_val = orc_rng_nextf(rng);
-  int xi = orc_select(f, rand_val);
+  int xi;
+  if (f->has_xaos) {
+    xi = orc_select_xaos(f, rand_val, (int)w->prev);
+    w->prev = (uint32_t)xi;
+  } else {
+    xi = orc_select(f, rand_val);
+  }
   *xf_out = xi;
   const ffm_xform* xf = &f->xforms[xi];
   int should_plot = orc_rng_nextf(rng) < xf->opacity;
@@ -569,7 +647,8 @@ static int step(const step_ctx* c, orc_walker* w, orc_rng* rng, int32_t* ppx, in
   return 0;
 }
 
-static void walker_start(orc_walker* w, orc_rng* rng, uint32_t burn_in) {
+static void walker_start(const ffm_flame_desc* f, orc_walker* w, orc_rng* rng,
+                         uint32_t burn_in) {
   /* main_template.wgsl:25-56: two draws for the start point, always */
   float x = orc_rng_nextf(rng) * 2.0f - 1.0f;
   float y = orc_rng_nextf(rng) * 2.0f - 1.0f;
@@ -577,12 +656,30 @@ static void walker_start(orc_walker* w, orc_rng* rng, uint32_t burn_in) {
   w->y = y;
   w->color_index = 0.0f;
   w->fuse = burn_in;
+  /* main_template.wgsl:63-71: with xaos, an unconditioned pick seeds the
+   * previous transform (one more draw) */
+  w->prev = f->has_xaos ? (uint32_t)orc_select(f, orc_rng_nextf(rng)) : 0u;
+}
+
+/* main_template.wgsl:125-181: a persistent orbit resumes unless the
+ * dispatch starts it (fresh, or the refresh period picks it); one that
+ * starts with xaos picks its first transform at the Kronecker point
+ * (thread * 2654435769 + seed) >> 8, over 2^24. */
+static void walker_resume(const ffm_flame_desc* f, orc_walker* w, const orc_walker* stored,
+                          uint32_t tid, uint32_t seed, int fresh, uint32_t period) {
+  int restart = fresh || (period != 0 && (tid + seed) % period == 0);
+  if (!restart) {
+    *w = *stored;
+  } else if (f->has_xaos) {
+    uint32_t u = tid * 2654435769u + seed;
+    w->prev = (uint32_t)orc_select(f, (float)(u >> 8u) / 16777216.0f);
+  }
 }
 
 uint64_t orc_iterate(const ffm_flame_desc* f, const float* pal, int pal_size, const ffm_view* v,
                      uint32_t walker0, uint32_t n, uint32_t ipt, uint32_t burn_in,
-                     uint32_t seed, orc_walker* orbits, int fresh, uint32_t* hist,
-                     uint64_t* inframe) {
+                     uint32_t seed, orc_walker* orbits, int fresh, uint32_t period,
+                     uint32_t* hist, uint64_t* inframe) {
   step_ctx c = {f, pal, pal_size, v, burn_in};
   uint64_t plotted = 0, in = 0;
   for (uint32_t i = 0; i < n; i++) {
@@ -590,8 +687,8 @@ uint64_t orc_iterate(const ffm_flame_desc* f, const float* pal, int pal_size, co
     orc_rng rng;
     orc_rng_init(&rng, tid, seed);
     orc_walker w;
-    walker_start(&w, &rng, burn_in);
-    if (orbits && !fresh) w = orbits[i];
+    walker_start(f, &w, &rng, burn_in);
+    if (orbits) walker_resume(f, &w, &orbits[i], tid, seed, fresh, period);
     for (uint32_t it = 0; it < ipt; it++) {
       int32_t px, py;
       int xi;
@@ -624,20 +721,21 @@ typedef struct {
   uint32_t* hist;
   orc_walker* orbits;
   int fresh;
+  uint32_t period;
   uint64_t plotted;
 } mt_job;
 
 static void* mt_run(void* p) {
   mt_job* j = (mt_job*)p;
   j->plotted = orc_iterate(j->f, j->pal, j->pal_size, j->v, j->w0, j->n, j->ipt, j->burn_in,
-                           j->seed, j->orbits, j->fresh, j->hist, NULL);
+                           j->seed, j->orbits, j->fresh, j->period, j->hist, NULL);
   return NULL;
 }
 
 static uint64_t iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
                            const ffm_view* v, uint32_t n, uint32_t ipt, uint32_t burn_in,
-                           uint32_t seed, orc_walker* orbits, int fresh, uint32_t* hist,
-                           int threads) {
+                           uint32_t seed, orc_walker* orbits, int fresh, uint32_t period,
+                           uint32_t* hist, int threads) {
   if (threads < 1) threads = 1;
   size_t words = (size_t)v->width * (size_t)v->height * 4;
   mt_job* jobs = (mt_job*)calloc((size_t)threads, sizeof(mt_job));
@@ -646,7 +744,7 @@ static uint64_t iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_si
   for (int t = 0; t < threads; t++) {
     uint32_t cnt = per + ((uint32_t)t < rem ? 1u : 0u);
     jobs[t] = (mt_job){f, pal, pal_size, v, w0, cnt, ipt, burn_in, seed, NULL,
-                       orbits ? orbits + w0 : NULL, fresh, 0};
+                       orbits ? orbits + w0 : NULL, fresh, period, 0};
     jobs[t].hist = t == 0 ? hist : (uint32_t*)calloc(words, 4);
     w0 += cnt;
     pthread_create(&th[t], NULL, mt_run, &jobs[t]);
@@ -668,7 +766,7 @@ static uint64_t iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_si
 uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal, int pal_size,
                         const ffm_view* v, uint32_t n, uint32_t ipt, uint32_t burn_in,
                         uint32_t seed, uint32_t* hist, int threads) {
-  return iterate_mt(f, pal, pal_size, v, n, ipt, burn_in, seed, NULL, 1, hist, threads);
+  return iterate_mt(f, pal, pal_size, v, n, ipt, burn_in, seed, NULL, 1, 0, hist, threads);
 }
 
 void orc_trace(const ffm_flame_desc* f, const float* pal, int pal_size, const ffm_view* v,
@@ -677,7 +775,7 @@ void orc_trace(const ffm_flame_desc* f, const float* pal, int pal_size, const ff
   orc_rng rng;
   orc_rng_init(&rng, walker, seed);
   orc_walker w;
-  walker_start(&w, &rng, burn_in);
+  walker_start(f, &w, &rng, burn_in);
   for (uint32_t s = 0; s < steps; s++) {
     int32_t px, py;
     int xi;
@@ -1201,6 +1299,7 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
   uint32_t nw = (uint32_t)job->walkers;
   orc_walker* orbits = job->persistent_orbits ? (orc_walker*)calloc(nw, sizeof(orc_walker)) : NULL;
   orc_orbit_fuses fuses = {NULL, 0};
+  const uint32_t period = orbits ? orc_xaos_refresh_period(f) : 0u;
   uint64_t total = 0, batch_samples = 0, samples_in_buffer = 0;
   uint32_t batch_count = 0, empty = 0, k = 0;
   uint32_t stall_limit = job->burn_in / (job->ipt ? job->ipt : 1) + 4;
@@ -1210,17 +1309,17 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
     if (batch_count == 0) memset(hist, 0, np * 16);
     if (threads > 1) {
       uint32_t* h2 = (uint32_t*)calloc(np * 4, 4);
-      iterate_mt(f, pal, pal_size, v, nw, job->ipt, job->burn_in, seed, orbits, k == 0, h2,
-                 threads);
+      iterate_mt(f, pal, pal_size, v, nw, job->ipt, job->burn_in, seed, orbits, k == 0, period,
+                 h2, threads);
       for (size_t i = 0; i < np * 4; i++) hist[i] += h2[i];
       free(h2);
     } else {
       orc_iterate(f, pal, pal_size, v, 0, nw, job->ipt, job->burn_in, seed, orbits, k == 0,
-                  hist, NULL);
+                  period, hist, NULL);
     }
     uint64_t plotted;
     if (orbits) {
-      plotted = orc_orbit_fuses_dispatch(&fuses, nw, job->ipt, job->burn_in, 0, seed);
+      plotted = orc_orbit_fuses_dispatch(&fuses, nw, job->ipt, job->burn_in, period, seed);
     } else {
       uint32_t per = job->ipt > job->burn_in ? job->ipt - job->burn_in : 0;
       plotted = (uint64_t)nw * per;

@@ -27,6 +27,7 @@ typedef struct {
 typedef struct {
   float x, y, color_index;
   uint32_t fuse;
+  uint32_t prev; /* with xaos: the transform applied last */
 } orc_walker;
 
 /* --- RNG (shaders/core/rng.wgsl:22-55) --- */
@@ -39,6 +40,11 @@ void orc_rng_stream(uint32_t tid, uint32_t seed, uint32_t n, uint32_t* out);
 
 /* --- transform selection (shaders/core/utilities.wgsl:35-55) --- */
 int orc_select(const ffm_flame_desc* f, float rand_val);
+/* select_transform_xaos (shaders/core/utilities.wgsl:107-140) */
+int orc_select_xaos(const ffm_flame_desc* f, float rand_val, int prev);
+/* Flame::xaos_walk_splits_orbits (src/scene/xaos.rs:64-124) as the orbit
+ * refresh period (src/renderer/compute_kernel.rs:46-60): 4 or 0. */
+uint32_t orc_xaos_refresh_period(const ffm_flame_desc* f);
 
 /* --- one variation / one transform --- */
 void orc_variation(int var_id, const float* params, const ffm_xform* xf, float x, float y,
@@ -72,7 +78,7 @@ void orc_world_to_pixel(const ffm_view* v, float x, float y, int32_t* px, int32_
 uint64_t orc_iterate(const ffm_flame_desc* f, const float* pal_lin, int pal_size,
                      const ffm_view* v, uint32_t walker0, uint32_t n, uint32_t ipt,
                      uint32_t burn_in, uint32_t seed, orc_walker* orbits, int fresh,
-                     uint32_t* hist, uint64_t* inframe);
+                     uint32_t refresh_period, uint32_t* hist, uint64_t* inframe);
 /* Same, with several host threads each splatting into a private histogram
  * that is summed at the end (the CPU baseline). */
 uint64_t orc_iterate_mt(const ffm_flame_desc* f, const float* pal_lin, int pal_size,

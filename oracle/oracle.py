@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libffm_oracle.so")
 
 class Walker(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("color_index", C.c_float),
-                ("fuse", C.c_uint32)]
+                ("fuse", C.c_uint32), ("prev", C.c_uint32)]
 
 
 class Rng(C.Structure):
@@ -55,6 +55,8 @@ def lib():
             "orc_rng_nextf": (C.c_float, [C.POINTER(Rng)]),
             "orc_rng_stream": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _u32p]),
             "orc_select": (C.c_int, [C.POINTER(FlameDesc), C.c_float]),
+            "orc_select_xaos": (C.c_int, [C.POINTER(FlameDesc), C.c_float, C.c_int]),
+            "orc_xaos_refresh_period": (C.c_uint32, [C.POINTER(FlameDesc)]),
             "orc_variation": (None, [C.c_int, _fp, C.POINTER(_abi.Xform), C.c_float, C.c_float,
                                      C.POINTER(Rng), _fp, _fp]),
             "orc_apply_xform": (None, [C.POINTER(FlameDesc), C.POINTER(_abi.Xform), _fp, _fp,
@@ -69,7 +71,8 @@ def lib():
                                           C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
             "orc_iterate": (C.c_uint64, [C.POINTER(FlameDesc), _fp, C.c_int, C.POINTER(View),
                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                                         C.c_uint32, C.POINTER(Walker), C.c_int, _u32p, _u64p]),
+                                         C.c_uint32, C.POINTER(Walker), C.c_int, C.c_uint32,
+                                         _u32p, _u64p]),
             "orc_iterate_mt": (C.c_uint64, [C.POINTER(FlameDesc), _fp, C.c_int, C.POINTER(View),
                                             C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u32p,
                                             C.c_int]),
@@ -139,17 +142,26 @@ def palette_linear(stops, size=256):
 
 
 def iterate(fd, stops, view, walkers, ipt, burn_in, seed, *, walker0=0, pal_size=256,
-            orbits=None, fresh=True):
-    """(hist u32 (H, W, 4), plotted, inframe) of one dispatch."""
+            orbits=None, fresh=True, refresh_period=0):
+    """(hist u32 (H, W, 4), plotted, inframe) of one dispatch. `orbits` is a
+    Walker array (persistent orbits) or None."""
     f2 = prepared(fd)
     pal = palette_linear(stops, pal_size)
     hist = np.zeros((view.height, view.width, 4), dtype=np.uint32)
     infr = C.c_uint64(0)
-    orb = orbits.ctypes.data_as(C.POINTER(Walker)) if orbits is not None else None
+    orb = C.cast(orbits, C.POINTER(Walker)) if orbits is not None else None
     plotted = lib().orc_iterate(C.byref(f2), fp(pal), pal_size, C.byref(view), walker0, walkers,
                                 ipt, burn_in, seed & 0xFFFFFFFF, orb, int(bool(fresh)),
-                                u32p(hist), C.byref(infr))
+                                refresh_period, u32p(hist), C.byref(infr))
     return hist, plotted, infr.value
+
+
+def select_xaos(fd, u, prev):
+    return lib().orc_select_xaos(C.byref(fd), u, prev)
+
+
+def xaos_refresh_period(fd):
+    return lib().orc_xaos_refresh_period(C.byref(fd))
 
 
 def iterate_mt(fd, stops, view, walkers, ipt, burn_in, seed, threads, pal_size=256):
@@ -186,6 +198,8 @@ def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=25
     w.y = float(np.float32(L.orc_rng_nextf(C.byref(r))) * two - one)
     w.color_index = 0.0
     w.fuse = burn_in
+    if fd.has_xaos:  # the unconditioned first pick draws once more
+        w.prev = L.orc_select(C.byref(f2), L.orc_rng_nextf(C.byref(r)))
     fuse = burn_in
     out = np.zeros((len(trace_rows), 8), dtype=np.uint32)
     rec = (C.c_uint32 * 8)()
@@ -201,6 +215,7 @@ def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=25
         w.y = np.uint32(row[3]).view(np.float32).item()
         w.color_index = np.uint32(row[4]).view(np.float32).item()
         w.fuse = fuse
+        w.prev = int(row[1])  # with xaos, the pick conditions the next one
     return out
 
 

@@ -256,10 +256,21 @@ def test_unsupported_flame_features_are_refused():
     cfg["flame"]["transforms"][0]["variations"] = {"linear": 1.0, "spiral": 0.0}
     assert vars_of(pyfae.load(cfg)) == ["linear"]  # a weight of 0 draws nothing
     cfg = minimal()
+    tr = cfg["flame"]["transforms"]
+    tr.append(dict(tr[0], a=0.3))
+    tr.append(dict(tr[0], a=0.2))
+    # Flame::has_xaos: some weight differs from 1 by more than 1e-6; short
+    # rows read 1 (get_xaos)
+    cfg["flame"]["xaos"] = [[1.0, 1.0000001], [1.0]]
+    assert pyfae.load(cfg).flame.has_xaos == 0
+    cfg["flame"]["xaos"] = [[0.5, 0.0], [1.0], [2.0, 3.0, 0.25, 7.0]]
+    fd = pyfae.load(cfg).flame
+    assert fd.has_xaos == 1
+    got = [[fd.xaos[i][j] for j in range(3)] for i in range(3)]
+    assert got == [[0.5, 0.0, 1.0], [1.0, 1.0, 1.0], [2.0, 3.0, 0.25]]
+    cfg["flame"]["transforms"] = [tr[0]] * 33
     cfg["flame"]["xaos"] = [[0.5]]
-    assert "xaos" in str(load_err(cfg))
-    cfg["flame"]["xaos"] = [[1.0]]
-    assert pyfae.load(cfg).flame.n_xforms == 1
+    assert load_err(cfg).status == _abi.ERR_TOO_LARGE
     cfg = minimal()
     cfg["flame"]["transforms"][0]["linked_attachments"] = [0]
     assert "linked" in str(load_err(cfg))
@@ -364,6 +375,20 @@ def test_xml_flame_mapping():
     np.testing.assert_array_equal(pyfae.scene_stops(second), F32([[0, 0, 0, 0], [1, 1, 1, 1]]))
 
 
+def test_xml_chaos_rows_become_xaos():
+    doc = """<flame><xform weight="1" color="0" coefs="1 0 0 1 0 0" linear="1" chaos="0 2"/>
+      <xform weight="1" color="0" coefs="1 0 0 1 0 0" linear="1"/>
+      <xform weight="1" color="0" coefs="1 0 0 1 0 0" linear="1" chaos="1 1 0.5 9"/></flame>"""
+    fd = pyfae.load(doc).flame
+    assert fd.has_xaos == 1
+    got = [[fd.xaos[i][j] for j in range(3)] for i in range(3)]
+    # rows without chaos stay 1; short rows are 1 past their end; extra
+    # entries past the transform count are dropped
+    assert got == [[0.0, 2.0, 1.0], [1.0, 1.0, 1.0], [1.0, 1.0, 0.5]]
+    plain = doc.replace('chaos="0 2"', 'chaos="1"').replace('chaos="1 1 0.5 9"', 'chaos="1 1 1"')
+    assert pyfae.load(plain).flame.has_xaos == 0
+
+
 def test_xml_jwf_layers_take_the_first():
     doc = """<jwf-flame name="L" scale="100" cam_zoom="2">
       <layer><xform weight="1" color="0" coefs="1 0 0 1 0 0" linear="1"/></layer>
@@ -377,7 +402,7 @@ def test_xml_jwf_layers_take_the_first():
     ('linear="1"', None),
     ('linear="1" spiral="0.5"', _abi.ERR_UNSUPPORTED),   # a variation this engine lacks
     ('linear="1" spiral="0"', None),                      # ... at weight 0 draws nothing
-    ('linear="1" chaos="0 1"', _abi.ERR_UNSUPPORTED),     # xaos
+    ('linear="1" chaos="0 1"', None),                     # xaos
     ('linear="1" chaos="1 1 1"', None),
     ('linear="1" mod_gamma="0.5" material="2"', None),    # JWildfire colouring, inert
     ('linear="1" linear_fx_priority="1"', _abi.ERR_UNSUPPORTED),
