@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -55,7 +56,7 @@ struct ffm_ctx {
   int pal_size = 0;
   DevBuf pal_u32, pal_f32;
   // render buffers
-  DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv;
+  DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv, count;
   // LDS tile splat: per-tile fill/region/capacity, work list, record buffer
   DevBuf t_fill, t_region, t_cap, t_items, t_records, t_sorted;
   int t_W = 0, t_H = 0, t_nrows = 0, t_pal = 0;
@@ -726,7 +727,7 @@ int ffm_ctx_destroy(ffm_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->table, &c->pal_u32, &c->pal_f32, &c->hist, &c->copies, &c->scratch,
                     &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
-                    &c->de_inv, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records, &c->t_sorted,
+                    &c->de_inv, &c->count, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records, &c->t_sorted,
                     &c->s_fits_d, &c->s_tables, &c->s_ckpt, &c->s_part, &c->s_xcd})
     b->release();
   free_batch(c->batch);
@@ -927,6 +928,20 @@ int ffm_tonemap(ffm_ctx* c, const float* accum, uint8_t* rgba, int width, int he
 // accumulate every batch_size dispatches (blur first when asked) and once
 // more for a partial batch, stop early after burn_in/ipt + 4 empty
 // dispatches; then the density estimator and the tonemap.
+// EXACT_COUNT_AFTER_PASSES (src/renderer/compute_kernel.rs:360-372): from
+// this accumulate pass on a render counts each pixel's density exactly in 64
+// bits (the f32 running sum drifts past ~10^6 passes). FFM_EXACT_COUNT_AFTER
+// in the environment overrides it, as FAE_EXACT_COUNT_AFTER does there.
+static uint64_t exact_count_after() {
+  const char* e = std::getenv("FFM_EXACT_COUNT_AFTER");
+  if (e && *e) {
+    char* end = nullptr;
+    unsigned long long v = std::strtoull(e, &end, 10);
+    if (end && *end == 0) return v;
+  }
+  return 1ull << 18;
+}
+
 static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float* accum_out,
                        ffm_render_stats* stats, uint8_t* rgba_dev = nullptr) {
   if (!c) return FFM_ERR_INVALID_ARGUMENT;
@@ -976,6 +991,9 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   }
   const int grid = iterate_grid(c, job->walkers);
   OrbitFuses fuses;
+  const uint64_t exact_after = exact_count_after();
+  uint64_t passes = 0;
+  uint32_t* exact = nullptr;  // the 64-bit counts, once the switch is made
   uint64_t total = 0, batch_samples = 0, samples_in_buffer = 0, k = 0;
   uint32_t batch_count = 0, empty = 0;
   const uint32_t stall_limit = job->burn_in / std::max(job->ipt, 1u) + 4;
@@ -1029,9 +1047,15 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
                                 reinterpret_cast<uint32_t*>(c->scratch.p), W, H, job->filter_radius,
                                 dsig, s));
       }
+      if (++passes >= exact_after && !exact) {
+        // counts start at zero and are seeded from the running sum there
+        if ((st = ensure(c, c->count, npix * 8, "exact density counts"))) return st;
+        HIPCHECK(c, launch_clear_u32(c->count.p, npix * 2, s));
+        exact = reinterpret_cast<uint32_t*>(c->count.p);
+      }
       HIPCHECK(c, launch_accumulate(c->hist.p, f32h,
                                     reinterpret_cast<float*>(c->accum.p),
-                                    reinterpret_cast<float*>(c->accum.p), nullptr, W, H, 0, 0.0f, s));
+                                    reinterpret_cast<float*>(c->accum.p), exact, W, H, 0, 0.0f, s));
       samples_in_buffer += batch_samples;
       batch_count = 0;
       batch_samples = 0;

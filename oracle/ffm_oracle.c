@@ -1282,8 +1282,8 @@ void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
  * batch_size dispatches and once more for a partial batch, stop at the
  * target or after burn_in/ipt+4 empty dispatches; then tonemap. */
 int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int pal_size,
-               const ffm_palette_xf* pal_xf, const ffm_job* job, float* accum_out,
-               uint8_t* rgba_out, uint64_t* total_out, int threads) {
+               const ffm_palette_xf* pal_xf, const ffm_job* job, uint64_t exact_after,
+               float* accum_out, uint8_t* rgba_out, uint64_t* total_out, int threads) {
   ffm_flame_desc* f = (ffm_flame_desc*)malloc(sizeof(ffm_flame_desc));
   memcpy(f, f_in, sizeof(*f));
   orc_flame_prepare(f);
@@ -1300,6 +1300,10 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
   orc_walker* orbits = job->persistent_orbits ? (orc_walker*)calloc(nw, sizeof(orc_walker)) : NULL;
   orc_orbit_fuses fuses = {NULL, 0};
   const uint32_t period = orbits ? orc_xaos_refresh_period(f) : 0u;
+  /* the exact-count switch (src/renderer/compute_kernel.rs:1562-1575):
+   * 64-bit counts from accumulate pass exact_after on */
+  uint32_t* count = NULL;
+  uint64_t passes = 0;
   uint64_t total = 0, batch_samples = 0, samples_in_buffer = 0;
   uint32_t batch_count = 0, empty = 0, k = 0;
   uint32_t stall_limit = job->burn_in / (job->ipt ? job->ipt : 1) + 4;
@@ -1336,7 +1340,8 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
         float dsig = orc_blur_density_sigma((double)batch_samples, W, H, job->filter_blur_edges);
         orc_blur(hist, scratch, W, H, job->filter_radius, dsig);
       }
-      orc_accumulate(hist, 0, acc, acc2, NULL, W, H, 0, 0.0f);
+      if (++passes >= exact_after && !count) count = (uint32_t*)calloc(np * 2, 4);
+      orc_accumulate(hist, 0, acc, acc2, count, W, H, 0, 0.0f);
       float* t = acc;
       acc = acc2;
       acc2 = t;
@@ -1357,6 +1362,7 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
   if (total_out) *total_out = total;
   orc_orbit_fuses_free(&fuses);
   free(orbits);
+  free(count);
   free(hist);
   free(scratch);
   free(acc);

@@ -115,8 +115,8 @@ void orc_tonemap_defaults(ffm_tonemap_params* tp);
 
 /* --- whole render (src/renderer/render.rs:483-800) --- */
 int orc_render(const ffm_flame_desc* f, const float* stops_prgb, int n_stops, int pal_size,
-               const ffm_palette_xf* pal_xf, const ffm_job* job, float* accum_out,
-               uint8_t* rgba_out, uint64_t* total_out, int threads);
+               const ffm_palette_xf* pal_xf, const ffm_job* job, uint64_t exact_after,
+               float* accum_out, uint8_t* rgba_out, uint64_t* total_out, int threads);
 
 /* --- soft splat, FP64 (ffm_soft.c; no reference counterpart) --- */
 /* Parameters of a fittable flame, n * (6 + v), or -1. */

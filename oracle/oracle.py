@@ -95,8 +95,8 @@ def lib():
             "orc_tonemap": (None, [_fp, _u8p, C.c_int, C.c_int, C.c_float, C.c_double,
                                    C.POINTER(TonemapParams)]),
             "orc_render": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.c_int,
-                                     C.POINTER(_abi.PaletteXf), C.POINTER(Job), _fp, _u8p, _u64p,
-                                     C.c_int]),
+                                     C.POINTER(_abi.PaletteXf), C.POINTER(Job), C.c_uint64, _fp,
+                                     _u8p, _u64p, C.c_int]),
             "orc_palette_index_map": (None, [C.c_int, C.POINTER(_abi.PaletteXf),
                                              C.POINTER(C.c_int32)]),
             "orc_palette_linear_xf": (None, [_fp, C.c_int, C.c_int, C.POINTER(_abi.PaletteXf),
@@ -219,16 +219,17 @@ def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=25
     return out
 
 
-def render(fd, stops, job, pal_size=256, threads=1, palette_xf=None):
-    """(rgba uint8 (H, W, 4), accum float32 (H, W, 4), total samples)."""
+def render(fd, stops, job, pal_size=256, threads=1, palette_xf=None, exact_after=1 << 18):
+    """(rgba uint8 (H, W, 4), accum float32 (H, W, 4), total samples).
+    `exact_after`: the accumulate pass from which densities count exactly."""
     w, h = job.view.width, job.view.height
     rgba = np.zeros((h, w, 4), dtype=np.uint8)
     acc = np.zeros((h, w, 4), dtype=np.float32)
     tot = C.c_uint64(0)
     s = np.ascontiguousarray(stops, dtype=np.float32)
     xf = C.byref(palette_xf) if palette_xf is not None else None
-    lib().orc_render(C.byref(fd), fp(s), s.shape[0], pal_size, xf, C.byref(job), fp(acc),
-                     rgba.ctypes.data_as(_u8p), C.byref(tot), threads)
+    lib().orc_render(C.byref(fd), fp(s), s.shape[0], pal_size, xf, C.byref(job), exact_after,
+                     fp(acc), rgba.ctypes.data_as(_u8p), C.byref(tot), threads)
     return rgba, acc, tot.value
 
 

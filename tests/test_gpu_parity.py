@@ -309,6 +309,23 @@ def test_plain_render_equals_oracle(engine):
     assert d.max() <= 1
 
 
+@pytest.mark.parametrize("after", [1, 3])
+def test_exact_count_switch_equals_oracle(engine, after, monkeypatch):
+    """The exact 64-bit density count from accumulate pass `after` on
+    (FFM_EXACT_COUNT_AFTER, as the reference's FAE_EXACT_COUNT_AFTER):
+    the accumulator equals the oracle's bit for bit (affine flame)."""
+    say(f"exact count switch after {after}")
+    monkeypatch.setenv("FFM_EXACT_COUNT_AFTER", str(after))
+    fd, pal, job = configs.plain(walkers=1 << 14, ipt=100)
+    job.view.width, job.view.height = 192, 128
+    job.target_samples = (1 << 14) * 80 * 4 * 6
+    set_flame(engine, fd, pal)
+    gacc, stats = engine.render_accum(job)
+    _, oacc, tot = O.render(fd, pal, job, threads=8, exact_after=after)
+    assert stats.total_samples == tot
+    np.testing.assert_array_equal(gacc, oacc)
+
+
 def block_sums(acc, b=16):
     h, w = acc.shape[:2]
     d = acc[: h // b * b, : w // b * b, 3].reshape(h // b, b, w // b, b).sum((1, 3))

@@ -546,3 +546,24 @@ def test_final_attachments_per_transform():
         assert (int(np.int32(row[5])), int(np.int32(row[6]))) == (px.value, py.value), xi
         seen.add(xi)
     assert seen == {0, 1, 2}
+
+
+# ------------------------------------------------ exact density count switch
+
+def test_render_exact_count_switch():
+    """From accumulate pass `exact_after` on, densities are counted in 64
+    bits (src/renderer/compute_kernel.rs:360-372,1562-1575): alpha is then
+    f32(count) / 100 of an integer count, seeded from the running sum when
+    the switch comes mid-render; before it, the f32 running sum."""
+    fd, pal, job = configs.plain(walkers=2048, ipt=100)
+    job.view.width, job.view.height = 96, 64
+    job.target_samples = 2048 * 80 * 4 * 6  # six accumulate passes
+    _, a_sum, t0 = O.render(fd, pal, job)
+    for after in (1, 3):
+        _, a_ex, t1 = O.render(fd, pal, job, exact_after=after)
+        assert t1 == t0
+        alpha = a_ex[..., 3]
+        cnt = np.round(alpha.astype(np.float64) * 100.0)
+        np.testing.assert_array_equal(np.float32(cnt) / np.float32(100), alpha)
+        np.testing.assert_allclose(alpha, a_sum[..., 3], rtol=2e-6, atol=0)
+        np.testing.assert_array_equal(a_ex[..., :3] == 0, a_sum[..., :3] == 0)
