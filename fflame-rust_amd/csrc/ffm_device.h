@@ -210,11 +210,72 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       oy = (y * re - x * im) * r;
       break;
     }
+    case FFM_VAR_BLOB: {
+      FFM_NO_SPECULATE();
+      float high = p[0], low = p[1], waves = p[2];
+      float r = ffm_sqrt(x * x + y * y);
+      float theta = ff_atan2(x, y);  // Apophysis's atan2(x, y)
+      float scale = r * (low + ((high - low) / 2.0f) * (ffm_sinf(waves * theta) + 1.0f));
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = scale * c;
+      oy = scale * s;
+      break;
+    }
+    case FFM_VAR_NOISE: {
+      FFM_NO_SPECULATE();
+      float theta = rng_nextf(rng) * TWO_PI_F;
+      float r = rng_nextf(rng);
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = x * r * c;
+      oy = y * r * s;
+      break;
+    }
+    case FFM_VAR_BLUR: {
+      float theta = rng_nextf(rng) * TWO_PI_F;
+      float r = rng_nextf(rng);
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = r * c;
+      oy = r * s;
+      break;
+    }
+    case FFM_VAR_GAUSSIAN_BLUR: {
+      float theta = rng_nextf(rng) * TWO_PI_F;
+      float d1 = rng_nextf(rng);
+      float d2 = rng_nextf(rng);
+      float d3 = rng_nextf(rng);
+      float d4 = rng_nextf(rng);
+      float r = d1 + d2 + d3 + d4 - 2.0f;
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = r * c;
+      oy = r * s;
+      break;
+    }
     default:
       ox = 0.0f;
       oy = 0.0f;
       break;
   }
+}
+
+// pre_blur (src/variations/defs/blur.rs PRE_BLUR), a pre-phase variation:
+// it moves the affine point that the normal variations then read, by its
+// own weight times (six draws summed - 3) at the angle 2 pi x a seventh
+// draw. The packed flame puts pre-phase slots first
+// (src/shader_builder_v2.rs:2596-2650 runs them before the weighted sum).
+constexpr float PRE_BLUR_TWO_PI = 6.28318530717959f;
+__device__ __forceinline__ void pre_blur(float w, float& x, float& y, Rng& rng) {
+  float g1 = rng_nextf(rng), g2 = rng_nextf(rng), g3 = rng_nextf(rng);
+  float g4 = rng_nextf(rng), g5 = rng_nextf(rng), g6 = rng_nextf(rng);
+  float rnd_g = w * (g1 + g2 + g3 + g4 + g5 + g6 - 3.0f);
+  float rnd_a = rng_nextf(rng) * PRE_BLUR_TWO_PI;
+  float s, c;
+  ffm_sincosf(rnd_a, &s, &c);
+  x = x + rnd_g * c;
+  y = y + rnd_g * s;
 }
 
 // Affine -> weighted variation sum in the flame's local-index order
@@ -230,6 +291,10 @@ __device__ __forceinline__ void apply_xform(const float* __restrict__ row, const
     int id = __builtin_amdgcn_readfirstlane(ids[k]);
     float w = row[XF_W0 + k];
     if (w != 0.0f) {
+      if (id == FFM_VAR_PRE_BLUR) {
+        pre_blur(w, ax, ay, rng);
+        continue;
+      }
       float vx, vy;
       eval_variation(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
       rx += w * vx;
@@ -274,6 +339,10 @@ __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x
     const int k = __builtin_amdgcn_readfirstlane(fl.hdr->fin_act[i][j]);
     const int id = __builtin_amdgcn_readfirstlane(fl.ids[k]);
     const float w = row[XF_W0 + k];
+    if (id == FFM_VAR_PRE_BLUR) {
+      pre_blur(w, ax, ay, rng);
+      continue;
+    }
     float vx, vy;
     eval_variation(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
     rx += w * vx;

@@ -123,7 +123,8 @@ bool supported_var(int id) {
     case FFM_VAR_LINEAR: case FFM_VAR_SINUSOIDAL: case FFM_VAR_SPHERICAL: case FFM_VAR_SWIRL:
     case FFM_VAR_HORSESHOE: case FFM_VAR_POLAR: case FFM_VAR_DISC: case FFM_VAR_HYPERBOLIC:
     case FFM_VAR_JULIA: case FFM_VAR_WAVES: case FFM_VAR_JULIAN: case FFM_VAR_BUBBLE:
-    case FFM_VAR_CURL:
+    case FFM_VAR_CURL: case FFM_VAR_BLOB: case FFM_VAR_NOISE: case FFM_VAR_BLUR:
+    case FFM_VAR_GAUSSIAN_BLUR: case FFM_VAR_PRE_BLUR:
       return true;
     default:
       return false;
@@ -240,13 +241,21 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
   h.n_xforms = f->n_xforms;
   h.n_finals = f->n_finals;
   h.n_vars = f->n_vars;
+  // Packed slot k holds the descriptor's slot src[k]: pre-phase variations
+  // first, then the rest, each group in local-index order, so one pass over
+  // the slots runs the pre phase before the weighted sum.
+  int src[FFM_MAX_VARS], ns = 0;
+  for (int k = 0; k < f->n_vars; k++)
+    if (f->var_id[k] == FFM_VAR_PRE_BLUR) src[ns++] = k;
+  for (int k = 0; k < f->n_vars; k++)
+    if (f->var_id[k] != FFM_VAR_PRE_BLUR) src[ns++] = k;
   for (int k = 0; k < f->n_vars; k++) {
-    h.var_id[k] = f->var_id[k];
-    h.var_mask |= 1 << (f->var_id[k] & 31);
+    h.var_id[k] = f->var_id[src[k]];
+    h.var_mask |= 1 << (h.var_id[k] & 31);
   }
   for (int i = 0; i < f->n_finals; i++)
     for (int k = 0; k < f->n_vars; k++)
-      if (f->finals[i].var_weight[k] != 0.0f) h.fin_act[i][h.fin_nact[i]++] = k;
+      if (f->finals[i].var_weight[src[k]] != 0.0f) h.fin_act[i][h.fin_nact[i]++] = k;
   // each normal transform's final chain (check_flame validated the lists)
   auto chain = [&](int i, int* out) {
     if (!f->per_transform_finals) {
@@ -289,9 +298,9 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
       for (int j = 0; j < n; j++) r[XF_FIN0 + j] = (float)c[j];
     }
     for (int k = 0; k < f->n_vars; k++) {
-      r[XF_W0 + k] = x.var_weight[k];
+      r[XF_W0 + k] = x.var_weight[src[k]];
       for (int q = 0; q < FFM_MAX_VAR_PARAMS; q++)
-        r[XF_P0 + k * FFM_MAX_VAR_PARAMS + q] = x.var_param[k][q];
+        r[XF_P0 + k * FFM_MAX_VAR_PARAMS + q] = x.var_param[src[k]][q];
     }
   }
   if (f->has_xaos) {
@@ -1563,6 +1572,11 @@ int soft_params(ffm_ctx* c, const ffm_flame_desc* f) {
     if (f->var_id[k] == FFM_VAR_WAVES)
       return fail(c, FFM_ERR_INVALID_ARGUMENT,
                   "waves reads the affine coefficients itself and has no soft-splat adjoint");
+    if (f->var_id[k] == FFM_VAR_BLOB || f->var_id[k] == FFM_VAR_NOISE ||
+        f->var_id[k] == FFM_VAR_BLUR || f->var_id[k] == FFM_VAR_GAUSSIAN_BLUR ||
+        f->var_id[k] == FFM_VAR_PRE_BLUR)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "variation id %d has no soft-splat adjoint",
+                  f->var_id[k]);
   }
   return f->n_xforms * (6 + f->n_vars);
 }

@@ -553,27 +553,34 @@ class XmlReader {
 struct VarInfo {
   const char* name;
   int id;                      // registry index (src/variations/defs/mod.rs:446-...)
-  const char* params[2];       // user parameters, slot order
-  float defaults[2];
+  const char* params[3];       // user parameters, slot order
+  float defaults[3];
   int n_params;
 };
 
 // The registry's own names and parameter defaults for the variations the
-// kernels implement (src/variations/defs/basic.rs, advanced.rs).
+// kernels implement (src/variations/defs/basic.rs, advanced.rs, blur.rs),
+// in registry order (the order a transform's variations take after its
+// variation_order).
 const VarInfo kVars[] = {
-    {"linear", FFM_VAR_LINEAR, {nullptr, nullptr}, {0, 0}, 0},
-    {"sinusoidal", FFM_VAR_SINUSOIDAL, {nullptr, nullptr}, {0, 0}, 0},
-    {"spherical", FFM_VAR_SPHERICAL, {nullptr, nullptr}, {0, 0}, 0},
-    {"swirl", FFM_VAR_SWIRL, {nullptr, nullptr}, {0, 0}, 0},
-    {"horseshoe", FFM_VAR_HORSESHOE, {nullptr, nullptr}, {0, 0}, 0},
-    {"polar", FFM_VAR_POLAR, {nullptr, nullptr}, {0, 0}, 0},
-    {"disc", FFM_VAR_DISC, {nullptr, nullptr}, {0, 0}, 0},
-    {"hyperbolic", FFM_VAR_HYPERBOLIC, {nullptr, nullptr}, {0, 0}, 0},
-    {"julia", FFM_VAR_JULIA, {nullptr, nullptr}, {0, 0}, 0},
-    {"waves", FFM_VAR_WAVES, {nullptr, nullptr}, {0, 0}, 0},
+    {"linear", FFM_VAR_LINEAR, {}, {}, 0},
+    {"sinusoidal", FFM_VAR_SINUSOIDAL, {}, {}, 0},
+    {"spherical", FFM_VAR_SPHERICAL, {}, {}, 0},
+    {"swirl", FFM_VAR_SWIRL, {}, {}, 0},
+    {"horseshoe", FFM_VAR_HORSESHOE, {}, {}, 0},
+    {"polar", FFM_VAR_POLAR, {}, {}, 0},
+    {"disc", FFM_VAR_DISC, {}, {}, 0},
+    {"hyperbolic", FFM_VAR_HYPERBOLIC, {}, {}, 0},
+    {"julia", FFM_VAR_JULIA, {}, {}, 0},
+    {"waves", FFM_VAR_WAVES, {}, {}, 0},
     {"julian", FFM_VAR_JULIAN, {"power", "dist"}, {2.0f, 1.0f}, 2},
-    {"bubble", FFM_VAR_BUBBLE, {nullptr, nullptr}, {0, 0}, 0},
+    {"blob", FFM_VAR_BLOB, {"high", "low", "waves"}, {1.0f, 1.0f, 6.0f}, 3},
+    {"bubble", FFM_VAR_BUBBLE, {}, {}, 0},
+    {"noise", FFM_VAR_NOISE, {}, {}, 0},
+    {"blur", FFM_VAR_BLUR, {}, {}, 0},
+    {"gaussian_blur", FFM_VAR_GAUSSIAN_BLUR, {}, {}, 0},
     {"curl", FFM_VAR_CURL, {"c1", "c2"}, {1.0f, 0.0f}, 2},
+    {"pre_blur", FFM_VAR_PRE_BLUR, {}, {}, 0},
 };
 
 const VarInfo* var_by_name(const std::string& n) {
@@ -1453,8 +1460,11 @@ void build_scene(const SceneIn& s, ffm_scene* out) {
     for (auto& v : x.vars)
       if (!var_by_name(v.first) && !plane_noop(v.first) && v.second != 0.0f)
         unsupported("variation `" + v.first + "` (" + what + " " + std::to_string(idx) + ")");
+    // an fx priority moves an Any-phase variation to the pre or post phase
+    // (resolve_phase_buckets, src/shader_builder_v2.rs:2406-2470); pre_blur
+    // is pre-phase by definition and keeps its place
     for (auto& p : x.priorities)
-      if (p.second != 0 && x.has_var(p.first) && var_by_name(p.first))
+      if (p.second != 0 && x.has_var(p.first) && var_by_name(p.first) && p.first != "pre_blur")
         unsupported("variation priority on `" + p.first + "`");
     if (!x.plane_identity) unsupported("YZ / ZX plane affines");
     if (x.n_linked_att) unsupported("linked transforms");

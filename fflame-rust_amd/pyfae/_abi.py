@@ -34,12 +34,14 @@ SPLAT_F32_LDS = 4
 VARIATIONS = {
     "linear": 0, "sinusoidal": 1, "spherical": 2, "swirl": 3, "horseshoe": 4,
     "polar": 5, "disc": 8, "hyperbolic": 10, "julia": 13, "waves": 15,
-    "julian": 24, "bubble": 27, "curl": 47,
+    "julian": 24, "blob": 25, "bubble": 27, "noise": 29, "blur": 30, "gaussian_blur": 31,
+    "curl": 47, "pre_blur": 54,
 }
 VARIATION_NAMES = {v: k for k, v in VARIATIONS.items()}
 # User parameters per variation, in slot order (julian's slot 2 is derived).
-VARIATION_PARAMS = {"julian": ("power", "dist"), "curl": ("c1", "c2")}
-PARAM_DEFAULTS = {"julian": (2.0, 1.0), "curl": (1.0, 0.0)}
+VARIATION_PARAMS = {"julian": ("power", "dist"), "curl": ("c1", "c2"),
+                    "blob": ("high", "low", "waves")}
+PARAM_DEFAULTS = {"julian": (2.0, 1.0), "curl": (1.0, 0.0), "blob": (1.0, 1.0, 6.0)}
 
 
 class Xform(C.Structure):

@@ -12,6 +12,8 @@ palette stops) so the same descriptor drives the GPU and the oracle.
   fit        4 xforms {linear, spherical, sinusoidal}, 256x256 soft splat
   fit_batch  64 independent 3-xform soft-splat fits at 128x128
   stress     12 xforms incl. julian/curl/waves, 12288^2 f32 histogram
+  blurs      the reference presets' other variations (pre_blur, blur, noise,
+             gaussian_blur, blob) beside julian/bubble/spherical, 640x480
 """
 import math
 
@@ -24,6 +26,8 @@ C1_VARIATIONS = ["linear", "sinusoidal", "spherical", "swirl", "horseshoe", "pol
                  "bubble", "hyperbolic", "disc"]
 C4_VARIATIONS = C1_VARIATIONS + ["julian", "curl", "waves"]
 FIT_VARIATIONS = ["linear", "spherical", "sinusoidal"]
+BLUR_VARIATIONS = ["pre_blur", "blur", "noise", "gaussian_blur", "blob", "julian", "bubble",
+                   "spherical", "linear"]
 
 
 class SplitMix64:
@@ -78,6 +82,10 @@ def _params(rng, names):
     if "curl" in names:
         p["curl.c1"] = rng.uniform(-1.0, 1.0)
         p["curl.c2"] = rng.uniform(-0.5, 0.5)
+    if "blob" in names:
+        p["blob.high"] = rng.uniform(0.5, 1.5)
+        p["blob.low"] = rng.uniform(0.0, 0.8)
+        p["blob.waves"] = float(rng.randint(2, 8))
     return p
 
 
@@ -142,6 +150,17 @@ def single4k(walkers=1 << 24, ipt=256, width=3840, height=2160, splat_mode=0):
     job = make_job(width, height, walkers=walkers, ipt=ipt, burn_in=20, seed_base=1,
                    seed_stride=SEED_STRIDE, target_samples=0, zoom=4.0 / 3.0,
                    splat_mode=splat_mode, de=(9.0, 0.0, 0.4), tonemap=log_tonemap(2.2))
+    return flame_desc(xs, fins), pal, job
+
+
+def blurs(walkers=1 << 16, ipt=256, width=640, height=480):
+    """5 xforms + final over BLUR_VARIATIONS (3 each; pre_blur, when drawn,
+    runs before the others whatever its local index)."""
+    rng = SplitMix64(1006)
+    xs, fins = random_flame(rng, 5, BLUR_VARIATIONS, norm=0.7)
+    pal = random_palette(rng)
+    job = make_job(width, height, walkers=walkers, ipt=ipt, burn_in=20, seed_base=1,
+                   seed_stride=SEED_STRIDE, zoom=1.0, tonemap=log_tonemap(2.2))
     return flame_desc(xs, fins), pal, job
 
 

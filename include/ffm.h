@@ -75,8 +75,13 @@ typedef enum {
   FFM_VAR_JULIA = 13,
   FFM_VAR_WAVES = 15,
   FFM_VAR_JULIAN = 24, /* params: [0]=power, [1]=dist; [2]=cpower derived */
+  FFM_VAR_BLOB = 25,   /* params: [0]=high, [1]=low, [2]=waves */
   FFM_VAR_BUBBLE = 27,
-  FFM_VAR_CURL = 47    /* params: [0]=c1, [1]=c2 */
+  FFM_VAR_NOISE = 29,
+  FFM_VAR_BLUR = 30,
+  FFM_VAR_GAUSSIAN_BLUR = 31,
+  FFM_VAR_CURL = 47,   /* params: [0]=c1, [1]=c2 */
+  FFM_VAR_PRE_BLUR = 54 /* pre phase: moves the affine point before the others */
 } ffm_variation_id;
 
 /* One transform. Field meaning follows the reference Transform / GPU

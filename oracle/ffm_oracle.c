@@ -275,6 +275,42 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
       *oy = (y * re - x * im) * r;
       return;
     }
+    case FFM_VAR_BLOB: {
+      /* advanced.rs BLOB: r = length(p), theta = ff_atan2(p.x, p.y) */
+      float high = p[0], low = p[1], waves = p[2];
+      float r = sqrtf(len2(x, y));
+      float theta = ff_atan2(x, y);
+      float scale = r * (low + ((high - low) / 2.0f) * (sinf(waves * theta) + 1.0f));
+      *ox = scale * cosf(theta);
+      *oy = scale * sinf(theta);
+      return;
+    }
+    case FFM_VAR_NOISE: {
+      /* WGSL evaluates the draws left to right: theta first, then r */
+      float theta = orc_rng_nextf(rng) * F32(6.28318530718);
+      float r = orc_rng_nextf(rng);
+      *ox = x * r * cosf(theta);
+      *oy = y * r * sinf(theta);
+      return;
+    }
+    case FFM_VAR_BLUR: {
+      float theta = orc_rng_nextf(rng) * F32(6.28318530718);
+      float r = orc_rng_nextf(rng);
+      *ox = r * cosf(theta);
+      *oy = r * sinf(theta);
+      return;
+    }
+    case FFM_VAR_GAUSSIAN_BLUR: {
+      float theta = orc_rng_nextf(rng) * F32(6.28318530718);
+      float d1 = orc_rng_nextf(rng);
+      float d2 = orc_rng_nextf(rng);
+      float d3 = orc_rng_nextf(rng);
+      float d4 = orc_rng_nextf(rng);
+      float r = d1 + d2 + d3 + d4 - 2.0f;
+      *ox = r * cosf(theta);
+      *oy = r * sinf(theta);
+      return;
+    }
     default:
       *ox = 0.0f;
       *oy = 0.0f;
@@ -305,8 +341,24 @@ void orc_apply_xform(const ffm_flame_desc* f, const ffm_xform* xf, float* px, fl
   float ax = xf->a * x + xf->b * y + xf->e;
   float ay = xf->c * x + xf->d * y + xf->f;
   float rx = 0.0f, ry = 0.0f;
+  /* phase 1, pre variations in local-index order, each replacing the point
+   * the normal phase reads (src/shader_builder_v2.rs:2596-2650): pre_blur
+   * (src/variations/defs/blur.rs PRE_BLUR) adds a Gaussian-ish offset of
+   * its own weight times (six draws summed - 3) at angle 2 pi * draw */
   for (int k = 0; k < f->n_vars; k++) {
     float w = xf->var_weight[k];
+    if (f->var_id[k] != FFM_VAR_PRE_BLUR || w == 0.0f) continue;
+    float g1 = orc_rng_nextf(rng), g2 = orc_rng_nextf(rng), g3 = orc_rng_nextf(rng);
+    float g4 = orc_rng_nextf(rng), g5 = orc_rng_nextf(rng), g6 = orc_rng_nextf(rng);
+    float rnd_g = w * (g1 + g2 + g3 + g4 + g5 + g6 - 3.0f);
+    float rnd_a = orc_rng_nextf(rng) * F32(6.28318530717959);
+    ax = ax + rnd_g * cosf(rnd_a);
+    ay = ay + rnd_g * sinf(rnd_a);
+  }
+  /* phase 3, normal variations */
+  for (int k = 0; k < f->n_vars; k++) {
+    float w = xf->var_weight[k];
+    if (f->var_id[k] == FFM_VAR_PRE_BLUR) continue;
     if (w != 0.0f) {
       float vx, vy;
       orc_variation(f->var_id[k], xf->var_param[k], xf, ax, ay, rng, &vx, &vy);

@@ -173,7 +173,7 @@ def ulp(x):
     return np.spacing(np.maximum(x, np.float32(2.0 ** -20))).astype(np.float64)
 
 
-@pytest.mark.parametrize("cfg", ["single4k", "stress"])
+@pytest.mark.parametrize("cfg", ["single4k", "stress", "blurs"])
 def test_step_parity_transcendental(engine, cfg):
     """One iterator step on the GPU vs the oracle's step from the same state.
     RNG words, transform choices, respawn flags and the colour index are

@@ -366,6 +366,78 @@ def test_julia_and_julian_consume_one_draw():
         assert (r.state, r.inc) == (99, 5)
 
 
+def _draws(state, inc, n):
+    """The next n rng_nextf values of a stream (the pinned PCG step)."""
+    L = O.lib()
+    r = O.Rng()
+    r.state, r.inc = state, inc
+    return [L.orc_rng_nextf(C.byref(r)) for _ in range(n)], r
+
+
+def test_blob_against_float64_closed_form():
+    high, low, waves = 1.3, 0.4, 5.0
+    for x, y in PROBE[2:]:
+        X, Y = float(np.float32(x)), float(np.float32(y))
+        if X * X + Y * Y > 1e4:
+            continue  # sin(waves * theta) is fine, but scale * cos loses the float32 digits
+        r = math.hypot(X, Y)
+        th = math.atan2(X, Y)  # Apophysis's atan2(x, y)
+        sc = r * (low + (high - low) / 2 * (math.sin(waves * th) + 1))
+        ox, oy, st = _var("blob", x, y, params=(high, low, waves, 0), rng_state=(5, 3))
+        assert _near(ox, sc * math.cos(th), 1e-5, 1e-6) and _near(oy, sc * math.sin(th), 1e-5, 1e-6)
+        assert (st.state, st.inc) == (5, 3)  # no draw
+
+
+def test_blur_noise_gaussian_blur_draws_and_closed_forms():
+    tp = float(np.float32(6.28318530718))
+    for x, y in PROBE[2:8]:
+        X, Y = float(np.float32(x)), float(np.float32(y))
+        u, end2 = _draws(77, 9, 2)
+        th = float(np.float32(u[0]) * np.float32(tp))
+        ox, oy, st = _var("blur", x, y, rng_state=(77, 9))
+        assert _near(ox, u[1] * math.cos(th), 1e-5, 1e-7) and _near(oy, u[1] * math.sin(th), 1e-5, 1e-7)
+        assert (st.state, st.inc) == (end2.state, end2.inc)
+        ox, oy, st = _var("noise", x, y, rng_state=(77, 9))
+        assert _near(ox, X * u[1] * math.cos(th), 1e-5, 1e-7)
+        assert _near(oy, Y * u[1] * math.sin(th), 1e-5, 1e-7)
+        assert (st.state, st.inc) == (end2.state, end2.inc)
+        u, end5 = _draws(77, 9, 5)
+        g = float(np.float32(np.float32(np.float32(np.float32(u[1]) + np.float32(u[2])) + np.float32(u[3]))
+                             + np.float32(u[4])) - np.float32(2.0))
+        ox, oy, st = _var("gaussian_blur", x, y, rng_state=(77, 9))
+        assert _near(ox, g * math.cos(th), 1e-5, 1e-7) and _near(oy, g * math.sin(th), 1e-5, 1e-7)
+        assert (st.state, st.inc) == (end5.state, end5.inc)
+
+
+def test_pre_blur_runs_before_the_sum_whatever_its_slot():
+    """pre_blur is pre-phase: it moves the affine point by w * (six draws
+    - 3) at angle 2 pi * a seventh draw, and the normal variations read the
+    moved point, even when pre_blur's local index comes after theirs."""
+    L = O.lib()
+    xs = [pyfae.xform(0.8, 0.1, -0.2, 0.9, 0.05, -0.1, variations={"spherical": 0.7,
+                                                                   "pre_blur": 0.35})]
+    fd = pyfae.flame_desc(xs)
+    assert [fd.var_id[k] for k in range(fd.n_vars)] == [2, 54]
+    for x, y in ((0.3, 0.4), (-0.7, 0.2), (1.5, -1.1)):
+        X, Y = np.float32(x), np.float32(y)
+        ax = float(np.float32(np.float32(0.8) * X + np.float32(0.1) * Y) + np.float32(0.05))
+        ay = float(np.float32(np.float32(-0.2) * X + np.float32(0.9) * Y) + np.float32(-0.1))
+        u, end = _draws(1234, 11, 7)
+        g = np.float32(0)
+        for v in u[:6]:
+            g = np.float32(g + np.float32(v))
+        rg = float(np.float32(0.35) * np.float32(g - np.float32(3.0)))
+        ang = float(np.float32(u[6]) * np.float32(6.28318530717959))
+        tx, ty = ax + rg * math.cos(ang), ay + rg * math.sin(ang)
+        r2 = tx * tx + ty * ty + 1e-6
+        px, py = C.c_float(x), C.c_float(y)
+        r = O.Rng()
+        r.state, r.inc = 1234, 11
+        L.orc_apply_xform(C.byref(fd), C.byref(fd.xforms[0]), C.byref(px), C.byref(py), C.byref(r))
+        assert _near(px.value, 0.7 * tx / r2, 1e-5, 1e-6) and _near(py.value, 0.7 * ty / r2, 1e-5, 1e-6)
+        assert (r.state, r.inc) == (end.state, end.inc)  # seven draws
+
+
 def test_curl_is_the_moebius_like_closed_form():
     c1, c2 = 0.7, 0.2
     for x, y in PROBE[2:8]:
