@@ -13,12 +13,10 @@ raises EngineError.
 import ctypes as C
 import json
 import os
-import struct
-import zlib
 
 import numpy as np
 
-from . import _abi
+from . import _abi, formats
 from ._abi import (FlameDesc, Job, PaletteXf, RenderStats, Scene, SoftJob,  # noqa: F401
                    TonemapParams, View, Xform, VARIATIONS, tonemap_defaults)
 
@@ -216,19 +214,22 @@ def soft_theta(fd):
 # ---------------------------------------------------------------- configs
 
 def _text(x):
-    """Config text from a dict / list (JSON), bytes, JSON or XML text, or a
-    path (python/pyfae/__init__.py:49-60, with XML text besides)."""
+    """Config text from a dict / list (JSON), bytes, JSON or XML text, a
+    metadata PNG (its `Config` chunk), or a path to any of these
+    (python/pyfae/__init__.py:49-60, with XML and PNG besides)."""
     if isinstance(x, (dict, list)):
         return json.dumps(x)
-    if isinstance(x, (bytes, bytearray)):
-        return bytes(x).decode("utf-8")
     if isinstance(x, os.PathLike):
         x = os.fspath(x)
     if isinstance(x, str) and not x.lstrip().startswith(("{", "[", "<")) and os.path.exists(x):
-        with open(x, encoding="utf-8") as f:
-            return f.read()
+        with open(x, "rb") as f:
+            x = f.read()
+    if isinstance(x, (bytes, bytearray)):
+        if bytes(x[:8]) == formats.PNG_SIGNATURE:
+            return formats.config_from_png(bytes(x))
+        return bytes(x).decode("utf-8")
     if not isinstance(x, str):
-        raise TypeError(f"expected a config (dict, JSON / XML text or a path), got {type(x)}")
+        raise TypeError(f"expected a config (dict, JSON / XML text, PNG or a path), got {type(x)}")
     return x
 
 
@@ -275,28 +276,17 @@ def scene_job(sc, width, height, *, iterations=None, iterations_per_thread=None,
     return j
 
 
-def _png(rgba):
-    """A plain RGBA8 PNG of an (h, w, 4) uint8 array."""
-    h, w = rgba.shape[:2]
-    raw = b"".join(b"\x00" + rgba[y].tobytes() for y in range(h))
-
-    def chunk(tag, data):
-        return (struct.pack(">I", len(data)) + tag + data
-                + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF))
-    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0))
-            + chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
-
-
 class Image:
     """A rendered frame (python/pyfae/__init__.py:76-113): RGBA8 pixels,
     `width` x `height`, and its numbers (`info`: iterations, ms,
     iterations_per_thread, dispatches)."""
 
-    def __init__(self, rgba, info):
+    def __init__(self, rgba, info, scene=None):
         self.rgba = rgba
         self.info = info
         self.width = info["width"]
         self.height = info["height"]
+        self.scene = scene
 
     def to_numpy(self):
         """An (height, width, 4) uint8 array."""
@@ -306,9 +296,17 @@ class Image:
         from PIL import Image as PILImage
         return PILImage.fromarray(self.rgba, "RGBA")
 
-    def png(self):
-        """The frame as PNG bytes (pixels only: no config metadata)."""
-        return _png(self.rgba)
+    def png(self, strip_metadata=False):
+        """The PNG the reference's export writes for this frame: its config
+        (as FractalConfig JSON) and numbers in tEXt chunks
+        (src/png_metadata.rs:144-260), so it reloads as its flame
+        (`load(png)`); `strip_metadata` writes the pixels alone."""
+        text = ()
+        if self.scene is not None and not strip_metadata:
+            text = formats.metadata_chunks(self.width, self.height, self.info["iterations"],
+                                           self.info["ms"], self.info["iterations_per_thread"],
+                                           formats.scene_to_json(self.scene), self.scene)
+        return formats.encode_png(self.rgba, text)
 
     def save(self, path):
         with open(path, "wb") as f:
@@ -387,7 +385,8 @@ class Engine:
             return self._render_job(config)
         if width is None or height is None:
             raise TypeError("render(config, width, height): width and height are required")
-        raw = _text(config).encode("utf-8")
+        text = _text(config)
+        raw = text.encode("utf-8")
         out = np.zeros((int(height), int(width), 4), dtype=np.uint8)
         stats = RenderStats()
         ipt = int(iterations_per_thread or 0)
@@ -398,7 +397,7 @@ class Engine:
         return Image(out, {"width": int(width), "height": int(height),
                            "iterations": int(stats.total_samples), "ms": stats.render_time_ms,
                            "iterations_per_thread": ipt or 1024,
-                           "dispatches": int(stats.dispatches)})
+                           "dispatches": int(stats.dispatches)}, scene=load(text))
 
     def _render_job(self, job):
         """RGBA8 (H, W, 4) uint8 pixels of the current flame; returns (pixels, stats)."""

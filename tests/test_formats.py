@@ -475,3 +475,74 @@ def test_palette_index_map_closed_forms():
     # the stages compose in the reference's order: squeeze, log, rotate, reverse
     both = O.palette_index_map(n, xf(factor=2.0, rot=0.25, rev=1))
     assert (both == ((2 * ((i[::-1] + 64) % n)) % n)).all()
+
+
+# ------------------------------------------- FractalConfig writer and PNG metadata
+
+from pyfae import formats  # noqa: E402
+
+
+def _roundtrip(config, index=0):
+    sc = pyfae.load(config, index)
+    text = formats.scene_to_json(sc)
+    back = pyfae.load(text)
+    assert formats.scene_summary(back) == formats.scene_summary(sc)
+    # the same slots in the same order: the descriptor itself is identical
+    assert bytes(back.flame) == bytes(sc.flame)
+    assert json.loads(text)["version"] == 7
+    return sc, back
+
+
+def test_scene_to_config_round_trips():
+    _roundtrip(os.path.join(GOLDEN, "ref_simple_linear.fflame"))
+    _roundtrip(FLAME, 0)
+    _roundtrip(FLAME, 1)
+    t = lambda v, **kw: dict({"a": 0.7, "b": 0.1, "c": -0.1, "d": 0.6, "e": 0.2, "f": -0.3,
+                              "weight": 0.5, "color": 0.3, "variations": v}, **kw)
+    cfg = {"version": 7, "flame": {"name": "mix", "transforms": [
+        t({"julian": 0.6, "pre_blur": 0.2, "linear": 0.4}, variation_params={"julian.power": 3},
+          final_attachments=[1]),
+        t({"curl": 1.0, "blob": 0.3}, variation_order=["blob", "curl"],
+          variation_params={"curl.c1": 0.3, "blob.waves": 4}, post_affine_enabled=True,
+          post_a=0.9, post_b=0.1, post_c=0.0, post_d=1.1, post_e=0.0, post_f=0.2),
+        t({"spherical": 1.0}, final_attachments=[0, 1])],
+        "final_transforms": [t({"linear": 1.0}), t({"bubble": 0.5, "noise": 0.5})],
+        "xaos": [[1, 0.5, 0], [2, 1, 1], [0, 1, 3]]},
+        "palette_squeeze_mode": "geometric", "palette_squeeze_falloff": 0.4, "palette_reverse": True,
+        "palette_rotation": 0.2, "palette_size": 1024, "highlight_mode": "reinhard",
+        "tonemap_mode": "linear", "reset_orbits": True, "deterministic_rng": True,
+        "pan_x": 0.123456789012345, "zoom": 1.7, "filter_radius": 0.8}
+    sc, back = _roundtrip(cfg)
+    assert back.flame.has_xaos == 1 and back.palette_xf.squeeze_mode == 1
+    assert back.pan_x == 0.123456789012345  # pan stays f64
+
+
+def test_png_metadata_round_trip():
+    from PIL import Image as PILImage
+    import io
+    sc = pyfae.load(FLAME)
+    rgba = np.random.default_rng(3).integers(0, 256, (12, 17, 4), dtype=np.uint8)
+    img = pyfae.Image(rgba, {"width": 17, "height": 12, "iterations": 123456, "ms": 45.678,
+                             "iterations_per_thread": 256, "dispatches": 3}, scene=sc)
+    data = img.png()
+    pil = PILImage.open(io.BytesIO(data))
+    np.testing.assert_array_equal(np.asarray(pil.convert("RGBA")), rgba)
+    assert pil.text["Resolution"] == "17x12" and pil.text["Iterations"] == "123456"
+    md = formats.read_png_metadata(data)
+    assert (md["width"], md["height"], md["total_iterations"]) == (17, 12, 123456)
+    assert md["render_time_ms"] == 45.68 and md["iterations_per_thread"] == 256
+    assert md["render_mode"] == "3D" and md["version"] == "0.5.0+mi355x"
+    import hashlib
+    assert md["config_checksum"] == hashlib.sha256(md["config_json"].encode()).hexdigest()
+    # the PNG reloads as its flame
+    assert formats.scene_summary(pyfae.load(data)) == formats.scene_summary(sc)
+    # a name outside Latin-1 travels in an iTXt chunk
+    cfg = json.loads(formats.scene_to_json(sc))
+    cfg["flame"]["name"] = "flamme ☃"
+    sc2 = pyfae.load(cfg)
+    data2 = pyfae.Image(rgba, dict(img.info), scene=sc2).png()
+    assert pyfae.load(data2).name.decode() == "flamme ☃"
+    bare = img.png(strip_metadata=True)
+    assert formats.png_text(bare) == {}
+    with pytest.raises(ValueError):
+        formats.config_from_png(bare)

@@ -134,3 +134,12 @@ def test_unsupported_config_raises_before_rendering(engine):
     with pytest.raises(pyfae.EngineError) as e:
         engine.render(json.dumps(cfg), 64, 64)
     assert e.value.status == _abi.ERR_UNSUPPORTED and "spiral" in str(e.value)
+
+
+def test_png_reloads_to_the_same_picture(engine):
+    """The exported PNG carries its config (src/png_metadata.rs): rendering
+    the PNG gives the same pixels."""
+    say("png metadata round trip")
+    img = engine.render(affine_config(), 160, 120, iterations=500_000, iterations_per_thread=64)
+    img2 = engine.render(img.png(), 160, 120, iterations=500_000, iterations_per_thread=64)
+    np.testing.assert_array_equal(img.to_numpy(), img2.to_numpy())
