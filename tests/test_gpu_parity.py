@@ -120,10 +120,12 @@ def gpu_hist_dev(engine, fd, pal, view, walkers, ipt, burn_in, seed, mode):
     return h, tuple(int(v) for v in cnt.cpu())
 
 
-@pytest.mark.parametrize("side", [(6144, 3072), (12288, 12288)])
+@pytest.mark.parametrize("side", [(6144, 3072), (12288, 12288), (32768, 32768)])
 def test_lds_splat_with_tile_bins(engine, side):
     """Images past 2048 tiles sort records into bins of 2^k tiles (6144x3072:
-    2 tiles per bin; 12288^2, the stress size: 16). The integer histogram
+    2 tiles per bin; 12288^2, the stress size: 16; 32768^2, the reference's
+    largest side (src/config/defaults.rs:33), 64 tiles per bin in one 17 GB
+    histogram where the reference would tile it). The integer histogram
     still equals the atomic splat's bit for bit; the float one to rounding."""
     w, h = side
     say(f"lds splat with tile bins {w}x{h}")
