@@ -162,6 +162,7 @@ struct TonemapConst {
   float levels_low, levels_high, levels_gamma, sample_density;
   float bg_lin[3];
   float inv22;
+  int32_t transparent;  // transparent_mode: 0 opaque, 1 straight, 2 premultiplied
 };
 hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H,
                           const TonemapConst& tc, hipStream_t s);

@@ -698,6 +698,7 @@ TonemapConst make_tonemap(const ffm_tonemap_params& tp, int W, int H, float zoom
   t.sample_density = sd;
   for (int k = 0; k < 3; k++) t.bg_lin[k] = std::pow(std::max(tp.background[k], 0.0f), 2.2f);
   t.inv22 = (float)(1.0 / 2.2);
+  t.transparent = tp.transparent_mode;
   return t;
 }
 
@@ -924,9 +925,10 @@ int ffm_tonemap(ffm_ctx* c, const float* accum, uint8_t* rgba, int width, int he
                 double samples_in_buffer, const ffm_tonemap_params* tp, void* stream) {
   if (!c || !accum || !rgba || !tp || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
   if (tp->tonemap_mode < 0 || tp->tonemap_mode > 2 || tp->highlight_mode < 0 ||
-      tp->highlight_mode > 3)
-    return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode %d / highlight mode %d not supported",
-                tp->tonemap_mode, tp->highlight_mode);
+      tp->highlight_mode > 3 || tp->transparent_mode < 0 || tp->transparent_mode > 2)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                "tonemap mode %d / highlight mode %d / transparent mode %d not supported",
+                tp->tonemap_mode, tp->highlight_mode, tp->transparent_mode);
   (void)hipSetDevice(c->device);
   TonemapConst t = make_tonemap(*tp, width, height, zoom, samples_in_buffer);
   HIPCHECK(c, launch_tonemap(accum, rgba, width, height, t, pick(c, stream)));
@@ -1092,7 +1094,8 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     uint8_t* pix = rgba_dev ? rgba_dev : reinterpret_cast<uint8_t*>(c->rgba.p);
     TonemapConst t = make_tonemap(job->tonemap, W, H, job->view.zoom, (double)samples_in_buffer);
     if (job->tonemap.tonemap_mode < 0 || job->tonemap.tonemap_mode > 2 ||
-        job->tonemap.highlight_mode < 0 || job->tonemap.highlight_mode > 3)
+        job->tonemap.highlight_mode < 0 || job->tonemap.highlight_mode > 3 ||
+        job->tonemap.transparent_mode < 0 || job->tonemap.transparent_mode > 2)
       return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode not supported");
     HIPCHECK(c, launch_tonemap(final_acc, pix, W, H, t, s));
     if (rgba_out) {

@@ -1697,6 +1697,10 @@ int ffm_scene_job(const ffm_scene* s, int width, int height, uint64_t iterations
   j->filter_radius = s->filter_radius;
   j->filter_blur_edges = s->filter_blur_edges;
   j->tonemap = s->tonemap;
+  // transparent_mode (src/renderer/compute_kernel.rs:3771-3784)
+  j->tonemap.transparent_mode = !(flags & FFM_RENDER_TRANSPARENT)  ? 0
+                                : (flags & FFM_RENDER_PREMULTIPLIED) ? 2
+                                                                     : 1;
   return FFM_OK;
 }
 

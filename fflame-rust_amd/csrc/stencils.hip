@@ -407,6 +407,26 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
   float base_alpha = gamma_alpha * (1.0f - blend_t) + linear_alpha * blend_t;
   float fa = t.levels_enabled ? fminf(base_alpha, leveled) : base_alpha;
   uchar4 o;
+  if (t.transparent) {
+    // transparent export (tonemap.wgsl:531-576): the over-black picture
+    // C = srgb(colour x alpha); premultiplied keeps C with the opacity,
+    // straight alpha takes coverage = max(C) and stores C / coverage
+    float ob[3];
+    for (int k = 0; k < 3; k++) ob[k] = powf(fmaxf(color[k] * fa, 0.0f), t.inv22);
+    if (t.transparent == 2) {
+      o.x = unorm8(ob[0]);
+      o.y = unorm8(ob[1]);
+      o.z = unorm8(ob[2]);
+      o.w = unorm8(fa);
+      return o;
+    }
+    float cov = fmaxf(ob[0], fmaxf(ob[1], ob[2]));
+    o.x = unorm8(cov > 0.0f ? ob[0] / cov : 0.0f);
+    o.y = unorm8(cov > 0.0f ? ob[1] / cov : 0.0f);
+    o.z = unorm8(cov > 0.0f ? ob[2] / cov : 0.0f);
+    o.w = unorm8(cov);
+    return o;
+  }
   o.x = unorm8(powf(t.bg_lin[0] * (1.0f - fa) + color[0] * fa, t.inv22));
   o.y = unorm8(powf(t.bg_lin[1] * (1.0f - fa) + color[1] * fa, t.inv22));
   o.z = unorm8(powf(t.bg_lin[2] * (1.0f - fa) + color[2] * fa, t.inv22));

@@ -264,13 +264,16 @@ def scene_stops(sc):
     return np.ctypeslib.as_array(sc.stops)[:sc.n_stops].astype(np.float32).copy()
 
 
-def scene_job(sc, width, height, *, iterations=None, iterations_per_thread=None, as_2d=False):
+def scene_job(sc, width, height, *, iterations=None, iterations_per_thread=None, as_2d=False,
+              transparent=False, premultiplied=False):
     """The ffm_job the reference's export builds for a scene at width x height."""
     j = Job()
     err = C.create_string_buffer(512)
+    flags = ((_abi.RENDER_3D_AS_2D if as_2d else 0)
+             | (_abi.RENDER_TRANSPARENT if transparent else 0)
+             | (_abi.RENDER_PREMULTIPLIED if premultiplied else 0))
     st = library().ffm_scene_job(C.byref(sc), int(width), int(height), int(iterations or 0),
-                                 int(iterations_per_thread or 0),
-                                 _abi.RENDER_3D_AS_2D if as_2d else 0, C.byref(j), err, 512)
+                                 int(iterations_per_thread or 0), flags, C.byref(j), err, 512)
     if st != 0:
         raise EngineError(st, err.value.decode(errors="replace"))
     return j
@@ -370,14 +373,15 @@ class Engine:
 
     # -- whole renders
     def render(self, config, width=None, height=None, *, iterations=None,
-               iterations_per_thread=None, as_2d=False):
+               iterations_per_thread=None, transparent=False, premultiplied=False, as_2d=False):
         """Render a config (python/pyfae/__init__.py:201-221): `config` is a
         .fflame / .fae JSON or .flame XML text, a dict or a path, rendered at
         `width` x `height` with the config's flame, palette, view and tone map
         (`iterations`: the sample budget, default the config's own;
-        `iterations_per_thread`: default 1024). Returns an Image. `as_2d`
-        renders a 3D-mode config (every .flame import) whose camera is flat
-        through the 2D chaos game.
+        `iterations_per_thread`: default 1024; `transparent`: alpha instead
+        of the background, straight or `premultiplied`). Returns an Image.
+        `as_2d` renders a 3D-mode config (every .flame import) whose camera
+        is flat through the 2D chaos game.
 
         Given an ffm_job instead, renders the current flame and returns
         (pixels, stats)."""
@@ -390,10 +394,12 @@ class Engine:
         out = np.zeros((int(height), int(width), 4), dtype=np.uint8)
         stats = RenderStats()
         ipt = int(iterations_per_thread or 0)
+        flags = ((_abi.RENDER_3D_AS_2D if as_2d else 0)
+                 | (_abi.RENDER_TRANSPARENT if transparent else 0)
+                 | (_abi.RENDER_PREMULTIPLIED if premultiplied else 0))
         self._check(self.lib.ffm_render_config(
-            self.ctx, raw, len(raw), int(width), int(height), int(iterations or 0), ipt,
-            _abi.RENDER_3D_AS_2D if as_2d else 0, out.ctypes.data_as(C.POINTER(C.c_uint8)),
-            C.byref(stats)))
+            self.ctx, raw, len(raw), int(width), int(height), int(iterations or 0), ipt, flags,
+            out.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(stats)))
         return Image(out, {"width": int(width), "height": int(height),
                            "iterations": int(stats.total_samples), "ms": stats.render_time_ms,
                            "iterations_per_thread": ipt or 1024,

@@ -11,6 +11,8 @@ MAX_PALETTE = 4096
 MAX_STOPS = 1024
 SCENE_NAME = 128
 RENDER_3D_AS_2D = 1
+RENDER_TRANSPARENT = 2
+RENDER_PREMULTIPLIED = 4
 
 OK = 0
 ERR_INVALID_ARGUMENT = -1
@@ -87,6 +89,7 @@ class TonemapParams(C.Structure):
         ("levels_enabled", C.c_int32),
         ("levels_low", C.c_float), ("levels_high", C.c_float), ("levels_gamma", C.c_float),
         ("background", C.c_float * 3),
+        ("transparent_mode", C.c_int32),
     ]
 
 

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 4
+#define FFM_ABI_VERSION 5
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -162,6 +162,11 @@ typedef struct {
   int32_t levels_enabled;
   float levels_low, levels_high, levels_gamma;
   float background[3]; /* sRGB */
+  /* transparent_mode of TonemapParams (shaders/tonemap.wgsl:531-576):
+   * 0 opaque over the background; 1 transparent, straight alpha split so a
+   * flatten over black (rgb x a) gives the over-black picture; 2
+   * transparent, premultiplied (rgb = the over-black colour, a = opacity) */
+  int32_t transparent_mode;
 } ffm_tonemap_params;
 
 /* A render job: RenderJob (src/renderer/render.rs:111-138) plus the knobs
@@ -438,9 +443,14 @@ int ffm_scene_parse(const char* text, size_t len, int index, ffm_scene* out, cha
 int ffm_palette_set_xf(ffm_ctx* ctx, const float* stops_prgb, int n_stops, int table_size,
                        const ffm_palette_xf* xf);
 
-/* A scene's 3D render mode (a flat camera) renders through the 2D chaos
- * game only when the caller asks for it with this flag. */
+/* ffm_scene_job / ffm_render_config flags. A scene's 3D render mode (a
+ * flat camera) renders through the 2D chaos game only with
+ * FFM_RENDER_3D_AS_2D. FFM_RENDER_TRANSPARENT and FFM_RENDER_PREMULTIPLIED
+ * are pyfae render's transparent= / premultiplied= (transparent_mode 1 or
+ * 2, src/renderer/compute_kernel.rs:3771-3784). */
 #define FFM_RENDER_3D_AS_2D 1u
+#define FFM_RENDER_TRANSPARENT 2u
+#define FFM_RENDER_PREMULTIPLIED 4u
 
 /* The job the reference's export builds for a config (RenderJob::new,
  * src/renderer/render.rs:111-138, with pyfae's render arguments,

@@ -1319,6 +1319,20 @@ void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
     float blend_t = t * t * (3.0f - 2.0f * t);
     float base_alpha = gamma_alpha * (1.0f - blend_t) + linear_alpha * blend_t;
     float fa = tp->levels_enabled ? fminf(base_alpha, leveled) : base_alpha;
+    if (tp->transparent_mode != 0) {
+      /* tonemap.wgsl:531-576: over_black = pow(max(colour * alpha, 0), 1/2.2) */
+      float ob[3];
+      for (int k = 0; k < 3; k++) ob[k] = powf(fmaxf(color[k] * fa, 0.0f), inv22);
+      if (tp->transparent_mode == 2) { /* premultiplied */
+        for (int k = 0; k < 3; k++) rgba[4 * p + k] = unorm8(ob[k]);
+        rgba[4 * p + 3] = unorm8(fa);
+      } else { /* straight alpha with coverage = max channel */
+        float cov = fmaxf(ob[0], fmaxf(ob[1], ob[2]));
+        for (int k = 0; k < 3; k++) rgba[4 * p + k] = unorm8(cov > 0.0f ? ob[k] / cov : 0.0f);
+        rgba[4 * p + 3] = unorm8(cov);
+      }
+      continue;
+    }
     for (int k = 0; k < 3; k++) {
       float fin = bg_lin[k] * (1.0f - fa) + color[k] * fa;
       rgba[4 * p + k] = unorm8(powf(fin, inv22));

@@ -143,3 +143,22 @@ def test_png_reloads_to_the_same_picture(engine):
     img = engine.render(affine_config(), 160, 120, iterations=500_000, iterations_per_thread=64)
     img2 = engine.render(img.png(), 160, 120, iterations=500_000, iterations_per_thread=64)
     np.testing.assert_array_equal(img.to_numpy(), img2.to_numpy())
+
+
+@pytest.mark.parametrize("premultiplied", [False, True])
+def test_transparent_render_equals_oracle(engine, premultiplied):
+    """pyfae render(transparent=True, premultiplied=...) (transparent_mode 1
+    or 2) against the oracle's render of the same job."""
+    say(f"transparent render premultiplied={premultiplied}")
+    cfg = affine_config()
+    img = engine.render(cfg, 200, 150, iterations=1_000_000, iterations_per_thread=64,
+                        transparent=True, premultiplied=premultiplied)
+    sc = pyfae.load(cfg)
+    job = pyfae.scene_job(sc, 200, 150, iterations=1_000_000, iterations_per_thread=64,
+                          transparent=True, premultiplied=premultiplied)
+    assert job.tonemap.transparent_mode == (2 if premultiplied else 1)
+    orgba, _, _ = O.render(sc.flame, pyfae.scene_stops(sc), job, threads=8,
+                           palette_xf=sc.palette_xf)
+    d = np.abs(img.to_numpy().astype(int) - orgba.astype(int))
+    assert d.max() <= 1
+    assert (img.to_numpy()[..., 3] < 255).any()  # a real alpha channel

@@ -242,7 +242,12 @@ def test_tonemap_within_one_level(engine):
     acc[..., :3] = rng.uniform(0, 1, (H, W, 3))
     acc[..., 3] = np.exp(rng.uniform(-3, 12, (H, W))).astype(np.float32)
     acc[::7, ::3] = 0
-    for tp in (pyfae.tonemap_defaults(), configs.log_tonemap(2.2)):
+    tps = [pyfae.tonemap_defaults(), configs.log_tonemap(2.2)]
+    for mode in (1, 2):  # transparent export: straight, premultiplied
+        t = configs.log_tonemap(2.2)
+        t.transparent_mode = mode
+        tps.append(t)
+    for tp in tps:
         tp.saturation = 1.3
         tp.background[:] = [0.1, 0.2, 0.05]
         o = O.tonemap(acc, 4.0 / 3.0, 5e6, tp)

@@ -639,3 +639,29 @@ def test_render_exact_count_switch():
         np.testing.assert_array_equal(np.float32(cnt) / np.float32(100), alpha)
         np.testing.assert_allclose(alpha, a_sum[..., 3], rtol=2e-6, atol=0)
         np.testing.assert_array_equal(a_ex[..., :3] == 0, a_sum[..., :3] == 0)
+
+
+def test_tonemap_transparent_modes():
+    """Transparent export (shaders/tonemap.wgsl:531-576). Premultiplied: rgb
+    is the over-black picture, so it equals the opaque tonemap over a black
+    background bit for bit; straight alpha: the coverage alpha is the largest
+    channel of that picture, and rgb x a over black gives it back."""
+    rng = np.random.default_rng(7)
+    H, W = 40, 50
+    acc = np.zeros((H, W, 4), np.float32)
+    acc[..., :3] = rng.uniform(0, 1, (H, W, 3))
+    acc[..., 3] = np.exp(rng.uniform(-4, 9, (H, W))).astype(np.float32)
+    acc[::5, ::4] = 0
+    out = {}
+    for mode in (0, 1, 2):
+        tp = configs.log_tonemap(2.2)
+        tp.transparent_mode = mode
+        out[mode] = O.tonemap(acc, 1.0, 2e5, tp).astype(int)
+    opaque, straight, premul = out[0], out[1], out[2]
+    assert (opaque[..., 3] == 255).all()
+    np.testing.assert_array_equal(premul[..., :3], opaque[..., :3])
+    np.testing.assert_array_equal(straight[..., 3], premul[..., :3].max(-1))
+    flat = np.round(straight[..., :3] * straight[..., 3:4] / 255.0)
+    assert np.abs(flat - opaque[..., :3]).max() <= 1
+    empty = acc[..., 3] == 0
+    assert (straight[empty] == 0).all() and (premul[empty] == 0).all()
