@@ -91,6 +91,64 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
 // the refined hardware reciprocal (ffm_fdiv, ffm_rcp) and square roots the
 // 1-ulp hardware form (ffm_sqrt); the reference's WGSL allows 2.5 ulp for
 // both, and the per-step bar against the IEEE oracle is 4 ulp.
+// The variations beyond the thirteen of the workloads (blob and the RNG
+// blurs of the reference's presets), kept out of eval_variation's switch
+// unless a flame uses them (the EXT kernels).
+__device__ __forceinline__ void eval_ext(int id, const float* __restrict__ p, float x, float y,
+                                         Rng& rng, float& ox, float& oy) {
+  switch (id) {
+    case FFM_VAR_BLOB: {
+      FFM_NO_SPECULATE();
+      float high = p[0], low = p[1], waves = p[2];
+      float r = ffm_sqrt(x * x + y * y);
+      float theta = ff_atan2(x, y);  // Apophysis's atan2(x, y)
+      float scale = r * (low + ((high - low) / 2.0f) * (ffm_sinf(waves * theta) + 1.0f));
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = scale * c;
+      oy = scale * s;
+      break;
+    }
+    case FFM_VAR_NOISE: {
+      FFM_NO_SPECULATE();
+      float theta = rng_nextf(rng) * TWO_PI_F;
+      float r = rng_nextf(rng);
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = x * r * c;
+      oy = y * r * s;
+      break;
+    }
+    case FFM_VAR_BLUR: {
+      float theta = rng_nextf(rng) * TWO_PI_F;
+      float r = rng_nextf(rng);
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = r * c;
+      oy = r * s;
+      break;
+    }
+    case FFM_VAR_GAUSSIAN_BLUR: {
+      float theta = rng_nextf(rng) * TWO_PI_F;
+      float d1 = rng_nextf(rng);
+      float d2 = rng_nextf(rng);
+      float d3 = rng_nextf(rng);
+      float d4 = rng_nextf(rng);
+      float r = d1 + d2 + d3 + d4 - 2.0f;
+      float s, c;
+      ffm_sincosf(theta, &s, &c);
+      ox = r * c;
+      oy = r * s;
+      break;
+    }
+    default:
+      ox = 0.0f;
+      oy = 0.0f;
+      break;
+  }
+}
+
+template <bool EXT>
 __device__ __forceinline__ void eval_variation(int id, const float* __restrict__ row,
                                                const float* __restrict__ p, float x, float y,
                                                Rng& rng, float& ox, float& oy) {
@@ -210,53 +268,13 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       oy = (y * re - x * im) * r;
       break;
     }
-    case FFM_VAR_BLOB: {
-      FFM_NO_SPECULATE();
-      float high = p[0], low = p[1], waves = p[2];
-      float r = ffm_sqrt(x * x + y * y);
-      float theta = ff_atan2(x, y);  // Apophysis's atan2(x, y)
-      float scale = r * (low + ((high - low) / 2.0f) * (ffm_sinf(waves * theta) + 1.0f));
-      float s, c;
-      ffm_sincosf(theta, &s, &c);
-      ox = scale * c;
-      oy = scale * s;
-      break;
-    }
-    case FFM_VAR_NOISE: {
-      FFM_NO_SPECULATE();
-      float theta = rng_nextf(rng) * TWO_PI_F;
-      float r = rng_nextf(rng);
-      float s, c;
-      ffm_sincosf(theta, &s, &c);
-      ox = x * r * c;
-      oy = y * r * s;
-      break;
-    }
-    case FFM_VAR_BLUR: {
-      float theta = rng_nextf(rng) * TWO_PI_F;
-      float r = rng_nextf(rng);
-      float s, c;
-      ffm_sincosf(theta, &s, &c);
-      ox = r * c;
-      oy = r * s;
-      break;
-    }
-    case FFM_VAR_GAUSSIAN_BLUR: {
-      float theta = rng_nextf(rng) * TWO_PI_F;
-      float d1 = rng_nextf(rng);
-      float d2 = rng_nextf(rng);
-      float d3 = rng_nextf(rng);
-      float d4 = rng_nextf(rng);
-      float r = d1 + d2 + d3 + d4 - 2.0f;
-      float s, c;
-      ffm_sincosf(theta, &s, &c);
-      ox = r * c;
-      oy = r * s;
-      break;
-    }
     default:
-      ox = 0.0f;
-      oy = 0.0f;
+      if constexpr (EXT) {
+        eval_ext(id, p, x, y, rng, ox, oy);
+      } else {
+        ox = 0.0f;
+        oy = 0.0f;
+      }
       break;
   }
 }
@@ -282,6 +300,7 @@ __device__ __forceinline__ void pre_blur(float w, float& x, float& y, Rng& rng) 
 // (src/shader_builder_v2.rs:2525-2806) -> post-affine when enabled
 // (main_template.wgsl:355-384). `ids` are the flame's variation ids
 // (uniform across the wave), `row` the transform's row in LDS.
+template <bool EXT>
 __device__ __forceinline__ void apply_xform(const float* __restrict__ row, const int* ids, int nv,
                                             float& x, float& y, Rng& rng) {
   float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
@@ -291,12 +310,12 @@ __device__ __forceinline__ void apply_xform(const float* __restrict__ row, const
     int id = __builtin_amdgcn_readfirstlane(ids[k]);
     float w = row[XF_W0 + k];
     if (w != 0.0f) {
-      if (id == FFM_VAR_PRE_BLUR) {
+      if (EXT && id == FFM_VAR_PRE_BLUR) {
         pre_blur(w, ax, ay, rng);
         continue;
       }
       float vx, vy;
-      eval_variation(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
+      eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
       rx += w * vx;
       ry += w * vy;
     }
@@ -324,11 +343,12 @@ struct FlameView {
 // over the final's non-zero variation slots only (a zero weight adds
 // nothing and draws nothing, so the sum is the same). Every lane applies the
 // same final, so each read is one uniform LDS address.
+template <bool EXT>
 __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x, float& y,
                                             Rng& rng) {
   const float* row = fl.rows + (fl.n_xforms + i) * XF_STRIDE;
   if (!fl.hdr) {
-    apply_xform(row, fl.ids, fl.n_vars, x, y, rng);
+    apply_xform<EXT>(row, fl.ids, fl.n_vars, x, y, rng);
     return;
   }
   float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
@@ -339,12 +359,12 @@ __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x
     const int k = __builtin_amdgcn_readfirstlane(fl.hdr->fin_act[i][j]);
     const int id = __builtin_amdgcn_readfirstlane(fl.ids[k]);
     const float w = row[XF_W0 + k];
-    if (id == FFM_VAR_PRE_BLUR) {
+    if (EXT && id == FFM_VAR_PRE_BLUR) {
       pre_blur(w, ax, ay, rng);
       continue;
     }
     float vx, vy;
-    eval_variation(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
+    eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
     rx += w * vx;
     ry += w * vy;
   }
@@ -409,11 +429,12 @@ __device__ __forceinline__ int select_const(const FlameView& fl, float u) {
 // With xaos the pick is select_transform_xaos (utilities.wgsl:107-140): the
 // same rule over row `prev` of the table of running sums of
 // weight[j] * xaos[prev][j], and the pick becomes the next `prev`.
+template <bool EXT>
 __device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool& should_plot,
                                             int& prev) {
   float u = rng_nextf(rng);
   int xi;
-  if (fl.xcum) {
+  if (EXT && fl.xcum) {
     const int n = fl.n_xforms;
     const float* xr = fl.xcum + prev * n;
     float target = u * xr[n - 1];
@@ -435,6 +456,7 @@ __device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool&
 // colour mode, direct histogram, for transform xi. Walker state is
 // (x, y, ci, fuse). Returns ST_* flags; on ST_PLOT, (px, py) is the
 // in-frame pixel and `texel` the palette texel of the sample.
+template <bool EXT>
 __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& v, int pal_size,
                                           uint32_t burn_in, Rng& rng, float& x, float& y,
                                           float& ci, uint32_t& fuse, int xi, bool should_plot,
@@ -442,7 +464,7 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
   const int n = fl.n_xforms;
   const float* row = fl.rows + xi * XF_STRIDE;
   float nx = x, ny = y;
-  apply_xform(row, fl.ids, fl.n_vars, nx, ny, rng);
+  apply_xform<EXT>(row, fl.ids, fl.n_vars, nx, ny, rng);
   // Bad-value respawn + re-fuse (main_template.wgsl:397-479).
   if (!(fabsf(nx) <= 1e32f) || !(fabsf(ny) <= 1e32f)) {
     x = rng_nextf(rng) * 2.0f - 1.0f;
@@ -467,12 +489,12 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
     if (fl.hdr && fl.hdr->fin_uniform) {
       const int nc = __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain_n);
       for (int j = 0; j < nc; j++)
-        apply_final(fl, __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain[j]), fx, fy, rng);
+        apply_final<EXT>(fl, __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain[j]), fx, fy, rng);
     } else {
       const int nc = (int)row[XF_NFIN];
       for (int j = 0; j < nc; j++)
-        apply_xform(fl.rows + (n + (int)row[XF_FIN0 + j]) * XF_STRIDE, fl.ids, fl.n_vars, fx, fy,
-                    rng);
+        apply_xform<EXT>(fl.rows + (n + (int)row[XF_FIN0 + j]) * XF_STRIDE, fl.ids, fl.n_vars, fx,
+                         fy, rng);
     }
   }
   world_to_pixel(v, fx, fy, px, py);
@@ -484,14 +506,15 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
 }
 
 // One whole iteration of main_template.wgsl's loop body (:193-1383).
+template <bool EXT>
 __device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& v, int pal_size,
                                           uint32_t burn_in, Rng& rng, float& x, float& y,
                                           float& ci, uint32_t& fuse, int& prev, int& xi_out,
                                           int32_t& px, int32_t& py, int& texel) {
   bool should_plot;
-  const int xi = choose_xform(fl, rng, should_plot, prev);
+  const int xi = choose_xform<EXT>(fl, rng, should_plot, prev);
   xi_out = xi;
-  return step_xform(fl, v, pal_size, burn_in, rng, x, y, ci, fuse, xi, should_plot, px, py,
+  return step_xform<EXT>(fl, v, pal_size, burn_in, rng, x, y, ci, fuse, xi, should_plot, px, py,
                     texel);
 }
 
@@ -504,6 +527,7 @@ __device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& 
 // ((walker + seed) % refresh_period == 0); an orbit that starts with xaos
 // takes a first pick stratified over the walkers instead: the Kronecker
 // point (walker * 2654435769 + seed) >> 8, over 2^24.
+template <bool EXT>
 __device__ __forceinline__ void walker_begin(const FlameView& fl, const IterArgs& a,
                                              uint32_t walker, const float4* orbit, Rng& rng,
                                              float& x, float& y, float& ci, uint32_t& fuse,
@@ -514,7 +538,7 @@ __device__ __forceinline__ void walker_begin(const FlameView& fl, const IterArgs
   ci = 0.0f;
   fuse = a.burn_in;
   prev = 0;
-  if (fl.xcum) prev = select_const(fl, rng_nextf(rng));
+  if (EXT && fl.xcum) prev = select_const(fl, rng_nextf(rng));
   if (!orbit) return;
   const bool restart =
       a.fresh || (a.refresh_period != 0u && (walker + a.seed) % a.refresh_period == 0u);
@@ -524,18 +548,19 @@ __device__ __forceinline__ void walker_begin(const FlameView& fl, const IterArgs
     y = o.y;
     ci = o.z;
     const uint32_t w = __float_as_uint(o.w);
-    fuse = fl.xcum ? (w & 0xFFFFFFu) : w;
-    if (fl.xcum) prev = min((int)(w >> 24), fl.n_xforms - 1);  // a foreign word stays in the table
-  } else if (fl.xcum) {
+    fuse = EXT && fl.xcum ? (w & 0xFFFFFFu) : w;
+    if (EXT && fl.xcum) prev = min((int)(w >> 24), fl.n_xforms - 1);  // a foreign word stays in the table
+  } else if (EXT && fl.xcum) {
     const uint32_t k = walker * 2654435769u + a.seed;
     prev = select_const(fl, (float)(k >> 8) * (1.0f / 16777216.0f));
   }
 }
 
 // The orbit word that walker_begin reads back.
+template <bool EXT>
 __device__ __forceinline__ float4 walker_orbit(const FlameView& fl, float x, float y, float ci,
                                                uint32_t fuse, int prev) {
-  const uint32_t w = fl.xcum ? (fuse | (uint32_t)prev << 24) : fuse;
+  const uint32_t w = EXT && fl.xcum ? (fuse | (uint32_t)prev << 24) : fuse;
   return make_float4(x, y, ci, __uint_as_float(w));
 }
 

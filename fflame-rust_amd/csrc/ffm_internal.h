@@ -89,6 +89,7 @@ struct IterArgs {
                             // transform << 24); may be null
   int32_t fresh;
   uint32_t refresh_period;  // walkers with (walker + seed) % period == 0 restart (0: none)
+  int32_t ext;              // the flame has xaos or a variation beyond the workloads' thirteen
   void* hist;               // u32 or f32 words
   uint32_t hist_copy_words; // XCD mode: words per private copy
   unsigned long long* plotted;
@@ -193,6 +194,7 @@ struct BatchArgs {
   const uint32_t* pal_u32;
   int32_t max_rows, max_pal;
   uint32_t count_div;  // pre-pass walks 1/count_div of every item
+  int32_t ext;         // some flame uses a variation beyond the workloads' thirteen
   uint32_t* next_item; // work-stealing counter of this launch (zeroed before it)
   TileArgs g;          // global tile arrays; g.hist = base of the histograms
 };

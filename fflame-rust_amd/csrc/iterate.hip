@@ -26,7 +26,7 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned l
     __hip_atomic_fetch_add(dst, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int MODE>
+template <int MODE, bool EXT>
 __global__ __launch_bounds__(256) void iterate_kernel(IterArgs a) {
   extern __shared__ uint32_t lds[];
   LdsFlame L = load_lds<MODE == FFM_SPLAT_F32_GLOBAL>(a, lds);
@@ -45,12 +45,13 @@ __global__ __launch_bounds__(256) void iterate_kernel(IterArgs a) {
     float x, y, ci;
     uint32_t fuse;
     int prev;
-    walker_begin(L.fv, a, walker, a.orbits ? a.orbits + wi : nullptr, rng, x, y, ci, fuse, prev);
+    walker_begin<EXT>(L.fv, a, walker, a.orbits ? a.orbits + wi : nullptr, rng, x, y, ci, fuse,
+                      prev);
     for (uint32_t it = 0; it < a.ipt; it++) {
       int xi, texel = 0;
       int32_t px, py;
-      int st = chaos_step(L.fv, v, pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi, px, py,
-                          texel);
+      int st = chaos_step<EXT>(L.fv, v, pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi, px,
+                               py, texel);
       n_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
       if (st & ST_PLOT) {
         n_in++;
@@ -72,13 +73,14 @@ __global__ __launch_bounds__(256) void iterate_kernel(IterArgs a) {
         }
       }
     }
-    if (a.orbits) a.orbits[wi] = walker_orbit(L.fv, x, y, ci, fuse, prev);
+    if (a.orbits) a.orbits[wi] = walker_orbit<EXT>(L.fv, x, y, ci, fuse, prev);
   }
   wave_add_u64(a.plotted, n_plotted);
   wave_add_u64(a.inframe, n_in);
 }
 
 // Per-step records of one walker (the parity trace): same step function.
+template <bool EXT>
 __global__ void trace_kernel(IterArgs a, uint32_t walker, uint32_t steps, uint32_t* out) {
   extern __shared__ uint32_t lds[];
   LdsFlame L = load_lds<false>(a, lds);
@@ -87,12 +89,12 @@ __global__ void trace_kernel(IterArgs a, uint32_t walker, uint32_t steps, uint32
   float x, y, ci;
   uint32_t fuse;
   int prev;
-  walker_begin(L.fv, a, walker, nullptr, rng, x, y, ci, fuse, prev);
+  walker_begin<EXT>(L.fv, a, walker, nullptr, rng, x, y, ci, fuse, prev);
   for (uint32_t s = 0; s < steps; s++) {
     int xi, texel = 0;
     int32_t px, py;
-    int st = chaos_step(L.fv, a.view, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi, px,
-                        py, texel);
+    int st = chaos_step<EXT>(L.fv, a.view, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi,
+                             px, py, texel);
     uint32_t* o = out + 8 * (size_t)s;
     o[0] = rng.state;
     o[1] = (uint32_t)xi;
@@ -140,23 +142,29 @@ static hipError_t allow_lds(const void* fn) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_DYNAMIC_MAX);
 }
 
-hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block, hipStream_t s) {
+template <bool EXT>
+static hipError_t launch_iterate_t(const IterArgs& a, int splat_mode, int grid, int block,
+                                   hipStream_t s) {
   static const hipError_t e0 =
-      allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_U32_GLOBAL>));
-  static const hipError_t e1 = allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_U32_XCD>));
+      allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_U32_GLOBAL, EXT>));
+  static const hipError_t e1 =
+      allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_U32_XCD, EXT>));
   static const hipError_t e3 =
-      allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_F32_GLOBAL>));
+      allow_lds(reinterpret_cast<const void*>(&iterate_kernel<FFM_SPLAT_F32_GLOBAL, EXT>));
   if (e0 != hipSuccess || e1 != hipSuccess || e3 != hipSuccess) return hipErrorInvalidValue;
   size_t shm = lds_bytes(a, a.n_rows, splat_mode == FFM_SPLAT_F32_GLOBAL);
   switch (splat_mode) {
     case FFM_SPLAT_U32_GLOBAL:
-      hipLaunchKernelGGL(iterate_kernel<FFM_SPLAT_U32_GLOBAL>, dim3(grid), dim3(block), shm, s, a);
+      hipLaunchKernelGGL((iterate_kernel<FFM_SPLAT_U32_GLOBAL, EXT>), dim3(grid), dim3(block), shm,
+                         s, a);
       break;
     case FFM_SPLAT_U32_XCD:
-      hipLaunchKernelGGL(iterate_kernel<FFM_SPLAT_U32_XCD>, dim3(grid), dim3(block), shm, s, a);
+      hipLaunchKernelGGL((iterate_kernel<FFM_SPLAT_U32_XCD, EXT>), dim3(grid), dim3(block), shm,
+                         s, a);
       break;
     case FFM_SPLAT_F32_GLOBAL:
-      hipLaunchKernelGGL(iterate_kernel<FFM_SPLAT_F32_GLOBAL>, dim3(grid), dim3(block), shm, s, a);
+      hipLaunchKernelGGL((iterate_kernel<FFM_SPLAT_F32_GLOBAL, EXT>), dim3(grid), dim3(block), shm,
+                         s, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -164,13 +172,25 @@ hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block
   return hipGetLastError();
 }
 
-hipError_t launch_trace(const IterArgs& a, uint32_t walker, uint32_t steps, uint32_t* out,
-                        hipStream_t s) {
-  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&trace_kernel));
+hipError_t launch_iterate(const IterArgs& a, int splat_mode, int grid, int block, hipStream_t s) {
+  return a.ext ? launch_iterate_t<true>(a, splat_mode, grid, block, s)
+               : launch_iterate_t<false>(a, splat_mode, grid, block, s);
+}
+
+template <bool EXT>
+static hipError_t launch_trace_t(const IterArgs& a, uint32_t walker, uint32_t steps,
+                                 uint32_t* out, hipStream_t s) {
+  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&trace_kernel<EXT>));
   if (e0 != hipSuccess) return e0;
   size_t shm = lds_bytes(a, a.n_rows, false);
-  hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(64), shm, s, a, walker, steps, out);
+  hipLaunchKernelGGL(trace_kernel<EXT>, dim3(1), dim3(64), shm, s, a, walker, steps, out);
   return hipGetLastError();
+}
+
+hipError_t launch_trace(const IterArgs& a, uint32_t walker, uint32_t steps, uint32_t* out,
+                        hipStream_t s) {
+  return a.ext ? launch_trace_t<true>(a, walker, steps, out, s)
+               : launch_trace_t<false>(a, walker, steps, out, s);
 }
 
 hipError_t launch_fold_xcd(uint32_t* hist, const uint32_t* copies, size_t words, int ncopies,

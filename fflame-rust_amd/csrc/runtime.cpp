@@ -50,6 +50,7 @@ struct ffm_ctx {
   ffm_flame_desc flame;
   int n_rows = 0;
   uint32_t refresh_period = 0;  // xaos flames whose walk splits orbits: 4
+  bool ext = false;             // the flame needs the EXT kernels
   DevBuf table;
   // palette
   bool has_palette = false;
@@ -129,6 +130,21 @@ bool supported_var(int id) {
     default:
       return false;
   }
+}
+
+// Whether a flame needs the EXT kernels: xaos, or a variation beyond the
+// thirteen of the workloads (their kernels leave those paths out).
+bool flame_ext(const ffm_flame_desc* f) {
+  if (f->has_xaos) return true;
+  for (int k = 0; k < f->n_vars; k++)
+    switch (f->var_id[k]) {
+      case FFM_VAR_BLOB: case FFM_VAR_NOISE: case FFM_VAR_BLUR: case FFM_VAR_GAUSSIAN_BLUR:
+      case FFM_VAR_PRE_BLUR:
+        return true;
+      default:
+        break;
+    }
+  return false;
 }
 
 // Validation and derived values shared by every flame entry point.
@@ -485,6 +501,7 @@ IterArgs base_args(ffm_ctx* c, const ffm_view& v) {
   IterArgs a{};
   a.flame = reinterpret_cast<const float*>(c->table.p);
   a.n_rows = c->n_rows;
+  a.ext = c->ext ? 1 : 0;
   a.pal_u32 = reinterpret_cast<const uint32_t*>(c->pal_u32.p);
   a.pal_f32 = reinterpret_cast<const float*>(c->pal_f32.p);
   a.pal_size = c->pal_size;
@@ -770,6 +787,7 @@ int ffm_flame_set(ffm_ctx* c, const ffm_flame_desc* f) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->n_rows = reinterpret_cast<const FlameHeader*>(t.data())->table_rows;
   c->refresh_period = xaos_refresh_period(&c->flame);
+  c->ext = flame_ext(&c->flame);
   c->has_flame = true;
   c->t_planned = false;  // the attractor, hence the tile capacities, changed
   return FFM_OK;
@@ -1240,7 +1258,8 @@ std::vector<uint64_t> batch_signature(int n, const ffm_flame_desc* flames, const
     sig.push_back(((uint64_t)jobs[i].view.width << 32) | (uint32_t)jobs[i].view.height);
     sig.push_back(jobs[i].walkers);
     sig.push_back(((uint64_t)jobs[i].ipt << 32) | jobs[i].burn_in);
-    sig.push_back((uint64_t)(flames[i].n_xforms + flames[i].n_finals));
+    sig.push_back((uint64_t)(flames[i].n_xforms + flames[i].n_finals) |
+                  (uint64_t)flame_ext(&flames[i]) << 32);
   }
   return sig;
 }
@@ -1340,6 +1359,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.max_rows = max_rows;
   b.max_pal = BATCH_PALETTE;
   b.count_div = BATCH_COUNT_DIV;
+  for (int i = 0; i < n; i++) b.ext |= flame_ext(&flames[i]) ? 1 : 0;
   b.g.tile_fill = reinterpret_cast<uint32_t*>(P.fill.p);
   b.g.region = reinterpret_cast<uint32_t*>(P.region.p);
   b.g.cap = reinterpret_cast<uint32_t*>(P.cap.p);

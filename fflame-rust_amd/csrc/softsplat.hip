@@ -307,7 +307,7 @@ __device__ __forceinline__ int soft_step(const FlameView& fl, uint32_t burn_in, 
   const float* row = fl.rows + xi * XF_STRIDE;
   bool should_plot = rng_nextf(rng) < row[XF_OPACITY];
   T nx = x, ny = y;
-  if constexpr (sizeof(T) == 4) apply_xform(row, fl.ids, fl.n_vars, nx, ny, rng);
+  if constexpr (sizeof(T) == 4) apply_xform<false>(row, fl.ids, fl.n_vars, nx, ny, rng);
   else apply_xform_f64(row, fl.ids, fl.n_vars, nx, ny, rng);
   if (!(fabs(nx) <= (T)1e32) || !(fabs(ny) <= (T)1e32)) {
     x = (T)(rng_nextf(rng) * 2.0f - 1.0f);

@@ -160,7 +160,7 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
 // staging every in-frame sample (or, COUNT_ONLY, counting it per tile).
 // Rounds are uniform across the workgroup so every barrier is reached by all
 // threads; lanes past `limit` step nothing.
-template <bool COUNT_ONLY>
+template <bool COUNT_ONLY, bool EXT>
 __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame& F,
                             const BinLds& L, uint64_t base, uint64_t stride, uint64_t limit,
                             unsigned long long& n_plotted, unsigned long long& n_in) {
@@ -179,15 +179,15 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
     float x, y, ci;
     uint32_t fuse;
     int prev;
-    walker_begin(F.fv, a, walker, active && a.orbits ? a.orbits + wi : nullptr, rng, x, y, ci,
-                 fuse, prev);
+    walker_begin<EXT>(F.fv, a, walker, active && a.orbits ? a.orbits + wi : nullptr, rng, x, y,
+                      ci, fuse, prev);
     uint32_t r_plotted = 0u, r_in = 0u;  // this round's counts (at most ipt each)
     for (uint32_t it = 0; it < a.ipt; it++) {
       if (active) {
         int xi, texel = 0;
         int32_t px, py;
-        int st = chaos_step(F.fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi, px,
-                            py, texel);
+        int st = chaos_step<EXT>(F.fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi,
+                                 px, py, texel);
         r_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
         if (st & ST_PLOT) {
           r_in++;
@@ -221,7 +221,8 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
     }
     n_plotted += r_plotted;
     n_in += r_in;
-    if (!COUNT_ONLY && active && a.orbits) a.orbits[wi] = walker_orbit(F.fv, x, y, ci, fuse, prev);
+    if (!COUNT_ONLY && active && a.orbits)
+      a.orbits[wi] = walker_orbit<EXT>(F.fv, x, y, ci, fuse, prev);
   }
 }
 
@@ -256,7 +257,7 @@ __device__ void finish_walkers(const TileArgs& t, const BinLds& L, const uint32_
 }
 
 // One dispatch of one flame: walkers grid-strided over the whole grid.
-template <bool COUNT_ONLY>
+template <bool COUNT_ONLY, bool EXT>
 __global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, TileArgs t) {
   extern __shared__ uint32_t lds[];
   LdsFlame F = load_lds<false>(a, lds);
@@ -266,8 +267,8 @@ __global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, Tile
   if (tid == 0) *L.fill = 0u;
   __syncthreads();
   unsigned long long n_plotted = 0, n_in = 0;
-  run_walkers<COUNT_ONLY>(a, t, F, L, (uint64_t)blockIdx.x * blockDim.x,
-                          (uint64_t)gridDim.x * blockDim.x, a.walkers, n_plotted, n_in);
+  run_walkers<COUNT_ONLY, EXT>(a, t, F, L, (uint64_t)blockIdx.x * blockDim.x,
+                               (uint64_t)gridDim.x * blockDim.x, a.walkers, n_plotted, n_in);
   finish_walkers<COUNT_ONLY>(t, L, F.pal_u32);
   if (!COUNT_ONLY) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -534,7 +535,7 @@ __device__ __forceinline__ TileArgs flame_tiles(const BatchArgs& b, const BatchF
 // Many flames in one persistent launch: each workgroup takes work items
 // (flame, walker range), loads that flame's table and palette into LDS and
 // walks the range with the same staging and flushes as one flame.
-template <bool COUNT_ONLY>
+template <bool COUNT_ONLY, bool EXT>
 __global__ __launch_bounds__(BIN_BLOCK) void bin_batch_kernel(BatchArgs b) {
   extern __shared__ uint32_t lds[];
   const BinLds L = bin_lds_views(lds + flame_table_floats(b.max_rows) + b.max_pal);
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void bin_batch_kernel(BatchArgs b) {
     const TileArgs t = flame_tiles(b, f);
     uint64_t limit = it.w1;
     if (COUNT_ONLY) limit = it.w0 + max((it.w1 - it.w0) / b.count_div, 1u);
-    run_walkers<COUNT_ONLY>(a, t, F, L, it.w0, blockDim.x, limit, n_plotted, n_in);
+    run_walkers<COUNT_ONLY, EXT>(a, t, F, L, it.w0, blockDim.x, limit, n_plotted, n_in);
     finish_walkers<COUNT_ONLY>(t, L, F.pal_u32);
     __syncthreads();  // the next item reloads the flame table
   }
@@ -607,31 +608,43 @@ static hipError_t allow_lds(const void* fn) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_DYNAMIC_MAX);
 }
 
+template <bool COUNT_ONLY, bool EXT>
+static hipError_t launch_bin_iterate_t(const IterArgs& a, const TileArgs& t, int grid,
+                                       hipStream_t s) {
+  static const hipError_t e =
+      allow_lds(reinterpret_cast<const void*>(&bin_iterate_kernel<COUNT_ONLY, EXT>));
+  if (e != hipSuccess) return e;
+  const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size);
+  hipLaunchKernelGGL((bin_iterate_kernel<COUNT_ONLY, EXT>), dim3(grid), dim3(BIN_BLOCK), shm, s,
+                     a, t);
+  return hipGetLastError();
+}
+
 hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_only, int grid,
                               hipStream_t s) {
-  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&bin_iterate_kernel<true>));
-  static const hipError_t e1 = allow_lds(reinterpret_cast<const void*>(&bin_iterate_kernel<false>));
-  if (e0 != hipSuccess) return e0;
-  if (e1 != hipSuccess) return e1;
-  const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size);
   if (count_only)
-    hipLaunchKernelGGL(bin_iterate_kernel<true>, dim3(grid), dim3(BIN_BLOCK), shm, s, a, t);
-  else
-    hipLaunchKernelGGL(bin_iterate_kernel<false>, dim3(grid), dim3(BIN_BLOCK), shm, s, a, t);
+    return a.ext ? launch_bin_iterate_t<true, true>(a, t, grid, s)
+                 : launch_bin_iterate_t<true, false>(a, t, grid, s);
+  return a.ext ? launch_bin_iterate_t<false, true>(a, t, grid, s)
+               : launch_bin_iterate_t<false, false>(a, t, grid, s);
+}
+
+template <bool COUNT_ONLY, bool EXT>
+static hipError_t launch_bin_batch_t(const BatchArgs& b, int grid, hipStream_t s) {
+  static const hipError_t e =
+      allow_lds(reinterpret_cast<const void*>(&bin_batch_kernel<COUNT_ONLY, EXT>));
+  if (e != hipSuccess) return e;
+  const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal);
+  hipLaunchKernelGGL((bin_batch_kernel<COUNT_ONLY, EXT>), dim3(grid), dim3(BIN_BLOCK), shm, s, b);
   return hipGetLastError();
 }
 
 hipError_t launch_bin_batch(const BatchArgs& b, bool count_only, int grid, hipStream_t s) {
-  static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&bin_batch_kernel<true>));
-  static const hipError_t e1 = allow_lds(reinterpret_cast<const void*>(&bin_batch_kernel<false>));
-  if (e0 != hipSuccess) return e0;
-  if (e1 != hipSuccess) return e1;
-  const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal);
   if (count_only)
-    hipLaunchKernelGGL(bin_batch_kernel<true>, dim3(grid), dim3(BIN_BLOCK), shm, s, b);
-  else
-    hipLaunchKernelGGL(bin_batch_kernel<false>, dim3(grid), dim3(BIN_BLOCK), shm, s, b);
-  return hipGetLastError();
+    return b.ext ? launch_bin_batch_t<true, true>(b, grid, s)
+                 : launch_bin_batch_t<true, false>(b, grid, s);
+  return b.ext ? launch_bin_batch_t<false, true>(b, grid, s)
+               : launch_bin_batch_t<false, false>(b, grid, s);
 }
 
 hipError_t launch_tile_capacity_n(const TileArgs& t, float scale, uint32_t margin,
