@@ -599,6 +599,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   t.width = W;
   t.height = H;
   t.flush_every = 4;
+  t.stage = bin_stage(a.n_rows, a.pal_size);
   const int grid = std::max(c->num_cu, 1);
   const bool same = c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
                     c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
@@ -1368,6 +1369,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.g.hist = reinterpret_cast<uint32_t*>(P.hist.p);
   b.g.n_tiles = P.n_tiles;
   b.g.flush_every = flush_every;
+  b.g.stage = bin_stage(max_rows, BATCH_PALETTE);
   const int grid = std::max(c->num_cu, 1);
   const BatchItem* items_d = reinterpret_cast<const BatchItem*>(P.items.p);
   HIPCHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

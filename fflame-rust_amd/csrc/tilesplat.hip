@@ -29,14 +29,28 @@
 namespace ffm {
 
 constexpr int BIN_BLOCK = 1024;
-constexpr int STAGE = 10240;  // staged records per workgroup
+// Staged records per workgroup: what the LDS left over by the flame table
+// and palette holds, in steps of 1024, between these bounds (the upper one
+// keeps record indices in the u16 permutation).
+constexpr int STAGE_MIN = 10240, STAGE_MAX = 16384;
 constexpr int MAX_FLUSH_EVERY = 8;  // iterations between fill checks, at most
 constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
 
+constexpr size_t STAGE_FIXED = (size_t)TILE_MAX * 8 + 16 * 4 + 16;  // cnt, gbase, wsum, fill
+constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4 + 2;                  // rec, tr, perm
+
+int bin_stage(int n_rows, int pal_size) {
+  const size_t flame = flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4;
+  const size_t used = flame + STAGE_FIXED;
+  const size_t fit = used < (size_t)LDS_DYNAMIC_MAX
+                         ? ((size_t)LDS_DYNAMIC_MAX - used) / STAGE_BYTES_PER_RECORD / 1024 * 1024
+                         : 0;
+  return (int)std::min<size_t>(std::max<size_t>(fit, STAGE_MIN), STAGE_MAX);
+}
+
 size_t bin_lds_bytes(int n_rows, int pal_size) {
-  size_t flame = flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4;
-  size_t stage = (size_t)STAGE * (4 + 4 + 2) + (size_t)TILE_MAX * 8 + 16 * 4 + 16;
-  return flame + stage;
+  return flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4 + STAGE_FIXED +
+         (size_t)bin_stage(n_rows, pal_size) * STAGE_BYTES_PER_RECORD;
 }
 
 size_t accum_lds_bytes(int pal_size, bool f32) {
@@ -73,9 +87,9 @@ __device__ void block_exclusive_scan(uint32_t* a, int n, uint32_t* wave_sums, ui
 }
 
 struct BinLds {
-  uint32_t* rec;    // STAGE
-  uint32_t* tr;     // STAGE: tile | rank << 16
-  uint16_t* perm;   // STAGE
+  uint32_t* rec;    // stage
+  uint32_t* tr;     // stage: tile | rank << 16
+  uint16_t* perm;   // stage
   uint32_t* cnt;    // TILE_MAX
   uint32_t* gbase;  // TILE_MAX
   uint32_t* wsum;   // 16
@@ -119,7 +133,7 @@ __device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t
 // Called by every thread of the workgroup.
 __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* pal) {
   const int tid = threadIdx.x;
-  const int n = (int)min(*L.fill, (uint32_t)STAGE);
+  const int n = (int)min(*L.fill, (uint32_t)t.stage);
   for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
   __syncthreads();
   for (int j = tid; j < n; j += blockDim.x) {
@@ -168,9 +182,9 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
   const ViewConst v = a.view;
   const uint64_t rounds = limit > base ? (limit - base + stride - 1) / stride : 0;
   // a fill check every `every` iterations; at most `every` x blockDim
-  // records arrive between checks, so flushing at STAGE - that never overflows
+  // records arrive between checks, so flushing at stage - that never overflows
   const uint32_t every = (uint32_t)min(max(t.flush_every, 1), MAX_FLUSH_EVERY);
-  const uint32_t flush_at = (uint32_t)STAGE - every * blockDim.x;
+  const uint32_t flush_at = (uint32_t)t.stage - every * blockDim.x;
   for (uint64_t rd = 0; rd < rounds; rd++) {
     const uint64_t wi = base + rd * stride + tid;
     const bool active = wi < limit;
@@ -226,12 +240,12 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
   }
 }
 
-__device__ __forceinline__ BinLds bin_lds_views(uint32_t* p) {
+__device__ __forceinline__ BinLds bin_lds_views(uint32_t* p, int stage) {
   BinLds L;
   L.rec = p;
-  L.tr = p + STAGE;
-  L.perm = reinterpret_cast<uint16_t*>(p + 2 * STAGE);
-  L.cnt = p + 2 * STAGE + STAGE / 2;
+  L.tr = p + stage;
+  L.perm = reinterpret_cast<uint16_t*>(p + 2 * stage);
+  L.cnt = p + 2 * stage + stage / 2;
   L.gbase = L.cnt + TILE_MAX;
   L.wsum = L.gbase + TILE_MAX;
   L.fill = L.wsum + 16;
@@ -261,7 +275,7 @@ template <bool COUNT_ONLY, bool EXT>
 __global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, TileArgs t) {
   extern __shared__ uint32_t lds[];
   LdsFlame F = load_lds<false>(a, lds);
-  const BinLds L = bin_lds_views(lds + flame_table_floats(a.n_rows) + a.pal_size);
+  const BinLds L = bin_lds_views(lds + flame_table_floats(a.n_rows) + a.pal_size, t.stage);
   const int tid = threadIdx.x;
   for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
   if (tid == 0) *L.fill = 0u;
@@ -538,7 +552,7 @@ __device__ __forceinline__ TileArgs flame_tiles(const BatchArgs& b, const BatchF
 template <bool COUNT_ONLY, bool EXT>
 __global__ __launch_bounds__(BIN_BLOCK) void bin_batch_kernel(BatchArgs b) {
   extern __shared__ uint32_t lds[];
-  const BinLds L = bin_lds_views(lds + flame_table_floats(b.max_rows) + b.max_pal);
+  const BinLds L = bin_lds_views(lds + flame_table_floats(b.max_rows) + b.max_pal, b.g.stage);
   const int tid = threadIdx.x;
   for (int i = tid; i < TILE_MAX; i += blockDim.x) L.cnt[i] = 0u;
   if (tid == 0) *L.fill = 0u;
