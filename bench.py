@@ -21,26 +21,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fflame-rust_amd"))
 
 
-def cpu_baseline(fd, pal, job, seconds=10.0):
+def cpu_baseline(fd, pal, job, seconds=10.0, single_seconds=4.0):
     """The oracle (a scalar C restatement of the reference's step) timed on
     this host's cores: calibrate on a small sample, then run about
-    `seconds` of work."""
+    `seconds` of work on all cores and `single_seconds` on one. Beside it,
+    the reference's Criterion microbenchmarks (benches/flame_bench.rs)
+    restated in C, with the per-op means the reference publishes."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
     threads = max(1, min(os.cpu_count() or 1, 16))
     v = job.view
-    n = 1 << 16
-    t0 = time.perf_counter()
-    O.iterate_mt(fd, pal, v, n, job.ipt, job.burn_in, 1, threads)
-    dt = time.perf_counter() - t0
-    n = int(max(1 << 16, min(1 << 24, n * seconds / max(dt, 1e-3))))
-    n = (n // threads) * threads
-    t0 = time.perf_counter()
-    O.iterate_mt(fd, pal, v, n, job.ipt, job.burn_in, 1, threads)
-    dt = time.perf_counter() - t0
+
+    def timed(threads, seconds):
+        n = 1 << 12
+        t0 = time.perf_counter()
+        O.iterate_mt(fd, pal, v, n * threads, job.ipt, job.burn_in, 1, threads)
+        dt = time.perf_counter() - t0
+        n = int(max(1 << 12, min(1 << 24, n * threads * seconds / max(dt, 1e-3))))
+        n = max(threads, (n // threads) * threads)
+        t0 = time.perf_counter()
+        O.iterate_mt(fd, pal, v, n, job.ipt, job.burn_in, 1, threads)
+        return n, time.perf_counter() - t0
+
+    n, dt = timed(threads, seconds)
+    n1, dt1 = timed(1, single_seconds)
     return {"value": n * job.ipt / dt, "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": f"{n} walkers x {job.ipt} iterations of the same flame at "
-                      f"{v.width}x{v.height} (iterate + splat), {dt:.1f} s"}
+                      f"{v.width}x{v.height} (iterate + splat), {dt:.1f} s",
+            "single_core": {"value": n1 * job.ipt / dt1, "unit": "samples/s", "cores": 1,
+                            "sample": f"{n1} walkers x {job.ipt} iterations, {dt1:.1f} s"},
+            "microbench_ns": O.microbench_suite()}
 
 
 def recorded_traffic(kernel, splat_mode, walkers):

@@ -131,6 +131,24 @@ int orc_soft_adjoint(const ffm_flame_desc* f, const double* theta, const ffm_sof
 /* One variation's value V[2] and Jacobian J[4] = [dVx/dx, dVx/dy, dVy/dx, dVy/dy]. */
 int orc_soft_variation(int id, const float* params, double x, double y, double* V, double* J);
 
+/* --- CPU microbenchmarks of benches/flame_bench.rs (flame_bench.c) --- */
+enum {
+  ORC_MB_AFFINE_2X3 = 0,
+  ORC_MB_AFFINE_FULL = 1,
+  ORC_MB_R_SQUARED = 2,
+  ORC_MB_R = 3,
+  ORC_MB_THETA = 4,
+  ORC_MB_PHI = 5,
+  ORC_MB_PALETTE_POSITION = 6,
+  ORC_MB_SINGLE_TRANSFORM_ITER = 7
+};
+/* ns per iteration of benchmark `which`. params: a b c d e f, x y (the
+ * point), speed, colour, then g (affine_full) or linear weight + has-linear
+ * flag (single_transform_iter). iters = 0 calibrates to min_seconds.
+ * state_out (3 floats, may be NULL): the last values computed. */
+double orc_microbench(int which, const float* params, uint64_t iters, double min_seconds,
+                      float* state_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,7 @@ def lib():
             "orc_soft_adjoint": (C.c_int, [C.POINTER(FlameDesc), _dp, C.POINTER(SoftJob), _dp,
                                            _dp, C.c_int]),
             "orc_soft_variation": (C.c_int, [C.c_int, _fp, C.c_double, C.c_double, _dp, _dp]),
+            "orc_microbench": (C.c_double, [C.c_int, _fp, C.c_uint64, C.c_double, _fp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -311,3 +312,52 @@ def soft_variation(var_id, x, y, params=(0.0, 0.0, 0.0, 0.0)):
     lib().orc_soft_variation(var_id, p, x, y, _dptr(V), _dptr(J))
     return V, J.reshape(2, 2)
 
+
+
+# ------------------------------------------------ the reference's CPU benches
+
+# The first transform of the first three presets (assets/presets.fae):
+# a b c d e f, colour, colour speed, linear weight (None: no linear variation).
+PRESET_FIRST_XFORMS = {
+    "Grand": (1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, None),
+    "Julian": (0.87169206, 0.38354817, -0.38360286, 0.8721385, 0.0, 0.0, 0.0, 0.9, None),
+    "Plastic": (1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, -0.8, None),
+}
+
+# (Criterion id, benchmark, params, the reference's published mean in ns on
+# one core: benchmark_results/last_run.json:5-75). Inputs as in
+# benches/flame_bench.rs: affine a..g = 0.7 0.2 -0.2 0.8 0.1 -0.1 0.05 at
+# (0.5, 0.5); point helpers at (0.7, 0.3); palette position 0.3, colour 0.7.
+_AFF = (0.7, 0.2, -0.2, 0.8, 0.1, -0.1)
+MICROBENCHES = [
+    ("affine_transform/affine_2x3_matrix", 0, _AFF + (0.5, 0.5, 0, 0, 0.05), 2.7385),
+    ("affine_transform/affine_full", 1, _AFF + (0.5, 0.5, 0, 0, 0.05), 3.1707),
+    ("point_helpers/r_squared", 2, (0,) * 6 + (0.7, 0.3), 0.66151),
+    ("point_helpers/r", 3, (0,) * 6 + (0.7, 0.3), 0.74991),
+    ("point_helpers/theta", 4, (0,) * 6 + (0.7, 0.3), 10.223),
+    ("point_helpers/phi", 5, (0,) * 6 + (0.7, 0.3), 11.307),
+    ("color_blending/palette_position/0", 6, (0,) * 6 + (0.3, 0.7, 0.0), 0.63141),
+    ("color_blending/palette_position/0.5", 6, (0,) * 6 + (0.3, 0.7, 0.5), 0.64191),
+    ("color_blending/palette_position/1", 6, (0,) * 6 + (0.3, 0.7, 1.0), 0.62891),
+] + [
+    (f"preset_iteration/single_transform_iter/{k}", 7,
+     v[:6] + (0.5, 0.5, v[7], v[6], v[8] or 0.0, 0.0 if v[8] is None else 1.0), pub)
+    for k, pub in (("Grand", 19.716), ("Julian", 162.53), ("Plastic", 20.303))
+    for v in (PRESET_FIRST_XFORMS[k],)
+]
+
+
+def microbench(which, params, iters=0, min_seconds=0.1):
+    """ns per iteration of one restated Criterion benchmark, and the last
+    (x, y, colour) values it computed."""
+    p = np.zeros(12, np.float32)
+    p[:len(params)] = params
+    st = np.zeros(3, np.float32)
+    ns = lib().orc_microbench(which, fp(p), iters, min_seconds, fp(st))
+    return ns, st
+
+
+def microbench_suite(min_seconds=0.1):
+    """Every restated benchmark: {id: {"ns": ours, "published_ns": reference}}."""
+    return {name: {"ns": round(microbench(w, p, 0, min_seconds)[0], 4), "published_ns": pub}
+            for name, w, p, pub in MICROBENCHES}
