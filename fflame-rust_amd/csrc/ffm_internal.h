@@ -170,6 +170,16 @@ struct TonemapConst {
 hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H,
                           const TonemapConst& tc, hipStream_t s);
 
+// ---- the tail's effects (effects.hip)
+struct FxConst {
+  int32_t type;                       // ffm_effect_type
+  float p[FFM_MAX_EFFECT_PARAMS];     // the effect's parameters
+  int32_t W, H;
+  float density_mean, pixel_scale;
+};
+hipError_t launch_density_effect(const FxConst& k, const float* in, float* out, hipStream_t s);
+hipError_t launch_color_effect(const FxConst& k, const uint8_t* in, uint8_t* out, hipStream_t s);
+
 // ---- batched gallery: many flames in one persistent launch
 
 struct BatchFlame {

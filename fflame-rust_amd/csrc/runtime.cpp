@@ -58,6 +58,8 @@ struct ffm_ctx {
   DevBuf pal_u32, pal_f32;
   // render buffers
   DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv, count;
+  DevBuf fx_a, fx_b, fx_rgba;    // effect chain ping-pong buffers
+  ffm_effects fx{};              // effects renders apply (none by default)
   // LDS tile splat: per-tile fill/region/capacity, work list, record buffer
   DevBuf t_fill, t_region, t_cap, t_items, t_records, t_sorted;
   int t_W = 0, t_H = 0, t_nrows = 0, t_pal = 0;
@@ -736,6 +738,7 @@ int ffm_ctx_create(int device, ffm_ctx** out) {
   if (device < 0 || device >= n) return FFM_ERR_INVALID_ARGUMENT;
   if (hipSetDevice(device) != hipSuccess) return FFM_ERR_DEVICE;
   ffm_ctx* c = new ffm_ctx();
+  c->fx.pixel_scale = 1.0f;
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
@@ -954,6 +957,175 @@ int ffm_tonemap(ffm_ctx* c, const float* accum, uint8_t* rgba, int width, int he
   return FFM_OK;
 }
 
+// ---- the tail's effects ---------------------------------------------------
+
+// 1: a density effect, 2: a colour effect, 0: not an effect this path runs.
+static int fx_chain(int32_t type) {
+  switch (type) {
+    case FFM_FX_DENSITY_BLUR:
+    case FFM_FX_BILATERAL_BLUR:
+    case FFM_FX_SHARPEN:
+      return 1;
+    case FFM_FX_VIGNETTE:
+    case FFM_FX_FILM_GRAIN:
+    case FFM_FX_CHROMATIC_ABERRATION:
+    case FFM_FX_HUE_SHIFT:
+    case FFM_FX_SOBEL_EDGES:
+      return 2;
+    default:
+      return 0;
+  }
+}
+
+// The Rhai PARAMS defaults, in declaration order (assets/effects/*.rhai).
+int ffm_effect_defaults(int32_t type, ffm_effect* out) {
+  if (!out) return FFM_ERR_INVALID_ARGUMENT;
+  ffm_effect e{};
+  e.type = type;
+  e.enabled = 1;
+  std::vector<float> v;
+  switch (type) {
+    case FFM_FX_DENSITY_BLUR: v = {3.0f, 0.25f, 0.5f}; break;          // density_blur.rhai:10-19
+    case FFM_FX_BILATERAL_BLUR: v = {3.0f, 3.0f, 0.1f}; break;         // bilateral_blur.rhai
+    case FFM_FX_SHARPEN: v = {1.0f, 1.0f}; break;                      // sharpen.rhai
+    case FFM_FX_VIGNETTE: v = {0.5f, 0.5f, 0.5f, 0.0f}; break;         // vignette.rhai:9-18
+    case FFM_FX_FILM_GRAIN: v = {0.1f, 0.0f, 4.0f}; break;             // film_grain.rhai
+    case FFM_FX_CHROMATIC_ABERRATION: v = {2.0f, 1.0f, 1.0f, 0.0f}; break;
+    case FFM_FX_HUE_SHIFT: v = {0.0f, 1.0f, 0.0f}; break;              // hue_shift.rhai
+    case FFM_FX_SOBEL_EDGES: v = {1.0f, 0.1f, 0.5f, 0.5f, 1.0f}; break;  // sobel_edges.rhai
+    default: return FFM_ERR_INVALID_ARGUMENT;
+  }
+  for (size_t i = 0; i < v.size(); i++) e.params[i] = v[i];
+  *out = e;
+  return FFM_OK;
+}
+
+// sharpen's tap loop spans ceil(radius * pixel_scale) pixels each way; its
+// parameter range is 0.5..5 (sharpen.rhai), and a radius past 64 pixels
+// is refused rather than run as a near-endless loop.
+constexpr float FX_MAX_SHARPEN_RADIUS = 64.0f;
+static int check_effect_params(ffm_ctx* c, const ffm_effect& e, float pixel_scale) {
+  if (e.type == FFM_FX_SHARPEN && !(e.params[1] * pixel_scale <= FX_MAX_SHARPEN_RADIUS))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "sharpen radius %g x pixel scale %g: at most %g",
+                e.params[1], pixel_scale, FX_MAX_SHARPEN_RADIUS);
+  return FFM_OK;
+}
+
+static int check_effects(ffm_ctx* c, const ffm_effects& fx) {
+  if (fx.n_density < 0 || fx.n_density > FFM_MAX_EFFECTS || fx.n_color < 0 ||
+      fx.n_color > FFM_MAX_EFFECTS)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "effect counts %d / %d outside 0..%d", fx.n_density,
+                fx.n_color, FFM_MAX_EFFECTS);
+  if (!(fx.pixel_scale > 0.0f) || !std::isfinite(fx.pixel_scale))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel_scale must be positive");
+  for (int i = 0; i < fx.n_density; i++) {
+    if (!fx.density[i].enabled) continue;
+    if (fx_chain(fx.density[i].type) != 1)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "density effect %d: type %d is not a density effect",
+                  i, fx.density[i].type);
+    if (int st = check_effect_params(c, fx.density[i], fx.pixel_scale)) return st;
+  }
+  for (int i = 0; i < fx.n_color; i++)
+    if (fx.color[i].enabled && fx_chain(fx.color[i].type) != 2)
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "colour effect %d: type %d is not a colour effect",
+                  i, fx.color[i].type);
+  return FFM_OK;
+}
+
+int ffm_effects_set(ffm_ctx* c, const ffm_effects* fx) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!fx) {
+    c->fx = ffm_effects{};
+    c->fx.pixel_scale = 1.0f;
+    return FFM_OK;
+  }
+  int st = check_effects(c, *fx);
+  if (st) return st;
+  c->fx = *fx;
+  return FFM_OK;
+}
+
+static FxConst fx_const(const ffm_effect& e, int W, int H, float density_mean, float pixel_scale) {
+  FxConst k{};
+  k.type = e.type;
+  for (int i = 0; i < FFM_MAX_EFFECT_PARAMS; i++) k.p[i] = e.params[i];
+  k.W = W;
+  k.H = H;
+  k.density_mean = density_mean;
+  k.pixel_scale = pixel_scale;
+  return k;
+}
+
+int ffm_density_effect(ffm_ctx* c, const ffm_effect* fx, const float* in, float* out, int width,
+                       int height, float density_mean, float pixel_scale, void* stream) {
+  if (!c || !fx || !in || !out || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
+  if (in == out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "an effect reads and writes different buffers");
+  if (fx_chain(fx->type) != 1)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "type %d is not a density effect", fx->type);
+  if (!(pixel_scale > 0.0f)) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel_scale must be positive");
+  if (int st = check_effect_params(c, *fx, pixel_scale)) return st;
+  (void)hipSetDevice(c->device);
+  HIPCHECK(c, launch_density_effect(fx_const(*fx, width, height, density_mean, pixel_scale), in,
+                                    out, pick(c, stream)));
+  return FFM_OK;
+}
+
+int ffm_color_effect(ffm_ctx* c, const ffm_effect* fx, const uint8_t* in, uint8_t* out, int width,
+                     int height, float pixel_scale, void* stream) {
+  if (!c || !fx || !in || !out || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
+  if (in == out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "an effect reads and writes different buffers");
+  if (fx_chain(fx->type) != 2)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "type %d is not a colour effect", fx->type);
+  if (!(pixel_scale > 0.0f)) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel_scale must be positive");
+  (void)hipSetDevice(c->device);
+  HIPCHECK(c, launch_color_effect(fx_const(*fx, width, height, 0.0f, pixel_scale), in, out,
+                                  pick(c, stream)));
+  return FFM_OK;
+}
+
+// The density chain over `acc` (each enabled effect reads the previous
+// one's output; run_density_effects, src/renderer/effect_chain.rs:737-829):
+// returns the buffer the tone map reads.
+static int run_density_chain(ffm_ctx* c, const ffm_effects& fx, float* acc, int W, int H,
+                             float density_mean, hipStream_t s, float** out) {
+  *out = acc;
+  const size_t npix = (size_t)W * H;
+  float* cur = acc;
+  int st;
+  for (int i = 0; i < fx.n_density; i++) {
+    if (!fx.density[i].enabled) continue;
+    if ((st = ensure(c, c->fx_a, npix * 16, "effect buffer")) ||
+        (st = ensure(c, c->fx_b, npix * 16, "effect buffer")))
+      return st;
+    float* a = reinterpret_cast<float*>(c->fx_a.p);
+    float* dst = cur == a ? reinterpret_cast<float*>(c->fx_b.p) : a;
+    HIPCHECK(c, launch_density_effect(fx_const(fx.density[i], W, H, density_mean, fx.pixel_scale),
+                                      cur, dst, s));
+    cur = dst;
+  }
+  *out = cur;
+  return FFM_OK;
+}
+
+// The colour chain on the tone-mapped pixels `pix` (run_color_effects,
+// src/renderer/effect_chain.rs:874-962), ending in `pix`.
+static int run_color_chain(ffm_ctx* c, const ffm_effects& fx, uint8_t* pix, int W, int H,
+                           hipStream_t s) {
+  const size_t npix = (size_t)W * H;
+  uint8_t* cur = pix;
+  int st;
+  for (int i = 0; i < fx.n_color; i++) {
+    if (!fx.color[i].enabled) continue;
+    if ((st = ensure(c, c->fx_rgba, npix * 4, "effect pixels"))) return st;
+    uint8_t* dst = cur == pix ? reinterpret_cast<uint8_t*>(c->fx_rgba.p) : pix;
+    HIPCHECK(c, launch_color_effect(fx_const(fx.color[i], W, H, 0.0f, fx.pixel_scale), cur, dst,
+                                    s));
+    cur = dst;
+  }
+  if (cur != pix) HIPCHECK(c, hipMemcpyAsync(pix, cur, npix * 4, hipMemcpyDeviceToDevice, s));
+  return FFM_OK;
+}
+
 // render_with (src/renderer/render.rs:483-800): dispatch until the target,
 // accumulate every batch_size dispatches (blur first when asked) and once
 // more for a partial batch, stop early after burn_in/ipt + 4 empty
@@ -973,7 +1145,8 @@ static uint64_t exact_count_after() {
 }
 
 static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float* accum_out,
-                       ffm_render_stats* stats, uint8_t* rgba_dev = nullptr) {
+                       ffm_render_stats* stats, uint8_t* rgba_dev = nullptr,
+                       const ffm_effects* fx = nullptr) {
   if (!c) return FFM_ERR_INVALID_ARGUMENT;
   if (!job) return fail(c, FFM_ERR_INVALID_ARGUMENT, "job is NULL");
   if (!c->has_flame) return fail(c, FFM_ERR_NO_FLAME, "no flame set");
@@ -1105,18 +1278,25 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     if (st) return st;
     final_acc = reinterpret_cast<float*>(c->accum2.p);
   }
+  const ffm_effects& effects = fx ? *fx : c->fx;
+  const TonemapConst tmc = make_tonemap(job->tonemap, W, H, job->view.zoom, (double)samples_in_buffer);
+  // density effects measure a pixel's hit count in multiples of the mean
+  // density: the tone map's sample density (FlameRenderer::mean_density)
+  if ((st = run_density_chain(c, effects, final_acc, W, H, tmc.sample_density, s, &final_acc)))
+    return st;
   if (accum_out) {
     HIPCHECK(c, hipMemcpyAsync(accum_out, final_acc, npix * 16, hipMemcpyDeviceToHost, s));
   }
   if (rgba_out || rgba_dev) {
     if (!rgba_dev && (st = ensure(c, c->rgba, npix * 4, "pixels"))) return st;
     uint8_t* pix = rgba_dev ? rgba_dev : reinterpret_cast<uint8_t*>(c->rgba.p);
-    TonemapConst t = make_tonemap(job->tonemap, W, H, job->view.zoom, (double)samples_in_buffer);
+    const TonemapConst& t = tmc;
     if (job->tonemap.tonemap_mode < 0 || job->tonemap.tonemap_mode > 2 ||
         job->tonemap.highlight_mode < 0 || job->tonemap.highlight_mode > 3 ||
         job->tonemap.transparent_mode < 0 || job->tonemap.transparent_mode > 2)
       return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode not supported");
     HIPCHECK(c, launch_tonemap(final_acc, pix, W, H, t, s));
+    if ((st = run_color_chain(c, effects, pix, W, H, s))) return st;
     if (rgba_out) {
       hipError_t e = hipMemcpyAsync(rgba_out, pix, npix * 4, hipMemcpyDeviceToHost, s);
       if (e != hipSuccess)
@@ -1159,7 +1339,8 @@ int ffm_render_config(ffm_ctx* c, const char* text, size_t len, int width, int h
   if ((st = ffm_palette_set_xf(c, &sc->stops[0][0], sc->n_stops, sc->palette_size,
                                &sc->palette_xf)))
     return st;
-  return render_impl(c, &job, rgba_out, nullptr, stats);
+  if ((st = check_effects(c, sc->effects))) return st;
+  return render_impl(c, &job, rgba_out, nullptr, stats, nullptr, &sc->effects);
 }
 
 int ffm_render_to_device(ffm_ctx* c, const ffm_job* job, void* rgba_dev, ffm_render_stats* stats) {

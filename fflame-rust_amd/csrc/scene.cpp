@@ -677,6 +677,7 @@ struct SceneIn {
   bool levels_enabled = true;
   float levels_low = 0, levels_high = 10, levels_gamma = 0.5f;
   bool deterministic_rng = false;
+  ffm_effects effects{};                 // the enabled built-in effects, parameters filled
   std::vector<std::string> unsupported;  // config features present that this path lacks
 };
 
@@ -719,6 +720,67 @@ uint64_t jopt_u64(const JVal& o, const char* key, uint64_t dflt) {
 bool nonempty_array(const JVal& o, const char* key) {
   const JVal* v = o.get(key);
   return v && v->k == JVal::ARR && !v->a.empty();
+}
+
+// The built-in effects this path runs, by registry name, with their
+// parameter names in declaration order (assets/effects/<name>.rhai PARAMS).
+struct FxName {
+  const char* name;
+  int32_t type;
+  int chain;  // 1 density, 2 colour
+  const char* params[FFM_MAX_EFFECT_PARAMS];
+};
+const FxName kEffects[] = {
+    {"density_blur", FFM_FX_DENSITY_BLUR, 1, {"radius", "threshold", "falloff"}},
+    {"bilateral_blur", FFM_FX_BILATERAL_BLUR, 1, {"radius", "sigma_spatial", "sigma_range"}},
+    {"sharpen", FFM_FX_SHARPEN, 1, {"amount", "radius"}},
+    {"vignette", FFM_FX_VIGNETTE, 2, {"intensity", "radius", "softness", "blend_mode"}},
+    {"film_grain", FFM_FX_FILM_GRAIN, 2, {"intensity", "seed", "blend_mode"}},
+    {"chromatic_aberration", FFM_FX_CHROMATIC_ABERRATION, 2,
+     {"amount", "radial", "intensity", "blend_mode"}},
+    {"hue_shift", FFM_FX_HUE_SHIFT, 2, {"offset", "intensity", "blend_mode"}},
+    {"sobel_edges", FFM_FX_SOBEL_EDGES, 2,
+     {"intensity", "threshold", "glow", "preserve_color", "blend_mode"}},
+};
+
+// A config's effect list (Vec<EffectInstance>, src/effects/mod.rs:319-336:
+// effect_type, enabled, params by name; a missing parameter takes the
+// registry's default). Disabled effects are dropped, as the chain skips
+// them (effect_chain.rs:745); an enabled one this path does not run is
+// named in `unsupported`.
+void json_effects(const JVal& root, const char* key, int chain, SceneIn& s) {
+  const JVal* arr = root.get(key);
+  if (!arr || arr->k == JVal::NUL) return;
+  if (arr->k != JVal::ARR) raise(FFM_ERR_PARSE, "%s: expected an array", key);
+  for (const JVal& e : arr->a) {
+    if (e.k != JVal::OBJ) raise(FFM_ERR_PARSE, "%s: expected objects", key);
+    const std::string type = jopt_str(e, "effect_type", "");
+    const JVal* en = e.get("enabled");
+    if (!en || en->k != JVal::BOOL) raise(FFM_ERR_PARSE, "%s: effect '%s' needs enabled", key, type.c_str());
+    if (!en->b) continue;
+    const FxName* d = nullptr;
+    for (const FxName& f : kEffects)
+      if (type == f.name && f.chain == chain) d = &f;
+    if (!d) {
+      s.unsupported.push_back(std::string(chain == 1 ? "density effect '" : "color effect '") +
+                              type + "'");
+      continue;
+    }
+    int& n = chain == 1 ? s.effects.n_density : s.effects.n_color;
+    if (n == FFM_MAX_EFFECTS) {
+      s.unsupported.push_back(std::string("more than 16 ") + key);
+      continue;
+    }
+    ffm_effect fx;
+    ffm_effect_defaults(d->type, &fx);
+    if (const JVal* p = e.get("params"); p && p->k != JVal::NUL) {
+      if (p->k != JVal::OBJ) raise(FFM_ERR_PARSE, "%s: params must be an object", key);
+      for (const auto& kv : p->o)
+        for (int i = 0; i < FFM_MAX_EFFECT_PARAMS && d->params[i]; i++)
+          if (kv.first == d->params[i]) fx.params[i] = jnum_f32(&kv.second, d->params[i]);
+    }
+    (chain == 1 ? s.effects.density : s.effects.color)[n++] = fx;
+  }
 }
 
 // A transform (Transform's Deserialize, src/scene/transforms.rs:1307-1560).
@@ -1060,8 +1122,9 @@ SceneIn json_config(const JVal& root) {
   s.deterministic_rng = jopt_bool(root, "deterministic_rng", false);
 
   // features outside this path, present in the file
-  if (nonempty_array(root, "density_effects")) s.unsupported.push_back("density effects");
-  if (nonempty_array(root, "color_effects")) s.unsupported.push_back("color effects");
+  s.effects.pixel_scale = 1.0f;
+  json_effects(root, "density_effects", 1, s);
+  json_effects(root, "color_effects", 2, s);
   if (nonempty_array(root, "custom_effects")) s.unsupported.push_back("custom effects");
   if (nonempty_array(root, "word_removals")) s.unsupported.push_back("word removals");
   if (jopt_bool(root, "cylinder_targeting", false) || jopt_bool(root, "cylinder_always", false))
@@ -1621,6 +1684,8 @@ void build_scene(const SceneIn& s, ffm_scene* out) {
   tp.levels_gamma = s.levels_gamma;
   for (int c = 0; c < 3; c++) tp.background[c] = s.background[c];
   snprintf(out->name, sizeof out->name, "%s", s.name.c_str());
+  out->effects = s.effects;
+  out->effects.pixel_scale = 1.0f;
 }
 
 }  // namespace

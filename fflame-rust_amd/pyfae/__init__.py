@@ -152,6 +152,65 @@ def palette_stops(stops):
     return a
 
 
+def effect(name, enabled=True, **params):
+    """An ffm_effect: the built-in effect `name` (its registry id, e.g.
+    "density_blur", "vignette") with the registry's defaults, overridden by
+    `params` by name (src/effects/mod.rs:319-336 EffectInstance)."""
+    if name not in _abi.EFFECTS:
+        raise ValueError(f"effect '{name}' is not one this engine runs "
+                         f"({', '.join(sorted(_abi.EFFECTS))})")
+    t, _, names = _abi.EFFECTS[name]
+    fx = _abi.Effect()
+    st = library().ffm_effect_defaults(t, C.byref(fx))
+    if st != 0:
+        raise EngineError(st, f"no defaults for effect {name}")
+    fx.enabled = 1 if enabled else 0
+    for k, v in params.items():
+        if k not in names:
+            raise ValueError(f"effect '{name}' has no parameter '{k}' ({', '.join(names)})")
+        fx.params[names.index(k)] = float(v)
+    return fx
+
+
+def _as_effect(e):
+    if isinstance(e, _abi.Effect):
+        return e
+    if isinstance(e, str):
+        return effect(e)
+    if isinstance(e, dict):  # a config's EffectInstance
+        return effect(e["effect_type"], e.get("enabled", True), **(e.get("params") or {}))
+    name, params = e
+    return effect(name, **params)
+
+
+def effects(density=(), color=(), pixel_scale=1.0):
+    """An ffm_effects: the density chain (before the tone map) and the
+    colour chain (after it), each item an ffm_effect, a name, a
+    (name, params) pair or a config's {"effect_type", "enabled", "params"}."""
+    fx = _abi.Effects()
+    density, color = [_as_effect(e) for e in density], [_as_effect(e) for e in color]
+    if len(density) > _abi.MAX_EFFECTS or len(color) > _abi.MAX_EFFECTS:
+        raise ValueError(f"at most {_abi.MAX_EFFECTS} effects a chain")
+    for i, e in enumerate(density):
+        if _abi.EFFECTS[effect_name(e.type)][1] != "density":
+            raise ValueError(f"{effect_name(e.type)} is a colour effect")
+        fx.density[i] = e
+    for i, e in enumerate(color):
+        if _abi.EFFECTS[effect_name(e.type)][1] != "color":
+            raise ValueError(f"{effect_name(e.type)} is a density effect")
+        fx.color[i] = e
+    fx.n_density, fx.n_color = len(density), len(color)
+    fx.pixel_scale = float(pixel_scale)
+    return fx
+
+
+def effect_name(t):
+    for k, (tt, _, _) in _abi.EFFECTS.items():
+        if tt == t:
+            return k
+    raise ValueError(f"no effect type {t}")
+
+
 def make_job(width, height, *, walkers=8192, ipt=256, burn_in=20, seed_base=12345,
              seed_stride=2654435761, target_samples=0, batch_size=4, splat_mode=0,
              persistent_orbits=1, zoom=1.0, pan_x=0.0, pan_y=0.0, rotation=0.0,
@@ -370,6 +429,28 @@ class Engine:
         self._check(self.lib.ffm_palette_set_xf(
             self.ctx, C.cast(C.addressof(sc.stops), C.POINTER(C.c_float)), sc.n_stops,
             sc.palette_size, C.byref(sc.palette_xf)))
+
+    def set_effects(self, fx=None, **chains):
+        """The tail later renders apply: an ffm_effects, or density=[...],
+        color=[...] (see `effects`); None clears it."""
+        if fx is None and chains:
+            fx = effects(**chains)
+        self._check(self.lib.ffm_effects_set(self.ctx, C.byref(fx) if fx is not None else None))
+
+    def density_effect(self, fx, src, dst, density_mean=1.0, pixel_scale=1.0, stream=None):
+        """One density effect between (H, W, 4) float32 device tensors."""
+        fx = _as_effect(fx)
+        self._check(self.lib.ffm_density_effect(self.ctx, C.byref(fx), _ptr(src), _ptr(dst),
+                                                int(src.shape[1]), int(src.shape[0]),
+                                                float(density_mean), float(pixel_scale),
+                                                _ptr(stream)))
+
+    def color_effect(self, fx, src, dst, pixel_scale=1.0, stream=None):
+        """One colour effect between (H, W, 4) uint8 device tensors."""
+        fx = _as_effect(fx)
+        self._check(self.lib.ffm_color_effect(self.ctx, C.byref(fx), _ptr(src), _ptr(dst),
+                                              int(src.shape[1]), int(src.shape[0]),
+                                              float(pixel_scale), _ptr(stream)))
 
     # -- whole renders
     def render(self, config, width=None, height=None, *, iterations=None,

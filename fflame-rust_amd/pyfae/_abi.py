@@ -123,6 +123,37 @@ class PaletteXf(C.Structure):
                 ("log_strength", C.c_float), ("rotation", C.c_float), ("reverse", C.c_int32)]
 
 
+MAX_EFFECTS = 16
+MAX_EFFECT_PARAMS = 8
+# ffm_effect_type: registry name -> (type, chain, parameter names in order)
+EFFECTS = {
+    "density_blur": (1, "density", ("radius", "threshold", "falloff")),
+    "bilateral_blur": (2, "density", ("radius", "sigma_spatial", "sigma_range")),
+    "sharpen": (3, "density", ("amount", "radius")),
+    "vignette": (16, "color", ("intensity", "radius", "softness", "blend_mode")),
+    "film_grain": (17, "color", ("intensity", "seed", "blend_mode")),
+    "chromatic_aberration": (18, "color", ("amount", "radial", "intensity", "blend_mode")),
+    "hue_shift": (19, "color", ("offset", "intensity", "blend_mode")),
+    "sobel_edges": (20, "color", ("intensity", "threshold", "glow", "preserve_color",
+                                  "blend_mode")),
+}
+BLEND_MODES = ("Normal", "Add", "Multiply", "Screen", "Overlay", "Soft Light", "Hard Light",
+               "Color Dodge", "Color Burn", "Hue", "Saturation", "Color", "Luminosity")
+
+
+class Effect(C.Structure):
+    """ffm_effect: one EffectInstance (type, enabled, params in order)."""
+    _fields_ = [("type", C.c_int32), ("enabled", C.c_int32),
+                ("params", C.c_float * MAX_EFFECT_PARAMS)]
+
+
+class Effects(C.Structure):
+    """ffm_effects: a config's density and colour chains."""
+    _fields_ = [("n_density", C.c_int32), ("n_color", C.c_int32),
+                ("density", Effect * MAX_EFFECTS), ("color", Effect * MAX_EFFECTS),
+                ("pixel_scale", C.c_float)]
+
+
 class Scene(C.Structure):
     """ffm_scene: one config (.fflame JSON or .flame XML) as this path reads it."""
     _fields_ = [
@@ -141,6 +172,7 @@ class Scene(C.Structure):
         ("image_width", C.c_uint32), ("image_height", C.c_uint32),
         ("tonemap", TonemapParams),
         ("name", C.c_char * SCENE_NAME),
+        ("effects", Effects),
     ]
 
 
@@ -224,6 +256,12 @@ SIGNATURES = [
                                 C.c_uint32, C.POINTER(Job), C.c_char_p, C.c_size_t]),
     ("ffm_render_config", C.c_int, [_vp, C.c_char_p, C.c_size_t, C.c_int, C.c_int, C.c_uint64,
                                     C.c_uint32, C.c_uint32, _u8p, C.POINTER(RenderStats)]),
+    ("ffm_effect_defaults", C.c_int, [C.c_int32, C.POINTER(Effect)]),
+    ("ffm_effects_set", C.c_int, [_vp, C.POINTER(Effects)]),
+    ("ffm_density_effect", C.c_int, [_vp, C.POINTER(Effect), _vp, _vp, C.c_int, C.c_int,
+                                     C.c_float, C.c_float, _vp]),
+    ("ffm_color_effect", C.c_int, [_vp, C.POINTER(Effect), _vp, _vp, C.c_int, C.c_int,
+                                   C.c_float, _vp]),
 ]
 
 

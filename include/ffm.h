@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 5
+#define FFM_ABI_VERSION 6
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -411,6 +411,65 @@ typedef struct {
   int32_t reverse;
 } ffm_palette_xf;
 
+/* ---- the tail: density and colour effects ----------------------------
+ * The stages after the chaos game (src/renderer/tail.rs:1-162): density
+ * effects on the accumulator (f32 RGBA: mean colour, hit count) before the
+ * tone map, colour effects on the tone-mapped RGBA8 picture after it, each
+ * effect reading the previous one's output (EffectChainRunner::
+ * run_density_effects / run_color_effects, src/renderer/effect_chain.rs:
+ * 737-830,874-962). The built-in effects below follow their Rhai
+ * definitions (assets/effects/<name>.rhai); a colour effect's colour is
+ * blended over its input by the blend library
+ * (shaders/effects/common/blend_modes.wgsl apply_blend; the input's alpha
+ * kept, src/effects/custom.rs:49-60) and stored as RGBA8. */
+#define FFM_MAX_EFFECTS 16
+#define FFM_MAX_EFFECT_PARAMS 8
+enum ffm_effect_type {
+  FFM_FX_DENSITY_BLUR = 1,           /* radius 3, threshold 0.25, falloff 0.5 */
+  FFM_FX_BILATERAL_BLUR = 2,         /* radius 3, sigma_spatial 3, sigma_range 0.1 */
+  FFM_FX_SHARPEN = 3,                /* amount 1, radius 1 */
+  FFM_FX_VIGNETTE = 16,              /* intensity 0.5, radius 0.5, softness 0.5, blend_mode 0 */
+  FFM_FX_FILM_GRAIN = 17,            /* intensity 0.1, seed 0, blend_mode 4 */
+  FFM_FX_CHROMATIC_ABERRATION = 18,  /* amount 2, radial 1, intensity 1, blend_mode 0 */
+  FFM_FX_HUE_SHIFT = 19,             /* offset 0 (degrees), intensity 1, blend_mode 0 */
+  FFM_FX_SOBEL_EDGES = 20            /* intensity 1, threshold 0.1, glow 0.5, preserve_color 0.5,
+                                        blend_mode 1 */
+};
+/* An EffectInstance (src/effects/mod.rs:319-336): params in the effect's
+ * declared order (the slot order of get_param), as listed above. */
+typedef struct {
+  int32_t type;     /* ffm_effect_type */
+  int32_t enabled;  /* 0: skipped */
+  float params[FFM_MAX_EFFECT_PARAMS];
+} ffm_effect;
+/* A config's two chains (FractalConfig density_effects / color_effects,
+ * src/config/fractal_config.rs:543-548). */
+typedef struct {
+  int32_t n_density, n_color;
+  ffm_effect density[FFM_MAX_EFFECTS];
+  ffm_effect color[FFM_MAX_EFFECTS];
+  float pixel_scale;  /* render pixels per logical point (set_pixel_scale; 1 for exports) */
+} ffm_effects;
+
+/* The registry's defaults for `type` (the Rhai PARAMS values). */
+int ffm_effect_defaults(int32_t type, ffm_effect* out);
+
+/* Effects every later ffm_render / ffm_render_to_device / ffm_render_accum
+ * of this context applies (NULL: none). ffm_render_accum returns the
+ * accumulator after the density chain: what the tone map reads. The
+ * batched gallery and the soft splat apply none. */
+int ffm_effects_set(ffm_ctx* ctx, const ffm_effects* fx);
+
+/* One density effect on device buffers (f32 RGBA, width*height*4 floats
+ * each, in != out); density_mean is the frame's mean density
+ * (FlameRenderer::mean_density, src/renderer/compute_kernel.rs:1786-1792). */
+int ffm_density_effect(ffm_ctx* ctx, const ffm_effect* fx, const float* in, float* out, int width,
+                       int height, float density_mean, float pixel_scale, void* stream);
+
+/* One colour effect on device buffers (RGBA8, in != out). */
+int ffm_color_effect(ffm_ctx* ctx, const ffm_effect* fx, const uint8_t* in, uint8_t* out,
+                     int width, int height, float pixel_scale, void* stream);
+
 /* One config as this path reads it. */
 typedef struct {
   ffm_flame_desc flame;
@@ -428,6 +487,7 @@ typedef struct {
   uint32_t image_width, image_height; /* the saved size (`image_size`, XML `size`) */
   ffm_tonemap_params tonemap;
   char name[FFM_SCENE_NAME];     /* the flame's name, truncated, NUL-terminated */
+  ffm_effects effects;           /* the config's enabled density and colour effects */
 } ffm_scene;
 
 /* Number of configs in `text` (a JSON object, a JSON array of configs --

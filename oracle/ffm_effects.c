@@ -1,0 +1,418 @@
+/* The render tail's effects, restated on the CPU.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker the HIP effect kernels are held to.
+ * Nothing of the product links or calls this file.
+ *
+ * Follows, as text:
+ *   texture read: uf_texture, src/lang/wgsl.rs:184-200 (bilinear between
+ *     texel centres in f32, clamped at the edges);
+ *   density effects: assets/effects/density_blur.rhai, bilateral_blur.rhai,
+ *     sharpen.rhai (their WGSL predecessors under shaders/effects/density fix the same pixels);
+ *   colour effects: assets/effects/vignette.rhai, film_grain.rhai,
+ *     chromatic_aberration.rhai, hue_shift.rhai, sobel_edges.rhai, their
+ *     colour blended by apply_blend (shaders/effects/common/
+ *     blend_modes.wgsl) over the input, the input's alpha kept
+ *     (src/effects/custom.rs:49-60), stored as RGBA8;
+ *   the chains: EffectChainRunner::run_density_effects /
+ *     run_color_effects (src/renderer/effect_chain.rs:737-962): each
+ *     enabled effect reads the previous one's output.
+ * A pixel's uv is its centre, ((x + 0.5) / W, (y + 0.5) / H); its own
+ * colour `c` is its texel. All arithmetic is f32 in source order
+ * (-ffp-contract=off); exp, sqrt and sin are the C library's.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "ffm_oracle.h"
+
+typedef struct {
+  float v[4];
+} px4;
+
+typedef struct {
+  const float* f32; /* W*H*4, or NULL */
+  const uint8_t* u8;
+  int W, H;
+} img_t;
+
+static px4 img_at(const img_t* im, int x, int y) {
+  px4 r;
+  const size_t i = 4 * ((size_t)y * im->W + x);
+  for (int c = 0; c < 4; c++)
+    r.v[c] = im->f32 ? im->f32[i + c] : (float)im->u8[i + c] / 255.0f;
+  return r;
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : v > hi ? hi : v; }
+static float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+/* uf_texture */
+static px4 tex(const img_t* im, float u, float v) {
+  float px = u * (float)im->W - 0.5f, py = v * (float)im->H - 0.5f;
+  px = fabsf(px) <= 1e30f ? fminf(fmaxf(px, -1.0f), (float)im->W) : -1.0f;
+  py = fabsf(py) <= 1e30f ? fminf(fmaxf(py, -1.0f), (float)im->H) : -1.0f;
+  const float fx = floorf(px), fy = floorf(py);
+  const float tx = px - fx, ty = py - fy;
+  const int x0 = clampi((int)fx, 0, im->W - 1), y0 = clampi((int)fy, 0, im->H - 1);
+  const int x1 = clampi((int)fx + 1, 0, im->W - 1), y1 = clampi((int)fy + 1, 0, im->H - 1);
+  const px4 a = img_at(im, x0, y0), b = img_at(im, x1, y0);
+  const px4 c = img_at(im, x0, y1), d = img_at(im, x1, y1);
+  px4 r;
+  for (int k = 0; k < 4; k++) {
+    const float top = a.v[k] + (b.v[k] - a.v[k]) * tx;
+    const float bottom = c.v[k] + (d.v[k] - c.v[k]) * tx;
+    r.v[k] = top + (bottom - top) * ty;
+  }
+  return r;
+}
+
+/* smoothstep as the Rhai effects define it */
+static float smooth(float a, float b, float x) {
+  const float t = clamp01((x - a) / (b - a));
+  return t * t * (3.0f - 2.0f * t);
+}
+
+static float lum3(float r, float g, float b) { return r * 0.2126f + g * 0.7152f + b * 0.0722f; }
+static float len2(float x, float y) { return sqrtf(x * x + y * y); }
+
+/* ---- density effects ---------------------------------------------------- */
+
+static px4 density_blur(const img_t* im, const float* p, float dm, float ps, px4 c, float u,
+                        float v) {
+  const float max_radius = p[0] * ps;
+  const float density = c.v[3] / fmaxf(dm, 0.000001f);
+  const float blur = density < p[1] ? 1.0f : 1.0f - smooth(p[1], p[1] + p[2], density);
+  const float radius = max_radius * blur;
+  if (radius < 0.5f) return c;
+  const float px = 1.0f / (float)im->W, py = 1.0f / (float)im->H;
+  const float sigma = radius / 2.0f;
+  const float step = fmaxf(1.0f, radius / 4.0f);
+  float sum[4] = {0, 0, 0, 0}, weights = 0.0f;
+  for (int yi = -4; yi < 5; yi++)
+    for (int xi = -4; xi < 5; xi++) {
+      const float ox = (float)xi * step, oy = (float)yi * step;
+      const float dist = len2(ox, oy);
+      if (dist <= radius) {
+        const float w = expf(-(dist * dist) / (2.0f * sigma * sigma));
+        const px4 s = tex(im, u + ox * px, v + oy * py);
+        for (int k = 0; k < 4; k++) sum[k] = sum[k] + s.v[k] * w;
+        weights += w;
+      }
+    }
+  if (weights <= 0.0f) return c;
+  px4 r;
+  for (int k = 0; k < 4; k++) r.v[k] = sum[k] / weights;
+  return r;
+}
+
+static float gaussian(float x, float sigma) { return expf(-(x * x) / (2.0f * sigma * sigma)); }
+
+static px4 bilateral_blur(const img_t* im, const float* p, float ps, px4 c, float u, float v) {
+  const int radius = (int)fminf(fmaxf(p[0] * ps, 1.0f), 10.0f);
+  const float sigma_spatial = fmaxf(0.1f, p[1] * ps);
+  const float sigma_range = fmaxf(0.01f, p[2]);
+  const float px = 1.0f / (float)im->W, py = 1.0f / (float)im->H;
+  const float centre = lum3(c.v[0], c.v[1], c.v[2]);
+  float sum[4] = {0, 0, 0, 0}, weights = 0.0f;
+  for (int y = -radius; y < radius + 1; y++)
+    for (int x = -radius; x < radius + 1; x++) {
+      const float fx = (float)x, fy = (float)y;
+      const px4 s = tex(im, u + fx * px, v + fy * py);
+      const float w = gaussian(len2(fx, fy), sigma_spatial) *
+                      gaussian(fabsf(centre - lum3(s.v[0], s.v[1], s.v[2])), sigma_range);
+      for (int k = 0; k < 4; k++) sum[k] = sum[k] + s.v[k] * w;
+      weights += w;
+    }
+  px4 r;
+  for (int k = 0; k < 4; k++) r.v[k] = sum[k] / fmaxf(weights, 0.0001f);
+  return r;
+}
+
+static px4 sharpen(const img_t* im, const float* p, float ps, px4 c, float u, float v) {
+  const float radius = p[1] * ps;
+  const float px = 1.0f / (float)im->W, py = 1.0f / (float)im->H;
+  const float sigma = radius / 2.0f;
+  const int kk = (int)ceilf(radius);
+  float sum[4] = {0, 0, 0, 0}, weights = 0.0f;
+  for (int y = -kk; y < kk + 1; y++)
+    for (int x = -kk; x < kk + 1; x++) {
+      const float fx = (float)x, fy = (float)y;
+      const float dist = len2(fx, fy);
+      if (dist <= radius) {
+        const float w = expf(-(dist * dist) / (2.0f * sigma * sigma));
+        const px4 s = tex(im, u + fx * px, v + fy * py);
+        for (int k = 0; k < 4; k++) sum[k] = sum[k] + s.v[k] * w;
+        weights += w;
+      }
+    }
+  if (p[0] < 0.01f) return c;
+  px4 r;
+  for (int k = 0; k < 3; k++) {
+    const float blurred = sum[k] / weights;
+    r.v[k] = clamp01(c.v[k] + (c.v[k] - blurred) * p[0]);
+  }
+  r.v[3] = c.v[3];
+  return r;
+}
+
+void orc_density_effect(const ffm_effect* fx, const float* in, float* out, int W, int H,
+                        float density_mean, float pixel_scale) {
+  const img_t im = {in, NULL, W, H};
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const float u = ((float)x + 0.5f) / (float)W, v = ((float)y + 0.5f) / (float)H;
+      const px4 c = img_at(&im, x, y);
+      px4 o = c;
+      switch (fx->type) {
+        case FFM_FX_DENSITY_BLUR:
+          o = density_blur(&im, fx->params, density_mean, pixel_scale, c, u, v);
+          break;
+        case FFM_FX_BILATERAL_BLUR: o = bilateral_blur(&im, fx->params, pixel_scale, c, u, v); break;
+        case FFM_FX_SHARPEN: o = sharpen(&im, fx->params, pixel_scale, c, u, v); break;
+        default: break;
+      }
+      memcpy(out + 4 * ((size_t)y * W + x), o.v, sizeof o.v);
+    }
+}
+
+/* ---- the blend library (blend_modes.wgsl) -------------------------------- */
+
+static void rgb_to_hsl(const float c[3], float o[3]) {
+  const float cmax = fmaxf(c[0], fmaxf(c[1], c[2])), cmin = fminf(c[0], fminf(c[1], c[2]));
+  const float delta = cmax - cmin;
+  const float l = (cmax + cmin) * 0.5f;
+  if (delta < 0.00001f) {
+    o[0] = 0.0f;
+    o[1] = 0.0f;
+    o[2] = l;
+    return;
+  }
+  /* select(f, t, cond): t when cond */
+  const float s = l > 0.5f ? delta / (2.0f - cmax - cmin) : delta / (cmax + cmin);
+  float h;
+  if (cmax == c[0]) {
+    h = (c[1] - c[2]) / delta;
+    if (c[1] < c[2]) h += 6.0f;
+  } else if (cmax == c[1]) {
+    h = (c[2] - c[0]) / delta + 2.0f;
+  } else {
+    h = (c[0] - c[1]) / delta + 4.0f;
+  }
+  o[0] = h / 6.0f;
+  o[1] = s;
+  o[2] = l;
+}
+
+static float hue_to_rgb(float p, float q, float t) {
+  if (t < 0.0f) t += 1.0f;
+  if (t > 1.0f) t -= 1.0f;
+  if (t < 1.0f / 6.0f) return p + (q - p) * 6.0f * t;
+  if (t < 1.0f / 2.0f) return q;
+  if (t < 2.0f / 3.0f) return p + (q - p) * (2.0f / 3.0f - t) * 6.0f;
+  return p;
+}
+
+static void hsl_to_rgb(const float hsl[3], float o[3]) {
+  const float h = hsl[0], s = hsl[1], l = hsl[2];
+  if (s < 0.00001f) {
+    o[0] = o[1] = o[2] = l;
+    return;
+  }
+  const float q = l < 0.5f ? l * (1.0f + s) : l + s - l * s;
+  const float p = 2.0f * l - q;
+  o[0] = hue_to_rgb(p, q, h + 1.0f / 3.0f);
+  o[1] = hue_to_rgb(p, q, h);
+  o[2] = hue_to_rgb(p, q, h - 1.0f / 3.0f);
+}
+
+void orc_apply_blend(const float base[3], const float e[3], int mode, float amount, float out[3]) {
+  float b[3];
+  if (mode >= 9 && mode <= 12) {
+    float hb[3], he[3], hm[3];
+    rgb_to_hsl(base, hb);
+    rgb_to_hsl(e, he);
+    if (mode == 9) { hm[0] = he[0]; hm[1] = hb[1]; hm[2] = hb[2]; }       /* hue */
+    else if (mode == 10) { hm[0] = hb[0]; hm[1] = he[1]; hm[2] = hb[2]; } /* saturation */
+    else if (mode == 11) { hm[0] = he[0]; hm[1] = he[1]; hm[2] = hb[2]; } /* color */
+    else { hm[0] = hb[0]; hm[1] = hb[1]; hm[2] = he[2]; }                 /* luminosity */
+    hsl_to_rgb(hm, b);
+  } else {
+    for (int i = 0; i < 3; i++) {
+      const float x = base[i], y = e[i];
+      float r;
+      switch (mode) {
+        case 1: r = fminf(x + y, 1.0f); break;                         /* add */
+        case 2: r = x * y; break;                                       /* multiply */
+        case 3: r = 1.0f - (1.0f - x) * (1.0f - y); break;              /* screen */
+        case 4:                                                          /* overlay */
+          r = x > 0.5f ? 1.0f - 2.0f * (1.0f - x) * (1.0f - y) : 2.0f * x * y;
+          break;
+        case 5: {                                                        /* soft light */
+          const float d = x > 0.25f ? sqrtf(x) : ((16.0f * x - 12.0f) * x + 4.0f) * x;
+          r = y > 0.5f ? x + (2.0f * y - 1.0f) * (d - x) : x - (1.0f - 2.0f * y) * x * (1.0f - x);
+          break;
+        }
+        case 6:                                                          /* hard light */
+          r = y > 0.5f ? 1.0f - 2.0f * (1.0f - x) * (1.0f - y) : 2.0f * x * y;
+          break;
+        case 7: r = y >= 1.0f ? 1.0f : fminf(1.0f, x / (1.0f - y + 0.00001f)); break;
+        case 8: r = y <= 0.0f ? 0.0f : 1.0f - fminf(1.0f, (1.0f - x) / (y + 0.00001f)); break;
+        default: r = y; break;                                           /* normal */
+      }
+      b[i] = r;
+    }
+  }
+  /* mix(base, blended, intensity) */
+  for (int i = 0; i < 3; i++) out[i] = base[i] * (1.0f - amount) + b[i] * amount;
+}
+
+/* ---- colour effects ------------------------------------------------------ */
+
+static uint8_t unorm8(float v) {
+  v = (v != v) ? 0.0f : clamp01(v);
+  return (uint8_t)floorf(v * 255.0f + 0.5f);
+}
+
+void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int W, int H,
+                      float pixel_scale) {
+  const img_t im = {NULL, in, W, H};
+  const float* p = fx->params;
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const float u = ((float)x + 0.5f) / (float)W, v = ((float)y + 0.5f) / (float)H;
+      const px4 c = img_at(&im, x, y);
+      float e[3] = {c.v[0], c.v[1], c.v[2]};
+      int mode = 0;
+      float amount = 1.0f;
+      switch (fx->type) {
+        case FFM_FX_VIGNETTE: {
+          const float aspect = (float)W / (float)H;
+          const float dist = len2((u - 0.5f) * aspect, v - 0.5f);
+          const float start = p[1] * 0.5f * fmaxf(1.0f, aspect);
+          const float end = start + p[2] * 0.5f * fmaxf(1.0f, aspect);
+          const float vig = 1.0f - smooth(start, end, dist);
+          for (int k = 0; k < 3; k++) e[k] = c.v[k] * vig;
+          amount = p[0];
+          mode = (int)p[3];
+          break;
+        }
+        case FFM_FX_FILM_GRAIN: {
+          const float ax = u * (float)W + p[1] * 127.1f, ay = v * (float)H + p[1] * 311.7f;
+          const float n = ax * 127.1f + ay * 311.7f;
+          const float s = sinf(n) * 43758.5453f;
+          e[0] = e[1] = e[2] = s - floorf(s);
+          amount = p[0];
+          mode = (int)p[2];
+          break;
+        }
+        case FFM_FX_CHROMATIC_ABERRATION: {
+          const float a = p[0] * pixel_scale;
+          const float px = 1.0f / (float)W, py = 1.0f / (float)H;
+          float ox, oy;
+          if (p[1] != 0.0f) {
+            const float ex = u - 0.5f, ey = v - 0.5f;
+            const float dist = len2(ex, ey) * 2.0f;
+            const float vx = ex + 0.0001f, vy = ey + 0.0001f;
+            const float vl = len2(vx, vy);
+            ox = vx / vl * a * px * dist;
+            oy = vy / vl * a * py * dist;
+          } else {
+            ox = a * px;
+            oy = 0.0f;
+          }
+          e[0] = tex(&im, u + ox, v + oy).v[0];
+          e[1] = c.v[1];
+          e[2] = tex(&im, u - ox, v - oy).v[2];
+          amount = p[2];
+          mode = (int)p[3];
+          break;
+        }
+        case FFM_FX_HUE_SHIFT: {
+          const float r = c.v[0], g = c.v[1], b = c.v[2];
+          const float cmax = fmaxf(r, fmaxf(g, b)), cmin = fminf(r, fminf(g, b));
+          const float delta = cmax - cmin;
+          float h = 0.0f, s = 0.0f;
+          if (delta > 0.0001f) {
+            s = delta / cmax;
+            if (cmax == r) {
+              h = (g - b) / delta;
+              if (g < b) h += 6.0f;
+            } else if (cmax == g) {
+              h = 2.0f + (b - r) / delta;
+            } else {
+              h = 4.0f + (r - g) / delta;
+            }
+            h /= 6.0f;
+          }
+          float hx = h + p[0] / 360.0f;
+          hx = hx - floorf(hx);
+          const float hh = hx * 6.0f, vv = cmax;
+          const float i = floorf(hh), f = hh - i;
+          const float pp = vv * (1.0f - s), q = vv * (1.0f - s * f);
+          const float t = vv * (1.0f - s * (1.0f - f));
+          const float tab[6][3] = {{vv, t, pp}, {q, vv, pp}, {pp, vv, t},
+                                   {pp, q, vv}, {t, pp, vv}, {vv, pp, q}};
+          const int si = i == 0.0f ? 0 : i == 1.0f ? 1 : i == 2.0f ? 2 : i == 3.0f ? 3 : i == 4.0f ? 4 : 5;
+          for (int k = 0; k < 3; k++) e[k] = tab[si][k];
+          amount = p[1];
+          mode = (int)p[2];
+          break;
+        }
+        case FFM_FX_SOBEL_EDGES: {
+          const float tx = 1.0f / (float)W, ty = 1.0f / (float)H;
+          const float du[8] = {-tx, 0.0f, tx, -tx, tx, -tx, 0.0f, tx};
+          const float dv[8] = {-ty, -ty, -ty, 0.0f, 0.0f, ty, ty, ty};
+          float n[8];
+          for (int k = 0; k < 8; k++) {
+            const px4 s = tex(&im, u + du[k], v + dv[k]);
+            n[k] = lum3(s.v[0], s.v[1], s.v[2]);
+          }
+          /* n0 n1 n2 / n3 . n5 / n6 n7 n8 */
+          const float n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3], n5 = n[4], n6 = n[5];
+          const float n7 = n[6], n8 = n[7];
+          const float gx = n2 + 2.0f * n5 + n8 - (n0 + 2.0f * n3 + n6);
+          const float gy = n0 + 2.0f * n1 + n2 - (n6 + 2.0f * n7 + n8);
+          const float edge = smooth(p[1] * 0.5f, p[1], sqrtf(gx * gx + gy * gy));
+          const float pc = p[3];
+          const float gain = (1.0f + p[2] * 2.0f) * edge;
+          for (int k = 0; k < 3; k++) e[k] = (1.0f * (1.0f - pc) + c.v[k] * pc) * gain;
+          amount = p[0];
+          mode = (int)p[4];
+          break;
+        }
+        default: break;
+      }
+      const float base[3] = {c.v[0], c.v[1], c.v[2]};
+      float r[3];
+      orc_apply_blend(base, e, mode, amount, r);
+      uint8_t* o = out + 4 * ((size_t)y * W + x);
+      o[0] = unorm8(r[0]);
+      o[1] = unorm8(r[1]);
+      o[2] = unorm8(r[2]);
+      o[3] = in[4 * ((size_t)y * W + x) + 3];
+    }
+}
+
+/* The density chain over acc (W*H*4 floats, in place through scratch). */
+void orc_density_chain(const ffm_effects* fx, float* acc, float* scratch, int W, int H,
+                       float density_mean) {
+  if (!fx) return;
+  const size_t n = (size_t)W * H * 4;
+  for (int i = 0; i < fx->n_density; i++) {
+    if (!fx->density[i].enabled) continue;
+    orc_density_effect(&fx->density[i], acc, scratch, W, H, density_mean, fx->pixel_scale);
+    memcpy(acc, scratch, n * sizeof(float));
+  }
+}
+
+/* The colour chain over rgba (W*H*4 bytes, in place through scratch). */
+void orc_color_chain(const ffm_effects* fx, uint8_t* rgba, uint8_t* scratch, int W, int H) {
+  if (!fx) return;
+  const size_t n = (size_t)W * H * 4;
+  for (int i = 0; i < fx->n_color; i++) {
+    if (!fx->color[i].enabled) continue;
+    orc_color_effect(&fx->color[i], rgba, scratch, W, H, fx->pixel_scale);
+    memcpy(rgba, scratch, n);
+  }
+}

@@ -1349,7 +1349,8 @@ void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
  * target or after burn_in/ipt+4 empty dispatches; then tonemap. */
 int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int pal_size,
                const ffm_palette_xf* pal_xf, const ffm_job* job, uint64_t exact_after,
-               float* accum_out, uint8_t* rgba_out, uint64_t* total_out, int threads) {
+               float* accum_out, uint8_t* rgba_out, uint64_t* total_out, int threads,
+               const ffm_effects* fx) {
   ffm_flame_desc* f = (ffm_flame_desc*)malloc(sizeof(ffm_flame_desc));
   memcpy(f, f_in, sizeof(*f));
   orc_flame_prepare(f);
@@ -1423,8 +1424,20 @@ int orc_render(const ffm_flame_desc* f_in, const float* stops, int n_stops, int 
     acc = acc2;
     acc2 = t;
   }
+  /* the tail: density effects (their mean density is the tone map's
+   * sample density), the tone map, colour effects (src/renderer/tail.rs) */
+  const float density_mean =
+      fmaxf((float)samples_in_buffer / fmaxf((float)W * (float)H, 1.0f), 1e-6f);
+  orc_density_chain(fx, acc, acc2, W, H, density_mean);
   if (accum_out) memcpy(accum_out, acc, np * 16);
-  if (rgba_out) orc_tonemap(acc, rgba_out, W, H, v->zoom, (double)samples_in_buffer, &job->tonemap);
+  if (rgba_out) {
+    orc_tonemap(acc, rgba_out, W, H, v->zoom, (double)samples_in_buffer, &job->tonemap);
+    if (fx && fx->n_color > 0) {
+      uint8_t* tmp = (uint8_t*)malloc(np * 4);
+      orc_color_chain(fx, rgba_out, tmp, W, H);
+      free(tmp);
+    }
+  }
   if (total_out) *total_out = total;
   orc_orbit_fuses_free(&fuses);
   free(orbits);

@@ -116,7 +116,18 @@ void orc_tonemap_defaults(ffm_tonemap_params* tp);
 /* --- whole render (src/renderer/render.rs:483-800) --- */
 int orc_render(const ffm_flame_desc* f, const float* stops_prgb, int n_stops, int pal_size,
                const ffm_palette_xf* pal_xf, const ffm_job* job, uint64_t exact_after,
-               float* accum_out, uint8_t* rgba_out, uint64_t* total_out, int threads);
+               float* accum_out, uint8_t* rgba_out, uint64_t* total_out, int threads,
+               const ffm_effects* fx);
+
+/* --- the render tail's effects (ffm_effects.c; src/renderer/tail.rs) --- */
+void orc_density_effect(const ffm_effect* fx, const float* in, float* out, int W, int H,
+                        float density_mean, float pixel_scale);
+void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int W, int H,
+                      float pixel_scale);
+void orc_apply_blend(const float base[3], const float e[3], int mode, float amount, float out[3]);
+void orc_density_chain(const ffm_effects* fx, float* acc, float* scratch, int W, int H,
+                       float density_mean);
+void orc_color_chain(const ffm_effects* fx, uint8_t* rgba, uint8_t* scratch, int W, int H);
 
 /* --- soft splat, FP64 (ffm_soft.c; no reference counterpart) --- */
 /* Parameters of a fittable flame, n * (6 + v), or -1. */

@@ -96,7 +96,12 @@ def lib():
                                    C.POINTER(TonemapParams)]),
             "orc_render": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.c_int,
                                      C.POINTER(_abi.PaletteXf), C.POINTER(Job), C.c_uint64, _fp,
-                                     _u8p, _u64p, C.c_int]),
+                                     _u8p, _u64p, C.c_int, C.POINTER(_abi.Effects)]),
+            "orc_density_effect": (None, [C.POINTER(_abi.Effect), _fp, _fp, C.c_int, C.c_int,
+                                          C.c_float, C.c_float]),
+            "orc_color_effect": (None, [C.POINTER(_abi.Effect), _u8p, _u8p, C.c_int, C.c_int,
+                                        C.c_float]),
+            "orc_apply_blend": (None, [_fp, _fp, C.c_int, C.c_float, _fp]),
             "orc_palette_index_map": (None, [C.c_int, C.POINTER(_abi.PaletteXf),
                                              C.POINTER(C.c_int32)]),
             "orc_palette_linear_xf": (None, [_fp, C.c_int, C.c_int, C.POINTER(_abi.PaletteXf),
@@ -220,9 +225,11 @@ def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=25
     return out
 
 
-def render(fd, stops, job, pal_size=256, threads=1, palette_xf=None, exact_after=1 << 18):
+def render(fd, stops, job, pal_size=256, threads=1, palette_xf=None, exact_after=1 << 18,
+           effects=None):
     """(rgba uint8 (H, W, 4), accum float32 (H, W, 4), total samples).
-    `exact_after`: the accumulate pass from which densities count exactly."""
+    `exact_after`: the accumulate pass from which densities count exactly;
+    `effects`: an ffm_effects applied as the render's tail."""
     w, h = job.view.width, job.view.height
     rgba = np.zeros((h, w, 4), dtype=np.uint8)
     acc = np.zeros((h, w, 4), dtype=np.float32)
@@ -230,8 +237,36 @@ def render(fd, stops, job, pal_size=256, threads=1, palette_xf=None, exact_after
     s = np.ascontiguousarray(stops, dtype=np.float32)
     xf = C.byref(palette_xf) if palette_xf is not None else None
     lib().orc_render(C.byref(fd), fp(s), s.shape[0], pal_size, xf, C.byref(job), exact_after,
-                     fp(acc), rgba.ctypes.data_as(_u8p), C.byref(tot), threads)
+                     fp(acc), rgba.ctypes.data_as(_u8p), C.byref(tot), threads,
+                     C.byref(effects) if effects is not None else None)
     return rgba, acc, tot.value
+
+
+def density_effect(fx, img, density_mean=1.0, pixel_scale=1.0):
+    """One density effect over an (H, W, 4) float32 image."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.empty_like(a)
+    lib().orc_density_effect(C.byref(fx), fp(a), fp(out), a.shape[1], a.shape[0], density_mean,
+                             pixel_scale)
+    return out
+
+
+def color_effect(fx, img, pixel_scale=1.0):
+    """One colour effect over an (H, W, 4) uint8 image."""
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    out = np.empty_like(a)
+    lib().orc_color_effect(C.byref(fx), a.ctypes.data_as(_u8p), out.ctypes.data_as(_u8p),
+                           a.shape[1], a.shape[0], pixel_scale)
+    return out
+
+
+def apply_blend(base, effect, mode, amount):
+    """The blend library's apply_blend on one rgb triple (float32)."""
+    b = np.ascontiguousarray(base, dtype=np.float32)
+    e = np.ascontiguousarray(effect, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    lib().orc_apply_blend(fp(b), fp(e), int(mode), float(amount), fp(out))
+    return out
 
 
 def palette_index_map(size, palette_xf=None):

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_binds():
     lib = pyfae.library()
-    assert lib.ffm_abi_version() == 5
+    assert lib.ffm_abi_version() == 6
     bound = {name for name, _, _ in _abi.SIGNATURES}
     assert bound == set(declared())
 
@@ -52,6 +52,8 @@ int main(void) {
          offsetof(ffm_job, tonemap), offsetof(ffm_flame_desc, finals));
   printf("%zu %zu %zu %zu %zu\n", sizeof(ffm_palette_xf), sizeof(ffm_scene),
          offsetof(ffm_scene, pan_x), offsetof(ffm_scene, tonemap), offsetof(ffm_scene, name));
+  printf("%zu %zu %zu %zu\n", sizeof(ffm_effect), sizeof(ffm_effects),
+         offsetof(ffm_effects, pixel_scale), offsetof(ffm_scene, effects));
   return 0;
 }
 """
@@ -66,7 +68,9 @@ int main(void) {
             C.sizeof(_abi.TonemapParams), C.sizeof(_abi.Job), C.sizeof(_abi.RenderStats),
             _abi.Job.tonemap.offset, _abi.FlameDesc.finals.offset,
             C.sizeof(_abi.PaletteXf), C.sizeof(_abi.Scene), _abi.Scene.pan_x.offset,
-            _abi.Scene.tonemap.offset, _abi.Scene.name.offset]
+            _abi.Scene.tonemap.offset, _abi.Scene.name.offset,
+            C.sizeof(_abi.Effect), C.sizeof(_abi.Effects), _abi.Effects.pixel_scale.offset,
+            _abi.Scene.effects.offset]
     assert got == want
 
 

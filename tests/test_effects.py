@@ -1,0 +1,245 @@
+"""The render tail's effects in the oracle (CPU): the reference's own effect
+tests restated (src/renderer/effect_chain.rs:1378-1498), closed forms of
+each effect and of the blend library, an independent float64 restatement of
+the blurs on non-square images, and the config loader's effect lists."""
+import json
+
+import numpy as np
+import pytest
+
+import pyfae
+from oracle import oracle as O
+
+RNG = np.random.default_rng(7)
+
+
+def stripes(W=32, H=8):
+    """Columns alternating red 0 and 1, 1000 hits each."""
+    img = np.zeros((H, W, 4), np.float32)
+    img[..., 0] = (np.arange(W) % 2)[None, :]
+    img[..., 3] = 1000.0
+    return img
+
+
+# ---------------------------------------------------------- reference KATs
+
+def test_a_hit_count_past_half_float_survives_the_density_chain():
+    img = np.tile(np.array([0.5, 0.25, 0.75, 1.0e6], np.float32), (4, 64, 1))
+    out = O.density_effect(pyfae.effect("sharpen", amount=0.0), img, 1.0)
+    assert np.all(np.abs(out[..., 3] / 1.0e6 - 1.0) < 1e-6)
+    assert np.all(np.abs(out[..., 0] - 0.5) < 1e-6) and np.all(np.abs(out[..., 2] - 0.75) < 1e-6)
+
+
+def test_density_blur_measures_density_in_multiples_of_the_mean():
+    img = stripes()
+    fx = pyfae.effect("density_blur")
+    sparse = O.density_effect(fx, img, 10_000.0)[4, 16, 0]
+    assert abs(sparse - 0.5) < 0.2
+    dense = O.density_effect(fx, img, 100.0)[4, 16, 0]
+    assert dense in (0.0, 1.0)
+
+
+def test_a_size_in_pixels_follows_the_pixel_scale():
+    img = stripes()
+    b = lambda r, s: O.density_effect(pyfae.effect("density_blur", radius=r), img, 10_000.0, s)
+    assert np.array_equal(b(8.0, 0.5), b(4.0, 1.0))
+    assert not np.array_equal(b(8.0, 1.0), b(4.0, 1.0))
+    W, H = 32, 8
+    i = np.arange(W * H)
+    col = np.stack([(i % W) * 8, np.zeros_like(i), (i // W) * 30, np.full_like(i, 255)], -1)
+    col = col.astype(np.uint8).reshape(H, W, 4)
+    ca = lambda a, s: O.color_effect(pyfae.effect("chromatic_aberration", amount=a, radial=0.0),
+                                     col, s)
+    assert np.array_equal(ca(6.0, 0.5), ca(3.0, 1.0))
+    assert not np.array_equal(ca(6.0, 1.0), ca(3.0, 1.0))
+
+
+# ------------------------------------------------------------ closed forms
+
+@pytest.mark.parametrize("name", ["density_blur", "bilateral_blur", "sharpen"])
+def test_density_effects_keep_a_flat_image(name):
+    img = np.tile(np.array([0.3, 0.6, 0.1, 0.02], np.float32), (9, 13, 1))
+    out = O.density_effect(pyfae.effect(name), img, 1.0)
+    np.testing.assert_allclose(out, img, rtol=2e-6, atol=0)
+
+
+def test_dense_pixels_and_sharpen_off_are_exact():
+    img = RNG.random((11, 17, 4)).astype(np.float32)
+    img[..., 3] = 50.0  # 50x the mean: past threshold + falloff, no blur at all
+    assert np.array_equal(O.density_effect(pyfae.effect("density_blur"), img, 1.0), img)
+    assert np.array_equal(O.density_effect(pyfae.effect("sharpen", amount=0.005), img, 1.0), img)
+
+
+def bilinear64(img, u, v):
+    H, W = img.shape[:2]
+    p = np.array([u * W - 0.5, v * H - 0.5])
+    p = np.clip(p, -1, [W, H])
+    f = np.floor(p)
+    t = p - f
+    x0, y0 = int(np.clip(f[0], 0, W - 1)), int(np.clip(f[1], 0, H - 1))
+    x1, y1 = int(np.clip(f[0] + 1, 0, W - 1)), int(np.clip(f[1] + 1, 0, H - 1))
+    top = img[y0, x0] * (1 - t[0]) + img[y0, x1] * t[0]
+    bot = img[y1, x0] * (1 - t[0]) + img[y1, x1] * t[0]
+    return top * (1 - t[1]) + bot * t[1]
+
+
+def test_blurs_against_a_float64_restatement_on_a_non_square_image():
+    H, W = 5, 7
+    img = RNG.random((H, W, 4)).astype(np.float32)
+    img[..., 3] = RNG.random((H, W)) * 0.2  # sparse: density_blur blurs fully
+    img64 = img.astype(np.float64)
+    db = O.density_effect(pyfae.effect("density_blur", radius=3.0), img, 1.0)
+    bl = O.density_effect(pyfae.effect("bilateral_blur", radius=2.0, sigma_spatial=1.5,
+                                       sigma_range=0.3), img, 1.0)
+    sh = O.density_effect(pyfae.effect("sharpen", amount=0.7, radius=1.5), img, 1.0)
+    lum = lambda c: c[0] * 0.2126 + c[1] * 0.7152 + c[2] * 0.0722
+    for y in range(H):
+        for x in range(W):
+            u, v = (x + 0.5) / W, (y + 0.5) / H
+            c = img64[y, x]
+            # density_blur at radius 3: 9x9 taps one pixel apart within the disc
+            s, ws = np.zeros(4), 0.0
+            for yi in range(-4, 5):
+                for xi in range(-4, 5):
+                    d = np.hypot(xi, yi)
+                    if d <= 3.0:
+                        w = np.exp(-d * d / (2 * 1.5 ** 2))
+                        s += bilinear64(img64, u + xi / W, v + yi / H) * w
+                        ws += w
+            np.testing.assert_allclose(db[y, x], s / ws, rtol=1e-5, atol=1e-6)
+            s, ws = np.zeros(4), 0.0
+            for yi in range(-2, 3):
+                for xi in range(-2, 3):
+                    t = bilinear64(img64, u + xi / W, v + yi / H)
+                    w = (np.exp(-(xi * xi + yi * yi) / (2 * 1.5 ** 2))
+                         * np.exp(-(lum(c) - lum(t)) ** 2 / (2 * 0.3 ** 2)))
+                    s += t * w
+                    ws += w
+            np.testing.assert_allclose(bl[y, x], s / ws, rtol=1e-5, atol=1e-6)
+            s, ws = np.zeros(4), 0.0
+            for yi in range(-2, 3):
+                for xi in range(-2, 3):
+                    d = np.hypot(xi, yi)
+                    if d <= 1.5:
+                        w = np.exp(-d * d / (2 * 0.75 ** 2))
+                        s += bilinear64(img64, u + xi / W, v + yi / H) * w
+                        ws += w
+            want = np.clip(c[:3] + (c[:3] - s[:3] / ws) * 0.7, 0, 1)
+            np.testing.assert_allclose(sh[y, x, :3], want, rtol=1e-5, atol=1e-6)
+            assert sh[y, x, 3] == img[y, x, 3]
+
+
+def test_blend_library_closed_forms():
+    base = RNG.random((50, 3)).astype(np.float32)
+    eff = RNG.random((50, 3)).astype(np.float32)
+    one, zero = np.ones(3, np.float32), np.zeros(3, np.float32)
+    for b, e in zip(base, eff):
+        assert np.array_equal(O.apply_blend(b, e, 0, 1.0), e)          # normal
+        for m in range(13):
+            assert np.array_equal(O.apply_blend(b, e, m, 0.0), b)      # intensity 0
+        np.testing.assert_allclose(O.apply_blend(b, e, 1, 1.0), np.minimum(b + e, 1), atol=0)
+        assert np.array_equal(O.apply_blend(b, one, 2, 1.0), b)        # multiply by white
+        assert np.array_equal(O.apply_blend(b, e, 2, 1.0), O.apply_blend(e, b, 2, 1.0))
+        np.testing.assert_allclose(O.apply_blend(b, zero, 3, 1.0), b, atol=1e-7)  # screen black
+        # screen is the complement of multiplying the complements
+        np.testing.assert_allclose(O.apply_blend(b, e, 3, 1.0),
+                                   1 - O.apply_blend(1 - b, 1 - e, 2, 1.0), atol=1e-6)
+        # overlay and hard light swap roles
+        np.testing.assert_allclose(O.apply_blend(b, e, 4, 1.0), O.apply_blend(e, b, 6, 1.0),
+                                   atol=1e-6)
+        np.testing.assert_allclose(O.apply_blend(b, zero, 7, 1.0), b / 1.00001, rtol=1e-6)
+        np.testing.assert_allclose(O.apply_blend(b, one, 8, 1.0),
+                                   1 - (1 - b) / 1.00001, atol=1e-6)
+        for m in (9, 10, 11, 12):  # HSL modes of a colour with itself
+            np.testing.assert_allclose(O.apply_blend(b, b, m, 1.0), b, atol=2e-6)
+    # luminosity: the effect's HSL lightness on the base's hue and saturation
+    r = O.apply_blend(np.array([0.8, 0.2, 0.2], np.float32), np.array([0.3, 0.3, 0.3], np.float32),
+                      12, 1.0)
+    np.testing.assert_allclose((r.max() + r.min()) / 2, 0.3, atol=1e-6)
+    np.testing.assert_allclose(r, [0.48, 0.12, 0.12], atol=1e-6)
+    # hue of pure green on red's saturation and lightness: pure green
+    g = O.apply_blend(np.array([1, 0, 0], np.float32), np.array([0, 0.4, 0], np.float32), 9, 1.0)
+    np.testing.assert_allclose(g, [0, 1, 0], atol=1e-6)
+
+
+def test_colour_effects_identities():
+    img = (RNG.random((13, 19, 4)) * 255).astype(np.uint8)
+    flat = np.tile(np.array([40, 120, 200, 77], np.uint8), (13, 19, 1))
+    assert np.array_equal(O.color_effect(pyfae.effect("film_grain", intensity=0.0), img), img)
+    assert np.array_equal(O.color_effect(pyfae.effect("chromatic_aberration", amount=0.0), img),
+                          img)
+    assert np.array_equal(O.color_effect(pyfae.effect("sobel_edges"), flat), flat)
+    for off in (0.0, 360.0):
+        d = O.color_effect(pyfae.effect("hue_shift", offset=off), img).astype(int) - img
+        assert np.abs(d).max() <= 1 and np.array_equal(d[..., 3], 0 * d[..., 3])
+    red = np.tile(np.array([255, 0, 0, 255], np.uint8), (3, 3, 1))
+    assert np.array_equal(O.color_effect(pyfae.effect("hue_shift", offset=120.0), red)[1, 1],
+                          [0, 255, 0, 255])
+    # the vignette leaves the centre of an odd-sized image as it is and
+    # darkens a far corner fully at radius 0, softness 0.1
+    v = O.color_effect(pyfae.effect("vignette", intensity=1.0, radius=0.0, softness=0.1), img)
+    assert np.array_equal(v[6, 9], img[6, 9]) and np.array_equal(v[0, 0, :3], [0, 0, 0])
+    assert np.array_equal(v[..., 3], img[..., 3])
+    # film grain is a value in [0, 1), the same at every call, different per seed
+    g = pyfae.effect("film_grain", intensity=1.0, blend_mode=0)
+    a = O.color_effect(g, img)
+    assert np.array_equal(a, O.color_effect(g, img)) and np.all(a[..., 0] == a[..., 1])
+    g2 = pyfae.effect("film_grain", intensity=1.0, blend_mode=0, seed=3.0)
+    assert not np.array_equal(a, O.color_effect(g2, img))
+    assert 100 < a[..., 0].mean() < 155  # uniform-ish grain
+
+
+def test_sobel_finds_a_vertical_edge():
+    img = np.zeros((8, 16, 4), np.uint8)
+    img[:, 8:, :3] = 255
+    img[..., 3] = 255
+    out = O.color_effect(pyfae.effect("sobel_edges", blend_mode=0, preserve_color=0.0, glow=0.0),
+                         img)
+    # edge columns glow white (gain 1 * smoothstep = 1), flat regions keep nothing
+    assert np.all(out[:, 7:9, :3] == 255) and np.all(out[:, :6, :3] == 0)
+
+
+# ------------------------------------------------------------ config lists
+
+def cfg_with(density=(), color=()):
+    t = json.load(open("tests/golden/ref_simple_linear.fflame"))
+    t["density_effects"] = list(density)
+    t["color_effects"] = list(color)
+    return json.dumps(t)
+
+
+def test_config_effect_lists_load():
+    sc = pyfae.load(cfg_with(
+        [{"effect_type": "density_blur", "enabled": True, "params": {"radius": 5.0}},
+         {"effect_type": "sharpen", "enabled": False, "params": {}}],
+        [{"effect_type": "vignette", "enabled": True, "params": {"blend_mode": 2.0, "x": 9}},
+         {"effect_type": "hue_shift", "enabled": True}]))
+    fx = sc.effects
+    assert (fx.n_density, fx.n_color, fx.pixel_scale) == (1, 2, 1.0)
+    assert fx.density[0].type == 1 and list(fx.density[0].params[:3]) == [5.0, 0.25, 0.5]
+    assert fx.color[0].type == 16 and list(fx.color[0].params[:4]) == [0.5, 0.5, 0.5, 2.0]
+    assert fx.color[1].type == 19 and list(fx.color[1].params[:3]) == [0.0, 1.0, 0.0]
+    assert pyfae.load(cfg_with()).effects.n_density == 0
+
+
+def test_config_effects_this_engine_lacks_are_refused():
+    for d, c in (([{"effect_type": "bloom", "enabled": True}], []),
+                 ([], [{"effect_type": "plasma", "enabled": True}]),
+                 ([{"effect_type": "vignette", "enabled": True}], [])):
+        with pytest.raises(pyfae.EngineError) as e:
+            pyfae.load(cfg_with(d, c))
+        assert e.value.status == -10
+    # a disabled one is dropped, whatever it is
+    assert pyfae.load(cfg_with([{"effect_type": "bloom", "enabled": False}])).effects.n_density == 0
+
+
+def test_effect_builders():
+    fx = pyfae.effects(density=["sharpen", ("density_blur", {"radius": 2})],
+                       color=[{"effect_type": "sobel_edges", "enabled": True,
+                               "params": {"glow": 0.1}}], pixel_scale=2.0)
+    assert (fx.n_density, fx.n_color, fx.pixel_scale) == (2, 1, 2.0)
+    assert fx.density[1].params[0] == 2.0 and abs(fx.color[0].params[2] - 0.1) < 1e-7
+    with pytest.raises(ValueError):
+        pyfae.effects(density=["vignette"])
+    with pytest.raises(ValueError):
+        pyfae.effect("sharpen", sigma=1)

@@ -238,7 +238,8 @@ def test_palette_forms():
     ({"render_mode": "escape", "version": 7}, "escape"),
     ({"color_mode": "speed"}, "color mode"),
     ({"highlight_mode": "agx"}, "highlight"),
-    ({"density_effects": [{"kind": "x"}]}, "density effects"),
+    ({"density_effects": [{"effect_type": "bloom", "enabled": True}]}, "density effect 'bloom'"),
+    ({"color_effects": [{"effect_type": "kaleidoscope", "enabled": True}]}, "kaleidoscope"),
     ({"auto_exposure": True}, "auto exposure"),
     ({"use_curve": True, "tonemap_curve": {"points": [{"x": 0, "y": 0}, {"x": 0.5, "y": 0.7},
                                                       {"x": 1, "y": 1}]}}, "tone curve"),
