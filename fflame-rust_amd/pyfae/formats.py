@@ -241,7 +241,21 @@ def scene_to_config(sc, palette_name="Imported"):
         "levels_enabled": bool(tp.levels_enabled), "levels_low": _f(tp.levels_low),
         "levels_high": _f(tp.levels_high), "levels_gamma": _f(tp.levels_gamma),
         "deterministic_rng": bool(sc.deterministic_rng),
+        "density_effects": _effects_json(sc.effects.density, sc.effects.n_density),
+        "color_effects": _effects_json(sc.effects.color, sc.effects.n_color),
     }
+
+
+def _effects_json(lst, n):
+    """An effect chain as the config's EffectInstance list (effect_type,
+    enabled, params by name; src/effects/mod.rs:319-336)."""
+    out = []
+    for e in lst[:n]:
+        name = next(k for k, (t, _, _) in _abi.EFFECTS.items() if t == e.type)
+        names = _abi.EFFECTS[name][2]
+        out.append({"effect_type": name, "enabled": bool(e.enabled),
+                    "params": {k: _f(e.params[i]) for i, k in enumerate(names)}})
+    return out
 
 
 def scene_to_json(sc, **kw):
@@ -277,4 +291,5 @@ def scene_summary(sc):
         "render": (sc.filter_radius, sc.filter_blur_edges, sc.max_iterations, sc.reset_orbits,
                    sc.deterministic_rng, sc.render_3d),
         "tonemap": bytes(C.string_at(C.addressof(tp), C.sizeof(tp))),
+        "effects": bytes(C.string_at(C.addressof(sc.effects), C.sizeof(sc.effects))),
     }

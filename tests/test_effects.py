@@ -243,3 +243,14 @@ def test_effect_builders():
         pyfae.effects(density=["vignette"])
     with pytest.raises(ValueError):
         pyfae.effect("sharpen", sigma=1)
+
+
+def test_effect_lists_round_trip_through_scene_to_config():
+    text = cfg_with([{"effect_type": "sharpen", "enabled": True, "params": {"amount": 0.3}}],
+                    [{"effect_type": "film_grain", "enabled": True, "params": {"seed": 9}},
+                     {"effect_type": "sobel_edges", "enabled": True, "params": {}}])
+    sc = pyfae.load(text)
+    again = pyfae.load(pyfae.formats.scene_to_json(sc))
+    assert pyfae.formats.scene_summary(again) == pyfae.formats.scene_summary(sc)
+    assert again.effects.n_density == 1 and again.effects.n_color == 2
+    assert abs(again.effects.density[0].params[0] - 0.3) < 1e-7
