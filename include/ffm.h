@@ -522,6 +522,7 @@ int ffm_scene_job(const ffm_scene* scene, int width, int height, uint64_t iterat
 /* pyfae's Engine.render(config, width, height, iterations=,
  * iterations_per_thread=) (python/pyfae/__init__.py:201-221; the engine's
  * "render" request): parse config 0 of `text`, set its flame and palette,
+ * render with its effects (not the context's),
  * render, RGBA8 into rgba_out (host, width*height*4). */
 int ffm_render_config(ffm_ctx* ctx, const char* text, size_t len, int width, int height,
                       uint64_t iterations, uint32_t ipt, uint32_t flags, uint8_t* rgba_out,
