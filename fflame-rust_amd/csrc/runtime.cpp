@@ -568,7 +568,9 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   if (n_tiles > TILE_MAX)
     return fail(c, FFM_ERR_TOO_LARGE, "the LDS tile splat holds %d bins of up to 128 %dx%d tiles; "
                 "%dx%d needs more", TILE_MAX, TILE_W, TILE_H, W, H);
-  if (bin_lds_bytes(a.n_rows, a.pal_size) > LDS_PER_CU ||
+  int walk_block = 0, stage = 0;
+  bin_shape(a.n_rows, a.pal_size, &walk_block, &stage);
+  if (bin_lds_bytes(a.n_rows, a.pal_size, stage) > LDS_PER_CU ||
       accum_lds_bytes(a.pal_size, f32) > LDS_PER_CU)
     return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
   const uint64_t per_walker = (uint64_t)a.ipt + 1;
@@ -601,7 +603,8 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   t.width = W;
   t.height = H;
   t.flush_every = 4;
-  t.stage = bin_stage(a.n_rows, a.pal_size);
+  t.stage = stage;
+  t.walk_block = walk_block;
   const int grid = std::max(c->num_cu, 1);
   const bool same = c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
                     c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
@@ -1550,7 +1553,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.g.hist = reinterpret_cast<uint32_t*>(P.hist.p);
   b.g.n_tiles = P.n_tiles;
   b.g.flush_every = flush_every;
-  b.g.stage = bin_stage(max_rows, BATCH_PALETTE);
+  bin_shape(max_rows, BATCH_PALETTE, &b.g.walk_block, &b.g.stage);
   const int grid = std::max(c->num_cu, 1);
   const BatchItem* items_d = reinterpret_cast<const BatchItem*>(P.items.p);
   HIPCHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -30,27 +30,45 @@ namespace ffm {
 
 constexpr int BIN_BLOCK = 1024;
 // Staged records per workgroup: what the LDS left over by the flame table
-// and palette holds, in steps of 1024, between these bounds (the upper one
-// keeps record indices in the u16 permutation).
+// and palette holds, in steps of 1024, up to STAGE_MAX (record indices
+// stay within the u16 permutation).
 constexpr int STAGE_MIN = 10240, STAGE_MAX = 16384;
 constexpr int MAX_FLUSH_EVERY = 8;  // iterations between fill checks, at most
 constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
-
 constexpr size_t STAGE_FIXED = (size_t)TILE_MAX * 8 + 16 * 4 + 16;  // cnt, gbase, wsum, fill
 constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4 + 2;                  // rec, tr, perm
 
-int bin_stage(int n_rows, int pal_size) {
-  const size_t flame = flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4;
-  const size_t used = flame + STAGE_FIXED;
-  const size_t fit = used < (size_t)LDS_DYNAMIC_MAX
-                         ? ((size_t)LDS_DYNAMIC_MAX - used) / STAGE_BYTES_PER_RECORD / 1024 * 1024
-                         : 0;
-  return (int)std::min<size_t>(std::max<size_t>(fit, STAGE_MIN), STAGE_MAX);
+// Walker workgroups: two of WALK_HALF lanes per CU when the flame table,
+// palette and a stage of at least STAGE_HALF_MIN records fit in half the
+// CU's LDS, so one workgroup's flushes overlap the other's walking; else
+// one of BIN_BLOCK lanes with the whole budget.
+constexpr int WALK_HALF = 512;
+constexpr size_t LDS_HALF = 78 * 1024;
+constexpr int STAGE_HALF_MIN = 4096;
+
+static size_t flame_lds(int n_rows, int pal_size) {
+  return flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4;
 }
 
-size_t bin_lds_bytes(int n_rows, int pal_size) {
-  return flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4 + STAGE_FIXED +
-         (size_t)bin_stage(n_rows, pal_size) * STAGE_BYTES_PER_RECORD;
+static int stage_fit(size_t budget, size_t used) {
+  const size_t fit = used < budget ? (budget - used) / STAGE_BYTES_PER_RECORD / 1024 * 1024 : 0;
+  return (int)std::min<size_t>(fit, STAGE_MAX);
+}
+
+void bin_shape(int n_rows, int pal_size, int* block, int* stage) {
+  const size_t used = flame_lds(n_rows, pal_size) + STAGE_FIXED;
+  const int half = stage_fit(LDS_HALF, used);
+  if (half >= STAGE_HALF_MIN) {
+    *block = WALK_HALF;
+    *stage = half;
+    return;
+  }
+  *block = BIN_BLOCK;
+  *stage = std::max(stage_fit((size_t)LDS_DYNAMIC_MAX, used), STAGE_MIN);
+}
+
+size_t bin_lds_bytes(int n_rows, int pal_size, int stage) {
+  return flame_lds(n_rows, pal_size) + STAGE_FIXED + (size_t)stage * STAGE_BYTES_PER_RECORD;
 }
 
 size_t accum_lds_bytes(int pal_size, bool f32) {
@@ -93,7 +111,7 @@ struct BinLds {
   uint32_t* cnt;    // TILE_MAX
   uint32_t* gbase;  // TILE_MAX
   uint32_t* wsum;   // 16
-  uint32_t* fill;   // 1
+  uint32_t* fill;   // 1, then one word of scan total
 };
 
 // Tile column of a bin's sub-tile.
@@ -149,7 +167,22 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
                    : 0u;
   }
   __syncthreads();
-  block_exclusive_scan(L.cnt, t.n_tiles, L.wsum, nullptr);  // cnt -> start
+  // cnt -> start; more bins than 2 x blockDim go 2 x blockDim at a time
+  // with a running carry
+  if (t.n_tiles <= 2 * (int)blockDim.x) {
+    block_exclusive_scan(L.cnt, t.n_tiles, L.wsum, nullptr);
+  } else {
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < t.n_tiles; c0 += 2 * (int)blockDim.x) {
+      const int m = min(2 * (int)blockDim.x, t.n_tiles - c0);
+      block_exclusive_scan(L.cnt + c0, m, L.wsum, L.fill + 1);
+      const uint32_t total = L.fill[1];
+      if (c0 > 0)
+        for (int i = tid; i < m; i += blockDim.x) L.cnt[c0 + i] += carry;
+      carry += total;
+      __syncthreads();
+    }
+  }
   for (int j = tid; j < n; j += blockDim.x) {
     const uint32_t v = L.tr[j];
     L.perm[L.cnt[v & 0xFFFFu] + (v >> 16)] = (uint16_t)j;
@@ -628,9 +661,10 @@ static hipError_t launch_bin_iterate_t(const IterArgs& a, const TileArgs& t, int
   static const hipError_t e =
       allow_lds(reinterpret_cast<const void*>(&bin_iterate_kernel<COUNT_ONLY, EXT>));
   if (e != hipSuccess) return e;
-  const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size);
-  hipLaunchKernelGGL((bin_iterate_kernel<COUNT_ONLY, EXT>), dim3(grid), dim3(BIN_BLOCK), shm, s,
-                     a, t);
+  const int blk = t.walk_block;
+  const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size, t.stage);
+  hipLaunchKernelGGL((bin_iterate_kernel<COUNT_ONLY, EXT>), dim3(grid * (BIN_BLOCK / blk)),
+                     dim3(blk), shm, s, a, t);
   return hipGetLastError();
 }
 
@@ -648,8 +682,10 @@ static hipError_t launch_bin_batch_t(const BatchArgs& b, int grid, hipStream_t s
   static const hipError_t e =
       allow_lds(reinterpret_cast<const void*>(&bin_batch_kernel<COUNT_ONLY, EXT>));
   if (e != hipSuccess) return e;
-  const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal);
-  hipLaunchKernelGGL((bin_batch_kernel<COUNT_ONLY, EXT>), dim3(grid), dim3(BIN_BLOCK), shm, s, b);
+  const int blk = b.g.walk_block;
+  const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal, b.g.stage);
+  hipLaunchKernelGGL((bin_batch_kernel<COUNT_ONLY, EXT>), dim3(grid * (BIN_BLOCK / blk)),
+                     dim3(blk), shm, s, b);
   return hipGetLastError();
 }
 
