@@ -383,6 +383,88 @@ __device__ void sobel_edges(const ImgU8& img, const FxConst& k, float4 c, float 
   o[2] = (1.0f * (1.0f - pc) + c.z * pc) * gain;
 }
 
+// The effect language's atan2: IEEE at (0, 0) (+-0 or +-pi by the signs),
+// the platform's elsewhere (src/lang/wgsl.rs uf_atan2).
+__device__ __forceinline__ float uf_atan2(float y, float x) {
+  if (y == 0.0f && x == 0.0f) {
+    const float mag = (__float_as_uint(x) & 0x80000000u) ? 3.14159265358979f : 0.0f;
+    return (__float_as_uint(y) & 0x80000000u) ? -mag : mag;
+  }
+  return atan2f(y, x);
+}
+
+__device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
+
+constexpr float FX_PI = 3.14159265359f;
+
+// kaleidoscope.rhai: one slice of the image folded into mirrored segments.
+__device__ void kaleidoscope(const ImgU8& img, const FxConst& k, float u, float v, float o[3]) {
+  const float two_pi = 6.28318530718f;
+  const float segments = fmaxf(2.0f, k.p[0]);
+  const float rotation = k.p[1] * FX_PI / 180.0f;
+  const float zoom = fmaxf(0.1f, k.p[2]);
+  const bool square = k.p[5] > 0.0f;
+  const float edge_offset = k.p[6] * k.pixel_scale;
+  const float aspect = (float)k.W / (float)k.H;
+  float x = (u - 0.5f) / zoom, y = (v - 0.5f) / zoom;
+  if (square) x = x * aspect;
+  const float radius = length2(x, y);
+  float angle = uf_atan2(y, x);
+  const float segment = two_pi / segments;
+  angle = angle - segment * floorf(angle / segment);
+  if (angle > segment * 0.5f) angle = segment - angle;
+  angle = angle + rotation;
+  float rx = cosf(angle) * radius;
+  const float ry = sinf(angle) * radius;
+  if (square) rx = rx / aspect;
+  const float dx = -cosf(rotation), dy = -sinf(rotation);
+  const float uu = rx + 0.5f + dx * edge_offset * (1.0f / (float)k.W);
+  const float vv = ry + 0.5f + dy * edge_offset * (1.0f / (float)k.H);
+  const float4 s = tex(img, fminf(fmaxf(uu, 0.001f), 0.999f), fminf(fmaxf(vv, 0.001f), 0.999f));
+  o[0] = s.x;
+  o[1] = s.y;
+  o[2] = s.z;
+}
+
+// plasma.rhai: the demoscene plasma, a sum of sines travelling along
+// `direction`.
+__device__ void plasma(const FxConst& k, float u, float v, float o[3]) {
+  const float scale = fmaxf(0.5f, k.p[1]);
+  const float time = k.p[3] * k.p[2];
+  const float direction = k.p[5] * FX_PI / 180.0f;
+  const float x = u * scale, y = v * scale;
+  const float dir_phase = x * cosf(direction) + y * sinf(direction);
+  const float t = time + dir_phase;
+  float w = sinf(x + t);
+  w += sinf((y + t) * 0.5f);
+  w += sinf((x + y + t) * 0.5f);
+  const float cx = x + 0.5f * sinf(t * 0.25f), cy = y + 0.5f * cosf(t * 0.5f);
+  w += sinf(sqrtf(cx * cx + cy * cy + 1.0f) + t);
+  w = w * 0.25f;
+  o[0] = sinf(w * FX_PI) * 0.5f + 0.5f;
+  o[1] = sinf(w * FX_PI + 2.094f) * 0.5f + 0.5f;
+  o[2] = sinf(w * FX_PI + 4.189f) * 0.5f + 0.5f;
+}
+
+// tunnel.rhai: the image wrapped on the inside of a tunnel; the centre,
+// where the tunnel has no wall, keeps the input (amount 0).
+__device__ float tunnel(const ImgU8& img, const FxConst& k, float u, float v, float o[3]) {
+  const float aspect = (float)k.W / (float)k.H;
+  const float x = (u - 0.5f) * aspect, y = v - 0.5f;
+  const float angle = uf_atan2(y, x) / FX_PI;
+  const float radius = length2(x, y);
+  const float wall = fmaxf(radius, 0.001f);
+  const float r = wall + k.p[2] * sinf(angle * FX_PI * 4.0f + k.p[3]) * 0.1f;
+  const float au = fractf(angle * 0.5f + 0.5f + k.p[3] * k.p[1] * 0.1f);
+  const float av = fractf(1.0f / r + k.p[3] * k.p[0]);
+  const float4 s = tex(img, au, av);
+  const float fade = smooth(0.0f, 0.5f, radius);
+  o[0] = s.x * fade;
+  o[1] = s.y * fade;
+  o[2] = s.z * fade;
+  return radius < 0.001f ? 0.0f : k.p[4];
+}
+
 __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restrict__ in,
                                                            uchar4* __restrict__ out, FxConst k) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -418,6 +500,20 @@ __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restr
       sobel_edges(img, k, c, u, v, e);
       amount = k.p[0];
       mode = (int)k.p[4];
+      break;
+    case FFM_FX_KALEIDOSCOPE:
+      kaleidoscope(img, k, u, v, e);
+      amount = k.p[3];
+      mode = (int)k.p[4];
+      break;
+    case FFM_FX_PLASMA:
+      plasma(k, u, v, e);
+      amount = k.p[0];
+      mode = (int)k.p[4];
+      break;
+    case FFM_FX_TUNNEL:
+      amount = tunnel(img, k, u, v, e);
+      mode = (int)k.p[5];
       break;
     default:
       e[0] = c.x;

@@ -974,6 +974,9 @@ static int fx_chain(int32_t type) {
     case FFM_FX_CHROMATIC_ABERRATION:
     case FFM_FX_HUE_SHIFT:
     case FFM_FX_SOBEL_EDGES:
+    case FFM_FX_KALEIDOSCOPE:
+    case FFM_FX_PLASMA:
+    case FFM_FX_TUNNEL:
       return 2;
     default:
       return 0;
@@ -996,6 +999,9 @@ int ffm_effect_defaults(int32_t type, ffm_effect* out) {
     case FFM_FX_CHROMATIC_ABERRATION: v = {2.0f, 1.0f, 1.0f, 0.0f}; break;
     case FFM_FX_HUE_SHIFT: v = {0.0f, 1.0f, 0.0f}; break;              // hue_shift.rhai
     case FFM_FX_SOBEL_EDGES: v = {1.0f, 0.1f, 0.5f, 0.5f, 1.0f}; break;  // sobel_edges.rhai
+    case FFM_FX_KALEIDOSCOPE: v = {6.0f, 0.0f, 1.0f, 1.0f, 0.0f, 1.0f, 0.0f}; break;
+    case FFM_FX_PLASMA: v = {0.3f, 3.0f, 1.0f, 0.0f, 1.0f, 225.0f}; break;
+    case FFM_FX_TUNNEL: v = {1.0f, 0.5f, 0.0f, 0.0f, 1.0f, 0.0f}; break;
     default: return FFM_ERR_INVALID_ARGUMENT;
   }
   for (size_t i = 0; i < v.size(); i++) e.params[i] = v[i];

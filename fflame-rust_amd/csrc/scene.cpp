@@ -741,6 +741,11 @@ const FxName kEffects[] = {
     {"hue_shift", FFM_FX_HUE_SHIFT, 2, {"offset", "intensity", "blend_mode"}},
     {"sobel_edges", FFM_FX_SOBEL_EDGES, 2,
      {"intensity", "threshold", "glow", "preserve_color", "blend_mode"}},
+    {"kaleidoscope", FFM_FX_KALEIDOSCOPE, 2,
+     {"segments", "rotation", "zoom", "intensity", "blend_mode", "square_mode", "edge_offset"}},
+    {"plasma", FFM_FX_PLASMA, 2, {"intensity", "scale", "speed", "time", "blend_mode", "direction"}},
+    {"tunnel", FFM_FX_TUNNEL, 2,
+     {"speed", "rotation_speed", "distortion", "time", "intensity", "blend_mode"}},
 };
 
 // A config's effect list (Vec<EffectInstance>, src/effects/mod.rs:319-336:

@@ -136,6 +136,11 @@ EFFECTS = {
     "hue_shift": (19, "color", ("offset", "intensity", "blend_mode")),
     "sobel_edges": (20, "color", ("intensity", "threshold", "glow", "preserve_color",
                                   "blend_mode")),
+    "kaleidoscope": (21, "color", ("segments", "rotation", "zoom", "intensity", "blend_mode",
+                                   "square_mode", "edge_offset")),
+    "plasma": (22, "color", ("intensity", "scale", "speed", "time", "blend_mode", "direction")),
+    "tunnel": (23, "color", ("speed", "rotation_speed", "distortion", "time", "intensity",
+                             "blend_mode")),
 }
 BLEND_MODES = ("Normal", "Add", "Multiply", "Screen", "Overlay", "Soft Light", "Hard Light",
                "Color Dodge", "Color Burn", "Hue", "Saturation", "Color", "Luminosity")

@@ -432,8 +432,14 @@ enum ffm_effect_type {
   FFM_FX_FILM_GRAIN = 17,            /* intensity 0.1, seed 0, blend_mode 4 */
   FFM_FX_CHROMATIC_ABERRATION = 18,  /* amount 2, radial 1, intensity 1, blend_mode 0 */
   FFM_FX_HUE_SHIFT = 19,             /* offset 0 (degrees), intensity 1, blend_mode 0 */
-  FFM_FX_SOBEL_EDGES = 20            /* intensity 1, threshold 0.1, glow 0.5, preserve_color 0.5,
+  FFM_FX_SOBEL_EDGES = 20,           /* intensity 1, threshold 0.1, glow 0.5, preserve_color 0.5,
                                         blend_mode 1 */
+  FFM_FX_KALEIDOSCOPE = 21,          /* segments 6, rotation 0 (degrees), zoom 1, intensity 1,
+                                        blend_mode 0, square_mode 1, edge_offset 0 */
+  FFM_FX_PLASMA = 22,                /* intensity 0.3, scale 3, speed 1, time 0, blend_mode 1,
+                                        direction 225 (degrees) */
+  FFM_FX_TUNNEL = 23                 /* speed 1, rotation_speed 0.5, distortion 0, time 0,
+                                        intensity 1, blend_mode 0 */
 };
 /* An EffectInstance (src/effects/mod.rs:319-336): params in the effect's
  * declared order (the slot order of get_param), as listed above. */

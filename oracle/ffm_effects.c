@@ -269,6 +269,19 @@ void orc_apply_blend(const float base[3], const float e[3], int mode, float amou
 
 /* ---- colour effects ------------------------------------------------------ */
 
+#define FX_PI 3.14159265359f
+
+/* the effect language's atan2 (src/lang/wgsl.rs uf_atan2): IEEE at (0, 0) */
+static float uf_atan2(float y, float x) {
+  if (y == 0.0f && x == 0.0f) {
+    const float mag = signbit(x) ? 3.14159265358979f : 0.0f;
+    return signbit(y) ? -mag : mag;
+  }
+  return atan2f(y, x);
+}
+
+static float fractf(float x) { return x - floorf(x); }
+
 static uint8_t unorm8(float v) {
   v = (v != v) ? 0.0f : clamp01(v);
   return (uint8_t)floorf(v * 255.0f + 0.5f);
@@ -379,6 +392,70 @@ void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int
           for (int k = 0; k < 3; k++) e[k] = (1.0f * (1.0f - pc) + c.v[k] * pc) * gain;
           amount = p[0];
           mode = (int)p[4];
+          break;
+        }
+        case FFM_FX_KALEIDOSCOPE: {
+          const float two_pi = 6.28318530718f;
+          const float segments = fmaxf(2.0f, p[0]);
+          const float rotation = p[1] * FX_PI / 180.0f;
+          const float zoom = fmaxf(0.1f, p[2]);
+          const int square = p[5] > 0.0f;
+          const float edge_offset = p[6] * pixel_scale;
+          const float aspect = (float)W / (float)H;
+          float x = (u - 0.5f) / zoom, y = (v - 0.5f) / zoom;
+          if (square) x = x * aspect;
+          const float radius = len2(x, y);
+          float angle = uf_atan2(y, x);
+          const float segment = two_pi / segments;
+          angle = angle - segment * floorf(angle / segment);
+          if (angle > segment * 0.5f) angle = segment - angle;
+          angle = angle + rotation;
+          float rx = cosf(angle) * radius;
+          const float ry = sinf(angle) * radius;
+          if (square) rx = rx / aspect;
+          const float dx = -cosf(rotation), dy = -sinf(rotation);
+          const float uu = rx + 0.5f + dx * edge_offset * (1.0f / (float)W);
+          const float vv = ry + 0.5f + dy * edge_offset * (1.0f / (float)H);
+          const px4 s = tex(&im, fminf(fmaxf(uu, 0.001f), 0.999f), fminf(fmaxf(vv, 0.001f), 0.999f));
+          for (int q = 0; q < 3; q++) e[q] = s.v[q];
+          amount = p[3];
+          mode = (int)p[4];
+          break;
+        }
+        case FFM_FX_PLASMA: {
+          const float scale = fmaxf(0.5f, p[1]);
+          const float time = p[3] * p[2];
+          const float direction = p[5] * FX_PI / 180.0f;
+          const float x = u * scale, y = v * scale;
+          const float dir_phase = x * cosf(direction) + y * sinf(direction);
+          const float t = time + dir_phase;
+          float w = sinf(x + t);
+          w += sinf((y + t) * 0.5f);
+          w += sinf((x + y + t) * 0.5f);
+          const float cx = x + 0.5f * sinf(t * 0.25f), cy = y + 0.5f * cosf(t * 0.5f);
+          w += sinf(sqrtf(cx * cx + cy * cy + 1.0f) + t);
+          w = w * 0.25f;
+          e[0] = sinf(w * FX_PI) * 0.5f + 0.5f;
+          e[1] = sinf(w * FX_PI + 2.094f) * 0.5f + 0.5f;
+          e[2] = sinf(w * FX_PI + 4.189f) * 0.5f + 0.5f;
+          amount = p[0];
+          mode = (int)p[4];
+          break;
+        }
+        case FFM_FX_TUNNEL: {
+          const float aspect = (float)W / (float)H;
+          const float x = (u - 0.5f) * aspect, y = v - 0.5f;
+          const float angle = uf_atan2(y, x) / FX_PI;
+          const float radius = len2(x, y);
+          const float wall = fmaxf(radius, 0.001f);
+          const float r = wall + p[2] * sinf(angle * FX_PI * 4.0f + p[3]) * 0.1f;
+          const float au = fractf(angle * 0.5f + 0.5f + p[3] * p[1] * 0.1f);
+          const float av = fractf(1.0f / r + p[3] * p[0]);
+          const px4 s = tex(&im, au, av);
+          const float fade = smooth(0.0f, 0.5f, radius);
+          for (int q = 0; q < 3; q++) e[q] = s.v[q] * fade;
+          amount = radius < 0.001f ? 0.0f : p[4];
+          mode = (int)p[5];
           break;
         }
         default: break;

@@ -224,7 +224,7 @@ def test_config_effect_lists_load():
 
 def test_config_effects_this_engine_lacks_are_refused():
     for d, c in (([{"effect_type": "bloom", "enabled": True}], []),
-                 ([], [{"effect_type": "plasma", "enabled": True}]),
+                 ([], [{"effect_type": "worley_noise", "enabled": True}]),
                  ([{"effect_type": "vignette", "enabled": True}], [])):
         with pytest.raises(pyfae.EngineError) as e:
             pyfae.load(cfg_with(d, c))
@@ -254,3 +254,30 @@ def test_effect_lists_round_trip_through_scene_to_config():
     assert pyfae.formats.scene_summary(again) == pyfae.formats.scene_summary(sc)
     assert again.effects.n_density == 1 and again.effects.n_color == 2
     assert abs(again.effects.density[0].params[0] - 0.3) < 1e-7
+
+
+def test_generator_effects_closed_forms():
+    H, W = 31, 31
+    img = (RNG.random((H, W, 4)) * 255).astype(np.uint8)
+    for name in ("kaleidoscope", "plasma", "tunnel"):
+        assert np.array_equal(O.color_effect(pyfae.effect(name, intensity=0.0), img), img), name
+    # the kaleidoscope folds the angle into mirrored segments: a square
+    # image mirrored top to bottom about its centre row reads the same
+    k = O.color_effect(pyfae.effect("kaleidoscope", segments=5.0), img)
+    same = np.all(k[:, :, :3] == k[::-1, :, :3], axis=-1)
+    assert same.mean() > 0.97
+    # the tunnel keeps the pixel at its centre (no wall there)
+    t = O.color_effect(pyfae.effect("tunnel", time=3.0, distortion=0.5), img)
+    assert np.array_equal(t[15, 15], img[15, 15])
+    # plasma: a sum of sines, against float64 (intensity 1, normal blend)
+    pl = O.color_effect(pyfae.effect("plasma", intensity=1.0, blend_mode=0.0, time=2.0), img)
+    y, x = np.mgrid[0:H, 0:W]
+    u, v = (x + 0.5) / W, (y + 0.5) / H
+    X, Y = u * 3.0, v * 3.0
+    d = np.deg2rad(225.0)
+    t = 2.0 + X * np.cos(d) + Y * np.sin(d)
+    w = np.sin(X + t) + np.sin((Y + t) * 0.5) + np.sin((X + Y + t) * 0.5)
+    cx, cy = X + 0.5 * np.sin(t * 0.25), Y + 0.5 * np.cos(t * 0.5)
+    w = (w + np.sin(np.sqrt(cx * cx + cy * cy + 1.0) + t)) * 0.25
+    want = np.stack([np.sin(w * np.pi + ph) * 0.5 + 0.5 for ph in (0.0, 2.094, 4.189)], -1)
+    assert np.abs(pl[..., :3].astype(float) - np.round(want * 255)).max() <= 1

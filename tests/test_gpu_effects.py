@@ -72,8 +72,18 @@ COLOUR_CASES = [
     ("hue_shift", {"offset": 75.0}), ("hue_shift", {"offset": 200.0, "intensity": 0.5,
                                                     "blend_mode": 11.0}),
     ("sobel_edges", {}), ("sobel_edges", {"threshold": 0.3, "glow": 1.0, "preserve_color": 0.0}),
+    ("kaleidoscope", {}), ("kaleidoscope", {"segments": 9.0, "rotation": 40.0, "zoom": 0.7,
+                                            "square_mode": 0.0, "edge_offset": 30.0}),
+    ("plasma", {}), ("plasma", {"scale": 12.0, "time": 7.5, "direction": 10.0,
+                                "blend_mode": 0.0, "intensity": 1.0}),
+    ("tunnel", {}), ("tunnel", {"time": 4.0, "distortion": 0.7, "speed": 2.0}),
 ] + [("vignette", {"intensity": 0.8, "radius": 0.3, "softness": 0.4, "blend_mode": float(m)})
      for m in range(13)]
+
+# effects whose value wraps (fract) or jumps (atan2's branch, a segment
+# boundary) where one more ulp on one side moves a pixel across: the
+# platform's sin / atan2 bits decide those few pixels
+WRAPS = {"film_grain", "kaleidoscope", "tunnel"}
 
 
 @pytest.mark.parametrize("name,params", COLOUR_CASES)
@@ -88,7 +98,7 @@ def test_colour_effect_equals_oracle(engine, name, params):
     got = dst.cpu().numpy().astype(int)
     want = O.color_effect(fx, img, 1.0).astype(int)
     d = np.abs(got - want)
-    if name == "film_grain":
+    if name in WRAPS:
         # the grain is fract(sin(n) * 43758.5453): a one-ulp difference in
         # sin moves it by ~1e-3 and, next to a whole number, wraps it from
         # ~1 to ~0; such pixels are few
