@@ -12,6 +12,7 @@
 // These are per-pixel stencils of at most 21x21 taps read through L1/L2:
 // bound by the tap loop (VALU, one exp per tap), not by HBM; each pass
 // moves 16 B (density) or 4 B (colour) per pixel in and out.
+#include "ffm_fastmath.h"
 #include "ffm_internal.h"
 
 namespace ffm {
@@ -87,6 +88,21 @@ __device__ __forceinline__ float lum3(float r, float g, float b) {
 }
 
 __device__ __forceinline__ float length2(float x, float y) { return sqrtf(x * x + y * y); }
+
+// sin and cos of the colour effects: Cody-Waite reduction and the Cephes
+// polynomials (ffm_fastmath.h), a fixed method rather than the platform's
+// (whose last bits the reference leaves to the GPU), so the hash-based
+// effects' fract(sin(n) * 43758.5453) is reproducible.
+__device__ __forceinline__ float fx_sin(float x) {
+  float s, c;
+  ffm_sincosf(x, &s, &c);
+  return s;
+}
+__device__ __forceinline__ float fx_cos(float x) {
+  float s, c;
+  ffm_sincosf(x, &s, &c);
+  return c;
+}
 
 // ---------------------------------------------------------- density effects
 
@@ -294,7 +310,7 @@ __device__ void vignette(const FxConst& k, float4 c, float u, float v, float o[3
 __device__ void film_grain(const FxConst& k, float u, float v, float o[3]) {
   const float ax = u * (float)k.W + k.p[1] * 127.1f, ay = v * (float)k.H + k.p[1] * 311.7f;
   const float n = ax * 127.1f + ay * 311.7f;
-  const float s = sinf(n) * 43758.5453f;
+  const float s = fx_sin(n) * 43758.5453f;
   const float g = s - floorf(s);
   o[0] = o[1] = o[2] = g;
 }
@@ -414,10 +430,10 @@ __device__ void kaleidoscope(const ImgU8& img, const FxConst& k, float u, float 
   angle = angle - segment * floorf(angle / segment);
   if (angle > segment * 0.5f) angle = segment - angle;
   angle = angle + rotation;
-  float rx = cosf(angle) * radius;
-  const float ry = sinf(angle) * radius;
+  float rx = fx_cos(angle) * radius;
+  const float ry = fx_sin(angle) * radius;
   if (square) rx = rx / aspect;
-  const float dx = -cosf(rotation), dy = -sinf(rotation);
+  const float dx = -fx_cos(rotation), dy = -fx_sin(rotation);
   const float uu = rx + 0.5f + dx * edge_offset * (1.0f / (float)k.W);
   const float vv = ry + 0.5f + dy * edge_offset * (1.0f / (float)k.H);
   const float4 s = tex(img, fminf(fmaxf(uu, 0.001f), 0.999f), fminf(fmaxf(vv, 0.001f), 0.999f));
@@ -433,17 +449,17 @@ __device__ void plasma(const FxConst& k, float u, float v, float o[3]) {
   const float time = k.p[3] * k.p[2];
   const float direction = k.p[5] * FX_PI / 180.0f;
   const float x = u * scale, y = v * scale;
-  const float dir_phase = x * cosf(direction) + y * sinf(direction);
+  const float dir_phase = x * fx_cos(direction) + y * fx_sin(direction);
   const float t = time + dir_phase;
-  float w = sinf(x + t);
-  w += sinf((y + t) * 0.5f);
-  w += sinf((x + y + t) * 0.5f);
-  const float cx = x + 0.5f * sinf(t * 0.25f), cy = y + 0.5f * cosf(t * 0.5f);
-  w += sinf(sqrtf(cx * cx + cy * cy + 1.0f) + t);
+  float w = fx_sin(x + t);
+  w += fx_sin((y + t) * 0.5f);
+  w += fx_sin((x + y + t) * 0.5f);
+  const float cx = x + 0.5f * fx_sin(t * 0.25f), cy = y + 0.5f * fx_cos(t * 0.5f);
+  w += fx_sin(sqrtf(cx * cx + cy * cy + 1.0f) + t);
   w = w * 0.25f;
-  o[0] = sinf(w * FX_PI) * 0.5f + 0.5f;
-  o[1] = sinf(w * FX_PI + 2.094f) * 0.5f + 0.5f;
-  o[2] = sinf(w * FX_PI + 4.189f) * 0.5f + 0.5f;
+  o[0] = fx_sin(w * FX_PI) * 0.5f + 0.5f;
+  o[1] = fx_sin(w * FX_PI + 2.094f) * 0.5f + 0.5f;
+  o[2] = fx_sin(w * FX_PI + 4.189f) * 0.5f + 0.5f;
 }
 
 // tunnel.rhai: the image wrapped on the inside of a tunnel; the centre,
@@ -454,7 +470,7 @@ __device__ float tunnel(const ImgU8& img, const FxConst& k, float u, float v, fl
   const float angle = uf_atan2(y, x) / FX_PI;
   const float radius = length2(x, y);
   const float wall = fmaxf(radius, 0.001f);
-  const float r = wall + k.p[2] * sinf(angle * FX_PI * 4.0f + k.p[3]) * 0.1f;
+  const float r = wall + k.p[2] * fx_sin(angle * FX_PI * 4.0f + k.p[3]) * 0.1f;
   const float au = fractf(angle * 0.5f + 0.5f + k.p[3] * k.p[1] * 0.1f);
   const float av = fractf(1.0f / r + k.p[3] * k.p[0]);
   const float4 s = tex(img, au, av);
@@ -463,6 +479,124 @@ __device__ float tunnel(const ImgU8& img, const FxConst& k, float u, float v, fl
   o[1] = s.y * fade;
   o[2] = s.z * fade;
   return radius < 0.001f ? 0.0f : k.p[4];
+}
+
+// hsv_to_rgb as julia.rhai and worley_noise.rhai write it.
+__device__ void fx_hsv(float h, float s, float v, float o[3]) {
+  const float c = v * s, h6 = h * 6.0f;
+  const float x = c * (1.0f - fabsf(h6 - 2.0f * truncf(h6 / 2.0f) - 1.0f));
+  const float m = v - c;
+  float r, g, b;
+  if (h6 < 1.0f) {
+    r = c; g = x; b = 0.0f;
+  } else if (h6 < 2.0f) {
+    r = x; g = c; b = 0.0f;
+  } else if (h6 < 3.0f) {
+    r = 0.0f; g = c; b = x;
+  } else if (h6 < 4.0f) {
+    r = 0.0f; g = x; b = c;
+  } else if (h6 < 5.0f) {
+    r = x; g = 0.0f; b = c;
+  } else {
+    r = c; g = 0.0f; b = x;
+  }
+  o[0] = r + m;
+  o[1] = g + m;
+  o[2] = b + m;
+}
+
+// julia.rhai: a Mandelbrot (mode 0) or Julia (mode 1) set, escape-time
+// coloured (color_mode 0), smooth (1) or masking the input (2).
+__device__ void julia(const FxConst& k, float4 c, float u, float v, float o[3]) {
+  const float max_iter = fmaxf(10.0f, k.p[3]);
+  const float zoom = fmaxf(0.1f, k.p[4]);
+  const float aspect = (float)k.W / (float)k.H;
+  const float px = (u - 0.5f) * 2.0f * aspect / zoom + k.p[5];
+  const float py = (v - 0.5f) * 2.0f / zoom + k.p[6];
+  const bool is_julia = k.p[0] == 1.0f;
+  float zr = is_julia ? px : 0.0f, zi = is_julia ? py : 0.0f;
+  const float kr = is_julia ? k.p[1] : px, ki = is_julia ? k.p[2] : py;
+  int i = 0;
+  float x2 = zr * zr, y2 = zi * zi;
+  while ((float)i < max_iter && !(x2 + y2 > 4.0f)) {
+    const float nr = x2 - y2 + kr;
+    const float ni = 2.0f * zr * zi + ki;
+    zr = nr;
+    zi = ni;
+    x2 = zr * zr;
+    y2 = zi * zi;
+    i += 1;
+  }
+  const float iter = (float)i;
+  const float zmag = zr * zr + zi * zi;
+  if (iter >= max_iter) {
+    o[0] = o[1] = o[2] = 0.0f;
+  } else if (k.p[7] == 1.0f) {
+    const float ln2 = logf(2.0f);
+    const float log_zn = logf(zmag) / 2.0f;
+    const float nu = logf(log_zn / ln2) / ln2;
+    const float t = (iter + 1.0f - nu) / max_iter;
+    fx_hsv(fractf(t * 5.0f), 0.8f, 1.0f, o);
+  } else if (k.p[7] == 2.0f) {
+    const float f = iter / max_iter;
+    o[0] = c.x * f;
+    o[1] = c.y * f;
+    o[2] = c.z * f;
+  } else {
+    const float t = iter / max_iter;
+    fx_hsv(fractf(t * 3.0f), 0.7f, t, o);
+  }
+}
+
+// The effect language's hash2: fract(sin(p . (127.1, 311.7), p . (269.5,
+// 183.3)) * 43758.5453) (src/lang/wgsl.rs uf_hash2).
+__device__ __forceinline__ void fx_hash2(float x, float y, float& hx, float& hy) {
+  const float n0 = x * 127.1f + y * 311.7f, n1 = x * 269.5f + y * 183.3f;
+  hx = fractf(fx_sin(n0) * 43758.5453f);
+  hy = fractf(fx_sin(n1) * 43758.5453f);
+}
+
+// worley_noise.rhai: distances to the nearest and second-nearest of the
+// moving points in the 3x3 cells around.
+__device__ void worley(const FxConst& k, float4 c, float u, float v, float o[3]) {
+  const float scale = fmaxf(1.0f, k.p[1]);
+  const float time = k.p[2];
+  const float direction = k.p[6] * FX_PI / 180.0f;
+  const float ax = (u + fx_cos(direction) * time * 0.1f) * scale;
+  const float ay = (v + fx_sin(direction) * time * 0.1f) * scale;
+  const float cx = floorf(ax), cy = floorf(ay);
+  const float fx = fractf(ax), fy = fractf(ay);
+  float d1 = 8.0f, d2 = 8.0f;
+  for (int j = -1; j < 2; j++) {
+    for (int i = -1; i < 2; i++) {
+      const float nx = (float)i, ny = (float)j;
+      float hx, hy;
+      fx_hash2(cx + nx, cy + ny, hx, hy);
+      const float qx = 0.5f + 0.5f * fx_sin(time * 0.5f + 6.2831f * hx);
+      const float qy = 0.5f + 0.5f * fx_sin(time * 0.5f + 6.2831f * hy);
+      const float dist = length2(nx + qx - fx, ny + qy - fy);
+      if (dist < d1) {
+        d2 = d1;
+        d1 = dist;
+      } else if (dist < d2) {
+        d2 = dist;
+      }
+    }
+  }
+  const float mode = k.p[3];
+  const float pattern = mode == 1.0f   ? smooth(0.0f, 0.15f, d2 - d1)
+                        : mode == 2.0f ? 1.0f - smooth(0.0f, 0.5f, d1)
+                        : mode == 3.0f ? clamp01(d1 * d2 * 2.0f)
+                                       : d1;
+  if (k.p[4] == 1.0f) {
+    fx_hsv(fractf(d1 + time * 0.1f), 0.8f, pattern, o);
+  } else if (k.p[4] == 2.0f) {
+    o[0] = c.x * pattern;
+    o[1] = c.y * pattern;
+    o[2] = c.z * pattern;
+  } else {
+    o[0] = o[1] = o[2] = pattern;
+  }
 }
 
 __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restrict__ in,
@@ -513,6 +647,16 @@ __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restr
       break;
     case FFM_FX_TUNNEL:
       amount = tunnel(img, k, u, v, e);
+      mode = (int)k.p[5];
+      break;
+    case FFM_FX_JULIA:
+      julia(k, c, u, v, e);
+      amount = k.p[8];
+      mode = (int)k.p[9];
+      break;
+    case FFM_FX_WORLEY_NOISE:
+      worley(k, c, u, v, e);
+      amount = k.p[0];
       mode = (int)k.p[5];
       break;
     default:

@@ -977,6 +977,8 @@ static int fx_chain(int32_t type) {
     case FFM_FX_KALEIDOSCOPE:
     case FFM_FX_PLASMA:
     case FFM_FX_TUNNEL:
+    case FFM_FX_JULIA:
+    case FFM_FX_WORLEY_NOISE:
       return 2;
     default:
       return 0;
@@ -1002,6 +1004,8 @@ int ffm_effect_defaults(int32_t type, ffm_effect* out) {
     case FFM_FX_KALEIDOSCOPE: v = {6.0f, 0.0f, 1.0f, 1.0f, 0.0f, 1.0f, 0.0f}; break;
     case FFM_FX_PLASMA: v = {0.3f, 3.0f, 1.0f, 0.0f, 1.0f, 225.0f}; break;
     case FFM_FX_TUNNEL: v = {1.0f, 0.5f, 0.0f, 0.0f, 1.0f, 0.0f}; break;
+    case FFM_FX_JULIA: v = {1.0f, -0.7f, 0.27f, 100.0f, 1.0f, 0.0f, 0.0f, 1.0f, 1.0f, 0.0f}; break;
+    case FFM_FX_WORLEY_NOISE: v = {0.5f, 8.0f, 0.0f, 0.0f, 1.0f, 4.0f, 225.0f}; break;
     default: return FFM_ERR_INVALID_ARGUMENT;
   }
   for (size_t i = 0; i < v.size(); i++) e.params[i] = v[i];
@@ -1013,10 +1017,15 @@ int ffm_effect_defaults(int32_t type, ffm_effect* out) {
 // parameter range is 0.5..5 (sharpen.rhai), and a radius past 64 pixels
 // is refused rather than run as a near-endless loop.
 constexpr float FX_MAX_SHARPEN_RADIUS = 64.0f;
+// julia's escape loop runs up to max_iter steps a pixel (UI range 10..500).
+constexpr float FX_MAX_JULIA_ITER = 5000.0f;
 static int check_effect_params(ffm_ctx* c, const ffm_effect& e, float pixel_scale) {
   if (e.type == FFM_FX_SHARPEN && !(e.params[1] * pixel_scale <= FX_MAX_SHARPEN_RADIUS))
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "sharpen radius %g x pixel scale %g: at most %g",
                 e.params[1], pixel_scale, FX_MAX_SHARPEN_RADIUS);
+  if (e.type == FFM_FX_JULIA && !(e.params[3] <= FX_MAX_JULIA_ITER))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "julia max_iter %g: at most %g", e.params[3],
+                FX_MAX_JULIA_ITER);
   return FFM_OK;
 }
 
@@ -1034,10 +1043,13 @@ static int check_effects(ffm_ctx* c, const ffm_effects& fx) {
                   i, fx.density[i].type);
     if (int st = check_effect_params(c, fx.density[i], fx.pixel_scale)) return st;
   }
-  for (int i = 0; i < fx.n_color; i++)
-    if (fx.color[i].enabled && fx_chain(fx.color[i].type) != 2)
+  for (int i = 0; i < fx.n_color; i++) {
+    if (!fx.color[i].enabled) continue;
+    if (fx_chain(fx.color[i].type) != 2)
       return fail(c, FFM_ERR_INVALID_ARGUMENT, "colour effect %d: type %d is not a colour effect",
                   i, fx.color[i].type);
+    if (int st = check_effect_params(c, fx.color[i], fx.pixel_scale)) return st;
+  }
   return FFM_OK;
 }
 
@@ -1086,6 +1098,7 @@ int ffm_color_effect(ffm_ctx* c, const ffm_effect* fx, const uint8_t* in, uint8_
   if (fx_chain(fx->type) != 2)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "type %d is not a colour effect", fx->type);
   if (!(pixel_scale > 0.0f)) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel_scale must be positive");
+  if (int st = check_effect_params(c, *fx, pixel_scale)) return st;
   (void)hipSetDevice(c->device);
   HIPCHECK(c, launch_color_effect(fx_const(*fx, width, height, 0.0f, pixel_scale), in, out,
                                   pick(c, stream)));

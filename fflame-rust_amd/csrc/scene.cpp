@@ -746,6 +746,11 @@ const FxName kEffects[] = {
     {"plasma", FFM_FX_PLASMA, 2, {"intensity", "scale", "speed", "time", "blend_mode", "direction"}},
     {"tunnel", FFM_FX_TUNNEL, 2,
      {"speed", "rotation_speed", "distortion", "time", "intensity", "blend_mode"}},
+    {"julia", FFM_FX_JULIA, 2,
+     {"mode", "julia_cx", "julia_cy", "max_iter", "zoom", "center_x", "center_y", "color_mode",
+      "intensity", "blend_mode"}},
+    {"worley_noise", FFM_FX_WORLEY_NOISE, 2,
+     {"intensity", "scale", "time", "mode", "color_mode", "blend_mode", "direction"}},
 };
 
 // A config's effect list (Vec<EffectInstance>, src/effects/mod.rs:319-336:

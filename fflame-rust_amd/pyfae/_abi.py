@@ -124,7 +124,7 @@ class PaletteXf(C.Structure):
 
 
 MAX_EFFECTS = 16
-MAX_EFFECT_PARAMS = 8
+MAX_EFFECT_PARAMS = 12
 # ffm_effect_type: registry name -> (type, chain, parameter names in order)
 EFFECTS = {
     "density_blur": (1, "density", ("radius", "threshold", "falloff")),
@@ -141,6 +141,10 @@ EFFECTS = {
     "plasma": (22, "color", ("intensity", "scale", "speed", "time", "blend_mode", "direction")),
     "tunnel": (23, "color", ("speed", "rotation_speed", "distortion", "time", "intensity",
                              "blend_mode")),
+    "julia": (24, "color", ("mode", "julia_cx", "julia_cy", "max_iter", "zoom", "center_x",
+                            "center_y", "color_mode", "intensity", "blend_mode")),
+    "worley_noise": (25, "color", ("intensity", "scale", "time", "mode", "color_mode",
+                                   "blend_mode", "direction")),
 }
 BLEND_MODES = ("Normal", "Add", "Multiply", "Screen", "Overlay", "Soft Light", "Hard Light",
                "Color Dodge", "Color Burn", "Hue", "Saturation", "Color", "Luminosity")

@@ -423,7 +423,7 @@ typedef struct {
  * (shaders/effects/common/blend_modes.wgsl apply_blend; the input's alpha
  * kept, src/effects/custom.rs:49-60) and stored as RGBA8. */
 #define FFM_MAX_EFFECTS 16
-#define FFM_MAX_EFFECT_PARAMS 8
+#define FFM_MAX_EFFECT_PARAMS 12
 enum ffm_effect_type {
   FFM_FX_DENSITY_BLUR = 1,           /* radius 3, threshold 0.25, falloff 0.5 */
   FFM_FX_BILATERAL_BLUR = 2,         /* radius 3, sigma_spatial 3, sigma_range 0.1 */
@@ -438,8 +438,13 @@ enum ffm_effect_type {
                                         blend_mode 0, square_mode 1, edge_offset 0 */
   FFM_FX_PLASMA = 22,                /* intensity 0.3, scale 3, speed 1, time 0, blend_mode 1,
                                         direction 225 (degrees) */
-  FFM_FX_TUNNEL = 23                 /* speed 1, rotation_speed 0.5, distortion 0, time 0,
+  FFM_FX_TUNNEL = 23,                /* speed 1, rotation_speed 0.5, distortion 0, time 0,
                                         intensity 1, blend_mode 0 */
+  FFM_FX_JULIA = 24,                 /* mode 1 (0 Mandelbrot, 1 Julia), julia_cx -0.7, julia_cy 0.27,
+                                        max_iter 100 (at most 5000), zoom 1, center_x 0, center_y 0,
+                                        color_mode 1, intensity 1, blend_mode 0 */
+  FFM_FX_WORLEY_NOISE = 25           /* intensity 0.5, scale 8, time 0, mode 0, color_mode 1,
+                                        blend_mode 4, direction 225 (degrees) */
 };
 /* An EffectInstance (src/effects/mod.rs:319-336): params in the effect's
  * declared order (the slot order of get_param), as listed above. */

@@ -74,6 +74,46 @@ static float smooth(float a, float b, float x) {
 }
 
 static float lum3(float r, float g, float b) { return r * 0.2126f + g * 0.7152f + b * 0.0722f; }
+
+/* sin and cos for the colour effects, by the method the kernels use:
+ * Cody-Waite reduction by pi/2 (three-part constant 1.5703125 +
+ * 4.837512969970703125e-4 + 7.54978995489188216e-8, with fused
+ * multiply-adds) and the Cephes sinf/cosf minimax polynomials on
+ * [-pi/4, pi/4]; beyond |x| = 32768 the C library's. Pinned against the
+ * mathematical sin/cos to a few ulp by the tests. */
+void orc_fx_sincos(float x, float* s, float* c) {
+  if (!(fabsf(x) <= 32768.0f)) {
+    *s = sinf(x);
+    *c = cosf(x);
+    return;
+  }
+  const float k = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-k, 1.5703125f, x);
+  r = fmaf(-k, 4.837512969970703125e-4f, r);
+  r = fmaf(-k, 7.54978995489188216e-8f, r);
+  const float z = r * r;
+  const float sp = fmaf(fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f),
+                        z * r, r);
+  const float cp = fmaf(fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z,
+                             4.166664568298827e-2f),
+                        z * z, fmaf(-0.5f, z, 1.0f));
+  const int q = (int)k & 3;
+  const float ss = (q & 1) ? cp : sp, cc = (q & 1) ? sp : cp;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+static float fx_sin(float x) {
+  float s, c;
+  orc_fx_sincos(x, &s, &c);
+  return s;
+}
+
+static float fx_cos(float x) {
+  float s, c;
+  orc_fx_sincos(x, &s, &c);
+  return c;
+}
 static float len2(float x, float y) { return sqrtf(x * x + y * y); }
 
 /* ---- density effects ---------------------------------------------------- */
@@ -282,6 +322,30 @@ static float uf_atan2(float y, float x) {
 
 static float fractf(float x) { return x - floorf(x); }
 
+/* hsv_to_rgb as julia.rhai and worley_noise.rhai write it */
+static void fx_hsv(float h, float s, float v, float o[3]) {
+  const float c = v * s, h6 = h * 6.0f;
+  const float x = c * (1.0f - fabsf(h6 - 2.0f * truncf(h6 / 2.0f) - 1.0f));
+  const float m = v - c;
+  float r, g, b;
+  if (h6 < 1.0f) { r = c; g = x; b = 0.0f; }
+  else if (h6 < 2.0f) { r = x; g = c; b = 0.0f; }
+  else if (h6 < 3.0f) { r = 0.0f; g = c; b = x; }
+  else if (h6 < 4.0f) { r = 0.0f; g = x; b = c; }
+  else if (h6 < 5.0f) { r = x; g = 0.0f; b = c; }
+  else { r = c; g = 0.0f; b = x; }
+  o[0] = r + m;
+  o[1] = g + m;
+  o[2] = b + m;
+}
+
+/* the effect language's hash2 (src/lang/wgsl.rs uf_hash2) */
+static void fx_hash2(float x, float y, float* hx, float* hy) {
+  const float n0 = x * 127.1f + y * 311.7f, n1 = x * 269.5f + y * 183.3f;
+  *hx = fractf(fx_sin(n0) * 43758.5453f);
+  *hy = fractf(fx_sin(n1) * 43758.5453f);
+}
+
 static uint8_t unorm8(float v) {
   v = (v != v) ? 0.0f : clamp01(v);
   return (uint8_t)floorf(v * 255.0f + 0.5f);
@@ -313,7 +377,7 @@ void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int
         case FFM_FX_FILM_GRAIN: {
           const float ax = u * (float)W + p[1] * 127.1f, ay = v * (float)H + p[1] * 311.7f;
           const float n = ax * 127.1f + ay * 311.7f;
-          const float s = sinf(n) * 43758.5453f;
+          const float s = fx_sin(n) * 43758.5453f;
           e[0] = e[1] = e[2] = s - floorf(s);
           amount = p[0];
           mode = (int)p[2];
@@ -410,10 +474,10 @@ void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int
           angle = angle - segment * floorf(angle / segment);
           if (angle > segment * 0.5f) angle = segment - angle;
           angle = angle + rotation;
-          float rx = cosf(angle) * radius;
-          const float ry = sinf(angle) * radius;
+          float rx = fx_cos(angle) * radius;
+          const float ry = fx_sin(angle) * radius;
           if (square) rx = rx / aspect;
-          const float dx = -cosf(rotation), dy = -sinf(rotation);
+          const float dx = -fx_cos(rotation), dy = -fx_sin(rotation);
           const float uu = rx + 0.5f + dx * edge_offset * (1.0f / (float)W);
           const float vv = ry + 0.5f + dy * edge_offset * (1.0f / (float)H);
           const px4 s = tex(&im, fminf(fmaxf(uu, 0.001f), 0.999f), fminf(fmaxf(vv, 0.001f), 0.999f));
@@ -427,17 +491,17 @@ void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int
           const float time = p[3] * p[2];
           const float direction = p[5] * FX_PI / 180.0f;
           const float x = u * scale, y = v * scale;
-          const float dir_phase = x * cosf(direction) + y * sinf(direction);
+          const float dir_phase = x * fx_cos(direction) + y * fx_sin(direction);
           const float t = time + dir_phase;
-          float w = sinf(x + t);
-          w += sinf((y + t) * 0.5f);
-          w += sinf((x + y + t) * 0.5f);
-          const float cx = x + 0.5f * sinf(t * 0.25f), cy = y + 0.5f * cosf(t * 0.5f);
-          w += sinf(sqrtf(cx * cx + cy * cy + 1.0f) + t);
+          float w = fx_sin(x + t);
+          w += fx_sin((y + t) * 0.5f);
+          w += fx_sin((x + y + t) * 0.5f);
+          const float cx = x + 0.5f * fx_sin(t * 0.25f), cy = y + 0.5f * fx_cos(t * 0.5f);
+          w += fx_sin(sqrtf(cx * cx + cy * cy + 1.0f) + t);
           w = w * 0.25f;
-          e[0] = sinf(w * FX_PI) * 0.5f + 0.5f;
-          e[1] = sinf(w * FX_PI + 2.094f) * 0.5f + 0.5f;
-          e[2] = sinf(w * FX_PI + 4.189f) * 0.5f + 0.5f;
+          e[0] = fx_sin(w * FX_PI) * 0.5f + 0.5f;
+          e[1] = fx_sin(w * FX_PI + 2.094f) * 0.5f + 0.5f;
+          e[2] = fx_sin(w * FX_PI + 4.189f) * 0.5f + 0.5f;
           amount = p[0];
           mode = (int)p[4];
           break;
@@ -448,13 +512,90 @@ void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int
           const float angle = uf_atan2(y, x) / FX_PI;
           const float radius = len2(x, y);
           const float wall = fmaxf(radius, 0.001f);
-          const float r = wall + p[2] * sinf(angle * FX_PI * 4.0f + p[3]) * 0.1f;
+          const float r = wall + p[2] * fx_sin(angle * FX_PI * 4.0f + p[3]) * 0.1f;
           const float au = fractf(angle * 0.5f + 0.5f + p[3] * p[1] * 0.1f);
           const float av = fractf(1.0f / r + p[3] * p[0]);
           const px4 s = tex(&im, au, av);
           const float fade = smooth(0.0f, 0.5f, radius);
           for (int q = 0; q < 3; q++) e[q] = s.v[q] * fade;
           amount = radius < 0.001f ? 0.0f : p[4];
+          mode = (int)p[5];
+          break;
+        }
+        case FFM_FX_JULIA: {
+          const float max_iter = fmaxf(10.0f, p[3]);
+          const float zoom = fmaxf(0.1f, p[4]);
+          const float aspect = (float)W / (float)H;
+          const float px = (u - 0.5f) * 2.0f * aspect / zoom + p[5];
+          const float py = (v - 0.5f) * 2.0f / zoom + p[6];
+          const int is_julia = p[0] == 1.0f;
+          float zr = is_julia ? px : 0.0f, zi = is_julia ? py : 0.0f;
+          const float kr = is_julia ? p[1] : px, ki = is_julia ? p[2] : py;
+          int it = 0;
+          float x2 = zr * zr, y2 = zi * zi;
+          while ((float)it < max_iter && !(x2 + y2 > 4.0f)) {
+            const float nr = x2 - y2 + kr;
+            const float ni = 2.0f * zr * zi + ki;
+            zr = nr;
+            zi = ni;
+            x2 = zr * zr;
+            y2 = zi * zi;
+            it += 1;
+          }
+          const float iter = (float)it;
+          const float zmag = zr * zr + zi * zi;
+          if (iter >= max_iter) {
+            e[0] = e[1] = e[2] = 0.0f;
+          } else if (p[7] == 1.0f) {
+            const float ln2 = logf(2.0f);
+            const float log_zn = logf(zmag) / 2.0f;
+            const float nu = logf(log_zn / ln2) / ln2;
+            const float t = (iter + 1.0f - nu) / max_iter;
+            fx_hsv(fractf(t * 5.0f), 0.8f, 1.0f, e);
+          } else if (p[7] == 2.0f) {
+            const float f = iter / max_iter;
+            for (int q = 0; q < 3; q++) e[q] = c.v[q] * f;
+          } else {
+            const float t = iter / max_iter;
+            fx_hsv(fractf(t * 3.0f), 0.7f, t, e);
+          }
+          amount = p[8];
+          mode = (int)p[9];
+          break;
+        }
+        case FFM_FX_WORLEY_NOISE: {
+          const float scale = fmaxf(1.0f, p[1]);
+          const float time = p[2];
+          const float direction = p[6] * FX_PI / 180.0f;
+          const float ax = (u + fx_cos(direction) * time * 0.1f) * scale;
+          const float ay = (v + fx_sin(direction) * time * 0.1f) * scale;
+          const float cx = floorf(ax), cy = floorf(ay);
+          const float fx = fractf(ax), fy = fractf(ay);
+          float d1 = 8.0f, d2 = 8.0f;
+          for (int j = -1; j < 2; j++)
+            for (int i = -1; i < 2; i++) {
+              const float nx = (float)i, ny = (float)j;
+              float hx, hy;
+              fx_hash2(cx + nx, cy + ny, &hx, &hy);
+              const float qx = 0.5f + 0.5f * fx_sin(time * 0.5f + 6.2831f * hx);
+              const float qy = 0.5f + 0.5f * fx_sin(time * 0.5f + 6.2831f * hy);
+              const float dist = len2(nx + qx - fx, ny + qy - fy);
+              if (dist < d1) {
+                d2 = d1;
+                d1 = dist;
+              } else if (dist < d2) {
+                d2 = dist;
+              }
+            }
+          const float m = p[3];
+          const float pattern = m == 1.0f   ? smooth(0.0f, 0.15f, d2 - d1)
+                                : m == 2.0f ? 1.0f - smooth(0.0f, 0.5f, d1)
+                                : m == 3.0f ? clamp01(d1 * d2 * 2.0f)
+                                            : d1;
+          if (p[4] == 1.0f) fx_hsv(fractf(d1 + time * 0.1f), 0.8f, pattern, e);
+          else if (p[4] == 2.0f) for (int q = 0; q < 3; q++) e[q] = c.v[q] * pattern;
+          else e[0] = e[1] = e[2] = pattern;
+          amount = p[0];
           mode = (int)p[5];
           break;
         }

@@ -125,6 +125,7 @@ void orc_density_effect(const ffm_effect* fx, const float* in, float* out, int W
 void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int W, int H,
                       float pixel_scale);
 void orc_apply_blend(const float base[3], const float e[3], int mode, float amount, float out[3]);
+void orc_fx_sincos(float x, float* s, float* c);
 void orc_density_chain(const ffm_effects* fx, float* acc, float* scratch, int W, int H,
                        float density_mean);
 void orc_color_chain(const ffm_effects* fx, uint8_t* rgba, uint8_t* scratch, int W, int H);

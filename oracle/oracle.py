@@ -102,6 +102,7 @@ def lib():
             "orc_color_effect": (None, [C.POINTER(_abi.Effect), _u8p, _u8p, C.c_int, C.c_int,
                                         C.c_float]),
             "orc_apply_blend": (None, [_fp, _fp, C.c_int, C.c_float, _fp]),
+            "orc_fx_sincos": (None, [C.c_float, _fp, _fp]),
             "orc_palette_index_map": (None, [C.c_int, C.POINTER(_abi.PaletteXf),
                                              C.POINTER(C.c_int32)]),
             "orc_palette_linear_xf": (None, [_fp, C.c_int, C.c_int, C.POINTER(_abi.PaletteXf),

@@ -224,7 +224,7 @@ def test_config_effect_lists_load():
 
 def test_config_effects_this_engine_lacks_are_refused():
     for d, c in (([{"effect_type": "bloom", "enabled": True}], []),
-                 ([], [{"effect_type": "worley_noise", "enabled": True}]),
+                 ([], [{"effect_type": "simplex_noise", "enabled": True}]),
                  ([{"effect_type": "vignette", "enabled": True}], [])):
         with pytest.raises(pyfae.EngineError) as e:
             pyfae.load(cfg_with(d, c))
@@ -281,3 +281,79 @@ def test_generator_effects_closed_forms():
     w = (w + np.sin(np.sqrt(cx * cx + cy * cy + 1.0) + t)) * 0.25
     want = np.stack([np.sin(w * np.pi + ph) * 0.5 + 0.5 for ph in (0.0, 2.094, 4.189)], -1)
     assert np.abs(pl[..., :3].astype(float) - np.round(want * 255)).max() <= 1
+
+
+def julia_counts(W, H, max_iter, cx=0.0, cy=0.0, zoom=1.0, c=None):
+    """Escape counts, float32, in julia.rhai's operation order."""
+    f = np.float32
+    y, x = np.mgrid[0:H, 0:W].astype(np.float32)
+    u, v = (x + f(0.5)) / f(W), (y + f(0.5)) / f(H)
+    aspect = f(W) / f(H)
+    px = (u - f(0.5)) * f(2) * aspect / f(zoom) + f(cx)
+    py = (v - f(0.5)) * f(2) / f(zoom) + f(cy)
+    if c is None:  # Mandelbrot
+        zr, zi, kr, ki = np.zeros_like(px), np.zeros_like(py), px, py
+    else:
+        zr, zi = px.copy(), py.copy()
+        kr, ki = np.full_like(px, f(c[0])), np.full_like(py, f(c[1]))
+    it = np.zeros(px.shape, np.int32)
+    x2, y2 = zr * zr, zi * zi
+    for _ in range(max_iter):
+        live = ~(x2 + y2 > f(4))
+        nr, ni = x2 - y2 + kr, f(2) * zr * zi + ki
+        zr, zi = np.where(live, nr, zr), np.where(live, ni, zi)
+        x2, y2 = zr * zr, zi * zi
+        it += live
+    return it
+
+
+def test_julia_escape_counts():
+    H, W = 21, 29
+    img = (RNG.random((H, W, 4)) * 255).astype(np.uint8)
+    for mode, c in ((0.0, None), (1.0, (-0.7, 0.27))):
+        fx = pyfae.effect("julia", mode=mode, color_mode=2.0, max_iter=40.0, zoom=1.3,
+                          center_x=-0.3, blend_mode=0.0, intensity=1.0)
+        out = O.color_effect(fx, img)
+        it = julia_counts(W, H, 40, cx=-0.3, zoom=1.3, c=c)
+        f = np.where(it >= 40, 0.0, it / np.float32(40)).astype(np.float32)
+        want = np.round(np.clip(img[..., :3] / np.float32(255) * f[..., None], 0, 1) * 255)
+        assert np.abs(out[..., :3].astype(int) - want).max() <= 1
+        assert 0 < (it >= 40).mean() < 1  # inside and outside points both present
+    # escape colouring: the set itself is black
+    fx = pyfae.effect("julia", mode=0.0, color_mode=0.0, max_iter=30.0, blend_mode=0.0)
+    out = O.color_effect(fx, img)
+    inside = julia_counts(W, H, 30) >= 30
+    assert np.all(out[inside][:, :3] == 0)
+
+
+def test_worley_grayscale_is_a_distance():
+    H, W = 24, 40
+    img = (RNG.random((H, W, 4)) * 255).astype(np.uint8)
+    fx = pyfae.effect("worley_noise", color_mode=0.0, mode=0.0, blend_mode=0.0, intensity=1.0,
+                      time=3.0)
+    out = O.color_effect(fx, img)
+    assert np.all(out[..., 0] == out[..., 1]) and np.all(out[..., 1] == out[..., 2])
+    # the nearest of the 3x3 cells' points: within a cell diagonal
+    assert out[..., 0].max() <= 255 and out[..., 0].min() >= 0 and out[..., 0].std() > 10
+    assert np.array_equal(out, O.color_effect(fx, img))
+
+
+def test_effect_sincos_is_sin_and_cos():
+    """The colour effects' sin/cos (Cody-Waite + Cephes) against the
+    mathematical functions: a few ulp up to 32768, the library's beyond."""
+    import math
+    import ctypes as C
+    xs = np.concatenate([np.linspace(-40.0, 40.0, 20001), np.linspace(-32768.0, 32768.0, 20001),
+                         RNG.uniform(-1e4, 1e4, 5000)]).astype(np.float32)
+    s, c = C.c_float(), C.c_float()
+    worst = 0.0
+    for x in xs:
+        O.lib().orc_fx_sincos(float(x), C.byref(s), C.byref(c))
+        for got, want in ((s.value, math.sin(float(x))), (c.value, math.cos(float(x)))):
+            ulp = np.spacing(np.float32(max(abs(want), 1e-30)))
+            worst = max(worst, abs(got - want) / ulp if abs(want) > 1e-3 else abs(got - want) / 6e-8)
+    assert worst <= 4.0, worst
+    # quadrant bookkeeping: sin(pi/2 * k) signs
+    for k, (ws, wc) in enumerate([(0, 1), (1, 0), (0, -1), (-1, 0)]):
+        O.lib().orc_fx_sincos(float(np.float32(math.pi / 2 * k)), C.byref(s), C.byref(c))
+        assert abs(s.value - ws) < 1e-6 and abs(c.value - wc) < 1e-6
