@@ -93,14 +93,41 @@ __device__ __forceinline__ float length2(float x, float y) { return sqrtf(x * x 
 // polynomials (ffm_fastmath.h), a fixed method rather than the platform's
 // (whose last bits the reference leaves to the GPU), so the hash-based
 // effects' fract(sin(n) * 43758.5453) is reproducible.
+// Beyond |x| = 32768 the reduction runs in f64 (two-part pi/2, fused
+// multiply-adds) and the same polynomials finish: no library call, so the
+// bits are the same wherever f64 is IEEE.
+__device__ void fx_sincos(float x, float* s, float* c) {
+  if (fabsf(x) <= 32768.0f) {
+    ffm_sincos_reduced(x, s, c);
+    return;
+  }
+  if (!(fabsf(x) <= 3.4028235e38f)) {
+    *s = *c = __int_as_float(0x7fc00000);
+    return;
+  }
+  const double xd = (double)x, k = rint(xd * 0.63661977236758134);
+  double rd = fma(-k, 1.5707963267948966, xd);
+  rd = fma(-k, 6.123233995736766e-17, rd);
+  const float r = (float)rd, r2 = r * r;
+  const float sp = fmaf(fmaf(fmaf(-1.9515295891e-4f, r2, 8.3321608736e-3f), r2,
+                             -1.6666654611e-1f),
+                        r2 * r, r);
+  const float cp = fmaf(fmaf(fmaf(2.443315711809948e-5f, r2, -1.388731625493765e-3f), r2,
+                             4.166664568298827e-2f),
+                        r2 * r2, fmaf(-0.5f, r2, 1.0f));
+  const int q = (int)fmod(k, 4.0) & 3;
+  const float ss = (q & 1) ? cp : sp, cc = (q & 1) ? sp : cp;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
 __device__ __forceinline__ float fx_sin(float x) {
   float s, c;
-  ffm_sincosf(x, &s, &c);
+  fx_sincos(x, &s, &c);
   return s;
 }
 __device__ __forceinline__ float fx_cos(float x) {
   float s, c;
-  ffm_sincosf(x, &s, &c);
+  fx_sincos(x, &s, &c);
   return c;
 }
 
@@ -599,6 +626,114 @@ __device__ void worley(const FxConst& k, float4 c, float u, float v, float o[3])
   }
 }
 
+// Gradient noise of domain_warp.rhai: per lattice corner a hash2 gradient,
+// blended by the smoothstep of the fraction.
+__device__ __forceinline__ float dw_grad(float lx, float ly, float ox, float oy) {
+  float gx, gy;
+  fx_hash2(lx, ly, gx, gy);
+  return (gx * 2.0f - 1.0f) * ox + (gy * 2.0f - 1.0f) * oy;
+}
+
+__device__ float dw_noise(float x, float y) {
+  const float ix = floorf(x), iy = floorf(y);
+  const float fx = fractf(x), fy = fractf(y);
+  const float ux = fx * fx * (3.0f - 2.0f * fx), uy = fy * fy * (3.0f - 2.0f * fy);
+  const float a = dw_grad(ix, iy, fx, fy);
+  const float b = dw_grad(ix + 1.0f, iy, fx - 1.0f, fy);
+  const float c = dw_grad(ix, iy + 1.0f, fx, fy - 1.0f);
+  const float d = dw_grad(ix + 1.0f, iy + 1.0f, fx - 1.0f, fy - 1.0f);
+  const float ab = a * (1.0f - ux) + b * ux, cd = c * (1.0f - ux) + d * ux;
+  return ab * (1.0f - uy) + cd * uy;
+}
+
+// Simplex noise of simplex_noise.rhai: three corners of the skewed
+// triangle, each (0.5 - r^2)^4 times its hash2 gradient's dot product.
+__device__ __forceinline__ float sx_corner(float lx, float ly, float ox, float oy) {
+  const float t = 0.5f - (ox * ox + oy * oy);
+  if (!(t >= 0.0f)) return 0.0f;
+  float gx, gy;
+  fx_hash2(lx, ly, gx, gy);
+  const float t2 = t * t;
+  return t2 * t2 * ((gx * 2.0f - 1.0f) * ox + (gy * 2.0f - 1.0f) * oy);
+}
+
+__device__ float sx_noise(float x, float y) {
+  const float f2 = 0.36602540378443865f, g2 = 0.21132486540518713f;
+  const float s = (x + y) * f2;
+  const float ix = floorf(x + s), iy = floorf(y + s);
+  const float t = (ix + iy) * g2;
+  const float x0 = x - (ix - t), y0 = y - (iy - t);
+  const float i1x = x0 > y0 ? 1.0f : 0.0f, i1y = x0 > y0 ? 0.0f : 1.0f;
+  const float n0 = sx_corner(ix, iy, x0, y0);
+  const float n1 = sx_corner(ix + i1x, iy + i1y, x0 - i1x + g2, y0 - i1y + g2);
+  const float n2 = sx_corner(ix + 1.0f, iy + 1.0f, x0 - 1.0f + 2.0f * g2, y0 - 1.0f + 2.0f * g2);
+  return 70.0f * (n0 + n1 + n2);
+}
+
+// fbm of both noise effects: `octaves` octaves, each rotated and scaled by
+// 2.02, amplitudes halving from 0.5.
+template <bool SIMPLEX>
+__device__ float fx_fbm(float x, float y, float octaves) {
+  float value = 0.0f, amplitude = 0.5f, px = x, py = y;
+  for (int o = 0; (float)o < octaves; o++) {
+    value += amplitude * (SIMPLEX ? sx_noise(px, py) : dw_noise(px, py));
+    const float nx = (0.8f * px + 0.6f * py) * 2.02f;
+    const float ny = (-0.6f * px + 0.8f * py) * 2.02f;
+    px = nx;
+    py = ny;
+    amplitude *= 0.5f;
+  }
+  return value;
+}
+
+// domain_warp.rhai: the image read through noise warped by noise.
+__device__ void domain_warp(const ImgU8& img, const FxConst& k, float u, float v, float o[3]) {
+  const float scale = fmaxf(0.5f, k.p[1]);
+  const float octaves = fminf(fmaxf(k.p[2], 1.0f), 6.0f);
+  const float direction = k.p[5] * FX_PI / 180.0f;
+  const float tx = fx_cos(direction) * k.p[3] * 0.1f, ty = fx_sin(direction) * k.p[3] * 0.1f;
+  const float x = u * scale, y = v * scale;
+  const float qx = fx_fbm<false>(x + 0.0f + tx, y + 0.0f + ty, octaves);
+  const float qy = fx_fbm<false>(x + 5.2f + tx * 1.2f, y + 1.3f + ty * 1.2f, octaves);
+  const float rx = fx_fbm<false>(x + 4.0f * qx + 1.7f, y + 4.0f * qy + 9.2f, octaves);
+  const float ry = fx_fbm<false>(x + 4.0f * qx + 8.3f, y + 4.0f * qy + 2.8f, octaves);
+  const float4 s = tex(img, clamp01(u + rx * k.p[0]), clamp01(v + ry * k.p[0]));
+  o[0] = s.x;
+  o[1] = s.y;
+  o[2] = s.z;
+}
+
+// simplex_noise.rhai: fbm simplex noise as colour (mode 0), a distortion of
+// the image written unblended (mode 1, its alpha too) or a mask (mode 2).
+// Returns the alpha of an unblended result, or -1 for a blended one.
+__device__ float simplex(const ImgU8& img, const FxConst& k, float4 c, float u, float v,
+                         float o[3]) {
+  const float scale = fmaxf(0.5f, k.p[1]);
+  const float octaves = fminf(fmaxf(k.p[2], 1.0f), 6.0f);
+  const float time = k.p[3];
+  const float direction = k.p[6] * FX_PI / 180.0f;
+  const float nx = u * scale + fx_cos(direction) * time * 0.1f;
+  const float ny = v * scale + fx_sin(direction) * time * 0.1f;
+  const float noise = fx_fbm<true>(nx, ny, octaves);
+  const float noise2 = fx_fbm<true>(nx + 5.2f, ny + 1.3f, octaves);
+  if (k.p[4] == 1.0f) {
+    const float4 w = tex(img, clamp01(u + noise * k.p[0] * 0.1f), clamp01(v + noise2 * k.p[0] * 0.1f));
+    o[0] = w.x;
+    o[1] = w.y;
+    o[2] = w.z;
+    return w.w;
+  }
+  if (k.p[4] == 2.0f) {
+    const float m = smooth(-0.5f, 0.5f, noise);
+    o[0] = c.x * m;
+    o[1] = c.y * m;
+    o[2] = c.z * m;
+  } else {
+    fx_hsv(fractf(noise * 0.5f + 0.5f + time * 0.1f), 0.8f, noise2 * 0.5f + 0.5f, o);
+  }
+  return -1.0f;
+}
+
 __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restrict__ in,
                                                            uchar4* __restrict__ out, FxConst k) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -609,6 +744,7 @@ __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restr
   float e[3];
   int mode = 0;
   float amount = 1.0f;
+  float raw_alpha = -1.0f;  // >= 0: the effect's colour is written as is, unblended
   switch (k.type) {
     case FFM_FX_VIGNETTE:
       vignette(k, c, u, v, e);
@@ -659,11 +795,26 @@ __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restr
       amount = k.p[0];
       mode = (int)k.p[5];
       break;
+    case FFM_FX_SIMPLEX_NOISE:
+      raw_alpha = simplex(img, k, c, u, v, e);
+      amount = k.p[0];
+      mode = (int)k.p[5];
+      break;
+    case FFM_FX_DOMAIN_WARP:
+      domain_warp(img, k, u, v, e);
+      amount = k.p[0] * 2.0f;
+      mode = (int)k.p[4];
+      break;
     default:
       e[0] = c.x;
       e[1] = c.y;
       e[2] = c.z;
       break;
+  }
+  if (raw_alpha >= 0.0f) {  // blend -1 (src/effects/custom.rs:49-60): the colour itself
+    out[(size_t)y * k.W + x] =
+        make_uchar4(to_unorm8(e[0]), to_unorm8(e[1]), to_unorm8(e[2]), to_unorm8(raw_alpha));
+    return;
   }
   const float base[3] = {c.x, c.y, c.z};
   float r[3];

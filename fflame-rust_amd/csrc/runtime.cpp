@@ -979,6 +979,8 @@ static int fx_chain(int32_t type) {
     case FFM_FX_TUNNEL:
     case FFM_FX_JULIA:
     case FFM_FX_WORLEY_NOISE:
+    case FFM_FX_SIMPLEX_NOISE:
+    case FFM_FX_DOMAIN_WARP:
       return 2;
     default:
       return 0;
@@ -1006,6 +1008,8 @@ int ffm_effect_defaults(int32_t type, ffm_effect* out) {
     case FFM_FX_TUNNEL: v = {1.0f, 0.5f, 0.0f, 0.0f, 1.0f, 0.0f}; break;
     case FFM_FX_JULIA: v = {1.0f, -0.7f, 0.27f, 100.0f, 1.0f, 0.0f, 0.0f, 1.0f, 1.0f, 0.0f}; break;
     case FFM_FX_WORLEY_NOISE: v = {0.5f, 8.0f, 0.0f, 0.0f, 1.0f, 4.0f, 225.0f}; break;
+    case FFM_FX_SIMPLEX_NOISE: v = {0.5f, 5.0f, 4.0f, 0.0f, 0.0f, 4.0f, 225.0f}; break;
+    case FFM_FX_DOMAIN_WARP: v = {0.1f, 3.0f, 4.0f, 0.0f, 0.0f, 225.0f}; break;
     default: return FFM_ERR_INVALID_ARGUMENT;
   }
   for (size_t i = 0; i < v.size(); i++) e.params[i] = v[i];

@@ -751,6 +751,10 @@ const FxName kEffects[] = {
       "intensity", "blend_mode"}},
     {"worley_noise", FFM_FX_WORLEY_NOISE, 2,
      {"intensity", "scale", "time", "mode", "color_mode", "blend_mode", "direction"}},
+    {"simplex_noise", FFM_FX_SIMPLEX_NOISE, 2,
+     {"intensity", "scale", "octaves", "time", "mode", "blend_mode", "direction"}},
+    {"domain_warp", FFM_FX_DOMAIN_WARP, 2,
+     {"intensity", "scale", "octaves", "time", "blend_mode", "direction"}},
 };
 
 // A config's effect list (Vec<EffectInstance>, src/effects/mod.rs:319-336:

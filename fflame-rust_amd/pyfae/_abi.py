@@ -145,6 +145,10 @@ EFFECTS = {
                             "center_y", "color_mode", "intensity", "blend_mode")),
     "worley_noise": (25, "color", ("intensity", "scale", "time", "mode", "color_mode",
                                    "blend_mode", "direction")),
+    "simplex_noise": (26, "color", ("intensity", "scale", "octaves", "time", "mode",
+                                    "blend_mode", "direction")),
+    "domain_warp": (27, "color", ("intensity", "scale", "octaves", "time", "blend_mode",
+                                  "direction")),
 }
 BLEND_MODES = ("Normal", "Add", "Multiply", "Screen", "Overlay", "Soft Light", "Hard Light",
                "Color Dodge", "Color Burn", "Hue", "Saturation", "Color", "Luminosity")

@@ -443,8 +443,12 @@ enum ffm_effect_type {
   FFM_FX_JULIA = 24,                 /* mode 1 (0 Mandelbrot, 1 Julia), julia_cx -0.7, julia_cy 0.27,
                                         max_iter 100 (at most 5000), zoom 1, center_x 0, center_y 0,
                                         color_mode 1, intensity 1, blend_mode 0 */
-  FFM_FX_WORLEY_NOISE = 25           /* intensity 0.5, scale 8, time 0, mode 0, color_mode 1,
+  FFM_FX_WORLEY_NOISE = 25,          /* intensity 0.5, scale 8, time 0, mode 0, color_mode 1,
                                         blend_mode 4, direction 225 (degrees) */
+  FFM_FX_SIMPLEX_NOISE = 26,         /* intensity 0.5, scale 5, octaves 4, time 0, mode 0
+                                        (colour, distort, mask), blend_mode 4, direction 225 */
+  FFM_FX_DOMAIN_WARP = 27            /* intensity 0.1, scale 3, octaves 4, time 0, blend_mode 0,
+                                        direction 225 (degrees) */
 };
 /* An EffectInstance (src/effects/mod.rs:319-336): params in the effect's
  * declared order (the slot order of get_param), as listed above. */

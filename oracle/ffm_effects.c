@@ -79,25 +79,35 @@ static float lum3(float r, float g, float b) { return r * 0.2126f + g * 0.7152f 
  * Cody-Waite reduction by pi/2 (three-part constant 1.5703125 +
  * 4.837512969970703125e-4 + 7.54978995489188216e-8, with fused
  * multiply-adds) and the Cephes sinf/cosf minimax polynomials on
- * [-pi/4, pi/4]; beyond |x| = 32768 the C library's. Pinned against the
- * mathematical sin/cos to a few ulp by the tests. */
+ * [-pi/4, pi/4]; beyond |x| = 32768 the reduction in f64 (two-part pi/2)
+ * before the same polynomials. Pinned against the mathematical sin/cos
+ * to a few ulp by the tests. */
 void orc_fx_sincos(float x, float* s, float* c) {
-  if (!(fabsf(x) <= 32768.0f)) {
-    *s = sinf(x);
-    *c = cosf(x);
+  float r;
+  int q;
+  if (fabsf(x) <= 32768.0f) {
+    const float k = rintf(x * 0.636619772367581343f);
+    r = fmaf(-k, 1.5703125f, x);
+    r = fmaf(-k, 4.837512969970703125e-4f, r);
+    r = fmaf(-k, 7.54978995489188216e-8f, r);
+    q = (int)k & 3;
+  } else if (!(fabsf(x) <= 3.4028235e38f)) {
+    *s = *c = NAN;
     return;
+  } else {
+    /* beyond 32768: the reduction in f64, two-part pi/2 */
+    const double xd = (double)x, k = rint(xd * 0.63661977236758134);
+    double rd = fma(-k, 1.5707963267948966, xd);
+    rd = fma(-k, 6.123233995736766e-17, rd);
+    r = (float)rd;
+    q = (int)fmod(k, 4.0) & 3;
   }
-  const float k = rintf(x * 0.636619772367581343f);
-  float r = fmaf(-k, 1.5703125f, x);
-  r = fmaf(-k, 4.837512969970703125e-4f, r);
-  r = fmaf(-k, 7.54978995489188216e-8f, r);
   const float z = r * r;
   const float sp = fmaf(fmaf(fmaf(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f),
                         z * r, r);
   const float cp = fmaf(fmaf(fmaf(2.443315711809948e-5f, z, -1.388731625493765e-3f), z,
                              4.166664568298827e-2f),
                         z * z, fmaf(-0.5f, z, 1.0f));
-  const int q = (int)k & 3;
   const float ss = (q & 1) ? cp : sp, cc = (q & 1) ? sp : cp;
   *s = (q & 2) ? -ss : ss;
   *c = ((q + 1) & 2) ? -cc : cc;
@@ -344,6 +354,61 @@ static void fx_hash2(float x, float y, float* hx, float* hy) {
   const float n0 = x * 127.1f + y * 311.7f, n1 = x * 269.5f + y * 183.3f;
   *hx = fractf(fx_sin(n0) * 43758.5453f);
   *hy = fractf(fx_sin(n1) * 43758.5453f);
+}
+
+/* domain_warp.rhai's gradient noise */
+static float dw_grad(float lx, float ly, float ox, float oy) {
+  float gx, gy;
+  fx_hash2(lx, ly, &gx, &gy);
+  return (gx * 2.0f - 1.0f) * ox + (gy * 2.0f - 1.0f) * oy;
+}
+
+static float dw_noise(float x, float y) {
+  const float ix = floorf(x), iy = floorf(y);
+  const float fx = fractf(x), fy = fractf(y);
+  const float ux = fx * fx * (3.0f - 2.0f * fx), uy = fy * fy * (3.0f - 2.0f * fy);
+  const float a = dw_grad(ix, iy, fx, fy);
+  const float b = dw_grad(ix + 1.0f, iy, fx - 1.0f, fy);
+  const float c = dw_grad(ix, iy + 1.0f, fx, fy - 1.0f);
+  const float d = dw_grad(ix + 1.0f, iy + 1.0f, fx - 1.0f, fy - 1.0f);
+  const float ab = a * (1.0f - ux) + b * ux, cd = c * (1.0f - ux) + d * ux;
+  return ab * (1.0f - uy) + cd * uy;
+}
+
+/* simplex_noise.rhai's corners and simplex */
+static float sx_corner(float lx, float ly, float ox, float oy) {
+  const float t = 0.5f - (ox * ox + oy * oy);
+  if (!(t >= 0.0f)) return 0.0f;
+  float gx, gy;
+  fx_hash2(lx, ly, &gx, &gy);
+  const float t2 = t * t;
+  return t2 * t2 * ((gx * 2.0f - 1.0f) * ox + (gy * 2.0f - 1.0f) * oy);
+}
+
+static float sx_noise(float x, float y) {
+  const float f2 = 0.36602540378443865f, g2 = 0.21132486540518713f;
+  const float s = (x + y) * f2;
+  const float ix = floorf(x + s), iy = floorf(y + s);
+  const float t = (ix + iy) * g2;
+  const float x0 = x - (ix - t), y0 = y - (iy - t);
+  const float i1x = x0 > y0 ? 1.0f : 0.0f, i1y = x0 > y0 ? 0.0f : 1.0f;
+  const float n0 = sx_corner(ix, iy, x0, y0);
+  const float n1 = sx_corner(ix + i1x, iy + i1y, x0 - i1x + g2, y0 - i1y + g2);
+  const float n2 = sx_corner(ix + 1.0f, iy + 1.0f, x0 - 1.0f + 2.0f * g2, y0 - 1.0f + 2.0f * g2);
+  return 70.0f * (n0 + n1 + n2);
+}
+
+static float fx_fbm(int simplex, float x, float y, float octaves) {
+  float value = 0.0f, amplitude = 0.5f, px = x, py = y;
+  for (int o = 0; (float)o < octaves; o++) {
+    value += amplitude * (simplex ? sx_noise(px, py) : dw_noise(px, py));
+    const float nx = (0.8f * px + 0.6f * py) * 2.02f;
+    const float ny = (-0.6f * px + 0.8f * py) * 2.02f;
+    px = nx;
+    py = ny;
+    amplitude *= 0.5f;
+  }
+  return value;
 }
 
 static uint8_t unorm8(float v) {
@@ -597,6 +662,48 @@ void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int
           else e[0] = e[1] = e[2] = pattern;
           amount = p[0];
           mode = (int)p[5];
+          break;
+        }
+        case FFM_FX_DOMAIN_WARP: {
+          const float scale = fmaxf(0.5f, p[1]);
+          const float octaves = fminf(fmaxf(p[2], 1.0f), 6.0f);
+          const float direction = p[5] * FX_PI / 180.0f;
+          const float tx = fx_cos(direction) * p[3] * 0.1f, ty = fx_sin(direction) * p[3] * 0.1f;
+          const float x = u * scale, y = v * scale;
+          const float qx = fx_fbm(0, x + 0.0f + tx, y + 0.0f + ty, octaves);
+          const float qy = fx_fbm(0, x + 5.2f + tx * 1.2f, y + 1.3f + ty * 1.2f, octaves);
+          const float rx = fx_fbm(0, x + 4.0f * qx + 1.7f, y + 4.0f * qy + 9.2f, octaves);
+          const float ry = fx_fbm(0, x + 4.0f * qx + 8.3f, y + 4.0f * qy + 2.8f, octaves);
+          const px4 s = tex(&im, clamp01(u + rx * p[0]), clamp01(v + ry * p[0]));
+          for (int q = 0; q < 3; q++) e[q] = s.v[q];
+          amount = p[0] * 2.0f;
+          mode = (int)p[4];
+          break;
+        }
+        case FFM_FX_SIMPLEX_NOISE: {
+          const float scale = fmaxf(0.5f, p[1]);
+          const float octaves = fminf(fmaxf(p[2], 1.0f), 6.0f);
+          const float time = p[3];
+          const float direction = p[6] * FX_PI / 180.0f;
+          const float nx = u * scale + fx_cos(direction) * time * 0.1f;
+          const float ny = v * scale + fx_sin(direction) * time * 0.1f;
+          const float noise = fx_fbm(1, nx, ny, octaves);
+          const float noise2 = fx_fbm(1, nx + 5.2f, ny + 1.3f, octaves);
+          amount = p[0];
+          mode = (int)p[5];
+          if (p[4] == 1.0f) {
+            /* blend -1: the warped read written as is, its alpha too */
+            const px4 w = tex(&im, clamp01(u + noise * p[0] * 0.1f), clamp01(v + noise2 * p[0] * 0.1f));
+            uint8_t* o = out + 4 * ((size_t)y * W + x);
+            for (int q = 0; q < 4; q++) o[q] = unorm8(w.v[q]);
+            continue;
+          }
+          if (p[4] == 2.0f) {
+            const float m = smooth(-0.5f, 0.5f, noise);
+            for (int q = 0; q < 3; q++) e[q] = c.v[q] * m;
+          } else {
+            fx_hsv(fractf(noise * 0.5f + 0.5f + time * 0.1f), 0.8f, noise2 * 0.5f + 0.5f, e);
+          }
           break;
         }
         default: break;

@@ -224,7 +224,7 @@ def test_config_effect_lists_load():
 
 def test_config_effects_this_engine_lacks_are_refused():
     for d, c in (([{"effect_type": "bloom", "enabled": True}], []),
-                 ([], [{"effect_type": "simplex_noise", "enabled": True}]),
+                 ([], [{"effect_type": "distance_field", "enabled": True}]),
                  ([{"effect_type": "vignette", "enabled": True}], [])):
         with pytest.raises(pyfae.EngineError) as e:
             pyfae.load(cfg_with(d, c))
@@ -259,8 +259,17 @@ def test_effect_lists_round_trip_through_scene_to_config():
 def test_generator_effects_closed_forms():
     H, W = 31, 31
     img = (RNG.random((H, W, 4)) * 255).astype(np.uint8)
-    for name in ("kaleidoscope", "plasma", "tunnel"):
+    for name in ("kaleidoscope", "plasma", "tunnel", "julia", "worley_noise", "simplex_noise",
+                 "domain_warp"):
         assert np.array_equal(O.color_effect(pyfae.effect(name, intensity=0.0), img), img), name
+    # simplex's distortion mode writes the warped read unblended, alpha too:
+    # at intensity 0 that is the image itself
+    assert np.array_equal(O.color_effect(pyfae.effect("simplex_noise", mode=1.0, intensity=0.0),
+                                         img), img)
+    # its mask mode at intensity 1 (normal blend) scales the colour by a
+    # smoothstep in [0, 1]: never brighter
+    m = O.color_effect(pyfae.effect("simplex_noise", mode=2.0, intensity=1.0, blend_mode=0.0), img)
+    assert np.all(m[..., :3] <= img[..., :3] + 1)
     # the kaleidoscope folds the angle into mirrored segments: a square
     # image mirrored top to bottom about its centre row reads the same
     k = O.color_effect(pyfae.effect("kaleidoscope", segments=5.0), img)
@@ -339,12 +348,13 @@ def test_worley_grayscale_is_a_distance():
 
 
 def test_effect_sincos_is_sin_and_cos():
-    """The colour effects' sin/cos (Cody-Waite + Cephes) against the
-    mathematical functions: a few ulp up to 32768, the library's beyond."""
+    """The colour effects' sin/cos (Cody-Waite + Cephes; an f64 reduction
+    past 32768) against the mathematical functions: a few ulp."""
     import math
     import ctypes as C
     xs = np.concatenate([np.linspace(-40.0, 40.0, 20001), np.linspace(-32768.0, 32768.0, 20001),
-                         RNG.uniform(-1e4, 1e4, 5000)]).astype(np.float32)
+                         RNG.uniform(-1e4, 1e4, 5000), RNG.uniform(-2e6, 2e6, 5000),
+                         RNG.uniform(-1e9, 1e9, 2000)]).astype(np.float32)
     s, c = C.c_float(), C.c_float()
     worst = 0.0
     for x in xs:
