@@ -12,6 +12,9 @@
 // These are per-pixel stencils of at most 21x21 taps read through L1/L2:
 // bound by the tap loop (VALU, one exp per tap), not by HBM; each pass
 // moves 16 B (density) or 4 B (colour) per pixel in and out.
+#include <algorithm>
+#include <cmath>
+
 #include "ffm_fastmath.h"
 #include "ffm_internal.h"
 
@@ -823,12 +826,125 @@ __global__ __launch_bounds__(256) void color_effect_kernel(const uchar4* __restr
                                          in[(size_t)y * k.W + x].w);
 }
 
+// ------------------------------------------------------------------ bloom
+
+// bright(): the part of a colour above the threshold, with a soft knee.
+__device__ __forceinline__ float4 bloom_bright(float4 c, float threshold, float softness) {
+  const float l = fmaxf(c.x, fmaxf(c.y, c.z));
+  const float knee = fmaxf(threshold * softness, 0.00001f);
+  const float soft = fminf(fmaxf(l - threshold + knee, 0.0f), 2.0f * knee);
+  const float w = fmaxf(soft * soft / (4.0f * knee), l - threshold) / fmaxf(l, 0.00001f);
+  return f4mul(c, w);
+}
+
+// One level of the down pyramid: level 0 averages the bright parts of
+// four input reads half a level-0 texel away; higher levels weigh the
+// level below's read at the centre 4 and at four diagonals 1, over 8.
+__global__ __launch_bounds__(256) void bloom_down_kernel(BloomArgs b, int r) {
+  const int w = b.lw[r], h = b.lh[r];
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w || y >= h) return;
+  const float u = ((float)x + 0.5f) / (float)w, v = ((float)y + 0.5f) / (float)h;
+  const float dx = 0.5f / (float)w, dy = 0.5f / (float)h;
+  float4 o;
+  if (r == 0) {
+    const ImgU8 in{reinterpret_cast<const uchar4*>(b.in), b.W, b.H};
+    o = f4add(f4add(f4add(bloom_bright(tex(in, u - dx, v - dy), b.threshold, b.softness),
+                          bloom_bright(tex(in, u + dx, v - dy), b.threshold, b.softness)),
+                    bloom_bright(tex(in, u - dx, v + dy), b.threshold, b.softness)),
+              bloom_bright(tex(in, u + dx, v + dy), b.threshold, b.softness));
+    o = f4mul(o, 0.25f);
+  } else {
+    const ImgF32 src{reinterpret_cast<const float4*>(b.down[r - 1]), b.lw[r - 1], b.lh[r - 1]};
+    o = f4add(f4add(f4add(f4add(f4mul(tex(src, u, v), 4.0f), tex(src, u - dx, v - dy)),
+                          tex(src, u + dx, v - dy)),
+                    tex(src, u - dx, v + dy)),
+              tex(src, u + dx, v + dy));
+    o = f4mul(o, 0.125f);
+  }
+  reinterpret_cast<float4*>(b.down[r])[(size_t)y * w + x] = o;
+}
+
+// One level of the up pyramid, from the top: the down level plus, below
+// the top, a 3x3 tent of the level above times spread / 16.
+__global__ __launch_bounds__(256) void bloom_up_kernel(BloomArgs b, int l) {
+  const int w = b.lw[l], h = b.lh[l];
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w || y >= h) return;
+  const float u = ((float)x + 0.5f) / (float)w, v = ((float)y + 0.5f) / (float)h;
+  const ImgF32 dn{reinterpret_cast<const float4*>(b.down[l]), w, h};
+  const float4 here = tex(dn, u, v);
+  float4 o = here;
+  if (l != b.levels - 1) {
+    const ImgF32 up{reinterpret_cast<const float4*>(b.up[l + 1]), b.lw[l + 1], b.lh[l + 1]};
+    const float dx = 1.0f / (float)w, dy = 1.0f / (float)h;
+    const float4 side =
+        f4add(f4add(f4add(tex(up, u - dx, v), tex(up, u + dx, v)), tex(up, u, v - dy)),
+              tex(up, u, v + dy));
+    float4 tent = f4add(f4mul(tex(up, u, v), 4.0f), f4mul(side, 2.0f));
+    tent = f4add(tent, tex(up, u - dx, v - dy));
+    tent = f4add(tent, tex(up, u + dx, v - dy));
+    tent = f4add(tent, tex(up, u - dx, v + dy));
+    tent = f4add(tent, tex(up, u + dx, v + dy));
+    o = f4add(here, f4mul(tent, 0.0625f * b.spread));
+  }
+  reinterpret_cast<float4*>(b.up[l])[(size_t)y * w + x] = o;
+}
+
+// The composite: the picture plus the glow (up level 0) times the
+// intensity, clamped; the picture's alpha kept; written unblended.
+__global__ __launch_bounds__(256) void bloom_composite_kernel(BloomArgs b) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= b.W || y >= b.H) return;
+  const float u = ((float)x + 0.5f) / (float)b.W, v = ((float)y + 0.5f) / (float)b.H;
+  const ImgU8 in{reinterpret_cast<const uchar4*>(b.in), b.W, b.H};
+  const ImgF32 up{reinterpret_cast<const float4*>(b.up[0]), b.lw[0], b.lh[0]};
+  const float4 c = in.at(x, y), glow = tex(up, u, v);
+  const uchar4 a = reinterpret_cast<const uchar4*>(b.in)[(size_t)y * b.W + x];
+  reinterpret_cast<uchar4*>(b.out)[(size_t)y * b.W + x] =
+      make_uchar4(to_unorm8(clamp01(c.x + glow.x * 1.0f * b.intensity)),
+                  to_unorm8(clamp01(c.y + glow.y * 1.0f * b.intensity)),
+                  to_unorm8(clamp01(c.z + glow.z * 1.0f * b.intensity)), a.w);
+}
+
 }  // namespace
 
 hipError_t launch_density_effect(const FxConst& k, const float* in, float* out, hipStream_t s) {
   dim3 grid((k.W + 15) / 16, (k.H + 15) / 16);
   hipLaunchKernelGGL(density_effect_kernel, grid, dim3(256), 0, s,
                      reinterpret_cast<const float4*>(in), reinterpret_cast<float4*>(out), k);
+  return hipGetLastError();
+}
+
+// Buffer sizes of bloom's pyramids (BufferDecl::size / levels_at,
+// src/effects/passes.rs:21-33: scale 0.5, rounded half away from zero,
+// at most 7 levels and no more than the bit length of the longer side).
+int bloom_levels(int W, int H, int32_t* lw, int32_t* lh) {
+  const int bw = std::max((int)std::round((float)W * 0.5f), 1);
+  const int bh = std::max((int)std::round((float)H * 0.5f), 1);
+  int bits = 0;
+  for (int m = std::max(bw, bh); m; m >>= 1) bits++;
+  const int n = std::max(std::min(BLOOM_LEVELS, bits), 1);
+  for (int l = 0; l < n; l++) {
+    lw[l] = std::max(bw >> l, 1);
+    lh[l] = std::max(bh >> l, 1);
+  }
+  return n;
+}
+
+hipError_t launch_bloom(const BloomArgs& b, hipStream_t s) {
+  for (int r = 0; r < b.levels; r++) {
+    hipLaunchKernelGGL(bloom_down_kernel, dim3((b.lw[r] + 15) / 16, (b.lh[r] + 15) / 16),
+                       dim3(256), 0, s, b, r);
+    if (hipError_t e = hipGetLastError()) return e;
+  }
+  for (int l = b.levels - 1; l >= 0; l--) {
+    hipLaunchKernelGGL(bloom_up_kernel, dim3((b.lw[l] + 15) / 16, (b.lh[l] + 15) / 16),
+                       dim3(256), 0, s, b, l);
+    if (hipError_t e = hipGetLastError()) return e;
+  }
+  hipLaunchKernelGGL(bloom_composite_kernel, dim3((b.W + 15) / 16, (b.H + 15) / 16), dim3(256), 0,
+                     s, b);
   return hipGetLastError();
 }
 

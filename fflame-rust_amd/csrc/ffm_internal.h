@@ -183,6 +183,23 @@ struct FxConst {
 hipError_t launch_density_effect(const FxConst& k, const float* in, float* out, hipStream_t s);
 hipError_t launch_color_effect(const FxConst& k, const uint8_t* in, uint8_t* out, hipStream_t s);
 
+// bloom (assets/effects/bloom.rhai): a down pyramid of the bright parts, an
+// up pyramid of tent-filtered sums, and the composite. Level l of each is
+// (lw[l], lh[l]) float4 texels.
+constexpr int BLOOM_LEVELS = 7;
+struct BloomArgs {
+  const uint8_t* in;  // the picture (RGBA8), W x H
+  uint8_t* out;
+  int32_t W, H, levels;
+  int32_t lw[BLOOM_LEVELS], lh[BLOOM_LEVELS];
+  float* down[BLOOM_LEVELS];
+  float* up[BLOOM_LEVELS];
+  float threshold, softness, intensity, spread;
+};
+// Pyramid level sizes for a W x H frame; returns the level count.
+int bloom_levels(int W, int H, int32_t* lw, int32_t* lh);
+hipError_t launch_bloom(const BloomArgs& b, hipStream_t s);
+
 // ---- batched gallery: many flames in one persistent launch
 
 struct BatchFlame {

@@ -59,6 +59,7 @@ struct ffm_ctx {
   // render buffers
   DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv, count;
   DevBuf fx_a, fx_b, fx_rgba;    // effect chain ping-pong buffers
+  DevBuf fx_pyr;                 // bloom's down and up pyramids
   ffm_effects fx{};              // effects renders apply (none by default)
   // LDS tile splat: per-tile fill/region/capacity, work list, record buffer
   DevBuf t_fill, t_region, t_cap, t_items, t_records, t_sorted;
@@ -981,6 +982,7 @@ static int fx_chain(int32_t type) {
     case FFM_FX_WORLEY_NOISE:
     case FFM_FX_SIMPLEX_NOISE:
     case FFM_FX_DOMAIN_WARP:
+    case FFM_FX_BLOOM:
       return 2;
     default:
       return 0;
@@ -1010,6 +1012,7 @@ int ffm_effect_defaults(int32_t type, ffm_effect* out) {
     case FFM_FX_WORLEY_NOISE: v = {0.5f, 8.0f, 0.0f, 0.0f, 1.0f, 4.0f, 225.0f}; break;
     case FFM_FX_SIMPLEX_NOISE: v = {0.5f, 5.0f, 4.0f, 0.0f, 0.0f, 4.0f, 225.0f}; break;
     case FFM_FX_DOMAIN_WARP: v = {0.1f, 3.0f, 4.0f, 0.0f, 0.0f, 225.0f}; break;
+    case FFM_FX_BLOOM: v = {0.45f, 0.3f, 1.5f, 0.7f}; break;  // bloom.rhai (tint white)
     default: return FFM_ERR_INVALID_ARGUMENT;
   }
   for (size_t i = 0; i < v.size(); i++) e.params[i] = v[i];
@@ -1095,6 +1098,41 @@ int ffm_density_effect(ffm_ctx* c, const ffm_effect* fx, const float* in, float*
   return FFM_OK;
 }
 
+// One colour effect from `in` to `out`: bloom's passes over its
+// pyramids, any other effect's one pass.
+static int color_effect_on(ffm_ctx* c, const ffm_effect& e, const uint8_t* in, uint8_t* out, int W,
+                           int H, float pixel_scale, hipStream_t s) {
+  if (e.type != FFM_FX_BLOOM) {
+    HIPCHECK(c, launch_color_effect(fx_const(e, W, H, 0.0f, pixel_scale), in, out, s));
+    return FFM_OK;
+  }
+  BloomArgs b{};
+  b.in = in;
+  b.out = out;
+  b.W = W;
+  b.H = H;
+  b.levels = bloom_levels(W, H, b.lw, b.lh);
+  size_t texels = 0;
+  for (int l = 0; l < b.levels; l++) texels += (size_t)b.lw[l] * b.lh[l];
+  int st;
+  if ((st = ensure(c, c->fx_pyr, 2 * texels * 16, "bloom pyramids"))) return st;
+  float* p = reinterpret_cast<float*>(c->fx_pyr.p);
+  for (int l = 0; l < b.levels; l++) {
+    b.down[l] = p;
+    p += (size_t)b.lw[l] * b.lh[l] * 4;
+  }
+  for (int l = 0; l < b.levels; l++) {
+    b.up[l] = p;
+    p += (size_t)b.lw[l] * b.lh[l] * 4;
+  }
+  b.threshold = e.params[0];
+  b.softness = e.params[1];
+  b.intensity = e.params[2];
+  b.spread = e.params[3];
+  HIPCHECK(c, launch_bloom(b, s));
+  return FFM_OK;
+}
+
 int ffm_color_effect(ffm_ctx* c, const ffm_effect* fx, const uint8_t* in, uint8_t* out, int width,
                      int height, float pixel_scale, void* stream) {
   if (!c || !fx || !in || !out || width < 1 || height < 1) return FFM_ERR_INVALID_ARGUMENT;
@@ -1104,9 +1142,7 @@ int ffm_color_effect(ffm_ctx* c, const ffm_effect* fx, const uint8_t* in, uint8_
   if (!(pixel_scale > 0.0f)) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel_scale must be positive");
   if (int st = check_effect_params(c, *fx, pixel_scale)) return st;
   (void)hipSetDevice(c->device);
-  HIPCHECK(c, launch_color_effect(fx_const(*fx, width, height, 0.0f, pixel_scale), in, out,
-                                  pick(c, stream)));
-  return FFM_OK;
+  return color_effect_on(c, *fx, in, out, width, height, pixel_scale, pick(c, stream));
 }
 
 // The density chain over `acc` (each enabled effect reads the previous
@@ -1144,8 +1180,7 @@ static int run_color_chain(ffm_ctx* c, const ffm_effects& fx, uint8_t* pix, int 
     if (!fx.color[i].enabled) continue;
     if ((st = ensure(c, c->fx_rgba, npix * 4, "effect pixels"))) return st;
     uint8_t* dst = cur == pix ? reinterpret_cast<uint8_t*>(c->fx_rgba.p) : pix;
-    HIPCHECK(c, launch_color_effect(fx_const(fx.color[i], W, H, 0.0f, fx.pixel_scale), cur, dst,
-                                    s));
+    if ((st = color_effect_on(c, fx.color[i], cur, dst, W, H, fx.pixel_scale, s))) return st;
     cur = dst;
   }
   if (cur != pix) HIPCHECK(c, hipMemcpyAsync(pix, cur, npix * 4, hipMemcpyDeviceToDevice, s));

@@ -755,6 +755,7 @@ const FxName kEffects[] = {
      {"intensity", "scale", "octaves", "time", "mode", "blend_mode", "direction"}},
     {"domain_warp", FFM_FX_DOMAIN_WARP, 2,
      {"intensity", "scale", "octaves", "time", "blend_mode", "direction"}},
+    {"bloom", FFM_FX_BLOOM, 2, {"threshold", "softness", "intensity", "spread"}},
 };
 
 // A config's effect list (Vec<EffectInstance>, src/effects/mod.rs:319-336:

@@ -149,6 +149,7 @@ EFFECTS = {
                                     "blend_mode", "direction")),
     "domain_warp": (27, "color", ("intensity", "scale", "octaves", "time", "blend_mode",
                                   "direction")),
+    "bloom": (28, "color", ("threshold", "softness", "intensity", "spread")),
 }
 BLEND_MODES = ("Normal", "Add", "Multiply", "Screen", "Overlay", "Soft Light", "Hard Light",
                "Color Dodge", "Color Burn", "Hue", "Saturation", "Color", "Luminosity")

@@ -447,8 +447,10 @@ enum ffm_effect_type {
                                         blend_mode 4, direction 225 (degrees) */
   FFM_FX_SIMPLEX_NOISE = 26,         /* intensity 0.5, scale 5, octaves 4, time 0, mode 0
                                         (colour, distort, mask), blend_mode 4, direction 225 */
-  FFM_FX_DOMAIN_WARP = 27            /* intensity 0.1, scale 3, octaves 4, time 0, blend_mode 0,
+  FFM_FX_DOMAIN_WARP = 27,           /* intensity 0.1, scale 3, octaves 4, time 0, blend_mode 0,
                                         direction 225 (degrees) */
+  FFM_FX_BLOOM = 28                  /* threshold 0.45, softness 0.3, intensity 1.5, spread 0.7
+                                        (tint white); a 7-level half-size pyramid */
 };
 /* An EffectInstance (src/effects/mod.rs:319-336): params in the effect's
  * declared order (the slot order of get_param), as listed above. */

@@ -21,6 +21,7 @@
  * (-ffp-contract=off); exp, sqrt and sin are the C library's.
  */
 #include <math.h>
+#include <stdlib.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -416,8 +417,115 @@ static uint8_t unorm8(float v) {
   return (uint8_t)floorf(v * 255.0f + 0.5f);
 }
 
+/* ---- bloom (assets/effects/bloom.rhai; passes by src/renderer/effect_passes.rs:273-371) ---- */
+
+static px4 p4add(px4 a, px4 b) {
+  for (int k = 0; k < 4; k++) a.v[k] = a.v[k] + b.v[k];
+  return a;
+}
+static px4 p4mul(px4 a, float s) {
+  for (int k = 0; k < 4; k++) a.v[k] = a.v[k] * s;
+  return a;
+}
+
+static px4 bloom_bright(px4 c, float threshold, float softness) {
+  const float l = fmaxf(c.v[0], fmaxf(c.v[1], c.v[2]));
+  const float knee = fmaxf(threshold * softness, 0.00001f);
+  const float soft = fminf(fmaxf(l - threshold + knee, 0.0f), 2.0f * knee);
+  const float w = fmaxf(soft * soft / (4.0f * knee), l - threshold) / fmaxf(l, 0.00001f);
+  return p4mul(c, w);
+}
+
+#define ORC_BLOOM_LEVELS 7
+
+/* Buffers "down" and "up": scale 0.5 (rounded half away from zero), up to
+ * 7 levels and no more than the longer side's bit length
+ * (src/effects/passes.rs:21-33); level l is the base size >> l. */
+static void orc_bloom(const float* p, const uint8_t* in, uint8_t* out, int W, int H) {
+  const int bw = (int)fmaxf(roundf((float)W * 0.5f), 1.0f), bh = (int)fmaxf(roundf((float)H * 0.5f), 1.0f);
+  int bits = 0;
+  for (int m = bw > bh ? bw : bh; m; m >>= 1) bits++;
+  const int n = bits < ORC_BLOOM_LEVELS ? (bits < 1 ? 1 : bits) : ORC_BLOOM_LEVELS;
+  int lw[ORC_BLOOM_LEVELS], lh[ORC_BLOOM_LEVELS];
+  float *down[ORC_BLOOM_LEVELS], *up[ORC_BLOOM_LEVELS];
+  for (int l = 0; l < n; l++) {
+    lw[l] = bw >> l > 0 ? bw >> l : 1;
+    lh[l] = bh >> l > 0 ? bh >> l : 1;
+    down[l] = (float*)malloc((size_t)lw[l] * lh[l] * 16);
+    up[l] = (float*)malloc((size_t)lw[l] * lh[l] * 16);
+  }
+  const img_t src = {NULL, in, W, H};
+  const float th = p[0], so = p[1], inten = p[2], spread = p[3];
+  for (int r = 0; r < n; r++) {
+    const int w = lw[r], h = lh[r];
+    const img_t prev = {r ? down[r - 1] : NULL, NULL, r ? lw[r - 1] : 1, r ? lh[r - 1] : 1};
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < w; x++) {
+        const float u = ((float)x + 0.5f) / (float)w, v = ((float)y + 0.5f) / (float)h;
+        const float dx = 0.5f / (float)w, dy = 0.5f / (float)h;
+        px4 o;
+        if (r == 0) {
+          o = p4add(p4add(p4add(bloom_bright(tex(&src, u - dx, v - dy), th, so),
+                                bloom_bright(tex(&src, u + dx, v - dy), th, so)),
+                          bloom_bright(tex(&src, u - dx, v + dy), th, so)),
+                    bloom_bright(tex(&src, u + dx, v + dy), th, so));
+          o = p4mul(o, 0.25f);
+        } else {
+          o = p4add(p4add(p4add(p4add(p4mul(tex(&prev, u, v), 4.0f), tex(&prev, u - dx, v - dy)),
+                                tex(&prev, u + dx, v - dy)),
+                          tex(&prev, u - dx, v + dy)),
+                    tex(&prev, u + dx, v + dy));
+          o = p4mul(o, 0.125f);
+        }
+        memcpy(down[r] + 4 * ((size_t)y * w + x), o.v, 16);
+      }
+  }
+  for (int l = n - 1; l >= 0; l--) {
+    const int w = lw[l], h = lh[l];
+    const img_t dn = {down[l], NULL, w, h};
+    const img_t above = {l + 1 < n ? up[l + 1] : NULL, NULL, l + 1 < n ? lw[l + 1] : 1,
+                         l + 1 < n ? lh[l + 1] : 1};
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < w; x++) {
+        const float u = ((float)x + 0.5f) / (float)w, v = ((float)y + 0.5f) / (float)h;
+        const px4 here = tex(&dn, u, v);
+        px4 o = here;
+        if (l != n - 1) {
+          const float dx = 1.0f / (float)w, dy = 1.0f / (float)h;
+          const px4 side = p4add(p4add(p4add(tex(&above, u - dx, v), tex(&above, u + dx, v)),
+                                       tex(&above, u, v - dy)),
+                                 tex(&above, u, v + dy));
+          px4 tent = p4add(p4mul(tex(&above, u, v), 4.0f), p4mul(side, 2.0f));
+          tent = p4add(tent, tex(&above, u - dx, v - dy));
+          tent = p4add(tent, tex(&above, u + dx, v - dy));
+          tent = p4add(tent, tex(&above, u - dx, v + dy));
+          tent = p4add(tent, tex(&above, u + dx, v + dy));
+          o = p4add(here, p4mul(tent, 0.0625f * spread));
+        }
+        memcpy(up[l] + 4 * ((size_t)y * w + x), o.v, 16);
+      }
+  }
+  const img_t glow_img = {up[0], NULL, lw[0], lh[0]};
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const float u = ((float)x + 0.5f) / (float)W, v = ((float)y + 0.5f) / (float)H;
+      const px4 c = img_at(&src, x, y), glow = tex(&glow_img, u, v);
+      uint8_t* o = out + 4 * ((size_t)y * W + x);
+      for (int k = 0; k < 3; k++) o[k] = unorm8(clamp01(c.v[k] + glow.v[k] * 1.0f * inten));
+      o[3] = in[4 * ((size_t)y * W + x) + 3];
+    }
+  for (int l = 0; l < n; l++) {
+    free(down[l]);
+    free(up[l]);
+  }
+}
+
 void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int W, int H,
                       float pixel_scale) {
+  if (fx->type == FFM_FX_BLOOM) {
+    orc_bloom(fx->params, in, out, W, H);
+    return;
+  }
   const img_t im = {NULL, in, W, H};
   const float* p = fx->params;
   for (int y = 0; y < H; y++)

@@ -225,6 +225,7 @@ def test_config_effect_lists_load():
 def test_config_effects_this_engine_lacks_are_refused():
     for d, c in (([{"effect_type": "bloom", "enabled": True}], []),
                  ([], [{"effect_type": "distance_field", "enabled": True}]),
+                 ([], [{"effect_type": "trails", "enabled": True}]),
                  ([{"effect_type": "vignette", "enabled": True}], [])):
         with pytest.raises(pyfae.EngineError) as e:
             pyfae.load(cfg_with(d, c))
@@ -367,3 +368,25 @@ def test_effect_sincos_is_sin_and_cos():
     for k, (ws, wc) in enumerate([(0, 1), (1, 0), (0, -1), (-1, 0)]):
         O.lib().orc_fx_sincos(float(np.float32(math.pi / 2 * k)), C.byref(s), C.byref(c))
         assert abs(s.value - ws) < 1e-6 and abs(c.value - wc) < 1e-6
+
+
+def test_bloom_closed_forms():
+    H, W = 31, 45
+    black = np.zeros((H, W, 4), np.uint8)
+    black[..., 3] = 200
+    assert np.array_equal(O.color_effect(pyfae.effect("bloom"), black), black)
+    img = (RNG.random((H, W, 4)) * 255).astype(np.uint8)
+    assert np.array_equal(O.color_effect(pyfae.effect("bloom", intensity=0.0), img), img)
+    # below threshold - knee nothing glows
+    grey = np.full((H, W, 4), 60, np.uint8)
+    assert np.array_equal(O.color_effect(pyfae.effect("bloom"), grey), grey)
+    # a bright 3x3 patch glows around itself, falling off with distance,
+    # symmetric left to right, never darkening anything
+    dot = np.zeros((H, W, 4), np.uint8)
+    dot[..., 3] = 255
+    dot[14:17, 21:24, :3] = 255
+    out = O.color_effect(pyfae.effect("bloom", intensity=4.0, spread=1.0), dot).astype(int)
+    row = out[15, :, 0]
+    assert np.all(out >= dot) and row[16] > row[12] > row[8] > 0
+    assert np.abs(out[:, :22] - out[:, 44:22:-1]).max() <= 1
+    assert out[15, 22, 0] == 255

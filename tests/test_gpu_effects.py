@@ -86,6 +86,7 @@ COLOUR_CASES = [
     ("simplex_noise", {"mode": 2.0, "scale": 11.0, "time": 3.0, "blend_mode": 0.0}),
     ("domain_warp", {}), ("domain_warp", {"intensity": 0.4, "scale": 7.0, "octaves": 2.0,
                                           "time": 20.0, "blend_mode": 5.0}),
+    ("bloom", {}), ("bloom", {"threshold": 0.1, "softness": 0.8, "intensity": 3.0, "spread": 1.0}),
 ] + [("vignette", {"intensity": 0.8, "radius": 0.3, "softness": 0.4, "blend_mode": float(m)})
      for m in range(13)]
 
@@ -158,7 +159,7 @@ def test_render_with_both_chains_equals_oracle(engine):
     engine.set_flame(fd)
     engine.set_palette(STOPS, 256)
     fx = pyfae.effects(density=[("density_blur", {"radius": 4.0}), ("sharpen", {"amount": 0.5})],
-                       color=[("hue_shift", {"offset": 40.0}),
+                       color=[("hue_shift", {"offset": 40.0}), ("bloom", {"intensity": 2.0}),
                               ("vignette", {"intensity": 0.7})])
     job = pyfae.make_job(200, 150, walkers=8192, ipt=64, target_samples=2_000_000, splat_mode=2,
                          zoom=1.1)
