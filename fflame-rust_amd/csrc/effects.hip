@@ -907,6 +907,94 @@ __global__ __launch_bounds__(256) void bloom_composite_kernel(BloomArgs b) {
                   to_unorm8(clamp01(c.z + glow.z * 1.0f * b.intensity)), a.w);
 }
 
+// --------------------------------------------------------- distance field
+
+// A pixel's coordinates are its centre (x + 0.5, y + 0.5), as the fragment
+// position; input_at / buffer_at truncate them back to the texel.
+__device__ __forceinline__ float4 texel_u8(const uint8_t* img, int W, int H, int x, int y) {
+  x = min(max(x, 0), W - 1);
+  y = min(max(y, 0), H - 1);
+  const uchar4 c = reinterpret_cast<const uchar4*>(img)[(size_t)y * W + x];
+  return make_float4((float)c.x / 255.0f, (float)c.y / 255.0f, (float)c.z / 255.0f,
+                     (float)c.w / 255.0f);
+}
+
+// seed(): a pixel brighter than the threshold seeds itself, others -1.
+__global__ __launch_bounds__(256) void df_seed_kernel(DistArgs d) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= d.W || y >= d.H) return;
+  const float4 here = texel_u8(d.in, d.W, d.H, x, y);
+  const bool seed = lum3(here.x, here.y, here.z) > d.threshold;
+  reinterpret_cast<float2*>(d.a)[(size_t)y * d.W + x] =
+      seed ? make_float2((float)x + 0.5f, (float)y + 0.5f) : make_float2(-1.0f, -1.0f);
+}
+
+// flood(): the nearest seed among the 3x3 neighbours `step` apart.
+__global__ __launch_bounds__(256) void df_flood_kernel(DistArgs d, const float2* __restrict__ src,
+                                                       float2* __restrict__ dst, int step) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= d.W || y >= d.H) return;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  float2 best = make_float2(-1.0f, -1.0f);
+  float best_d = 1.0e30f;
+  for (int dy = -1; dy < 2; dy++) {
+    for (int dx = -1; dx < 2; dx++) {
+      const int qx = x + dx * step, qy = y + dy * step;
+      if (qx >= 0 && qy >= 0 && qx < d.W && qy < d.H) {
+        const float2 s = src[(size_t)qy * d.W + qx];
+        if (s.x >= 0.0f) {
+          const float dist = length2(s.x - px, s.y - py);
+          if (dist < best_d) {
+            best_d = dist;
+            best = s;
+          }
+        }
+      }
+    }
+  }
+  dst[(size_t)y * d.W + x] = best;
+}
+
+// effect(): glow (mode 0), contour bands (1) or nearest-seed fill (2)
+// over the picture, by the distance to the nearest seed; unreached pixels
+// and the alpha as they were; written unblended.
+__global__ __launch_bounds__(256) void df_effect_kernel(DistArgs d,
+                                                        const float2* __restrict__ coords) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= d.W || y >= d.H) return;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  const float4 base = texel_u8(d.in, d.W, d.H, x, y);
+  const float2 s = coords[(size_t)y * d.W + x];
+  float r = base.x, g = base.y, bl = base.z;
+  if (s.x >= 0.0f) {
+    const float range_px = fmaxf(d.spread, 0.01f) * (float)max(d.W, d.H);
+    const float dist = length2(s.x - px, s.y - py);
+    const float t = clamp01(dist / range_px);
+    const float4 sc = texel_u8(d.in, d.W, d.H, (int)s.x, (int)s.y);
+    const float off = dist > 0.5f ? 1.0f : 0.0f;
+    if (d.mode == 0.0f) {
+      const float k = expf(-4.0f * t);
+      r = base.x + sc.x * k * off * d.intensity;
+      g = base.y + sc.y * k * off * d.intensity;
+      bl = base.z + sc.z * k * off * d.intensity;
+    } else if (d.mode == 1.0f) {
+      const float phase = fractf(t * fmaxf(d.band_count, 1.0f));
+      const float ring = smooth(0.0f, 0.15f, phase) * (1.0f - smooth(0.35f, 0.5f, phase));
+      r = base.x + sc.x * ring * (1.0f - t) * d.intensity * off;
+      g = base.y + sc.y * ring * (1.0f - t) * d.intensity * off;
+      bl = base.z + sc.z * ring * (1.0f - t) * d.intensity * off;
+    } else if (dist > 0.5f) {
+      const float m = clamp01(d.intensity) * (1.0f - t);
+      r = base.x * (1.0f - m) + sc.x * m;
+      g = base.y * (1.0f - m) + sc.y * m;
+      bl = base.z * (1.0f - m) + sc.z * m;
+    }
+  }
+  const uchar4 a = reinterpret_cast<const uchar4*>(d.in)[(size_t)y * d.W + x];
+  reinterpret_cast<uchar4*>(d.out)[(size_t)y * d.W + x] =
+      make_uchar4(to_unorm8(r), to_unorm8(g), to_unorm8(bl), a.w);
+}
+
 }  // namespace
 
 hipError_t launch_density_effect(const FxConst& k, const float* in, float* out, hipStream_t s) {
@@ -945,6 +1033,25 @@ hipError_t launch_bloom(const BloomArgs& b, hipStream_t s) {
   }
   hipLaunchKernelGGL(bloom_composite_kernel, dim3((b.W + 15) / 16, (b.H + 15) / 16), dim3(256), 0,
                      s, b);
+  return hipGetLastError();
+}
+
+// The jump flood's passes: log2_passes (src/effects/passes.rs:110-113),
+// step 2^(n-1-r) for pass r.
+hipError_t launch_distance_field(const DistArgs& d, hipStream_t s) {
+  const dim3 grid((d.W + 15) / 16, (d.H + 15) / 16);
+  hipLaunchKernelGGL(df_seed_kernel, grid, dim3(256), 0, s, d);
+  if (hipError_t e = hipGetLastError()) return e;
+  const uint32_t m = (uint32_t)std::max(std::max(d.W, d.H), 2) - 1u;
+  const int n = 32 - __builtin_clz(m);
+  float2* cur = reinterpret_cast<float2*>(d.a);
+  float2* nxt = reinterpret_cast<float2*>(d.b);
+  for (int r = 0; r < n; r++) {
+    hipLaunchKernelGGL(df_flood_kernel, grid, dim3(256), 0, s, d, cur, nxt, 1 << (n - 1 - r));
+    if (hipError_t e = hipGetLastError()) return e;
+    std::swap(cur, nxt);
+  }
+  hipLaunchKernelGGL(df_effect_kernel, grid, dim3(256), 0, s, d, cur);
   return hipGetLastError();
 }
 

@@ -200,6 +200,18 @@ struct BloomArgs {
 int bloom_levels(int W, int H, int32_t* lw, int32_t* lh);
 hipError_t launch_bloom(const BloomArgs& b, hipStream_t s);
 
+// distance_field (assets/effects/distance_field.rhai): seed, jump flood
+// (log2 passes, ping-pong a / b of nearest-seed coordinates), shade.
+struct DistArgs {
+  const uint8_t* in;
+  uint8_t* out;
+  float* a;  // W x H float2
+  float* b;
+  int32_t W, H;
+  float threshold, spread, mode, intensity, band_count;
+};
+hipError_t launch_distance_field(const DistArgs& d, hipStream_t s);
+
 // ---- batched gallery: many flames in one persistent launch
 
 struct BatchFlame {

@@ -59,7 +59,7 @@ struct ffm_ctx {
   // render buffers
   DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv, count;
   DevBuf fx_a, fx_b, fx_rgba;    // effect chain ping-pong buffers
-  DevBuf fx_pyr;                 // bloom's down and up pyramids
+  DevBuf fx_pyr;                 // bloom's pyramids, the distance field's coordinates
   ffm_effects fx{};              // effects renders apply (none by default)
   // LDS tile splat: per-tile fill/region/capacity, work list, record buffer
   DevBuf t_fill, t_region, t_cap, t_items, t_records, t_sorted;
@@ -983,6 +983,7 @@ static int fx_chain(int32_t type) {
     case FFM_FX_SIMPLEX_NOISE:
     case FFM_FX_DOMAIN_WARP:
     case FFM_FX_BLOOM:
+    case FFM_FX_DISTANCE_FIELD:
       return 2;
     default:
       return 0;
@@ -1013,6 +1014,7 @@ int ffm_effect_defaults(int32_t type, ffm_effect* out) {
     case FFM_FX_SIMPLEX_NOISE: v = {0.5f, 5.0f, 4.0f, 0.0f, 0.0f, 4.0f, 225.0f}; break;
     case FFM_FX_DOMAIN_WARP: v = {0.1f, 3.0f, 4.0f, 0.0f, 0.0f, 225.0f}; break;
     case FFM_FX_BLOOM: v = {0.45f, 0.3f, 1.5f, 0.7f}; break;  // bloom.rhai (tint white)
+    case FFM_FX_DISTANCE_FIELD: v = {0.15f, 0.25f, 0.0f, 1.0f, 12.0f}; break;
     default: return FFM_ERR_INVALID_ARGUMENT;
   }
   for (size_t i = 0; i < v.size(); i++) e.params[i] = v[i];
@@ -1102,6 +1104,24 @@ int ffm_density_effect(ffm_ctx* c, const ffm_effect* fx, const float* in, float*
 // pyramids, any other effect's one pass.
 static int color_effect_on(ffm_ctx* c, const ffm_effect& e, const uint8_t* in, uint8_t* out, int W,
                            int H, float pixel_scale, hipStream_t s) {
+  if (e.type == FFM_FX_DISTANCE_FIELD) {
+    int st;
+    if ((st = ensure(c, c->fx_pyr, (size_t)W * H * 16, "distance field coordinates"))) return st;
+    DistArgs d{};
+    d.in = in;
+    d.out = out;
+    d.a = reinterpret_cast<float*>(c->fx_pyr.p);
+    d.b = d.a + (size_t)W * H * 2;
+    d.W = W;
+    d.H = H;
+    d.threshold = e.params[0];
+    d.spread = e.params[1];
+    d.mode = e.params[2];
+    d.intensity = e.params[3];
+    d.band_count = e.params[4];
+    HIPCHECK(c, launch_distance_field(d, s));
+    return FFM_OK;
+  }
   if (e.type != FFM_FX_BLOOM) {
     HIPCHECK(c, launch_color_effect(fx_const(e, W, H, 0.0f, pixel_scale), in, out, s));
     return FFM_OK;

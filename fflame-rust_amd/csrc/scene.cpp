@@ -756,6 +756,8 @@ const FxName kEffects[] = {
     {"domain_warp", FFM_FX_DOMAIN_WARP, 2,
      {"intensity", "scale", "octaves", "time", "blend_mode", "direction"}},
     {"bloom", FFM_FX_BLOOM, 2, {"threshold", "softness", "intensity", "spread"}},
+    {"distance_field", FFM_FX_DISTANCE_FIELD, 2,
+     {"threshold", "spread", "mode", "intensity", "band_count"}},
 };
 
 // A config's effect list (Vec<EffectInstance>, src/effects/mod.rs:319-336:

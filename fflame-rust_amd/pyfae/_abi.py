@@ -150,6 +150,7 @@ EFFECTS = {
     "domain_warp": (27, "color", ("intensity", "scale", "octaves", "time", "blend_mode",
                                   "direction")),
     "bloom": (28, "color", ("threshold", "softness", "intensity", "spread")),
+    "distance_field": (29, "color", ("threshold", "spread", "mode", "intensity", "band_count")),
 }
 BLEND_MODES = ("Normal", "Add", "Multiply", "Screen", "Overlay", "Soft Light", "Hard Light",
                "Color Dodge", "Color Burn", "Hue", "Saturation", "Color", "Luminosity")

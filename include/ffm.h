@@ -449,8 +449,10 @@ enum ffm_effect_type {
                                         (colour, distort, mask), blend_mode 4, direction 225 */
   FFM_FX_DOMAIN_WARP = 27,           /* intensity 0.1, scale 3, octaves 4, time 0, blend_mode 0,
                                         direction 225 (degrees) */
-  FFM_FX_BLOOM = 28                  /* threshold 0.45, softness 0.3, intensity 1.5, spread 0.7
+  FFM_FX_BLOOM = 28,                 /* threshold 0.45, softness 0.3, intensity 1.5, spread 0.7
                                         (tint white); a 7-level half-size pyramid */
+  FFM_FX_DISTANCE_FIELD = 29         /* threshold 0.15, spread 0.25, mode 0 (glow, contours, fill),
+                                        intensity 1, band_count 12; a jump flood */
 };
 /* An EffectInstance (src/effects/mod.rs:319-336): params in the effect's
  * declared order (the slot order of get_param), as listed above. */

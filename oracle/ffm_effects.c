@@ -520,8 +520,96 @@ static void orc_bloom(const float* p, const uint8_t* in, uint8_t* out, int W, in
   }
 }
 
+/* ---- distance_field (assets/effects/distance_field.rhai): seed, jump
+ * flood over log2_passes (src/effects/passes.rs:110-113) with step
+ * 2^(n-1-r), shade. A pixel's coordinates are its centre. ---- */
+static px4 texel_u8(const uint8_t* img, int W, int H, int x, int y) {
+  const img_t im = {NULL, img, W, H};
+  return img_at(&im, clampi(x, 0, W - 1), clampi(y, 0, H - 1));
+}
+
+static void orc_distance_field(const float* p, const uint8_t* in, uint8_t* out, int W, int H) {
+  float* a = (float*)malloc((size_t)W * H * 8);
+  float* b = (float*)malloc((size_t)W * H * 8);
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const px4 h = texel_u8(in, W, H, x, y);
+      const int seed = lum3(h.v[0], h.v[1], h.v[2]) > p[0];
+      a[2 * ((size_t)y * W + x)] = seed ? (float)x + 0.5f : -1.0f;
+      a[2 * ((size_t)y * W + x) + 1] = seed ? (float)y + 0.5f : -1.0f;
+    }
+  int m = (W > H ? W : H);
+  if (m < 2) m = 2;
+  int n = 0;
+  for (unsigned v = (unsigned)(m - 1); v; v >>= 1) n++;
+  for (int r = 0; r < n; r++) {
+    const int step = 1 << (n - 1 - r);
+    for (int y = 0; y < H; y++)
+      for (int x = 0; x < W; x++) {
+        const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+        float bx = -1.0f, by = -1.0f, best_d = 1.0e30f;
+        for (int dy = -1; dy < 2; dy++)
+          for (int dx = -1; dx < 2; dx++) {
+            const int qx = x + dx * step, qy = y + dy * step;
+            if (qx >= 0 && qy >= 0 && qx < W && qy < H) {
+              const float sx = a[2 * ((size_t)qy * W + qx)], sy = a[2 * ((size_t)qy * W + qx) + 1];
+              if (sx >= 0.0f) {
+                const float d = len2(sx - px, sy - py);
+                if (d < best_d) {
+                  best_d = d;
+                  bx = sx;
+                  by = sy;
+                }
+              }
+            }
+          }
+        b[2 * ((size_t)y * W + x)] = bx;
+        b[2 * ((size_t)y * W + x) + 1] = by;
+      }
+    float* t = a;
+    a = b;
+    b = t;
+  }
+  const float spread = p[1], mode = p[2], inten = p[3], bands = p[4];
+  for (int y = 0; y < H; y++)
+    for (int x = 0; x < W; x++) {
+      const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+      const px4 base = texel_u8(in, W, H, x, y);
+      const float sx = a[2 * ((size_t)y * W + x)], sy = a[2 * ((size_t)y * W + x) + 1];
+      float rgb[3] = {base.v[0], base.v[1], base.v[2]};
+      if (sx >= 0.0f) {
+        const float range_px = fmaxf(spread, 0.01f) * (float)(W > H ? W : H);
+        const float d = len2(sx - px, sy - py);
+        const float t = clamp01(d / range_px);
+        const px4 sc = texel_u8(in, W, H, (int)sx, (int)sy);
+        const float off = d > 0.5f ? 1.0f : 0.0f;
+        for (int k = 0; k < 3; k++) {
+          if (mode == 0.0f) {
+            rgb[k] = base.v[k] + sc.v[k] * expf(-4.0f * t) * off * inten;
+          } else if (mode == 1.0f) {
+            const float phase = fractf(t * fmaxf(bands, 1.0f));
+            const float ring = smooth(0.0f, 0.15f, phase) * (1.0f - smooth(0.35f, 0.5f, phase));
+            rgb[k] = base.v[k] + sc.v[k] * ring * (1.0f - t) * inten * off;
+          } else if (d > 0.5f) {
+            const float mm = clamp01(inten) * (1.0f - t);
+            rgb[k] = base.v[k] * (1.0f - mm) + sc.v[k] * mm;
+          }
+        }
+      }
+      uint8_t* o = out + 4 * ((size_t)y * W + x);
+      for (int k = 0; k < 3; k++) o[k] = unorm8(rgb[k]);
+      o[3] = in[4 * ((size_t)y * W + x) + 3];
+    }
+  free(a);
+  free(b);
+}
+
 void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int W, int H,
                       float pixel_scale) {
+  if (fx->type == FFM_FX_DISTANCE_FIELD) {
+    orc_distance_field(fx->params, in, out, W, H);
+    return;
+  }
   if (fx->type == FFM_FX_BLOOM) {
     orc_bloom(fx->params, in, out, W, H);
     return;

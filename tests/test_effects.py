@@ -224,7 +224,6 @@ def test_config_effect_lists_load():
 
 def test_config_effects_this_engine_lacks_are_refused():
     for d, c in (([{"effect_type": "bloom", "enabled": True}], []),
-                 ([], [{"effect_type": "distance_field", "enabled": True}]),
                  ([], [{"effect_type": "trails", "enabled": True}]),
                  ([{"effect_type": "vignette", "enabled": True}], [])):
         with pytest.raises(pyfae.EngineError) as e:
@@ -390,3 +389,25 @@ def test_bloom_closed_forms():
     assert np.all(out >= dot) and row[16] > row[12] > row[8] > 0
     assert np.abs(out[:, :22] - out[:, 44:22:-1]).max() <= 1
     assert out[15, 22, 0] == 255
+
+
+def test_distance_field_is_the_distance_to_the_nearest_seed():
+    """The jump flood against brute force: each unseeded pixel's nearest
+    seed distance (ties aside) from the glow it shades by."""
+    H, W = 23, 37
+    img = np.zeros((H, W, 4), np.uint8)
+    img[..., 3] = 255
+    seeds = [(3, 4), (18, 30), (11, 20), (20, 2)]
+    for y, x in seeds:
+        img[y, x, :3] = 200
+    inten = 1.0
+    out = O.color_effect(pyfae.effect("distance_field", intensity=inten, spread=0.5), img)
+    yy, xx = np.mgrid[0:H, 0:W]
+    d = np.min([np.hypot(xx - x, yy - y) for y, x in seeds], axis=0)
+    t = np.clip(d / (0.5 * max(W, H)), 0, 1)
+    want = np.where(d > 0.5, 200 / 255 * np.exp(-4 * t) * inten, 200 / 255)
+    got = out[..., 0] / 255.0
+    assert np.abs(got - np.clip(want, 0, 1)).max() <= 1.5 / 255
+    # no seed at all: the picture as it was
+    dark = np.zeros((H, W, 4), np.uint8)
+    assert np.array_equal(O.color_effect(pyfae.effect("distance_field"), dark), dark)
