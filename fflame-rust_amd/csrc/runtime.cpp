@@ -984,6 +984,7 @@ static int fx_chain(int32_t type) {
     case FFM_FX_DOMAIN_WARP:
     case FFM_FX_BLOOM:
     case FFM_FX_DISTANCE_FIELD:
+    case FFM_FX_TRAILS:
       return 2;
     default:
       return 0;
@@ -1015,6 +1016,7 @@ int ffm_effect_defaults(int32_t type, ffm_effect* out) {
     case FFM_FX_DOMAIN_WARP: v = {0.1f, 3.0f, 4.0f, 0.0f, 0.0f, 225.0f}; break;
     case FFM_FX_BLOOM: v = {0.45f, 0.3f, 1.5f, 0.7f}; break;  // bloom.rhai (tint white)
     case FFM_FX_DISTANCE_FIELD: v = {0.15f, 0.25f, 0.0f, 1.0f, 12.0f}; break;
+    case FFM_FX_TRAILS: v = {0.85f, 1.0f}; break;
     default: return FFM_ERR_INVALID_ARGUMENT;
   }
   for (size_t i = 0; i < v.size(); i++) e.params[i] = v[i];
@@ -1104,6 +1106,14 @@ int ffm_density_effect(ffm_ctx* c, const ffm_effect* fx, const float* in, float*
 // pyramids, any other effect's one pass.
 static int color_effect_on(ffm_ctx* c, const ffm_effect& e, const uint8_t* in, uint8_t* out, int W,
                            int H, float pixel_scale, hipStream_t s) {
+  if (e.type == FFM_FX_TRAILS) {
+    // trails.rhai keeps max(c, history * fade) in a buffer that persists
+    // across frames; a still starts from a zero history (EffectClock::STILL,
+    // src/renderer/effect_passes.rs:18-23), so the trail is the picture and
+    // c + (trail - c) * intensity is c.
+    HIPCHECK(c, hipMemcpyAsync(out, in, (size_t)W * H * 4, hipMemcpyDeviceToDevice, s));
+    return FFM_OK;
+  }
   if (e.type == FFM_FX_DISTANCE_FIELD) {
     int st;
     if ((st = ensure(c, c->fx_pyr, (size_t)W * H * 16, "distance field coordinates"))) return st;

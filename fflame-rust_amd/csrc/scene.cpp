@@ -758,6 +758,7 @@ const FxName kEffects[] = {
     {"bloom", FFM_FX_BLOOM, 2, {"threshold", "softness", "intensity", "spread"}},
     {"distance_field", FFM_FX_DISTANCE_FIELD, 2,
      {"threshold", "spread", "mode", "intensity", "band_count"}},
+    {"trails", FFM_FX_TRAILS, 2, {"persistence", "intensity"}},
 };
 
 // A config's effect list (Vec<EffectInstance>, src/effects/mod.rs:319-336:

@@ -151,6 +151,7 @@ EFFECTS = {
                                   "direction")),
     "bloom": (28, "color", ("threshold", "softness", "intensity", "spread")),
     "distance_field": (29, "color", ("threshold", "spread", "mode", "intensity", "band_count")),
+    "trails": (30, "color", ("persistence", "intensity")),
 }
 BLEND_MODES = ("Normal", "Add", "Multiply", "Screen", "Overlay", "Soft Light", "Hard Light",
                "Color Dodge", "Color Burn", "Hue", "Saturation", "Color", "Luminosity")

@@ -451,8 +451,10 @@ enum ffm_effect_type {
                                         direction 225 (degrees) */
   FFM_FX_BLOOM = 28,                 /* threshold 0.45, softness 0.3, intensity 1.5, spread 0.7
                                         (tint white); a 7-level half-size pyramid */
-  FFM_FX_DISTANCE_FIELD = 29         /* threshold 0.15, spread 0.25, mode 0 (glow, contours, fill),
+  FFM_FX_DISTANCE_FIELD = 29,        /* threshold 0.15, spread 0.25, mode 0 (glow, contours, fill),
                                         intensity 1, band_count 12; a jump flood */
+  FFM_FX_TRAILS = 30                 /* persistence 0.85, intensity 1: a temporal effect; a still
+                                        render has no history, so the picture passes unchanged */
 };
 /* An EffectInstance (src/effects/mod.rs:319-336): params in the effect's
  * declared order (the slot order of get_param), as listed above. */

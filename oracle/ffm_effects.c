@@ -610,6 +610,12 @@ void orc_color_effect(const ffm_effect* fx, const uint8_t* in, uint8_t* out, int
     orc_distance_field(fx->params, in, out, W, H);
     return;
   }
+  if (fx->type == FFM_FX_TRAILS) {
+    /* trails.rhai on a still: history zero, so keep() stores the picture
+     * (max(c, 0)) and effect() returns c + (trail - c) * intensity = c */
+    memcpy(out, in, (size_t)W * H * 4);
+    return;
+  }
   if (fx->type == FFM_FX_BLOOM) {
     orc_bloom(fx->params, in, out, W, H);
     return;

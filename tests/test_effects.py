@@ -224,7 +224,7 @@ def test_config_effect_lists_load():
 
 def test_config_effects_this_engine_lacks_are_refused():
     for d, c in (([{"effect_type": "bloom", "enabled": True}], []),
-                 ([], [{"effect_type": "trails", "enabled": True}]),
+                 ([], [{"effect_type": "my_rhai_effect", "enabled": True}]),
                  ([{"effect_type": "vignette", "enabled": True}], [])):
         with pytest.raises(pyfae.EngineError) as e:
             pyfae.load(cfg_with(d, c))
@@ -411,3 +411,12 @@ def test_distance_field_is_the_distance_to_the_nearest_seed():
     # no seed at all: the picture as it was
     dark = np.zeros((H, W, 4), np.uint8)
     assert np.array_equal(O.color_effect(pyfae.effect("distance_field"), dark), dark)
+
+
+def test_trails_on_a_still_is_the_picture():
+    """trails keeps max(c, history * fade); a still render starts from a
+    zero history (src/renderer/effect_chain.rs:1746 'a still has no
+    trail'), so c + (trail - c) * intensity is c."""
+    img = (RNG.random((9, 14, 4)) * 255).astype(np.uint8)
+    for inten in (0.0, 0.4, 1.0):
+        assert np.array_equal(O.color_effect(pyfae.effect("trails", intensity=inten), img), img)

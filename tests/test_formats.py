@@ -239,7 +239,7 @@ def test_palette_forms():
     ({"color_mode": "speed"}, "color mode"),
     ({"highlight_mode": "agx"}, "highlight"),
     ({"density_effects": [{"effect_type": "bloom", "enabled": True}]}, "density effect 'bloom'"),
-    ({"color_effects": [{"effect_type": "trails", "enabled": True}]}, "trails"),
+    ({"color_effects": [{"effect_type": "my_rhai_effect", "enabled": True}]}, "my_rhai_effect"),
     ({"auto_exposure": True}, "auto exposure"),
     ({"use_curve": True, "tonemap_curve": {"points": [{"x": 0, "y": 0}, {"x": 0.5, "y": 0.7},
                                                       {"x": 1, "y": 1}]}}, "tone curve"),

@@ -89,6 +89,7 @@ COLOUR_CASES = [
     ("bloom", {}), ("bloom", {"threshold": 0.1, "softness": 0.8, "intensity": 3.0, "spread": 1.0}),
     ("distance_field", {}), ("distance_field", {"threshold": 0.5, "mode": 1.0, "band_count": 7.0}),
     ("distance_field", {"threshold": 0.6, "mode": 2.0, "spread": 0.6, "intensity": 0.8}),
+    ("trails", {}),
 ] + [("vignette", {"intensity": 0.8, "radius": 0.3, "softness": 0.4, "blend_mode": float(m)})
      for m in range(13)]
 
