@@ -73,10 +73,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="single4k",
-                    choices=["single4k", "gallery", "plain", "fit", "fit_batch", "stress"],
+                    choices=["single4k", "gallery", "plain", "fit", "fit_batch", "stress",
+                             "effects"],
                     help="single4k (the headline), gallery (batched vs one by one), plain, "
                          "fit (differentiable soft-splat fit step), fit_batch (64 fits), "
-                         "stress (12288^2 f32, 2^34 samples, LDS bins vs global atomics)")
+                         "stress (12288^2 f32, 2^34 samples, LDS bins vs global atomics), "
+                         "effects (every tail effect at 3840x2160, defaults)")
     ap.add_argument("--no-fd", action="store_true", help="fit: skip the finite-difference check")
     ap.add_argument("--gallery-size", type=int, default=512)
     ap.add_argument("--steps", type=int, default=5)
@@ -99,6 +101,8 @@ def main():
 
     if args.workload == "gallery":
         return bench_gallery(args)
+    if args.workload == "effects":
+        return bench_effects(args)
     if args.workload in ("fit", "fit_batch"):
         return bench_fit(args)
     if args.workload == "stress":
@@ -122,6 +126,48 @@ def main():
                        "single4k: 6 xforms + final xform, 3 of 10 variations each, 3840x2160, "
                        "2^%d walkers x 256 iterations, density estimation 9->0 curve 0.4, log "
                        "tonemap" % args.walkers_log2, "chaos-game samples/s at 4K")
+
+
+def bench_effects(args):
+    """Each built-in tail effect at its defaults on a 3840x2160 picture
+    (density effects on a random f32 accumulator, colour effects on random
+    RGBA8), HIP events around `steps` launches after `warmup`."""
+    import torch
+    import pyfae
+    from pyfae import _abi
+    eng = pyfae.Engine(0)
+    H, W = 2160, 3840
+    g = torch.Generator(device="cuda").manual_seed(1)
+    src = torch.rand((H, W, 4), device="cuda", generator=g)
+    dst = torch.empty_like(src)
+    s8 = (torch.rand((H, W, 4), device="cuda", generator=g) * 255).to(torch.uint8)
+    d8 = torch.empty_like(s8)
+    ms = {}
+    for name, (_, chain, _) in _abi.EFFECTS.items():
+        fx = pyfae.effect(name)
+        if chain == "density":
+            run = lambda: eng.density_effect(fx, src, dst, density_mean=1.0)  # noqa: E731
+        else:
+            run = lambda: eng.color_effect(fx, s8, d8)  # noqa: E731
+        for _ in range(max(args.warmup, 1)):
+            run()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.steps):
+            run()
+        b.record()
+        torch.cuda.synchronize()
+        ms[name] = round(a.elapsed_time(b) / args.steps, 4)
+    total = sum(ms.values())
+    print(json.dumps({
+        "metric": "tail effects ms at 3840x2160 (sum over the 18 built-ins)", "value": total,
+        "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": total, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "effects: each built-in density and colour effect at its "
+                               "defaults, 3840x2160"},
+        "effect_ms": ms}))
 
 
 def bench_gallery(args):
