@@ -420,3 +420,18 @@ def test_trails_on_a_still_is_the_picture():
     img = (RNG.random((9, 14, 4)) * 255).astype(np.uint8)
     for inten in (0.0, 0.4, 1.0):
         assert np.array_equal(O.color_effect(pyfae.effect("trails", intensity=inten), img), img)
+
+
+def test_effect_defaults_and_bad_types():
+    import ctypes as C
+    from pyfae import _abi
+    lib = pyfae.library()
+    fx = _abi.Effect()
+    assert lib.ffm_effect_defaults(0, C.byref(fx)) != 0
+    assert lib.ffm_effect_defaults(99, C.byref(fx)) != 0
+    assert lib.ffm_effect_defaults(1, None) != 0
+    # every registry effect has defaults, enabled, type echoed
+    for name, (t, _, names) in _abi.EFFECTS.items():
+        assert lib.ffm_effect_defaults(t, C.byref(fx)) == 0, name
+        assert fx.type == t and fx.enabled == 1
+        assert all(fx.params[i] == 0.0 for i in range(len(names), _abi.MAX_EFFECT_PARAMS))
