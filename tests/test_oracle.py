@@ -298,7 +298,12 @@ def test_palette_sample_edges_and_interpolation():
 # ------------------------------------------------------------ variations
 
 PROBE = [(0.0, 0.0), (-0.0, 0.0), (1.0, 0.0), (0.0, 1.0), (-1.0, 0.0), (0.0, -1.0), (0.5, 0.5),
-         (-0.5, 0.25), (1e-20, 1e-20), (3.0, -4.0), (1e5, 2e5), (-2.5, -0.125), (0.1, 0.9)]
+         (-0.5, 0.25), (1e-20, 1e-20), (3.0, -4.0), (1e5, 2e5), (-2.5, -0.125), (0.1, 0.9),
+         # the rest of the reference's probe grid where WGSL pins the arithmetic
+         # (src/probe/inputs.rs:44-99): signed zeros, diagonals, the unit circle
+         (0.0, -0.0), (-0.0, -0.0), (1.0, -0.0), (-0.0, 1.0), (1.0, 1.0), (-1.0, 1.0),
+         (-1.0, -1.0), (1.0, -1.0), (0.5, 0.3), (0.12345679, -0.9876543), (2.5, -1.5),
+         (math.pi, math.e), (0.70710677, 0.70710677)]
 
 
 def _var(name, x, y, params=(0, 0, 0, 0), xf=None, rng_state=(0, 1)):
