@@ -35,6 +35,9 @@ constexpr int BIN_BLOCK = 1024;
 constexpr int STAGE_MIN = 10240, STAGE_MAX = 16384;
 constexpr int MAX_FLUSH_EVERY = 8;  // iterations between fill checks, at most
 constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
+// accum_run adds two channels per 64-bit LDS atomic: a slice's channel sums
+// must not carry out of 32 bits (at most 255, or 100 for density, a record)
+static_assert((unsigned long long)SLICE * 255ull < (1ull << 32), "tile channel sums carry");
 constexpr size_t STAGE_FIXED = (size_t)TILE_MAX * 8 + 16 * 4 + 16;  // cnt, gbase, wsum, fill
 constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4 + 2;                  // rec, tr, perm
 
@@ -410,11 +413,17 @@ __device__ __forceinline__ void accum_run(const uint32_t* run, uint32_t n, uint3
       atomicAdd(&tilef[4 * local + 2], c[2]);
       atomicAdd(&tilef[4 * local + 3], 100.0f);
     } else {
+      // Two 64-bit LDS atomics add (r, g) and (b, density) to the pixel's
+      // little-endian u32 quad. No carry crosses a channel: a run holds at
+      // most SLICE = 2^20 records and a channel adds at most 255 a record
+      // (2^28 in all), so each 32-bit half stays below 2^32.
       const uint32_t pk = pal[rec_texel(v)];
-      atomicAdd(&tile[4 * local + 0], pk & 0xFFu);
-      atomicAdd(&tile[4 * local + 1], (pk >> 8) & 0xFFu);
-      atomicAdd(&tile[4 * local + 2], (pk >> 16) & 0xFFu);
-      atomicAdd(&tile[4 * local + 3], 100u);
+      unsigned long long* t64 = reinterpret_cast<unsigned long long*>(tile);
+      const unsigned long long rg =
+          (unsigned long long)(pk & 0xFFu) | (unsigned long long)((pk >> 8) & 0xFFu) << 32;
+      const unsigned long long bd = (unsigned long long)((pk >> 16) & 0xFFu) | 100ull << 32;
+      atomicAdd(&t64[2 * local + 0], rg);
+      atomicAdd(&t64[2 * local + 1], bd);
     }
   }
 }
