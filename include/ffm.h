@@ -228,8 +228,10 @@ int ffm_palette_set(ffm_ctx* ctx, const float* stops_prgb, int n_stops, int tabl
  * pixels into `rgba_out` (width*height*4 bytes, host memory). */
 int ffm_render(ffm_ctx* ctx, const ffm_job* job, uint8_t* rgba_out, ffm_render_stats* stats);
 
-/* The same render, leaving the accumulator (RGBA32F, row-major) in
- * `accum_out` (host, width*height*4 floats) instead of tonemapping. */
+/* The same render, leaving the accumulator (RGBA32F, row-major; after the
+ * density estimator and the context's density effects: what the tone map
+ * would read) in `accum_out` (host, width*height*4 floats) instead of
+ * tonemapping. */
 int ffm_render_accum(ffm_ctx* ctx, const ffm_job* job, float* accum_out, ffm_render_stats* stats);
 
 /* The same render with the RGBA8 pixels left in device memory
