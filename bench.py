@@ -91,7 +91,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the walkers would shard with one histogram reduce per batch; see DESIGN.md)",
+              "(the walkers would shard over GPUs by walker range, with one histogram "
+              "reduce per batch)",
               file=sys.stderr)
         sys.exit(2)
 
