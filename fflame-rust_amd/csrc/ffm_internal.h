@@ -169,6 +169,8 @@ struct TonemapConst {
   float bg_lin[3];
   float inv22;
   int32_t transparent;  // transparent_mode: 0 opaque, 1 straight, 2 premultiplied
+  int32_t use_curve;    // map clipped colours through curve_lut
+  float curve_lut[256]; // the tone curve at i/255, each rounded to f16
 };
 hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H,
                           const TonemapConst& tc, hipStream_t s);

@@ -689,6 +689,70 @@ void de_table(float max_r, float min_r, float curve, std::vector<float>& h2,
   }
 }
 
+// A tone curve the tone map can sample: 2..FFM_MAX_CURVE_POINTS finite
+// points with non-decreasing x (curve_n 0 stands for the linear curve).
+bool curve_ok(const ffm_tonemap_params& tp) {
+  if (!tp.use_curve || tp.curve_n == 0) return true;
+  if (tp.curve_n < 2 || tp.curve_n > FFM_MAX_CURVE_POINTS) return false;
+  for (int i = 0; i < tp.curve_n; i++) {
+    if (!std::isfinite(tp.curve_x[i]) || !std::isfinite(tp.curve_y[i])) return false;
+    if (i > 0 && tp.curve_x[i] < tp.curve_x[i - 1]) return false;
+  }
+  return true;
+}
+
+// v rounded to the nearest f16 value, ties to even, as a float (the
+// f16::from_f32 of ToneCurve::generate_lut, src/scene/tonemap.rs:289-303).
+float round_f16(float v) {
+  if (std::isnan(v)) return v;
+  if (!(std::fabs(v) < 65520.0f)) return std::copysign(INFINITY, v);
+  int e;
+  std::frexp(v, &e);                 // 2^(e-1) <= |v| < 2^e
+  const int q = std::max(e - 11, -24);  // f16's spacing there: 10 fraction bits, subnormals 2^-24
+  return std::ldexp(std::nearbyint(std::ldexp(v, -q)), q);
+}
+
+// ToneCurve::evaluate (src/scene/tonemap.rs:209-286): the cubic Hermite
+// through the points, tangents the mean of the neighbouring slopes (the
+// segment's own at the ends), limited by Fritsch-Carlson: zero on a flat
+// segment, the secant when a tangent opposes it, scaled onto
+// alpha^2 + beta^2 = 9 when beyond. Every step in f32.
+float curve_eval(const float* px, const float* py, int n, float x) {
+  x = std::min(std::max(x, 0.0f), 1.0f);
+  for (int i = 0; i + 1 < n; i++) {
+    const float x0 = px[i], y0 = py[i], x1 = px[i + 1], y1 = py[i + 1];
+    if (!(x >= x0 && x <= x1)) continue;
+    const float delta = (y1 - y0) / (x1 - x0);
+    const float m0 = i == 0 ? delta : ((y0 - py[i - 1]) / (x0 - px[i - 1]) + delta) * 0.5f;
+    const float m1 = i == n - 2 ? delta : (delta + (py[i + 2] - y1) / (px[i + 2] - x1)) * 0.5f;
+    float t0, t1;
+    if (std::fabs(delta) < 1e-6f) {
+      t0 = t1 = 0.0f;
+    } else {
+      const float alpha = m0 / delta, beta = m1 / delta;
+      if (alpha < 0.0f || beta < 0.0f) {
+        t0 = t1 = delta;
+      } else {
+        const float sum_sq = alpha * alpha + beta * beta;
+        if (sum_sq > 9.0f) {
+          const float tau = 3.0f / std::sqrt(sum_sq);
+          t0 = tau * alpha * delta;
+          t1 = tau * beta * delta;
+        } else {
+          t0 = m0;
+          t1 = m1;
+        }
+      }
+    }
+    const float t = (x - x0) / (x1 - x0), t2 = t * t, t3 = t2 * t;
+    const float h00 = 2.0f * t3 - 3.0f * t2 + 1.0f, h10 = t3 - 2.0f * t2 + t;
+    const float h01 = -2.0f * t3 + 3.0f * t2, h11 = t3 - t2;
+    const float dx = x1 - x0;
+    return h00 * y0 + h10 * dx * t0 + h01 * y1 + h11 * dx * t1;
+  }
+  return x;
+}
+
 TonemapConst make_tonemap(const ffm_tonemap_params& tp, int W, int H, float zoom,
                           double samples_in_buffer) {
   // update_tonemap (src/renderer/compute_kernel.rs:3813-3918) and
@@ -723,6 +787,15 @@ TonemapConst make_tonemap(const ffm_tonemap_params& tp, int W, int H, float zoom
   for (int k = 0; k < 3; k++) t.bg_lin[k] = std::pow(std::max(tp.background[k], 0.0f), 2.2f);
   t.inv22 = (float)(1.0 / 2.2);
   t.transparent = tp.transparent_mode;
+  t.use_curve = tp.use_curve != 0;
+  if (t.use_curve) {
+    static const float lin_x[2] = {0.0f, 1.0f};
+    const bool lin = tp.curve_n == 0;
+    const float* px = lin ? lin_x : tp.curve_x;
+    const float* py = lin ? lin_x : tp.curve_y;
+    const int n = lin ? 2 : tp.curve_n;
+    for (int i = 0; i < 256; i++) t.curve_lut[i] = round_f16(curve_eval(px, py, n, (float)i / 255.0f));
+  }
   return t;
 }
 
@@ -955,6 +1028,10 @@ int ffm_tonemap(ffm_ctx* c, const float* accum, uint8_t* rgba, int width, int he
     return fail(c, FFM_ERR_INVALID_ARGUMENT,
                 "tonemap mode %d / highlight mode %d / transparent mode %d not supported",
                 tp->tonemap_mode, tp->highlight_mode, tp->transparent_mode);
+  if (!curve_ok(*tp))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                "tone curve needs 2..%d finite points with non-decreasing x (curve_n %d)",
+                FFM_MAX_CURVE_POINTS, tp->curve_n);
   (void)hipSetDevice(c->device);
   TonemapConst t = make_tonemap(*tp, width, height, zoom, samples_in_buffer);
   HIPCHECK(c, launch_tonemap(accum, rgba, width, height, t, pick(c, stream)));
@@ -1386,6 +1463,10 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
         job->tonemap.highlight_mode < 0 || job->tonemap.highlight_mode > 3 ||
         job->tonemap.transparent_mode < 0 || job->tonemap.transparent_mode > 2)
       return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode not supported");
+    if (!curve_ok(job->tonemap))
+      return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                  "tone curve needs 2..%d finite points with non-decreasing x",
+                  FFM_MAX_CURVE_POINTS);
     HIPCHECK(c, launch_tonemap(final_acc, pix, W, H, t, s));
     if ((st = run_color_chain(c, effects, pix, W, H, s))) return st;
     if (rgba_out) {
@@ -1508,6 +1589,9 @@ int check_batch(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* co
     if (flames[i].has_xaos)
       return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame %d: xaos is not part of the batched path", i);
     if ((st = check_view(c, &jobs[i].view))) return st;
+    if (!curve_ok(jobs[i].tonemap))
+      return fail(c, FFM_ERR_INVALID_ARGUMENT, "job %d: tone curve needs 2..%d finite points "
+                  "with non-decreasing x", i, FFM_MAX_CURVE_POINTS);
     if (!stops[i] || n_stops[i] < 1) return fail(c, FFM_ERR_NO_PALETTE, "flame %d: No palette found", i);
     if (!rgba_out[i]) return fail(c, FFM_ERR_INVALID_ARGUMENT, "flame %d: pixel buffer is NULL", i);
     const ffm_job& j = jobs[i];

@@ -670,7 +670,8 @@ struct SceneIn {
   bool palette_reverse = false;
   float background[3] = {0, 0, 0};
   std::string tonemap_mode = "logarithmic", highlight_mode = "clip";
-  bool use_curve = true, curve_identity = true;
+  bool use_curve = true;
+  std::vector<float> curve_x, curve_y;  // tone curve points; empty: the linear curve
   float exposure = 1, gamma = 4, gamma_threshold = 5, brightness = 4, vibrancy = 1;
   float white_level = 200, saturation = 1, hue_shift = 0;
   float alpha_blend_low = 0.3f, alpha_blend_high = 0.8f;
@@ -920,18 +921,17 @@ float projection_strength(const JVal& v) {
   return 0.0f;
 }
 
-// The tone curve the reference's LUT leaves unchanged: every point on y = x.
-bool curve_is_identity(const JVal* c) {
-  if (!c) return true;
+// tonemap_curve: {"points": [{"x": .., "y": ..}, ..]} (ToneCurve,
+// src/scene/tonemap.rs:142-149); absent, the linear curve (no points kept).
+void json_curve(const JVal* c, SceneIn& s) {
+  if (!c || c->k == JVal::NUL) return;
   const JVal* pts = c->get("points");
-  if (!pts || pts->k != JVal::ARR) return true;
+  if (!pts || pts->k != JVal::ARR) raise(FFM_ERR_PARSE, "tonemap_curve: missing field `points`");
   for (auto& p : pts->a) {
-    const JVal* x = p.get("x");
-    const JVal* y = p.get("y");
-    if (!x || !y || x->k != JVal::NUM || y->k != JVal::NUM) return false;
-    if ((float)x->n != (float)y->n) return false;
+    s.curve_x.push_back(jnum_f32(p.get("x"), "tonemap_curve.points.x"));
+    s.curve_y.push_back(jnum_f32(p.get("y"), "tonemap_curve.points.y"));
   }
-  return true;
+  if (s.curve_x.size() < 2) raise(FFM_ERR_PARSE, "tonemap_curve: at least two points are needed");
 }
 
 void json_palette(const JVal& p, SceneIn& s) {
@@ -1121,7 +1121,7 @@ SceneIn json_config(const JVal& root) {
   s.tonemap_mode = jopt_str(root, "tonemap_mode", "logarithmic");
   s.highlight_mode = jopt_str(root, "highlight_mode", "clip");
   s.use_curve = jopt_bool(root, "use_curve", true);
-  s.curve_identity = curve_is_identity(root.get("tonemap_curve"));
+  json_curve(root.get("tonemap_curve"), s);
   s.exposure = jopt_f32(root, "exposure", 1.0f);
   s.gamma = jopt_f32(root, "gamma", 4.0f);
   s.gamma_threshold = jopt_f32(root, "gamma_threshold", 5.0f);
@@ -1345,6 +1345,7 @@ SceneIn xml_flame(XmlReader& rd, const XEvent& start) {
       for (auto& q : split_ws(v))
         if (parse_f32(q, &t)) cv.push_back(t);
       if (cv.size() == 48) {
+        std::vector<std::pair<float, float>> pts{{0.0f, 0.0f}};
         for (float tt : {0.25f, 0.5f, 0.75f}) {
           float sv = 1.0f - tt, s2 = sv * sv, s3 = s2 * sv, t2 = tt * tt, t3 = t2 * tt;
           float b0 = cv[2] * s3, b1 = cv[5] * 3.0f * s2 * tt, b2 = cv[8] * 3.0f * sv * t2,
@@ -1353,7 +1354,18 @@ SceneIn xml_flame(XmlReader& rd, const XEvent& start) {
           float ny = b0 * cv[1] + b1 * cv[4] + b2 * cv[7] + b3 * cv[10];
           float den = b0 + b1 + b2 + b3;
           if (std::fabs(den) < 1e-10f) continue;
-          if (nx / den != ny / den) s.curve_identity = false;
+          pts.push_back({nx / den, ny / den});
+        }
+        pts.push_back({1.0f, 1.0f});
+        std::stable_sort(pts.begin(), pts.end(),
+                         [](const std::pair<float, float>& a, const std::pair<float, float>& b) {
+                           return a.first < b.first;
+                         });
+        s.curve_x.clear();
+        s.curve_y.clear();
+        for (auto& q : pts) {
+          s.curve_x.push_back(q.first);
+          s.curve_y.push_back(q.second);
         }
       }
     } else if (k == "soloxform") {
@@ -1522,7 +1534,13 @@ void build_scene(const SceneIn& s, ffm_scene* out) {
   if (s.subflames) unsupported("subflames");
   if (s.post_symmetry) unsupported("post symmetry");
   if (s.custom_variations) unsupported("user-written variations");
-  if (s.use_curve && !s.curve_identity) unsupported("tone curve");
+  if (s.curve_x.size() > FFM_MAX_CURVE_POINTS)
+    raise(FFM_ERR_TOO_LARGE, "tone curve of %zu points: at most %d fit", s.curve_x.size(),
+          FFM_MAX_CURVE_POINTS);
+  for (size_t i = 0; i < s.curve_x.size(); i++)
+    if (!std::isfinite(s.curve_x[i]) || !std::isfinite(s.curve_y[i]) ||
+        (i && s.curve_x[i] < s.curve_x[i - 1]))
+      unsupported("tone curve whose points are not finite and sorted by x");
   if (s.xforms.empty()) raise(FFM_ERR_NO_FLAME, "the flame has no transforms");
   if ((int)s.xforms.size() > FFM_MAX_XFORMS)
     raise(FFM_ERR_TOO_LARGE, "%zu transforms: at most %d fit", s.xforms.size(), FFM_MAX_XFORMS);
@@ -1684,6 +1702,12 @@ void build_scene(const SceneIn& s, ffm_scene* out) {
   if (h < 0) raise(FFM_ERR_PARSE, "highlight_mode: unknown variant `%s`", s.highlight_mode.c_str());
   if (h >= 4) unsupported("highlight mode `" + s.highlight_mode + "`");
   tp.highlight_mode = h;
+  tp.use_curve = s.use_curve ? 1 : 0;
+  tp.curve_n = (int32_t)s.curve_x.size();
+  for (size_t i = 0; i < s.curve_x.size(); i++) {
+    tp.curve_x[i] = s.curve_x[i];
+    tp.curve_y[i] = s.curve_y[i];
+  }
   tp.exposure = s.exposure;
   tp.gamma = s.gamma;
   tp.gamma_threshold = s.gamma_threshold;

@@ -389,6 +389,15 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
   } else {
     for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
   }
+  // The tone curve (shaders/tonemap.wgsl:479-492): texel i of the table
+  // holds the curve at i/255 and is read with nearest filtering at
+  // u = (255c + 0.5)/256, i.e. texel floor(255c + 0.5), clamped to the edge.
+  if (t.use_curve && bucket_count > 0.001f) {
+    for (int k = 0; k < 3; k++) {
+      const float u = fminf(fmaxf(color[k] * 255.0f + 0.5f, 0.0f), 255.0f);
+      color[k] = t.curve_lut[(int)u];
+    }
+  }
   float leveled = 1.0f;
   if (t.sample_density > 0.0f) {
     float nd = a.w / t.sample_density;

@@ -79,6 +79,9 @@ class View(C.Structure):
                 ("pan_x", C.c_float), ("pan_y", C.c_float), ("rotation", C.c_float)]
 
 
+MAX_CURVE_POINTS = 16
+
+
 class TonemapParams(C.Structure):
     _fields_ = [
         ("tonemap_mode", C.c_int32), ("highlight_mode", C.c_int32),
@@ -90,6 +93,8 @@ class TonemapParams(C.Structure):
         ("levels_low", C.c_float), ("levels_high", C.c_float), ("levels_gamma", C.c_float),
         ("background", C.c_float * 3),
         ("transparent_mode", C.c_int32),
+        ("use_curve", C.c_int32), ("curve_n", C.c_int32),
+        ("curve_x", C.c_float * MAX_CURVE_POINTS), ("curve_y", C.c_float * MAX_CURVE_POINTS),
     ]
 
 
@@ -216,6 +221,7 @@ def tonemap_defaults():
     t.alpha_blend_low = 0.3
     t.alpha_blend_high = 0.8
     t.levels_enabled = 1
+    t.use_curve = 1  # the linear curve (curve_n 0), on by default
     t.levels_low = 0.0
     t.levels_high = 10.0
     t.levels_gamma = 0.5

@@ -232,7 +232,9 @@ def scene_to_config(sc, palette_name="Imported"):
         "background_color": [_f(v) for v in tp.background],
         "tonemap_mode": _TONEMAP[tp.tonemap_mode],
         "highlight_mode": _HIGHLIGHT[tp.highlight_mode],
-        "use_curve": False,
+        "use_curve": bool(tp.use_curve),
+        **({"tonemap_curve": {"points": [{"x": _f(tp.curve_x[i]), "y": _f(tp.curve_y[i])}
+                                         for i in range(tp.curve_n)]}} if tp.curve_n else {}),
         "exposure": _f(tp.exposure), "gamma": _f(tp.gamma),
         "gamma_threshold": _f(tp.gamma_threshold), "brightness": _f(tp.brightness),
         "vibrancy": _f(tp.vibrancy), "white_level": _f(tp.white_level),

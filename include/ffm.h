@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 6
+#define FFM_ABI_VERSION 7
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -150,6 +150,8 @@ typedef enum {
   FFM_SPLAT_F32_LDS = 4     /* f32 through the LDS screen tiles (palette <= 2048 texels) */
 } ffm_splat_mode;
 
+#define FFM_MAX_CURVE_POINTS 16
+
 /* Tonemap parameters: TonemapParams (shaders/tonemap.wgsl:9-63) as the
  * host fills it (src/renderer/compute_kernel.rs:3813-3918), log branch and
  * the modes around it. */
@@ -167,6 +169,16 @@ typedef struct {
    * flatten over black (rgb x a) gives the over-black picture; 2
    * transparent, premultiplied (rgb = the over-black colour, a = opacity) */
   int32_t transparent_mode;
+  /* The tone curve (ToneCurve, src/scene/tonemap.rs:142-300; applied at
+   * shaders/tonemap.wgsl:479-492): with use_curve set, every pixel whose
+   * bucket count exceeds 0.001 maps each clipped channel c through a
+   * 256-texel f16 table of the curve at i/255, read at texel
+   * round(255c). The curve is the monotone cubic Hermite through curve_n
+   * points (2..FFM_MAX_CURVE_POINTS, x non-decreasing, finite);
+   * curve_n = 0 with use_curve set is the linear curve (0,0)-(1,1). */
+  int32_t use_curve;
+  int32_t curve_n;
+  float curve_x[FFM_MAX_CURVE_POINTS], curve_y[FFM_MAX_CURVE_POINTS];
 } ffm_tonemap_params;
 
 /* A render job: RenderJob (src/renderer/render.rs:111-138) plus the knobs

@@ -1135,6 +1135,7 @@ void orc_tonemap_defaults(ffm_tonemap_params* tp) {
   tp->alpha_blend_low = 0.3f;
   tp->alpha_blend_high = 0.8f;
   tp->levels_enabled = 1;
+  tp->use_curve = 1; /* fractal_config.rs:475-476, the linear ToneCurve */
   tp->levels_low = 0.0f;
   tp->levels_high = 10.0f;
   tp->levels_gamma = 0.5f;
@@ -1195,8 +1196,86 @@ static uint8_t unorm8(float v) {
 
 /* shaders/tonemap.wgsl:224-590 fs_main, with TonemapParams as
  * update_tonemap fills them (src/renderer/compute_kernel.rs:3813-3918). */
+/* f16::from_f32 (round to nearest, ties to even; overflow to inf, 2^-24
+ * subnormals) taken back to f32, by the bit fields of the f32 input. */
+float orc_round_f16(float v) {
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  uint32_t sign = u & 0x80000000u, a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return v;                   /* NaN */
+  int e = (int)(a >> 23) - 127;                    /* unbiased exponent */
+  int drop;                                        /* f32 fraction bits f16 lacks */
+  if (e < -25) { a = 0; drop = 0; }                /* below half the smallest subnormal */
+  else drop = e < -14 ? 13 + (-14 - e) : 13;       /* subnormal f16 keeps fewer bits */
+  if (a) {
+    uint32_t m = (a & 0x7fffffu) | (e > -127 ? 0x800000u : 0);  /* significand, implicit 1 */
+    uint32_t keep = drop < 32 ? m >> drop : 0;
+    uint32_t rest = drop < 32 ? m & ((1u << drop) - 1u) : m, half = 1u << (drop - 1);
+    if (rest > half || (rest == half && (keep & 1u))) keep++;
+    /* back to a value: keep x 2^(e - 23 + drop) */
+    double val = ldexp((double)keep, e - 23 + drop);
+    if (val >= 65520.0) val = INFINITY;
+    float f = (float)val;
+    memcpy(&a, &f, 4);
+  }
+  a |= sign;
+  memcpy(&v, &a, 4);
+  return v;
+}
+
+/* ToneCurve::evaluate (src/scene/tonemap.rs:209-286): monotone cubic
+ * Hermite through the points in f32, tangents the mean of the neighbouring
+ * secants (the segment's own at the ends) limited by Fritsch-Carlson. */
+float orc_curve_eval(const float* px, const float* py, int n, float x) {
+  if (x < 0.0f) x = 0.0f;
+  if (x > 1.0f) x = 1.0f;
+  for (int i = 0; i < n - 1; i++) {
+    if (!(x >= px[i] && x <= px[i + 1])) continue;
+    float h = px[i + 1] - px[i];
+    float delta = (py[i + 1] - py[i]) / h;
+    float m0 = delta, m1 = delta;
+    if (i > 0) m0 = ((py[i] - py[i - 1]) / (px[i] - px[i - 1]) + delta) * 0.5f;
+    if (i < n - 2) m1 = (delta + (py[i + 2] - py[i + 1]) / (px[i + 2] - px[i + 1])) * 0.5f;
+    float g0, g1;
+    if (fabsf(delta) < 1e-6f) {
+      g0 = 0.0f;
+      g1 = 0.0f;
+    } else {
+      float al = m0 / delta, be = m1 / delta;
+      if (al < 0.0f || be < 0.0f) {
+        g0 = delta;
+        g1 = delta;
+      } else if (al * al + be * be > 9.0f) {
+        float tau = 3.0f / sqrtf(al * al + be * be);
+        g0 = tau * al * delta;
+        g1 = tau * be * delta;
+      } else {
+        g0 = m0;
+        g1 = m1;
+      }
+    }
+    float t = (x - px[i]) / h, t2 = t * t, t3 = t2 * t;
+    float h00 = 2.0f * t3 - 3.0f * t2 + 1.0f, h10 = t3 - 2.0f * t2 + t;
+    float h01 = -2.0f * t3 + 3.0f * t2, h11 = t3 - t2;
+    return h00 * py[i] + h10 * h * g0 + h01 * py[i + 1] + h11 * h * g1;
+  }
+  return x;
+}
+
+/* ToneCurve::generate_lut (src/scene/tonemap.rs:289-303): texel i holds
+ * f16(curve(i/255)); curve_n 0 is the linear curve. */
+void orc_curve_lut(const ffm_tonemap_params* tp, float* lut) {
+  static const float lin[2] = {0.0f, 1.0f};
+  int n = tp->curve_n ? tp->curve_n : 2;
+  const float* px = tp->curve_n ? tp->curve_x : lin;
+  const float* py = tp->curve_n ? tp->curve_y : lin;
+  for (int i = 0; i < 256; i++) lut[i] = orc_round_f16(orc_curve_eval(px, py, n, (float)i / 255.0f));
+}
+
 void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
                  double samples_in_buffer, const ffm_tonemap_params* tp) {
+  float lut[256];
+  if (tp->use_curve) orc_curve_lut(tp, lut);
   const float BRIGHT_ADJUST = 2.3f, PREFILTER_WHITE = 67108864.0f;
   float apo_zoom = log2f(zoom);
   float ppu = (float)(W < H ? W : H) * 0.25f * powf(2.0f, apo_zoom);
@@ -1295,6 +1374,15 @@ void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
       }
     } else {
       for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
+    }
+    /* the tone curve (tonemap.wgsl:479-492): nearest texel of
+     * u = (255c + 0.5)/256 in the 256-texel table, clamped to the edge */
+    if (tp->use_curve && bucket_count > 0.001f) {
+      for (int k = 0; k < 3; k++) {
+        float u = color[k] * 255.0f + 0.5f;
+        int i = u >= 255.0f ? 255 : (u > 0.0f ? (int)floorf(u) : 0);
+        color[k] = lut[i];
+      }
     }
     /* apply_levels (tonemap.wgsl:125-151) on accum.a, mean = sample_density */
     float leveled = 1.0f;

@@ -92,6 +92,9 @@ def lib():
                                       C.c_float]),
             "orc_density_estimate": (None, [_fp, _fp, C.c_int, C.c_int, C.c_float, C.c_float,
                                             C.c_float]),
+            "orc_round_f16": (C.c_float, [C.c_float]),
+            "orc_curve_eval": (C.c_float, [_fp, _fp, C.c_int, C.c_float]),
+            "orc_curve_lut": (None, [C.c_void_p, _fp]),
             "orc_tonemap": (None, [_fp, _u8p, C.c_int, C.c_int, C.c_float, C.c_double,
                                    C.POINTER(TonemapParams)]),
             "orc_render": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.c_int,
@@ -284,6 +287,23 @@ def accumulate(hist, prev, count=None, mode=0, blend=0.0):
     is_f32 = hist.dtype == np.float32
     lib().orc_accumulate(hist.ctypes.data_as(_vp), int(is_f32), fp(prev), fp(out),
                          u32p(count) if count is not None else None, w, h, mode, blend)
+    return out
+
+
+def round_f16(v):
+    return lib().orc_round_f16(float(v))
+
+
+def curve_eval(points, x):
+    """The tone curve through `points` [(x, y), ...] at x."""
+    px = np.array([p[0] for p in points], dtype=np.float32)
+    py = np.array([p[1] for p in points], dtype=np.float32)
+    return lib().orc_curve_eval(fp(px), fp(py), len(points), float(x))
+
+
+def curve_lut(tp):
+    out = np.zeros(256, dtype=np.float32)
+    lib().orc_curve_lut(C.byref(tp), fp(out))
     return out
 
 

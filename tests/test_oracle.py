@@ -501,6 +501,7 @@ def _simple_linear_job():
     tp = pyfae.tonemap_defaults()
     tp.gamma, tp.gamma_threshold = d["gamma"], d["gamma_threshold"]
     tp.brightness, tp.vibrancy, tp.exposure = d["brightness"], d["vibrancy"], d["exposure"]
+    tp.use_curve = int(d.get("use_curve", True))
     # the CLI export the baseline came from: 800x600, ipt 256, burn-in 20,
     # 128 x 64 threads, deterministic seeds (tests/visual/run_tests.py:273-284)
     job = pyfae.make_job(800, 600, walkers=8192, ipt=256, burn_in=20,
