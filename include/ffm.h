@@ -406,8 +406,9 @@ int ffm_softsplat_adjoint(ffm_ctx* ctx, const float* theta, const float* grad_de
  * src/scene/transforms.rs:1307-1560,2567-2690) and Apophysis / JWildfire
  * XML (parse_flame_xml, src/flame_xml.rs:41-560,600-1067). A config that
  * asks for something this path does not render (a variation outside
- * ffm_variation, xaos, linked transforms, subflames, post symmetry, a
- * non-identity tone curve, effects, escape-time or simulation modes, a 3D
+ * ffm_variation, linked transforms, subflames, post symmetry, a tone
+ * curve beyond FFM_MAX_CURVE_POINTS or with unsorted x, user-written
+ * effects, escape-time or simulation modes, a 3D
  * camera) is refused with FFM_ERR_UNSUPPORTED and a message naming it,
  * never rendered approximately. */
 #define FFM_MAX_STOPS 1024
