@@ -753,6 +753,18 @@ float curve_eval(const float* px, const float* py, int n, float x) {
   return x;
 }
 
+// The tone-map modes and curve a render or ffm_tonemap accepts.
+static int check_tonemap(ffm_ctx* c, const ffm_tonemap_params& tp) {
+  if (tp.tonemap_mode < 0 || tp.tonemap_mode > 2 || tp.highlight_mode < 0 ||
+      tp.highlight_mode > 3 || tp.transparent_mode < 0 || tp.transparent_mode > 2)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode not supported");
+  if (!curve_ok(tp))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                "tone curve needs 2..%d finite points with non-decreasing x",
+                FFM_MAX_CURVE_POINTS);
+  return FFM_OK;
+}
+
 TonemapConst make_tonemap(const ffm_tonemap_params& tp, int W, int H, float zoom,
                           double samples_in_buffer) {
   // update_tonemap (src/renderer/compute_kernel.rs:3813-3918) and
@@ -793,7 +805,9 @@ TonemapConst make_tonemap(const ffm_tonemap_params& tp, int W, int H, float zoom
     const bool lin = tp.curve_n == 0;
     const float* px = lin ? lin_x : tp.curve_x;
     const float* py = lin ? lin_x : tp.curve_y;
-    const int n = lin ? 2 : tp.curve_n;
+    // a caller's curve_n is checked by curve_ok first; the clamp keeps
+    // the reads inside curve_x / curve_y whatever it holds
+    const int n = lin ? 2 : std::min(std::max(tp.curve_n, 1), FFM_MAX_CURVE_POINTS);
     for (int i = 0; i < 256; i++) t.curve_lut[i] = round_f16(curve_eval(px, py, n, (float)i / 255.0f));
   }
   return t;
@@ -1329,6 +1343,9 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   const bool f32h = mode == FFM_SPLAT_F32_GLOBAL || mode == FFM_SPLAT_F32_LDS;
   if (f32h && job->filter_radius > 0.0f)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "the histogram blur reads a u32 histogram");
+  // the tone map's parameters are checked before any work, also when the
+  // render only returns the accumulator
+  if ((st = check_tonemap(c, job->tonemap))) return st;
   (void)hipSetDevice(c->device);
   c->cancel.store(0);
   auto t0 = std::chrono::steady_clock::now();
@@ -1458,16 +1475,7 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   if (rgba_out || rgba_dev) {
     if (!rgba_dev && (st = ensure(c, c->rgba, npix * 4, "pixels"))) return st;
     uint8_t* pix = rgba_dev ? rgba_dev : reinterpret_cast<uint8_t*>(c->rgba.p);
-    const TonemapConst& t = tmc;
-    if (job->tonemap.tonemap_mode < 0 || job->tonemap.tonemap_mode > 2 ||
-        job->tonemap.highlight_mode < 0 || job->tonemap.highlight_mode > 3 ||
-        job->tonemap.transparent_mode < 0 || job->tonemap.transparent_mode > 2)
-      return fail(c, FFM_ERR_INVALID_ARGUMENT, "tonemap mode not supported");
-    if (!curve_ok(job->tonemap))
-      return fail(c, FFM_ERR_INVALID_ARGUMENT,
-                  "tone curve needs 2..%d finite points with non-decreasing x",
-                  FFM_MAX_CURVE_POINTS);
-    HIPCHECK(c, launch_tonemap(final_acc, pix, W, H, t, s));
+    HIPCHECK(c, launch_tonemap(final_acc, pix, W, H, tmc, s));
     if ((st = run_color_chain(c, effects, pix, W, H, s))) return st;
     if (rgba_out) {
       hipError_t e = hipMemcpyAsync(rgba_out, pix, npix * 4, hipMemcpyDeviceToHost, s);

@@ -1354,7 +1354,10 @@ SceneIn xml_flame(XmlReader& rd, const XEvent& start) {
           float ny = b0 * cv[1] + b1 * cv[4] + b2 * cv[7] + b3 * cv[10];
           float den = b0 + b1 + b2 + b3;
           if (std::fabs(den) < 1e-10f) continue;
-          pts.push_back({nx / den, ny / den});
+          // CurvePoint::new clamps both coordinates to [0, 1]
+          // (src/scene/tonemap.rs:134-139)
+          pts.push_back({std::min(std::max(nx / den, 0.0f), 1.0f),
+                         std::min(std::max(ny / den, 0.0f), 1.0f)});
         }
         pts.push_back({1.0f, 1.0f});
         std::stable_sort(pts.begin(), pts.end(),

@@ -269,6 +269,17 @@ __device__ __forceinline__ uint8_t unorm8(float v) {
   return (uint8_t)floorf(v * 255.0f + 0.5f);
 }
 
+// The 256-texel curve table under a linear, clamp-to-edge sampler at
+// texel-space position s = 255c: lut[i] (1 - f) + lut[i + 1] f with
+// i = floor(s), f = s - i; s <= 0 (or NaN) reads texel 0, s >= 255 texel 255.
+__device__ __forceinline__ float curve_sample(const float* lut, float c) {
+  const float s = fminf(fmaxf(c * 255.0f, 0.0f), 255.0f);
+  const int i = (int)s;
+  const float f = s - (float)i;
+  const int j = i < 255 ? i + 1 : 255;
+  return lut[i] * (1.0f - f) + lut[j] * f;
+}
+
 // shaders/tonemap.wgsl:153-211
 __device__ void hsv_adjust(float c[3], float hue_shift, float saturation) {
   float r = c[0], g = c[1], b = c[2];
@@ -390,13 +401,12 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
     for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
   }
   // The tone curve (shaders/tonemap.wgsl:479-492): texel i of the table
-  // holds the curve at i/255 and is read with nearest filtering at
-  // u = (255c + 0.5)/256, i.e. texel floor(255c + 0.5), clamped to the edge.
+  // holds the curve at i/255 and is read with a linear-filtering,
+  // clamp-to-edge sampler (src/gpu/buffers.rs:1877-1886) at
+  // u = (255c + 0.5)/256, whose texel-space position is 255c: the lerp of
+  // texels floor(255c) and the next by the fraction, the edge texel outside.
   if (t.use_curve && bucket_count > 0.001f) {
-    for (int k = 0; k < 3; k++) {
-      const float u = fminf(fmaxf(color[k] * 255.0f + 0.5f, 0.0f), 255.0f);
-      color[k] = t.curve_lut[(int)u];
-    }
+    for (int k = 0; k < 3; k++) color[k] = curve_sample(t.curve_lut, color[k]);
   }
   float leveled = 1.0f;
   if (t.sample_density > 0.0f) {
@@ -468,7 +478,9 @@ hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H, const
 __global__ __launch_bounds__(256) void batch_finish_kernel(const uint4* __restrict__ hist,
                                                            uchar4* __restrict__ rgba,
                                                            const BatchPix* __restrict__ fl) {
-  const BatchPix f = fl[blockIdx.y];
+  // by reference: a copy of the descriptor (its 1 KB curve table indexed by
+  // pixel values) would live in scratch memory
+  const BatchPix& f = fl[blockIdx.y];
   const uint4* h = hist + f.pix_off;
   uchar4* o = rgba + f.pix_off;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < f.npix;

@@ -172,8 +172,10 @@ typedef struct {
   /* The tone curve (ToneCurve, src/scene/tonemap.rs:142-300; applied at
    * shaders/tonemap.wgsl:479-492): with use_curve set, every pixel whose
    * bucket count exceeds 0.001 maps each clipped channel c through a
-   * 256-texel f16 table of the curve at i/255, read at texel
-   * round(255c). The curve is the monotone cubic Hermite through curve_n
+   * 256-texel f16 table of the curve at i/255, read by linear filtering at
+   * texel-space position 255c (the reference's linear clamp-to-edge curve
+   * sampler, src/gpu/buffers.rs:1877-1886). The curve is the monotone
+   * cubic Hermite through curve_n
    * points (2..FFM_MAX_CURVE_POINTS, x non-decreasing, finite);
    * curve_n = 0 with use_curve set is the linear curve (0,0)-(1,1). */
   int32_t use_curve;

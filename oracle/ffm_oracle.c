@@ -1272,6 +1272,19 @@ void orc_curve_lut(const ffm_tonemap_params* tp, float* lut) {
   for (int i = 0; i < 256; i++) lut[i] = orc_round_f16(orc_curve_eval(px, py, n, (float)i / 255.0f));
 }
 
+/* A linear-filtering, clamp-to-edge read of the 256-texel table at
+ * texel-space position s = 255c: lut[i] (1 - f) + lut[i + 1] f for
+ * i = floor(s), f = s - i; s <= 0 or NaN reads texel 0, s >= 255 texel 255.
+ * Written the way a texture unit filters (texel centres at (i + 0.5)/256). */
+float orc_curve_sample(const float* lut, float c) {
+  float s = c * 255.0f;
+  if (!(s > 0.0f)) return lut[0];
+  if (s >= 255.0f) return lut[255];
+  int i = (int)floorf(s);
+  float f = s - (float)i;
+  return lut[i] * (1.0f - f) + lut[i + 1] * f;
+}
+
 void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
                  double samples_in_buffer, const ffm_tonemap_params* tp) {
   float lut[256];
@@ -1375,14 +1388,11 @@ void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
     } else {
       for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
     }
-    /* the tone curve (tonemap.wgsl:479-492): nearest texel of
-     * u = (255c + 0.5)/256 in the 256-texel table, clamped to the edge */
+    /* the tone curve (tonemap.wgsl:479-492): the 256-texel table read at
+     * u = (255c + 0.5)/256 by a linear, clamp-to-edge sampler
+     * (src/gpu/buffers.rs:1877-1886), i.e. at texel-space position 255c */
     if (tp->use_curve && bucket_count > 0.001f) {
-      for (int k = 0; k < 3; k++) {
-        float u = color[k] * 255.0f + 0.5f;
-        int i = u >= 255.0f ? 255 : (u > 0.0f ? (int)floorf(u) : 0);
-        color[k] = lut[i];
-      }
+      for (int k = 0; k < 3; k++) color[k] = orc_curve_sample(lut, color[k]);
     }
     /* apply_levels (tonemap.wgsl:125-151) on accum.a, mean = sample_density */
     float leveled = 1.0f;

@@ -112,6 +112,7 @@ void orc_density_estimate(const float* in, float* out, int W, int H, float max_r
 float orc_round_f16(float v);
 float orc_curve_eval(const float* px, const float* py, int n, float x);
 void orc_curve_lut(const ffm_tonemap_params* tp, float* lut);
+float orc_curve_sample(const float* lut, float c);
 void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
                  double samples_in_buffer, const ffm_tonemap_params* tp);
 void orc_tonemap_defaults(ffm_tonemap_params* tp);

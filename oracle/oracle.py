@@ -95,6 +95,7 @@ def lib():
             "orc_round_f16": (C.c_float, [C.c_float]),
             "orc_curve_eval": (C.c_float, [_fp, _fp, C.c_int, C.c_float]),
             "orc_curve_lut": (None, [C.c_void_p, _fp]),
+            "orc_curve_sample": (C.c_float, [_fp, C.c_float]),
             "orc_tonemap": (None, [_fp, _u8p, C.c_int, C.c_int, C.c_float, C.c_double,
                                    C.POINTER(TonemapParams)]),
             "orc_render": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.c_int,
@@ -305,6 +306,12 @@ def curve_lut(tp):
     out = np.zeros(256, dtype=np.float32)
     lib().orc_curve_lut(C.byref(tp), fp(out))
     return out
+
+
+def curve_sample(lut, c):
+    """The curve table `lut` (256 float32) read by the tone map at channel c."""
+    lut = np.ascontiguousarray(lut, dtype=np.float32)
+    return lib().orc_curve_sample(fp(lut), float(c))
 
 
 def tonemap(acc, zoom, samples, tp):

@@ -140,22 +140,81 @@ def accum(seed=2, H=48, W=64):
     return acc
 
 
-def test_identity_curve_only_quantises():
-    # the identity table moves a channel by at most half of its 1/255 step
-    # (the reference checks the same on its escape path, gamma 1: at most a
-    # level, src/escape/app_repro_test.rs:10465-10507). Here the curve acts
-    # before the 1/2.2 encode, which stretches that half step to a level in
-    # the mid tones and a few in the darkest ones; empty pixels are untouched.
-    acc = accum()
+def test_curve_table_is_read_with_linear_filtering():
+    # the curve sampler filters linearly (src/gpu/buffers.rs:1877-1886) and
+    # the shader reads it at u = (255c + 0.5)/256 (tonemap.wgsl:479-488):
+    # texel-space position 255c, so between texel centres the read is the
+    # straight line through the two texels, and it clamps at both ends. A
+    # nearest read would miss the midpoints by half a texel step.
+    rng = np.random.default_rng(11)
+    lut = np.sort(rng.uniform(0, 1, 256)).astype(np.float32)
+    cs = np.concatenate([rng.uniform(0, 1, 2000), (np.arange(255) + 0.5) / 255.0,
+                         [0.0, 1.0, -0.5, 1.5, np.nan]]).astype(np.float32)
+    for c in cs:
+        s = np.float64(np.float32(c * np.float32(255)))
+        want = np.interp(s, np.arange(256.0), lut.astype(np.float64)) if s == s else lut[0]
+        assert abs(O.curve_sample(lut, c) - want) <= 2e-7, (c, O.curve_sample(lut, c), want)
+
+
+def ramp_accum(H=96, W=128):
+    # a black-to-white ramp of mean colours at a count well past the
+    # curve's 0.001 gate
+    acc = np.zeros((H, W, 4), np.float32)
+    g = np.linspace(0.0, 1.0, H * W, dtype=np.float32).reshape(H, W)
+    acc[..., 0] = g ** np.float32(2.2)  # grey after the 1/2.2 encode
+    acc[..., 1] = g ** 4
+    acc[..., 2] = g
+    acc[..., 3] = 50.0
+    return acc
+
+
+def test_the_identity_tone_curve_changes_nothing():
+    # src/escape/app_repro_test.rs:10455-10507: through a black-to-white
+    # ramp, linear tone map at gamma 1, no levels, black background, the
+    # identity curve on is the curve off to a level, and the ramp's dark end
+    # is in the frame
+    acc = ramp_accum()
     on = _abi.tonemap_defaults()
+    on.tonemap_mode = 0
+    on.gamma = 1.0
+    on.exposure = 1.0
+    on.levels_enabled = 0
+    on.background[:] = [0.0, 0.0, 0.0]
+    on.use_curve = 1
+    on.curve_n = 0
     off = _abi.tonemap_defaults()
+    ctypes_copy(off, on)
     off.use_curve = 0
     a = O.tonemap(acc, 4.0 / 3.0, 5e6, on).astype(int)
     b = O.tonemap(acc, 4.0 / 3.0, 5e6, off).astype(int)
-    d = np.abs(a - b)
-    assert d[b >= 100].max() <= 1 and d.max() <= 4 and d[..., 3].max() == 0
-    empty = acc[..., 3] == 0
-    assert np.array_equal(a[empty], b[empty])
+    assert (b[..., 0] < 40).sum() > 500
+    assert np.abs(a - b)[..., :3].max() <= 1
+    # the log tone map with the default (identity) curve, too: a level at most
+    acc2 = accum()
+    on2 = _abi.tonemap_defaults()
+    off2 = _abi.tonemap_defaults()
+    off2.use_curve = 0
+    d = np.abs(O.tonemap(acc2, 4.0 / 3.0, 5e6, on2).astype(int)
+               - O.tonemap(acc2, 4.0 / 3.0, 5e6, off2).astype(int))
+    assert d.max() <= 1 and d[..., 3].max() == 0
+
+
+def ctypes_copy(dst, src):
+    import ctypes
+    ctypes.memmove(ctypes.addressof(dst), ctypes.addressof(src), ctypes.sizeof(src))
+
+
+def test_identity_curve_nearest_read_would_fail():
+    # the rule above distinguishes the filters: reading the identity table
+    # at the nearest texel moves dark channels by many levels
+    acc = ramp_accum()
+    lut = O.curve_lut(_abi.tonemap_defaults())
+    c = acc[..., 0].ravel()
+    nearest = lut[np.clip(np.floor(c * np.float32(255) + np.float32(0.5)), 0, 255).astype(int)]
+    enc = lambda v: np.floor(np.clip(v, 0, 1) ** (1 / 2.2) * 255 + 0.5)
+    assert np.abs(enc(nearest) - enc(c)).max() > 5
+    linear = np.array([O.curve_sample(lut, v) for v in c[::37]])
+    assert np.abs(enc(linear) - enc(c[::37])).max() <= 1
 
 
 def test_s_curve_darkens_and_brightens():
@@ -220,3 +279,23 @@ def test_apophysis_curves_sample_the_bezier():
         b = np.array([(1 - t) ** 3, 3 * (1 - t) ** 2 * t, 3 * (1 - t) * t ** 2, t ** 3])
         x, y = b @ ctrl  # unit weights: the denominator is 1
         assert abs(tp.curve_x[k + 1] - x) < 1e-6 and abs(tp.curve_y[k + 1] - y) < 1e-6
+
+
+def test_apophysis_curve_points_are_clamped_to_the_unit_square():
+    # CurvePoint::new clamps x and y to [0, 1] (src/scene/tonemap.rs:134-139),
+    # and parse_apophysis_curves builds every sampled point with it
+    # (src/flame_xml.rs:1100-1110): a control point above the square gives
+    # samples at y = 1, not past it
+    cv = "0 0 1 0.25 2.5 1 0.75 2.5 1 1 1 1 " + "0 0 1 0.25 0.25 1 0.75 0.75 1 1 1 1 " * 3
+    xml = ('<flame curves="%s"><xform weight="1" color="0" coefs="1 0 0 1 0 0" linear="1"/>'
+           '</flame>') % cv
+    tp = pyfae.load(xml).tonemap
+    assert tp.curve_n == 5
+    ys = [tp.curve_y[i] for i in range(5)]
+    ctrl = np.array([[0, 0], [0.25, 2.5], [0.75, 2.5], [1, 1]], np.float64)
+    for k, t in enumerate((0.25, 0.5, 0.75)):
+        b = np.array([(1 - t) ** 3, 3 * (1 - t) ** 2 * t, 3 * (1 - t) * t ** 2, t ** 3])
+        x, y = b @ ctrl
+        assert y > 1.0
+        assert ys[k + 1] == 1.0 and abs(tp.curve_x[k + 1] - x) < 1e-6
+    assert max(ys) == 1.0 and min(ys) == 0.0
