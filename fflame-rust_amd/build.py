@@ -19,7 +19,8 @@ OUT_DIR = os.path.join(PKG, "lib")
 LIB = os.path.join(OUT_DIR, "libffm.so")
 OBJ_DIR = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["iterate.hip", "stencils.hip", "tilesplat.hip", "softsplat.hip", "effects.hip"]
+HIP_SOURCES = ["iterate.hip", "stencils.hip", "tilesplat.hip", "softsplat.hip", "effects.hip",
+               "noalloc.hip"]
 HOST_SOURCES = ["runtime.cpp", "scene.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",

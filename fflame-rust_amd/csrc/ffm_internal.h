@@ -152,6 +152,9 @@ hipError_t launch_trace(const IterArgs& a, uint32_t walker, uint32_t steps, uint
 hipError_t launch_fold_xcd(uint32_t* hist, const uint32_t* copies, size_t words, int ncopies,
                            hipStream_t s);
 hipError_t launch_clear_u32(void* p, size_t words, hipStream_t s);
+// The per-device scratch of ffm_iterate_noalloc (noalloc.hip).
+hipError_t noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
 hipError_t launch_accumulate(const void* hist, int hist_is_f32, const float* prev, float* out,
                              uint32_t* count, int W, int H, int mode, float blend,
                              hipStream_t s);

@@ -558,7 +558,34 @@ void collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, double& 
   c->kev_used = 0;
 }
 
-int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = false) {
+// Tile tables and record buffers the caller provides (ffm_iterate_noalloc):
+// nothing is allocated, the plan is made afresh on every call (the memory
+// is shared by every call on the device) and no timing events are recorded.
+struct TileScratch {
+  uint32_t *fill, *region, *cap, *items, *records;
+  uint64_t records_cap;  // u32 words at `records`
+};
+
+// The scratch of ffm_iterate_noalloc laid out: fill, region, cap, items,
+// then the records (and, for bins of several tiles, their sorted copy).
+constexpr size_t NOALLOC_TABLE_WORDS = 3 * (size_t)TILE_MAX + TILE_MAX + 64;
+static_assert(FFM_NOALLOC_SCRATCH_BYTES > NOALLOC_TABLE_WORDS * 4 + (64u << 20),
+              "scratch holds the tables and a useful record buffer");
+
+TileScratch noalloc_layout(void* base) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(base);
+  TileScratch t;
+  t.fill = w;
+  t.region = w + TILE_MAX;
+  t.cap = w + 2 * TILE_MAX;
+  t.items = w + 3 * TILE_MAX;
+  t.records = w + NOALLOC_TABLE_WORDS;
+  t.records_cap = FFM_NOALLOC_SCRATCH_BYTES / 4 - NOALLOC_TABLE_WORDS;
+  return t;
+}
+
+int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = false,
+              const TileScratch* ext = nullptr) {
   const int W = a.view.width, H = a.view.height;
   const int tiles_x = (W + TILE_W - 1) / TILE_W, tiles_y = (H + TILE_H - 1) / TILE_H;
   // bins of 2^g tiles along a row, as few tiles per bin as keep bins <= TILE_MAX
@@ -575,25 +602,43 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
       accum_lds_bytes(a.pal_size, f32) > LDS_PER_CU)
     return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
   const uint64_t per_walker = (uint64_t)a.ipt + 1;
-  uint64_t chunk = std::max<uint64_t>(1, (RECORD_BUDGET * 3 / 4) / per_walker);
+  // records held at once: 2^31 in context buffers; in a caller's scratch
+  // what it holds (half of it for bins of several tiles, whose records
+  // also need a copy sorted by tile)
+  const uint64_t budget =
+      ext ? (g > 0 ? ext->records_cap / 2 : ext->records_cap) : RECORD_BUDGET;
+  if (per_walker > budget * 3 / 4)
+    return fail(c, FFM_ERR_TOO_LARGE,
+                "%u iterations per walker do not fit the %llu-record splat buffer", a.ipt,
+                (unsigned long long)budget);
+  uint64_t chunk = std::max<uint64_t>(1, (budget * 3 / 4) / per_walker);
   chunk = std::min<uint64_t>(chunk, a.walkers);
-  const uint64_t rec_cap = std::min<uint64_t>(RECORD_BUDGET, chunk * per_walker * 5 / 4 +
-                                                             (uint64_t)n_tiles * 4096);
+  const uint64_t rec_cap = std::min<uint64_t>(budget, chunk * per_walker * 5 / 4 +
+                                                      (uint64_t)n_tiles * 4096);
   int st;
-  if ((st = ensure(c, c->t_fill, (size_t)TILE_MAX * 4, "tile fills"))) return st;
-  if ((st = ensure(c, c->t_region, (size_t)TILE_MAX * 4, "tile regions"))) return st;
-  if ((st = ensure(c, c->t_cap, (size_t)TILE_MAX * 4, "tile capacities"))) return st;
-  if ((st = ensure(c, c->t_items, (size_t)(TILE_MAX + 4) * 4, "tile work list"))) return st;
-  if ((st = ensure(c, c->t_records, rec_cap * 4, "tile records"))) return st;
-  if (g > 0 && (st = ensure(c, c->t_sorted, rec_cap * 4, "tile records sorted by tile")))
-    return st;
   TileArgs t{};
-  t.sorted = g > 0 ? reinterpret_cast<uint32_t*>(c->t_sorted.p) : nullptr;
-  t.tile_fill = reinterpret_cast<uint32_t*>(c->t_fill.p);
-  t.region = reinterpret_cast<uint32_t*>(c->t_region.p);
-  t.cap = reinterpret_cast<uint32_t*>(c->t_cap.p);
-  t.item_start = reinterpret_cast<uint32_t*>(c->t_items.p);
-  t.records = reinterpret_cast<uint32_t*>(c->t_records.p);
+  if (ext) {
+    t.tile_fill = ext->fill;
+    t.region = ext->region;
+    t.cap = ext->cap;
+    t.item_start = ext->items;
+    t.records = ext->records;
+    t.sorted = g > 0 ? ext->records + rec_cap : nullptr;
+  } else {
+    if ((st = ensure(c, c->t_fill, (size_t)TILE_MAX * 4, "tile fills"))) return st;
+    if ((st = ensure(c, c->t_region, (size_t)TILE_MAX * 4, "tile regions"))) return st;
+    if ((st = ensure(c, c->t_cap, (size_t)TILE_MAX * 4, "tile capacities"))) return st;
+    if ((st = ensure(c, c->t_items, (size_t)(TILE_MAX + 4) * 4, "tile work list"))) return st;
+    if ((st = ensure(c, c->t_records, rec_cap * 4, "tile records"))) return st;
+    if (g > 0 && (st = ensure(c, c->t_sorted, rec_cap * 4, "tile records sorted by tile")))
+      return st;
+    t.sorted = g > 0 ? reinterpret_cast<uint32_t*>(c->t_sorted.p) : nullptr;
+    t.tile_fill = reinterpret_cast<uint32_t*>(c->t_fill.p);
+    t.region = reinterpret_cast<uint32_t*>(c->t_region.p);
+    t.cap = reinterpret_cast<uint32_t*>(c->t_cap.p);
+    t.item_start = reinterpret_cast<uint32_t*>(c->t_items.p);
+    t.records = reinterpret_cast<uint32_t*>(c->t_records.p);
+  }
   t.hist = hist;
   t.pal_f32 = a.pal_f32;
   t.f32 = f32 ? 1 : 0;
@@ -607,7 +652,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   t.stage = stage;
   t.walk_block = walk_block;
   const int grid = std::max(c->num_cu, 1);
-  const bool same = c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
+  const bool same = !ext && c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
                     c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
                     std::memcmp(&c->t_view, &a.view, sizeof(ViewConst)) == 0;
   if (!same) {
@@ -618,6 +663,8 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
     HIPCHECK(c, launch_bin_iterate(pre, t, true, grid, s));
     HIPCHECK(c, launch_tile_capacity(t, (float)((double)chunk / (double)pre.walkers), 1024,
                                      rec_cap, s));
+  }
+  if (!same && !ext) {
     c->t_planned = true;
     c->t_W = W;
     c->t_H = H;
@@ -633,12 +680,12 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
     ch.walkers = std::min<uint64_t>(chunk, a.walkers - w0);
     if (a.orbits) ch.orbits = a.orbits + w0;
     HIPCHECK(c, launch_clear_u32(t.tile_fill, TILE_MAX, s));
-    HIPCHECK(c, mark(c, s, 0, true));
+    if (!ext) HIPCHECK(c, mark(c, s, 0, true));
     HIPCHECK(c, launch_bin_iterate(ch, t, false, grid, s));
-    HIPCHECK(c, mark(c, s, 0, false));
-    HIPCHECK(c, mark(c, s, 1, true));
+    if (!ext) HIPCHECK(c, mark(c, s, 0, false));
+    if (!ext) HIPCHECK(c, mark(c, s, 1, true));
     HIPCHECK(c, launch_tile_accum(t, a.pal_u32, a.pal_size, grid, s));
-    HIPCHECK(c, mark(c, s, 1, false));
+    if (!ext) HIPCHECK(c, mark(c, s, 1, false));
   }
   return FFM_OK;
 }
@@ -839,6 +886,9 @@ int ffm_ctx_create(int device, ffm_ctx** out) {
     return FFM_ERR_DEVICE;
   }
   c->num_cu = prop.multiProcessorCount;
+  // this device's copy of ffm_iterate_noalloc's scratch (looked up once per
+  // device; a failed lookup is reported by that entry point)
+  (void)noalloc_scratch_init(device);
   *out = c;
   return FFM_OK;
 }
@@ -961,6 +1011,53 @@ int ffm_iterate(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt
   if (splat_mode == FFM_SPLAT_U32_XCD)
     HIPCHECK(c, launch_fold_xcd(reinterpret_cast<uint32_t*>(hist),
                                 reinterpret_cast<const uint32_t*>(c->copies.p), words, XCD_COPIES, s));
+  return FFM_OK;
+}
+
+int ffm_iterate_noalloc(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint32_t ipt,
+                        uint32_t burn_in, uint32_t seed, void* orbits, int fresh, void* hist,
+                        int splat_mode, uint64_t* plotted, uint64_t* inframe, void* stream) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!c->has_flame) return fail(c, FFM_ERR_NO_FLAME, "no flame set");
+  if (!c->has_palette) return fail(c, FFM_ERR_NO_PALETTE, "No palette found");
+  int st = check_view(c, view);
+  if (st) return st;
+  if (!hist) return fail(c, FFM_ERR_INVALID_ARGUMENT, "histogram is NULL");
+  if (walkers == 0 || walkers > 0xFFFFFFFFull)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "walkers %llu outside 1..2^32-1",
+                (unsigned long long)walkers);
+  if (splat_mode != FFM_SPLAT_U32_GLOBAL && splat_mode != FFM_SPLAT_U32_LDS &&
+      splat_mode != FFM_SPLAT_F32_GLOBAL && splat_mode != FFM_SPLAT_F32_LDS)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT,
+                "splat mode %d is not available without allocation (its per-XCD histograms "
+                "exceed the %u-byte scratch)", splat_mode, (unsigned)FFM_NOALLOC_SCRATCH_BYTES);
+  if (orbits && c->flame.has_xaos && burn_in >= (1u << 24))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "xaos orbits keep burn-in below 2^24 (got %u)",
+                burn_in);
+  (void)hipSetDevice(c->device);
+  const hipError_t e = noalloc_scratch_init(c->device);
+  void* base = noalloc_scratch(c->device);
+  if (e != hipSuccess || !base)
+    return fail(c, FFM_ERR_DEVICE, "the device scratch of this entry point is not available: %s",
+                hipGetErrorString(e));
+  hipStream_t s = pick(c, stream);
+  IterArgs a = base_args(c, *view);
+  a.walkers = walkers;
+  a.ipt = ipt;
+  a.burn_in = burn_in;
+  a.seed = seed;
+  a.orbits = reinterpret_cast<float4*>(orbits);
+  a.fresh = fresh;
+  a.refresh_period = orbits ? c->refresh_period : 0u;
+  a.plotted = reinterpret_cast<unsigned long long*>(plotted);
+  a.inframe = reinterpret_cast<unsigned long long*>(inframe);
+  if (splat_mode == FFM_SPLAT_U32_LDS || splat_mode == FFM_SPLAT_F32_LDS) {
+    const TileScratch ts = noalloc_layout(base);
+    return lds_splat(c, a, reinterpret_cast<uint32_t*>(hist), s,
+                     splat_mode == FFM_SPLAT_F32_LDS, &ts);
+  }
+  a.hist = hist;
+  HIPCHECK(c, launch_iterate(a, splat_mode, iterate_grid(c, walkers), 256, s));
   return FFM_OK;
 }
 

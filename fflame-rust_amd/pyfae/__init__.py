@@ -569,6 +569,15 @@ class Engine:
                                          int(bool(fresh)), _ptr(hist), int(splat_mode),
                                          _ptr(plotted), _ptr(inframe), _ptr(stream)))
 
+    def iterate_noalloc(self, view, walkers, ipt, burn_in, seed, hist, *, orbits=None,
+                        fresh=True, splat_mode=2, plotted=None, inframe=None, stream=None):
+        """iterate() through ffm_iterate_noalloc: the splat's scratch is the
+        library's per-GPU device array, nothing is allocated."""
+        self._check(self.lib.ffm_iterate_noalloc(
+            self.ctx, C.byref(view), int(walkers), int(ipt), int(burn_in),
+            int(seed) & 0xFFFFFFFF, _ptr(orbits), int(bool(fresh)), _ptr(hist),
+            int(splat_mode), _ptr(plotted), _ptr(inframe), _ptr(stream)))
+
     def trace(self, view, walker, seed, burn_in, steps, out, stream=None):
         self._check(self.lib.ffm_trace(self.ctx, C.byref(view), int(walker), int(seed) & 0xFFFFFFFF,
                                        int(burn_in), int(steps), _ptr(out), _ptr(stream)))

@@ -247,6 +247,9 @@ SIGNATURES = [
     ("ffm_render_accum", C.c_int, [_vp, C.POINTER(Job), _fp, C.POINTER(RenderStats)]),
     ("ffm_iterate", C.c_int, [_vp, C.POINTER(View), C.c_uint64, C.c_uint32, C.c_uint32,
                               C.c_uint32, _vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp]),
+    ("ffm_iterate_noalloc", C.c_int, [_vp, C.POINTER(View), C.c_uint64, C.c_uint32,
+                                      C.c_uint32, C.c_uint32, _vp, C.c_int, _vp, C.c_int, _vp,
+                                      _vp, _vp]),
     ("ffm_trace", C.c_int, [_vp, C.POINTER(View), C.c_uint32, C.c_uint32, C.c_uint32,
                             C.c_uint32, _vp, _vp]),
     ("ffm_blur", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_float, _vp]),

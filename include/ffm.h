@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 7
+#define FFM_ABI_VERSION 8
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -313,6 +313,25 @@ int ffm_accumulate(ffm_ctx* ctx, const void* hist, int hist_is_f32, const float*
 int ffm_tonemap(ffm_ctx* ctx, const float* accum, uint8_t* rgba, int width, int height,
                 float zoom, double samples_in_buffer, const ffm_tonemap_params* tp,
                 void* stream);
+
+/* ffm_iterate without allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc). The same dispatch as
+ * ffm_iterate for splat modes U32_GLOBAL, U32_LDS, F32_GLOBAL and F32_LDS
+ * (U32_XCD's per-XCD histograms do not fit; refused), with the LDS tile
+ * splat's bin tables and staged records in a device-resident scratch of
+ * FFM_NOALLOC_SCRATCH_BYTES per GPU instead of context buffers: walkers run
+ * in chunks whose records fit it, the bin plan is made afresh each call, no
+ * timing events are recorded, and nothing is allocated or freed. Refused
+ * with FFM_ERR_TOO_LARGE when one walker's ipt + 1 records exceed three
+ * quarters of the record space (the whole scratch for single-tile bins,
+ * half of it for images past 2048 tiles). Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU race, so issue them one at a
+ * time per device. */
+#define FFM_NOALLOC_SCRATCH_BYTES (256u << 20)
+int ffm_iterate_noalloc(ffm_ctx* ctx, const ffm_view* view, uint64_t walkers, uint32_t ipt,
+                        uint32_t burn_in, uint32_t seed, void* orbits, int fresh, void* hist,
+                        int splat_mode, uint64_t* plotted, uint64_t* inframe, void* stream);
 
 /* Fold the per-XCD private histograms of FFM_SPLAT_U32_XCD into `hist`. */
 int ffm_fold_xcd(ffm_ctx* ctx, uint32_t* hist, int width, int height, void* stream);
