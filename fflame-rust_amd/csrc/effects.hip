@@ -92,37 +92,12 @@ __device__ __forceinline__ float lum3(float r, float g, float b) {
 
 __device__ __forceinline__ float length2(float x, float y) { return sqrtf(x * x + y * y); }
 
-// sin and cos of the colour effects: Cody-Waite reduction and the Cephes
-// polynomials (ffm_fastmath.h), a fixed method rather than the platform's
-// (whose last bits the reference leaves to the GPU), so the hash-based
-// effects' fract(sin(n) * 43758.5453) is reproducible.
-// Beyond |x| = 32768 the reduction runs in f64 (two-part pi/2, fused
-// multiply-adds) and the same polynomials finish: no library call, so the
-// bits are the same wherever f64 is IEEE.
-__device__ void fx_sincos(float x, float* s, float* c) {
-  if (fabsf(x) <= 32768.0f) {
-    ffm_sincos_reduced(x, s, c);
-    return;
-  }
-  if (!(fabsf(x) <= 3.4028235e38f)) {
-    *s = *c = __int_as_float(0x7fc00000);
-    return;
-  }
-  const double xd = (double)x, k = rint(xd * 0.63661977236758134);
-  double rd = fma(-k, 1.5707963267948966, xd);
-  rd = fma(-k, 6.123233995736766e-17, rd);
-  const float r = (float)rd, r2 = r * r;
-  const float sp = fmaf(fmaf(fmaf(-1.9515295891e-4f, r2, 8.3321608736e-3f), r2,
-                             -1.6666654611e-1f),
-                        r2 * r, r);
-  const float cp = fmaf(fmaf(fmaf(2.443315711809948e-5f, r2, -1.388731625493765e-3f), r2,
-                             4.166664568298827e-2f),
-                        r2 * r2, fmaf(-0.5f, r2, 1.0f));
-  const int q = (int)fmod(k, 4.0) & 3;
-  const float ss = (q & 1) ? cp : sp, cc = (q & 1) ? sp : cp;
-  *s = (q & 2) ? -ss : ss;
-  *c = ((q + 1) & 2) ? -cc : cc;
-}
+// sin and cos of the colour effects: the walker's fixed method
+// (ffm_fastmath.h: Cody-Waite reduction and the Cephes polynomials, an f64
+// reduction past |x| = 32768), not the platform's (whose last bits the
+// reference leaves to the GPU), so the hash-based effects'
+// fract(sin(n) * 43758.5453) is reproducible.
+__device__ __forceinline__ void fx_sincos(float x, float* s, float* c) { ffm_sincosf(x, s, c); }
 __device__ __forceinline__ float fx_sin(float x) {
   float s, c;
   fx_sincos(x, &s, &c);

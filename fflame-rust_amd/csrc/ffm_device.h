@@ -180,7 +180,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     }
     case FFM_VAR_HORSESHOE: {
       FFM_NO_SPECULATE();
-      float r = ffm_sqrt(x * x + y * y) + 1e-6f;
+      float r = ffm_sqrt_cr(x * x + y * y) + 1e-6f;  // its rounding reaches both outputs
       float r_inv = ffm_rcp(r);
       ox = (x - y) * (x + y) * r_inv;
       oy = 2.0f * x * y * r_inv;
@@ -197,7 +197,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     case FFM_VAR_DISC: {
       FFM_NO_SPECULATE();
       float theta = ffm_atan2f(x, y);
-      float r = ffm_sqrt(x * x + y * y);
+      float r = ffm_sqrt_cr(x * x + y * y);  // feeds the angle pi r
       float theta_pi = ffm_fdiv(theta, PI_F);
       float pi_r = PI_F * r;
       float s, c;

@@ -1,19 +1,31 @@
-// ffm_fastmath.h -- single-precision sin/cos for the walker's hot path.
+// ffm_fastmath.h -- single-precision sin/cos/atan2 for the walker's hot path.
 //
-// Method: Cody-Waite reduction by pi/2 with a three-part constant
-// (1.5703125 + 4.837512969970703125e-4 + 7.54978995489188216e-8, as in
-// Moshier's Cephes sinf/cosf, applied with fused multiply-adds), then the
-// Cephes minimax polynomials on [-pi/4, pi/4]:
+// sin/cos: Cody-Waite reduction by pi/2 with the constant split into three
+// floats (pi/2 rounded to float, then the rounded remainders; 72 bits in
+// all), applied with fused multiply-adds: for |x| <= 32768 the first
+// product-difference is exact (its result lies on x's grid and is below 1),
+// so the reduced argument carries two roundings and the split's 2^-76
+// error. Then the Cephes minimax polynomials on [-pi/4, pi/4]:
 //   sin r = r + r^3 (-1.6666654611e-1 + r^2 (8.3321608736e-3 - r^2 1.9515295891e-4))
 //   cos r = 1 - r^2/2 + r^4 (4.166664568298827e-2 + r^2 (-1.388731625493765e-3
 //                                                  + r^2 2.443315711809948e-5))
-// and the quadrant k mod 4 selects sign and function. C1 has 9 significant
-// bits, so k*C1 is exact for |k| < 2^15: the fast path serves |x| <= 32768;
-// larger arguments (rare on the chaos game's attractors) take the library's
-// full-range sincosf in a separate non-inlined function, keeping its
-// Payne-Hanek reduction out of the hot loop's registers.
+// and the quadrant k mod 4 selects sign and function. Larger arguments
+// (rare on the chaos game's attractors) are reduced in f64 in a separate
+// non-inlined function, then take the same polynomials.
 //
-// The same text compiles as C (for a host accuracy check) and as HIP.
+// atan2: a = min(|x|,|y|) / max(|x|,|y|) in [0, 1] with the quotient's
+// residual a_lo = (min - a max) / max, then one odd polynomial on the whole
+// of [0, 1]: atan a = a + a z P(z), z = a^2, P of degree 7 fitted here for
+// the least maximum relative error of atan (2.8e-1 ulp before rounding),
+// plus a_lo / (1 + z) (taken as a_lo (1 - z/2): the residual is at most
+// half an ulp of a, so the factor's 12% error is invisible); octant
+// fix-ups with two-part pi/2 and pi.
+//
+// Measured on the host build (the same text as C, IEEE divisions) against
+// double-precision libm: sin and cos within 1.55 ulp for |x| <= 32768,
+// atan2 within 1.53 ulp (tests/test_fastmath.py).
+//
+// The same text compiles as C (for the host accuracy check) and as HIP.
 #pragma once
 
 #ifdef __HIPCC__
@@ -32,8 +44,14 @@
 // rcp instead of the division expansion's scaling/fixup sequence; for
 // operands away from the exponent extremes (|b| within 2^+-126, where the
 // walk lives) the quotient is within 0.5 ulp plus the residual's rounding.
-// The square root is the hardware's 1-ulp v_sqrt_f32 (WGSL's own sqrt bound
-// is 2.5 ulp). The host build (accuracy checks) uses the IEEE operations.
+// Square roots: ffm_sqrt is the hardware's 1-ulp v_sqrt_f32 (WGSL's own
+// sqrt bound is 2.5 ulp), where a root scales one output; ffm_sqrt_cr is
+// correctly rounded (the compiler's expansion: v_sqrt_f32 and a residual
+// test of its neighbours), as the IEEE oracle's, where a root's last bit
+// is amplified: disc's pi r is rounded at magnitude up to 4, so a 1-ulp root
+// there moves the angle by a whole ulp of 4; horseshoe divides both of its
+// outputs by r, and a step summing it measured 4 ulp with the fast root.
+// The host build (accuracy checks) uses the IEEE operations.
 FFM_FM_INLINE float ffm_rcp(float b) {
 #ifdef __HIPCC__
   const float r = __builtin_amdgcn_rcpf(b);
@@ -61,20 +79,11 @@ FFM_FM_INLINE float ffm_sqrt(float x) {
 #endif
 }
 
-FFM_FM_NOINLINE void ffm_sincos_wide(float x, float* s, float* c) {
-#ifdef __HIPCC__
-  sincosf(x, s, c);
-#else
-  *s = sinf(x);
-  *c = cosf(x);
-#endif
-}
+FFM_FM_INLINE float ffm_sqrt_cr(float x) { return sqrtf(x); }
 
-FFM_FM_INLINE void ffm_sincos_reduced(float x, float* s, float* c) {
-  const float k = rintf(x * 0.636619772367581343f);
-  float r = fmaf(-k, 1.5703125f, x);
-  r = fmaf(-k, 4.837512969970703125e-4f, r);
-  r = fmaf(-k, 7.54978995489188216e-8f, r);
+// Sign and function by the quadrant k mod 4 from the Cephes polynomials of
+// the reduced argument r in [-pi/4, pi/4].
+FFM_FM_INLINE void ffm_sincos_poly(float r, int q, float* s, float* c) {
   const float r2 = r * r;
   const float sp = fmaf(fmaf(fmaf(-1.9515295891e-4f, r2, 8.3321608736e-3f), r2,
                              -1.6666654611e-1f),
@@ -82,11 +91,33 @@ FFM_FM_INLINE void ffm_sincos_reduced(float x, float* s, float* c) {
   const float cp = fmaf(fmaf(fmaf(2.443315711809948e-5f, r2, -1.388731625493765e-3f), r2,
                              4.166664568298827e-2f),
                         r2 * r2, fmaf(-0.5f, r2, 1.0f));
-  const int q = (int)k & 3;
   const float ss = (q & 1) ? cp : sp;
   const float cc = (q & 1) ? sp : cp;
   *s = (q & 2) ? -ss : ss;
   *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// Past |x| = 32768 the reduction runs in f64 (two-part pi/2, fused
+// multiply-adds) before the same polynomials: no library call, so device
+// and host give the same bits; infinities and NaN give NaN. Kept out of
+// line: such arguments are rare on the chaos game's attractors.
+FFM_FM_NOINLINE void ffm_sincos_wide(float x, float* s, float* c) {
+  if (!(fabsf(x) <= 3.4028235e38f)) {
+    *s = *c = x - x;
+    return;
+  }
+  const double xd = (double)x, k = rint(xd * 0.63661977236758134);
+  double rd = fma(-k, 1.5707963267948966, xd);
+  rd = fma(-k, 6.123233995736766e-17, rd);
+  ffm_sincos_poly((float)rd, (int)fmod(k, 4.0) & 3, s, c);
+}
+
+FFM_FM_INLINE void ffm_sincos_reduced(float x, float* s, float* c) {
+  const float k = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-k, 1.57079637050628662109375f, x);
+  r = fmaf(-k, -4.37113882867379300e-8f, r);
+  r = fmaf(-k, -1.71512451000588190e-15f, r);
+  ffm_sincos_poly(r, (int)k & 3, s, c);
 }
 
 FFM_FM_INLINE void ffm_sincosf(float x, float* s, float* c) {
@@ -97,29 +128,33 @@ FFM_FM_INLINE void ffm_sincosf(float x, float* s, float* c) {
   }
 }
 
-// atan2 by reduction to a = min(|x|,|y|) / max(|x|,|y|) in [0, 1] (1.5-ulp
-// quotients), then
-// a -> (a - 1) / (a + 1) above tan(pi/8) (adding pi/4), and Cephes atanf's
-// minimax polynomial on [-tan(pi/8), tan(pi/8)]:
-//   atan z = z + z^3 (-3.33329491539e-1 + z^2 (1.99777106478e-1
-//                     + z^2 (-1.38776856032e-1 + z^2 8.05374449538e-2)))
-// Octant fix-ups use the operands' sign bits, so the signed-zero cases
-// follow IEEE atan2: atan2(+-0, +0) = +-0, atan2(+-0, -0) = +-pi.
 FFM_FM_INLINE float ffm_atan2f(float y, float x) {
-  // pi/4, pi/2, pi as float + remainder (two-part constants)
-  const float PIO4_HI = 0.785398185253143310546875f, PIO4_LO = -2.18556941e-8f;
+  // pi/2 and pi as float + remainder (two-part constants)
   const float PIO2_HI = 1.57079637050628662109375f, PIO2_LO = -4.37113883e-8f;
   const float PI_HI = 3.1415927410125732421875f, PI_LO = -8.74227766e-8f;
   const float ax = fabsf(x), ay = fabsf(y);
   const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-  float a = mx > 0.0f ? ffm_fdiv(mn, mx) : 0.0f;
-  const int reduced = a > 0.41421356237309503f;
-  if (reduced) a = ffm_fdiv(a - 1.0f, a + 1.0f);
+  float a = 0.0f, a_lo = 0.0f;
+  if (mx > 0.0f) {
+#ifdef __HIPCC__
+    const float rc = ffm_rcp(mx);
+    const float q = mn * rc;
+    a = fmaf(rc, fmaf(-mx, q, mn), q);
+    a_lo = fmaf(-mx, a, mn) * rc;
+#else
+    a = mn / mx;
+    a_lo = fmaf(-mx, a, mn) / mx;
+#endif
+  }
   const float z = a * a;
-  const float p = fmaf(fmaf(fmaf(8.05374449538e-2f, z, -1.38776856032e-1f), z, 1.99777106478e-1f),
-                       z, -3.33329491539e-1f);
-  const float q = p * z * a;  // atan(a) = a + q
-  float t = reduced ? PIO4_HI + (a + (q + PIO4_LO)) : a + q;
+  float p = fmaf(2.920681611e-03f, z, -1.636788808e-02f);
+  p = fmaf(p, z, 4.321179539e-02f);
+  p = fmaf(p, z, -7.552208751e-02f);
+  p = fmaf(p, z, 1.066600233e-01f);
+  p = fmaf(p, z, -1.421105415e-01f);
+  p = fmaf(p, z, 1.999377310e-01f);
+  p = fmaf(p, z, -3.333315253e-01f);
+  float t = a + fmaf(a * z, p, a_lo * fmaf(-0.5f, z, 1.0f));
   if (ay > ax) t = (PIO2_HI - t) + PIO2_LO;
 #ifdef __HIPCC__
   const bool xneg = (__float_as_uint(x) & 0x80000000u) != 0u;

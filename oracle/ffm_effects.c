@@ -77,8 +77,9 @@ static float smooth(float a, float b, float x) {
 static float lum3(float r, float g, float b) { return r * 0.2126f + g * 0.7152f + b * 0.0722f; }
 
 /* sin and cos for the colour effects, by the method the kernels use:
- * Cody-Waite reduction by pi/2 (three-part constant 1.5703125 +
- * 4.837512969970703125e-4 + 7.54978995489188216e-8, with fused
+ * Cody-Waite reduction by pi/2 (pi/2 split into three floats: its float
+ * rounding 1.57079637050628662109375 and the rounded remainders
+ * -4.371138828673793e-8 and -1.7151245100058819e-15, with fused
  * multiply-adds) and the Cephes sinf/cosf minimax polynomials on
  * [-pi/4, pi/4]; beyond |x| = 32768 the reduction in f64 (two-part pi/2)
  * before the same polynomials. Pinned against the mathematical sin/cos
@@ -88,9 +89,9 @@ void orc_fx_sincos(float x, float* s, float* c) {
   int q;
   if (fabsf(x) <= 32768.0f) {
     const float k = rintf(x * 0.636619772367581343f);
-    r = fmaf(-k, 1.5703125f, x);
-    r = fmaf(-k, 4.837512969970703125e-4f, r);
-    r = fmaf(-k, 7.54978995489188216e-8f, r);
+    r = fmaf(-k, 1.57079637050628662109375f, x);
+    r = fmaf(-k, -4.37113882867379300e-8f, r);
+    r = fmaf(-k, -1.71512451000588190e-15f, r);
     q = (int)k & 3;
   } else if (!(fabsf(x) <= 3.4028235e38f)) {
     *s = *c = NAN;

@@ -7,9 +7,11 @@
  *
  * Build: -O2 -ffp-contract=off (no FMA contraction, no fast-math), so each
  * FP32 operation rounds once, in the order the reference's WGSL writes it.
- * Transcendentals are the host libm's (sinf, cosf, atan2f, powf, expf,
- * logf); the reference's are whatever the GPU driver compiles, which no
- * reference test pins.
+ * Transcendentals: the reference's are whatever the GPU driver compiles,
+ * which no reference test pins. The variation bodies' sin, cos and atan2
+ * are a fixed method (below, at orc_use_libm), itself pinned against
+ * double-precision libm; everything else is the host libm's (powf, expf,
+ * logf, and sinf/cosf/atan2f under orc_use_libm(1)).
  *
  * Pinning (what checks each function against something other than itself):
  *  - RNG: published PCG-RXS-M-XS constants plus an independent numpy
@@ -18,7 +20,8 @@
  *    OrbitFuses: the reference's known-answer tests
  *    (src/renderer/compute_kernel.rs:6056-6144,6404-6508,6997-7120) and
  *    brute-force/closed-form checks.
- *  - variations: closed forms evaluated independently in FP64 (tests).
+ *  - variations: closed forms evaluated independently in FP64 (tests);
+ *    their sin/cos/atan2 method: a sweep against double-precision libm.
  *  - tonemap: the reference's own rendered baseline
  *    tests/visual/baseline/2d-simple-linear.png (golden fixture) through
  *    the full render.
@@ -161,6 +164,7 @@ static const float PI_F = F32(3.14159265359);  /* the WGSL literal */
 
 /* shaders/core/utilities.wgsl:604-611 ff_atan2: signed-zero origin handled
  * by sign bits, else atan2. */
+static float v_atan2(float y, float x);
 static float ff_atan2(float y, float x) {
   if (y == 0.0f && x == 0.0f) {
     const float pi = F32(3.14159265358979);
@@ -170,10 +174,60 @@ static float ff_atan2(float y, float x) {
     float mag = (bx & 0x80000000u) ? pi : 0.0f;
     return (by & 0x80000000u) ? -mag : mag;
   }
-  return atan2f(y, x);
+  return v_atan2(y, x);
 }
 
 static float len2(float x, float y) { return x * x + y * y; }
+
+/* sin, cos and atan2 of the variation bodies. WGSL leaves their last bits
+ * to the platform (the reference notes vendor differences,
+ * src/probe/mod.rs:17-34), so the oracle fixes a method: the device's
+ * (ffm_fastmath.h; restated here, and both pinned against double-precision
+ * libm by tests/test_fastmath.py): Cody-Waite reduction by a three-float
+ * pi/2 and the Cephes polynomials for sin/cos (orc_fx_sincos, which the
+ * colour effects use too), and for atan2 the quotient min/max with its
+ * residual and one degree-7 odd polynomial on [0, 1]. orc_use_libm(1)
+ * switches the bodies to the host libm's sinf/cosf/atan2f instead. */
+static int g_libm;
+void orc_use_libm(int on) { g_libm = on; }
+
+float orc_fm_atan2(float y, float x) {
+  const float PIO2_HI = 1.57079637050628662109375f, PIO2_LO = -4.37113883e-8f;
+  const float PI_HI = 3.1415927410125732421875f, PI_LO = -8.74227766e-8f;
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  float a = 0.0f, a_lo = 0.0f;
+  if (mx > 0.0f) {
+    a = mn / mx;
+    a_lo = fmaf(-mx, a, mn) / mx;
+  }
+  const float z = a * a;
+  float p = fmaf(2.920681611e-03f, z, -1.636788808e-02f);
+  p = fmaf(p, z, 4.321179539e-02f);
+  p = fmaf(p, z, -7.552208751e-02f);
+  p = fmaf(p, z, 1.066600233e-01f);
+  p = fmaf(p, z, -1.421105415e-01f);
+  p = fmaf(p, z, 1.999377310e-01f);
+  p = fmaf(p, z, -3.333315253e-01f);
+  float t = a + fmaf(a * z, p, a_lo * fmaf(-0.5f, z, 1.0f));
+  if (ay > ax) t = (PIO2_HI - t) + PIO2_LO;
+  if (signbit(x)) t = (PI_HI - t) + PI_LO;
+  return signbit(y) ? -t : t;
+}
+
+static float v_sin(float x) {
+  if (g_libm) return sinf(x);
+  float s, c;
+  orc_fx_sincos(x, &s, &c);
+  return s;
+}
+static float v_cos(float x) {
+  if (g_libm) return cosf(x);
+  float s, c;
+  orc_fx_sincos(x, &s, &c);
+  return c;
+}
+static float v_atan2(float y, float x) { return g_libm ? atan2f(y, x) : orc_fm_atan2(y, x); }
 
 /* The 13 bodies. Sources: src/variations/defs/basic.rs (linear :37,
  * sinusoidal :100, spherical :130, swirl :162, horseshoe :198) and
@@ -187,8 +241,8 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
       *oy = y;
       return;
     case FFM_VAR_SINUSOIDAL:
-      *ox = sinf(x);
-      *oy = sinf(y);
+      *ox = v_sin(x);
+      *oy = v_sin(y);
       return;
     case FFM_VAR_SPHERICAL: {
       float r2 = len2(x, y) + 1e-6f;
@@ -198,7 +252,7 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
     }
     case FFM_VAR_SWIRL: {
       float r2 = len2(x, y);
-      float s = sinf(r2), c = cosf(r2);
+      float s = v_sin(r2), c = v_cos(r2);
       *ox = x * s - y * c;
       *oy = x * c + y * s;
       return;
@@ -211,19 +265,19 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
       return;
     }
     case FFM_VAR_POLAR: {
-      float theta = atan2f(x, y);
+      float theta = v_atan2(x, y);
       float r = sqrtf(len2(x, y));
       *ox = theta / PI_F;
       *oy = r - 1.0f;
       return;
     }
     case FFM_VAR_DISC: {
-      float theta = atan2f(x, y);
+      float theta = v_atan2(x, y);
       float r = sqrtf(len2(x, y));
       float theta_pi = theta / PI_F;
       float pi_r = PI_F * r;
-      *ox = theta_pi * sinf(pi_r);
-      *oy = theta_pi * cosf(pi_r);
+      *ox = theta_pi * v_sin(pi_r);
+      *oy = theta_pi * v_cos(pi_r);
       return;
     }
     case FFM_VAR_HYPERBOLIC: {
@@ -234,17 +288,17 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
     }
     case FFM_VAR_JULIA: {
       float r = sqrtf(len2(x, y));
-      float theta = atan2f(y, x);
+      float theta = v_atan2(y, x);
       float sqrt_r = sqrtf(r);
       float omega = (orc_rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
       float half_theta = theta / 2.0f + omega;
-      *ox = sqrt_r * cosf(half_theta);
-      *oy = sqrt_r * sinf(half_theta);
+      *ox = sqrt_r * v_cos(half_theta);
+      *oy = sqrt_r * v_sin(half_theta);
       return;
     }
     case FFM_VAR_WAVES:
-      *ox = x + xf->b * sinf(y / (xf->e * xf->e + 1e-6f));
-      *oy = y + xf->d * sinf(x / (xf->f * xf->f + 1e-6f));
+      *ox = x + xf->b * v_sin(y / (xf->e * xf->e + 1e-6f));
+      *oy = y + xf->d * v_sin(x / (xf->f * xf->f + 1e-6f));
       return;
     case FFM_VAR_JULIAN: {
       float power = p[0];
@@ -254,8 +308,8 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
       float theta = ff_atan2(y, x);
       float trunc_val = floorf(fabsf(power) * orc_rng_nextf(rng));
       float t = (theta + F32(6.28318530718) * trunc_val) / power;
-      *ox = r * cosf(t);
-      *oy = r * sinf(t);
+      *ox = r * v_cos(t);
+      *oy = r * v_sin(t);
       return;
     }
     case FFM_VAR_BUBBLE: {
@@ -280,24 +334,24 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
       float high = p[0], low = p[1], waves = p[2];
       float r = sqrtf(len2(x, y));
       float theta = ff_atan2(x, y);
-      float scale = r * (low + ((high - low) / 2.0f) * (sinf(waves * theta) + 1.0f));
-      *ox = scale * cosf(theta);
-      *oy = scale * sinf(theta);
+      float scale = r * (low + ((high - low) / 2.0f) * (v_sin(waves * theta) + 1.0f));
+      *ox = scale * v_cos(theta);
+      *oy = scale * v_sin(theta);
       return;
     }
     case FFM_VAR_NOISE: {
       /* WGSL evaluates the draws left to right: theta first, then r */
       float theta = orc_rng_nextf(rng) * F32(6.28318530718);
       float r = orc_rng_nextf(rng);
-      *ox = x * r * cosf(theta);
-      *oy = y * r * sinf(theta);
+      *ox = x * r * v_cos(theta);
+      *oy = y * r * v_sin(theta);
       return;
     }
     case FFM_VAR_BLUR: {
       float theta = orc_rng_nextf(rng) * F32(6.28318530718);
       float r = orc_rng_nextf(rng);
-      *ox = r * cosf(theta);
-      *oy = r * sinf(theta);
+      *ox = r * v_cos(theta);
+      *oy = r * v_sin(theta);
       return;
     }
     case FFM_VAR_GAUSSIAN_BLUR: {
@@ -307,8 +361,8 @@ void orc_variation(int var_id, const float* p, const ffm_xform* xf, float x, flo
       float d3 = orc_rng_nextf(rng);
       float d4 = orc_rng_nextf(rng);
       float r = d1 + d2 + d3 + d4 - 2.0f;
-      *ox = r * cosf(theta);
-      *oy = r * sinf(theta);
+      *ox = r * v_cos(theta);
+      *oy = r * v_sin(theta);
       return;
     }
     default:
@@ -331,6 +385,17 @@ void orc_flame_prepare(ffm_flame_desc* f) {
   }
 }
 
+/* The largest magnitude among the terms one step sums (the walker's point,
+ * the affine point, each weighted variation output, the post-affine
+ * products), recorded while orc_step_from runs: the scale at which the
+ * step's FP32 sums round, for per-step error bars. */
+static _Thread_local int g_track;
+static _Thread_local float g_term;
+static void track(float v) {
+  if (g_track && fabsf(v) > g_term) g_term = fabsf(v);
+}
+float orc_step_term_scale(void) { return g_term; }
+
 /* affine (shaders/core/affine.wgsl:4-19) -> variations in local-index
  * order, weight-gated (src/shader_builder_v2.rs:2525-2806, normal phase
  * `result += w * V(temp)`) -> post-affine if enabled
@@ -341,6 +406,14 @@ void orc_apply_xform(const ffm_flame_desc* f, const ffm_xform* xf, float* px, fl
   float ax = xf->a * x + xf->b * y + xf->e;
   float ay = xf->c * x + xf->d * y + xf->f;
   float rx = 0.0f, ry = 0.0f;
+  track(x);
+  track(y);
+  track(xf->a * x);
+  track(xf->b * y);
+  track(xf->c * x);
+  track(xf->d * y);
+  track(ax);
+  track(ay);
   /* phase 1, pre variations in local-index order, each replacing the point
    * the normal phase reads (src/shader_builder_v2.rs:2596-2650): pre_blur
    * (src/variations/defs/blur.rs PRE_BLUR) adds a Gaussian-ish offset of
@@ -352,8 +425,10 @@ void orc_apply_xform(const ffm_flame_desc* f, const ffm_xform* xf, float* px, fl
     float g4 = orc_rng_nextf(rng), g5 = orc_rng_nextf(rng), g6 = orc_rng_nextf(rng);
     float rnd_g = w * (g1 + g2 + g3 + g4 + g5 + g6 - 3.0f);
     float rnd_a = orc_rng_nextf(rng) * F32(6.28318530717959);
-    ax = ax + rnd_g * cosf(rnd_a);
-    ay = ay + rnd_g * sinf(rnd_a);
+    ax = ax + rnd_g * v_cos(rnd_a);
+    ay = ay + rnd_g * v_sin(rnd_a);
+    track(ax);
+    track(ay);
   }
   /* phase 3, normal variations */
   for (int k = 0; k < f->n_vars; k++) {
@@ -364,14 +439,26 @@ void orc_apply_xform(const ffm_flame_desc* f, const ffm_xform* xf, float* px, fl
       orc_variation(f->var_id[k], xf->var_param[k], xf, ax, ay, rng, &vx, &vy);
       rx += w * vx;
       ry += w * vy;
+      track(w * vx);
+      track(w * vy);
+      track(rx);
+      track(ry);
     }
   }
   if (xf->post_enabled) {
     float qx = xf->post_a * rx + xf->post_b * ry + xf->post_e;
     float qy = xf->post_c * rx + xf->post_d * ry + xf->post_f;
+    track(xf->post_a * rx);
+    track(xf->post_b * ry);
+    track(xf->post_c * rx);
+    track(xf->post_d * ry);
+    track(xf->post_e);
+    track(xf->post_f);
     rx = qx;
     ry = qy;
   }
+  track(rx);
+  track(ry);
   *px = rx;
   *py = ry;
 }
@@ -677,6 +764,7 @@ static int step(const step_ctx* c, orc_walker* w, orc_rng* rng, int32_t* ppx, in
   /* final chain (main_template.wgsl:663-717): reshapes the plot only; the
    * finals this transform attaches (its AttachmentList,
    * shaders/core/header.wgsl:173-178), or every final in order */
+  g_track = 0; /* the final chain reshapes the plot, not the walker */
   float fx = x, fy = y;
   int nc = f->per_transform_finals ? xf->n_final_attach : f->n_finals;
   for (int j = 0; j < nc; j++) {
@@ -853,7 +941,10 @@ int orc_step_from(const ffm_flame_desc* f, const float* pal, int pal_size, const
   int32_t px, py;
   int xi;
   uint32_t rgb[3];
+  g_track = 1;
+  g_term = 0.0f;
   int st = step(&c, w, rng, &px, &py, &xi, rgb);
+  g_track = 0;
   o[0] = rng->state;
   o[1] = (uint32_t)xi;
   memcpy(&o[2], &w->x, 4);

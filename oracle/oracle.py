@@ -96,6 +96,9 @@ def lib():
             "orc_curve_eval": (C.c_float, [_fp, _fp, C.c_int, C.c_float]),
             "orc_curve_lut": (None, [C.c_void_p, _fp]),
             "orc_curve_sample": (C.c_float, [_fp, C.c_float]),
+            "orc_step_term_scale": (C.c_float, []),
+            "orc_use_libm": (None, [C.c_int]),
+            "orc_fm_atan2": (C.c_float, [C.c_float, C.c_float]),
             "orc_tonemap": (None, [_fp, _u8p, C.c_int, C.c_int, C.c_float, C.c_double,
                                    C.POINTER(TonemapParams)]),
             "orc_render": (C.c_int, [C.POINTER(FlameDesc), _fp, C.c_int, C.c_int,
@@ -193,10 +196,14 @@ def trace(fd, stops, view, walker, seed, burn_in, steps, pal_size=256):
     return out
 
 
-def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=256):
+def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=256,
+                 with_scale=False):
     """Apply one oracle step to each state of a (device) trace: row t of the
     result is the oracle's step from the trace's state after step t-1 (the
-    walker's start state for t = 0). Returns an (n, 8) record array."""
+    walker's start state for t = 0). Returns an (n, 8) record array; with
+    `with_scale`, also each step's term scale (the largest magnitude among
+    the terms the step sums: the point, the affine point, the weighted
+    variation outputs, the post-affine products)."""
     f2 = prepared(fd)
     pal = palette_linear(stops, pal_size)
     L = lib()
@@ -214,10 +221,12 @@ def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=25
     fuse = burn_in
     out = np.zeros((len(trace_rows), 8), dtype=np.uint32)
     rec = (C.c_uint32 * 8)()
+    scales = np.zeros(len(trace_rows), dtype=np.float64)
     for t, row in enumerate(trace_rows):
         L.orc_step_from(C.byref(f2), fp(pal), pal_size, C.byref(view), burn_in, C.byref(r),
                         C.byref(w), rec)
         out[t] = list(rec)
+        scales[t] = L.orc_step_term_scale()
         # continue from the device's state after this step
         flags = int(row[7])
         fuse = burn_in if flags & 2 else (fuse - 1 if flags & 4 else fuse)
@@ -227,7 +236,7 @@ def replay_steps(fd, stops, view, walker, seed, burn_in, trace_rows, pal_size=25
         w.color_index = np.uint32(row[4]).view(np.float32).item()
         w.fuse = fuse
         w.prev = int(row[1])  # with xaos, the pick conditions the next one
-    return out
+    return (out, scales) if with_scale else out
 
 
 def render(fd, stops, job, pal_size=256, threads=1, palette_xf=None, exact_after=1 << 18,

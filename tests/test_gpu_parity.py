@@ -175,37 +175,52 @@ def ulp(x):
     return np.spacing(np.maximum(x, np.float32(2.0 ** -20))).astype(np.float64)
 
 
-@pytest.mark.parametrize("cfg,floor", [("single4k", 0.0), ("stress", 0.0), ("blurs", 1.0)])
-def test_step_parity_transcendental(engine, cfg, floor):
-    """One iterator step on the GPU vs the oracle's step from the same state.
-    RNG words, transform choices, respawn flags and the colour index are
-    exact. x and y are within 4 ulp of the step's scale -- the largest of
-    |x|, |y| going in and coming out, the magnitude at which the step's
-    affine, variation and post-affine terms are summed -- for 99.9% of steps
-    and within 16 ulp for all. (Device and host transcendentals each round
-    within ~1 ulp of their own result; sums of such terms cancel.) blur,
-    gaussian_blur and pre_blur add offsets of magnitude ~1 whatever the
-    point's size, so for that flame the scale is at least 1 (`floor`)."""
+@pytest.mark.parametrize("cfg", ["single4k", "stress", "blurs"])
+def test_step_parity_transcendental(engine, cfg):
+    """One iterator step on the GPU vs the oracle's step from the same state,
+    24000 steps over six walkers. RNG words, transform choices, respawn
+    flags and the colour index are exact. x and y are within 4 ulp of the
+    step's term scale at EVERY step: the largest magnitude among the terms
+    the step sums (the point, the affine point, each weighted variation
+    output, the post-affine products), which is the scale its FP32 sums
+    round at -- an output that cancels terms of magnitude 1 down to 0.1
+    carries their rounding, on any implementation (the oracle's own step
+    is then ~7 ulp of the output from the exact value).
+
+    The oracle's sin/cos/atan2 are the device's fixed method (pinned
+    against double-precision libm by tests/test_fastmath.py). Against the
+    host libm's instead, a one-ulp atan2 difference moves julia's
+    theta/2 + pi by a whole ulp at magnitude up to 4.7 (and julian's angle
+    alike), up to 8 ulp of a unit-scale term: that comparison is held to
+    p99.9 <= 4 and max <= 8 ulp of the term scale."""
     say("step parity", cfg)
     fd, pal, job = getattr(configs, cfg)()
-    errs = []
-    for walker in (3, 1000, 123456):
-        g = gpu_trace(engine, fd, pal, job.view, walker, 1, 20, 2000)
-        o = O.replay_steps(fd, pal, job.view, walker, 1, 20, g)
+    errs, errs_libm = [], []
+    for walker in (3, 7, 1000, 99999, 123456, 5555555):
+        g = gpu_trace(engine, fd, pal, job.view, walker, 1, 20, 4000)
+        o, term = O.replay_steps(fd, pal, job.view, walker, 1, 20, g, with_scale=True)
         assert np.array_equal(g[:, 0], o[:, 0]), "rng stream"
         assert np.array_equal(g[:, 1], o[:, 1]), "transform choice"
         assert np.array_equal(g[:, 7] & 2, o[:, 7] & 2), "respawns"
-        gx = g[:, 2:4].view(np.float32).astype(np.float64)
-        ox = o[:, 2:4].view(np.float32).astype(np.float64)
-        prev = np.vstack([np.zeros((1, 2)), gx[:-1]])
-        scale = ulp(np.maximum(np.maximum(np.abs(ox).max(1), np.abs(prev).max(1)), floor))
-        ok = np.isfinite(ox).all(1) & np.isfinite(gx).all(1)
-        errs.append((np.abs(gx - ox).max(1) / scale)[ok])
         # colour index: a pure +,* chain
         assert np.array_equal(g[:, 4], o[:, 4])
-    e = np.concatenate(errs)
-    say(f"  ulp error: median {np.median(e):.2f}  p99.9 {np.quantile(e, 0.999):.2f}  max {e.max():.2f}")
-    assert np.quantile(e, 0.999) <= 4.0 and e.max() <= 16.0
+        gx = g[:, 2:4].view(np.float32).astype(np.float64)
+        ox = o[:, 2:4].view(np.float32).astype(np.float64)
+        ok = np.isfinite(ox).all(1) & np.isfinite(gx).all(1) & ((g[:, 7] & 2) == 0)
+        errs.append((np.abs(gx - ox).max(1) / ulp(term))[ok])
+        O.lib().orc_use_libm(1)
+        try:
+            ol, lterm = O.replay_steps(fd, pal, job.view, walker, 1, 20, g, with_scale=True)
+        finally:
+            O.lib().orc_use_libm(0)
+        lx = ol[:, 2:4].view(np.float32).astype(np.float64)
+        okl = np.isfinite(lx).all(1) & np.isfinite(gx).all(1) & ((g[:, 7] & 2) == 0)
+        errs_libm.append((np.abs(gx - lx).max(1) / ulp(lterm))[okl])
+    e, el = np.concatenate(errs), np.concatenate(errs_libm)
+    say(f"  ulp of the term scale: median {np.median(e):.2f}  p99.9 {np.quantile(e, 0.999):.2f}"
+        f"  max {e.max():.2f};  vs libm: p99.9 {np.quantile(el, 0.999):.2f}  max {el.max():.2f}")
+    assert e.max() <= 4.0
+    assert np.quantile(el, 0.999) <= 4.0 and el.max() <= 8.0
 
 
 # -------------------------------------------------------------- stencils

@@ -17,7 +17,7 @@ import pytest
 
 from oracle import oracle as O
 import pyfae
-from pyfae import configs
+from pyfae import _abi, configs
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -377,6 +377,107 @@ def _draws(state, inc, n):
     r = O.Rng()
     r.state, r.inc = state, inc
     return [L.orc_rng_nextf(C.byref(r)) for _ in range(n)], r
+
+
+# ---- independent FP64 pins of the bodies that draw or read the transform
+# (advanced.rs:329-340 julia, :414-422 waves, :467-506 julian with
+# init_julian) and of the colour-index update (main_template.wgsl:496-503)
+
+SMALL = [(x, y) for x, y in PROBE[2:] if 1e-6 < math.hypot(x, y) < 10.0]
+
+
+def test_julia_closed_form_with_its_branch():
+    # r = |p|, theta = atan2(y, x), omega = pi when the draw < 0.5,
+    # out = sqrt(r) (cos, sin)(theta / 2 + omega); the draw is the pinned
+    # stream's next value. f32 rounds theta/2 + omega at magnitude up to
+    # 4.7, so the bar is that half-ulp times sqrt(r) plus a few ulp.
+    branches = set()
+    for x, y in SMALL:
+        X, Y = float(np.float32(x)), float(np.float32(y))
+        r = math.hypot(X, Y)
+        for state in range(1, 40, 3):
+            u, _ = _draws(state, 7, 1)
+            om = math.pi if u[0] < 0.5 else 0.0
+            branches.add(om)
+            t = math.atan2(Y, X) / 2 + om
+            ex, ey = math.sqrt(r) * math.cos(t), math.sqrt(r) * math.sin(t)
+            ox, oy, st = _var("julia", x, y, rng_state=(state, 7))
+            tol = 4e-7 * math.sqrt(r) + 4e-7 * math.sqrt(r)
+            assert abs(ox - ex) <= tol and abs(oy - ey) <= tol, (x, y, state, ox, oy, ex, ey)
+    assert branches == {0.0, math.pi}
+    # the swapped operand order atan2(x, y) is another map: it fails the form
+    X, Y = 0.3, 0.8
+    ox, oy, _ = _var("julia", X, Y, rng_state=(1, 7))
+    u, _ = _draws(1, 7, 1)
+    t = math.atan2(X, Y) / 2 + (math.pi if u[0] < 0.5 else 0.0)
+    assert abs(ox - math.sqrt(math.hypot(X, Y)) * math.cos(t)) > 1e-2
+
+
+def _julian_flame(power, dist):
+    return pyfae.flame_desc([pyfae.xform(variations={"julian": 1.0},
+                                         params={"julian.power": power, "julian.dist": dist})])
+
+
+@pytest.mark.parametrize("power,dist", [(2, 1.0), (3, 0.5), (-3, 1.0), (5, -1.5), (-7, 2.0)])
+def test_julian_init_and_closed_form(power, dist):
+    # init_julian: cpower = dist / max(|power|, 1e-30) / 2 (advanced.rs:467-474)
+    f2 = O.prepared(_julian_flame(power, dist))
+    cp = f2.xforms[0].var_param[0][2]
+    assert cp == float(np.float32(np.float32(dist) / np.float32(abs(power)) / np.float32(2)))
+    assert abs(cp - dist / abs(power) / 2) <= 1e-7 * abs(dist)
+    # body: r = (x^2 + y^2)^cpower, theta = atan2(y, x), k = floor(|power| u),
+    # t = (theta + 2 pi k) / power, out = r (cos t, sin t)
+    ks = set()
+    for x, y in SMALL:
+        X, Y = float(np.float32(x)), float(np.float32(y))
+        r = (X * X + Y * Y) ** (dist / abs(power) / 2)
+        for state in range(1, 160, 4):
+            u, _ = _draws(state, 11, 1)
+            k = math.floor(abs(power) * u[0])
+            ks.add(k)
+            t = (math.atan2(Y, X) + 2 * math.pi * k) / power
+            ox, oy, st = _var("julian", x, y, params=(power, dist, cp, 0), rng_state=(state, 11))
+            tol = 2e-6 * r + 1e-6 * r * abs(power)
+            assert abs(ox - r * math.cos(t)) <= tol and abs(oy - r * math.sin(t)) <= tol, \
+                (power, dist, x, y, state, ox, oy, r * math.cos(t), r * math.sin(t))
+            assert (st.state, st.inc) == (_draws(state, 11, 1)[1].state, 11)
+    assert ks == set(range(abs(power)))  # every branch angle is taken
+
+
+def test_waves_reads_its_own_b_e_d_f():
+    # x + b sin(y / (e^2 + 1e-6)), y + d sin(x / (f^2 + 1e-6)) with the
+    # transform's own b, d, e, f (advanced.rs:414-422); distinct values, so
+    # an e/f or b/d swap is caught
+    xf = pyfae.Xform()
+    xf.b, xf.d, xf.e, xf.f = np.float32(0.37), np.float32(-0.81), np.float32(0.7), np.float32(1.3)
+    b, d, e, f = (float(v) for v in (xf.b, xf.d, xf.e, xf.f))
+    for x, y in SMALL:
+        X, Y = float(np.float32(x)), float(np.float32(y))
+        ex = X + b * math.sin(Y / (e * e + 1e-6))
+        ey = Y + d * math.sin(X / (f * f + 1e-6))
+        ox, oy, st = _var("waves", x, y, xf=xf, rng_state=(5, 3))
+        assert _near(ox, ex, 4e-6, 2e-6) and _near(oy, ey, 4e-6, 2e-6), (x, y, ox, oy, ex, ey)
+        assert (st.state, st.inc) == (5, 3)
+        sx = X + b * math.sin(Y / (f * f + 1e-6))  # e and f swapped
+        if abs(sx - ex) > 1e-3:
+            assert abs(ox - sx) > 1e-4
+
+
+@pytest.mark.parametrize("color,speed", [(1.0, 0.5), (0.6, -0.3), (0.25, 0.9)])
+def test_colour_index_update_is_the_symmetry_form(color, speed):
+    # c <- c (1 + s)/2 + colour (1 - s)/2 (main_template.wgsl:496-503):
+    # from c0 = 0 with one transform, c_n = colour (1 - ((1 + s)/2)^n).
+    # flam3's c <- c (1 - s) + colour s gives colour (1 - (1 - s)^n)
+    # instead, which differs for s != 1/2... and at s = 1/2 (0.75^n vs 0.5^n)
+    fd = pyfae.flame_desc([pyfae.xform(0.5, 0.0, 0.0, 0.5, color=color, color_speed=speed)])
+    view = _abi.View(64, 64, 1.0, 0.0, 0.0, 0.0)
+    tr = O.trace(fd, np.array([[0, 0, 0, 0], [1, 1, 1, 1]], np.float32), view, 3, 1, 0, 30)
+    ci = tr[:, 4].view(np.float32).astype(np.float64)
+    n = np.arange(1, 31)
+    want = color * (1 - ((1 + speed) / 2) ** n)
+    flam3 = color * (1 - (1 - speed) ** n)
+    assert np.abs(ci - want).max() <= 1e-6
+    assert np.abs(ci - flam3).max() > 1e-2
 
 
 def test_blob_against_float64_closed_form():
