@@ -144,6 +144,71 @@ def test_lds_splat_with_tile_bins(engine, side):
     assert torch.allclose(g4[..., :3], g3[..., :3], rtol=1e-4, atol=1e-3)
 
 
+@pytest.mark.parametrize("side", [(6144, 3072), (12288, 12288)])
+def test_binned_splat_equals_oracle(engine, side):
+    """The binned LDS splat (bins of 2 and 16 tiles) against the oracle's
+    direct splat (shaders/core/accumulate_samples.wgsl:280-365 deposits the
+    same u32 sums): an affine flame with a post-affine and a final, whose
+    steps are +,* chains the device rounds as the oracle does, so the u32
+    histogram is bit-identical; the f32 one has the same density sums
+    exactly and colour sums between the u32 truncations and one per hit."""
+    w, h = side
+    say(f"binned splat vs oracle {w}x{h}")
+    xs = [pyfae.xform(0.5, 0.0, 0.0, 0.5, -0.5, -0.5, weight=1.0, color=0.0),
+          pyfae.xform(0.5, 0.1, -0.1, 0.5, 0.5, -0.5, weight=1.0, color=0.4,
+                      post=(0.9, 0.05, -0.05, 0.9, 0.0, 0.02)),
+          pyfae.xform(0.45, -0.2, 0.2, 0.45, 0.0, 0.5, weight=0.8, color=1.0)]
+    fin = [pyfae.xform(1.1, 0.0, 0.0, 1.1, 0.0, 0.0)]
+    fd = pyfae.flame_desc(xs, finals=fin)
+    _, pal, _ = configs.plain()
+    view = pyfae.View(w, h, 4.0 / 3.0, 0.0, 0.0, 0.0)
+    walkers, ipt = 1 << 16, 64
+    o, oplotted, oin = O.iterate(fd, pal, view, walkers, ipt, 20, 9)
+    g, c = gpu_hist_dev(engine, fd, pal, view, walkers, ipt, 20, 9, mode=2)
+    assert c == (oplotted, oin) and oin > walkers * 30
+    assert np.array_equal(as_u32(g), o)
+    del g
+    g4, c4 = gpu_hist_dev(engine, fd, pal, view, walkers, ipt, 20, 9, mode=4)
+    assert c4 == (oplotted, oin)
+    g4 = g4.cpu().numpy()
+    assert np.array_equal(g4[..., 3], o[..., 3].astype(np.float32))
+    hits = o[..., 3:4].astype(np.float64) / 100.0
+    d = g4[..., :3].astype(np.float64) - o[..., :3]
+    assert (d >= -1e-3 * g4[..., :3] - 1e-3).all() and (d <= hits + 1e-3 * g4[..., :3] + 1e-3).all()
+
+
+def test_single4k_full_size_with_density_estimation(engine):
+    """The headline configuration at its own size, 3840x2160 with density
+    estimation 9 -> 0 (curve 0.4) and the log tone map, at 2^18 walkers:
+    the GPU's accumulator block densities and tone-mapped block colours
+    differ from the oracle's render no more than a second oracle seed's."""
+    say("single4k 3840x2160 with DE vs oracle")
+    fd, pal, job = configs.single4k(walkers=1 << 18, ipt=256)
+    assert job.de_max_radius == 9.0 and job.view.width == 3840
+    set_flame(engine, fd, pal)
+    gacc, _ = engine.render_accum(job)
+    grgba, _ = engine.render(job)
+    org1, o1, _ = O.render(fd, pal, job, threads=16)
+    job2 = configs.single4k(walkers=1 << 18, ipt=256)[2]
+    job2.seed_base = 2
+    org2, o2, _ = O.render(fd, pal, job2, threads=16)
+    b = 32
+    bg, b1, b2 = block_sums(gacc, b), block_sums(o1, b), block_sums(o2, b)
+    floor = np.abs(b1 - b2).sum() / b1.sum()
+    diff = np.abs(bg - b1).sum() / b1.sum()
+
+    def blocks(img):
+        f = img[..., :3].astype(np.float64)
+        hh, ww = f.shape[0] // b * b, f.shape[1] // b * b
+        return f[:hh, :ww].reshape(hh // b, b, ww // b, b, 3).mean((1, 3))
+
+    cf = np.abs(blocks(org1) - blocks(org2)).mean()
+    cd = np.abs(blocks(grgba) - blocks(org1)).mean()
+    say(f"  density floor {floor:.4f} gpu {diff:.4f}; colour floor {cf:.3f} gpu {cd:.3f}")
+    assert diff <= 1.25 * floor + 1e-3
+    assert cd <= 1.25 * cf + 0.05
+
+
 def test_f32_lds_tile_splat_matches_f32_atomics(engine):
     say("f32 lds tile splat")
     fd, pal, job = configs.single4k(walkers=1 << 16, ipt=64, width=960, height=540)
