@@ -41,13 +41,28 @@ static_assert((unsigned long long)SLICE * 255ull < (1ull << 32), "tile channel s
 constexpr size_t STAGE_FIXED = (size_t)TILE_MAX * 8 + 16 * 4 + 16;  // cnt, gbase, wsum, fill
 constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4 + 2;                  // rec, tr, perm
 
-// Walker workgroups: two of WALK_HALF lanes per CU when the flame table,
-// palette and a stage of at least STAGE_HALF_MIN records fit in half the
-// CU's LDS, so one workgroup's flushes overlap the other's walking; else
-// one of BIN_BLOCK lanes with the whole budget.
-constexpr int WALK_HALF = 512;
+// Walker workgroups of WALK_LANES lanes, as many per CU as the LDS holds
+// with a useful stage: three (52 KB each; 6 waves a SIMD, the occupancy the
+// walker kernels are built for) when the flame table, palette and a stage of
+// at least STAGE_THIRD_MIN records fit in a third of the CU's LDS, else two
+// (78 KB, a stage of at least STAGE_HALF_MIN), else one of BIN_BLOCK lanes
+// with the whole budget. The walk is latency-bound at 4 waves a SIMD: 3
+// workgroups a CU walk single4k 1.2x faster than 2, and while one
+// workgroup's flush waits at its barriers the others keep the SIMDs issuing.
+// Every shape stages at least 6 x its lanes, so the fill check (at most
+// every stage / lanes - 1 iterations) never lets the stage overflow.
+constexpr int WALK_LANES = 512;
+constexpr size_t LDS_THIRD = 52 * 1024;
 constexpr size_t LDS_HALF = 78 * 1024;
+constexpr int STAGE_THIRD_MIN = 3072;
 constexpr int STAGE_HALF_MIN = 4096;
+static_assert(STAGE_THIRD_MIN >= 6 * WALK_LANES && STAGE_MIN >= 6 * BIN_BLOCK / 2,
+              "every walker shape stages several fill checks' worth of records");
+// The one-flame walker kernel is compiled for 6 waves a SIMD (at most 80
+// VGPRs), so three workgroups a CU are resident together (its walkers are
+// strided over the whole grid). The batched kernel keeps its registers (at
+// 80 it spills) and takes at most two workgroups a CU.
+#define FFM_WALKER_WAVES __attribute__((amdgpu_waves_per_eu(6)))
 
 static size_t flame_lds(int n_rows, int pal_size) {
   return flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4;
@@ -58,16 +73,25 @@ static int stage_fit(size_t budget, size_t used) {
   return (int)std::min<size_t>(fit, STAGE_MAX);
 }
 
-void bin_shape(int n_rows, int pal_size, int* block, int* stage) {
+void bin_shape(int n_rows, int pal_size, int* block, int* stage, int* per_cu, int max_per_cu) {
   const size_t used = flame_lds(n_rows, pal_size) + STAGE_FIXED;
+  const int third = stage_fit(LDS_THIRD, used);
+  if (max_per_cu >= 3 && third >= STAGE_THIRD_MIN) {
+    *block = WALK_LANES;
+    *stage = third;
+    *per_cu = 3;
+    return;
+  }
   const int half = stage_fit(LDS_HALF, used);
   if (half >= STAGE_HALF_MIN) {
-    *block = WALK_HALF;
+    *block = WALK_LANES;
     *stage = half;
+    *per_cu = 2;
     return;
   }
   *block = BIN_BLOCK;
   *stage = std::max(stage_fit((size_t)LDS_DYNAMIC_MAX, used), STAGE_MIN);
+  *per_cu = 1;
 }
 
 size_t bin_lds_bytes(int n_rows, int pal_size, int stage) {
@@ -218,8 +242,11 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
   const ViewConst v = a.view;
   const uint64_t rounds = limit > base ? (limit - base + stride - 1) / stride : 0;
   // a fill check every `every` iterations; at most `every` x blockDim
-  // records arrive between checks, so flushing at stage - that never overflows
-  const uint32_t every = (uint32_t)min(max(t.flush_every, 1), MAX_FLUSH_EVERY);
+  // records arrive between checks, so flushing at stage - that never
+  // overflows (every is capped so that stays positive: the stage holds at
+  // least 6 x blockDim records)
+  const uint32_t every = (uint32_t)max(
+      min(min(max(t.flush_every, 1), MAX_FLUSH_EVERY), t.stage / (int)blockDim.x - 1), 1);
   const uint32_t flush_at = (uint32_t)t.stage - every * blockDim.x;
   for (uint64_t rd = 0; rd < rounds; rd++) {
     const uint64_t wi = base + rd * stride + tid;
@@ -308,7 +335,8 @@ __device__ void finish_walkers(const TileArgs& t, const BinLds& L, const uint32_
 
 // One dispatch of one flame: walkers grid-strided over the whole grid.
 template <bool COUNT_ONLY, bool EXT>
-__global__ __launch_bounds__(BIN_BLOCK) void bin_iterate_kernel(IterArgs a, TileArgs t) {
+__global__ __launch_bounds__(BIN_BLOCK) FFM_WALKER_WAVES void bin_iterate_kernel(IterArgs a,
+                                                                                 TileArgs t) {
   extern __shared__ uint32_t lds[];
   LdsFlame F = load_lds<false>(a, lds);
   const BinLds L = bin_lds_views(lds + flame_table_floats(a.n_rows) + a.pal_size, t.stage);
@@ -672,8 +700,8 @@ static hipError_t launch_bin_iterate_t(const IterArgs& a, const TileArgs& t, int
   if (e != hipSuccess) return e;
   const int blk = t.walk_block;
   const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size, t.stage);
-  hipLaunchKernelGGL((bin_iterate_kernel<COUNT_ONLY, EXT>), dim3(grid * (BIN_BLOCK / blk)),
-                     dim3(blk), shm, s, a, t);
+  hipLaunchKernelGGL((bin_iterate_kernel<COUNT_ONLY, EXT>), dim3(grid * t.walk_per_cu), dim3(blk),
+                     shm, s, a, t);
   return hipGetLastError();
 }
 
@@ -693,7 +721,7 @@ static hipError_t launch_bin_batch_t(const BatchArgs& b, int grid, hipStream_t s
   if (e != hipSuccess) return e;
   const int blk = b.g.walk_block;
   const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal, b.g.stage);
-  hipLaunchKernelGGL((bin_batch_kernel<COUNT_ONLY, EXT>), dim3(grid * (BIN_BLOCK / blk)),
+  hipLaunchKernelGGL((bin_batch_kernel<COUNT_ONLY, EXT>), dim3(grid * b.g.walk_per_cu),
                      dim3(blk), shm, s, b);
   return hipGetLastError();
 }
