@@ -131,12 +131,14 @@ struct TileArgs {
   int32_t stage;         // staged records per walker workgroup (bin_shape)
   int32_t walk_block;    // lanes per walker workgroup (bin_shape): 512 or 1024
   int32_t walk_per_cu;   // walker workgroups per CU (bin_shape): 3 or 2 of 512, 1 of 1024
+  int32_t bins_lds;      // bin slots of the walkers' LDS counting sort: >= n_tiles, a multiple of 64
 };
 
 // The walker launch for a flame's LDS needs: lanes per workgroup and
 // staged records per workgroup.
-void bin_shape(int n_rows, int pal_size, int* block, int* stage, int* per_cu, int max_per_cu);
-size_t bin_lds_bytes(int n_rows, int pal_size, int stage);
+void bin_shape(int n_rows, int pal_size, int nb, int* block, int* stage, int* per_cu,
+               int max_per_cu);
+size_t bin_lds_bytes(int n_rows, int pal_size, int stage, int nb);
 size_t accum_lds_bytes(int pal_size, bool f32);
 hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_only, int grid,
                               hipStream_t s);

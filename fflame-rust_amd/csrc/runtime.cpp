@@ -597,8 +597,9 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
     return fail(c, FFM_ERR_TOO_LARGE, "the LDS tile splat holds %d bins of up to 128 %dx%d tiles; "
                 "%dx%d needs more", TILE_MAX, TILE_W, TILE_H, W, H);
   int walk_block = 0, stage = 0, per_cu = 0;
-  bin_shape(a.n_rows, a.pal_size, &walk_block, &stage, &per_cu, 3);
-  if (bin_lds_bytes(a.n_rows, a.pal_size, stage) > LDS_PER_CU ||
+  const int nb = (n_tiles + 63) / 64 * 64;
+  bin_shape(a.n_rows, a.pal_size, nb, &walk_block, &stage, &per_cu, 3);
+  if (bin_lds_bytes(a.n_rows, a.pal_size, stage, nb) > LDS_PER_CU ||
       accum_lds_bytes(a.pal_size, f32) > LDS_PER_CU)
     return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
   const uint64_t per_walker = (uint64_t)a.ipt + 1;
@@ -652,6 +653,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   t.stage = stage;
   t.walk_block = walk_block;
   t.walk_per_cu = per_cu;
+  t.bins_lds = nb;
   const int grid = std::max(c->num_cu, 1);
   const bool same = !ext && c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
                     c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
@@ -1831,7 +1833,8 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.g.hist = reinterpret_cast<uint32_t*>(P.hist.p);
   b.g.n_tiles = P.n_tiles;
   b.g.flush_every = flush_every;
-  bin_shape(max_rows, BATCH_PALETTE, &b.g.walk_block, &b.g.stage, &b.g.walk_per_cu, 2);
+  b.g.bins_lds = TILE_MAX;  // flames of every size share the launch
+  bin_shape(max_rows, BATCH_PALETTE, TILE_MAX, &b.g.walk_block, &b.g.stage, &b.g.walk_per_cu, 2);
   const int grid = std::max(c->num_cu, 1);
   const BatchItem* items_d = reinterpret_cast<const BatchItem*>(P.items.p);
   HIPCHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -38,8 +38,10 @@ constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
 // accum_run adds two channels per 64-bit LDS atomic: a slice's channel sums
 // must not carry out of 32 bits (at most 255, or 100 for density, a record)
 static_assert((unsigned long long)SLICE * 255ull < (1ull << 32), "tile channel sums carry");
-constexpr size_t STAGE_FIXED = (size_t)TILE_MAX * 8 + 16 * 4 + 16;  // cnt, gbase, wsum, fill
-constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4 + 2;                  // rec, tr, perm
+constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4 + 2;  // rec, tr, perm
+// LDS of the counting sort for nb bin slots: cnt, gbase (u32) and lim (u16)
+// per bin, then the scan's wave sums and the fill words
+static size_t stage_fixed(int nb) { return (size_t)nb * (4 + 4 + 2) + 16 * 4 + 16; }
 
 // Walker workgroups of WALK_LANES lanes, as many per CU as the LDS holds
 // with a useful stage: three (52 KB each; 6 waves a SIMD, the occupancy the
@@ -69,12 +71,13 @@ static size_t flame_lds(int n_rows, int pal_size) {
 }
 
 static int stage_fit(size_t budget, size_t used) {
-  const size_t fit = used < budget ? (budget - used) / STAGE_BYTES_PER_RECORD / 1024 * 1024 : 0;
+  const size_t fit = used < budget ? (budget - used) / STAGE_BYTES_PER_RECORD / 512 * 512 : 0;
   return (int)std::min<size_t>(fit, STAGE_MAX);
 }
 
-void bin_shape(int n_rows, int pal_size, int* block, int* stage, int* per_cu, int max_per_cu) {
-  const size_t used = flame_lds(n_rows, pal_size) + STAGE_FIXED;
+void bin_shape(int n_rows, int pal_size, int nb, int* block, int* stage, int* per_cu,
+               int max_per_cu) {
+  const size_t used = flame_lds(n_rows, pal_size) + stage_fixed(nb);
   const int third = stage_fit(LDS_THIRD, used);
   if (max_per_cu >= 3 && third >= STAGE_THIRD_MIN) {
     *block = WALK_LANES;
@@ -94,8 +97,8 @@ void bin_shape(int n_rows, int pal_size, int* block, int* stage, int* per_cu, in
   *per_cu = 1;
 }
 
-size_t bin_lds_bytes(int n_rows, int pal_size, int stage) {
-  return flame_lds(n_rows, pal_size) + STAGE_FIXED + (size_t)stage * STAGE_BYTES_PER_RECORD;
+size_t bin_lds_bytes(int n_rows, int pal_size, int stage, int nb) {
+  return flame_lds(n_rows, pal_size) + stage_fixed(nb) + (size_t)stage * STAGE_BYTES_PER_RECORD;
 }
 
 size_t accum_lds_bytes(int pal_size, bool f32) {
@@ -135,8 +138,9 @@ struct BinLds {
   uint32_t* rec;    // stage
   uint32_t* tr;     // stage: tile | rank << 16
   uint16_t* perm;   // stage
-  uint32_t* cnt;    // TILE_MAX
-  uint32_t* gbase;  // TILE_MAX
+  uint32_t* cnt;    // bins_lds: records per bin, then their start in the sorted stage
+  uint32_t* gbase;  // bins_lds: the bin's first record slot of this flush (region + fill)
+  uint16_t* lim;    // bins_lds: slots left in the bin's region (at most 65535)
   uint32_t* wsum;   // 16
   uint32_t* fill;   // 1, then one word of scan total
 };
@@ -187,11 +191,22 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
     L.tr[j] = tile | (r << 16);
   }
   __syncthreads();
+  // reserve each bin's run, and keep where it starts and how much of it
+  // the region holds, so the record loop below reads LDS only
   for (int i = tid; i < t.n_tiles; i += blockDim.x) {
     const uint32_t c = L.cnt[i];
-    L.gbase[i] = c ? __hip_atomic_fetch_add(t.tile_fill + i, c, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT)
-                   : 0u;
+    uint32_t base = 0u, room = 0u;
+    if (c) {
+      const uint32_t old =
+          __hip_atomic_fetch_add(t.tile_fill + i, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t cap = t.cap[i];
+      if (cap > old) {
+        base = t.region[i] + old;
+        room = min(cap - old, 0xFFFFu);
+      }
+    }
+    L.gbase[i] = base;
+    L.lim[i] = (uint16_t)room;
   }
   __syncthreads();
   // cnt -> start; more bins than 2 x blockDim go 2 x blockDim at a time
@@ -215,13 +230,14 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
     L.perm[L.cnt[v & 0xFFFFu] + (v >> 16)] = (uint16_t)j;
   }
   __syncthreads();
+  // ranks below 2^14 (the stage), so `rank < lim` is `fill + rank < cap`
   for (int i = tid; i < n; i += blockDim.x) {
     const int j = L.perm[i];
     const uint32_t v = L.tr[j];
     const int tile = (int)(v & 0xFFFFu);
-    const uint32_t dst = L.gbase[tile] + (v >> 16);
-    if (dst < t.cap[tile])
-      t.records[(size_t)t.region[tile] + dst] = L.rec[j];
+    const uint32_t rank = v >> 16;
+    if (rank < L.lim[tile])
+      t.records[(size_t)(L.gbase[tile] + rank)] = L.rec[j];
     else
       overflow_splat(t, pal, tile, L.rec[j]);
   }
@@ -303,14 +319,17 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
   }
 }
 
-__device__ __forceinline__ BinLds bin_lds_views(uint32_t* p, int stage) {
+// The staging layout for `stage` records (a multiple of 512) and `nb` bin
+// slots (a multiple of 64).
+__device__ __forceinline__ BinLds bin_lds_views(uint32_t* p, int stage, int nb) {
   BinLds L;
   L.rec = p;
   L.tr = p + stage;
   L.perm = reinterpret_cast<uint16_t*>(p + 2 * stage);
   L.cnt = p + 2 * stage + stage / 2;
-  L.gbase = L.cnt + TILE_MAX;
-  L.wsum = L.gbase + TILE_MAX;
+  L.gbase = L.cnt + nb;
+  L.lim = reinterpret_cast<uint16_t*>(L.gbase + nb);
+  L.wsum = L.gbase + nb + nb / 2;
   L.fill = L.wsum + 16;
   return L;
 }
@@ -339,7 +358,8 @@ __global__ __launch_bounds__(BIN_BLOCK) FFM_WALKER_WAVES void bin_iterate_kernel
                                                                                  TileArgs t) {
   extern __shared__ uint32_t lds[];
   LdsFlame F = load_lds<false>(a, lds);
-  const BinLds L = bin_lds_views(lds + flame_table_floats(a.n_rows) + a.pal_size, t.stage);
+  const BinLds L =
+      bin_lds_views(lds + flame_table_floats(a.n_rows) + a.pal_size, t.stage, t.bins_lds);
   const int tid = threadIdx.x;
   for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
   if (tid == 0) *L.fill = 0u;
@@ -426,13 +446,12 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
   if (threadIdx.x == 0) t.item_start[t.n_tiles] = all;
 }
 
-// Add records run[0..n) into the LDS tile (LDS atomics). All threads.
+// Add one record into the LDS tile (LDS atomics).
 template <bool F32>
-__device__ __forceinline__ void accum_run(const uint32_t* run, uint32_t n, uint32_t* tile,
-                                          const uint32_t* pal, const float* palf) {
+__device__ __forceinline__ void accum_record(uint32_t v, uint32_t* tile, const uint32_t* pal,
+                                             const float* palf) {
   float* tilef = reinterpret_cast<float*>(tile);
-  for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
-    const uint32_t v = run[r];
+  {
     const uint32_t local = v & REC_LOCAL_MASK;
     if (F32) {
       const float* c = palf + 3 * rec_texel(v);
@@ -454,6 +473,25 @@ __device__ __forceinline__ void accum_run(const uint32_t* run, uint32_t n, uint3
       atomicAdd(&t64[2 * local + 1], bd);
     }
   }
+}
+
+// Add records run[0..n) into the LDS tile. All threads. Eight records a
+// thread per round, their loads issued together, so each lane keeps eight
+// reads in flight instead of waiting out one at a time (integer sums
+// commute: the order of the atomics does not change the tile).
+template <bool F32>
+__device__ __forceinline__ void accum_run(const uint32_t* run, uint32_t n, uint32_t* tile,
+                                          const uint32_t* pal, const float* palf) {
+  const uint32_t bd = blockDim.x;
+  uint32_t r = threadIdx.x;
+  for (; r + 7 * bd < n; r += 8 * bd) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = run[r + k * bd];
+#pragma unroll
+    for (int k = 0; k < 8; k++) accum_record<F32>(v[k], tile, pal, palf);
+  }
+  for (; r < n; r += bd) accum_record<F32>(run[r], tile, pal, palf);
 }
 
 // Fold the LDS tile (tile column tx, row ty) into the histogram: a
@@ -622,7 +660,8 @@ __device__ __forceinline__ TileArgs flame_tiles(const BatchArgs& b, const BatchF
 template <bool COUNT_ONLY, bool EXT>
 __global__ __launch_bounds__(BIN_BLOCK) void bin_batch_kernel(BatchArgs b) {
   extern __shared__ uint32_t lds[];
-  const BinLds L = bin_lds_views(lds + flame_table_floats(b.max_rows) + b.max_pal, b.g.stage);
+  const BinLds L =
+      bin_lds_views(lds + flame_table_floats(b.max_rows) + b.max_pal, b.g.stage, b.g.bins_lds);
   const int tid = threadIdx.x;
   for (int i = tid; i < TILE_MAX; i += blockDim.x) L.cnt[i] = 0u;
   if (tid == 0) *L.fill = 0u;
@@ -699,7 +738,7 @@ static hipError_t launch_bin_iterate_t(const IterArgs& a, const TileArgs& t, int
       allow_lds(reinterpret_cast<const void*>(&bin_iterate_kernel<COUNT_ONLY, EXT>));
   if (e != hipSuccess) return e;
   const int blk = t.walk_block;
-  const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size, t.stage);
+  const size_t shm = bin_lds_bytes(a.n_rows, a.pal_size, t.stage, t.bins_lds);
   hipLaunchKernelGGL((bin_iterate_kernel<COUNT_ONLY, EXT>), dim3(grid * t.walk_per_cu), dim3(blk),
                      shm, s, a, t);
   return hipGetLastError();
@@ -720,7 +759,7 @@ static hipError_t launch_bin_batch_t(const BatchArgs& b, int grid, hipStream_t s
       allow_lds(reinterpret_cast<const void*>(&bin_batch_kernel<COUNT_ONLY, EXT>));
   if (e != hipSuccess) return e;
   const int blk = b.g.walk_block;
-  const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal, b.g.stage);
+  const size_t shm = bin_lds_bytes(b.max_rows, b.max_pal, b.g.stage, b.g.bins_lds);
   hipLaunchKernelGGL((bin_batch_kernel<COUNT_ONLY, EXT>), dim3(grid * b.g.walk_per_cu),
                      dim3(blk), shm, s, b);
   return hipGetLastError();
