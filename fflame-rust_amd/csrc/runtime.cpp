@@ -1637,14 +1637,29 @@ int ffm_render_accum(ffm_ctx* c, const ffm_job* job, float* accum_out, ffm_rende
 
 namespace {
 
-constexpr uint64_t BATCH_RECORD_BUDGET = 3ull << 30;  // records staged per wave (12 GB)
+// Records staged per wave of a batch: an eighth of the device memory free
+// when the batch is laid out (on a 288 GB MI355X: 9 G records, 36 GB), at
+// least 2^28 and at most 2^34 records, and no more than the batch needs.
+// Fewer, larger waves cut the persistent launches' tails: the 512-flame
+// gallery walks in 4 waves instead of 11 at a fixed 12 GB (702 vs 654
+// flames/s).
+constexpr uint64_t BATCH_RECORD_MIN = 1ull << 28, BATCH_RECORD_MAX = 1ull << 34;
+
+uint64_t batch_record_budget(uint64_t needed, size_t held_bytes) {
+  size_t free_b = 0, total_b = 0;
+  uint64_t b = 3ull << 30;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) b = ((uint64_t)free_b + held_bytes) / 8 / 4;
+  else (void)hipGetLastError();
+  b = std::min(std::max(b, BATCH_RECORD_MIN), BATCH_RECORD_MAX);
+  return std::min(b, std::max(needed, BATCH_RECORD_MIN));
+}
 constexpr int BATCH_PALETTE = 256;
 constexpr uint32_t BATCH_COUNT_DIV = 8;  // the pre-pass walks 1/8 of wave 0
 
 struct BatchPlan {
   // what the graph was built for
   std::vector<uint64_t> sig;
-  int n = 0, n_waves = 0, max_rows = 0, n_tiles = 0;
+  int n = 0, n_waves = 0, max_rows = 0, n_tiles = 0, max_tiles = 0;
   uint64_t total_pix = 0;
   uint32_t item_walkers = 0;
   int flush_every = 4;
@@ -1739,7 +1754,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   P.flush_every = flush_every;
   std::vector<BatchFlame> bf(n);
   uint64_t tile_base = 0, pix_off = 0, table_off = 0, total_samples = 0;
-  int max_rows = 0;
+  int max_rows = 0, max_tiles = 0;
   P.pix.assign(n, BatchPix{});
   P.samples.assign(n, 0);
   for (int i = 0; i < n; i++) {
@@ -1757,6 +1772,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     f.seed = j.seed_base;  // one dispatch per flame: seed_0
     f.tiles_x = (j.view.width + TILE_W - 1) / TILE_W;
     f.n_tiles = f.tiles_x * ((j.view.height + TILE_H - 1) / TILE_H);
+    max_tiles = std::max(max_tiles, f.n_tiles);
     f.tile_base = (uint32_t)tile_base;
     f.hist_off = pix_off;
     const uint64_t npix = (uint64_t)j.view.width * j.view.height;
@@ -1771,11 +1787,13 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     total_samples += j.walkers * (uint64_t)(j.ipt + 1);
   }
   P.max_rows = max_rows;
+  P.max_tiles = max_tiles;
   P.n_tiles = (int)tile_base;
   P.total_pix = pix_off;
   // waves: each a proportional slice of every flame, sized to the record budget
-  P.n_waves = (int)std::max<uint64_t>(1, (total_samples * 5 / 4 + BATCH_RECORD_BUDGET - 1) /
-                                             BATCH_RECORD_BUDGET);
+  // the records buffer already held counts as free: it is reused or replaced
+  const uint64_t budget = batch_record_budget(total_samples * 5 / 4, P.records.bytes);
+  P.n_waves = (int)std::max<uint64_t>(1, (total_samples * 5 / 4 + budget - 1) / budget);
   std::vector<BatchItem> items;
   P.wave_start.assign(1, 0);
   for (int w = 0; w < P.n_waves; w++) {
@@ -1802,7 +1820,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   if ((st = ensure(c, P.region, tiles_pad * 4, "batch tiles"))) return st;
   if ((st = ensure(c, P.cap, tiles_pad * 4, "batch tiles"))) return st;
   if ((st = ensure(c, P.item_start, tiles_pad * 4, "batch tiles"))) return st;
-  if ((st = ensure(c, P.records, BATCH_RECORD_BUDGET * 4, "batch records"))) return st;
+  if ((st = ensure(c, P.records, budget * 4, "batch records"))) return st;
   const size_t n_counters = ((size_t)P.n_waves + 1 + 3) & ~(size_t)3;
   if ((st = ensure(c, P.counters, n_counters * 4, "batch counters"))) return st;
   uint32_t* counters = reinterpret_cast<uint32_t*>(P.counters.p);
@@ -1833,8 +1851,10 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.g.hist = reinterpret_cast<uint32_t*>(P.hist.p);
   b.g.n_tiles = P.n_tiles;
   b.g.flush_every = flush_every;
-  b.g.bins_lds = TILE_MAX;  // flames of every size share the launch
-  bin_shape(max_rows, BATCH_PALETTE, TILE_MAX, &b.g.walk_block, &b.g.stage, &b.g.walk_per_cu, 2);
+  // the largest flame's bins: a workgroup sorts one flame's records at a time
+  b.g.bins_lds = (P.max_tiles + 63) / 64 * 64;
+  bin_shape(max_rows, BATCH_PALETTE, b.g.bins_lds, &b.g.walk_block, &b.g.stage, &b.g.walk_per_cu,
+            3);
   const int grid = std::max(c->num_cu, 1);
   const BatchItem* items_d = reinterpret_cast<const BatchItem*>(P.items.p);
   HIPCHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1851,7 +1871,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     step = "count pre-pass";
     if ((e = launch_bin_batch(w0, true, grid, s)) != hipSuccess) return e;
     step = "capacities";
-    if ((e = launch_tile_capacity_n(b.g, (float)BATCH_COUNT_DIV, 1024, BATCH_RECORD_BUDGET, s)) !=
+    if ((e = launch_tile_capacity_n(b.g, (float)BATCH_COUNT_DIV, 1024, budget, s)) !=
         hipSuccess)
       return e;
     for (int w = 0; w < P.n_waves; w++) {
