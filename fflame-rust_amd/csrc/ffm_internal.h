@@ -178,6 +178,7 @@ struct TonemapConst {
   int32_t use_curve;    // map clipped colours through curve_lut
   float curve_lut[256]; // the tone curve at i/255, each rounded to f16
 };
+hipError_t launch_downsample2x(const uint8_t* in, uint8_t* out, int ow, int oh, hipStream_t s);
 hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H,
                           const TonemapConst& tc, hipStream_t s);
 

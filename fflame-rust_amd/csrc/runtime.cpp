@@ -58,6 +58,7 @@ struct ffm_ctx {
   DevBuf pal_u32, pal_f32;
   // render buffers
   DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv, count;
+  DevBuf rgba_out;               // a supersampled render's boxed-down picture
   DevBuf fx_a, fx_b, fx_rgba;    // effect chain ping-pong buffers
   DevBuf fx_pyr;                 // bloom's pyramids, the distance field's coordinates
   ffm_effects fx{};              // effects renders apply (none by default)
@@ -73,6 +74,8 @@ struct ffm_ctx {
   std::vector<hipEvent_t> kev;
   std::vector<int> kev_kind;  // 0 walker kernel, 1 tile accumulate
   size_t kev_used = 0;
+  std::vector<hipEvent_t> dev;  // event pairs around whole dispatches
+  size_t dev_used = 0;
   struct ffm_batch_state* batch = nullptr;
   // differentiable soft splat: the fits set last
   std::vector<SoftFit> s_fits;
@@ -528,8 +531,19 @@ constexpr size_t LDS_PER_CU = LDS_DYNAMIC_MAX;
 // dispatch shape stay the same); per chunk: clear fills, binned iterate,
 // tile accumulate into `hist`.
 // Bracket a kernel launch with a pair of timing events (kind 0: walker
-// kernel, 1: tile accumulate); read back by collect_timings.
+// kernel, 1: tile accumulate), or (kind 2) a dispatch's whole splat phase,
+// whose pairs hold the kernels' and so live in a pool of their own; read
+// back by collect_timings.
 hipError_t mark(ffm_ctx* c, hipStream_t s, int kind, bool begin) {
+  if (kind == 2) {
+    if (c->dev_used == c->dev.size()) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      c->dev.push_back(e);
+    }
+    return hipEventRecord(c->dev[c->dev_used++], s);
+  }
   if (begin) {
     while (c->kev.size() < c->kev_used + 2) {
       hipEvent_t e;
@@ -544,7 +558,16 @@ hipError_t mark(ffm_ctx* c, hipStream_t s, int kind, bool begin) {
   return hipEventRecord(c->kev[c->kev_used++], s);
 }
 
-void collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, double& accum_ms) {
+// Waits for the last recorded event, sums the pairs by kind (2: a whole
+// dispatch's splat phase) and frees the pool for reuse.
+hipError_t collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, double& accum_ms,
+                           double& dispatch_ms) {
+  for (hipEvent_t* last : {c->kev_used ? &c->kev[c->kev_used - 1] : nullptr,
+                           c->dev_used ? &c->dev[c->dev_used - 1] : nullptr}) {
+    if (!last) continue;
+    const hipError_t e = hipEventSynchronize(*last);
+    if (e != hipSuccess) return e;
+  }
   for (size_t i = 0; i + 1 < c->kev_used; i += 2) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, c->kev[i], c->kev[i + 1]) != hipSuccess) continue;
@@ -555,7 +578,13 @@ void collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, double& 
       accum_ms += ms;
     }
   }
+  for (size_t i = 0; i + 1 < c->dev_used; i += 2) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->dev[i], c->dev[i + 1]) == hipSuccess) dispatch_ms += ms;
+  }
   c->kev_used = 0;
+  c->dev_used = 0;
+  return hipSuccess;
 }
 
 // Tile tables and record buffers the caller provides (ffm_iterate_noalloc):
@@ -901,12 +930,15 @@ int ffm_ctx_destroy(ffm_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->table, &c->pal_u32, &c->pal_f32, &c->hist, &c->copies, &c->scratch,
-                    &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba, &c->de_h2,
-                    &c->de_inv, &c->count, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items, &c->t_records, &c->t_sorted,
-                    &c->s_fits_d, &c->s_tables, &c->s_ckpt, &c->s_part, &c->s_xcd})
+                    &c->rgba_out, &c->accum, &c->accum2, &c->orbits, &c->counters, &c->rgba,
+                    &c->de_h2, &c->de_inv, &c->count, &c->fx_a, &c->fx_b, &c->fx_rgba,
+                    &c->fx_pyr, &c->t_fill, &c->t_region, &c->t_cap, &c->t_items,
+                    &c->t_records, &c->t_sorted, &c->s_fits_d, &c->s_tables, &c->s_ckpt,
+                    &c->s_part, &c->s_xcd})
     b->release();
   free_batch(c->batch);
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->dev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1061,6 +1093,17 @@ int ffm_iterate_noalloc(ffm_ctx* c, const ffm_view* view, uint64_t walkers, uint
   }
   a.hist = hist;
   HIPCHECK(c, launch_iterate(a, splat_mode, iterate_grid(c, walkers), 256, s));
+  return FFM_OK;
+}
+
+int ffm_downsample2x(ffm_ctx* c, const uint8_t* in, uint8_t* out, int out_w, int out_h,
+                     void* stream) {
+  if (!c) return FFM_ERR_INVALID_ARGUMENT;
+  if (!in || !out) return fail(c, FFM_ERR_INVALID_ARGUMENT, "pixel buffer is NULL");
+  if (out_w < 1 || out_h < 1 || out_w > 16384 || out_h > 16384)
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "output size %dx%d outside 1..16384", out_w, out_h);
+  (void)hipSetDevice(c->device);
+  HIPCHECK(c, launch_downsample2x(in, out, out_w, out_h, pick(c, stream)));
   return FFM_OK;
 }
 
@@ -1426,9 +1469,13 @@ static uint64_t exact_count_after() {
   return 1ull << 18;
 }
 
+// `supersample` 2: the job renders at twice the output size and the final
+// picture (after the colour effects) is boxed down into rgba_out / rgba_dev,
+// which are then (width / 2) x (height / 2) (downsample_2x_firefly,
+// src/export/supersample.rs:117-187).
 static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float* accum_out,
                        ffm_render_stats* stats, uint8_t* rgba_dev = nullptr,
-                       const ffm_effects* fx = nullptr) {
+                       const ffm_effects* fx = nullptr, int supersample = 1) {
   if (!c) return FFM_ERR_INVALID_ARGUMENT;
   if (!job) return fail(c, FFM_ERR_INVALID_ARGUMENT, "job is NULL");
   if (!c->has_flame) return fail(c, FFM_ERR_NO_FLAME, "no flame set");
@@ -1443,6 +1490,9 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   const bool f32h = mode == FFM_SPLAT_F32_GLOBAL || mode == FFM_SPLAT_F32_LDS;
   if (f32h && job->filter_radius > 0.0f)
     return fail(c, FFM_ERR_INVALID_ARGUMENT, "the histogram blur reads a u32 histogram");
+  if (supersample == 2 && (job->view.width % 2 || job->view.height % 2))
+    return fail(c, FFM_ERR_INVALID_ARGUMENT, "a 2x supersampled render needs an even size (%dx%d)",
+                job->view.width, job->view.height);
   // the tone map's parameters are checked before any work, also when the
   // render only returns the accumulator
   if ((st = check_tonemap(c, job->tonemap))) return st;
@@ -1486,12 +1536,21 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   uint32_t batch_count = 0, empty = 0;
   const uint32_t stall_limit = job->burn_in / std::max(job->ipt, 1u) + 4;
   const uint32_t bs = job->batch_size ? job->batch_size : 4;
-  float iterate_ms = 0.0f;
-  double walker_ms = 0.0, accum_ms = 0.0;
+  double iterate_ms = 0.0, walker_ms = 0.0, accum_ms = 0.0;
   uint64_t launches = 0;
   c->kev_used = 0;
+  c->dev_used = 0;
+  // Dispatches are enqueued back to back: the host never waits for one to
+  // finish before launching the next (the sample count each adds is the
+  // host's OrbitFuses model), only every SYNC_DISPATCHES dispatches, to read
+  // the timing events back and to poll for cancellation with little queued.
+  constexpr uint64_t SYNC_DISPATCHES = 16;
   for (;;) {
-    if (c->cancel.load()) return fail(c, FFM_ERR_CANCELLED, "Render cancelled");
+    if (c->cancel.load()) {
+      (void)hipStreamSynchronize(s);
+      c->kev_used = c->dev_used = 0;
+      return fail(c, FFM_ERR_CANCELLED, "Render cancelled");
+    }
     if (batch_count == 0) {
       if (mode == FFM_SPLAT_U32_XCD)
         HIPCHECK(c, launch_clear_u32(c->copies.p, words * XCD_COPIES, s));
@@ -1500,7 +1559,7 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     }
     a.seed = job->seed_base + (uint32_t)k * job->seed_stride;
     a.fresh = k == 0;
-    HIPCHECK(c, hipEventRecord(c->ev0, s));
+    HIPCHECK(c, mark(c, s, 2, true));
     if (mode == FFM_SPLAT_U32_LDS || mode == FFM_SPLAT_F32_LDS) {
       if ((st = lds_splat(c, a, reinterpret_cast<uint32_t*>(c->hist.p), s, f32h))) return st;
     } else {
@@ -1508,7 +1567,7 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
       HIPCHECK(c, launch_iterate(a, mode, grid, 256, s));
       HIPCHECK(c, mark(c, s, 0, false));
     }
-    HIPCHECK(c, hipEventRecord(c->ev1, s));
+    HIPCHECK(c, mark(c, s, 2, false));
     uint64_t plotted;
     if (job->persistent_orbits) {
       plotted = fuses.dispatch((uint32_t)job->walkers, job->ipt, job->burn_in, a.refresh_period,
@@ -1548,11 +1607,8 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
       batch_count = 0;
       batch_samples = 0;
     }
-    HIPCHECK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    iterate_ms += ms;
-    collect_timings(c, walker_ms, launches, accum_ms);
+    if (done || k % SYNC_DISPATCHES == 0)
+      HIPCHECK(c, collect_timings(c, walker_ms, launches, accum_ms, iterate_ms));
     if (done) break;
   }
   float* final_acc = reinterpret_cast<float*>(c->accum.p);
@@ -1573,12 +1629,23 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
     HIPCHECK(c, hipMemcpyAsync(accum_out, final_acc, npix * 16, hipMemcpyDeviceToHost, s));
   }
   if (rgba_out || rgba_dev) {
-    if (!rgba_dev && (st = ensure(c, c->rgba, npix * 4, "pixels"))) return st;
-    uint8_t* pix = rgba_dev ? rgba_dev : reinterpret_cast<uint8_t*>(c->rgba.p);
+    const bool ss = supersample == 2;
+    const size_t out_px = ss ? npix / 4 : npix;
+    if ((!rgba_dev || ss) && (st = ensure(c, c->rgba, npix * 4, "pixels"))) return st;
+    uint8_t* pix = rgba_dev && !ss ? rgba_dev : reinterpret_cast<uint8_t*>(c->rgba.p);
     HIPCHECK(c, launch_tonemap(final_acc, pix, W, H, tmc, s));
     if ((st = run_color_chain(c, effects, pix, W, H, s))) return st;
+    if (ss) {
+      uint8_t* dst = rgba_dev;
+      if (!dst) {
+        if ((st = ensure(c, c->rgba_out, out_px * 4, "downsampled pixels"))) return st;
+        dst = reinterpret_cast<uint8_t*>(c->rgba_out.p);
+      }
+      HIPCHECK(c, launch_downsample2x(pix, dst, W / 2, H / 2, s));
+      pix = dst;
+    }
     if (rgba_out) {
-      hipError_t e = hipMemcpyAsync(rgba_out, pix, npix * 4, hipMemcpyDeviceToHost, s);
+      hipError_t e = hipMemcpyAsync(rgba_out, pix, out_px * 4, hipMemcpyDeviceToHost, s);
       if (e != hipSuccess)
         return fail(c, FFM_ERR_PIXEL_READ_FAILED, "pixel readback: %s", hipGetErrorString(e));
     }
@@ -1620,7 +1687,8 @@ int ffm_render_config(ffm_ctx* c, const char* text, size_t len, int width, int h
                                &sc->palette_xf)))
     return st;
   if ((st = check_effects(c, sc->effects))) return st;
-  return render_impl(c, &job, rgba_out, nullptr, stats, nullptr, &sc->effects);
+  return render_impl(c, &job, rgba_out, nullptr, stats, nullptr, &sc->effects,
+                     (flags & FFM_RENDER_SUPERSAMPLE_2X) ? 2 : 1);
 }
 
 int ffm_render_to_device(ffm_ctx* c, const ffm_job* job, void* rgba_dev, ffm_render_stats* stats) {

@@ -1782,29 +1782,56 @@ int ffm_scene_job(const ffm_scene* s, int width, int height, uint64_t iterations
             "camera through the 2D chaos game");
     return FFM_ERR_UNSUPPORTED;
   }
+  // 2x antialiasing renders twice the size with 4x the samples and twice
+  // the filter radius (scale_config_for_supersample,
+  // src/export/supersample.rs:54-58); the caller boxes the picture down
+  const bool ss = (flags & FFM_RENDER_SUPERSAMPLE_2X) != 0;
+  if (ss && (width > 16384 || height > 16384)) {
+    put_err(err, err_cap, "a 2x supersampled render of " + std::to_string(width) + "x" +
+                              std::to_string(height) + " exceeds 32768 pixels a side");
+    return FFM_ERR_TOO_LARGE;
+  }
   memset(j, 0, sizeof *j);
-  j->view.width = width;
-  j->view.height = height;
+  j->view.width = ss ? 2 * width : width;
+  j->view.height = ss ? 2 * height : height;
   j->view.zoom = s->zoom;
   j->view.pan_x = (float)s->pan_x;
   j->view.pan_y = (float)s->pan_y;
   j->view.rotation = s->rotation;
-  // RenderJob::new (src/renderer/render.rs:111-138): burn-in 20; the
-  // dispatch is 128 workgroups of 64 walkers; ipt is pyfae's default 1024
-  j->walkers = 8192;
+  // RenderJob::new (src/renderer/render.rs:111-138): burn-in 20; ipt is
+  // pyfae's default 1024; the dispatch is 128 workgroups of 64 walkers. With
+  // FFM_RENDER_FILL_GPU a dispatch walks enough walkers to fill the GPU:
+  // about half the budget in one dispatch, 8192 to 2^20, a multiple of 8192.
   j->ipt = ipt ? ipt : 1024;
   j->burn_in = 20;
+  uint64_t target = iterations ? iterations : s->max_iterations;
+  if (ss) target = target > UINT64_MAX / 4 ? UINT64_MAX : target * 4;
+  j->persistent_orbits = s->reset_orbits ? 0 : 1;
+  if (!(flags & FFM_RENDER_FILL_GPU)) {
+    j->walkers = 8192;
+  } else {
+    // at least two dispatches; as few walkers (a multiple of 8192) as plot
+    // the budget in that many: a walker plots ipt - burn_in samples in its
+    // first dispatch and, with persistent orbits, ipt in every later one
+    const uint64_t first = j->ipt > j->burn_in ? j->ipt - j->burn_in : 1;
+    const uint64_t later = j->persistent_orbits ? j->ipt : first;
+    const uint64_t most = 1u << 20;
+    uint64_t n = 2;
+    if (target / most > first + later) n = 1 + (target / most - first + later - 1) / later;
+    const uint64_t each = first + (n - 1) * later;
+    const uint64_t w = (target / each + 8191) / 8192 * 8192;
+    j->walkers = std::min<uint64_t>(std::max<uint64_t>(w, 8192), most);
+  }
   // the deterministic seed schedule 12345 + k * 2654435761
   // (src/renderer/compute_kernel.rs:3541-3554); a config without
   // deterministic_rng starts it from a random base
   j->seed_base = 12345u;
   if (!s->deterministic_rng) j->seed_base = std::random_device{}();
   j->seed_stride = 2654435761u;
-  j->target_samples = iterations ? iterations : s->max_iterations;
+  j->target_samples = target;
   j->batch_size = 4;
   j->splat_mode = FFM_SPLAT_U32_LDS;
-  j->persistent_orbits = s->reset_orbits ? 0 : 1;
-  j->filter_radius = s->filter_radius;
+  j->filter_radius = ss ? 2.0f * s->filter_radius : s->filter_radius;
   j->filter_blur_edges = s->filter_blur_edges;
   j->tonemap = s->tonemap;
   // transparent_mode (src/renderer/compute_kernel.rs:3771-3784)

@@ -500,3 +500,62 @@ hipError_t launch_batch_finish(const uint32_t* hist, uint8_t* rgba, const BatchP
 }
 
 }  // namespace ffm
+
+// ------------------------------------------------------------ supersample
+
+namespace ffm {
+
+// downsample_2x_firefly (src/export/supersample.rs:117-187): each output
+// pixel is the 2x2 box of its four RGBA8 samples (row-major order
+// (0,0), (1,0), (0,1), (1,1)), with every sample whose luminance proxy
+// r + 2g + b exceeds 4 x the quad's median + 64 scaled down to that cap
+// first; alpha is averaged unclamped; the mean rounds half away from zero.
+// The median of four is the mean of the two middle values: the larger of
+// the pairwise minima and the smaller of the pairwise maxima.
+__global__ __launch_bounds__(256) void downsample2x_kernel(const uchar4* __restrict__ in,
+                                                           uchar4* __restrict__ out, int ow,
+                                                           int oh) {
+  const int w2 = 2 * ow;
+  const size_t n = (size_t)ow * oh;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const int ox = (int)(p % (size_t)ow), oy = (int)(p / (size_t)ow);
+    const size_t r0 = (size_t)(2 * oy) * w2 + 2 * ox, r1 = r0 + w2;
+    const uchar4 q[4] = {in[r0], in[r0 + 1], in[r1], in[r1 + 1]};
+    float s[4][4], lum[4];
+    for (int i = 0; i < 4; i++) {
+      s[i][0] = (float)q[i].x;
+      s[i][1] = (float)q[i].y;
+      s[i][2] = (float)q[i].z;
+      s[i][3] = (float)q[i].w;
+      lum[i] = s[i][0] + 2.0f * s[i][1] + s[i][2];
+    }
+    const float lo = fmaxf(fminf(lum[0], lum[1]), fminf(lum[2], lum[3]));
+    const float hi = fminf(fmaxf(lum[0], lum[1]), fmaxf(lum[2], lum[3]));
+    const float median = 0.5f * (lo + hi);
+    const float cap = median * 4.0f + 64.0f;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 4; i++) {
+      const float scale = lum[i] > cap ? cap / lum[i] : 1.0f;
+      acc[0] += s[i][0] * scale;
+      acc[1] += s[i][1] * scale;
+      acc[2] += s[i][2] * scale;
+      acc[3] += s[i][3];
+    }
+    uchar4 o;
+    o.x = (uint8_t)fminf(fmaxf(roundf(acc[0] * 0.25f), 0.0f), 255.0f);
+    o.y = (uint8_t)fminf(fmaxf(roundf(acc[1] * 0.25f), 0.0f), 255.0f);
+    o.z = (uint8_t)fminf(fmaxf(roundf(acc[2] * 0.25f), 0.0f), 255.0f);
+    o.w = (uint8_t)fminf(fmaxf(roundf(acc[3] * 0.25f), 0.0f), 255.0f);
+    out[p] = o;
+  }
+}
+
+hipError_t launch_downsample2x(const uint8_t* in, uint8_t* out, int ow, int oh, hipStream_t s) {
+  const size_t n = (size_t)ow * oh;
+  hipLaunchKernelGGL(downsample2x_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const uchar4*>(in), reinterpret_cast<uchar4*>(out), ow, oh);
+  return hipGetLastError();
+}
+
+}  // namespace ffm

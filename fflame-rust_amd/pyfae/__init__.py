@@ -324,13 +324,17 @@ def scene_stops(sc):
 
 
 def scene_job(sc, width, height, *, iterations=None, iterations_per_thread=None, as_2d=False,
-              transparent=False, premultiplied=False):
-    """The ffm_job the reference's export builds for a scene at width x height."""
+              transparent=False, premultiplied=False, supersample=False, fill_gpu=False):
+    """The ffm_job the reference's export builds for a scene at width x height
+    (with `supersample`, the 2x render's job: twice the size, 4x the budget;
+    with `fill_gpu`, a GPU-filling walker count a dispatch)."""
     j = Job()
     err = C.create_string_buffer(512)
     flags = ((_abi.RENDER_3D_AS_2D if as_2d else 0)
              | (_abi.RENDER_TRANSPARENT if transparent else 0)
-             | (_abi.RENDER_PREMULTIPLIED if premultiplied else 0))
+             | (_abi.RENDER_PREMULTIPLIED if premultiplied else 0)
+             | (_abi.RENDER_SUPERSAMPLE_2X if supersample else 0)
+             | (_abi.RENDER_FILL_GPU if fill_gpu else 0))
     st = library().ffm_scene_job(C.byref(sc), int(width), int(height), int(iterations or 0),
                                  int(iterations_per_thread or 0), flags, C.byref(j), err, 512)
     if st != 0:
@@ -454,7 +458,8 @@ class Engine:
 
     # -- whole renders
     def render(self, config, width=None, height=None, *, iterations=None,
-               iterations_per_thread=None, transparent=False, premultiplied=False, as_2d=False):
+               iterations_per_thread=None, transparent=False, premultiplied=False, as_2d=False,
+               supersample=False, fill_gpu=False):
         """Render a config (python/pyfae/__init__.py:201-221): `config` is a
         .fflame / .fae JSON or .flame XML text, a dict or a path, rendered at
         `width` x `height` with the config's flame, palette, view and tone map
@@ -462,7 +467,12 @@ class Engine:
         `iterations_per_thread`: default 1024; `transparent`: alpha instead
         of the background, straight or `premultiplied`). Returns an Image.
         `as_2d` renders a 3D-mode config (every .flame import) whose camera
-        is flat through the 2D chaos game.
+        is flat through the 2D chaos game. `supersample` is the export's 2x
+        antialiasing (twice the size, 4x the samples, boxed down with the
+        firefly clamp); `fill_gpu` dispatches a GPU-filling walker count
+        instead of the reference's 8192 (the same picture within the noise
+        for a flame that mixes within the burn-in; not for one whose
+        picture is its walkers' start points).
 
         Given an ffm_job instead, renders the current flame and returns
         (pixels, stats)."""
@@ -477,7 +487,9 @@ class Engine:
         ipt = int(iterations_per_thread or 0)
         flags = ((_abi.RENDER_3D_AS_2D if as_2d else 0)
                  | (_abi.RENDER_TRANSPARENT if transparent else 0)
-                 | (_abi.RENDER_PREMULTIPLIED if premultiplied else 0))
+                 | (_abi.RENDER_PREMULTIPLIED if premultiplied else 0)
+                 | (_abi.RENDER_SUPERSAMPLE_2X if supersample else 0)
+                 | (_abi.RENDER_FILL_GPU if fill_gpu else 0))
         self._check(self.lib.ffm_render_config(
             self.ctx, raw, len(raw), int(width), int(height), int(iterations or 0), ipt, flags,
             out.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(stats)))
@@ -577,6 +589,12 @@ class Engine:
             self.ctx, C.byref(view), int(walkers), int(ipt), int(burn_in),
             int(seed) & 0xFFFFFFFF, _ptr(orbits), int(bool(fresh)), _ptr(hist),
             int(splat_mode), _ptr(plotted), _ptr(inframe), _ptr(stream)))
+
+    def downsample2x(self, src, dst, out_width, out_height, stream=None):
+        """Box (2w, 2h) RGBA8 `src` down into (w, h) `dst` with the firefly
+        clamp (device tensors)."""
+        self._check(self.lib.ffm_downsample2x(self.ctx, _ptr(src), _ptr(dst), int(out_width),
+                                              int(out_height), _ptr(stream)))
 
     def trace(self, view, walker, seed, burn_in, steps, out, stream=None):
         self._check(self.lib.ffm_trace(self.ctx, C.byref(view), int(walker), int(seed) & 0xFFFFFFFF,

@@ -13,6 +13,8 @@ SCENE_NAME = 128
 RENDER_3D_AS_2D = 1
 RENDER_TRANSPARENT = 2
 RENDER_PREMULTIPLIED = 4
+RENDER_SUPERSAMPLE_2X = 8
+RENDER_FILL_GPU = 16
 
 OK = 0
 ERR_INVALID_ARGUMENT = -1
@@ -260,6 +262,7 @@ SIGNATURES = [
     ("ffm_tonemap", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_double,
                               C.POINTER(TonemapParams), _vp]),
     ("ffm_fold_xcd", C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp]),
+    ("ffm_downsample2x", C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     ("ffm_batch_render", C.c_int, [_vp, C.c_int, C.POINTER(FlameDesc), C.POINTER(_fp),
                                    C.POINTER(C.c_int), C.POINTER(Job), C.POINTER(_u8p),
                                    C.POINTER(RenderStats)]),

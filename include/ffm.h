@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 8
+#define FFM_ABI_VERSION 9
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -333,6 +333,14 @@ int ffm_iterate_noalloc(ffm_ctx* ctx, const ffm_view* view, uint64_t walkers, ui
                         uint32_t burn_in, uint32_t seed, void* orbits, int fresh, void* hist,
                         int splat_mode, uint64_t* plotted, uint64_t* inframe, void* stream);
 
+/* downsample_2x_firefly (src/export/supersample.rs:117-187): the RGBA8
+ * picture `in` of (2 out_w) x (2 out_h) boxed 2x2 down into `out` (out_w x
+ * out_h), each quad's samples brighter (r + 2g + b) than 4x the quad's
+ * median + 64 first scaled down to that, alpha averaged unclamped. Device
+ * pointers. */
+int ffm_downsample2x(ffm_ctx* ctx, const uint8_t* in, uint8_t* out, int out_w, int out_h,
+                     void* stream);
+
 /* Fold the per-XCD private histograms of FFM_SPLAT_U32_XCD into `hist`. */
 int ffm_fold_xcd(ffm_ctx* ctx, uint32_t* hist, int width, int height, void* stream);
 
@@ -564,10 +572,26 @@ int ffm_palette_set_xf(ffm_ctx* ctx, const float* stops_prgb, int n_stops, int t
  * flat camera) renders through the 2D chaos game only with
  * FFM_RENDER_3D_AS_2D. FFM_RENDER_TRANSPARENT and FFM_RENDER_PREMULTIPLIED
  * are pyfae render's transparent= / premultiplied= (transparent_mode 1 or
- * 2, src/renderer/compute_kernel.rs:3771-3784). */
+ * 2, src/renderer/compute_kernel.rs:3771-3784).
+ * FFM_RENDER_SUPERSAMPLE_2X is the export's 2x antialiasing
+ * (supersample_plan / scale_config_for_supersample,
+ * src/export/supersample.rs:54-100): the job renders at twice the width and
+ * height with 4x the sample budget and twice the histogram filter radius,
+ * and ffm_render_config boxes its final picture down to the requested size
+ * (ffm_downsample2x). By default a job dispatches the reference's 8192
+ * walkers (128 workgroups of 64, src/renderer/render.rs:350-351): the same
+ * walker streams, seeds and orbits as the reference's render, whose picture
+ * depends on that count wherever the walk mixes slowly (the reference's own
+ * visual config, an identity map, draws exactly its 8192 start points).
+ * FFM_RENDER_FILL_GPU dispatches enough walkers to fill the GPU instead
+ * (about half the budget in one dispatch, 8192 to 2^20): for a flame whose
+ * walk mixes within the burn-in, the same picture within the Monte-Carlo
+ * noise in a few dispatches. */
 #define FFM_RENDER_3D_AS_2D 1u
 #define FFM_RENDER_TRANSPARENT 2u
 #define FFM_RENDER_PREMULTIPLIED 4u
+#define FFM_RENDER_SUPERSAMPLE_2X 8u
+#define FFM_RENDER_FILL_GPU 16u
 
 /* The job the reference's export builds for a config (RenderJob::new,
  * src/renderer/render.rs:111-138, with pyfae's render arguments,

@@ -1376,6 +1376,48 @@ float orc_curve_sample(const float* lut, float c) {
   return lut[i] * (1.0f - f) + lut[i + 1] * f;
 }
 
+/* downsample_2x_firefly (src/export/supersample.rs:117-187): RGBA8 of
+ * (2 ow) x (2 oh) boxed 2x2 down to ow x oh. Samples in the order (0,0),
+ * (1,0), (0,1), (1,1); luminance proxy r + 2g + b; the median of the four
+ * is the mean of the two middle values (sorted); a sample above
+ * cap = 4 median + 64 is scaled by cap / lum; alpha is averaged unclamped;
+ * (acc / 4) rounds half away from zero, clamped to 0..255. */
+void orc_downsample2x(const uint8_t* in, uint8_t* out, int ow, int oh) {
+  const int w2 = 2 * ow;
+  for (int oy = 0; oy < oh; oy++)
+    for (int ox = 0; ox < ow; ox++) {
+      static const int dxy[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+      float smp[4][4], lum[4], srt[4];
+      for (int i = 0; i < 4; i++) {
+        const uint8_t* q = in + 4 * ((size_t)(2 * oy + dxy[i][1]) * w2 + 2 * ox + dxy[i][0]);
+        for (int c = 0; c < 4; c++) smp[i][c] = (float)q[c];
+        lum[i] = smp[i][0] + 2.0f * smp[i][1] + smp[i][2];
+        srt[i] = lum[i];
+      }
+      for (int i = 1; i < 4; i++) /* insertion sort of four */
+        for (int j = i; j > 0 && srt[j - 1] > srt[j]; j--) {
+          float t = srt[j];
+          srt[j] = srt[j - 1];
+          srt[j - 1] = t;
+        }
+      const float median = 0.5f * (srt[1] + srt[2]);
+      const float cap = median * 4.0f + 64.0f;
+      float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int i = 0; i < 4; i++) {
+        const float scale = lum[i] > cap ? cap / lum[i] : 1.0f;
+        acc[0] += smp[i][0] * scale;
+        acc[1] += smp[i][1] * scale;
+        acc[2] += smp[i][2] * scale;
+        acc[3] += smp[i][3];
+      }
+      uint8_t* o = out + 4 * ((size_t)oy * ow + ox);
+      for (int c = 0; c < 4; c++) {
+        float v = roundf(acc[c] * 0.25f);
+        o[c] = (uint8_t)(v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v));
+      }
+    }
+}
+
 void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,
                  double samples_in_buffer, const ffm_tonemap_params* tp) {
   float lut[256];

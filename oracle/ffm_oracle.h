@@ -114,6 +114,7 @@ float orc_curve_eval(const float* px, const float* py, int n, float x);
 void orc_curve_lut(const ffm_tonemap_params* tp, float* lut);
 float orc_curve_sample(const float* lut, float c);
 float orc_step_term_scale(void);
+void orc_downsample2x(const uint8_t* in, uint8_t* out, int ow, int oh);
 void orc_use_libm(int on);
 float orc_fm_atan2(float y, float x);
 void orc_tonemap(const float* accum, uint8_t* rgba, int W, int H, float zoom,

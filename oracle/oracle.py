@@ -97,6 +97,7 @@ def lib():
             "orc_curve_lut": (None, [C.c_void_p, _fp]),
             "orc_curve_sample": (C.c_float, [_fp, C.c_float]),
             "orc_step_term_scale": (C.c_float, []),
+            "orc_downsample2x": (None, [_u8p, _u8p, C.c_int, C.c_int]),
             "orc_use_libm": (None, [C.c_int]),
             "orc_fm_atan2": (C.c_float, [C.c_float, C.c_float]),
             "orc_tonemap": (None, [_fp, _u8p, C.c_int, C.c_int, C.c_float, C.c_double,
@@ -321,6 +322,15 @@ def curve_sample(lut, c):
     """The curve table `lut` (256 float32) read by the tone map at channel c."""
     lut = np.ascontiguousarray(lut, dtype=np.float32)
     return lib().orc_curve_sample(fp(lut), float(c))
+
+
+def downsample2x(img):
+    """(2h, 2w, 4) uint8 -> (h, w, 4): the export's 2x box with the firefly clamp."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    h, w = img.shape[0] // 2, img.shape[1] // 2
+    out = np.zeros((h, w, 4), dtype=np.uint8)
+    lib().orc_downsample2x(img.ctypes.data_as(_u8p), out.ctypes.data_as(_u8p), w, h)
+    return out
 
 
 def tonemap(acc, zoom, samples, tp):
