@@ -5,7 +5,8 @@
 Default (the headline): a step is one whole render of the single4k workload
 through libffm.so: 6 transforms + a final (3 variations each), 3840x2160,
 2^24 walkers x 256 iterations (one dispatch), accumulate, density estimation
-(radius 9 -> 0, curve 0.4) and the log tonemap, pixels left in HBM. `value`
+(radius 9 -> 0, curve 0.4), the log tonemap and the 2x box filter with the
+firefly clamp down to 1920x1080, pixels left in HBM. `value`
 is chaos-game samples (walker iterations) per second over exactly K timed
 steps after W warmup steps.
 
@@ -113,34 +114,44 @@ def recorded_traffic(kernel, splat_mode, walkers):
 
 # ------------------------------------------------------------- one flame
 
-def time_renders(eng, job, rgba, steps, warmup):
-    """Render `warmup` then exactly `steps` times; (seconds, [stats])."""
+def time_renders(eng, job, rgba, steps, warmup, out=None):
+    """Render `warmup` then exactly `steps` times (each boxed down 2x into
+    `out` when given); (seconds, [stats])."""
     import torch
+
+    def step():
+        st = eng.render_to_device(job, rgba)
+        if out is not None:  # on the engine's stream, after the render's tonemap
+            eng.downsample2x(rgba, out, out.shape[1], out.shape[0])
+        return st
+
     for _ in range(warmup):
-        eng.render_to_device(job, rgba)
+        step()
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        stats.append(eng.render_to_device(job, rgba))
+        stats.append(step())
     torch.cuda.synchronize()
     return time.perf_counter() - t0, stats
 
 
-def bench_flame(args, eng, fd, pal, job, workload, metric):
+def bench_flame(args, eng, fd, pal, job, workload, metric, supersample=False):
     import torch
     from pyfae import roofline
     W, H = job.view.width, job.view.height
     eng.set_flame(fd)
     eng.set_palette(pal, 256)
     rgba = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda:0")
+    out_px = (torch.empty((H // 2, W // 2, 4), dtype=torch.uint8, device="cuda:0")
+              if supersample else None)
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    dt, stats = time_renders(eng, job, rgba, args.steps, args.warmup)
+    dt, stats = time_renders(eng, job, rgba, args.steps, args.warmup, out_px)
     samples = job.walkers * job.ipt
     ms_per_step = dt / args.steps * 1e3
     value = samples / (ms_per_step / 1e3)
@@ -169,7 +180,8 @@ def bench_flame(args, eng, fd, pal, job, workload, metric):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload, "splat_mode": job.splat_mode,
-                   "samples_per_step": samples},
+                   "samples_per_step": samples,
+                   "output": "%dx%d" % ((W // 2, H // 2) if supersample else (W, H))},
         "roofline": {"bound": "valu",
                      "kernel": kname,
                      "launch_ms": launch_ms, "samples_per_launch": samples_per_launch,
@@ -490,8 +502,9 @@ def main():
         out, rgba = bench_flame(args, eng, fd, pal, job,
                                 "single4k: 6 xforms + final xform, 3 of 10 variations each, "
                                 "3840x2160, 2^%d walkers x 256 iterations, density estimation "
-                                "9->0 curve 0.4, log tonemap" % args.walkers_log2,
-                                "chaos-game samples/s at 4K")
+                                "9->0 curve 0.4, log tonemap, boxed down 2x to 1920x1080 "
+                                "(firefly clamp)" % args.walkers_log2,
+                                "chaos-game samples/s at 4K", supersample=True)
         progress(f"single4k: {out['value']:.4g} samples/s")
         if not args.no_extras:
             progress("splat compare: global u32 atomics at 4K")
