@@ -9,11 +9,12 @@
 //  1. bin_iterate_kernel: the same walkers as iterate_kernel, but each
 //     in-frame sample becomes a 4-byte record (pixel index inside its
 //     128x64 screen tile | palette texel << 13 | tile-in-bin << 25) staged
-//     in LDS. Every few iterations the workgroup counting-sorts the staged
-//     records by bin (LDS atomics + a block scan), reserves a run in each
-//     bin's region of the record buffer with ONE global atomic per
-//     (flush, bin), and writes the runs bin-contiguously. A bin is one tile
-//     up to TILE_MAX tiles per image, else a strip of 2^k tiles of a row.
+//     in LDS with its rank among the stage's records of its bin (an LDS
+//     atomic on the bin's count when it is staged). Every few iterations
+//     the workgroup reserves a run in each bin's region of the record
+//     buffer with ONE global atomic per (flush, bin) and writes every
+//     staged record to its run at its rank. A bin is one tile up to
+//     TILE_MAX tiles per image, else a strip of 2^k tiles of a row.
 //  2. tile_accum_kernel: one workgroup per (bin, slice of its records)
 //     accumulates the slice, tile by tile, into a 128x64 u32x4 tile held in
 //     LDS (LDS atomics) and folds the tile into the histogram: a plain
@@ -30,17 +31,17 @@ namespace ffm {
 
 constexpr int BIN_BLOCK = 1024;
 // Staged records per workgroup: what the LDS left over by the flame table
-// and palette holds, in steps of 1024, up to STAGE_MAX (record indices
-// stay within the u16 permutation).
+// and palette holds, in steps of 512, up to STAGE_MAX (a rank within a
+// bin stays below 2^16).
 constexpr int STAGE_MIN = 10240, STAGE_MAX = 16384;
 constexpr int MAX_FLUSH_EVERY = 8;  // iterations between fill checks, at most
 constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
 // accum_run adds two channels per 64-bit LDS atomic: a slice's channel sums
 // must not carry out of 32 bits (at most 255, or 100 for density, a record)
 static_assert((unsigned long long)SLICE * 255ull < (1ull << 32), "tile channel sums carry");
-constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4 + 2;  // rec, tr, perm
-// LDS of the counting sort for nb bin slots: cnt, gbase (u32) and lim (u16)
-// per bin, then the scan's wave sums and the fill words
+constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4;  // rec, tr
+// LDS of the bins for nb bin slots: cnt, gbase (u32) and lim (u16) per bin,
+// then 16 spare words and the fill words
 static size_t stage_fixed(int nb) { return (size_t)nb * (4 + 4 + 2) + 16 * 4 + 16; }
 
 // Walker workgroups of WALK_LANES lanes, as many per CU as the LDS holds
@@ -137,9 +138,8 @@ __device__ void block_exclusive_scan(uint32_t* a, int n, uint32_t* wave_sums, ui
 
 struct BinLds {
   uint32_t* rec;    // stage
-  uint32_t* tr;     // stage: tile | rank << 16
-  uint16_t* perm;   // stage
-  uint32_t* cnt;    // bins_lds: records per bin, then their start in the sorted stage
+  uint32_t* tr;     // stage: bin | rank among the stage's records of the bin << 16
+  uint32_t* cnt;    // bins_lds: the stage's records per bin (COUNT_ONLY: the walk's)
   uint32_t* gbase;  // bins_lds: the bin's first record slot of this flush (region + fill)
   uint16_t* lim;    // bins_lds: slots left in the bin's region (at most 65535)
   uint32_t* wsum;   // 16
@@ -179,21 +179,14 @@ __device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t
   __hip_atomic_fetch_add(h + 3, 100u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sort the staged records by tile and write them to their tile regions.
-// Called by every thread of the workgroup.
+// Write the staged records to their bins' regions. Called by every thread
+// of the workgroup, after a barrier that follows the last staging.
 __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* pal) {
   const int tid = threadIdx.x;
   const int n = (int)min(*L.fill, (uint32_t)t.stage);
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
-  __syncthreads();
-  for (int j = tid; j < n; j += blockDim.x) {
-    const uint32_t tile = L.tr[j];
-    const uint32_t r = atomicAdd(&L.cnt[tile], 1u);
-    L.tr[j] = tile | (r << 16);
-  }
-  __syncthreads();
   // reserve each bin's run, and keep where it starts and how much of it
-  // the region holds, so the record loop below reads LDS only
+  // the region holds, so the record loop below reads LDS only; the counts
+  // start over for the next stage
   for (int i = tid; i < t.n_tiles; i += blockDim.x) {
     const uint32_t c = L.cnt[i];
     uint32_t base = 0u, room = 0u;
@@ -205,35 +198,14 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
         base = t.region[i] + old;
         room = min(cap - old, 0xFFFFu);
       }
+      L.cnt[i] = 0u;
     }
     L.gbase[i] = base;
     L.lim[i] = (uint16_t)room;
   }
   __syncthreads();
-  // cnt -> start; more bins than 2 x blockDim go 2 x blockDim at a time
-  // with a running carry
-  if (t.n_tiles <= 2 * (int)blockDim.x) {
-    block_exclusive_scan(L.cnt, t.n_tiles, L.wsum, nullptr);
-  } else {
-    uint32_t carry = 0;
-    for (int c0 = 0; c0 < t.n_tiles; c0 += 2 * (int)blockDim.x) {
-      const int m = min(2 * (int)blockDim.x, t.n_tiles - c0);
-      block_exclusive_scan(L.cnt + c0, m, L.wsum, L.fill + 1);
-      const uint32_t total = L.fill[1];
-      if (c0 > 0)
-        for (int i = tid; i < m; i += blockDim.x) L.cnt[c0 + i] += carry;
-      carry += total;
-      __syncthreads();
-    }
-  }
-  for (int j = tid; j < n; j += blockDim.x) {
-    const uint32_t v = L.tr[j];
-    L.perm[L.cnt[v & 0xFFFFu] + (v >> 16)] = (uint16_t)j;
-  }
-  __syncthreads();
   // ranks below 2^14 (the stage), so `rank < lim` is `fill + rank < cap`
-  for (int i = tid; i < n; i += blockDim.x) {
-    const int j = L.perm[i];
+  for (int j = tid; j < n; j += blockDim.x) {
     const uint32_t v = L.tr[j];
     const int tile = (int)(v & 0xFFFFu);
     const uint32_t rank = v >> 16;
@@ -288,9 +260,8 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
           // in-frame: px, py >= 0, so tile arithmetic is unsigned shifts and masks
           const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, tx = ux / TILE_W;
           const uint32_t tile = __umul24(uy / TILE_H, (uint32_t)t.bins_x) + (tx >> t.group_shift);
-          if (COUNT_ONLY) {
-            atomicAdd(&L.cnt[tile], 1u);
-          } else {
+          const uint32_t r = atomicAdd(&L.cnt[tile], 1u);  // the record's rank in its bin
+          if (!COUNT_ONLY) {
             const uint32_t local = (uy % TILE_H) * TILE_W + ux % TILE_W |
                                    (tx & ((1u << t.group_shift) - 1u)) << REC_SUB_SHIFT;
             // wave-compacted append: one LDS atomic per wave, by the first
@@ -302,7 +273,7 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
             if (rank == 0u) b = atomicAdd(L.fill, (uint32_t)__popcll(m));
             b = __builtin_amdgcn_readfirstlane(b);
             L.rec[b + rank] = local | ((uint32_t)texel << REC_TEXEL_SHIFT);
-            L.tr[b + rank] = tile;
+            L.tr[b + rank] = tile | r << 16;
           }
         }
       }
@@ -326,8 +297,7 @@ __device__ __forceinline__ BinLds bin_lds_views(uint32_t* p, int stage, int nb) 
   BinLds L;
   L.rec = p;
   L.tr = p + stage;
-  L.perm = reinterpret_cast<uint16_t*>(p + 2 * stage);
-  L.cnt = p + 2 * stage + stage / 2;
+  L.cnt = p + 2 * stage;
   L.gbase = L.cnt + nb;
   L.lim = reinterpret_cast<uint16_t*>(L.gbase + nb);
   L.wsum = L.gbase + nb + nb / 2;
