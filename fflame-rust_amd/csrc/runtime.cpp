@@ -522,7 +522,9 @@ int iterate_grid(ffm_ctx* c, uint64_t walkers) {
 }
 
 constexpr int XCD_COPIES = 8;
-constexpr uint64_t RECORD_BUDGET = 1ull << 31;  // staged samples held at once (8 GB)
+// Staged samples held at once: 12 GiB of records (a record's slot is a u32
+// offset, so the buffer stays below 2^32 records).
+constexpr uint64_t RECORD_BUDGET = 3ull << 30;
 constexpr size_t LDS_PER_CU = LDS_DYNAMIC_MAX;
 
 // The LDS tile splat of one dispatch (see tilesplat.hip): walkers in chunks
@@ -642,6 +644,10 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
                 "%u iterations per walker do not fit the %llu-record splat buffer", a.ipt,
                 (unsigned long long)budget);
   uint64_t chunk = std::max<uint64_t>(1, (budget * 3 / 4) / per_walker);
+  // whole rounds of the walker grid: only the last chunk ends on a partial
+  // round, whose idle lanes leave their CUs' workgroups waiting
+  const uint64_t lanes = (uint64_t)std::max(c->num_cu, 1) * per_cu * walk_block;
+  if (chunk > lanes) chunk = chunk / lanes * lanes;
   chunk = std::min<uint64_t>(chunk, a.walkers);
   const uint64_t rec_cap = std::min<uint64_t>(budget, chunk * per_walker * 5 / 4 +
                                                       (uint64_t)n_tiles * 4096);
