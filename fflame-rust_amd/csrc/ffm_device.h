@@ -61,6 +61,13 @@ __device__ __forceinline__ float rng_nextf(Rng& r) {
   return (float)(rng_next(r) >> 8u) * (1.0f / 16777216.0f);
 }
 
+// Pointers into a flame table in global memory, in the constant address
+// space: the table does not change while a kernel runs, so the wave-uniform
+// reads of it (variation ids, final-chain slot lists) are scalar loads,
+// with no VALU address arithmetic and no readfirstlane.
+typedef const __attribute__((address_space(4))) int KInt;
+typedef const __attribute__((address_space(4))) FlameHeader KHeader;
+
 // ----------------------------------------------------------- variations
 
 constexpr float PI_F = 3.14159265359f;        // the reference's literal
@@ -301,7 +308,7 @@ __device__ __forceinline__ void pre_blur(float w, float& x, float& y, Rng& rng) 
 // (main_template.wgsl:355-384). `ids` are the flame's variation ids
 // (uniform across the wave), `row` the transform's row in LDS.
 template <bool EXT>
-__device__ __forceinline__ void apply_xform(const float* __restrict__ row, const int* ids, int nv,
+__device__ __forceinline__ void apply_xform(const float* __restrict__ row, KInt* ids, int nv,
                                             float& x, float& y, Rng& rng) {
   float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
   float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
@@ -330,11 +337,12 @@ __device__ __forceinline__ void apply_xform(const float* __restrict__ row, const
   y = ry;
 }
 
-// The flame as one lane sees it: LDS rows + uniform header values.
+// The flame as one lane sees it: LDS rows, plus the header and variation
+// ids read with scalar loads from the table in global memory.
 struct FlameView {
   const float* rows;         // LDS, (n_xforms + n_finals) * XF_STRIDE
-  const int* ids;            // LDS, n_vars
-  const FlameHeader* hdr;    // LDS copy of the header (final-chain slot lists), or null
+  KInt* ids;                 // the table's variation ids (global), n_vars
+  KHeader* hdr;              // the table's header (final-chain slot lists; global), or null
   const float* xcum;         // LDS xaos selection table (n_xforms^2), or null without xaos
   int n_xforms, n_finals, n_vars;
 };
@@ -342,7 +350,7 @@ struct FlameView {
 // Final transform i of the flame: apply_xform's arithmetic with the loop
 // over the final's non-zero variation slots only (a zero weight adds
 // nothing and draws nothing, so the sum is the same). Every lane applies the
-// same final, so each read is one uniform LDS address.
+// same final, so each row read is one uniform LDS address.
 template <bool EXT>
 __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x, float& y,
                                             Rng& rng) {
@@ -588,8 +596,8 @@ __device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_
   __syncthreads();
   LdsFlame l;
   l.fv.rows = reinterpret_cast<const float*>(lds + hdr_w);
-  l.fv.ids = reinterpret_cast<const int*>(lds + 4);
-  l.fv.hdr = reinterpret_cast<const FlameHeader*>(lds);
+  l.fv.ids = (KInt*)table + 4;
+  l.fv.hdr = (KHeader*)table;
   l.fv.xcum = __builtin_amdgcn_readfirstlane(hdr->xaos)
                   ? l.fv.rows + __builtin_amdgcn_readfirstlane(hdr->xaos_off)
                   : nullptr;

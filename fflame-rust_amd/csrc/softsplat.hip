@@ -94,7 +94,7 @@ __device__ __forceinline__ FlameView soft_load(const SoftArgs& a, const SoftFit&
   __syncthreads();
   FlameView fv;
   fv.rows = rows;
-  fv.ids = reinterpret_cast<const int*>(lds) + 4;
+  fv.ids = (KInt*)(a.tables + F.table_off) + 4;
   fv.hdr = nullptr;  // no final transforms in a soft-splat flame
   fv.xcum = nullptr;  // nor xaos
   fv.n_xforms = nx;
@@ -268,7 +268,7 @@ __device__ __forceinline__ void var_jac(int id, const float* __restrict__ p, T x
 
 // The transform in f64: affine -> weight-gated variation sum in local-index
 // order -> post-affine (the f64 form of apply_xform).
-__device__ __forceinline__ void apply_xform_f64(const float* __restrict__ row, const int* ids,
+__device__ __forceinline__ void apply_xform_f64(const float* __restrict__ row, KInt* ids,
                                                 int nv, double& x, double& y, Rng& rng) {
   const double qx = (double)row[XF_A] * x + (double)row[XF_B] * y + (double)row[XF_E];
   const double qy = (double)row[XF_C] * x + (double)row[XF_D] * y + (double)row[XF_F];
