@@ -39,6 +39,15 @@ struct DevBuf {
 
 }  // namespace
 
+// The estimator's filters for one (max radius, min radius, curve): the
+// squared half-width and normaliser of each (de_table), kept by the
+// context so repeated renders upload them once.
+struct DeTable {
+  float max_r = -1.0f, min_r = -1.0f, curve = -1.0f;
+  int R = 0;
+  std::vector<float> h2, inv;
+};
+
 struct ffm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -58,6 +67,7 @@ struct ffm_ctx {
   DevBuf pal_u32, pal_f32;
   // render buffers
   DevBuf hist, copies, scratch, accum, accum2, orbits, counters, rgba, de_h2, de_inv, count;
+  DeTable de;  // the estimator filters de_h2 and de_inv hold on the device
   DevBuf rgba_out;               // a supersampled render's boxed-down picture
   DevBuf fx_a, fx_b, fx_rgba;    // effect chain ping-pong buffers
   DevBuf fx_pyr;                 // bloom's pyramids, the distance field's coordinates
@@ -634,7 +644,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
       accum_lds_bytes(a.pal_size, f32) > LDS_PER_CU)
     return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
   const uint64_t per_walker = (uint64_t)a.ipt + 1;
-  // records held at once: 2^31 in context buffers; in a caller's scratch
+  // records held at once: RECORD_BUDGET in context buffers; in a caller's scratch
   // what it holds (half of it for bins of several tiles, whose records
   // also need a copy sorted by tile)
   const uint64_t budget =
@@ -746,13 +756,12 @@ struct OrbitFuses {
 };
 
 // flam3 filter table (see ffm_density_estimate in include/ffm.h).
-void de_table(float max_r, float min_r, float curve, std::vector<float>& h2,
-              std::vector<float>& inv, int& R) {
+void de_table(float max_r, float min_r, float curve, DeTable& t) {
   double comp_max = (double)max_r + 1.0, comp_min = (double)min_r + 1.0;
-  h2.clear();
-  inv.clear();
+  t.h2.clear();
+  t.inv.clear();
   const int keep = 100;
-  R = (int)std::ceil(comp_max);
+  t.R = (int)std::ceil(comp_max);
   for (int i = 0; i < 4096; i++) {
     double adj = i < keep ? (double)i : std::pow((double)(i - keep), 1.0 / curve) + keep;
     double h = comp_max / std::pow(adj + 1.0, curve);
@@ -768,10 +777,13 @@ void de_table(float max_r, float min_r, float curve, std::vector<float>& h2,
         double d = std::sqrt((double)(dx * dx + dy * dy)) / h;
         if (d <= 1.0) sum += std::exp(-2.0 * (1.5 * d) * (1.5 * d));
       }
-    h2.push_back((float)(h * h));
-    inv.push_back((float)(1.0 / sum));
+    t.h2.push_back((float)(h * h));
+    t.inv.push_back((float)(1.0 / sum));
     if (last) break;
   }
+  t.max_r = max_r;
+  t.min_r = min_r;
+  t.curve = curve;
 }
 
 // A tone curve the tone map can sample: 2..FFM_MAX_CURVE_POINTS finite
@@ -1154,20 +1166,23 @@ int ffm_density_estimate(ffm_ctx* c, const float* in, float* out, int width, int
     return fail(c, FFM_ERR_INVALID_ARGUMENT,
                 "density estimation needs 0 < max_radius <= 15, min_radius >= 0, curve > 0");
   (void)hipSetDevice(c->device);
-  std::vector<float> h2, inv;
-  int R;
-  de_table(max_radius, min_radius, curve, h2, inv, R);
-  int st;
-  if ((st = ensure(c, c->de_h2, h2.size() * 4, "estimator table"))) return st;
-  if ((st = ensure(c, c->de_inv, inv.size() * 4, "estimator table"))) return st;
   hipStream_t s = pick(c, stream);
-  HIPCHECK(c, hipMemcpyAsync(c->de_h2.p, h2.data(), h2.size() * 4, hipMemcpyHostToDevice, s));
-  HIPCHECK(c, hipMemcpyAsync(c->de_inv.p, inv.data(), inv.size() * 4, hipMemcpyHostToDevice, s));
+  DeTable& t = c->de;
+  if (t.max_r != max_radius || t.min_r != min_radius || t.curve != curve) {
+    // new filters; the device copy of the old ones may still be read by
+    // work queued earlier on any stream
+    HIPCHECK(c, hipDeviceSynchronize());
+    de_table(max_radius, min_radius, curve, t);
+    int st;
+    if ((st = ensure(c, c->de_h2, t.h2.size() * 4, "estimator table"))) return st;
+    if ((st = ensure(c, c->de_inv, t.inv.size() * 4, "estimator table"))) return st;
+    HIPCHECK(c, hipMemcpy(c->de_h2.p, t.h2.data(), t.h2.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->de_inv.p, t.inv.data(), t.inv.size() * 4, hipMemcpyHostToDevice));
+  }
   HIPCHECK(c, launch_density_estimate(in, out, width, height,
                                       reinterpret_cast<const float*>(c->de_h2.p),
-                                      reinterpret_cast<const float*>(c->de_inv.p), (int)h2.size(),
-                                      R, curve, s));
-  HIPCHECK(c, hipStreamSynchronize(s));  // the host tables are released on return
+                                      reinterpret_cast<const float*>(c->de_inv.p),
+                                      (int)t.h2.size(), t.R, curve, s));
   return FFM_OK;
 }
 
@@ -1510,7 +1525,7 @@ static int render_impl(ffm_ctx* c, const ffm_job* job, uint8_t* rgba_out, float*
   const size_t npix = (size_t)W * H, words = npix * 4;
   if ((st = ensure(c, c->hist, words * 4, "histogram"))) return st;
   if ((st = ensure(c, c->accum, npix * 16, "accumulator"))) return st;
-  if (job->filter_radius > 0.0f && (st = ensure(c, c->scratch, words * 8, "blur scratch"))) return st;
+  if (job->filter_radius > 0.0f && (st = ensure(c, c->scratch, words * 4, "blur scratch"))) return st;
   if (mode == FFM_SPLAT_U32_XCD &&
       (st = ensure(c, c->copies, words * 4 * XCD_COPIES, "per-XCD histograms")))
     return st;

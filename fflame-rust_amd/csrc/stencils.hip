@@ -98,10 +98,11 @@ hipError_t launch_accumulate(const void* hist, int hist_is_f32, const float* pre
 // ------------------------------------------------------------------ blur
 
 // shaders/histogram_blur.wgsl:39-119, one pass. `center` is the original
-// primary buffer, the joint-bilateral reference in both passes.
-__global__ __launch_bounds__(256) void blur_kernel(const uint4* __restrict__ in,
-                                                   uint4* __restrict__ out,
-                                                   const uint4* __restrict__ center, int W, int H,
+// primary buffer, the joint-bilateral reference in both passes. `out` may
+// be `center`: a thread reads its own pixel's centre before it writes that
+// pixel, and no other thread reads it.
+__global__ __launch_bounds__(256) void blur_kernel(const uint4* __restrict__ in, uint4* out,
+                                                   const uint4* center, int W, int H,
                                                    float radius_pixels, float density_sigma,
                                                    int direction) {
   const int KMAX = 8;
@@ -142,27 +143,18 @@ __global__ __launch_bounds__(256) void blur_kernel(const uint4* __restrict__ in,
   }
 }
 
-__global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
-}
-
-// H: hist -> scratch (centre from hist). V: scratch -> tmp (centre from hist),
-// where tmp is the second half of `scratch` (the caller gives 2x room), then
-// tmp -> hist.
+// H: hist -> scratch (centre from hist). V: scratch -> hist in place (the
+// centre is each pixel's own word of hist, read before it is written).
 hipError_t launch_blur(uint32_t* hist, uint32_t* scratch, int W, int H, float sigma,
                        float density_sigma, hipStream_t s) {
   size_t npix = (size_t)W * H;
   int grid = grid_for(npix, 256);
   uint4* h = reinterpret_cast<uint4*>(hist);
   uint4* sc = reinterpret_cast<uint4*>(scratch);
-  uint4* tmp = sc + npix;
   hipLaunchKernelGGL(blur_kernel, dim3(grid), dim3(256), 0, s, h, sc, h, W, H, sigma,
                      density_sigma, 0);
-  hipLaunchKernelGGL(blur_kernel, dim3(grid), dim3(256), 0, s, sc, tmp, h, W, H, sigma,
+  hipLaunchKernelGGL(blur_kernel, dim3(grid), dim3(256), 0, s, sc, h, h, W, H, sigma,
                      density_sigma, 1);
-  hipLaunchKernelGGL(copy_u4_kernel, dim3(grid), dim3(256), 0, s, h, tmp, npix);
   return hipGetLastError();
 }
 
@@ -172,7 +164,9 @@ hipError_t launch_blur(uint32_t* hist, uint32_t* scratch, int W, int H, float si
 // c uses filter idx(c) with squared half-width h2[idx] and normaliser
 // inv_norm[idx]; q reaches p when |p-q|^2 <= h2, with weight
 // exp(-4.5 |p-q|^2 / h2) * inv_norm. Each block gathers a 16x16 output
-// tile from an LDS tile holding the (16+2R)^2 sources.
+// tile from an LDS tile holding the (16+2R)^2 sources, over the taps of the
+// disc |p-q|^2 <= R^2 (no filter reaches further: h2 <= R^2), which is the
+// same for every thread.
 constexpr int DE_T = 16;
 constexpr int DE_RMAX = 16;
 
@@ -222,6 +216,7 @@ __global__ __launch_bounds__(256) void de_kernel(const float4* __restrict__ in,
   for (int dy = -R; dy <= R; dy++) {
     const int row = (ty + R + dy) * S + tx + R;
     for (int dx = -R; dx <= R; dx++) {
+      if (dx * dx + dy * dy > R * R) continue;
       const int i = row + dx;
       const float d2 = (float)(dx * dx + dy * dy);
       if (d2 <= fh2[i]) {
@@ -280,6 +275,24 @@ __device__ __forceinline__ float curve_sample(const float* lut, float c) {
   return lut[i] * (1.0f - f) + lut[j] * f;
 }
 
+// The tone map's transcendentals and quotients on the hardware's
+// v_log_f32 / v_exp_f32 / v_rcp_f32 (about 1 ulp each): x^g =
+// exp2(g log2 x), log10 x = log2 x log10(2), a / b = a rcp(b). The output
+// is 8-bit, a level being 1/255 of the range, so these few-ulp differences
+// from the library forms move a channel by at most one level (at a rounding
+// tie). tm_pow keeps powf's special values on the tone map's inputs: x = 0
+// gives 0 for g > 0 and +inf for g < 0, g = 0 gives 1, x < 0 or NaN NaN.
+__device__ __forceinline__ float tm_pow(float x, float g) {
+  if (g == 0.0f) return 1.0f;
+  return __builtin_amdgcn_exp2f(g * __builtin_amdgcn_logf(x));
+}
+
+__device__ __forceinline__ float tm_log10(float x) {
+  return __builtin_amdgcn_logf(x) * 0.30102999566398120f;
+}
+
+__device__ __forceinline__ float tm_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // shaders/tonemap.wgsl:153-211
 __device__ void hsv_adjust(float c[3], float hue_shift, float saturation) {
   float r = c[0], g = c[1], b = c[2];
@@ -287,13 +300,13 @@ __device__ void hsv_adjust(float c[3], float hue_shift, float saturation) {
   float delta = mx - mn;
   float h = 0.0f, s = 0.0f, v = mx;
   if (delta > 0.00001f) {
-    s = delta / mx;
+    s = tm_div(delta, mx);
     if (r >= mx)
-      h = (g - b) / delta;
+      h = tm_div(g - b, delta);
     else if (g >= mx)
-      h = 2.0f + (b - r) / delta;
+      h = 2.0f + tm_div(b - r, delta);
     else
-      h = 4.0f + (r - g) / delta;
+      h = 4.0f + tm_div(r - g, delta);
     h = h * 60.0f;
     if (h < 0.0f) h = h + 360.0f;
   }
@@ -311,7 +324,7 @@ __device__ void hsv_adjust(float c[3], float hue_shift, float saturation) {
   }
   float hh = h;
   if (hh >= 360.0f) hh = 0.0f;
-  hh = hh / 60.0f;
+  hh = hh * (1.0f / 60.0f);
   uint32_t i = (uint32_t)hh;
   float ff = hh - (float)i;
   float p = v * (1.0f - s);
@@ -334,39 +347,39 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
   float color[3];
   float alpha = 0.0f;
   if (t.mode == 0) {
-    color[0] = powf(a.x * t.exposure, t.gamma);
-    color[1] = powf(a.y * t.exposure, t.gamma);
-    color[2] = powf(a.z * t.exposure, t.gamma);
+    color[0] = tm_pow(a.x * t.exposure, t.gamma);
+    color[1] = tm_pow(a.y * t.exposure, t.gamma);
+    color[2] = tm_pow(a.z * t.exposure, t.gamma);
     alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
   } else if (t.mode == 2) {
     float nd = clampf(bucket_count * 0.01f * t.exposure, 0.0f, 1.0f);
-    float gd = powf(nd, t.gamma);
+    float gd = tm_pow(nd, t.gamma);
     color[0] = color[1] = color[2] = gd;
     alpha = nd;
   } else {
     float bs = 0.0f;
     if (!(bucket_count < 0.001f)) {
-      float l10 = logf(1.0f + t.white * bucket_count * t.k2) / logf(10.0f);
-      bs = (t.k1 * l10) / (t.white * bucket_count);
+      float l10 = tm_log10(1.0f + t.white * bucket_count * t.k2);
+      bs = tm_div(t.k1 * l10, t.white * bucket_count);
     }
-    float ls = bs / 67108864.0f;
+    float ls = bs * (1.0f / 67108864.0f);
     float fp0 = ls * (a.x * bucket_count);
     float fp1 = ls * (a.y * bucket_count);
     float fp2 = ls * (a.z * bucket_count);
     float fp3 = ls * bucket_count * t.white;
     if (fp3 > 0.0f) {
       if (fp3 <= t.gamma_threshold) {
-        float frac = fp3 / t.gamma_threshold;
-        alpha = (1.0f - frac) * fp3 * t.funcval + frac * powf(fp3, t.gamma);
+        float frac = tm_div(fp3, t.gamma_threshold);
+        alpha = (1.0f - frac) * fp3 * t.funcval + frac * tm_pow(fp3, t.gamma);
       } else {
-        alpha = powf(fp3, t.gamma);
+        alpha = tm_pow(fp3, t.gamma);
       }
     }
-    ls = fp3 > 0.0f ? t.vib * alpha / fp3 : 0.0f;
+    ls = fp3 > 0.0f ? tm_div(t.vib * alpha, fp3) : 0.0f;
     if (t.notvib > 0.0f) {
-      color[0] = ls * fp0 + t.notvib * powf(fp0, t.gamma);
-      color[1] = ls * fp1 + t.notvib * powf(fp1, t.gamma);
-      color[2] = ls * fp2 + t.notvib * powf(fp2, t.gamma);
+      color[0] = ls * fp0 + t.notvib * tm_pow(fp0, t.gamma);
+      color[1] = ls * fp1 + t.notvib * tm_pow(fp1, t.gamma);
+      color[2] = ls * fp2 + t.notvib * tm_pow(fp2, t.gamma);
     } else {
       color[0] = ls * fp0;
       color[1] = ls * fp1;
@@ -383,18 +396,18 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
   if (t.highlight == 1) {
     float m = fmaxf(color[0], fmaxf(color[1], color[2]));
     if (m > 1.0f)
-      for (int k = 0; k < 3; k++) color[k] = color[k] / m;
+      for (int k = 0; k < 3; k++) color[k] = tm_div(color[k], m);
   } else if (t.highlight == 2) {
     float lum = color[0] * 0.2126f + color[1] * 0.7152f + color[2] * 0.0722f;
     if (lum > 1e-6f) {
-      float lm = lum / (1.0f + lum);
-      for (int k = 0; k < 3; k++) color[k] = color[k] * (lm / lum);
+      float lm = tm_div(lum, 1.0f + lum);
+      for (int k = 0; k < 3; k++) color[k] = color[k] * tm_div(lm, lum);
     }
     for (int k = 0; k < 3; k++) color[k] = fminf(color[k], 1.0f);
   } else if (t.highlight == 3) {
     for (int k = 0; k < 3; k++) {
       float c = color[k];
-      c = (c * (2.51f * c + 0.03f)) / (c * (2.43f * c + 0.59f) + 0.14f);
+      c = tm_div(c * (2.51f * c + 0.03f), c * (2.43f * c + 0.59f) + 0.14f);
       color[k] = clampf(c, 0.0f, 1.0f);
     }
   } else {
@@ -410,18 +423,18 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
   }
   float leveled = 1.0f;
   if (t.sample_density > 0.0f) {
-    float nd = a.w / t.sample_density;
+    float nd = tm_div(a.w, t.sample_density);
     if (t.levels_high <= t.levels_low) {
       leveled = nd > t.levels_low ? 1.0f : 0.0f;
     } else {
-      float nrm = (nd - t.levels_low) / (t.levels_high - t.levels_low);
+      float nrm = tm_div(nd - t.levels_low, t.levels_high - t.levels_low);
       float cl = clampf(nrm, 0.0f, 1.0f);
-      leveled = (t.levels_gamma != 1.0f && t.levels_gamma > 0.0f) ? powf(cl, t.levels_gamma) : cl;
+      leveled = (t.levels_gamma != 1.0f && t.levels_gamma > 0.0f) ? tm_pow(cl, t.levels_gamma) : cl;
     }
   }
   float linear_alpha = clampf(bucket_count * 0.01f * t.density_scale, 0.0f, 1.0f);
   float gamma_alpha = clampf(alpha, 0.0f, 1.0f);
-  float st = clampf((gamma_alpha - t.alpha_low) / (t.alpha_high - t.alpha_low), 0.0f, 1.0f);
+  float st = clampf(tm_div(gamma_alpha - t.alpha_low, t.alpha_high - t.alpha_low), 0.0f, 1.0f);
   float blend_t = st * st * (3.0f - 2.0f * st);
   float base_alpha = gamma_alpha * (1.0f - blend_t) + linear_alpha * blend_t;
   float fa = t.levels_enabled ? fminf(base_alpha, leveled) : base_alpha;
@@ -431,7 +444,7 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
     // C = srgb(colour x alpha); premultiplied keeps C with the opacity,
     // straight alpha takes coverage = max(C) and stores C / coverage
     float ob[3];
-    for (int k = 0; k < 3; k++) ob[k] = powf(fmaxf(color[k] * fa, 0.0f), t.inv22);
+    for (int k = 0; k < 3; k++) ob[k] = tm_pow(fmaxf(color[k] * fa, 0.0f), t.inv22);
     if (t.transparent == 2) {
       o.x = unorm8(ob[0]);
       o.y = unorm8(ob[1]);
@@ -440,15 +453,15 @@ __device__ __forceinline__ uchar4 tonemap_one(const float4 a, const TonemapConst
       return o;
     }
     float cov = fmaxf(ob[0], fmaxf(ob[1], ob[2]));
-    o.x = unorm8(cov > 0.0f ? ob[0] / cov : 0.0f);
-    o.y = unorm8(cov > 0.0f ? ob[1] / cov : 0.0f);
-    o.z = unorm8(cov > 0.0f ? ob[2] / cov : 0.0f);
+    o.x = unorm8(cov > 0.0f ? tm_div(ob[0], cov) : 0.0f);
+    o.y = unorm8(cov > 0.0f ? tm_div(ob[1], cov) : 0.0f);
+    o.z = unorm8(cov > 0.0f ? tm_div(ob[2], cov) : 0.0f);
     o.w = unorm8(cov);
     return o;
   }
-  o.x = unorm8(powf(t.bg_lin[0] * (1.0f - fa) + color[0] * fa, t.inv22));
-  o.y = unorm8(powf(t.bg_lin[1] * (1.0f - fa) + color[1] * fa, t.inv22));
-  o.z = unorm8(powf(t.bg_lin[2] * (1.0f - fa) + color[2] * fa, t.inv22));
+  o.x = unorm8(tm_pow(t.bg_lin[0] * (1.0f - fa) + color[0] * fa, t.inv22));
+  o.y = unorm8(tm_pow(t.bg_lin[1] * (1.0f - fa) + color[1] * fa, t.inv22));
+  o.z = unorm8(tm_pow(t.bg_lin[2] * (1.0f - fa) + color[2] * fa, t.inv22));
   o.w = 255;
   return o;
 }

@@ -284,7 +284,8 @@ int ffm_trace(ffm_ctx* ctx, const ffm_view* view, uint32_t walker, uint32_t seed
 
 /* Separable bilateral Gaussian on the batch histogram
  * (shaders/histogram_blur.wgsl:39-119, host compute_kernel.rs:1668-1727):
- * H pass hist -> scratch, V pass scratch -> hist. */
+ * H pass hist -> scratch, V pass scratch -> hist in place; scratch holds
+ * width x height x 4 u32 words. */
 int ffm_blur(ffm_ctx* ctx, uint32_t* hist, uint32_t* scratch, int width, int height,
              float sigma_px, float density_sigma, void* stream);
 
