@@ -455,6 +455,9 @@ def main():
     ap.add_argument("--splat-mode", type=int, default=2)
     ap.add_argument("--walkers-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", default="", choices=["", "nofinal", "linear"],
+                    help="single4k instruction accounting (profiles), not a benchmark: drop the "
+                         "final transform (nofinal), or also make every variation linear (linear)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB device-to-device first (a known-size kernel for HBM counters)")
     args = ap.parse_args()
@@ -498,13 +501,20 @@ def main():
     else:
         fd, pal, job = configs.single4k(walkers=1 << args.walkers_log2,
                                         splat_mode=args.splat_mode)
+        metric = "chaos-game samples/s at 4K"
+        if args.variant:
+            fd.n_finals = 0
+            if args.variant == "linear":
+                for k in range(fd.n_vars):
+                    fd.var_id[k] = pyfae.VARIATIONS["linear"]
+            metric += " (instruction-accounting variant %s, not the workload)" % args.variant
+            args.no_extras = True
         progress(f"single4k: {args.warmup} warmup + {args.steps} timed renders")
         out, rgba = bench_flame(args, eng, fd, pal, job,
                                 "single4k: 6 xforms + final xform, 3 of 10 variations each, "
                                 "3840x2160, 2^%d walkers x 256 iterations, density estimation "
                                 "9->0 curve 0.4, log tonemap, boxed down 2x to 1920x1080 "
-                                "(firefly clamp)" % args.walkers_log2,
-                                "chaos-game samples/s at 4K", supersample=True)
+                                "(firefly clamp)" % args.walkers_log2, metric, supersample=True)
         progress(f"single4k: {out['value']:.4g} samples/s")
         if not args.no_extras:
             progress("splat compare: global u32 atomics at 4K")
