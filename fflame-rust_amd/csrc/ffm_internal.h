@@ -100,8 +100,13 @@ struct IterArgs {
 // stay for the kernels' static __shared__ variables.
 constexpr int LDS_DYNAMIC_MAX = 152 * 1024;
 
-// Screen tiles of the LDS splat: 128x64 pixels = 128 KB of u32x4 in LDS.
+// Screen tiles of the LDS splat: 2^tile_wl x 2^tile_hl pixels (TileArgs),
+// at most TILE_PX_MAX (a record's tile-local pixel has 13 bits). One flame
+// renders with 128x64 tiles = 128 KB of u32x4 in LDS; the batched gallery
+// tunes between those and 64x64 (64 KB: two accumulating workgroups a CU).
 constexpr int TILE_W = 128, TILE_H = 64, TILE_PX = TILE_W * TILE_H;
+constexpr int TILE_WL = 7, TILE_HL = 6;
+constexpr int TILE_PX_MAX = 8192;
 constexpr int TILE_MAX = 2048;  // bins per image the LDS counting sort holds
 
 // Records are sorted into bins: a bin is 2^group_shift consecutive tiles of
@@ -132,6 +137,8 @@ struct TileArgs {
   int32_t walk_block;    // lanes per walker workgroup (bin_shape): 512 or 1024
   int32_t walk_per_cu;   // walker workgroups per CU (bin_shape): 3 or 2 of 512, 1 of 1024
   int32_t bins_lds;      // bin slots of the walkers' LDS counting sort: >= n_tiles, a multiple of 64
+  int32_t tile_wl;       // log2 of the tile width (pixels)
+  int32_t tile_hl;       // log2 of the tile height
 };
 
 // The walker launch for a flame's LDS needs: lanes per workgroup and
@@ -139,7 +146,7 @@ struct TileArgs {
 void bin_shape(int n_rows, int pal_size, int nb, int* block, int* stage, int* per_cu,
                int max_per_cu);
 size_t bin_lds_bytes(int n_rows, int pal_size, int stage, int nb);
-size_t accum_lds_bytes(int pal_size, bool f32);
+size_t accum_lds_bytes(int pal_size, bool f32, int tile_px);
 hipError_t launch_bin_iterate(const IterArgs& a, const TileArgs& t, bool count_only, int grid,
                               hipStream_t s);
 hipError_t launch_tile_capacity(const TileArgs& t, float scale, uint32_t margin,

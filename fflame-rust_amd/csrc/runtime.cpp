@@ -641,7 +641,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   const int nb = (n_tiles + 63) / 64 * 64;
   bin_shape(a.n_rows, a.pal_size, nb, &walk_block, &stage, &per_cu, 3);
   if (bin_lds_bytes(a.n_rows, a.pal_size, stage, nb) > LDS_PER_CU ||
-      accum_lds_bytes(a.pal_size, f32) > LDS_PER_CU)
+      accum_lds_bytes(a.pal_size, f32, TILE_PX) > LDS_PER_CU)
     return fail(c, FFM_ERR_TOO_LARGE, "flame table + palette too large for the LDS tile splat");
   const uint64_t per_walker = (uint64_t)a.ipt + 1;
   // records held at once: RECORD_BUDGET in context buffers; in a caller's scratch
@@ -699,6 +699,8 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
   t.walk_block = walk_block;
   t.walk_per_cu = per_cu;
   t.bins_lds = nb;
+  t.tile_wl = TILE_WL;
+  t.tile_hl = TILE_HL;
   const int grid = std::max(c->num_cu, 1);
   const bool same = !ext && c->t_planned && c->t_W == W && c->t_H == H && c->t_nrows == a.n_rows &&
                     c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
@@ -1752,6 +1754,7 @@ struct BatchPlan {
   uint64_t total_pix = 0;
   uint32_t item_walkers = 0;
   int flush_every = 4;
+  int tile_mode = 0;
   std::vector<uint32_t> wave_start;  // items of wave w: [wave_start[w], wave_start[w+1])
   std::vector<BatchPix> pix;
   std::vector<uint64_t> samples;     // plotted samples per flame
@@ -1777,7 +1780,61 @@ struct Tuning {
   std::vector<uint64_t> sig;
   uint32_t item_walkers;
   int flush_every;
+  int tile_mode;
 };
+
+// The gallery's tile shapes (log2 width, log2 height): 0 = 128x64, the one
+// flame's; 1 = 64x64, whose LDS tile lets two accumulating workgroups share
+// a CU, at twice the bins for the walkers to sort into.
+constexpr int BATCH_TILE_MODES = 2;
+void tile_dims(int mode, int* wl, int* hl) {
+  *wl = mode == 1 ? 6 : TILE_WL;
+  *hl = TILE_HL;
+}
+
+// Tunings kept across processes: with FFM_TUNE_CACHE naming a file, a
+// gallery signature's winner is appended to it as one text line
+// "<FNV-1a 64 of the signature> <walkers per item> <flush cadence> <tile
+// shape>", and a first call finds it there instead of tuning again.
+uint64_t sig_hash(const std::vector<uint64_t>& sig) {
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t v : sig)
+    for (int b = 0; b < 8; b++) {
+      h ^= (v >> (8 * b)) & 0xFFu;
+      h *= 1099511628211ull;
+    }
+  return h;
+}
+
+bool tune_cache_find(const std::vector<uint64_t>& sig, int forced, Tuning* out) {
+  const char* path = std::getenv("FFM_TUNE_CACHE");
+  if (!path) return false;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  const unsigned long long h = sig_hash(sig);
+  unsigned long long fh;
+  unsigned iw;
+  int fe, mode;
+  bool found = false;
+  while (std::fscanf(f, "%llx %u %d %d", &fh, &iw, &fe, &mode) == 4) {
+    if (fh != h || iw == 0 || fe < 1 || fe > 8 || mode < 0 || mode >= BATCH_TILE_MODES) continue;
+    if (forced >= 0 && mode != forced) continue;
+    *out = Tuning{sig, iw, fe, mode};
+    found = true;  // the last line for a signature wins
+  }
+  std::fclose(f);
+  return found;
+}
+
+void tune_cache_add(const Tuning& t) {
+  const char* path = std::getenv("FFM_TUNE_CACHE");
+  if (!path) return;
+  if (FILE* f = std::fopen(path, "a")) {
+    std::fprintf(f, "%016llx %u %d %d\n", (unsigned long long)sig_hash(t.sig), t.item_walkers,
+                 t.flush_every, t.tile_mode);
+    std::fclose(f);
+  }
+}
 
 }  // namespace
 
@@ -1836,11 +1893,14 @@ std::vector<uint64_t> batch_signature(int n, const ffm_flame_desc* flames, const
 // Lay out the batch (buffers, waves, items) and capture its launches in a
 // graph. Flame tables and palettes are uploaded separately, every call.
 int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, const ffm_job* jobs,
-                uint32_t item_walkers, int flush_every) {
+                uint32_t item_walkers, int flush_every, int tile_mode) {
   P.drop_graph();
   P.n = n;
   P.item_walkers = item_walkers;
   P.flush_every = flush_every;
+  P.tile_mode = tile_mode;
+  int wl, hl;
+  tile_dims(tile_mode, &wl, &hl);
   std::vector<BatchFlame> bf(n);
   uint64_t tile_base = 0, pix_off = 0, table_off = 0, total_samples = 0;
   int max_rows = 0, max_tiles = 0;
@@ -1859,8 +1919,8 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     f.ipt = j.ipt;
     f.burn_in = j.burn_in;
     f.seed = j.seed_base;  // one dispatch per flame: seed_0
-    f.tiles_x = (j.view.width + TILE_W - 1) / TILE_W;
-    f.n_tiles = f.tiles_x * ((j.view.height + TILE_H - 1) / TILE_H);
+    f.tiles_x = (j.view.width + (1 << wl) - 1) >> wl;
+    f.n_tiles = f.tiles_x * ((j.view.height + (1 << hl) - 1) >> hl);
     max_tiles = std::max(max_tiles, f.n_tiles);
     f.tile_base = (uint32_t)tile_base;
     f.hist_off = pix_off;
@@ -1940,6 +2000,8 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.g.hist = reinterpret_cast<uint32_t*>(P.hist.p);
   b.g.n_tiles = P.n_tiles;
   b.g.flush_every = flush_every;
+  b.g.tile_wl = wl;
+  b.g.tile_hl = hl;
   // the largest flame's bins: a workgroup sorts one flame's records at a time
   b.g.bins_lds = (P.max_tiles + 63) / 64 * 64;
   bin_shape(max_rows, BATCH_PALETTE, b.g.bins_lds, &b.g.walk_block, &b.g.stage, &b.g.walk_per_cu,
@@ -2031,9 +2093,10 @@ int run_batch_graph(ffm_ctx* c, BatchPlan& P, float* gpu_ms) {
 
 // Many flames in one persistent launch per wave, the whole sequence a
 // hipGraph. The first call for a batch layout autotunes walkers per work
-// item (walkers per lane = item / 1024) and the staging flush cadence on
-// the batch itself, keeping the fastest; the graph is reused while the
-// layout (sizes, walker counts, transform counts) stays the same.
+// item (walkers per lane = item / 512) and the staging flush cadence on
+// the batch itself, then the tile shape with the winner of those, keeping
+// the fastest; the graph is reused while the layout (sizes, walker counts,
+// transform counts) stays the same.
 int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* const* stops,
                      const int* n_stops, const ffm_job* jobs, uint8_t* const* rgba_out,
                      ffm_render_stats* stats) {
@@ -2045,18 +2108,41 @@ int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const floa
   ffm_batch_state* B = batch_state(c);
   BatchPlan& P = B->plan;
   const std::vector<uint64_t> sig = batch_signature(n, flames, jobs);
+  // FFM_BATCH_TILE=128x64 or 64x64 fixes the tile shape instead of tuning it
+  int forced = -1;
+  if (const char* e = std::getenv("FFM_BATCH_TILE")) {
+    if (std::strcmp(e, "128x64") == 0) forced = 0;
+    else if (std::strcmp(e, "64x64") == 0) forced = 1;
+    else return fail(c, FFM_ERR_INVALID_ARGUMENT, "FFM_BATCH_TILE is 128x64 or 64x64, not %s", e);
+  }
   const Tuning* tune = nullptr;
   for (const Tuning& t : B->tuned)
-    if (t.sig == sig) tune = &t;
+    if (t.sig == sig && (forced < 0 || t.tile_mode == forced)) tune = &t;
   float gpu_ms = 0.0f;
+  Tuning cached;
+  if (!tune && tune_cache_find(sig, forced, &cached)) {
+    B->tuned.push_back(cached);
+    tune = &B->tuned.back();
+    P.sig.clear();
+  }
   if (!tune) {
     static const uint32_t kItems[] = {2048, 8192, 32768};
     static const int kFlush[] = {2, 4, 8};
     float best = 1e30f;
-    Tuning tb{sig, 65536, 4};
+    const int mode0 = forced < 0 ? 0 : forced;
+    if (mode0 != 0) {
+      int wl, hl;
+      tile_dims(mode0, &wl, &hl);
+      for (int i = 0; i < n; i++)
+        if ((int64_t)((jobs[i].view.width + (1 << wl) - 1) >> wl) *
+                ((jobs[i].view.height + (1 << hl) - 1) >> hl) > TILE_MAX)
+          return fail(c, FFM_ERR_TOO_LARGE, "flame %d: %dx%d needs more than %d tiles of %dx%d", i,
+                      jobs[i].view.width, jobs[i].view.height, TILE_MAX, 1 << wl, 1 << hl);
+    }
+    Tuning tb{sig, 65536, 4, mode0};
     for (uint32_t iw : kItems)
       for (int fe : kFlush) {
-        if ((st = build_batch(c, P, n, flames, jobs, iw, fe))) return st;
+        if ((st = build_batch(c, P, n, flames, jobs, iw, fe, mode0))) return st;
         if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
         float ms = 0.0f;
         if ((st = run_batch_graph(c, P, &ms))) return st;
@@ -2066,12 +2152,34 @@ int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const floa
           tb.flush_every = fe;
         }
       }
+    for (int mode = 1; forced < 0 && mode < BATCH_TILE_MODES; mode++) {
+      int wl, hl;
+      tile_dims(mode, &wl, &hl);
+      bool fits = true;
+      for (int i = 0; i < n; i++)
+        fits = fits && (int64_t)((jobs[i].view.width + (1 << wl) - 1) >> wl) *
+                               ((jobs[i].view.height + (1 << hl) - 1) >> hl) <= TILE_MAX;
+      if (!fits) continue;
+      if ((st = build_batch(c, P, n, flames, jobs, tb.item_walkers, tb.flush_every, mode)))
+        return st;
+      if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
+      float ms = 0.0f;
+      if ((st = run_batch_graph(c, P, &ms))) return st;
+      if (ms < best) {
+        best = ms;
+        tb.tile_mode = mode;
+      }
+    }
     B->tuned.push_back(tb);
     tune = &B->tuned.back();
+    tune_cache_add(tb);
     P.sig.clear();  // rebuild with the winner below
   }
-  if (P.sig != sig || P.item_walkers != tune->item_walkers || P.flush_every != tune->flush_every || !P.exec) {
-    if ((st = build_batch(c, P, n, flames, jobs, tune->item_walkers, tune->flush_every))) return st;
+  if (P.sig != sig || P.item_walkers != tune->item_walkers ||
+      P.flush_every != tune->flush_every || P.tile_mode != tune->tile_mode || !P.exec) {
+    if ((st = build_batch(c, P, n, flames, jobs, tune->item_walkers, tune->flush_every,
+                          tune->tile_mode)))
+      return st;
   }
   if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
   if ((st = run_batch_graph(c, P, &gpu_ms))) return st;

@@ -103,8 +103,8 @@ size_t bin_lds_bytes(int n_rows, int pal_size, int stage, int nb) {
   return flame_lds(n_rows, pal_size) + stage_fixed(nb) + (size_t)stage * STAGE_BYTES_PER_RECORD;
 }
 
-size_t accum_lds_bytes(int pal_size, bool f32) {
-  return (size_t)TILE_PX * 16 + (size_t)pal_size * (f32 ? 12 : 4) + 2 * 129 * 4;
+size_t accum_lds_bytes(int pal_size, bool f32, int tile_px) {
+  return (size_t)tile_px * 16 + (size_t)pal_size * (f32 ? 12 : 4) + 2 * 129 * 4;
 }
 
 // Exclusive scan of a[0..n) (n <= 2 * blockDim.x) in place; all threads.
@@ -160,7 +160,8 @@ __device__ __forceinline__ void overflow_splat(const TileArgs& t, const uint32_t
                                                uint32_t rec) {
   const int tx = bin_tile_x(t, bin, (int)(rec >> REC_SUB_SHIFT)), ty = bin / t.bins_x;
   const int local = (int)(rec & REC_LOCAL_MASK);
-  const int px = tx * TILE_W + (local % TILE_W), py = ty * TILE_H + (local / TILE_W);
+  const int px = (tx << t.tile_wl) + (local & ((1 << t.tile_wl) - 1));
+  const int py = (ty << t.tile_hl) + (local >> t.tile_wl);
   const size_t base = 4 * ((size_t)py * t.width + px);
   if (t.f32) {
     const float* c = t.pal_f32 + 3 * rec_texel(rec);
@@ -258,11 +259,13 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
         if (st & ST_PLOT) {
           r_in++;
           // in-frame: px, py >= 0, so tile arithmetic is unsigned shifts and masks
-          const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, tx = ux / TILE_W;
-          const uint32_t tile = __umul24(uy / TILE_H, (uint32_t)t.bins_x) + (tx >> t.group_shift);
+          const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, tx = ux >> t.tile_wl;
+          const uint32_t tile =
+              __umul24(uy >> t.tile_hl, (uint32_t)t.bins_x) + (tx >> t.group_shift);
           const uint32_t r = atomicAdd(&L.cnt[tile], 1u);  // the record's rank in its bin
           if (!COUNT_ONLY) {
-            const uint32_t local = (uy % TILE_H) * TILE_W + ux % TILE_W |
+            const uint32_t local = (uy & ((1u << t.tile_hl) - 1u)) << t.tile_wl |
+                                   (ux & ((1u << t.tile_wl) - 1u)) |
                                    (tx & ((1u << t.group_shift) - 1u)) << REC_SUB_SHIFT;
             // wave-compacted append: one LDS atomic per wave, by the first
             // active lane (rank 0), whose result readfirstlane broadcasts
@@ -471,8 +474,9 @@ template <bool F32>
 __device__ __forceinline__ void fold_tile(const TileArgs& t, int tx, int ty, uint32_t n_slices,
                                           const uint32_t* tile) {
   const float* tilef = reinterpret_cast<const float*>(tile);
-  for (int q = threadIdx.x; q < TILE_PX; q += blockDim.x) {
-    const int px = tx * TILE_W + (q % TILE_W), py = ty * TILE_H + (q / TILE_W);
+  const int wmask = (1 << t.tile_wl) - 1;
+  for (int q = threadIdx.x; q < 1 << (t.tile_wl + t.tile_hl); q += blockDim.x) {
+    const int px = (tx << t.tile_wl) + (q & wmask), py = (ty << t.tile_hl) + (q >> t.tile_wl);
     if (px >= t.width || py >= t.height) continue;
     if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
     const size_t p = (size_t)py * t.width + px;
@@ -561,7 +565,7 @@ __device__ void accum_slice(const TileArgs& t, int ti, uint32_t r0, uint32_t r1,
   for (int sub = 0; sub < group; sub++) {
     const int tx = bin_tile_x(t, ti, sub);
     if (tx >= t.tiles_x) break;
-    for (int i = tid; i < 4 * TILE_PX; i += blockDim.x) tile[i] = 0u;
+    for (int i = tid; i < 4 << (t.tile_wl + t.tile_hl); i += blockDim.x) tile[i] = 0u;
     __syncthreads();
     const uint32_t s0 = start[sub], s1 = start[sub + 1];
     if (s1 > s0) {  // uniform
@@ -588,8 +592,8 @@ template <bool F32>
 __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const uint32_t* pal_g,
                                                                int pal_size) {
   extern __shared__ uint32_t lds[];
-  uint32_t* tile = lds;               // TILE_PX x 4 (u32 or f32 bits)
-  uint32_t* pal = lds + 4 * TILE_PX;  // pal_size u32, or 3 x pal_size f32
+  uint32_t* tile = lds;                                 // tile pixels x 4 (u32 or f32 bits)
+  uint32_t* pal = lds + (4 << (t.tile_wl + t.tile_hl));  // pal_size u32, or 3 x pal_size f32
   float* palf = reinterpret_cast<float*>(pal);
   uint32_t* sort = pal + (F32 ? 3 : 1) * pal_size;  // 2 x 129
   const int tid = threadIdx.x;
@@ -670,7 +674,7 @@ __global__ __launch_bounds__(BIN_BLOCK) FFM_WALKER_WAVES void bin_batch_kernel(B
 __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_batch_kernel(BatchArgs b) {
   extern __shared__ uint32_t lds[];
   uint32_t* tile = lds;
-  uint32_t* pal = lds + 4 * TILE_PX;
+  uint32_t* pal = lds + (4 << (b.g.tile_wl + b.g.tile_hl));
   const int tid = threadIdx.x;
   const TileArgs& g = b.g;
   const uint32_t n_items = g.item_start[g.n_tiles];
@@ -751,12 +755,18 @@ hipError_t launch_tile_capacity_n(const TileArgs& t, float scale, uint32_t margi
   return hipGetLastError();
 }
 
+// Accumulating workgroups a CU: as many as the tile's LDS lets be resident
+// (two for a 64 KB tile, one for 128 KB).
+static int accum_per_cu(const TileArgs& t) {
+  return (t.tile_wl + t.tile_hl) <= 12 ? 2 : 1;
+}
+
 hipError_t launch_tile_accum_batch(const BatchArgs& b, int grid, hipStream_t s) {
   static const hipError_t e0 = allow_lds(reinterpret_cast<const void*>(&tile_accum_batch_kernel));
   if (e0 != hipSuccess) return e0;
   hipLaunchKernelGGL(tile_plan_kernel, dim3(1), dim3(BIN_BLOCK), 0, s, b.g);
-  hipLaunchKernelGGL(tile_accum_batch_kernel, dim3(grid), dim3(BIN_BLOCK),
-                     accum_lds_bytes(b.max_pal, false), s, b);
+  hipLaunchKernelGGL(tile_accum_batch_kernel, dim3(grid * accum_per_cu(b.g)), dim3(BIN_BLOCK),
+                     accum_lds_bytes(b.max_pal, false, 1 << (b.g.tile_wl + b.g.tile_hl)), s, b);
   return hipGetLastError();
 }
 
@@ -774,12 +784,15 @@ hipError_t launch_tile_accum(const TileArgs& t, const uint32_t* pal, int pal_siz
   if (e0 != hipSuccess) return e0;
   if (e1 != hipSuccess) return e1;
   hipLaunchKernelGGL(tile_plan_kernel, dim3(1), dim3(BIN_BLOCK), 0, s, t);
+  grid *= accum_per_cu(t);
   if (t.f32)
     hipLaunchKernelGGL(tile_accum_kernel<true>, dim3(grid), dim3(BIN_BLOCK),
-                       accum_lds_bytes(pal_size, true), s, t, pal, pal_size);
+                       accum_lds_bytes(pal_size, true, 1 << (t.tile_wl + t.tile_hl)), s, t, pal,
+                       pal_size);
   else
     hipLaunchKernelGGL(tile_accum_kernel<false>, dim3(grid), dim3(BIN_BLOCK),
-                       accum_lds_bytes(pal_size, false), s, t, pal, pal_size);
+                       accum_lds_bytes(pal_size, false, 1 << (t.tile_wl + t.tile_hl)), s, t, pal,
+                       pal_size);
   return hipGetLastError();
 }
 

@@ -443,11 +443,15 @@ def test_single4k_image_within_noise_floor(engine):
 
 # -------------------------------------------------------- batched gallery
 
-def test_batch_equals_serial_and_oracle(engine):
+@pytest.mark.parametrize("tile", [None, "128x64", "64x64"])
+def test_batch_equals_serial_and_oracle(engine, tile, monkeypatch):
     """A small gallery (mixed sizes and transform counts) in one batched
     launch gives exactly the images of the same flames rendered one by one,
-    and those match the oracle's within one level."""
-    say("batch vs serial")
+    and those match the oracle's within one level -- with the tuned tile
+    shape and with each one fixed."""
+    say(f"batch vs serial, tile {tile or 'tuned'}")
+    if tile:
+        monkeypatch.setenv("FFM_BATCH_TILE", tile)
     items = configs.gallery(n=6, spp=20.0, ipt=64)
     outs_b, st_b = engine.batch_render(items)
     outs_b2, _ = engine.batch_render(items)  # graph replay
@@ -471,6 +475,35 @@ def test_batch_equals_serial_and_oracle(engine):
         gm = ga[:h, :w].reshape(h // b, b, w // b, b, 3).mean((1, 3))
         om = oa[:h, :w].reshape(h // b, b, w // b, b, 3).mean((1, 3))
         assert np.abs(gm - om).mean() < 2.0
+
+
+def test_batch_tuning_cache_skips_the_tuning(engine, monkeypatch, tmp_path):
+    say("batch: FFM_TUNE_CACHE reused by a new context")
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("FFM_TUNE_CACHE", str(path))
+    items = configs.gallery(n=3, spp=8.0, ipt=64)
+    outs, _ = engine.batch_render(items)
+    lines = path.read_text().splitlines()
+    assert len(lines) == 1
+    h, iw, fe, mode = lines[0].split()
+    assert int(iw) in (2048, 8192, 32768) and int(fe) in (2, 4, 8) and int(mode) in (0, 1)
+    other = pyfae.Engine(0)  # a fresh context: nothing tuned in memory
+    try:
+        outs2, _ = other.batch_render(items)
+    finally:
+        other.close()
+    assert path.read_text().splitlines() == lines  # found, not tuned again
+    for a, b in zip(outs, outs2):
+        assert np.array_equal(a, b)
+
+
+def test_batch_tile_override_is_checked(engine, monkeypatch):
+    say("batch: FFM_BATCH_TILE refused unless 128x64 or 64x64")
+    items = configs.gallery(n=2, spp=4.0, ipt=64)
+    monkeypatch.setenv("FFM_BATCH_TILE", "32x32")
+    with pytest.raises(pyfae.EngineError) as e:
+        engine.batch_render(items)
+    assert "FFM_BATCH_TILE" in str(e.value)
 
 
 # ------------------------------------------------------------ error paths
