@@ -88,9 +88,11 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
 // change over the loop of active variations, so without a barrier the
 // compiler hoists ALL bodies' arithmetic (divisions, sqrt, sincos, atan2)
 // out of that loop and pays for every one on every transform evaluation.
-// Passing the point through an empty volatile asm at the top of each case
-// makes it opaque there, which keeps each body behind its branch.
-#define FFM_NO_SPECULATE() asm volatile("" : "+v"(x), "+v"(y))
+// The loops over variation slots pass the point through an empty volatile
+// asm at the top of every trip (FFM_OPAQUE_POINT): the point is then a new
+// value each trip, in the same registers (no copies), so each body stays
+// behind its branch.
+#define FFM_OPAQUE_POINT(px, py) asm volatile("" : "+v"(px), "+v"(py))
 
 // One variation body V(x, y). `row` is the transform's table row (waves
 // reads its own b, e, d, f), `p` the variation's params. Bodies:
@@ -105,7 +107,6 @@ __device__ __forceinline__ void eval_ext(int id, const float* __restrict__ p, fl
                                          Rng& rng, float& ox, float& oy) {
   switch (id) {
     case FFM_VAR_BLOB: {
-      FFM_NO_SPECULATE();
       float high = p[0], low = p[1], waves = p[2];
       float r = ffm_sqrt(x * x + y * y);
       float theta = ff_atan2(x, y);  // Apophysis's atan2(x, y)
@@ -117,7 +118,6 @@ __device__ __forceinline__ void eval_ext(int id, const float* __restrict__ p, fl
       break;
     }
     case FFM_VAR_NOISE: {
-      FFM_NO_SPECULATE();
       float theta = rng_nextf(rng) * TWO_PI_F;
       float r = rng_nextf(rng);
       float s, c;
@@ -165,19 +165,16 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       oy = y;
       break;
     case FFM_VAR_SINUSOIDAL:
-      FFM_NO_SPECULATE();
       ox = ffm_sinf(x);
       oy = ffm_sinf(y);
       break;
     case FFM_VAR_SPHERICAL: {
-      FFM_NO_SPECULATE();
       float r2 = x * x + y * y + 1e-6f;
       ox = ffm_fdiv(x, r2);
       oy = ffm_fdiv(y, r2);
       break;
     }
     case FFM_VAR_SWIRL: {
-      FFM_NO_SPECULATE();
       float r2 = x * x + y * y;
       float s, c;
       ffm_sincosf(r2, &s, &c);
@@ -186,7 +183,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_HORSESHOE: {
-      FFM_NO_SPECULATE();
       float r = ffm_sqrt_cr(x * x + y * y) + 1e-6f;  // its rounding reaches both outputs
       float r_inv = ffm_rcp(r);
       ox = (x - y) * (x + y) * r_inv;
@@ -194,7 +190,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_POLAR: {
-      FFM_NO_SPECULATE();
       float theta = ffm_atan2f(x, y);
       float r = ffm_sqrt(x * x + y * y);
       ox = ffm_fdiv(theta, PI_F);
@@ -202,7 +197,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_DISC: {
-      FFM_NO_SPECULATE();
       float theta = ffm_atan2f(x, y);
       float r = ffm_sqrt_cr(x * x + y * y);  // feeds the angle pi r
       float theta_pi = ffm_fdiv(theta, PI_F);
@@ -214,13 +208,11 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_HYPERBOLIC: {
-      FFM_NO_SPECULATE();
       ox = ffm_fdiv(x, x * x + y * y + 1e-6f);
       oy = y;
       break;
     }
     case FFM_VAR_JULIA: {
-      FFM_NO_SPECULATE();
       float r = ffm_sqrt(x * x + y * y);
       float theta = ffm_atan2f(y, x);
       float sqrt_r = ffm_sqrt(r);
@@ -233,7 +225,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_WAVES: {
-      FFM_NO_SPECULATE();
       float b = row[XF_B], e = row[XF_E], d = row[XF_D], f = row[XF_F];
       // the row is fixed over the variation loop too: keep e, f opaque here
       asm volatile("" : "+v"(e), "+v"(f));
@@ -242,7 +233,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_JULIAN: {
-      FFM_NO_SPECULATE();
       float power = p[0];
       float cpower = p[2];
       float r2 = x * x + y * y;
@@ -257,7 +247,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_BUBBLE: {
-      FFM_NO_SPECULATE();
       float r2 = x * x + y * y;
       float r = r2 / 4.0f + 1.0f;
       float scale = ffm_rcp(r);
@@ -266,7 +255,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_CURL: {
-      FFM_NO_SPECULATE();
       float c1 = p[0], c2 = p[1];
       float re = 1.0f + c1 * x + c2 * (x * x - y * y);
       float im = c1 * y + 2.0f * c2 * x * y;
@@ -314,6 +302,7 @@ __device__ __forceinline__ void apply_xform(const float* __restrict__ row, KInt*
   float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
   float rx = 0.0f, ry = 0.0f;
   for (int k = 0; k < nv; k++) {
+    FFM_OPAQUE_POINT(ax, ay);
     int id = __builtin_amdgcn_readfirstlane(ids[k]);
     float w = row[XF_W0 + k];
     if (w != 0.0f) {
@@ -344,6 +333,7 @@ struct FlameView {
   KInt* ids;                 // the table's variation ids (global), n_vars
   KHeader* hdr;              // the table's header (final-chain slot lists; global), or null
   const float* xcum;         // LDS xaos selection table (n_xforms^2), or null without xaos
+  const uint8_t* sel;        // LDS selection buckets (SEL_BITS of the draw), or null
   int n_xforms, n_finals, n_vars;
 };
 
@@ -364,8 +354,9 @@ __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x
   float rx = 0.0f, ry = 0.0f;
   const int nact = __builtin_amdgcn_readfirstlane(fl.hdr->fin_nact[i]);
   for (int j = 0; j < nact; j++) {
-    const int k = __builtin_amdgcn_readfirstlane(fl.hdr->fin_act[i][j]);
-    const int id = __builtin_amdgcn_readfirstlane(fl.ids[k]);
+    FFM_OPAQUE_POINT(ax, ay);
+    const int code = __builtin_amdgcn_readfirstlane(fl.hdr->fin_act[i][j]);
+    const int k = code & 0xFF, id = code >> 8;
     const float w = row[XF_W0 + k];
     if (EXT && id == FFM_VAR_PRE_BLUR) {
       pre_blur(w, ax, ay, rng);
@@ -434,15 +425,33 @@ __device__ __forceinline__ int select_const(const FlameView& fl, float u) {
   return xi;
 }
 
+// The same pick from the 24-bit draw d (u = d 2^-24) through the selection
+// buckets: one LDS byte; in a bucket that holds a boundary the pick is the
+// first transform from the bucket's lowest pick on whose cumulative weight
+// reaches the target (every earlier one falls short for the whole bucket).
+__device__ __forceinline__ int select_bucket(const FlameView& fl, uint32_t d, float u) {
+  const uint32_t e = fl.sel[d >> (24 - SEL_BITS)];
+  int xi = (int)(e & 0x7Fu);
+  if (e & 0x80u) {
+    const int n = fl.n_xforms;
+    const float target = u * fl.rows[(n - 1) * XF_STRIDE + XF_CUM];
+    while (xi < n - 1 && !(target <= fl.rows[__umul24(xi, XF_STRIDE) + XF_CUM])) xi++;
+  }
+  return xi;
+}
+
 // With xaos the pick is select_transform_xaos (utilities.wgsl:107-140): the
 // same rule over row `prev` of the table of running sums of
 // weight[j] * xaos[prev][j], and the pick becomes the next `prev`.
 template <bool EXT>
 __device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool& should_plot,
                                             int& prev) {
-  float u = rng_nextf(rng);
+  const uint32_t d = rng_next(rng) >> 8u;
+  const float u = (float)d * (1.0f / 16777216.0f);  // rng_nextf's value
   int xi;
-  if (EXT && fl.xcum) {
+  if (fl.sel) {
+    xi = select_bucket(fl, d, u);
+  } else if (EXT && fl.xcum) {
     const int n = fl.n_xforms;
     const float* xr = fl.xcum + prev * n;
     float target = u * xr[n - 1];
@@ -453,7 +462,7 @@ __device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool&
   } else {
     xi = select_const(fl, u);
   }
-  const float opacity = fl.rows[xi * XF_STRIDE + XF_OPACITY];
+  const float opacity = fl.rows[__umul24(xi, XF_STRIDE) + XF_OPACITY];
   should_plot = true;
   if (opacity >= 1.0f) rng_skip(rng);
   else should_plot = rng_nextf(rng) < opacity;
@@ -470,7 +479,7 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
                                           float& ci, uint32_t& fuse, int xi, bool should_plot,
                                           int32_t& px, int32_t& py, int& texel) {
   const int n = fl.n_xforms;
-  const float* row = fl.rows + xi * XF_STRIDE;
+  const float* row = fl.rows + __umul24(xi, XF_STRIDE);
   float nx = x, ny = y;
   apply_xform<EXT>(row, fl.ids, fl.n_vars, nx, ny, rng);
   // Bad-value respawn + re-fuse (main_template.wgsl:397-479).
@@ -601,6 +610,8 @@ __device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_
   l.fv.xcum = __builtin_amdgcn_readfirstlane(hdr->xaos)
                   ? l.fv.rows + __builtin_amdgcn_readfirstlane(hdr->xaos_off)
                   : nullptr;
+  const int sel_off = __builtin_amdgcn_readfirstlane(hdr->sel_off);
+  l.fv.sel = sel_off ? reinterpret_cast<const uint8_t*>(l.fv.rows + sel_off) : nullptr;
   l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
   l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
   l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);

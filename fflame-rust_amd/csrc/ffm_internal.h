@@ -39,8 +39,8 @@ struct FlameHeader {
   int32_t var_mask;  // bit per ffm variation slot family, for fast paths
   int32_t var_id[FFM_MAX_VARS];
   // Each final transform's variation slots with a non-zero weight, in
-  // local-index order: every lane applies the same final, so its loop runs
-  // over these slots only.
+  // local-index order, as slot | variation id << 8 (one scalar load a slot):
+  // every lane applies the same final, so its loop runs over these only.
   int32_t fin_nact[FFM_MAX_FINALS];
   int32_t fin_act[FFM_MAX_FINALS][FFM_MAX_VARS];
   // 1 when every normal transform applies the same final chain (then
@@ -57,10 +57,24 @@ struct FlameHeader {
   int32_t table_rows;
   int32_t xaos;
   int32_t xaos_off;
+  // Without xaos, SEL_ROWS rows after the transforms' hold the pick of each
+  // of 2^SEL_BITS buckets of the 24-bit selection draw, one byte each, at
+  // float sel_off from the first row (0: the buckets are not usable, as
+  // when a negative weight makes the cumulative sums decrease).
+  int32_t sel_off;
 };
 
 // Rows of the xaos selection table of a flame with n transforms.
 __host__ __device__ inline int xaos_rows(int n) { return (n * n + XF_STRIDE - 1) / XF_STRIDE; }
+
+// Selection buckets: the top SEL_BITS bits of the 24-bit draw d index a
+// byte holding the pick when it is the same for every d of the bucket (the
+// pick never decreases with d while the cumulative weights never decrease),
+// else the pick at the bucket's lowest d with bit 7 set (the pick is then
+// the first transform from there whose cumulative weight reaches the target).
+constexpr int SEL_BITS = 10;
+constexpr int SEL_ROWS = ((1 << SEL_BITS) / 4 + XF_STRIDE - 1) / XF_STRIDE;
+static_assert(FFM_MAX_XFORMS <= 128, "a bucket's pick fits in 7 bits");
 
 // A flame table in device memory: header then n_rows rows.
 __host__ __device__ inline size_t flame_table_floats(int n_rows) {

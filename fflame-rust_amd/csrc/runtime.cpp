@@ -261,10 +261,27 @@ void prepare_flame(ffm_flame_desc* f) {
   }
 }
 
+// Rows of a flame's packed table: the transforms' and finals' rows, then the
+// xaos selection table or the selection buckets (the layout depends on the
+// counts only, so a batch can lay tables out before their contents).
+int packed_rows(const ffm_flame_desc* f) {
+  return f->n_xforms + f->n_finals + (f->has_xaos ? xaos_rows(f->n_xforms) : SEL_ROWS);
+}
+
+// The pick of select_transform_const (shaders/core/utilities.wgsl:35-55) for
+// the 24-bit draw d: the first i with u * total <= cum_i, else the last.
+static int pick_for_draw(const float* cum, int n, uint32_t d) {
+  const float u = (float)d * (1.0f / 16777216.0f);
+  const float target = u * cum[n - 1];
+  for (int i = 0; i < n - 1; i++)
+    if (target <= cum[i]) return i;
+  return n - 1;
+}
+
 // Pack the flame into the device table layout of ffm_internal.h.
 std::vector<float> pack_flame(const ffm_flame_desc* f) {
   const int rows = f->n_xforms + f->n_finals;
-  const int table_rows = rows + (f->has_xaos ? xaos_rows(f->n_xforms) : 0);
+  const int table_rows = packed_rows(f);
   std::vector<float> t(flame_table_floats(table_rows), 0.0f);
   FlameHeader h{};
   h.table_rows = table_rows;
@@ -287,7 +304,8 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
   }
   for (int i = 0; i < f->n_finals; i++)
     for (int k = 0; k < f->n_vars; k++)
-      if (f->finals[i].var_weight[src[k]] != 0.0f) h.fin_act[i][h.fin_nact[i]++] = k;
+      if (f->finals[i].var_weight[src[k]] != 0.0f)
+        h.fin_act[i][h.fin_nact[i]++] = k | h.var_id[k] << 8;
   // each normal transform's final chain (check_flame validated the lists)
   auto chain = [&](int i, int* out) {
     if (!f->per_transform_finals) {
@@ -333,6 +351,26 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
       r[XF_W0 + k] = x.var_weight[src[k]];
       for (int q = 0; q < FFM_MAX_VAR_PARAMS; q++)
         r[XF_P0 + k * FFM_MAX_VAR_PARAMS + q] = x.var_param[src[k]][q];
+    }
+  }
+  if (!f->has_xaos) {
+    // the buckets, when the cumulative sums are finite and never decrease
+    std::vector<float> cum(f->n_xforms);
+    bool usable = true;
+    for (int i = 0; i < f->n_xforms; i++) {
+      cum[i] = r0[(size_t)i * XF_STRIDE + XF_CUM];
+      if (!std::isfinite(cum[i]) || (i > 0 && cum[i] < cum[i - 1])) usable = false;
+    }
+    if (usable) {
+      const int off = rows * XF_STRIDE;
+      uint8_t* sel = reinterpret_cast<uint8_t*>(r0 + off);
+      const uint32_t span = 1u << (24 - SEL_BITS);
+      for (uint32_t b = 0; b < (1u << SEL_BITS); b++) {
+        const int lo = pick_for_draw(cum.data(), f->n_xforms, b * span);
+        const int hi = pick_for_draw(cum.data(), f->n_xforms, b * span + span - 1);
+        sel[b] = (uint8_t)(lo | (lo == hi ? 0 : 0x80));
+      }
+      reinterpret_cast<FlameHeader*>(t.data())->sel_off = off;
     }
   }
   if (f->has_xaos) {
@@ -1908,7 +1946,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   P.samples.assign(n, 0);
   for (int i = 0; i < n; i++) {
     const ffm_job& j = jobs[i];
-    const int rows = flames[i].n_xforms + flames[i].n_finals;
+    const int rows = packed_rows(&flames[i]);
     max_rows = std::max(max_rows, rows);
     BatchFlame& f = bf[i];
     f.table_off = (uint32_t)table_off;

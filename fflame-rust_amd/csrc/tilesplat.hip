@@ -39,7 +39,7 @@ constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
 // accum_run adds two channels per 64-bit LDS atomic: a slice's channel sums
 // must not carry out of 32 bits (at most 255, or 100 for density, a record)
 static_assert((unsigned long long)SLICE * 255ull < (1ull << 32), "tile channel sums carry");
-constexpr size_t STAGE_BYTES_PER_RECORD = 4 + 4;  // rec, tr
+constexpr size_t STAGE_BYTES_PER_RECORD = 8;  // (record, bin | rank << 16)
 // LDS of the bins for nb bin slots: cnt, gbase (u32) and lim (u16) per bin,
 // then 16 spare words and the fill words
 static size_t stage_fixed(int nb) { return (size_t)nb * (4 + 4 + 2) + 16 * 4 + 16; }
@@ -68,9 +68,13 @@ static_assert(STAGE_THIRD_MIN >= 6 * WALK_LANES && STAGE_MIN >= 6 * BIN_BLOCK / 
 // with two workgroups a CU (650 vs 578 flames/s).
 #define FFM_WALKER_WAVES __attribute__((amdgpu_waves_per_eu(6)))
 
-static size_t flame_lds(int n_rows, int pal_size) {
-  return flame_table_floats(n_rows) * 4 + (size_t)pal_size * 4;
+// Words of LDS before the stage: the flame table and the palette, rounded up
+// to an even count so the stage's 8-byte entries are aligned.
+__host__ __device__ inline size_t stage_base_words(int n_rows, int pal_size) {
+  return (flame_table_floats(n_rows) + (size_t)pal_size + 1) & ~(size_t)1;
 }
+
+static size_t flame_lds(int n_rows, int pal_size) { return stage_base_words(n_rows, pal_size) * 4; }
 
 static int stage_fit(size_t budget, size_t used) {
   const size_t fit = used < budget ? (budget - used) / STAGE_BYTES_PER_RECORD / 512 * 512 : 0;
@@ -137,8 +141,7 @@ __device__ void block_exclusive_scan(uint32_t* a, int n, uint32_t* wave_sums, ui
 }
 
 struct BinLds {
-  uint32_t* rec;    // stage
-  uint32_t* tr;     // stage: bin | rank among the stage's records of the bin << 16
+  uint2* st;        // stage: (record, bin | rank among the stage's records of the bin << 16)
   uint32_t* cnt;    // bins_lds: the stage's records per bin (COUNT_ONLY: the walk's)
   uint32_t* gbase;  // bins_lds: the bin's first record slot of this flush (region + fill)
   uint16_t* lim;    // bins_lds: slots left in the bin's region (at most 65535)
@@ -207,13 +210,13 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
   __syncthreads();
   // ranks below 2^14 (the stage), so `rank < lim` is `fill + rank < cap`
   for (int j = tid; j < n; j += blockDim.x) {
-    const uint32_t v = L.tr[j];
-    const int tile = (int)(v & 0xFFFFu);
-    const uint32_t rank = v >> 16;
+    const uint2 e = L.st[j];
+    const int tile = (int)(e.y & 0xFFFFu);
+    const uint32_t rank = e.y >> 16;
     if (rank < L.lim[tile])
-      t.records[(size_t)(L.gbase[tile] + rank)] = L.rec[j];
+      t.records[(size_t)(L.gbase[tile] + rank)] = e.x;
     else
-      overflow_splat(t, pal, tile, L.rec[j]);
+      overflow_splat(t, pal, tile, e.x);
   }
   __syncthreads();
   if (tid == 0) *L.fill = 0u;
@@ -267,16 +270,11 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
             const uint32_t local = (uy & ((1u << t.tile_hl) - 1u)) << t.tile_wl |
                                    (ux & ((1u << t.tile_wl) - 1u)) |
                                    (tx & ((1u << t.group_shift) - 1u)) << REC_SUB_SHIFT;
-            // wave-compacted append: one LDS atomic per wave, by the first
-            // active lane (rank 0), whose result readfirstlane broadcasts
-            const uint64_t m = __ballot(1);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            uint32_t b = 0;
-            if (rank == 0u) b = atomicAdd(L.fill, (uint32_t)__popcll(m));
-            b = __builtin_amdgcn_readfirstlane(b);
-            L.rec[b + rank] = local | ((uint32_t)texel << REC_TEXEL_SHIFT);
-            L.tr[b + rank] = tile | r << 16;
+            // the append to the stage: the compiler's atomic optimizer makes
+            // the wave's increments of one address one LDS atomic by its
+            // first active lane, each lane taking base + its rank in the wave
+            const uint32_t b = atomicAdd(L.fill, 1u);
+            L.st[b] = make_uint2(local | ((uint32_t)texel << REC_TEXEL_SHIFT), tile | r << 16);
           }
         }
       }
@@ -298,8 +296,7 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
 // slots (a multiple of 64).
 __device__ __forceinline__ BinLds bin_lds_views(uint32_t* p, int stage, int nb) {
   BinLds L;
-  L.rec = p;
-  L.tr = p + stage;
+  L.st = reinterpret_cast<uint2*>(p);
   L.cnt = p + 2 * stage;
   L.gbase = L.cnt + nb;
   L.lim = reinterpret_cast<uint16_t*>(L.gbase + nb);
@@ -330,10 +327,10 @@ __device__ void finish_walkers(const TileArgs& t, const BinLds& L, const uint32_
 template <bool COUNT_ONLY, bool EXT>
 __global__ __launch_bounds__(BIN_BLOCK) FFM_WALKER_WAVES void bin_iterate_kernel(IterArgs a,
                                                                                  TileArgs t) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __align__(16) uint32_t lds[];
   LdsFlame F = load_lds<false>(a, lds);
   const BinLds L =
-      bin_lds_views(lds + flame_table_floats(a.n_rows) + a.pal_size, t.stage, t.bins_lds);
+      bin_lds_views(lds + stage_base_words(a.n_rows, a.pal_size), t.stage, t.bins_lds);
   const int tid = threadIdx.x;
   for (int i = tid; i < t.n_tiles; i += blockDim.x) L.cnt[i] = 0u;
   if (tid == 0) *L.fill = 0u;
@@ -634,9 +631,9 @@ __device__ __forceinline__ TileArgs flame_tiles(const BatchArgs& b, const BatchF
 // walks the range with the same staging and flushes as one flame.
 template <bool COUNT_ONLY, bool EXT>
 __global__ __launch_bounds__(BIN_BLOCK) FFM_WALKER_WAVES void bin_batch_kernel(BatchArgs b) {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __align__(16) uint32_t lds[];
   const BinLds L =
-      bin_lds_views(lds + flame_table_floats(b.max_rows) + b.max_pal, b.g.stage, b.g.bins_lds);
+      bin_lds_views(lds + stage_base_words(b.max_rows, b.max_pal), b.g.stage, b.g.bins_lds);
   const int tid = threadIdx.x;
   for (int i = tid; i < b.g.bins_lds; i += blockDim.x) L.cnt[i] = 0u;
   if (tid == 0) *L.fill = 0u;

@@ -558,3 +558,34 @@ def test_final_attachments_bit_exact(engine, attach):
     o, oplotted = O.iterate_mt(fd, pal, view, 1 << 14, 200, 20, 3, 8)
     assert plotted == oplotted
     assert np.array_equal(g, o)
+
+
+@pytest.mark.parametrize("kind", ["dense", "negative", "max"])
+def test_selection_buckets_bit_exact(engine, kind):
+    """The pick through the device's selection buckets (1024 buckets of the
+    24-bit draw) equals select_transform_const's scan, on linear flames
+    whose traces and histograms are bit-exact: many boundaries inside one
+    bucket (weights from 1e-7 to 1), a negative weight (the cumulative sums
+    decrease, so the buckets are off and the scan runs), and the 128-transform
+    maximum."""
+    say("selection buckets", kind)
+    rng = configs.SplitMix64(4242)
+    n = {"dense": 40, "negative": 7, "max": 128}[kind]
+    xs = []
+    for i in range(n):
+        a = configs._affine(rng, 0.7)
+        w = 10.0 ** (-7.0 * rng.uniform()) if kind == "dense" else rng.uniform(0.1, 1.0)
+        if kind == "negative" and i == 3:
+            w = -0.4
+        xs.append(pyfae.xform(*a, weight=w, color=i / (n - 1), variations={"linear": 1.0}))
+    fd = pyfae.flame_desc(xs)
+    pal = configs.random_palette(configs.SplitMix64(6))
+    view = pyfae.View(320, 240, 1.0, 0.0, 0.0, 0.0)
+    for w in (0, 12345):
+        assert np.array_equal(gpu_trace(engine, fd, pal, view, w, 1, 20, 400),
+                              O.trace(fd, pal, view, w, 1, 20, 400))
+    for mode in (0, 2):
+        g, plotted, _ = gpu_hist(engine, fd, pal, view, 1 << 14, 200, 20, 3, mode=mode)
+        o, oplotted = O.iterate_mt(fd, pal, view, 1 << 14, 200, 20, 3, 8)
+        assert plotted == oplotted
+        assert np.array_equal(g, o)
