@@ -345,10 +345,6 @@ template <bool EXT>
 __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x, float& y,
                                             Rng& rng) {
   const float* row = fl.rows + (fl.n_xforms + i) * XF_STRIDE;
-  if (!fl.hdr) {
-    apply_xform<EXT>(row, fl.ids, fl.n_vars, x, y, rng);
-    return;
-  }
   float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
   float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
   float rx = 0.0f, ry = 0.0f;
@@ -503,7 +499,9 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
   // finals this transform attaches, in order.
   float fx = nx, fy = ny;
   if (fl.n_finals > 0) {
-    if (fl.hdr && fl.hdr->fin_uniform) {
+    // without EXT every transform applies the same chain (the host picks
+    // the EXT kernels for chains that differ by transform)
+    if (!EXT || fl.hdr->fin_uniform) {
       const int nc = __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain_n);
       for (int j = 0; j < nc; j++)
         apply_final<EXT>(fl, __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain[j]), fx, fy, rng);

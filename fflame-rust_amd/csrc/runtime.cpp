@@ -152,6 +152,13 @@ bool supported_var(int id) {
 // thirteen of the workloads (their kernels leave those paths out).
 bool flame_ext(const ffm_flame_desc* f) {
   if (f->has_xaos) return true;
+  if (f->n_finals > 0 && f->per_transform_finals)  // chains that differ by transform
+    for (int i = 1; i < f->n_xforms; i++) {
+      const ffm_xform &x0 = f->xforms[0], &x = f->xforms[i];
+      if (x.n_final_attach != x0.n_final_attach ||
+          !std::equal(x.final_attach, x.final_attach + x.n_final_attach, x0.final_attach))
+        return true;
+    }
   for (int k = 0; k < f->n_vars; k++)
     switch (f->var_id[k]) {
       case FFM_VAR_BLOB: case FFM_VAR_NOISE: case FFM_VAR_BLUR: case FFM_VAR_GAUSSIAN_BLUR:
