@@ -513,7 +513,8 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
     }
   }
   world_to_pixel(v, fx, fy, px, py);
-  if (px >= 0 && px < v.width && py >= 0 && py < v.height && should_plot) {
+  // in frame: 0 <= px < width and 0 <= py < height, as unsigned compares
+  if ((uint32_t)px < (uint32_t)v.width && (uint32_t)py < (uint32_t)v.height && should_plot) {
     texel = palette_texel(ci, pal_size);
     return ST_PLOT;
   }

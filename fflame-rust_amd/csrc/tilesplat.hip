@@ -141,7 +141,8 @@ __device__ void block_exclusive_scan(uint32_t* a, int n, uint32_t* wave_sums, ui
 }
 
 struct BinLds {
-  uint2* st;        // stage: (record, bin | rank among the stage's records of the bin << 16)
+  uint32_t* rec;
+  uint32_t* tr;
   uint32_t* cnt;    // bins_lds: the stage's records per bin (COUNT_ONLY: the walk's)
   uint32_t* gbase;  // bins_lds: the bin's first record slot of this flush (region + fill)
   uint16_t* lim;    // bins_lds: slots left in the bin's region (at most 65535)
@@ -210,13 +211,13 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
   __syncthreads();
   // ranks below 2^14 (the stage), so `rank < lim` is `fill + rank < cap`
   for (int j = tid; j < n; j += blockDim.x) {
-    const uint2 e = L.st[j];
-    const int tile = (int)(e.y & 0xFFFFu);
-    const uint32_t rank = e.y >> 16;
+    const uint32_t v = L.tr[j];
+    const int tile = (int)(v & 0xFFFFu);
+    const uint32_t rank = v >> 16;
     if (rank < L.lim[tile])
-      t.records[(size_t)(L.gbase[tile] + rank)] = e.x;
+      t.records[(size_t)(L.gbase[tile] + rank)] = L.rec[j];
     else
-      overflow_splat(t, pal, tile, e.x);
+      overflow_splat(t, pal, tile, L.rec[j]);
   }
   __syncthreads();
   if (tid == 0) *L.fill = 0u;
@@ -274,7 +275,8 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
             // the wave's increments of one address one LDS atomic by its
             // first active lane, each lane taking base + its rank in the wave
             const uint32_t b = atomicAdd(L.fill, 1u);
-            L.st[b] = make_uint2(local | ((uint32_t)texel << REC_TEXEL_SHIFT), tile | r << 16);
+            L.rec[b] = local | ((uint32_t)texel << REC_TEXEL_SHIFT);
+            L.tr[b] = tile | r << 16;
           }
         }
       }
@@ -296,7 +298,8 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
 // slots (a multiple of 64).
 __device__ __forceinline__ BinLds bin_lds_views(uint32_t* p, int stage, int nb) {
   BinLds L;
-  L.st = reinterpret_cast<uint2*>(p);
+  L.rec = p;
+  L.tr = p + stage;
   L.cnt = p + 2 * stage;
   L.gbase = L.cnt + nb;
   L.lim = reinterpret_cast<uint16_t*>(L.gbase + nb);
@@ -610,6 +613,18 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const
 
 // ------------------------------------------------------------ batches
 
+// A copy of *p read through the constant address space: with a uniform p,
+// scalar loads into SGPRs (a global load would hold the words in VGPRs).
+template <class T>
+__device__ __forceinline__ T scalar_copy(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  typedef const __attribute__((address_space(4))) uint32_t KWord;
+  T v;
+  uint32_t* d = reinterpret_cast<uint32_t*>(&v);
+  for (int i = 0; i < (int)(sizeof(T) / 4); i++) d[i] = ((KWord*)p)[i];
+  return v;
+}
+
 // A flame's view of the global tile arrays.
 __device__ __forceinline__ TileArgs flame_tiles(const BatchArgs& b, const BatchFlame& f) {
   TileArgs t = b.g;
@@ -644,11 +659,13 @@ __global__ __launch_bounds__(BIN_BLOCK) FFM_WALKER_WAVES void bin_batch_kernel(B
     // dynamic work stealing: items differ in cost (transform counts, sizes)
     if (tid == 0) next = (int)atomicAdd(b.next_item, 1u);
     __syncthreads();
-    const int item = next;
+    const int item = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();
     if (item >= b.n_items) break;
-    const BatchItem it = b.items[item];
-    const BatchFlame f = b.flames[it.flame];
+    // the item and its flame by scalar loads: held in VGPRs, their thirty
+    // words would push the walk into scratch
+    const BatchItem it = scalar_copy(b.items + item);
+    const BatchFlame f = scalar_copy(b.flames + it.flame);
     const LdsFlame F = load_table<false>(b.tables + f.table_off, b.pal_u32 + f.pal_off, nullptr,
                                          f.pal_size, lds);
     IterArgs a{};
