@@ -231,27 +231,42 @@ def plain_extra(eng, steps=3):
 def gallery_run(eng, n, steps):
     """Batched flames/s (persistent launches in one hipGraph; the first call
     autotunes and builds the graph, untimed) against the same flames one by
-    one."""
+    one, both with the pictures left in device memory; then both again with
+    every picture read back into host memory (PCIe included)."""
+    import torch
     from pyfae import configs
     items = configs.gallery(n=n)
     t0 = time.perf_counter()
-    eng.batch_render(items)
+    eng.batch_render(items, device=True)
     first = time.perf_counter() - t0
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        _, st = eng.batch_render(items)
+        outs, st = eng.batch_render(items, device=True)
+        del outs
     dt = (time.perf_counter() - t0) / steps
     t1 = time.perf_counter()
-    _, st_s = eng.batch_render(items, serial=True)
+    outs, st_s = eng.batch_render(items, serial=True, device=True)
     dt_s = time.perf_counter() - t1
+    del outs
+    t2 = time.perf_counter()
+    eng.batch_render(items)
+    dt_h = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    eng.batch_render(items, serial=True)
+    dt_sh = time.perf_counter() - t3
     px = sum(j.view.width * j.view.height for _, _, j in items)
     return {"workload": "gallery: %d flames, 2..12 xforms, sides log-uniform 128..1024, "
-                        "200 samples/pixel, ipt 256" % n,
+                        "200 samples/pixel, ipt 256, pictures left in device memory" % n,
             "flames": n, "pixels": px, "samples_per_step": st.total_samples,
             "batched_flames_per_s": n / dt, "batched_ms": dt * 1e3, "steps": steps,
+            "batched_graph_gpu_ms": st.iterate_ms,
             "serial_flames_per_s": n / dt_s, "serial_ms": dt_s * 1e3,
             "speedup_vs_serial": dt_s / dt, "samples_per_s": st.total_samples / dt,
-            "first_call_s": first}
+            "first_call_s": first,
+            "host_buffers": {"note": "every picture read back into pageable host memory",
+                             "batched_ms": dt_h * 1e3, "serial_ms": dt_sh * 1e3,
+                             "speedup_vs_serial": dt_sh / dt_h}}
 
 
 # ------------------------------------------------------------------- fits

@@ -133,7 +133,7 @@ constexpr uint32_t REC_LOCAL_MASK = 0x1FFFu, REC_TEXEL_MASK = 0xFFFu;
 
 struct TileArgs {
   uint32_t* tile_fill;   // n_tiles: records requested per bin
-  uint32_t* region;      // n_tiles: first slot of the bin's region in `records`
+  uint64_t* region;      // n_tiles: first slot of the bin's region in `records` (up to 2^33)
   uint32_t* cap;         // n_tiles: slots in the region
   uint32_t* item_start;  // n_tiles + 1: work list of tile_accum
   uint32_t* records;     // local | texel << 13 | tile-in-bin << 25
@@ -283,7 +283,7 @@ hipError_t launch_bin_batch(const BatchArgs& b, bool count_only, int grid, hipSt
 hipError_t launch_tile_capacity_n(const TileArgs& t, float scale, uint32_t margin,
                                   uint64_t buffer_records, hipStream_t s);
 hipError_t launch_tile_accum_batch(const BatchArgs& b, int grid, hipStream_t s);
-hipError_t launch_batch_finish(const uint32_t* hist, uint8_t* rgba, const BatchPix* fl, int n,
-                               hipStream_t s);
+hipError_t launch_batch_finish(const uint32_t* hist, uint8_t* const* out, const BatchPix* fl,
+                               int n, hipStream_t s);
 
 }  // namespace ffm

@@ -277,10 +277,12 @@ int packed_rows(const ffm_flame_desc* f) {
 
 // The pick of select_transform_const (shaders/core/utilities.wgsl:35-55) for
 // the 24-bit draw d: the first i with u * total <= cum_i, else the last.
-static int pick_for_draw(const float* cum, int n, uint32_t d) {
+// With non-decreasing sums the pick never decreases with d, so a scan over
+// increasing draws starts from the previous pick (`from`).
+static int pick_for_draw(const float* cum, int n, uint32_t d, int from) {
   const float u = (float)d * (1.0f / 16777216.0f);
   const float target = u * cum[n - 1];
-  for (int i = 0; i < n - 1; i++)
+  for (int i = from; i < n - 1; i++)
     if (target <= cum[i]) return i;
   return n - 1;
 }
@@ -372,9 +374,10 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
       const int off = rows * XF_STRIDE;
       uint8_t* sel = reinterpret_cast<uint8_t*>(r0 + off);
       const uint32_t span = 1u << (24 - SEL_BITS);
+      int hi = 0;
       for (uint32_t b = 0; b < (1u << SEL_BITS); b++) {
-        const int lo = pick_for_draw(cum.data(), f->n_xforms, b * span);
-        const int hi = pick_for_draw(cum.data(), f->n_xforms, b * span + span - 1);
+        const int lo = pick_for_draw(cum.data(), f->n_xforms, b * span, hi);
+        hi = pick_for_draw(cum.data(), f->n_xforms, b * span + span - 1, lo);
         sel[b] = (uint8_t)(lo | (lo == hi ? 0 : 0x80));
       }
       reinterpret_cast<FlameHeader*>(t.data())->sel_off = off;
@@ -648,13 +651,15 @@ hipError_t collect_timings(ffm_ctx* c, double& walker_ms, uint64_t& launches, do
 // nothing is allocated, the plan is made afresh on every call (the memory
 // is shared by every call on the device) and no timing events are recorded.
 struct TileScratch {
-  uint32_t *fill, *region, *cap, *items, *records;
+  uint32_t *fill, *cap, *items, *records;
+  uint64_t* region;
   uint64_t records_cap;  // u32 words at `records`
 };
 
-// The scratch of ffm_iterate_noalloc laid out: fill, region, cap, items,
-// then the records (and, for bins of several tiles, their sorted copy).
-constexpr size_t NOALLOC_TABLE_WORDS = 3 * (size_t)TILE_MAX + TILE_MAX + 64;
+// The scratch of ffm_iterate_noalloc laid out: fill, region (two words a
+// bin), cap, items, then the records (and, for bins of several tiles, their
+// sorted copy).
+constexpr size_t NOALLOC_TABLE_WORDS = 5 * (size_t)TILE_MAX + 64;
 static_assert(FFM_NOALLOC_SCRATCH_BYTES > NOALLOC_TABLE_WORDS * 4 + (64u << 20),
               "scratch holds the tables and a useful record buffer");
 
@@ -662,9 +667,9 @@ TileScratch noalloc_layout(void* base) {
   uint32_t* w = reinterpret_cast<uint32_t*>(base);
   TileScratch t;
   t.fill = w;
-  t.region = w + TILE_MAX;
-  t.cap = w + 2 * TILE_MAX;
-  t.items = w + 3 * TILE_MAX;
+  t.region = reinterpret_cast<uint64_t*>(w + TILE_MAX);
+  t.cap = w + 3 * TILE_MAX;
+  t.items = w + 4 * TILE_MAX;
   t.records = w + NOALLOC_TABLE_WORDS;
   t.records_cap = FFM_NOALLOC_SCRATCH_BYTES / 4 - NOALLOC_TABLE_WORDS;
   return t;
@@ -717,7 +722,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
     t.sorted = g > 0 ? ext->records + rec_cap : nullptr;
   } else {
     if ((st = ensure(c, c->t_fill, (size_t)TILE_MAX * 4, "tile fills"))) return st;
-    if ((st = ensure(c, c->t_region, (size_t)TILE_MAX * 4, "tile regions"))) return st;
+    if ((st = ensure(c, c->t_region, (size_t)TILE_MAX * 8, "tile regions"))) return st;
     if ((st = ensure(c, c->t_cap, (size_t)TILE_MAX * 4, "tile capacities"))) return st;
     if ((st = ensure(c, c->t_items, (size_t)(TILE_MAX + 4) * 4, "tile work list"))) return st;
     if ((st = ensure(c, c->t_records, rec_cap * 4, "tile records"))) return st;
@@ -725,7 +730,7 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
       return st;
     t.sorted = g > 0 ? reinterpret_cast<uint32_t*>(c->t_sorted.p) : nullptr;
     t.tile_fill = reinterpret_cast<uint32_t*>(c->t_fill.p);
-    t.region = reinterpret_cast<uint32_t*>(c->t_region.p);
+    t.region = reinterpret_cast<uint64_t*>(c->t_region.p);
     t.cap = reinterpret_cast<uint32_t*>(c->t_cap.p);
     t.item_start = reinterpret_cast<uint32_t*>(c->t_items.p);
     t.records = reinterpret_cast<uint32_t*>(c->t_records.p);
@@ -1775,11 +1780,11 @@ namespace {
 
 // Records staged per wave of a batch: an eighth of the device memory free
 // when the batch is laid out (on a 288 GB MI355X: 9 G records, 36 GB), at
-// least 2^28 and at most 2^34 records, and no more than the batch needs.
-// Fewer, larger waves cut the persistent launches' tails: the 512-flame
-// gallery walks in 4 waves instead of 11 at a fixed 12 GB (702 vs 654
-// flames/s).
-constexpr uint64_t BATCH_RECORD_MIN = 1ull << 28, BATCH_RECORD_MAX = 1ull << 34;
+// least 2^28 records and no more than the batch needs, and below 2^33 (a
+// flush keeps a run's start in 32 bits plus one bit beside its room).
+// Fewer, larger waves cut the persistent launches' tails and let the tile
+// accumulation fold larger slices: the 512-flame gallery walks in 5 waves.
+constexpr uint64_t BATCH_RECORD_MIN = 1ull << 28, BATCH_RECORD_MAX = (1ull << 33) - (1ull << 24);
 
 uint64_t batch_record_budget(uint64_t needed, size_t held_bytes) {
   size_t free_b = 0, total_b = 0;
@@ -1805,6 +1810,7 @@ struct BatchPlan {
   std::vector<uint64_t> samples;     // plotted samples per flame
   DevBuf flames, items, tables, pals, pixd, hist, rgba, fill, region, cap, item_start, records,
       counters;
+  DevBuf outp;  // per flame, where the finish kernel writes its pixels (set every call)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   void drop_graph() {
@@ -2009,9 +2015,10 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   if ((st = ensure(c, P.pixd, P.pix.size() * sizeof(BatchPix), "batch images"))) return st;
   if ((st = ensure(c, P.hist, P.total_pix * 16, "batch histograms"))) return st;
   if ((st = ensure(c, P.rgba, P.total_pix * 4, "batch pixels"))) return st;
+  if ((st = ensure(c, P.outp, (size_t)n * sizeof(void*), "batch pixel pointers"))) return st;
   const size_t tiles_pad = ((size_t)P.n_tiles + 8) & ~(size_t)3;
   if ((st = ensure(c, P.fill, tiles_pad * 4, "batch tiles"))) return st;
-  if ((st = ensure(c, P.region, tiles_pad * 4, "batch tiles"))) return st;
+  if ((st = ensure(c, P.region, tiles_pad * 8, "batch tiles"))) return st;
   if ((st = ensure(c, P.cap, tiles_pad * 4, "batch tiles"))) return st;
   if ((st = ensure(c, P.item_start, tiles_pad * 4, "batch tiles"))) return st;
   if ((st = ensure(c, P.records, budget * 4, "batch records"))) return st;
@@ -2038,7 +2045,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.count_div = BATCH_COUNT_DIV;
   for (int i = 0; i < n; i++) b.ext |= flame_ext(&flames[i]) ? 1 : 0;
   b.g.tile_fill = reinterpret_cast<uint32_t*>(P.fill.p);
-  b.g.region = reinterpret_cast<uint32_t*>(P.region.p);
+  b.g.region = reinterpret_cast<uint64_t*>(P.region.p);
   b.g.cap = reinterpret_cast<uint32_t*>(P.cap.p);
   b.g.item_start = reinterpret_cast<uint32_t*>(P.item_start.p);
   b.g.records = reinterpret_cast<uint32_t*>(P.records.p);
@@ -2083,7 +2090,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     }
     step = "accumulate + tonemap";
     return launch_batch_finish(reinterpret_cast<const uint32_t*>(P.hist.p),
-                               reinterpret_cast<uint8_t*>(P.rgba.p),
+                               reinterpret_cast<uint8_t* const*>(P.outp.p),
                                reinterpret_cast<const BatchPix*>(P.pixd.p), n, s);
   };
   hipError_t eb = body();
@@ -2101,9 +2108,10 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
 }
 
 // Per-call uploads: flame tables and palettes (their content may change
-// between calls with the same layout).
+// between calls with the same layout) and where each flame's pixels go: the
+// caller's device buffers, or (dev_out NULL) the plan's pixel buffer.
 int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const float* const* stops,
-                 const int* n_stops) {
+                 const int* n_stops, uint8_t* const* dev_out) {
   std::vector<float> tables;
   std::vector<uint32_t> pals((size_t)P.n * BATCH_PALETTE);
   std::vector<float> lin;
@@ -2117,9 +2125,14 @@ int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const f
     palette_tables(stops[i], n_stops[i], BATCH_PALETTE, pk, lin);
     std::copy(pk.begin(), pk.end(), pals.begin() + (size_t)i * BATCH_PALETTE);
   }
+  std::vector<uint8_t*> outp((size_t)P.n);
+  for (int i = 0; i < P.n; i++)
+    outp[i] = dev_out ? dev_out[i] : reinterpret_cast<uint8_t*>(P.rgba.p) + P.pix[i].pix_off * 4;
   hipStream_t s = c->stream;
   HIPCHECK(c, hipMemcpyAsync(P.tables.p, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, s));
   HIPCHECK(c, hipMemcpyAsync(P.pals.p, pals.data(), pals.size() * 4, hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(P.outp.p, outp.data(), outp.size() * sizeof(void*),
+                             hipMemcpyHostToDevice, s));
   HIPCHECK(c, hipStreamSynchronize(s));  // the host vectors die on return
   return FFM_OK;
 }
@@ -2136,17 +2149,14 @@ int run_batch_graph(ffm_ctx* c, BatchPlan& P, float* gpu_ms) {
 
 }  // namespace
 
-// Many flames in one persistent launch per wave, the whole sequence a
-// hipGraph. The first call for a batch layout autotunes walkers per work
-// item (walkers per lane = item / 512) and the staging flush cadence on
-// the batch itself, then the tile shape with the winner of those, keeping
-// the fastest; the graph is reused while the layout (sizes, walker counts,
-// transform counts) stays the same.
-int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* const* stops,
-                     const int* n_stops, const ffm_job* jobs, uint8_t* const* rgba_out,
-                     ffm_render_stats* stats) {
+namespace {
+// The batched render; `dev_out` names device buffers for the pictures, else
+// they are read back into the host buffers `rgba_out`.
+int batch_render_impl(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* const* stops,
+                      const int* n_stops, const ffm_job* jobs, uint8_t* const* rgba_out,
+                      uint8_t* const* dev_out, ffm_render_stats* stats) {
   if (!c) return FFM_ERR_INVALID_ARGUMENT;
-  int st = check_batch(c, n, flames, stops, n_stops, jobs, rgba_out);
+  int st = check_batch(c, n, flames, stops, n_stops, jobs, dev_out ? dev_out : rgba_out);
   if (st) return st;
   (void)hipSetDevice(c->device);
   auto t0 = std::chrono::steady_clock::now();
@@ -2188,7 +2198,7 @@ int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const floa
     for (uint32_t iw : kItems)
       for (int fe : kFlush) {
         if ((st = build_batch(c, P, n, flames, jobs, iw, fe, mode0))) return st;
-        if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
+        if ((st = upload_batch(c, P, flames, stops, n_stops, dev_out))) return st;
         float ms = 0.0f;
         if ((st = run_batch_graph(c, P, &ms))) return st;
         if (ms < best) {
@@ -2207,7 +2217,7 @@ int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const floa
       if (!fits) continue;
       if ((st = build_batch(c, P, n, flames, jobs, tb.item_walkers, tb.flush_every, mode)))
         return st;
-      if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
+      if ((st = upload_batch(c, P, flames, stops, n_stops, dev_out))) return st;
       float ms = 0.0f;
       if ((st = run_batch_graph(c, P, &ms))) return st;
       if (ms < best) {
@@ -2226,14 +2236,17 @@ int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const floa
                           tune->tile_mode)))
       return st;
   }
-  if ((st = upload_batch(c, P, flames, stops, n_stops))) return st;
+  if ((st = upload_batch(c, P, flames, stops, n_stops, dev_out))) return st;
   if ((st = run_batch_graph(c, P, &gpu_ms))) return st;
   uint64_t total = 0;
   for (int i = 0; i < n; i++) {
-    hipError_t e = hipMemcpyAsync(rgba_out[i], reinterpret_cast<uint8_t*>(P.rgba.p) + P.pix[i].pix_off * 4,
-                                  P.pix[i].npix * 4, hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess)
-      return fail(c, FFM_ERR_PIXEL_READ_FAILED, "pixel readback: %s", hipGetErrorString(e));
+    if (!dev_out) {
+      hipError_t e = hipMemcpyAsync(rgba_out[i],
+                                    reinterpret_cast<uint8_t*>(P.rgba.p) + P.pix[i].pix_off * 4,
+                                    P.pix[i].npix * 4, hipMemcpyDeviceToHost, c->stream);
+      if (e != hipSuccess)
+        return fail(c, FFM_ERR_PIXEL_READ_FAILED, "pixel readback: %s", hipGetErrorString(e));
+    }
     total += P.samples[i];
   }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -2251,12 +2264,13 @@ int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const floa
 }
 
 // The reference's gallery behaviour: the same flames rendered one after
-// another (wasm/render/src/lib.rs:45-53), each a full ffm_render.
-int ffm_batch_render_serial(ffm_ctx* c, int n, const ffm_flame_desc* flames,
-                            const float* const* stops, const int* n_stops, const ffm_job* jobs,
-                            uint8_t* const* rgba_out, ffm_render_stats* stats) {
+// another (wasm/render/src/lib.rs:45-53), each a full ffm_render (or
+// ffm_render_to_device into dev_out[i]).
+int batch_serial_impl(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* const* stops,
+                      const int* n_stops, const ffm_job* jobs, uint8_t* const* rgba_out,
+                      uint8_t* const* dev_out, ffm_render_stats* stats) {
   if (!c) return FFM_ERR_INVALID_ARGUMENT;
-  int st = check_batch(c, n, flames, stops, n_stops, jobs, rgba_out);
+  int st = check_batch(c, n, flames, stops, n_stops, jobs, dev_out ? dev_out : rgba_out);
   if (st) return st;
   auto t0 = std::chrono::steady_clock::now();
   uint64_t total = 0;
@@ -2269,7 +2283,9 @@ int ffm_batch_render_serial(ffm_ctx* c, int n, const ffm_flame_desc* flames,
     j.persistent_orbits = 0;
     j.splat_mode = FFM_SPLAT_U32_LDS;
     ffm_render_stats one{};
-    if ((st = ffm_render(c, &j, rgba_out[i], &one))) return st;
+    if ((st = dev_out ? ffm_render_to_device(c, &j, dev_out[i], &one)
+                      : ffm_render(c, &j, rgba_out[i], &one)))
+      return st;
     total += one.total_samples;
     it_ms += one.iterate_ms;
   }
@@ -2284,6 +2300,39 @@ int ffm_batch_render_serial(ffm_ctx* c, int n, const ffm_flame_desc* flames,
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return FFM_OK;
+}
+
+}  // namespace
+
+// Many flames in one persistent launch per wave, the whole sequence a
+// hipGraph. The first call for a batch layout autotunes walkers per work
+// item (walkers per lane = item / 512) and the staging flush cadence on
+// the batch itself, then the tile shape with the winner of those, keeping
+// the fastest; the graph is reused while the layout (sizes, walker counts,
+// transform counts) stays the same.
+int ffm_batch_render(ffm_ctx* c, int n, const ffm_flame_desc* flames, const float* const* stops,
+                     const int* n_stops, const ffm_job* jobs, uint8_t* const* rgba_out,
+                     ffm_render_stats* stats) {
+  return batch_render_impl(c, n, flames, stops, n_stops, jobs, rgba_out, nullptr, stats);
+}
+
+int ffm_batch_render_to_device(ffm_ctx* c, int n, const ffm_flame_desc* flames,
+                               const float* const* stops, const int* n_stops, const ffm_job* jobs,
+                               uint8_t* const* rgba_dev, ffm_render_stats* stats) {
+  return batch_render_impl(c, n, flames, stops, n_stops, jobs, nullptr, rgba_dev, stats);
+}
+
+int ffm_batch_render_serial(ffm_ctx* c, int n, const ffm_flame_desc* flames,
+                            const float* const* stops, const int* n_stops, const ffm_job* jobs,
+                            uint8_t* const* rgba_out, ffm_render_stats* stats) {
+  return batch_serial_impl(c, n, flames, stops, n_stops, jobs, rgba_out, nullptr, stats);
+}
+
+int ffm_batch_render_serial_to_device(ffm_ctx* c, int n, const ffm_flame_desc* flames,
+                                      const float* const* stops, const int* n_stops,
+                                      const ffm_job* jobs, uint8_t* const* rgba_dev,
+                                      ffm_render_stats* stats) {
+  return batch_serial_impl(c, n, flames, stops, n_stops, jobs, nullptr, rgba_dev, stats);
 }
 
 }  // extern "C"

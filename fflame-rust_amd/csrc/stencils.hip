@@ -489,13 +489,13 @@ hipError_t launch_tonemap(const float* accum, uint8_t* rgba, int W, int H, const
 // Accumulate (fresh accumulator, cumulative mode) then tonemap, for many
 // flames in one launch: blockIdx.y is the flame, blocks stride its pixels.
 __global__ __launch_bounds__(256) void batch_finish_kernel(const uint4* __restrict__ hist,
-                                                           uchar4* __restrict__ rgba,
+                                                           uchar4* const* __restrict__ out,
                                                            const BatchPix* __restrict__ fl) {
   // by reference: a copy of the descriptor (its 1 KB curve table indexed by
   // pixel values) would live in scratch memory
   const BatchPix& f = fl[blockIdx.y];
   const uint4* h = hist + f.pix_off;
-  uchar4* o = rgba + f.pix_off;
+  uchar4* o = out[blockIdx.y];
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < f.npix;
        p += (size_t)gridDim.x * blockDim.x) {
     const uint4 v = h[p];
@@ -505,10 +505,11 @@ __global__ __launch_bounds__(256) void batch_finish_kernel(const uint4* __restri
   }
 }
 
-hipError_t launch_batch_finish(const uint32_t* hist, uint8_t* rgba, const BatchPix* fl, int n,
-                               hipStream_t s) {
+hipError_t launch_batch_finish(const uint32_t* hist, uint8_t* const* out, const BatchPix* fl,
+                               int n, hipStream_t s) {
   hipLaunchKernelGGL(batch_finish_kernel, dim3(64, n), dim3(256), 0, s,
-                     reinterpret_cast<const uint4*>(hist), reinterpret_cast<uchar4*>(rgba), fl);
+                     reinterpret_cast<const uint4*>(hist), reinterpret_cast<uchar4* const*>(out),
+                     fl);
   return hipGetLastError();
 }
 

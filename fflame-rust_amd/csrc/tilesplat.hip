@@ -144,8 +144,9 @@ struct BinLds {
   uint32_t* rec;
   uint32_t* tr;
   uint32_t* cnt;    // bins_lds: the stage's records per bin (COUNT_ONLY: the walk's)
-  uint32_t* gbase;  // bins_lds: the bin's first record slot of this flush (region + fill)
-  uint16_t* lim;    // bins_lds: slots left in the bin's region (at most 65535)
+  uint32_t* gbase;  // bins_lds: the low 32 bits of the bin's first record slot of this flush
+  uint16_t* lim;    // bins_lds: slots left in the bin's region (at most 2^15 - 1), bit 15 the
+                    // slot's bit 32
   uint32_t* wsum;   // 16
   uint32_t* fill;   // 1, then one word of scan total
 };
@@ -194,28 +195,30 @@ __device__ void flush_stage(const TileArgs& t, const BinLds& L, const uint32_t* 
   // start over for the next stage
   for (int i = tid; i < t.n_tiles; i += blockDim.x) {
     const uint32_t c = L.cnt[i];
-    uint32_t base = 0u, room = 0u;
+    uint64_t base = 0u;
+    uint32_t room = 0u;
     if (c) {
       const uint32_t old =
           __hip_atomic_fetch_add(t.tile_fill + i, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t cap = t.cap[i];
       if (cap > old) {
         base = t.region[i] + old;
-        room = min(cap - old, 0xFFFFu);
+        room = min(cap - old, 0x7FFFu);
       }
       L.cnt[i] = 0u;
     }
-    L.gbase[i] = base;
-    L.lim[i] = (uint16_t)room;
+    L.gbase[i] = (uint32_t)base;
+    L.lim[i] = (uint16_t)(room | (uint32_t)(base >> 32) << 15);
   }
   __syncthreads();
-  // ranks below 2^14 (the stage), so `rank < lim` is `fill + rank < cap`
+  // ranks below 2^14 (the stage), so `rank < room` is `fill + rank < cap`
   for (int j = tid; j < n; j += blockDim.x) {
     const uint32_t v = L.tr[j];
     const int tile = (int)(v & 0xFFFFu);
     const uint32_t rank = v >> 16;
-    if (rank < L.lim[tile])
-      t.records[(size_t)(L.gbase[tile] + rank)] = L.rec[j];
+    const uint32_t lim = L.lim[tile];
+    if (rank < (lim & 0x7FFFu))
+      t.records[((size_t)(lim >> 15) << 32) + L.gbase[tile] + rank] = L.rec[j];
     else
       overflow_splat(t, pal, tile, L.rec[j]);
   }
@@ -375,15 +378,13 @@ __device__ uint32_t block_scan_n(int n, Fn value, uint32_t* out, uint32_t* buf, 
 }
 
 // Capacities from the pre-pass counts: cap = est * scale * 1.25 + margin,
-// regions laid out back to back, scaled down to fit `buffer_records` if
-// needed; then the fill counters are cleared for the real pass. One
-// workgroup; any number of tiles.
+// regions laid out back to back (64-bit starts: a buffer may pass 2^32
+// records), scaled down to fit `buffer_records` if needed; then the fill
+// counters are cleared for the real pass. One workgroup; any number of tiles.
 __global__ __launch_bounds__(BIN_BLOCK) void tile_capacity_kernel(TileArgs t, float scale,
                                                                   uint32_t margin,
                                                                   uint64_t buffer_records) {
-  __shared__ uint32_t buf[2 * BIN_BLOCK];
-  __shared__ uint32_t wsum[16];
-  __shared__ uint32_t total;
+  __shared__ unsigned long long part[BIN_BLOCK];
   __shared__ unsigned long long sum64;
   const int tid = threadIdx.x;
   auto cap_of = [&](int i) -> uint32_t {
@@ -401,9 +402,29 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_capacity_kernel(TileArgs t, fl
     const uint32_t c = cap_of(i);
     return f < 1.0 ? (uint32_t)((double)c * f) : c;
   };
-  for (int i = tid; i < t.n_tiles; i += blockDim.x) t.cap[i] = scaled(i);
+  // exclusive scan of the capacities: each thread a contiguous run of
+  // tiles, the runs' sums scanned across the workgroup
+  const int per = (t.n_tiles + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int i0 = min(tid * per, t.n_tiles), i1 = min(i0 + per, t.n_tiles);
+  unsigned long long run = 0ull;
+  for (int i = i0; i < i1; i++) {
+    const uint32_t c = scaled(i);
+    t.cap[i] = c;
+    run += c;
+  }
+  part[tid] = run;
   __syncthreads();
-  block_scan_n(t.n_tiles, [&](int i) { return t.cap[i]; }, t.region, buf, wsum, &total);
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    const unsigned long long v = tid >= off ? part[tid - off] : 0ull;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  run = tid > 0 ? part[tid - 1] : 0ull;
+  for (int i = i0; i < i1; i++) {
+    t.region[i] = run;
+    run += t.cap[i];
+  }
   for (int i = tid; i < t.n_tiles; i += blockDim.x) t.tile_fill[i] = 0u;
 }
 

@@ -525,7 +525,7 @@ class Engine:
         return out, stats
 
     # -- batched gallery
-    def _batch_args(self, items):
+    def _batch_args(self, items, device=False):
         n = len(items)
         flames = (FlameDesc * n)()
         jobs = (Job * n)()
@@ -535,19 +535,31 @@ class Engine:
             C.memmove(C.byref(flames[i]), C.byref(fd), C.sizeof(FlameDesc))
             C.memmove(C.byref(jobs[i]), C.byref(job), C.sizeof(Job))
             stops.append(palette_stops(pal))
-            outs.append(np.zeros((job.view.height, job.view.width, 4), dtype=np.uint8))
+            shape = (job.view.height, job.view.width, 4)
+            if device:
+                import torch
+                outs.append(torch.empty(shape, dtype=torch.uint8, device="cuda:%d" % self.device))
+            else:
+                outs.append(np.zeros(shape, dtype=np.uint8))
         sp = (C.POINTER(C.c_float) * n)(*[a.ctypes.data_as(C.POINTER(C.c_float)) for a in stops])
         ns = (C.c_int * n)(*[a.shape[0] for a in stops])
-        op = (C.POINTER(C.c_uint8) * n)(*[o.ctypes.data_as(C.POINTER(C.c_uint8)) for o in outs])
+        ptrs = [C.cast(C.c_void_p(o.data_ptr()), C.POINTER(C.c_uint8)) if device
+                else o.ctypes.data_as(C.POINTER(C.c_uint8)) for o in outs]
+        op = (C.POINTER(C.c_uint8) * n)(*ptrs)
         return n, flames, sp, ns, jobs, op, outs, stops
 
-    def batch_render(self, items, serial=False):
+    def batch_render(self, items, serial=False, device=False):
         """Render [(flame_desc, palette_stops, job), ...]: one persistent launch
         per wave in a hipGraph (serial=False) or one ffm_render per flame
-        (serial=True). Returns ([rgba arrays], stats)."""
-        n, flames, sp, ns, jobs, op, outs, keep = self._batch_args(items)
+        (serial=True). Returns ([rgba arrays], stats); with device=True the
+        pictures stay on the GPU ([uint8 tensors on the engine's device])."""
+        n, flames, sp, ns, jobs, op, outs, keep = self._batch_args(items, device)
         stats = RenderStats()
-        fn = self.lib.ffm_batch_render_serial if serial else self.lib.ffm_batch_render
+        if device:
+            fn = (self.lib.ffm_batch_render_serial_to_device if serial
+                  else self.lib.ffm_batch_render_to_device)
+        else:
+            fn = self.lib.ffm_batch_render_serial if serial else self.lib.ffm_batch_render
         self._check(fn(self.ctx, n, flames, sp, ns, jobs, op, C.byref(stats)))
         return outs, stats
 

@@ -269,6 +269,12 @@ SIGNATURES = [
     ("ffm_batch_render_serial", C.c_int, [_vp, C.c_int, C.POINTER(FlameDesc), C.POINTER(_fp),
                                           C.POINTER(C.c_int), C.POINTER(Job), C.POINTER(_u8p),
                                           C.POINTER(RenderStats)]),
+    ("ffm_batch_render_to_device", C.c_int, [_vp, C.c_int, C.POINTER(FlameDesc), C.POINTER(_fp),
+                                             C.POINTER(C.c_int), C.POINTER(Job), C.POINTER(_u8p),
+                                             C.POINTER(RenderStats)]),
+    ("ffm_batch_render_serial_to_device", C.c_int,
+     [_vp, C.c_int, C.POINTER(FlameDesc), C.POINTER(_fp), C.POINTER(C.c_int), C.POINTER(Job),
+      C.POINTER(_u8p), C.POINTER(RenderStats)]),
     ("ffm_softsplat_param_count", C.c_int, [C.POINTER(FlameDesc)]),
     ("ffm_softsplat_pack", C.c_int, [C.POINTER(FlameDesc), _fp]),
     ("ffm_softsplat_set", C.c_int, [_vp, C.c_int, C.POINTER(FlameDesc), C.POINTER(SoftJob)]),

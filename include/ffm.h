@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FFM_ABI_VERSION 9
+#define FFM_ABI_VERSION 10
 
 /* Limits. The reference caps a flame at 128 transforms
  * (src/gpu/buffers.rs:28 MAX_TRANSFORMS) and a palette texture at 256..4096
@@ -361,6 +361,19 @@ int ffm_batch_render_serial(ffm_ctx* ctx, int n, const ffm_flame_desc* flames,
                             const float* const* stops_prgb, const int* n_stops,
                             const ffm_job* jobs, uint8_t* const* rgba_out,
                             ffm_render_stats* stats);
+
+/* ffm_batch_render and ffm_batch_render_serial with the pictures left in
+ * device memory: rgba_dev[i] is a device buffer of flame i's width x height
+ * RGBA8 pixels, as ffm_render_to_device gives one render's (no reference
+ * counterpart: the reference's gallery reads every picture back). */
+int ffm_batch_render_to_device(ffm_ctx* ctx, int n, const ffm_flame_desc* flames,
+                               const float* const* stops_prgb, const int* n_stops,
+                               const ffm_job* jobs, uint8_t* const* rgba_dev,
+                               ffm_render_stats* stats);
+int ffm_batch_render_serial_to_device(ffm_ctx* ctx, int n, const ffm_flame_desc* flames,
+                                      const float* const* stops_prgb, const int* n_stops,
+                                      const ffm_job* jobs, uint8_t* const* rgba_dev,
+                                      ffm_render_stats* stats);
 
 /* ---- differentiable soft splat ---------------------------------------------- */
 

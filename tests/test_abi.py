@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_binds():
     lib = pyfae.library()
-    assert lib.ffm_abi_version() == 9
+    assert lib.ffm_abi_version() == 10
     bound = {name for name, _, _ in _abi.SIGNATURES}
     assert bound == set(declared())
 

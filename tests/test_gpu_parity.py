@@ -477,6 +477,39 @@ def test_batch_equals_serial_and_oracle(engine, tile, monkeypatch):
         assert np.abs(gm - om).mean() < 2.0
 
 
+def test_batch_device_outputs_equal_host_outputs(engine):
+    """The device-resident forms write the same pictures into caller device
+    buffers: batched (then its graph replayed into other buffers) and one by
+    one, against the host-buffer batched render."""
+    say("batch: device outputs")
+    items = configs.gallery(n=5, spp=10.0, ipt=64)
+    outs_h, st_h = engine.batch_render(items)
+    outs_d, st_d = engine.batch_render(items, device=True)
+    outs_d2, _ = engine.batch_render(items, device=True)
+    outs_sd, st_sd = engine.batch_render(items, serial=True, device=True)
+    torch.cuda.synchronize()
+    assert st_h.total_samples == st_d.total_samples == st_sd.total_samples
+    for i, h in enumerate(outs_h):
+        for d in (outs_d[i], outs_d2[i], outs_sd[i]):
+            assert d.device.type == "cuda" and tuple(d.shape) == h.shape
+            assert np.array_equal(d.cpu().numpy(), h), i
+
+
+def test_batch_full_gallery_equals_serial(engine):
+    """The whole 512-flame gallery (28.1e9 samples, so its waves stage
+    billions of records) gives every flame exactly its one-by-one render:
+    wave record buffers past 2^32 records once overflowed the bins' u32
+    record offsets and mixed flames' records."""
+    say("batch: the full 512-flame gallery against the one-by-one renders")
+    items = configs.gallery(n=512)
+    ob, sb = engine.batch_render(items, device=True)
+    os_, ss = engine.batch_render(items, serial=True, device=True)
+    torch.cuda.synchronize()
+    assert sb.total_samples == ss.total_samples
+    bad = [i for i in range(len(items)) if not torch.equal(ob[i], os_[i])]
+    assert not bad, bad[:20]
+
+
 def test_batch_tuning_cache_skips_the_tuning(engine, monkeypatch, tmp_path):
     say("batch: FFM_TUNE_CACHE reused by a new context")
     path = tmp_path / "tune.txt"
