@@ -580,9 +580,10 @@ int iterate_grid(ffm_ctx* c, uint64_t walkers) {
 }
 
 constexpr int XCD_COPIES = 8;
-// Staged samples held at once: 12 GiB of records (a record's slot is a u32
-// offset, so the buffer stays below 2^32 records).
-constexpr uint64_t RECORD_BUDGET = 3ull << 30;
+// Staged samples held at once: up to 2^33 records (32 GiB; a flush keeps a
+// run's start in 32 bits plus one bit beside its room). single4k's 2^24
+// walkers then walk in one launch and one tile accumulation.
+constexpr uint64_t RECORD_BUDGET = (1ull << 33) - (1ull << 24);
 constexpr size_t LDS_PER_CU = LDS_DYNAMIC_MAX;
 
 // The LDS tile splat of one dispatch (see tilesplat.hip): walkers in chunks
