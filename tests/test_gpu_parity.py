@@ -120,6 +120,20 @@ def gpu_hist_dev(engine, fd, pal, view, walkers, ipt, burn_in, seed, mode):
     return h, tuple(int(v) for v in cnt.cpu())
 
 
+def test_single4k_full_walk_lds_equals_atomics(engine):
+    """The headline dispatch itself -- 2^24 walkers x 256 at 3840x2160 in
+    one walker launch, 4.3e9 records, so bins' regions pass 2^32 record
+    slots -- gives the u32 histogram of the global-atomic splat bit for bit."""
+    say("single4k full walk: LDS splat (one launch) vs global atomics")
+    fd, pal, job = configs.single4k()
+    h2, c2 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 24, 256, 20, 1, 2)
+    h0, c0 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 24, 256, 20, 1, 0)
+    say(f"  plotted {c2[0]}, in frame {c2[1]} (regions sized 1.25x: past 2^32 slots)")
+    assert c2 == c0
+    assert torch.equal(h2, h0)
+    del h2, h0
+
+
 @pytest.mark.parametrize("side", [(6144, 3072), (12288, 12288), (32768, 32768)])
 def test_lds_splat_with_tile_bins(engine, side):
     """Images past 2048 tiles sort records into bins of 2^k tiles (6144x3072:
