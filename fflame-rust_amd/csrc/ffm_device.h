@@ -88,11 +88,12 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
 // change over the loop of active variations, so without a barrier the
 // compiler hoists ALL bodies' arithmetic (divisions, sqrt, sincos, atan2)
 // out of that loop and pays for every one on every transform evaluation.
-// The loops over variation slots pass the point through an empty volatile
-// asm at the top of every trip (FFM_OPAQUE_POINT): the point is then a new
-// value each trip, in the same registers (no copies), so each body stays
-// behind its branch.
-#define FFM_OPAQUE_POINT(px, py) asm volatile("" : "+v"(px), "+v"(py))
+// The loops over variation slots pass the point and its r^2 = x^2 + y^2
+// (computed once for every body that reads it, in the bodies' own order:
+// x*x first) through an empty volatile asm at the top of every trip
+// (FFM_OPAQUE_POINT): they are then new values each trip, in the same
+// registers (no copies), so each body stays behind its branch.
+#define FFM_OPAQUE_POINT(px, py, pr2) asm volatile("" : "+v"(px), "+v"(py), "+v"(pr2))
 
 // One variation body V(x, y). `row` is the transform's table row (waves
 // reads its own b, e, d, f), `p` the variation's params. Bodies:
@@ -104,11 +105,11 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
 // blurs of the reference's presets), kept out of eval_variation's switch
 // unless a flame uses them (the EXT kernels).
 __device__ __forceinline__ void eval_ext(int id, const float* __restrict__ p, float x, float y,
-                                         Rng& rng, float& ox, float& oy) {
+                                         float r2, Rng& rng, float& ox, float& oy) {
   switch (id) {
     case FFM_VAR_BLOB: {
       float high = p[0], low = p[1], waves = p[2];
-      float r = ffm_sqrt(x * x + y * y);
+      float r = ffm_sqrt(r2);
       float theta = ff_atan2(x, y);  // Apophysis's atan2(x, y)
       float scale = r * (low + ((high - low) / 2.0f) * (ffm_sinf(waves * theta) + 1.0f));
       float s, c;
@@ -158,7 +159,7 @@ __device__ __forceinline__ void eval_ext(int id, const float* __restrict__ p, fl
 template <bool EXT>
 __device__ __forceinline__ void eval_variation(int id, const float* __restrict__ row,
                                                const float* __restrict__ p, float x, float y,
-                                               Rng& rng, float& ox, float& oy) {
+                                               float r2, Rng& rng, float& ox, float& oy) {
   switch (id) {
     case FFM_VAR_LINEAR:
       ox = x;
@@ -169,13 +170,12 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       oy = ffm_sinf(y);
       break;
     case FFM_VAR_SPHERICAL: {
-      float r2 = x * x + y * y + 1e-6f;
-      ox = ffm_fdiv(x, r2);
-      oy = ffm_fdiv(y, r2);
+      const float r2e = r2 + 1e-6f;
+      ox = ffm_fdiv(x, r2e);
+      oy = ffm_fdiv(y, r2e);
       break;
     }
     case FFM_VAR_SWIRL: {
-      float r2 = x * x + y * y;
       float s, c;
       ffm_sincosf(r2, &s, &c);
       ox = x * s - y * c;
@@ -183,7 +183,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_HORSESHOE: {
-      float r = ffm_sqrt_cr(x * x + y * y) + 1e-6f;  // its rounding reaches both outputs
+      float r = ffm_sqrt_cr(r2) + 1e-6f;  // its rounding reaches both outputs
       float r_inv = ffm_rcp(r);
       ox = (x - y) * (x + y) * r_inv;
       oy = 2.0f * x * y * r_inv;
@@ -191,14 +191,14 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     }
     case FFM_VAR_POLAR: {
       float theta = ffm_atan2f(x, y);
-      float r = ffm_sqrt(x * x + y * y);
+      float r = ffm_sqrt(r2);
       ox = ffm_fdiv(theta, PI_F);
       oy = r - 1.0f;
       break;
     }
     case FFM_VAR_DISC: {
       float theta = ffm_atan2f(x, y);
-      float r = ffm_sqrt_cr(x * x + y * y);  // feeds the angle pi r
+      float r = ffm_sqrt_cr(r2);  // feeds the angle pi r
       float theta_pi = ffm_fdiv(theta, PI_F);
       float pi_r = PI_F * r;
       float s, c;
@@ -208,12 +208,12 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_HYPERBOLIC: {
-      ox = ffm_fdiv(x, x * x + y * y + 1e-6f);
+      ox = ffm_fdiv(x, r2 + 1e-6f);
       oy = y;
       break;
     }
     case FFM_VAR_JULIA: {
-      float r = ffm_sqrt(x * x + y * y);
+      float r = ffm_sqrt(r2);
       float theta = ffm_atan2f(y, x);
       float sqrt_r = ffm_sqrt(r);
       float omega = (rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
@@ -235,7 +235,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     case FFM_VAR_JULIAN: {
       float power = p[0];
       float cpower = p[2];
-      float r2 = x * x + y * y;
       float r = powf(r2, cpower);
       float theta = ff_atan2(y, x);
       float trunc_val = floorf(fabsf(power) * rng_nextf(rng));
@@ -247,7 +246,6 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_BUBBLE: {
-      float r2 = x * x + y * y;
       float r = r2 / 4.0f + 1.0f;
       float scale = ffm_rcp(r);
       ox = scale * x;
@@ -265,7 +263,7 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
     }
     default:
       if constexpr (EXT) {
-        eval_ext(id, p, x, y, rng, ox, oy);
+        eval_ext(id, p, x, y, r2, rng, ox, oy);
       } else {
         ox = 0.0f;
         oy = 0.0f;
@@ -301,17 +299,19 @@ __device__ __forceinline__ void apply_xform(const float* __restrict__ row, KInt*
   float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
   float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
   float rx = 0.0f, ry = 0.0f;
+  float r2 = ax * ax + ay * ay;
   for (int k = 0; k < nv; k++) {
-    FFM_OPAQUE_POINT(ax, ay);
+    FFM_OPAQUE_POINT(ax, ay, r2);
     int id = __builtin_amdgcn_readfirstlane(ids[k]);
     float w = row[XF_W0 + k];
     if (w != 0.0f) {
       if (EXT && id == FFM_VAR_PRE_BLUR) {
         pre_blur(w, ax, ay, rng);
+        r2 = ax * ax + ay * ay;
         continue;
       }
       float vx, vy;
-      eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
+      eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, r2, rng, vx, vy);
       rx += w * vx;
       ry += w * vy;
     }
@@ -348,18 +348,20 @@ __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x
   float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
   float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
   float rx = 0.0f, ry = 0.0f;
+  float r2 = ax * ax + ay * ay;
   const int nact = __builtin_amdgcn_readfirstlane(fl.hdr->fin_nact[i]);
   for (int j = 0; j < nact; j++) {
-    FFM_OPAQUE_POINT(ax, ay);
+    FFM_OPAQUE_POINT(ax, ay, r2);
     const int code = __builtin_amdgcn_readfirstlane(fl.hdr->fin_act[i][j]);
     const int k = code & 0xFF, id = code >> 8;
     const float w = row[XF_W0 + k];
     if (EXT && id == FFM_VAR_PRE_BLUR) {
       pre_blur(w, ax, ay, rng);
+      r2 = ax * ax + ay * ay;
       continue;
     }
     float vx, vy;
-    eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, rng, vx, vy);
+    eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, r2, rng, vx, vy);
     rx += w * vx;
     ry += w * vy;
   }
