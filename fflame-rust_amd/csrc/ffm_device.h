@@ -93,7 +93,8 @@ __device__ __forceinline__ float ff_atan2(float y, float x) {
 // x*x first) through an empty volatile asm at the top of every trip
 // (FFM_OPAQUE_POINT): they are then new values each trip, in the same
 // registers (no copies), so each body stays behind its branch.
-#define FFM_OPAQUE_POINT(px, py, pr2) asm volatile("" : "+v"(px), "+v"(py), "+v"(pr2))
+#define FFM_OPAQUE_POINT(px, py, pr2, ptp) \
+  asm volatile("" : "+v"(px), "+v"(py), "+v"(pr2), "+v"(ptp))
 
 // One variation body V(x, y). `row` is the transform's table row (waves
 // reads its own b, e, d, f), `p` the variation's params. Bodies:
@@ -159,7 +160,8 @@ __device__ __forceinline__ void eval_ext(int id, const float* __restrict__ p, fl
 template <bool EXT>
 __device__ __forceinline__ void eval_variation(int id, const float* __restrict__ row,
                                                const float* __restrict__ p, float x, float y,
-                                               float r2, Rng& rng, float& ox, float& oy) {
+                                               float r2, bool has_tp, float tp, Rng& rng,
+                                               float& ox, float& oy) {
   switch (id) {
     case FFM_VAR_LINEAR:
       ox = x;
@@ -190,16 +192,14 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       break;
     }
     case FFM_VAR_POLAR: {
-      float theta = ffm_atan2f(x, y);
       float r = ffm_sqrt(r2);
-      ox = ffm_fdiv(theta, PI_F);
+      ox = has_tp ? tp : ffm_fdiv(ffm_atan2f(x, y), PI_F);
       oy = r - 1.0f;
       break;
     }
     case FFM_VAR_DISC: {
-      float theta = ffm_atan2f(x, y);
       float r = ffm_sqrt_cr(r2);  // feeds the angle pi r
-      float theta_pi = ffm_fdiv(theta, PI_F);
+      float theta_pi = has_tp ? tp : ffm_fdiv(ffm_atan2f(x, y), PI_F);
       float pi_r = PI_F * r;
       float s, c;
       ffm_sincosf(pi_r, &s, &c);
@@ -295,23 +295,26 @@ __device__ __forceinline__ void pre_blur(float w, float& x, float& y, Rng& rng) 
 // (uniform across the wave), `row` the transform's row in LDS.
 template <bool EXT>
 __device__ __forceinline__ void apply_xform(const float* __restrict__ row, KInt* ids, int nv,
-                                            float& x, float& y, Rng& rng) {
+                                            float& x, float& y, Rng& rng, bool has_tp = false) {
   float ax = row[XF_A] * x + row[XF_B] * y + row[XF_E];
   float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
   float rx = 0.0f, ry = 0.0f;
   float r2 = ax * ax + ay * ay;
+  float tp = has_tp ? ffm_fdiv(ffm_atan2f(ax, ay), PI_F) : 0.0f;
   for (int k = 0; k < nv; k++) {
-    FFM_OPAQUE_POINT(ax, ay, r2);
+    FFM_OPAQUE_POINT(ax, ay, r2, tp);
     int id = __builtin_amdgcn_readfirstlane(ids[k]);
     float w = row[XF_W0 + k];
     if (w != 0.0f) {
       if (EXT && id == FFM_VAR_PRE_BLUR) {
         pre_blur(w, ax, ay, rng);
         r2 = ax * ax + ay * ay;
+        if (has_tp) tp = ffm_fdiv(ffm_atan2f(ax, ay), PI_F);
         continue;
       }
       float vx, vy;
-      eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, r2, rng, vx, vy);
+      eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, r2, has_tp, tp, rng, vx,
+                          vy);
       rx += w * vx;
       ry += w * vy;
     }
@@ -335,6 +338,7 @@ struct FlameView {
   const float* xcum;         // LDS xaos selection table (n_xforms^2), or null without xaos
   const uint8_t* sel;        // LDS selection buckets (SEL_BITS of the draw), or null
   int n_xforms, n_finals, n_vars;
+  int share_tp;              // FlameHeader::share_tp
 };
 
 // Final transform i of the flame: apply_xform's arithmetic with the loop
@@ -349,19 +353,23 @@ __device__ __forceinline__ void apply_final(const FlameView& fl, int i, float& x
   float ay = row[XF_C] * x + row[XF_D] * y + row[XF_F];
   float rx = 0.0f, ry = 0.0f;
   float r2 = ax * ax + ay * ay;
+  const bool has_tp = (fl.share_tp >> (1 + i)) & 1;
+  float tp = has_tp ? ffm_fdiv(ffm_atan2f(ax, ay), PI_F) : 0.0f;
   const int nact = __builtin_amdgcn_readfirstlane(fl.hdr->fin_nact[i]);
   for (int j = 0; j < nact; j++) {
-    FFM_OPAQUE_POINT(ax, ay, r2);
+    FFM_OPAQUE_POINT(ax, ay, r2, tp);
     const int code = __builtin_amdgcn_readfirstlane(fl.hdr->fin_act[i][j]);
     const int k = code & 0xFF, id = code >> 8;
     const float w = row[XF_W0 + k];
     if (EXT && id == FFM_VAR_PRE_BLUR) {
       pre_blur(w, ax, ay, rng);
       r2 = ax * ax + ay * ay;
+      if (has_tp) tp = ffm_fdiv(ffm_atan2f(ax, ay), PI_F);
       continue;
     }
     float vx, vy;
-    eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, r2, rng, vx, vy);
+    eval_variation<EXT>(id, row, row + XF_P0 + k * FFM_MAX_VAR_PARAMS, ax, ay, r2, has_tp, tp, rng, vx,
+                        vy);
     rx += w * vx;
     ry += w * vy;
   }
@@ -479,7 +487,7 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
   const int n = fl.n_xforms;
   const float* row = fl.rows + __umul24(xi, XF_STRIDE);
   float nx = x, ny = y;
-  apply_xform<EXT>(row, fl.ids, fl.n_vars, nx, ny, rng);
+  apply_xform<EXT>(row, fl.ids, fl.n_vars, nx, ny, rng, fl.share_tp & 1);
   // Bad-value respawn + re-fuse (main_template.wgsl:397-479).
   if (!(fabsf(nx) <= 1e32f) || !(fabsf(ny) <= 1e32f)) {
     x = rng_nextf(rng) * 2.0f - 1.0f;
@@ -616,6 +624,7 @@ __device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_
   l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
   l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
   l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);
+  l.fv.share_tp = __builtin_amdgcn_readfirstlane(hdr->share_tp);
   l.pal_u32 = pal;
   l.pal_f32 = palf;
   return l;

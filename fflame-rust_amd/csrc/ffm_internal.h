@@ -36,7 +36,10 @@ struct FlameHeader {
   int32_t n_xforms;
   int32_t n_finals;
   int32_t n_vars;
-  int32_t var_mask;  // bit per ffm variation slot family, for fast paths
+  // Bit 0: some transforms use polar and some use disc (or one both), so the
+  // transform pass computes their common atan2(x, y) / pi once for every
+  // lane; bit 1 + i: final i uses both.
+  int32_t share_tp;
   int32_t var_id[FFM_MAX_VARS];
   // Each final transform's variation slots with a non-zero weight, in
   // local-index order, as slot | variation id << 8 (one scalar load a slot):

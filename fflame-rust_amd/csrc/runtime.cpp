@@ -307,10 +307,23 @@ std::vector<float> pack_flame(const ffm_flame_desc* f) {
     if (f->var_id[k] == FFM_VAR_PRE_BLUR) src[ns++] = k;
   for (int k = 0; k < f->n_vars; k++)
     if (f->var_id[k] != FFM_VAR_PRE_BLUR) src[ns++] = k;
-  for (int k = 0; k < f->n_vars; k++) {
-    h.var_id[k] = f->var_id[src[k]];
-    h.var_mask |= 1 << (h.var_id[k] & 31);
+  for (int k = 0; k < f->n_vars; k++) h.var_id[k] = f->var_id[src[k]];
+  // polar and disc both start from atan2(x, y) / pi: share it in a pass
+  // whose transforms use both
+  auto uses = [&](const ffm_xform& x, int id) {
+    for (int k = 0; k < f->n_vars; k++)
+      if (f->var_id[k] == id && x.var_weight[k] != 0.0f) return true;
+    return false;
+  };
+  bool polar = false, disc = false;
+  for (int i = 0; i < f->n_xforms; i++) {
+    polar = polar || uses(f->xforms[i], FFM_VAR_POLAR);
+    disc = disc || uses(f->xforms[i], FFM_VAR_DISC);
   }
+  h.share_tp = polar && disc ? 1 : 0;
+  for (int i = 0; i < f->n_finals; i++)
+    if (uses(f->finals[i], FFM_VAR_POLAR) && uses(f->finals[i], FFM_VAR_DISC))
+      h.share_tp |= 2 << i;
   for (int i = 0; i < f->n_finals; i++)
     for (int k = 0; k < f->n_vars; k++)
       if (f->finals[i].var_weight[src[k]] != 0.0f)

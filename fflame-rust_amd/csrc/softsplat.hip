@@ -101,6 +101,7 @@ __device__ __forceinline__ FlameView soft_load(const SoftArgs& a, const SoftFit&
   fv.n_xforms = nx;
   fv.n_finals = 0;
   fv.n_vars = nv;
+  fv.share_tp = 0;
   return fv;
 }
 
