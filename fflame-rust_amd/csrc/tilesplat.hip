@@ -35,7 +35,14 @@ constexpr int BIN_BLOCK = 1024;
 // bin stays below 2^16).
 constexpr int STAGE_MIN = 10240, STAGE_MAX = 16384;
 constexpr int MAX_FLUSH_EVERY = 8;  // iterations between fill checks, at most
-constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item
+constexpr uint32_t SLICE = 1u << 20;  // records per tile_accum work item (u32 tiles)
+// f32 tiles (the 12288^2 stress render: bins of 16 tiles, counting-sorted per
+// item) balance better over smaller items: 2^19 records take its
+// accumulation 411 -> 398 ms a render, 2^22 and 2^24 lengthen it to 458 and
+// 665 ms (the densest bins then sit in a few workgroups)
+constexpr uint32_t SLICE_F32 = 1u << 19;
+
+__host__ __device__ inline uint32_t slice_of(const TileArgs& t) { return t.f32 ? SLICE_F32 : SLICE; }
 // accum_run adds two channels per 64-bit LDS atomic: a slice's channel sums
 // must not carry out of 32 bits (at most 255, or 100 for density, a record)
 static_assert((unsigned long long)SLICE * 255ull < (1ull << 32), "tile channel sums carry");
@@ -436,7 +443,7 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
   __shared__ uint32_t total;
   const uint32_t all = block_scan_n(
       t.n_tiles,
-      [&](int i) { return (min(t.tile_fill[i], t.cap[i]) + SLICE - 1) / SLICE; },
+      [&](int i) { return (min(t.tile_fill[i], t.cap[i]) + slice_of(t) - 1) / slice_of(t); },
       t.item_start, buf, wsum, &total);
   if (threadIdx.x == 0) t.item_start[t.n_tiles] = all;
 }
@@ -627,7 +634,8 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_accum_kernel(TileArgs t, const
     const int ti = find_range(t.item_start, t.n_tiles, item);
     const uint32_t first = t.item_start[ti], n_slices = t.item_start[ti + 1] - first;
     const uint32_t stored = min(t.tile_fill[ti], t.cap[ti]);
-    const uint32_t r0 = (item - first) * SLICE, r1 = min(r0 + SLICE, stored);
+    const uint32_t sl = slice_of(t);
+    const uint32_t r0 = (item - first) * sl, r1 = min(r0 + sl, stored);
     accum_slice<F32>(t, ti, r0, r1, n_slices, tile, pal, palf, sort);
   }
 }
