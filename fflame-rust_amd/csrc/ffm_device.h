@@ -450,12 +450,13 @@ __device__ __forceinline__ int select_bucket(const FlameView& fl, uint32_t d, fl
 // same rule over row `prev` of the table of running sums of
 // weight[j] * xaos[prev][j], and the pick becomes the next `prev`.
 template <bool EXT>
-__device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool& should_plot,
+__device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, uint32_t& should_plot,
                                             int& prev) {
   const uint32_t d = rng_next(rng) >> 8u;
   const float u = (float)d * (1.0f / 16777216.0f);  // rng_nextf's value
   int xi;
-  if (fl.sel) {
+  // flames without usable buckets take the EXT kernels
+  if (!EXT || fl.sel) {
     xi = select_bucket(fl, d, u);
   } else if (EXT && fl.xcum) {
     const int n = fl.n_xforms;
@@ -469,9 +470,12 @@ __device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool&
     xi = select_const(fl, u);
   }
   const float opacity = fl.rows[__umul24(xi, XF_STRIDE) + XF_OPACITY];
-  should_plot = true;
+  should_plot = 1u;
   if (opacity >= 1.0f) rng_skip(rng);
-  else should_plot = rng_nextf(rng) < opacity;
+  else should_plot = rng_nextf(rng) < opacity ? 1u : 0u;
+  // held in a vector register over the step (as a lane mask it would live
+  // in a scalar pair the whole step and spill)
+  asm volatile("" : "+v"(should_plot));
   return xi;
 }
 
@@ -482,12 +486,15 @@ __device__ __forceinline__ int choose_xform(const FlameView& fl, Rng& rng, bool&
 template <bool EXT>
 __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& v, int pal_size,
                                           uint32_t burn_in, Rng& rng, float& x, float& y,
-                                          float& ci, uint32_t& fuse, int xi, bool should_plot,
-                                          int32_t& px, int32_t& py, int& texel) {
+                                          float& ci, uint32_t& fuse, int xi,
+                                          uint32_t should_plot, int32_t& px, int32_t& py,
+                                          int& texel) {
   const int n = fl.n_xforms;
   const float* row = fl.rows + __umul24(xi, XF_STRIDE);
   float nx = x, ny = y;
-  apply_xform<EXT>(row, fl.ids, fl.n_vars, nx, ny, rng, fl.share_tp & 1);
+  const int nv = fl.n_vars;
+  if (!EXT) __builtin_assume(nv > 0);  // flames without variations take the EXT kernels
+  apply_xform<EXT>(row, fl.ids, nv, nx, ny, rng, fl.share_tp & 1);
   // Bad-value respawn + re-fuse (main_template.wgsl:397-479).
   if (!(fabsf(nx) <= 1e32f) || !(fabsf(ny) <= 1e32f)) {
     x = rng_nextf(rng) * 2.0f - 1.0f;
@@ -508,9 +515,9 @@ __device__ __forceinline__ int step_xform(const FlameView& fl, const ViewConst& 
   // Final chain: reshapes the plot only (main_template.wgsl:663-717), the
   // finals this transform attaches, in order.
   float fx = nx, fy = ny;
-  if (fl.n_finals > 0) {
-    // without EXT every transform applies the same chain (the host picks
-    // the EXT kernels for chains that differ by transform)
+  // without EXT every transform applies the same chain (the host picks the
+  // EXT kernels for chains that differ by transform), empty without finals
+  if (!EXT || fl.n_finals > 0) {
     if (!EXT || fl.hdr->fin_uniform) {
       const int nc = __builtin_amdgcn_readfirstlane(fl.hdr->fin_chain_n);
       for (int j = 0; j < nc; j++)
@@ -537,7 +544,7 @@ __device__ __forceinline__ int chaos_step(const FlameView& fl, const ViewConst& 
                                           uint32_t burn_in, Rng& rng, float& x, float& y,
                                           float& ci, uint32_t& fuse, int& prev, int& xi_out,
                                           int32_t& px, int32_t& py, int& texel) {
-  bool should_plot;
+  uint32_t should_plot;
   const int xi = choose_xform<EXT>(fl, rng, should_plot, prev);
   xi_out = xi;
   return step_xform<EXT>(fl, v, pal_size, burn_in, rng, x, y, ci, fuse, xi, should_plot, px, py,

@@ -148,10 +148,21 @@ bool supported_var(int id) {
   }
 }
 
-// Whether a flame needs the EXT kernels: xaos, or a variation beyond the
-// thirteen of the workloads (their kernels leave those paths out).
+// Whether a flame needs the EXT kernels: xaos, no variation slots, weights
+// the selection buckets cannot serve, final chains that differ by transform,
+// or a variation beyond the thirteen of the workloads (the plain kernels
+// leave those paths out).
 bool flame_ext(const ffm_flame_desc* f) {
-  if (f->has_xaos) return true;
+  if (f->has_xaos || f->n_vars < 1) return true;
+  // the plain kernels pick through the selection buckets, usable while the
+  // cumulative weights (the sequential f32 sums the table holds) are finite
+  // and never decrease
+  float cum = 0.0f;
+  for (int i = 0; i < f->n_xforms; i++) {
+    const float next = cum + f->xforms[i].weight;
+    if (!std::isfinite(next) || (i > 0 && next < cum)) return true;
+    cum = next;
+  }
   if (f->n_finals > 0 && f->per_transform_finals)  // chains that differ by transform
     for (int i = 1; i < f->n_xforms; i++) {
       const ffm_xform &x0 = f->xforms[0], &x = f->xforms[i];

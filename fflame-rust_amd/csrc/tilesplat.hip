@@ -263,6 +263,7 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
     walker_begin<EXT>(F.fv, a, walker, active && a.orbits ? a.orbits + wi : nullptr, rng, x, y,
                       ci, fuse, prev);
     uint32_t r_plotted = 0u, r_in = 0u;  // this round's counts (at most ipt each)
+    uint32_t to_check = every;           // iterations left until the next fill check
     for (uint32_t it = 0; it < a.ipt; it++) {
       if (active) {
         int xi, texel = 0;
@@ -290,7 +291,8 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
           }
         }
       }
-      if (!COUNT_ONLY && ((it % every) == every - 1 || it + 1 == a.ipt)) {
+      if (!COUNT_ONLY && (--to_check == 0u || it + 1 == a.ipt)) {
+        to_check = every;
         __syncthreads();
         const uint32_t f = *L.fill;
         __syncthreads();
