@@ -337,6 +337,7 @@ struct FlameView {
   KHeader* hdr;              // the table's header (final-chain slot lists; global), or null
   const float* xcum;         // LDS xaos selection table (n_xforms^2), or null without xaos
   const uint8_t* sel;        // LDS selection buckets (SEL_BITS of the draw), or null
+  int sel_byte;              // their byte offset from rows (addressing; sel tells presence)
   int n_xforms, n_finals, n_vars;
   int share_tp;              // FlameHeader::share_tp
 };
@@ -436,9 +437,10 @@ __device__ __forceinline__ int select_const(const FlameView& fl, float u) {
 // first transform from the bucket's lowest pick on whose cumulative weight
 // reaches the target (every earlier one falls short for the whole bucket).
 __device__ __forceinline__ int select_bucket(const FlameView& fl, uint32_t d, float u) {
-  const uint32_t e = fl.sel[d >> (24 - SEL_BITS)];
+  const uint32_t e =
+      reinterpret_cast<const uint8_t*>(fl.rows)[fl.sel_byte + (int)(d >> (24 - SEL_BITS))];
   int xi = (int)(e & 0x7Fu);
-  if (e & 0x80u) {
+  if (e > 0x7Fu) {
     const int n = fl.n_xforms;
     const float target = u * fl.rows[(n - 1) * XF_STRIDE + XF_CUM];
     while (xi < n - 1 && !(target <= fl.rows[__umul24(xi, XF_STRIDE) + XF_CUM])) xi++;
@@ -628,6 +630,7 @@ __device__ __forceinline__ LdsFlame load_table(const float* table, const uint32_
                   : nullptr;
   const int sel_off = __builtin_amdgcn_readfirstlane(hdr->sel_off);
   l.fv.sel = sel_off ? reinterpret_cast<const uint8_t*>(l.fv.rows + sel_off) : nullptr;
+  l.fv.sel_byte = sel_off * 4;
   l.fv.n_xforms = __builtin_amdgcn_readfirstlane(nx);
   l.fv.n_finals = __builtin_amdgcn_readfirstlane(nf);
   l.fv.n_vars = __builtin_amdgcn_readfirstlane(nv);

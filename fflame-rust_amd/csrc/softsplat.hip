@@ -97,6 +97,7 @@ __device__ __forceinline__ FlameView soft_load(const SoftArgs& a, const SoftFit&
   fv.ids = (KInt*)(a.tables + F.table_off) + 4;
   fv.hdr = nullptr;  // no final transforms in a soft-splat flame
   fv.sel = nullptr;  // the LDS copy holds the transforms' rows only
+  fv.sel_byte = 0;
   fv.xcum = nullptr;  // nor xaos
   fv.n_xforms = nx;
   fv.n_finals = 0;

@@ -243,7 +243,15 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
                             const BinLds& L, uint64_t base, uint64_t stride, uint64_t limit,
                             unsigned long long& n_plotted, unsigned long long& n_in) {
   const int tid = threadIdx.x;
-  const ViewConst v = a.view;
+  // Uniform values that only vector instructions read, on every plotted
+  // iteration, held in vector registers: the scalar file is full, and a
+  // value spilled from it costs a v_readlane at every use.
+  ViewConst v = a.view;
+  uint32_t bins_x = (uint32_t)t.bins_x, hmask = (1u << t.tile_hl) - 1u,
+           wmask = (1u << t.tile_wl) - 1u, gmask = (1u << t.group_shift) - 1u;
+  FlameView fv = F.fv;
+  asm volatile("" : "+v"(v.cy), "+v"(v.width), "+v"(v.height), "+v"(bins_x), "+v"(hmask),
+               "+v"(wmask), "+v"(gmask), "+v"(fv.sel_byte));
   const uint64_t rounds = limit > base ? (limit - base + stride - 1) / stride : 0;
   // a fill check every `every` iterations; at most `every` x blockDim
   // records arrive between checks, so flushing at stage - that never
@@ -260,28 +268,29 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
     float x, y, ci;
     uint32_t fuse;
     int prev;
-    walker_begin<EXT>(F.fv, a, walker, active && a.orbits ? a.orbits + wi : nullptr, rng, x, y,
+    walker_begin<EXT>(fv, a, walker, active && a.orbits ? a.orbits + wi : nullptr, rng, x, y,
                       ci, fuse, prev);
     uint32_t r_plotted = 0u, r_in = 0u;  // this round's counts (at most ipt each)
     uint32_t to_check = every;           // iterations left until the next fill check
     for (uint32_t it = 0; it < a.ipt; it++) {
+      // the pass-sharing bits re-tested every iteration (a mask derived once
+      // would hold a scalar pair over the whole loop)
+      asm volatile("" : "+s"(fv.share_tp));
       if (active) {
         int xi, texel = 0;
         int32_t px, py;
-        int st = chaos_step<EXT>(F.fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi,
+        int st = chaos_step<EXT>(fv, v, a.pal_size, a.burn_in, rng, x, y, ci, fuse, prev, xi,
                                  px, py, texel);
         r_plotted += (st & (ST_RESPAWN | ST_BURN)) ? 0u : 1u;
         if (st & ST_PLOT) {
           r_in++;
           // in-frame: px, py >= 0, so tile arithmetic is unsigned shifts and masks
           const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, tx = ux >> t.tile_wl;
-          const uint32_t tile =
-              __umul24(uy >> t.tile_hl, (uint32_t)t.bins_x) + (tx >> t.group_shift);
+          const uint32_t tile = __umul24(uy >> t.tile_hl, bins_x) + (tx >> t.group_shift);
           const uint32_t r = atomicAdd(&L.cnt[tile], 1u);  // the record's rank in its bin
           if (!COUNT_ONLY) {
-            const uint32_t local = (uy & ((1u << t.tile_hl) - 1u)) << t.tile_wl |
-                                   (ux & ((1u << t.tile_wl) - 1u)) |
-                                   (tx & ((1u << t.group_shift) - 1u)) << REC_SUB_SHIFT;
+            const uint32_t local =
+                (uy & hmask) << t.tile_wl | (ux & wmask) | (tx & gmask) << REC_SUB_SHIFT;
             // the append to the stage: the compiler's atomic optimizer makes
             // the wave's increments of one address one LDS atomic by its
             // first active lane, each lane taking base + its rank in the wave
@@ -302,7 +311,7 @@ __device__ void run_walkers(const IterArgs& a, const TileArgs& t, const LdsFlame
     n_plotted += r_plotted;
     n_in += r_in;
     if (!COUNT_ONLY && active && a.orbits)
-      a.orbits[wi] = walker_orbit<EXT>(F.fv, x, y, ci, fuse, prev);
+      a.orbits[wi] = walker_orbit<EXT>(fv, x, y, ci, fuse, prev);
   }
 }
 
