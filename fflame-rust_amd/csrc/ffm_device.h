@@ -219,7 +219,8 @@ __device__ __forceinline__ void eval_variation(int id, const float* __restrict__
       float omega = (rng_nextf(rng) < 0.5f) ? PI_F : 0.0f;
       float half_theta = theta / 2.0f + omega;
       float s, c;
-      ffm_sincosf(half_theta, &s, &c);
+      // |theta / 2 + omega| <= 3 pi / 2: always the short reduction
+      ffm_sincos_reduced(half_theta, &s, &c);
       ox = sqrt_r * c;
       oy = sqrt_r * s;
       break;
