@@ -471,6 +471,14 @@ __global__ __launch_bounds__(BIN_BLOCK) void tile_plan_kernel(TileArgs t) {
   if (threadIdx.x == 0) t.item_start[t.n_tiles] = all;
 }
 
+// A u32 record's (r, g) and (b, density) channel pairs, each pair one
+// little-endian u64 of the tile's u32 quad.
+__device__ __forceinline__ void rec_channels(uint32_t pk, unsigned long long& rg,
+                                             unsigned long long& bd) {
+  rg = (unsigned long long)(pk & 0xFFu) | (unsigned long long)((pk >> 8) & 0xFFu) << 32;
+  bd = (unsigned long long)((pk >> 16) & 0xFFu) | 100ull << 32;
+}
+
 // Add one record into the LDS tile (LDS atomics).
 template <bool F32>
 __device__ __forceinline__ void accum_record(uint32_t v, uint32_t* tile, const uint32_t* pal,
@@ -489,11 +497,9 @@ __device__ __forceinline__ void accum_record(uint32_t v, uint32_t* tile, const u
       // little-endian u32 quad. No carry crosses a channel: a run holds at
       // most SLICE = 2^20 records and a channel adds at most 255 a record
       // (2^28 in all), so each 32-bit half stays below 2^32.
-      const uint32_t pk = pal[rec_texel(v)];
       unsigned long long* t64 = reinterpret_cast<unsigned long long*>(tile);
-      const unsigned long long rg =
-          (unsigned long long)(pk & 0xFFu) | (unsigned long long)((pk >> 8) & 0xFFu) << 32;
-      const unsigned long long bd = (unsigned long long)((pk >> 16) & 0xFFu) | 100ull << 32;
+      unsigned long long rg, bd;
+      rec_channels(pal[rec_texel(v)], rg, bd);
       atomicAdd(&t64[2 * local + 0], rg);
       atomicAdd(&t64[2 * local + 1], bd);
     }
@@ -513,8 +519,42 @@ __device__ __forceinline__ void accum_run(const uint32_t* run, uint32_t n, uint3
     uint32_t v[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = run[r + k * bd];
+    // a round whose lanes never repeat a pixel between their first two
+    // records (nearly every round of a spread attractor) adds record by record
+    if (F32 || !__ballot((v[0] & REC_LOCAL_MASK) == (v[1] & REC_LOCAL_MASK))) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) accum_record<F32>(v[k], tile, pal, palf);
+      for (int k = 0; k < 8; k++) accum_record<F32>(v[k], tile, pal, palf);
+    } else {
+      // A lane sums its eight records of the round in registers while they
+      // hit the same pixel, and adds each such run with one pair of atomics.
+      // On an attractor that puts most samples in a few pixels (some
+      // gallery flames hold 90% in one), every lane of a wave would
+      // otherwise hit the same LDS address with every atomic, which the
+      // hardware serialises; elsewhere a run is one record and the atomics
+      // are those of one record at a time.
+      unsigned long long* t64 = reinterpret_cast<unsigned long long*>(tile);
+      uint32_t cur = v[0] & REC_LOCAL_MASK;
+      unsigned long long rg, bdn;
+      rec_channels(pal[rec_texel(v[0])], rg, bdn);
+#pragma unroll
+      for (int k = 1; k < 8; k++) {
+        const uint32_t key = v[k] & REC_LOCAL_MASK;
+        unsigned long long rg1, bd1;
+        rec_channels(pal[rec_texel(v[k])], rg1, bd1);
+        if (key == cur) {
+          rg += rg1;
+          bdn += bd1;
+        } else {
+          atomicAdd(&t64[2 * cur + 0], rg);
+          atomicAdd(&t64[2 * cur + 1], bdn);
+          cur = key;
+          rg = rg1;
+          bdn = bd1;
+        }
+      }
+      atomicAdd(&t64[2 * cur + 0], rg);
+      atomicAdd(&t64[2 * cur + 1], bdn);
+    }
   }
   for (; r < n; r += bd) accum_record<F32>(run[r], tile, pal, palf);
 }

@@ -607,6 +607,31 @@ def test_final_attachments_bit_exact(engine, attach):
     assert np.array_equal(g, o)
 
 
+def test_flame_without_variation_slots_bit_exact(engine):
+    """A flame with no variation slots (every transform's sum is empty, so
+    the point is its post-affine of the origin) takes the EXT kernels; its
+    trace and histograms equal the oracle's bit for bit."""
+    say("no variation slots")
+    rng = configs.SplitMix64(99)
+    xs = []
+    for i in range(3):
+        a = configs._affine(rng, 0.7)
+        xs.append(pyfae.xform(*a, weight=1.0, color=i / 2.0, variations={"linear": 0.0},
+                              post=(0.9, 0.1, -0.1, 0.9, 0.3 * i, -0.2 * i)))
+    fd = pyfae.flame_desc(xs)
+    fd.n_vars = 0
+    pal = configs.random_palette(configs.SplitMix64(6))
+    view = pyfae.View(320, 240, 1.0, 0.0, 0.0, 0.0)
+    for w in (0, 777):
+        assert np.array_equal(gpu_trace(engine, fd, pal, view, w, 1, 20, 200),
+                              O.trace(fd, pal, view, w, 1, 20, 200))
+    for mode in (0, 2):
+        g, plotted, _ = gpu_hist(engine, fd, pal, view, 1 << 12, 100, 20, 3, mode=mode)
+        o, oplotted = O.iterate_mt(fd, pal, view, 1 << 12, 100, 20, 3, 8)
+        assert plotted == oplotted
+        assert np.array_equal(g, o)
+
+
 @pytest.mark.parametrize("kind", ["dense", "negative", "max"])
 def test_selection_buckets_bit_exact(engine, kind):
     """The pick through the device's selection buckets (1024 buckets of the
