@@ -271,6 +271,7 @@ struct BatchArgs {
   const uint32_t* pal_u32;
   int32_t max_rows, max_pal;
   uint32_t count_div;  // pre-pass walks 1/count_div of every item
+  uint32_t count_iter_div;  // for burn_in + 1/count_iter_div of the plotted iterations
   int32_t ext;         // some flame uses a variation beyond the workloads' thirteen
   uint32_t* next_item; // work-stealing counter of this launch (zeroed before it)
   TileArgs g;          // global tile arrays; g.hist = base of the histograms

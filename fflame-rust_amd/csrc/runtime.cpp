@@ -608,6 +608,8 @@ constexpr int XCD_COPIES = 8;
 // run's start in 32 bits plus one bit beside its room). single4k's 2^24
 // walkers then walk in one launch and one tile accumulation.
 constexpr uint64_t RECORD_BUDGET = (1ull << 33) - (1ull << 24);
+// The count pre-passes walk a quarter of the plotted iterations (rounded up).
+constexpr uint32_t PRE_ITER_DIV = 4;
 constexpr size_t LDS_PER_CU = LDS_DYNAMIC_MAX;
 
 // The LDS tile splat of one dispatch (see tilesplat.hip): walkers in chunks
@@ -781,13 +783,21 @@ int lds_splat(ffm_ctx* c, IterArgs a, uint32_t* hist, hipStream_t s, bool f32 = 
                     c->t_pal == a.pal_size && c->t_walkers == chunk && c->t_ipt == a.ipt &&
                     std::memcmp(&c->t_view, &a.view, sizeof(ViewConst)) == 0;
   if (!same) {
+    // 1/32 of the walkers (at least 65536), each for its burn-in and a
+    // quarter of its plotted iterations: the tiles' shares of an
+    // attractor's samples do not depend on how long it is walked, and a
+    // small flame's pre-pass is as long as its walkers' dependent iterations
     IterArgs pre = a;
     pre.walkers = std::min<uint64_t>(chunk, std::max<uint64_t>(chunk / 32, 65536));
     pre.plotted = pre.inframe = nullptr;
+    double scale = (double)chunk / (double)pre.walkers;
+    if (a.ipt > a.burn_in) {
+      pre.ipt = a.burn_in + (a.ipt - a.burn_in + PRE_ITER_DIV - 1) / PRE_ITER_DIV;
+      scale *= (double)(a.ipt - a.burn_in) / (double)(pre.ipt - a.burn_in);
+    }
     HIPCHECK(c, launch_clear_u32(t.tile_fill, TILE_MAX, s));
     HIPCHECK(c, launch_bin_iterate(pre, t, true, grid, s));
-    HIPCHECK(c, launch_tile_capacity(t, (float)((double)chunk / (double)pre.walkers), 1024,
-                                     rec_cap, s));
+    HIPCHECK(c, launch_tile_capacity(t, (float)scale, 1024, rec_cap, s));
   }
   if (!same && !ext) {
     c->t_planned = true;
@@ -1820,7 +1830,9 @@ uint64_t batch_record_budget(uint64_t needed, size_t held_bytes) {
   return std::min(b, std::max(needed, BATCH_RECORD_MIN));
 }
 constexpr int BATCH_PALETTE = 256;
-constexpr uint32_t BATCH_COUNT_DIV = 8;  // the pre-pass walks 1/8 of wave 0
+// The pre-pass walks 1/8 of wave 0's walkers, each for its burn-in and a
+// quarter of its plotted iterations (rounded up, so the estimate errs high).
+constexpr uint32_t BATCH_COUNT_DIV = 8, BATCH_COUNT_ITER_DIV = PRE_ITER_DIV;
 
 struct BatchPlan {
   // what the graph was built for
@@ -2068,6 +2080,7 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
   b.max_rows = max_rows;
   b.max_pal = BATCH_PALETTE;
   b.count_div = BATCH_COUNT_DIV;
+  b.count_iter_div = BATCH_COUNT_ITER_DIV;
   for (int i = 0; i < n; i++) b.ext |= flame_ext(&flames[i]) ? 1 : 0;
   b.g.tile_fill = reinterpret_cast<uint32_t*>(P.fill.p);
   b.g.region = reinterpret_cast<uint64_t*>(P.region.p);
@@ -2099,7 +2112,8 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     step = "count pre-pass";
     if ((e = launch_bin_batch(w0, true, grid, s)) != hipSuccess) return e;
     step = "capacities";
-    if ((e = launch_tile_capacity_n(b.g, (float)BATCH_COUNT_DIV, 1024, budget, s)) !=
+    if ((e = launch_tile_capacity_n(b.g, (float)(BATCH_COUNT_DIV * BATCH_COUNT_ITER_DIV), 1024,
+                                    budget, s)) !=
         hipSuccess)
       return e;
     for (int w = 0; w < P.n_waves; w++) {

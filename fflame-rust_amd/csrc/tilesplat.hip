@@ -768,7 +768,13 @@ __global__ __launch_bounds__(BIN_BLOCK) FFM_WALKER_WAVES void bin_batch_kernel(B
     a.seed = f.seed;
     const TileArgs t = flame_tiles(b, f);
     uint64_t limit = it.w1;
-    if (COUNT_ONLY) limit = it.w0 + max((it.w1 - it.w0) / b.count_div, 1u);
+    if (COUNT_ONLY) {
+      limit = it.w0 + max((it.w1 - it.w0) / b.count_div, 1u);
+      // the plotted share of the walk, shortened: the tiles' shares of an
+      // attractor's samples do not depend on how long it is walked
+      if (f.ipt > f.burn_in)
+        a.ipt = f.burn_in + (f.ipt - f.burn_in + b.count_iter_div - 1) / b.count_iter_div;
+    }
     run_walkers<COUNT_ONLY, EXT, false>(a, t, F, L, it.w0, blockDim.x, limit, n_plotted, n_in);
     finish_walkers<COUNT_ONLY, false>(t, L, F.pal_u32);
     __syncthreads();  // the next item reloads the flame table
