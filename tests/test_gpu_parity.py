@@ -158,6 +158,34 @@ def test_lds_splat_with_tile_bins(engine, side):
     assert torch.allclose(g4[..., :3], g3[..., :3], rtol=1e-4, atol=1e-3)
 
 
+def test_region_overflow_fallback_is_exact(engine, monkeypatch):
+    """Region capacities scaled to 1/1000 of the estimate
+    (FFM_TILE_CAPACITY_SCALE) send nearly every record past its region to
+    the direct-atomic fallback: the one-flame LDS splat still equals the
+    global-atomic splat bit for bit (one tile a bin, and bins of two tiles),
+    and a fresh context's batched gallery still equals its one-by-one
+    renders."""
+    say("region overflow fallback (capacities x 0.001)")
+    monkeypatch.setenv("FFM_TILE_CAPACITY_SCALE", "0.001")
+    for (w, h) in ((960, 540), (6144, 3072)):
+        fd, pal, job = configs.single4k(walkers=1 << 16, ipt=64, width=w, height=h)
+        g0, c0 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 16, 64, 20, 3, 0)
+        g2, c2 = gpu_hist_dev(engine, fd, pal, job.view, 1 << 16, 64, 20, 3, 2)
+        assert c0 == c2
+        assert torch.equal(g0, g2)
+        del g0, g2
+    other = pyfae.Engine(0)  # its batch plan captures the scaled capacities
+    try:
+        items = configs.gallery(n=4, spp=10.0, ipt=64)
+        ob, sb = other.batch_render(items)
+        os_, ss = other.batch_render(items, serial=True)
+    finally:
+        other.close()
+    assert sb.total_samples == ss.total_samples
+    for i in range(len(items)):
+        assert np.array_equal(ob[i], os_[i]), i
+
+
 @pytest.mark.parametrize("side", [(6144, 3072), (12288, 12288)])
 def test_binned_splat_equals_oracle(engine, side):
     """The binned LDS splat (bins of 2 and 16 tiles) against the oracle's
