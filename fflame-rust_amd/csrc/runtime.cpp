@@ -18,6 +18,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ffm_internal.h"
@@ -2169,19 +2170,33 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
 // caller's device buffers, or (dev_out NULL) the plan's pixel buffer.
 int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const float* const* stops,
                  const int* n_stops, uint8_t* const* dev_out) {
-  std::vector<float> tables;
+  // every flame's table and palette are independent: packed on up to 16
+  // host threads (the selection buckets and palette lookup tables are most
+  // of a gallery's host time), then laid end to end in flame order
+  std::vector<std::vector<float>> packed((size_t)P.n);
   std::vector<uint32_t> pals((size_t)P.n * BATCH_PALETTE);
-  std::vector<float> lin;
-  std::vector<uint32_t> pk;
-  for (int i = 0; i < P.n; i++) {
-    ffm_flame_desc* f = new ffm_flame_desc(flames[i]);
-    prepare_flame(f);
-    std::vector<float> t = pack_flame(f);
-    delete f;
-    tables.insert(tables.end(), t.begin(), t.end());
-    palette_tables(stops[i], n_stops[i], BATCH_PALETTE, pk, lin);
-    std::copy(pk.begin(), pk.end(), pals.begin() + (size_t)i * BATCH_PALETTE);
+  auto pack_range = [&](int i0, int i1) {
+    std::vector<float> lin;
+    std::vector<uint32_t> pk;
+    for (int i = i0; i < i1; i++) {
+      std::unique_ptr<ffm_flame_desc> f(new ffm_flame_desc(flames[i]));  // large: not on the stack
+      prepare_flame(f.get());
+      packed[(size_t)i] = pack_flame(f.get());
+      palette_tables(stops[i], n_stops[i], BATCH_PALETTE, pk, lin);
+      std::copy(pk.begin(), pk.end(), pals.begin() + (size_t)i * BATCH_PALETTE);
+    }
+  };
+  const int nt = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), P.n / 16}));
+  if (nt == 1) {
+    pack_range(0, P.n);
+  } else {
+    std::vector<std::thread> pool;
+    for (int k = 0; k < nt; k++)
+      pool.emplace_back(pack_range, (int)((int64_t)P.n * k / nt), (int)((int64_t)P.n * (k + 1) / nt));
+    for (std::thread& th : pool) th.join();
   }
+  std::vector<float> tables;
+  for (const std::vector<float>& t : packed) tables.insert(tables.end(), t.begin(), t.end());
   std::vector<uint8_t*> outp((size_t)P.n);
   for (int i = 0; i < P.n; i++)
     outp[i] = dev_out ? dev_out[i] : reinterpret_cast<uint8_t*>(P.rgba.p) + P.pix[i].pix_off * 4;
