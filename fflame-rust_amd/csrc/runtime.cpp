@@ -1996,28 +1996,26 @@ std::vector<uint64_t> batch_signature(int n, const ffm_flame_desc* flames, const
   return sig;
 }
 
-// Lay out the batch (buffers, waves, items) and capture its launches in a
-// graph. Flame tables and palettes are uploaded separately, every call.
-int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, const ffm_job* jobs,
-                uint32_t item_walkers, int flush_every, int tile_mode) {
-  P.drop_graph();
-  P.n = n;
-  P.item_walkers = item_walkers;
-  P.flush_every = flush_every;
-  P.tile_mode = tile_mode;
-  int wl, hl;
-  tile_dims(tile_mode, &wl, &hl);
-  std::vector<BatchFlame> bf(n);
+struct BatchLayout {
   uint64_t tile_base = 0, pix_off = 0, table_off = 0, total_samples = 0;
   int max_rows = 0, max_tiles = 0;
+};
+
+// Every flame's descriptor for the kernels (its slices of the tables, tiles
+// and histograms, which depend on the batch signature only, and its camera,
+// seed and tone map, which do not) and its picture's tone-map constants.
+BatchLayout describe_batch(BatchPlan& P, int n, const ffm_flame_desc* flames, const ffm_job* jobs,
+                           int wl, int hl, std::vector<BatchFlame>& bf) {
+  BatchLayout L;
+  bf.assign(n, BatchFlame{});
   P.pix.assign(n, BatchPix{});
   P.samples.assign(n, 0);
   for (int i = 0; i < n; i++) {
     const ffm_job& j = jobs[i];
     const int rows = packed_rows(&flames[i]);
-    max_rows = std::max(max_rows, rows);
+    L.max_rows = std::max(L.max_rows, rows);
     BatchFlame& f = bf[i];
-    f.table_off = (uint32_t)table_off;
+    f.table_off = (uint32_t)L.table_off;
     f.pal_off = (uint32_t)i * BATCH_PALETTE;
     f.n_rows = rows;
     f.pal_size = BATCH_PALETTE;
@@ -2027,20 +2025,40 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
     f.seed = j.seed_base;  // one dispatch per flame: seed_0
     f.tiles_x = (j.view.width + (1 << wl) - 1) >> wl;
     f.n_tiles = f.tiles_x * ((j.view.height + (1 << hl) - 1) >> hl);
-    max_tiles = std::max(max_tiles, f.n_tiles);
-    f.tile_base = (uint32_t)tile_base;
-    f.hist_off = pix_off;
+    L.max_tiles = std::max(L.max_tiles, f.n_tiles);
+    f.tile_base = (uint32_t)L.tile_base;
+    f.hist_off = L.pix_off;
     const uint64_t npix = (uint64_t)j.view.width * j.view.height;
-    P.pix[i].pix_off = pix_off;
+    P.pix[i].pix_off = L.pix_off;
     P.pix[i].npix = npix;
     P.samples[i] = j.walkers * (uint64_t)(j.ipt > j.burn_in ? j.ipt - j.burn_in : 0);
     P.pix[i].tc = make_tonemap(j.tonemap, j.view.width, j.view.height, j.view.zoom,
                                (double)P.samples[i]);
-    tile_base += (uint64_t)f.n_tiles;
-    pix_off += npix;
-    table_off += flame_table_floats(rows);
-    total_samples += j.walkers * (uint64_t)(j.ipt + 1);
+    L.tile_base += (uint64_t)f.n_tiles;
+    L.pix_off += npix;
+    L.table_off += flame_table_floats(rows);
+    L.total_samples += j.walkers * (uint64_t)(j.ipt + 1);
   }
+  return L;
+}
+
+// Lay out the batch (buffers, waves, items) and capture its launches in a
+// graph. Flame tables, palettes and the flames' descriptors are uploaded
+// separately, every call (a repeat call may move a camera or change a seed).
+int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, const ffm_job* jobs,
+                uint32_t item_walkers, int flush_every, int tile_mode) {
+  P.drop_graph();
+  P.n = n;
+  P.item_walkers = item_walkers;
+  P.flush_every = flush_every;
+  P.tile_mode = tile_mode;
+  int wl, hl;
+  tile_dims(tile_mode, &wl, &hl);
+  std::vector<BatchFlame> bf;
+  const BatchLayout lay = describe_batch(P, n, flames, jobs, wl, hl, bf);
+  const uint64_t tile_base = lay.tile_base, pix_off = lay.pix_off, table_off = lay.table_off,
+                 total_samples = lay.total_samples;
+  const int max_rows = lay.max_rows, max_tiles = lay.max_tiles;
   P.max_rows = max_rows;
   P.max_tiles = max_tiles;
   P.n_tiles = (int)tile_base;
@@ -2168,8 +2186,12 @@ int build_batch(ffm_ctx* c, BatchPlan& P, int n, const ffm_flame_desc* flames, c
 // Per-call uploads: flame tables and palettes (their content may change
 // between calls with the same layout) and where each flame's pixels go: the
 // caller's device buffers, or (dev_out NULL) the plan's pixel buffer.
-int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const float* const* stops,
-                 const int* n_stops, uint8_t* const* dev_out) {
+int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const ffm_job* jobs,
+                 const float* const* stops, const int* n_stops, uint8_t* const* dev_out) {
+  int wl, hl;
+  tile_dims(P.tile_mode, &wl, &hl);
+  std::vector<BatchFlame> bf;
+  describe_batch(P, P.n, flames, jobs, wl, hl, bf);
   // every flame's table and palette are independent: packed on up to 16
   // host threads (the selection buckets and palette lookup tables are most
   // of a gallery's host time), then laid end to end in flame order
@@ -2204,6 +2226,10 @@ int upload_batch(ffm_ctx* c, BatchPlan& P, const ffm_flame_desc* flames, const f
   HIPCHECK(c, hipMemcpyAsync(P.tables.p, tables.data(), tables.size() * 4, hipMemcpyHostToDevice, s));
   HIPCHECK(c, hipMemcpyAsync(P.pals.p, pals.data(), pals.size() * 4, hipMemcpyHostToDevice, s));
   HIPCHECK(c, hipMemcpyAsync(P.outp.p, outp.data(), outp.size() * sizeof(void*),
+                             hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(P.flames.p, bf.data(), bf.size() * sizeof(BatchFlame),
+                             hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(P.pixd.p, P.pix.data(), P.pix.size() * sizeof(BatchPix),
                              hipMemcpyHostToDevice, s));
   HIPCHECK(c, hipStreamSynchronize(s));  // the host vectors die on return
   return FFM_OK;
@@ -2270,7 +2296,7 @@ int batch_render_impl(ffm_ctx* c, int n, const ffm_flame_desc* flames, const flo
     for (uint32_t iw : kItems)
       for (int fe : kFlush) {
         if ((st = build_batch(c, P, n, flames, jobs, iw, fe, mode0))) return st;
-        if ((st = upload_batch(c, P, flames, stops, n_stops, dev_out))) return st;
+        if ((st = upload_batch(c, P, flames, jobs, stops, n_stops, dev_out))) return st;
         float ms = 0.0f;
         if ((st = run_batch_graph(c, P, &ms))) return st;
         if (ms < best) {
@@ -2289,7 +2315,7 @@ int batch_render_impl(ffm_ctx* c, int n, const ffm_flame_desc* flames, const flo
       if (!fits) continue;
       if ((st = build_batch(c, P, n, flames, jobs, tb.item_walkers, tb.flush_every, mode)))
         return st;
-      if ((st = upload_batch(c, P, flames, stops, n_stops, dev_out))) return st;
+      if ((st = upload_batch(c, P, flames, jobs, stops, n_stops, dev_out))) return st;
       float ms = 0.0f;
       if ((st = run_batch_graph(c, P, &ms))) return st;
       if (ms < best) {
@@ -2308,7 +2334,7 @@ int batch_render_impl(ffm_ctx* c, int n, const ffm_flame_desc* flames, const flo
                           tune->tile_mode)))
       return st;
   }
-  if ((st = upload_batch(c, P, flames, stops, n_stops, dev_out))) return st;
+  if ((st = upload_batch(c, P, flames, jobs, stops, n_stops, dev_out))) return st;
   if ((st = run_batch_graph(c, P, &gpu_ms))) return st;
   uint64_t total = 0;
   for (int i = 0; i < n; i++) {

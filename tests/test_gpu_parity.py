@@ -519,6 +519,28 @@ def test_batch_equals_serial_and_oracle(engine, tile, monkeypatch):
         assert np.abs(gm - om).mean() < 2.0
 
 
+def test_batch_repeat_with_new_cameras_seeds_and_tonemaps(engine):
+    """A second batched call with the same shapes (so the captured graph is
+    reused) but every camera moved, every seed changed and every gamma
+    changed renders the new pictures: each equals the same flame rendered
+    one by one with the new job, and differs from the first call's."""
+    say("batch: repeat call with new cameras, seeds and tone maps")
+    items = configs.gallery(n=4, spp=10.0, ipt=64)
+    first, _ = engine.batch_render(items)
+    for fd, pal, job in items:
+        job.view.pan_x += 0.05
+        job.view.pan_y -= 0.03
+        job.view.rotation += 0.1
+        job.seed_base += 7
+        job.tonemap.gamma = 1.8
+    again, sb = engine.batch_render(items)
+    serial, ss = engine.batch_render(items, serial=True)
+    assert sb.total_samples == ss.total_samples
+    for i in range(len(items)):
+        assert np.array_equal(again[i], serial[i]), i
+        assert not np.array_equal(again[i], first[i]), i
+
+
 def test_batch_device_outputs_equal_host_outputs(engine):
     """The device-resident forms write the same pictures into caller device
     buffers: batched (then its graph replayed into other buffers) and one by
