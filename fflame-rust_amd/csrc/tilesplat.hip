@@ -479,6 +479,17 @@ __device__ __forceinline__ void rec_channels(uint32_t pk, unsigned long long& rg
   bd = (unsigned long long)((pk >> 16) & 0xFFu) | 100ull << 32;
 }
 
+// LDS tile layouts, blocked so that a wave's atomics on scattered pixels
+// spread over all 64 LDS banks (an interleaved u32x4 or float4 per pixel
+// puts every pixel's first channel in one of 16 banks):
+// u32: blocks of 32 pixels, each the 32 (r, g) u64 pairs then the 32
+//      (b, density) pairs; the (r, g) pair of pixel px is u64 rg_slot(px),
+//      its (b, density) pair 32 further;
+// f32: blocks of 64 pixels, each 64 floats per channel in r, g, b, density
+//      order; channel c of pixel px is float f32_slot(px) + 64 c.
+__device__ __forceinline__ uint32_t rg_slot(uint32_t px) { return (px >> 5 << 6) | (px & 31u); }
+__device__ __forceinline__ uint32_t f32_slot(uint32_t px) { return (px >> 6 << 8) | (px & 63u); }
+
 // Add one record into the LDS tile (LDS atomics).
 template <bool F32>
 __device__ __forceinline__ void accum_record(uint32_t v, uint32_t* tile, const uint32_t* pal,
@@ -488,10 +499,11 @@ __device__ __forceinline__ void accum_record(uint32_t v, uint32_t* tile, const u
     const uint32_t local = v & REC_LOCAL_MASK;
     if (F32) {
       const float* c = palf + 3 * rec_texel(v);
-      atomicAdd(&tilef[4 * local + 0], c[0]);
-      atomicAdd(&tilef[4 * local + 1], c[1]);
-      atomicAdd(&tilef[4 * local + 2], c[2]);
-      atomicAdd(&tilef[4 * local + 3], 100.0f);
+      float* q = tilef + f32_slot(local);
+      atomicAdd(q + 0, c[0]);
+      atomicAdd(q + 64, c[1]);
+      atomicAdd(q + 128, c[2]);
+      atomicAdd(q + 192, 100.0f);
     } else {
       // Two 64-bit LDS atomics add (r, g) and (b, density) to the pixel's
       // little-endian u32 quad. No carry crosses a channel: a run holds at
@@ -500,8 +512,9 @@ __device__ __forceinline__ void accum_record(uint32_t v, uint32_t* tile, const u
       unsigned long long* t64 = reinterpret_cast<unsigned long long*>(tile);
       unsigned long long rg, bd;
       rec_channels(pal[rec_texel(v)], rg, bd);
-      atomicAdd(&t64[2 * local + 0], rg);
-      atomicAdd(&t64[2 * local + 1], bd);
+      unsigned long long* q = t64 + rg_slot(local);
+      atomicAdd(q, rg);
+      atomicAdd(q + 32, bd);
     }
   }
 }
@@ -545,15 +558,15 @@ __device__ __forceinline__ void accum_run(const uint32_t* run, uint32_t n, uint3
           rg += rg1;
           bdn += bd1;
         } else {
-          atomicAdd(&t64[2 * cur + 0], rg);
-          atomicAdd(&t64[2 * cur + 1], bdn);
+          atomicAdd(t64 + rg_slot(cur), rg);
+          atomicAdd(t64 + rg_slot(cur) + 32, bdn);
           cur = key;
           rg = rg1;
           bdn = bd1;
         }
       }
-      atomicAdd(&t64[2 * cur + 0], rg);
-      atomicAdd(&t64[2 * cur + 1], bdn);
+      atomicAdd(t64 + rg_slot(cur), rg);
+      atomicAdd(t64 + rg_slot(cur) + 32, bdn);
     }
   }
   for (; r < n; r += bd) accum_record<F32>(run[r], tile, pal, palf);
@@ -569,11 +582,12 @@ __device__ __forceinline__ void fold_tile(const TileArgs& t, int tx, int ty, uin
   for (int q = threadIdx.x; q < 1 << (t.tile_wl + t.tile_hl); q += blockDim.x) {
     const int px = (tx << t.tile_wl) + (q & wmask), py = (ty << t.tile_hl) + (q >> t.tile_wl);
     if (px >= t.width || py >= t.height) continue;
-    if (tile[4 * q + 3] == 0u) continue;  // no hit (0u is +0.0f too)
+    const uint32_t rgw = 2 * rg_slot((uint32_t)q), fq = f32_slot((uint32_t)q);
+    // no hit: a zero density (0u is +0.0f too)
+    if ((F32 ? tile[fq + 192] : tile[rgw + 65]) == 0u) continue;
     const size_t p = (size_t)py * t.width + px;
     if (F32) {
-      const float4 v =
-          make_float4(tilef[4 * q], tilef[4 * q + 1], tilef[4 * q + 2], tilef[4 * q + 3]);
+      const float4 v = make_float4(tilef[fq], tilef[fq + 64], tilef[fq + 128], tilef[fq + 192]);
       float4* h = reinterpret_cast<float4*>(t.hist) + p;
       if (n_slices == 1) {
         float4 o = *h;
@@ -590,7 +604,7 @@ __device__ __forceinline__ void fold_tile(const TileArgs& t, int tx, int ty, uin
         __hip_atomic_fetch_add(hw + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     } else {
-      const uint4 v = make_uint4(tile[4 * q], tile[4 * q + 1], tile[4 * q + 2], tile[4 * q + 3]);
+      const uint4 v = make_uint4(tile[rgw], tile[rgw + 1], tile[rgw + 64], tile[rgw + 65]);
       uint4* h = reinterpret_cast<uint4*>(t.hist) + p;
       if (n_slices == 1) {
         uint4 o = *h;
